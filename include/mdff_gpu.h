@@ -1,0 +1,137 @@
+/*
+ * mdff_gpu.h -- C ABI of the MI355X engine for mdff20's force and
+ * integration path (libmdff_gpu.so).  Plain C types only.
+ *
+ * Handle API: one context owns the device state of one periodic system
+ * (positions, velocities, forces, dipoles, neighbour list, k-space tables,
+ * thermostat) on one GPU and one HIP stream.  Every entry point returns an
+ * mdff status code (mdff_types.h) and never exits the process.
+ *
+ * Reference interfaces replaced (SURVEY of FilipeVasconcelos/mdff20, src/):
+ *   mdff_engforce        <- void engforce(void)                field.h:36, field.c:352-407
+ *   mdff_multipole_es    <- void multipole_ES(...)             ewald.h:21-24, ewald.c:75-348
+ *   mdff_get_dipoles     <- void get_dipoles(mu, upol)         coulombic.h:17, coulombic.c:67-130
+ *   mdff_pair_short      <- engforce_bhmftd_pbc / engforce_nmlj_pbc  bhmftd.h:22, nmlj.h:29
+ *   mdff_check_verlet    <- void check_verletlist(void)        verlet.h:21, verlet.c:124-170
+ *   mdff_md_step         <- void prop_agate(int) + the body of run_md's loop
+ *                                                              integration.h:3, md.c:229-248
+ *   mdff_md_run          <- void run_md(void)                  md.h:48, md.c:193-282 (no file output)
+ *   mdff_tt_damping      <- TT_damping_functions(b,c,r,f,fd,order)  tt_damp.h:7
+ * The global-state same-name wrappers (engforce(), multipole_ES(), ...) that
+ * a maintainer links into the reference's own main live in mdff_dropin.h.
+ */
+#ifndef MDFF_GPU_H
+#define MDFF_GPU_H
+
+#include <stddef.h>
+#include "mdff_types.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct mdff_ctx mdff_ctx;
+
+/* arithmetic forms */
+enum {
+    MDFF_FP64 = 0,  /* everything in FP64 (parity form)                       */
+    MDFF_MIXED = 1  /* FP32 real space and FP32 S(k) tables/contraction with
+                       FP64 accumulation of S(k), fields, forces; FP64 SCF
+                       residual, energies, virial, self terms, NHC chain      */
+};
+
+/* multipole_ES flags (argument order of ewald.h:21-24), packed as bits */
+enum {
+    MDFF_ES_LQ = 1 << 0, MDFF_ES_LD = 1 << 1, MDFF_ES_LT = 1 << 2, MDFF_ES_LDAMP = 1 << 3,
+    MDFF_ES_FORCES = 1 << 4, MDFF_ES_STRESS = 1 << 5, MDFF_ES_EF = 1 << 6, MDFF_ES_EFG = 1 << 7,
+    MDFF_ES_DIR = 1 << 8, MDFF_ES_REC = 1 << 9, MDFF_ES_INPIM = 1 << 10, MDFF_ES_UPDATE_SF = 1 << 11
+};
+
+/* Library version string and the HIP device count (0 when no GPU). */
+const char *mdff_version(void);
+int mdff_device_count(void);
+
+/* Create a context for nion ions of types typia[] (count-ordered as POSFF),
+ * lattice A (row-major 3x3, rows a,b,c) and cartesian positions.  All
+ * derived settings of init_field/init_config/check_md are applied
+ * (field.c:186-335, config.c:124-198, md.c:120-141), including lautoES.
+ * precision: MDFF_FP64 or MDFF_MIXED.  device: HIP device ordinal. */
+int mdff_create(mdff_ctx **out, const mdff_params *p, int nion, const int *typia,
+                const double *A9, const double *rx, const double *ry, const double *rz,
+                int precision, int device);
+void mdff_destroy(mdff_ctx *ctx);
+
+/* state transfer (host <-> device, synchronous) */
+int mdff_set_positions(mdff_ctx *ctx, const double *rx, const double *ry, const double *rz);
+int mdff_set_velocities(mdff_ctx *ctx, const double *vx, const double *vy, const double *vz);
+int mdff_get_positions(mdff_ctx *ctx, double *rx, double *ry, double *rz);
+int mdff_get_velocities(mdff_ctx *ctx, double *vx, double *vy, double *vz);
+int mdff_get_forces(mdff_ctx *ctx, double *fx, double *fy, double *fz);
+/* total dipoles (static + induced) of the last force call, [nion][3] */
+int mdff_get_dipoles(mdff_ctx *ctx, double *mu);
+/* electric field at the ions after the last SCF, [nion][3] (e/A^2) */
+int mdff_get_field(mdff_ctx *ctx, double *ef);
+int mdff_get_thermo(mdff_ctx *ctx, mdff_thermo *th);
+/* Ewald parameters in use (after lautoES) and the half-space k count */
+int mdff_get_ewald(mdff_ctx *ctx, double *alpha, int *kES, int *nk);
+
+/* Initial velocities of an MD start: host rand() after srand(1), exactly as
+ * init_velocities kinetic.c:15-28, uploaded to the device. */
+int mdff_init_velocities(mdff_ctx *ctx);
+
+/* Verlet list check/rebuild at the current positions (verlet.c:124-170).
+ * The first call always builds. */
+int mdff_check_verlet(mdff_ctx *ctx);
+
+/* One force evaluation at the current positions (engforce field.c:352-407):
+ * short-range pair forces, PIM dipoles (SCF), multipole Ewald forces,
+ * energies and stress.  istep is the step number seen by the ASPC ramp.
+ * Returns MDFF_ERR_SCF if the SCF stopped unconverged (results valid, as
+ * the reference accepts them). */
+int mdff_engforce(mdff_ctx *ctx, int istep, mdff_thermo *th);
+
+/* Short-range pair part only (engforce_bhmftd_pbc / engforce_nmlj_pbc):
+ * energy u, pvir and stress tau of the active potential; forces left in
+ * the context force array. */
+int mdff_pair_short(mdff_ctx *ctx, double *u, double *pvir, double tau[9]);
+
+/* multipole_ES on caller-supplied host arrays of nion rows: q[n], mu[n][3]
+ * (theta is accepted for signature parity and, as in the reference, takes
+ * no part in any term), flags MDFF_ES_*.  Outputs: u, pvir, tau[9],
+ * ef[n][3], efg[n][3][3]; forces (coulombic units) are ADDED to fx/fy/fz
+ * when non-NULL, as the reference adds them to the global force arrays. */
+int mdff_multipole_es(mdff_ctx *ctx, const double *q, const double *mu, const double *theta,
+                      int flags, double *u, double *pvir, double tau[9], double *ef, double *efg,
+                      double *fx, double *fy, double *fz);
+
+/* get_dipoles: static + SCF-induced dipoles at the current positions;
+ * mu[n][3] out, upol in eV. */
+int mdff_get_dipoles_scf(mdff_ctx *ctx, int istep, double *mu, double *upol);
+
+/* run_md start (md.c:196-211): kinetic energy + force call at t=0. */
+int mdff_md_init(mdff_ctx *ctx, mdff_thermo *th);
+/* One MD step (prop_agate + Verlet check + equilibration rescale,
+ * md.c:235-248).  State stays on the device. */
+int mdff_md_step(mdff_ctx *ctx, int istep, mdff_thermo *th);
+/* nsteps steps from step first_step; thermo of the last step.  Host and
+ * device synchronise only when the SCF loop needs its convergence flag. */
+int mdff_md_run(mdff_ctx *ctx, int first_step, int nsteps, mdff_thermo *th);
+
+/* Tang-Toennies damping f, f' at one distance (TT_damping_functions
+ * tt_damp.c:16-32), host evaluation of the exact device formula. */
+void mdff_tt_damping(double b, double c, double r, double *f, double *fd, int order);
+
+/* Tuning: lanes per ion in the neighbour kernels (16, 32 or 64; 0 = auto),
+ * atoms per block and k-split of the reciprocal kernels.  Chosen by the
+ * autotuner in mdff20_amd/autotune.py when not set. */
+int mdff_set_tuning(mdff_ctx *ctx, int lanes, int katoms, int ksplit);
+
+/* Device timing of the last engforce by phase, in ms (HIP events on the
+ * context stream): [0] short range, [1] static field, [2] SCF loop,
+ * [3] final Ewald force, [4] neighbour list, [5] total. */
+int mdff_get_timing(mdff_ctx *ctx, double t[6]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MDFF_GPU_H */

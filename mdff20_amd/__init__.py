@@ -1,0 +1,336 @@
+"""mdff20_amd -- MI355X engine for mdff20's force and integration path.
+
+Python side of the C ABI in include/mdff_gpu.h and include/mdff_host.h.
+The engine lives in libmdff_gpu.so next to this file (built by
+__graft_entry__.build()); importing a compute entry point without it raises
+instead of falling back to any CPU code.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from ._types import (ES, ERR_SCF, FP64, MIXED, NVE_LF, NVE_VV, NVT_NHCN, PIM_PCG, PIM_SCF,
+                     PIM_SCFKO, STATUS, Config, Params, Thermo)
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libmdff_gpu.so")
+_lib = None
+
+__all__ = ["lib", "MdffError", "read_control", "read_posff", "Engine", "Params", "Thermo",
+           "FP64", "MIXED", "ES", "NVE_LF", "NVE_VV", "NVT_NHCN", "PIM_SCF", "PIM_SCFKO", "PIM_PCG",
+           "device_count", "tt_damping", "init_velocities_host"]
+
+
+class MdffError(RuntimeError):
+    def __init__(self, code, where=""):
+        self.code = code
+        super().__init__(f"mdff error {code} ({STATUS.get(code, '?')}) in {where}")
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def lib():
+    """Load libmdff_gpu.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is missing: build it with __graft_entry__.build()")
+    L = C.CDLL(LIB_PATH)
+    P = C.POINTER
+    dp, ip, vp = P(C.c_double), P(C.c_int), C.c_void_p
+    sig = {
+        "mdff_version": (C.c_char_p, []),
+        "mdff_device_count": (C.c_int, []),
+        "mdff_sizeof_params": (C.c_size_t, []),
+        "mdff_sizeof_thermo": (C.c_size_t, []),
+        "mdff_sizeof_config": (C.c_size_t, []),
+        "mdff_params_default": (None, [P(Params)]),
+        "mdff_read_control": (C.c_int, [C.c_char_p, C.c_int, P(Params)]),
+        "mdff_read_posff": (C.c_int, [C.c_char_p, C.c_int, P(Config)]),
+        "mdff_write_config": (C.c_int, [C.c_char_p, C.c_char_p, P(Config)]),
+        "mdff_config_alloc": (C.c_int, [P(Config), C.c_int]),
+        "mdff_config_free": (None, [P(Config)]),
+        "mdff_host_init_velocities": (C.c_int, [P(Params), C.c_int, dp, dp, dp, dp, C.c_int]),
+        "mdff_tt_damping": (None, [C.c_double, C.c_double, C.c_double, dp, dp, C.c_int]),
+        "mdff_create": (C.c_int, [P(vp), P(Params), C.c_int, ip, dp, dp, dp, dp, C.c_int, C.c_int]),
+        "mdff_destroy": (None, [vp]),
+        "mdff_set_positions": (C.c_int, [vp, dp, dp, dp]),
+        "mdff_set_velocities": (C.c_int, [vp, dp, dp, dp]),
+        "mdff_get_positions": (C.c_int, [vp, dp, dp, dp]),
+        "mdff_get_velocities": (C.c_int, [vp, dp, dp, dp]),
+        "mdff_get_forces": (C.c_int, [vp, dp, dp, dp]),
+        "mdff_get_dipoles": (C.c_int, [vp, dp]),
+        "mdff_get_field": (C.c_int, [vp, dp]),
+        "mdff_get_thermo": (C.c_int, [vp, P(Thermo)]),
+        "mdff_get_ewald": (C.c_int, [vp, dp, ip, ip]),
+        "mdff_init_velocities": (C.c_int, [vp]),
+        "mdff_check_verlet": (C.c_int, [vp]),
+        "mdff_engforce": (C.c_int, [vp, C.c_int, P(Thermo)]),
+        "mdff_pair_short": (C.c_int, [vp, dp, dp, dp]),
+        "mdff_multipole_es": (C.c_int, [vp, dp, dp, dp, C.c_int, dp, dp, dp, dp, dp, dp, dp, dp]),
+        "mdff_get_dipoles_scf": (C.c_int, [vp, C.c_int, dp, dp]),
+        "mdff_md_init": (C.c_int, [vp, P(Thermo)]),
+        "mdff_md_step": (C.c_int, [vp, C.c_int, P(Thermo)]),
+        "mdff_md_run": (C.c_int, [vp, C.c_int, C.c_int, P(Thermo)]),
+        "mdff_set_tuning": (C.c_int, [vp, C.c_int, C.c_int, C.c_int]),
+        "mdff_get_timing": (C.c_int, [vp, dp]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    if L.mdff_sizeof_params() != C.sizeof(Params) or L.mdff_sizeof_thermo() != C.sizeof(Thermo) \
+            or L.mdff_sizeof_config() != C.sizeof(Config):
+        raise RuntimeError("struct layout mismatch between _types.py and mdff_types.h")
+    _lib = L
+    return L
+
+
+def device_count():
+    return lib().mdff_device_count()
+
+
+def _check(rc, where, allow_scf=True):
+    if rc != 0 and not (allow_scf and rc == ERR_SCF):
+        raise MdffError(rc, where)
+    return rc
+
+
+def read_control(path, ntype):
+    """Parse an mdff20 control file (keywords of global.c, md.c, field.c,
+    bhmftd.c, nmlj.c, pim.c) into Params."""
+    p = Params()
+    _check(lib().mdff_read_control(os.fsencode(path), ntype, C.byref(p)), "read_control", False)
+    return p
+
+
+def default_params():
+    p = Params()
+    lib().mdff_params_default(C.byref(p))
+    return p
+
+
+class PosffData:
+    """Content of a POSFF file with numpy arrays (positions wrapped to the
+    primary cell as read_config does)."""
+
+    def __init__(self, cfg):
+        n = cfg.nion
+        self.nion = n
+        self.ntype = cfg.ntype
+        self.name = cfg.name.decode()
+        self.A = np.array([[cfg.A[i][j] for j in range(3)] for i in range(3)])
+        self.atypit = [cfg.atypit[t].value.decode() for t in range(cfg.ntype)]
+        self.nionit = [cfg.nionit[t] for t in range(cfg.ntype)]
+        self.typia = np.ctypeslib.as_array(cfg.typia, (n,)).copy()
+        g = lambda p_: np.ctypeslib.as_array(p_, (n,)).copy()
+        self.r = np.stack([g(cfg.rx), g(cfg.ry), g(cfg.rz)])
+        self.v = np.stack([g(cfg.vx), g(cfg.vy), g(cfg.vz)])
+        self.f = np.stack([g(cfg.fx), g(cfg.fy), g(cfg.fz)])
+
+
+def read_posff(path, Fposff=2):
+    cfg = Config()
+    L = lib()
+    _check(L.mdff_read_posff(os.fsencode(path), Fposff, C.byref(cfg)), "read_posff", False)
+    try:
+        return PosffData(cfg)
+    finally:
+        L.mdff_config_free(C.byref(cfg))
+
+
+def write_config(path, data, mode="w"):
+    """CONTFF/TRAJFF writer (config.c:291-345)."""
+    L = lib()
+    cfg = Config()
+    _check(L.mdff_config_alloc(C.byref(cfg), data.nion), "config_alloc", False)
+    try:
+        cfg.ntype = data.ntype
+        cfg.name = data.name.encode()[:80]
+        for i in range(3):
+            for j in range(3):
+                cfg.A[i][j] = float(data.A[i, j])
+        for t in range(data.ntype):
+            cfg.atypit[t].value = data.atypit[t].encode()[:80]
+            cfg.nionit[t] = data.nionit[t]
+        for k, name in enumerate(("rx", "ry", "rz")):
+            C.memmove(getattr(cfg, name), np.ascontiguousarray(data.r[k]).ctypes.data, 8 * data.nion)
+        for k, name in enumerate(("vx", "vy", "vz")):
+            C.memmove(getattr(cfg, name), np.ascontiguousarray(data.v[k]).ctypes.data, 8 * data.nion)
+        for k, name in enumerate(("fx", "fy", "fz")):
+            C.memmove(getattr(cfg, name), np.ascontiguousarray(data.f[k]).ctypes.data, 8 * data.nion)
+        C.memmove(cfg.typia, np.ascontiguousarray(data.typia, dtype=np.int32).ctypes.data, 4 * data.nion)
+        _check(L.mdff_write_config(os.fsencode(path), mode.encode(), C.byref(cfg)), "write_config", False)
+    finally:
+        L.mdff_config_free(C.byref(cfg))
+
+
+def tt_damping(b, c, r, order):
+    f, fd = C.c_double(), C.c_double()
+    lib().mdff_tt_damping(b, c, r, C.byref(f), C.byref(fd), order)
+    return f.value, fd.value
+
+
+def init_velocities_host(params, mass):
+    """Initial velocities of an MD start (kinetic.c:15-28) from rand() after
+    srand(1); returns (3, n)."""
+    mass = np.ascontiguousarray(mass, dtype=np.float64)
+    n = mass.size
+    v = np.zeros((3, n))
+    vx, vy, vz = v[0].copy(), v[1].copy(), v[2].copy()
+    _check(lib().mdff_host_init_velocities(C.byref(params), n, _dp(mass), _dp(vx), _dp(vy), _dp(vz), 1),
+           "init_velocities", False)
+    return np.stack([vx, vy, vz])
+
+
+class Engine:
+    """One periodic system on one GPU (handle API of mdff_gpu.h)."""
+
+    def __init__(self, params, typia, A, r, precision=FP64, device=0):
+        L = lib()
+        self._L = L
+        self.params = params
+        typia = np.ascontiguousarray(typia, dtype=np.int32)
+        self.n = typia.size
+        A = np.ascontiguousarray(np.asarray(A, dtype=np.float64).reshape(9))
+        r = np.ascontiguousarray(np.asarray(r, dtype=np.float64).reshape(3, self.n))
+        h = C.c_void_p()
+        rx, ry, rz = (np.ascontiguousarray(r[k]) for k in range(3))
+        rc = L.mdff_create(C.byref(h), C.byref(params), self.n, typia.ctypes.data_as(C.POINTER(C.c_int)),
+                           _dp(A), _dp(rx), _dp(ry), _dp(rz), precision, device)
+        _check(rc, "create", False)
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._L.mdff_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # ---- state
+    def _get3(self, fn):
+        a = [np.zeros(self.n) for _ in range(3)]
+        _check(fn(self._h, _dp(a[0]), _dp(a[1]), _dp(a[2])), fn.__name__, False)
+        return np.stack(a)
+
+    def _set3(self, fn, v):
+        v = np.asarray(v, dtype=np.float64).reshape(3, self.n)
+        a = [np.ascontiguousarray(v[k]) for k in range(3)]
+        _check(fn(self._h, _dp(a[0]), _dp(a[1]), _dp(a[2])), fn.__name__, False)
+
+    def set_positions(self, r):
+        self._set3(self._L.mdff_set_positions, r)
+
+    def set_velocities(self, v):
+        self._set3(self._L.mdff_set_velocities, v)
+
+    def positions(self):
+        return self._get3(self._L.mdff_get_positions)
+
+    def velocities(self):
+        return self._get3(self._L.mdff_get_velocities)
+
+    def forces(self):
+        return self._get3(self._L.mdff_get_forces)
+
+    def dipoles(self):
+        mu = np.zeros((self.n, 3))
+        _check(self._L.mdff_get_dipoles(self._h, _dp(mu)), "get_dipoles", False)
+        return mu
+
+    def field(self):
+        ef = np.zeros((self.n, 3))
+        _check(self._L.mdff_get_field(self._h, _dp(ef)), "get_field", False)
+        return ef
+
+    def thermo(self):
+        t = Thermo()
+        _check(self._L.mdff_get_thermo(self._h, C.byref(t)), "get_thermo", False)
+        return t.as_dict()
+
+    def ewald(self):
+        a, k, nk = C.c_double(), (C.c_int * 3)(), C.c_int()
+        _check(self._L.mdff_get_ewald(self._h, C.byref(a), k, C.byref(nk)), "get_ewald", False)
+        return a.value, list(k), nk.value
+
+    def timing(self):
+        t = np.zeros(6)
+        _check(self._L.mdff_get_timing(self._h, _dp(t)), "get_timing", False)
+        return t
+
+    def set_tuning(self, lanes=0, katoms=0, ksplit=0):
+        _check(self._L.mdff_set_tuning(self._h, lanes, katoms, ksplit), "set_tuning", False)
+
+    # ---- compute
+    def init_velocities(self):
+        _check(self._L.mdff_init_velocities(self._h), "init_velocities", False)
+
+    def check_verlet(self):
+        _check(self._L.mdff_check_verlet(self._h), "check_verlet", False)
+
+    def engforce(self, istep=0):
+        t = Thermo()
+        rc = _check(self._L.mdff_engforce(self._h, istep, C.byref(t)), "engforce")
+        d = t.as_dict()
+        d["status"] = rc
+        return d
+
+    def pair_short(self):
+        u, pv = C.c_double(), C.c_double()
+        tau = np.zeros(9)
+        _check(self._L.mdff_pair_short(self._h, C.byref(u), C.byref(pv), _dp(tau)), "pair_short", False)
+        return u.value, pv.value, tau.reshape(3, 3)
+
+    def dipoles_scf(self, istep=0):
+        mu = np.zeros((self.n, 3))
+        up = C.c_double()
+        _check(self._L.mdff_get_dipoles_scf(self._h, istep, _dp(mu), C.byref(up)), "get_dipoles")
+        return mu, up.value
+
+    def multipole_es(self, q, mu, flags, forces=None):
+        """multipole_ES on host arrays; returns dict(u, pvir, tau, ef, efg,
+        f) with f the coulombic-unit forces added to `forces` (or zeros)."""
+        q = np.ascontiguousarray(q, dtype=np.float64)
+        mu = np.ascontiguousarray(np.asarray(mu, dtype=np.float64).reshape(self.n, 3))
+        theta = np.zeros((self.n, 9))
+        u, pv = C.c_double(), C.c_double()
+        tau = np.zeros(9)
+        ef = np.zeros((self.n, 3))
+        efg = np.zeros((self.n, 9))
+        f = np.zeros((3, self.n)) if forces is None else np.array(forces, dtype=np.float64).reshape(3, self.n)
+        fx, fy, fz = (np.ascontiguousarray(f[k]) for k in range(3))
+        _check(self._L.mdff_multipole_es(self._h, _dp(q), _dp(mu), _dp(theta), flags, C.byref(u), C.byref(pv),
+                                         _dp(tau), _dp(ef), _dp(efg), _dp(fx), _dp(fy), _dp(fz)),
+               "multipole_es", False)
+        return dict(u=u.value, pvir=pv.value, tau=tau.reshape(3, 3), ef=ef, efg=efg.reshape(self.n, 3, 3),
+                    f=np.stack([fx, fy, fz]))
+
+    def md_init(self):
+        t = Thermo()
+        _check(self._L.mdff_md_init(self._h, C.byref(t)), "md_init")
+        return t.as_dict()
+
+    def md_step(self, istep):
+        t = Thermo()
+        _check(self._L.mdff_md_step(self._h, istep, C.byref(t)), "md_step")
+        return t.as_dict()
+
+    def md_run(self, first_step, nsteps):
+        t = Thermo()
+        _check(self._L.mdff_md_run(self._h, first_step, nsteps, C.byref(t)), "md_run")
+        return t.as_dict()

@@ -1,0 +1,1318 @@
+// engine.hip -- context, orchestration and C ABI of the MI355X engine for
+// mdff20's force and integration path.  One HIP stream per context; state is
+// device resident; the host synchronises only to read the SCF convergence
+// flag, the neighbour-list overflow flag and observables.
+#include <hip/hip_runtime.h>
+
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "kernels.h"
+#include "md_kernels.h"
+#include "mdff_gpu.h"
+#include "mdff_host.h"
+
+#define HIPCHK(x)                                                                                 \
+    do {                                                                                          \
+        hipError_t e_ = (x);                                                                      \
+        if (e_ != hipSuccess) {                                                                   \
+            fprintf(stderr, "mdff: HIP error %s at %s:%d\n", hipGetErrorString(e_), __FILE__, __LINE__); \
+            return MDFF_ERR_DEVICE;                                                               \
+        }                                                                                         \
+    } while (0)
+
+namespace {
+
+template <typename T>
+int dalloc(T **p, size_t n)
+{
+    if (n == 0) n = 1;
+    if (hipMalloc((void **)p, n * sizeof(T)) != hipSuccess) return MDFF_ERR_NOMEM;
+    if (hipMemset(*p, 0, n * sizeof(T)) != hipSuccess) return MDFF_ERR_DEVICE;
+    return MDFF_OK;
+}
+
+template <typename T>
+void dfree(T *&p)
+{
+    if (p) (void)hipFree((void *)p);
+    p = nullptr;
+}
+
+inline unsigned cdiv(size_t a, size_t b) { return (unsigned)((a + b - 1) / b); }
+
+}  // namespace
+
+struct mdff_ctx {
+    mdff_params p;
+    mdff_units u;
+    mdff_cell cell;
+    CellD cd;
+    int n = 0, ntype = 0, precision = MDFF_FP64, device = 0;
+    hipStream_t st = nullptr;
+    // derived (field.c:186-335, md.c:120-141)
+    int lqch = 0, ldip_static = 0, lpol = 0, ldmp = 0, lpim = 0, lnonbonded = 0;
+    int ldip_sticky = 0;  // coulombic.c:115
+    int integ = MDFF_NVE_VV, rescale_allowed = 0;
+    double srcutsq = 0, lrcutsq = 0, alpha = 0;
+    int kES[3] = {0, 0, 0};
+    double kT = 0, dt = 0, tauT = 0, timesca = 0;
+    double selfa = 0, selfa2 = 0, ttpiV = 0, tfpiV = 0;
+    int npol = 0;
+    std::vector<int> typ;
+    std::vector<double> mass;
+    TypeTab tab;
+    // device state (SoA, 3n unless stated)
+    double *r = nullptr, *v = nullptr, *f = nullptr, *fold = nullptr, *rs = nullptr, *s = nullptr, *s0 = nullptr;
+    int *type = nullptr;
+    double *invmass = nullptr, *massd = nullptr, *q = nullptr, *qz = nullptr;
+    double *mu_static = nullptr, *mu_ind = nullptr, *mu_tot = nullptr, *hist = nullptr;
+    double *ef_stat = nullptr, *ef = nullptr, *ef_dir = nullptr, *ef_rec = nullptr, *efg = nullptr,
+           *efg_dir = nullptr, *efg_rec = nullptr;
+    double *f_sr = nullptr, *f_dir = nullptr, *f_rec = nullptr;
+    double *acc_sr = nullptr;  // 14n
+    double *uw_dir = nullptr;  // 7n
+    double *selfbuf = nullptr; // 2n
+    double *ke = nullptr;      // n
+    double *upol_i = nullptr, *rmsd_i = nullptr;  // n
+    double *tmp_q = nullptr, *tmp_mu = nullptr, *tmp_qraw = nullptr, *tmp_mur = nullptr;  // drop-in scratch
+    double *scal = nullptr;
+    int *ctl = nullptr;
+    TypeTab *tab_d = nullptr;
+    int *pidx = nullptr;
+    // neighbour list
+    unsigned *nbr = nullptr;
+    int *cnt = nullptr;
+    int cap = 0, have_list = 0;
+    // k space
+    int Kx = 0, Ky = 0, Kz = 0, P = 0;
+    size_t nkf = 0;
+    int nk = 0;
+    double *kvec = nullptr, *Ak = nullptr, *kcoe = nullptr, *W = nullptr, *ek = nullptr;
+    void *tx = nullptr, *ty = nullptr, *tz = nullptr;     // all ions
+    void *pxx = nullptr, *pyy = nullptr, *pzz = nullptr;  // polarisable ions
+    double *kpart = nullptr, *bpart = nullptr;
+    int nchunk = 0, nsplit = 0, chunk = 0, rows_per_split = 0;
+    int nchunk_p = 0, chunk_p = 0;
+    // tuning
+    int lanes = 32;
+    // host pinned mirrors
+    int *ctl_h = nullptr;
+    double *scal_h = nullptr;
+    // NHC constants
+    double nhcQ[16];
+    double yosh[7];
+    // timing
+    hipEvent_t ev[8];
+    int have_events = 0;
+    double t_ms[6] = {0, 0, 0, 0, 0, 0};
+    int last_nscf = 0;
+    double last_rmsd = 0;
+    int scf_chunk = 4;
+};
+
+// ------------------------------------------------------------------- helpers
+static size_t rsize(const mdff_ctx *c) { return c->precision == MDFF_MIXED ? sizeof(float) : sizeof(double); }
+
+static int sync_ctl(mdff_ctx *c)
+{
+    HIPCHK(hipMemcpyAsync(c->ctl_h, c->ctl, sizeof(int) * C_COUNT, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    return MDFF_OK;
+}
+
+static int sync_scal(mdff_ctx *c)
+{
+    HIPCHK(hipMemcpyAsync(c->scal_h, c->scal, sizeof(double) * S_COUNT, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipMemcpyAsync(c->ctl_h, c->ctl, sizeof(int) * C_COUNT, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    return MDFF_OK;
+}
+
+static void set_int(mdff_ctx *c, int slot, int val)
+{
+    hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, c->st, c->ctl + slot, val);
+}
+
+static void colsum(mdff_ctx *c, const double *in, int n, int ld, int ncols, double *out, const int *active = nullptr)
+{
+    hipLaunchKernelGGL(k_colsum, dim3(ncols), dim3(1024), 0, c->st, in, n, ld, out, active);
+}
+
+// ------------------------------------------------------------- type tables
+static void build_tables(mdff_ctx *c)
+{
+    const mdff_params &p = c->p;
+    TypeTab &T = c->tab;
+    memset(&T, 0, sizeof(T));
+    T.ntype = c->ntype;
+    T.lbh = p.lbhmft || p.lbhmftd;
+    T.lbhd = p.lbhmftd;
+    T.llj = p.lnmlj;
+    T.trunc = p.trunc;
+    double two16 = pow(2.0, 1.0 / 6.0);
+    for (int i = 0; i < NT; i++)
+        for (int j = 0; j < NT; j++) {
+            T.A[i][j] = p.Abhmftd[i][j];
+            T.B[i][j] = p.Bbhmftd[i][j];
+            T.C[i][j] = p.Cbhmftd[i][j];
+            T.D[i][j] = p.Dbhmftd[i][j];
+            T.BD[i][j] = p.BDbhmftd[i][j];
+            // init_nmlj nmlj.c:196-232
+            double pp = p.plj[i][j], qq = p.qlj[i][j];
+            T.plj[i][j] = pp;
+            T.qlj[i][j] = qq;
+            T.epsqp[i][j] = p.epslj[i][j] / (qq - pp);
+            T.sigsq[i][j] = two16 * two16 * p.sigmalj[i][j] * p.sigmalj[i][j];
+            T.fc[i][j] = (pp * qq * T.epsqp[i][j]) / T.sigsq[i][j];
+            T.ptwo[i][j] = pp * 0.5;
+            T.qtwo[i][j] = qq * 0.5;
+            if (c->srcutsq > 0) {
+                double sr2 = T.sigsq[i][j] / c->srcutsq, sr = sqrt(sr2);
+                T.uc[i][j] = T.epsqp[i][j] * (pp * pow(sr, qq) - qq * pow(sr, pp));
+            }
+            T.ldamp[i][j] = (p.lpoldamping[i][i][j] || p.lpoldamping[j][i][j]) ? 1 : 0;
+            T.db[i][j] = p.pol_damp_b[i][i][j];
+            T.dc[i][j] = p.pol_damp_c[i][i][j];
+            T.dk[i][j] = p.pol_damp_k[i][i][j];
+        }
+    T.E_TT[0] = 1.0;
+    for (int k = 1; k <= MDFF_MAX_TT_ORDER; k++) T.E_TT[k] = T.E_TT[k - 1] / (double)k;
+    for (int t = 0; t < NT; t++) {
+        T.q[t] = p.lcoulombic ? p.qit[t] : 0.0;
+        T.mass[t] = p.massit[t];
+        T.polar[t] = (p.lcoulombic && p.lpolar[t]) ? 1 : 0;
+        for (int k = 0; k < 9; k++) T.pol[t][k] = T.polar[t] ? p.polit[t][k / 3][k % 3] : 0.0;
+        if (T.polar[t]) {
+            // invertmatrix3x3 inverse_matrix.c:6-23, closed-form adjugate
+            const double *m = T.pol[t];
+            double det = m[0] * (m[4] * m[8] - m[5] * m[7]) - m[1] * (m[3] * m[8] - m[5] * m[6]) +
+                         m[2] * (m[3] * m[7] - m[4] * m[6]);
+            double id = 1.0 / det;
+            T.invpol[t][0] = (m[4] * m[8] - m[5] * m[7]) * id;
+            T.invpol[t][1] = (m[2] * m[7] - m[1] * m[8]) * id;
+            T.invpol[t][2] = (m[1] * m[5] - m[2] * m[4]) * id;
+            T.invpol[t][3] = (m[5] * m[6] - m[3] * m[8]) * id;
+            T.invpol[t][4] = (m[0] * m[8] - m[2] * m[6]) * id;
+            T.invpol[t][5] = (m[2] * m[3] - m[0] * m[5]) * id;
+            T.invpol[t][6] = (m[3] * m[7] - m[4] * m[6]) * id;
+            T.invpol[t][7] = (m[1] * m[6] - m[0] * m[7]) * id;
+            T.invpol[t][8] = (m[0] * m[4] - m[1] * m[3]) * id;
+        }
+    }
+}
+
+// ------------------------------------------------------------ neighbour list
+static int build_list(mdff_ctx *c, int force)
+{
+    int n = c->n;
+    hipLaunchKernelGGL(k_frac, dim3(cdiv(n, 256)), dim3(256), 0, c->st, n, c->cd, c->r, c->r + n, c->r + 2 * n, c->s);
+    double csr = c->p.cutshortrange + c->p.skindiff, clr = c->p.cutlongrange + c->p.skindiff;
+    int all = !c->p.lverletL;
+    for (int attempt = 0; attempt < 3; attempt++) {
+        set_int(c, C_OVERFLOW, 0);
+        set_int(c, C_MAXCNT, 0);
+        if (all) {
+            // no Verlet list: every pair is a candidate, independent of positions
+            set_int(c, C_REBUILD, (force || attempt > 0 || !c->have_list) ? 1 : 0);
+        } else {
+            if (force || attempt > 0) set_int(c, C_FIRST, 1);
+            hipLaunchKernelGGL(k_vcheck, dim3(1), dim3(1024), 0, c->st, n, c->cd, c->s, c->s0, c->p.skindiff, c->ctl);
+            if (attempt > 0) hipLaunchKernelGGL(k_add_int, dim3(1), dim3(1), 0, c->st, c->ctl + C_NUPD, -1);
+        }
+        hipLaunchKernelGGL(k_build_allpairs, dim3(cdiv(n, 4)), dim3(256), 0, c->st, n, c->cd, c->s, csr * csr,
+                           clr * clr, all, c->cap, c->nbr, c->cnt, c->s0, c->ctl);
+        int rc = sync_ctl(c);
+        if (rc) return rc;
+        if (c->ctl_h[C_OVERFLOW] == 0) {
+            c->have_list = 1;
+            return MDFF_OK;
+        }
+        // grow the capacity and build again at the same positions
+        int need = c->ctl_h[C_MAXCNT];
+        int ncap = ((need * 5) / 4 + 64 + 63) / 64 * 64;
+        if (ncap > ((n + 63) / 64) * 64) ncap = ((n + 63) / 64) * 64;
+        dfree(c->nbr);
+        if (dalloc(&c->nbr, (size_t)n * ncap)) return MDFF_ERR_NOMEM;
+        c->cap = ncap;
+    }
+    return MDFF_ERR_LIST_OVERFLOW;
+}
+
+// ----------------------------------------------------------------- dispatch
+template <typename real, bool EF, bool EQ, bool EM, bool F>
+static void launch_dir_t(mdff_ctx *c, const DirArgs &a)
+{
+    int L = c->lanes;
+    unsigned blocks = cdiv((size_t)c->n * L, 256);
+    if (L == 16) hipLaunchKernelGGL((k_ewald_dir<real, 16, EF, EQ, EM, F>), dim3(blocks), dim3(256), 0, c->st, a);
+    else if (L == 64) hipLaunchKernelGGL((k_ewald_dir<real, 64, EF, EQ, EM, F>), dim3(blocks), dim3(256), 0, c->st, a);
+    else hipLaunchKernelGGL((k_ewald_dir<real, 32, EF, EQ, EM, F>), dim3(blocks), dim3(256), 0, c->st, a);
+}
+
+template <typename real>
+static void launch_dir_r(mdff_ctx *c, const DirArgs &a, int ef, int eq, int em, int f)
+{
+    int code = (ef ? 8 : 0) | (eq ? 4 : 0) | (em ? 2 : 0) | (f ? 1 : 0);
+    switch (code) {
+    case 8: launch_dir_t<real, true, false, false, false>(c, a); break;
+    case 1: launch_dir_t<real, false, false, false, true>(c, a); break;
+    case 3: launch_dir_t<real, false, false, true, true>(c, a); break;
+    case 9: launch_dir_t<real, true, false, false, true>(c, a); break;
+    case 11: launch_dir_t<real, true, false, true, true>(c, a); break;
+    case 14: launch_dir_t<real, true, true, true, false>(c, a); break;
+    case 15: launch_dir_t<real, true, true, true, true>(c, a); break;
+    case 6: launch_dir_t<real, false, true, true, false>(c, a); break;
+    case 7: launch_dir_t<real, false, true, true, true>(c, a); break;
+    case 10: launch_dir_t<real, true, false, true, false>(c, a); break;
+    case 2: launch_dir_t<real, false, false, true, false>(c, a); break;
+    case 0: break;
+    default: launch_dir_t<real, true, true, true, true>(c, a); break;
+    }
+}
+
+static void launch_dir(mdff_ctx *c, const DirArgs &a, int ef, int eq, int em, int f)
+{
+    if (c->precision == MDFF_MIXED) launch_dir_r<float>(c, a, ef, eq, em, f);
+    else launch_dir_r<double>(c, a, ef, eq, em, f);
+}
+
+static DirArgs dir_args(mdff_ctx *c, const double *q, const double *mu, int lcouldamp, int q_zero,
+                        const int *active)
+{
+    DirArgs a;
+    memset(&a, 0, sizeof(a));
+    a.n = c->n;
+    a.cap = c->cap;
+    a.c = c->cd;
+    a.s = c->s;
+    a.type = c->type;
+    a.nbr = c->nbr;
+    a.cnt = c->cnt;
+    a.T = c->tab_d;
+    a.q = q;
+    a.mu = mu;
+    a.alpha = c->alpha;
+    a.rc2 = c->lrcutsq;
+    a.lcouldamp = lcouldamp;
+    a.active = active;
+    a.q_zero = q_zero;
+    a.ef = c->ef_dir;
+    a.efg = c->efg_dir;
+    a.f = c->f_dir;
+    a.uw = c->uw_dir;
+    return a;
+}
+
+template <typename real>
+static void launch_eikr_t(mdff_ctx *c, const int *idx, int nsel, void *tx, void *ty, void *tz, const int *active)
+{
+    hipLaunchKernelGGL(k_eikr<real>, dim3(cdiv(nsel, 128)), dim3(128), 0, c->st, c->n, c->cd, c->Kx, c->Ky, c->Kz,
+                       c->r, c->r + c->n, c->r + 2 * c->n, idx, nsel, (real *)tx, (real *)ty, (real *)tz, active);
+}
+
+static void launch_eikr(mdff_ctx *c, int polar, const int *active)
+{
+    if (polar) {
+        if (c->precision == MDFF_MIXED) launch_eikr_t<float>(c, c->pidx, c->npol, c->pxx, c->pyy, c->pzz, active);
+        else launch_eikr_t<double>(c, c->pidx, c->npol, c->pxx, c->pyy, c->pzz, active);
+    } else {
+        if (c->precision == MDFF_MIXED) launch_eikr_t<float>(c, nullptr, c->n, c->tx, c->ty, c->tz, active);
+        else launch_eikr_t<double>(c, nullptr, c->n, c->tx, c->ty, c->tz, active);
+    }
+}
+
+template <typename real, int C>
+static void launch_fwd_t(mdff_ctx *c, const SkArgs &a, int nchunk)
+{
+    const int NY = 2 * c->Ky + 1, NZ = 2 * c->Kz + 1;
+    size_t sh = (size_t)(2 * 32 + 2 * NY * 32 + 2 * NZ * 32 + C * 32) * sizeof(real);
+    int ntile = (c->P + 255) / 256;
+    hipLaunchKernelGGL((k_sk_fwd<real, C>), dim3((c->Kx + 1) * ntile, nchunk), dim3(256), sh, c->st, a);
+}
+
+template <typename real, int NCH>
+static void launch_back_t(mdff_ctx *c, const BackArgs &a, int nsel)
+{
+    const int NZ = 2 * c->Kz + 1;
+    size_t sh = (size_t)(2 * NZ * 64) * sizeof(real);
+    hipLaunchKernelGGL((k_sk_back<real, NCH>), dim3(cdiv(nsel, 64), c->nsplit), dim3(64), sh, c->st, a);
+}
+
+// reciprocal part of multipole_ES (ewald.c:876-1051) on the selected ions.
+// polar: use the polarisable-ion tables (SCF); C: channels of S(k);
+// NCH: 3 (field) or 9 (field + gradient); do_ef/do_f outputs; efg optional.
+static void rec_part(mdff_ctx *c, int polar, int C, int NCH, const double *q_w, const double *mu_w,
+                     const double *q_f, const double *mu_f, int do_ef, int do_f, int do_stress, double *efg_out,
+                     const int *active)
+{
+    int nsel = polar ? c->npol : c->n;
+    int chunk = polar ? c->chunk_p : c->chunk;
+    int nchunk = polar ? c->nchunk_p : c->nchunk;
+    SkArgs a;
+    a.nsel = nsel;
+    a.Kx = c->Kx; a.Ky = c->Ky; a.Kz = c->Kz; a.P = c->P;
+    a.tx = polar ? c->pxx : c->tx;
+    a.ty = polar ? c->pyy : c->ty;
+    a.tz = polar ? c->pzz : c->tz;
+    a.idx = polar ? c->pidx : nullptr;
+    a.q = q_w; a.mu = mu_w; a.n = c->n;
+    a.chunk = chunk;
+    a.part = c->kpart;
+    a.active = active;
+    bool mixed = c->precision == MDFF_MIXED;
+    if (C == 1) { if (mixed) launch_fwd_t<float, 1>(c, a, nchunk); else launch_fwd_t<double, 1>(c, a, nchunk); }
+    else if (C == 3) { if (mixed) launch_fwd_t<float, 3>(c, a, nchunk); else launch_fwd_t<double, 3>(c, a, nchunk); }
+    else { if (mixed) launch_fwd_t<float, 4>(c, a, nchunk); else launch_fwd_t<double, 4>(c, a, nchunk); }
+    KRedArgs r;
+    r.Kx = c->Kx; r.Ky = c->Ky; r.Kz = c->Kz; r.P = c->P; r.nchunk = nchunk; r.C = C;
+    r.part = c->kpart; r.kvec = c->kvec; r.Ak = c->Ak; r.kcoe = c->kcoe; r.W = c->W; r.ek = c->ek;
+    r.do_stress = do_stress; r.active = active;
+    hipLaunchKernelGGL(k_sk_reduce, dim3(cdiv(c->nkf, 256)), dim3(256), 0, c->st, r);
+    if (do_ef || do_f) {
+        BackArgs b;
+        b.nsel = nsel; b.Kx = c->Kx; b.Ky = c->Ky; b.Kz = c->Kz; b.P = c->P;
+        b.tx = a.tx; b.ty = a.ty; b.tz = a.tz;
+        b.W = c->W;
+        b.rows_per_split = c->rows_per_split;
+        b.part = c->bpart;
+        b.active = active;
+        if (NCH == 9) { if (mixed) launch_back_t<float, 9>(c, b, nsel); else launch_back_t<double, 9>(c, b, nsel); }
+        else { if (mixed) launch_back_t<float, 3>(c, b, nsel); else launch_back_t<double, 3>(c, b, nsel); }
+        BackFinArgs fa;
+        fa.n = c->n; fa.nsel = nsel; fa.nsplit = c->nsplit; fa.NCH = NCH;
+        fa.idx = a.idx; fa.c = c->cd; fa.tfpiV = c->tfpiV; fa.part = c->bpart;
+        fa.q = q_f; fa.mu = mu_f;
+        fa.ef = c->ef_rec; fa.efg = efg_out; fa.f = c->f_rec;
+        fa.do_ef = do_ef; fa.do_f = do_f; fa.active = active;
+        hipLaunchKernelGGL(k_sk_back_fin, dim3(cdiv(nsel, 256)), dim3(256), 0, c->st, fa);
+    }
+}
+
+static void short_range(mdff_ctx *c)
+{
+    int n = c->n;
+    if (!c->lnonbonded) return;
+    ShortArgs a;
+    a.n = n; a.cap = c->cap; a.c = c->cd; a.s = c->s; a.type = c->type; a.nbr = c->nbr; a.cnt = c->cnt;
+    a.T = c->tab_d; a.rc2 = c->srcutsq; a.f = c->f_sr; a.acc = c->acc_sr;
+    int L = c->lanes;
+    unsigned blocks = cdiv((size_t)n * L, 256);
+    bool mixed = c->precision == MDFF_MIXED;
+    if (L == 16) {
+        if (mixed) hipLaunchKernelGGL((k_short<float, 16>), dim3(blocks), dim3(256), 0, c->st, a);
+        else hipLaunchKernelGGL((k_short<double, 16>), dim3(blocks), dim3(256), 0, c->st, a);
+    } else if (L == 64) {
+        if (mixed) hipLaunchKernelGGL((k_short<float, 64>), dim3(blocks), dim3(256), 0, c->st, a);
+        else hipLaunchKernelGGL((k_short<double, 64>), dim3(blocks), dim3(256), 0, c->st, a);
+    } else {
+        if (mixed) hipLaunchKernelGGL((k_short<float, 32>), dim3(blocks), dim3(256), 0, c->st, a);
+        else hipLaunchKernelGGL((k_short<double, 32>), dim3(blocks), dim3(256), 0, c->st, a);
+    }
+    colsum(c, c->acc_sr, n, n, 14, c->scal + S_U_BH);
+}
+
+// static field of momentpolaSCF pim.c:216-218: charges (+ static dipoles)
+// with damping, field only
+static void static_field(mdff_ctx *c)
+{
+    int n = c->n;
+    DirArgs a = dir_args(c, c->q, c->mu_static, c->ldmp, 0, nullptr);
+    launch_dir(c, a, 1, 0, 0, 0);
+    int C = c->ldip_static ? 4 : 1;
+    rec_part(c, 0, C, 3, c->q, c->mu_static, c->q, c->mu_static, 1, 0, 0, nullptr, nullptr);
+    hipLaunchKernelGGL(k_field_sum, dim3(cdiv(3 * n, 256)), dim3(256), 0, c->st, n, c->selfa2, c->mu_static,
+                       c->ef_dir, c->ef_rec, (const double *)nullptr, c->ef_stat, (const int *)nullptr);
+}
+
+// the SCF loop of momentpolaSCF pim.c:236-278 on the device, gated by the
+// convergence flag; host polls the flag every scf_chunk iterations.
+static int scf_loop(mdff_ctx *c, int istep)
+{
+    int n = c->n;
+    const int *act = c->ctl + C_SCF_ACTIVE;
+    int maxit = c->p.max_scf_pim_iter, minit = c->p.min_scf_pim_iter;
+    hipLaunchKernelGGL(k_scf_begin, dim3(1), dim3(1), 0, c->st, c->ctl, (maxit > 0 || minit > 0) ? 1 : 0);
+    // ASPC order and coefficients (pim.c:324-379)
+    int order = c->p.extrapolate_order;
+    int eo = (istep >= order + 1) ? order : (istep ? istep - 1 : 0);
+    double B[6] = {0, 0, 0, 0, 0, 0}, Wc = 0.0;
+    switch (eo) {
+    case 0: B[0] = 1.0; Wc = 0.0; break;
+    case 1: B[0] = 2.0; B[1] = -1.0; Wc = 2.0 / 3.0; break;
+    case 2: B[0] = 2.5; B[1] = -2.0; B[2] = 0.5; Wc = 0.6; break;
+    case 3: B[0] = 2.8; B[1] = -2.8; B[2] = 1.2; B[3] = -0.2; Wc = 4.0 / 7.0; break;
+    case 4: B[0] = 3.0; B[1] = -24.0 / 7.0; B[2] = 27.0 / 14.0; B[3] = -4.0 / 7.0; B[4] = 1.0 / 14.0; Wc = 5.0 / 9.0; break;
+    case 5:
+        B[0] = 22.0 / 7.0; B[1] = -55.0 / 14.0; B[2] = 55.0 / 21.0; B[3] = -22.0 / 21.0; B[4] = 5.0 / 21.0;
+        B[5] = -1.0 / 42.0; Wc = 6.0 / 11.0;
+        break;
+    default: return MDFF_ERR_CONFIG;
+    }
+    int shift = (eo != order) ? eo + 1 : eo;
+    AspcArgs asp;
+    asp.n = n; asp.eo = eo; asp.shift = shift; asp.W = Wc;
+    for (int k = 0; k < 6; k++) asp.B[k] = B[k];
+    int launched = 0;
+    int limit = std::max(maxit, minit) + 1;
+    while (launched < limit) {
+        int chunk = launched == 0 ? std::max(1, std::min(minit, c->scf_chunk)) : c->scf_chunk;
+        for (int it = 0; it < chunk && launched < limit; it++, launched++) {
+            int first = launched == 0;
+            hipLaunchKernelGGL(k_scf_mu, dim3(cdiv(n, 256)), dim3(256), 0, c->st, n, first, asp, c->type, c->tab_d,
+                               c->hist, c->ef, c->mu_ind, c->upol_i, act);
+            colsum(c, c->upol_i, n, n, 1, c->scal + S_UPOL, act);
+            // induced field: dipoles only, no damping (pim.c:258-260)
+            DirArgs a = dir_args(c, c->qz, c->mu_ind, 0, 1, act);
+            launch_dir(c, a, 1, 0, 0, 0);
+            rec_part(c, 1, 3, 3, c->qz, c->mu_ind, c->qz, c->mu_ind, 1, 0, 0, nullptr, act);
+            hipLaunchKernelGGL(k_field_sum, dim3(cdiv(3 * n, 256)), dim3(256), 0, c->st, n, c->selfa2, c->mu_ind,
+                               c->ef_dir, c->ef_rec, c->ef_stat, c->ef, act);
+            if (first)
+                hipLaunchKernelGGL(k_scf_correct, dim3(cdiv(n, 256)), dim3(256), 0, c->st, n, asp, c->type,
+                                   c->tab_d, c->ef, c->mu_ind, act);
+            hipLaunchKernelGGL(k_scf_rmsd, dim3(cdiv(n, 256)), dim3(256), 0, c->st, n, c->type, c->tab_d, c->mu_ind,
+                               c->ef, c->rmsd_i, act);
+            colsum(c, c->rmsd_i, n, n, 1, c->scal + S_RMSD2, act);
+            hipLaunchKernelGGL(k_scf_control, dim3(1), dim3(1), 0, c->st, c->ctl, c->scal, c->npol, maxit, minit,
+                               c->p.conv_tol_ind);
+        }
+        int rc = sync_scal(c);
+        if (rc) return rc;
+        if (!c->ctl_h[C_SCF_ACTIVE]) break;
+    }
+    // dipia_ind[ia][0] = mu_ind (pim.c:281-285)
+    hipLaunchKernelGGL(k_store_hist, dim3(cdiv(3 * n, 256)), dim3(256), 0, c->st, n, c->mu_ind, c->hist);
+    c->last_nscf = c->ctl_h[C_ISCF];
+    c->last_rmsd = sqrt(c->scal_h[S_RMSD2] / (double)(c->npol > 0 ? c->npol : 1));
+    if (c->last_rmsd > c->p.conv_tol_ind) return MDFF_ERR_SCF;
+    return MDFF_OK;
+}
+
+// final multipole_ES of engforce (field.c:381-383): forces and stress
+static void coulomb_force(mdff_ctx *c)
+{
+    int n = c->n;
+    int ld = c->ldip_sticky || c->ldip_static;
+    DirArgs a = dir_args(c, c->q, c->mu_tot, c->ldmp, 0, nullptr);
+    launch_dir(c, a, 0, 0, 0, 1);
+    colsum(c, c->uw_dir, n, n, 7, c->scal + S_U_DIR);
+    int C = ld ? 4 : 1;
+    int NCH = ld ? 9 : 3;
+    rec_part(c, 0, C, NCH, c->q, c->mu_tot, c->q, c->mu_tot, 0, 1, 1, nullptr, nullptr);
+    colsum(c, c->ek, (int)c->nkf, (int)c->nkf, 7, c->scal + S_U_REC);
+    hipLaunchKernelGGL(k_self_terms, dim3(cdiv(n, 256)), dim3(256), 0, c->st, n, c->q, c->mu_tot, c->selfbuf);
+    colsum(c, c->selfbuf, n, n, 2, c->scal + S_QSQ);
+}
+
+static int engforce_impl(mdff_ctx *c, int istep)
+{
+    int n = c->n;
+    int rc = MDFF_OK;
+    if (c->have_events) HIPCHK(hipEventRecord(c->ev[0], c->st));
+    hipLaunchKernelGGL(k_frac, dim3(cdiv(n, 256)), dim3(256), 0, c->st, n, c->cd, c->r, c->r + n, c->r + 2 * n, c->s);
+    hipLaunchKernelGGL(k_zero, dim3(cdiv(3 * n, 256)), dim3(256), 0, c->st, c->f_sr, (size_t)3 * n);
+    short_range(c);
+    if (c->have_events) HIPCHK(hipEventRecord(c->ev[1], c->st));
+    c->last_nscf = 0;
+    if (c->p.lcoulombic) {
+        // k-space tables at the current positions
+        launch_eikr(c, 0, nullptr);
+        if (c->lpol && c->npol > 0) launch_eikr(c, 1, nullptr);
+        // get_dipoles coulombic.c:67-130
+        if (c->lpol) {
+            static_field(c);
+            if (c->have_events) HIPCHK(hipEventRecord(c->ev[2], c->st));
+            rc = scf_loop(c, istep);
+            if (rc != MDFF_OK && rc != MDFF_ERR_SCF) return rc;
+            c->ldip_sticky = 1;
+            hipLaunchKernelGGL(k_axpy3, dim3(cdiv(3 * n, 256)), dim3(256), 0, c->st, 3 * n, c->mu_static, c->mu_ind,
+                               c->mu_tot);
+        } else {
+            if (c->have_events) HIPCHK(hipEventRecord(c->ev[2], c->st));
+            HIPCHK(hipMemcpyAsync(c->mu_tot, c->mu_static, sizeof(double) * 3 * n, hipMemcpyDeviceToDevice, c->st));
+            hipLaunchKernelGGL(k_zero, dim3(1), dim3(1), 0, c->st, c->scal + S_UPOL, (size_t)1);
+        }
+        if (c->have_events) HIPCHK(hipEventRecord(c->ev[3], c->st));
+        coulomb_force(c);
+    } else {
+        if (c->have_events) {
+            HIPCHK(hipEventRecord(c->ev[2], c->st));
+            HIPCHK(hipEventRecord(c->ev[3], c->st));
+        }
+    }
+    hipLaunchKernelGGL(k_assemble_force, dim3(cdiv(3 * n, 256)), dim3(256), 0, c->st, 3 * n, c->f_sr, c->f_dir,
+                       c->f_rec, c->p.lcoulombic ? c->u.coul_unit : 0.0, c->f, c->ctl);
+    if (c->have_events) HIPCHK(hipEventRecord(c->ev[4], c->st));
+    return rc;
+}
+
+// thermo from the device scalars (thermo.c:12-33)
+static void thermo_from_scal(mdff_ctx *c, mdff_thermo *th)
+{
+    const double *S = c->scal_h;
+    mdff_thermo t;
+    memset(&t, 0, sizeof(t));
+    double iv = c->cell.inveOmegaPU;
+    auto sym = [&](const double *w, double scale, double out[3][3]) {
+        out[0][0] = w[0] * scale; out[0][1] = w[1] * scale; out[0][2] = w[2] * scale;
+        out[1][0] = w[1] * scale; out[1][1] = w[3] * scale; out[1][2] = w[4] * scale;
+        out[2][0] = w[2] * scale; out[2][1] = w[4] * scale; out[2][2] = w[5] * scale;
+    };
+    if (c->p.lbhmft || c->p.lbhmftd) {
+        t.u_bhmftd = S[S_U_BH];
+        sym(S + S_W_BH, iv, t.tau_bhmftd);
+        t.pvir_bhmftd = (t.tau_bhmftd[0][0] + t.tau_bhmftd[1][1] + t.tau_bhmftd[2][2]) * (1.0 / 3.0);
+    }
+    if (c->p.lnmlj) {
+        t.u_lj = S[S_U_LJ];
+        sym(S + S_W_LJ, iv, t.tau_lj);
+        t.pvir_lj = (t.tau_lj[0][0] + t.tau_lj[1][1] + t.tau_lj[2][2]) * (1.0 / 3.0);
+    }
+    if (c->p.lcoulombic) {
+        double u_self = -c->selfa * S[S_QSQ] - c->selfa2 * S[S_MUSQ];
+        t.u_coul = (S[S_U_DIR] + S[S_U_REC] * c->ttpiV + u_self) * c->u.coul_unit;
+        double td[3][3], tr[3][3];
+        sym(S + S_W_DIR, 0.5 * iv, td);
+        sym(S + S_W_REC, c->ttpiV * iv, tr);
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) t.tau_coul[i][j] = td[i][j] * c->u.coul_unit + tr[i][j] * c->u.coul_unit;
+        t.pvir_coul = (t.tau_coul[0][0] + t.tau_coul[1][1] + t.tau_coul[2][2]) * (1.0 / 3.0);
+        t.u_pol = c->lpol ? S[S_UPOL] * c->u.coul_unit : 0.0;
+    }
+    t.e_kin = S[S_KIN];
+    t.e_nvt = S[S_ENVT];
+    double l = 3.0 * (double)c->n;
+    t.temp_r = (2.0 * t.e_kin) / (l * c->u.boltz_unit);
+    t.u_tot = t.u_lj + t.u_bhmftd + t.u_coul + t.u_pol;
+    t.e_tot = t.u_tot + t.e_kin;
+    t.h_tot = t.e_tot + t.e_nvt;
+    t.pressure = t.pvir_lj + t.pvir_bhmftd + t.pvir_coul + t.temp_r * c->u.boltz_unit * iv;
+    t.n_scf = c->last_nscf;
+    t.rmsd = c->last_rmsd;
+    t.verlet_updates = c->ctl_h[C_NUPD];
+    *th = t;
+}
+
+static void kin_reduce(mdff_ctx *c)
+{
+    colsum(c, c->ke, c->n, c->n, 1, c->scal + S_KIN);
+}
+
+// ---------------------------------------------------------------- MD steps
+// prop_velocity_verlet integration.c:22-63
+static int step_vv(mdff_ctx *c, int istep)
+{
+    int n = c->n;
+    hipLaunchKernelGGL(k_vv_first, dim3(cdiv(n, 256)), dim3(256), 0, c->st, n, c->dt, c->r, c->v, c->f, c->fold,
+                       c->invmass);
+    int rc = engforce_impl(c, istep);
+    if (rc && rc != MDFF_ERR_SCF) return rc;
+    hipLaunchKernelGGL(k_vv_second, dim3(cdiv(n, 256)), dim3(256), 0, c->st, n, c->dt, c->v, c->f, c->fold,
+                       c->invmass, c->massd, c->ke);
+    kin_reduce(c);
+    return rc;
+}
+
+// nhcn integration.c:140-182 with chain_nhcn :189-294
+static void nhc_chain(mdff_ctx *c)
+{
+    NhcArgs a;
+    a.nh = c->p.nhc_n;
+    a.mults = c->p.nhc_mults;
+    a.yo = c->p.nhc_yosh_order;
+    a.dt = c->dt;
+    a.kT = c->kT;
+    a.L = 3.0 * (double)c->n;
+    for (int i = 0; i < 16; i++) a.Q[i] = c->nhcQ[i];
+    for (int i = 0; i < 7; i++) a.w[i] = c->yosh[i];
+    hipLaunchKernelGGL(k_nhc_chain, dim3(1), dim3(1), 0, c->st, a, c->scal);
+    hipLaunchKernelGGL(k_scale_v, dim3(cdiv(c->n, 256)), dim3(256), 0, c->st, c->n, c->scal + S_NHC_S, c->v,
+                       c->massd, c->ke);
+    kin_reduce(c);
+}
+
+static int step_nhc(mdff_ctx *c, int istep)
+{
+    int n = c->n;
+    hipLaunchKernelGGL(k_ke, dim3(cdiv(n, 256)), dim3(256), 0, c->st, n, c->v, c->massd, c->ke);
+    kin_reduce(c);
+    nhc_chain(c);
+    hipLaunchKernelGGL(k_kick_drift, dim3(cdiv(n, 256)), dim3(256), 0, c->st, n, c->dt, c->r, c->v, c->f,
+                       c->invmass);
+    int rc = engforce_impl(c, istep);
+    if (rc && rc != MDFF_ERR_SCF) return rc;
+    hipLaunchKernelGGL(k_kick, dim3(cdiv(n, 256)), dim3(256), 0, c->st, n, c->dt, c->v, c->f, c->invmass, c->massd,
+                       c->ke);
+    kin_reduce(c);
+    nhc_chain(c);
+    NhcArgs a;
+    a.nh = c->p.nhc_n;
+    a.kT = c->kT;
+    a.L = 3.0 * (double)n;
+    for (int i = 0; i < 16; i++) a.Q[i] = c->nhcQ[i];
+    hipLaunchKernelGGL(k_nhc_envt, dim3(1), dim3(1), 0, c->st, a, c->scal);
+    return rc;
+}
+
+// prop_leap_frog integration.c:93-137
+static int step_lf(mdff_ctx *c, int istep)
+{
+    int n = c->n;
+    int rc = engforce_impl(c, istep);
+    if (rc && rc != MDFF_ERR_SCF) return rc;
+    hipLaunchKernelGGL(k_leapfrog, dim3(cdiv(n, 256)), dim3(256), 0, c->st, n, c->dt, c->r, c->rs, c->v, c->f,
+                       c->invmass, c->massd, c->ke);
+    kin_reduce(c);
+    return rc;
+}
+
+// rescale_velocities kinetic.c:83-136 on the device
+static void rescale_dev(mdff_ctx *c)
+{
+    int n = c->n;
+    hipLaunchKernelGGL(k_ke, dim3(cdiv(n, 256)), dim3(256), 0, c->st, n, c->v, c->massd, c->ke);
+    kin_reduce(c);
+    double ratio = (c->integ != MDFF_NVT_NHCN) ? c->dt / c->tauT : -1.0;
+    hipLaunchKernelGGL(k_rescale_lambda, dim3(1), dim3(1), 0, c->st, c->scal, n, c->kT, c->u.boltz_unit, ratio);
+    hipLaunchKernelGGL(k_scale_v, dim3(cdiv(n, 256)), dim3(256), 0, c->st, n, c->scal + S_LAMBDA, c->v, c->massd,
+                       c->ke);
+    colsum(c, c->v, n, n, 3, c->scal + S_VMEAN);
+    hipLaunchKernelGGL(k_sub_mean, dim3(cdiv(3 * n, 256)), dim3(256), 0, c->st, n, c->scal + S_VMEAN, c->v);
+}
+
+static int md_step_impl(mdff_ctx *c, int istep)
+{
+    int rc;
+    switch (c->integ) {
+    case MDFF_NVE_LF: rc = step_lf(c, istep); break;
+    case MDFF_NVT_NHCN: rc = step_nhc(c, istep); break;
+    default: rc = step_vv(c, istep); break;
+    }
+    if (rc && rc != MDFF_ERR_SCF) return rc;
+    if (c->p.lverletL) {
+        int r2 = build_list(c, 0);
+        if (r2) return r2;
+    }
+    if (c->rescale_allowed && istep < c->p.nequil && c->p.nequilT > 0 && istep % c->p.nequilT == 0) rescale_dev(c);
+    return rc;
+}
+
+// ================================================================= C ABI
+extern "C" {
+
+const char *mdff_version(void) { return "mdff20_amd 0.1 (gfx950)"; }
+
+int mdff_device_count(void)
+{
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+void mdff_tt_damping(double b, double c, double r, double *f, double *fd, int order)
+{
+    double E[MDFF_MAX_TT_ORDER + 1];
+    E[0] = 1.0;
+    for (int k = 1; k <= MDFF_MAX_TT_ORDER; k++) E[k] = E[k - 1] / (double)k;
+    if (order < 0) order = 0;
+    if (order > MDFF_MAX_TT_ORDER) order = MDFF_MAX_TT_ORDER;
+    double br = b * r, ex = exp(-br) * c, ff = E[order];
+    for (int k = order - 1; k > -1; k--) ff = ff * br + E[k];
+    *f = 1.0 - ff * ex;
+    double pw = 1.0;
+    for (int k = 0; k < order; k++) pw *= br;
+    *fd = E[order] * pw * ex * b;
+}
+
+void mdff_destroy(mdff_ctx *c)
+{
+    if (!c) return;
+    if (c->st) (void)hipStreamSynchronize(c->st);
+    double **dp[] = {&c->r, &c->v, &c->f, &c->fold, &c->rs, &c->s, &c->s0, &c->invmass, &c->massd, &c->q, &c->qz,
+                     &c->mu_static, &c->mu_ind, &c->mu_tot, &c->hist, &c->ef_stat, &c->ef, &c->ef_dir, &c->ef_rec,
+                     &c->efg, &c->efg_dir, &c->efg_rec, &c->f_sr, &c->f_dir, &c->f_rec, &c->acc_sr, &c->uw_dir,
+                     &c->selfbuf, &c->ke, &c->upol_i, &c->rmsd_i, &c->tmp_q, &c->tmp_mu, &c->tmp_qraw, &c->tmp_mur, &c->scal,
+                     &c->kvec, &c->Ak, &c->kcoe, &c->W, &c->ek, &c->kpart, &c->bpart};
+    for (auto pp : dp) dfree(*pp);
+    dfree(c->type);
+    dfree(c->ctl);
+    dfree(c->tab_d);
+    dfree(c->pidx);
+    dfree(c->nbr);
+    dfree(c->cnt);
+    if (c->tx) (void)hipFree(c->tx);
+    if (c->ty) (void)hipFree(c->ty);
+    if (c->tz) (void)hipFree(c->tz);
+    if (c->pxx) (void)hipFree(c->pxx);
+    if (c->pyy) (void)hipFree(c->pyy);
+    if (c->pzz) (void)hipFree(c->pzz);
+    if (c->ctl_h) (void)hipHostFree(c->ctl_h);
+    if (c->scal_h) (void)hipHostFree(c->scal_h);
+    if (c->have_events)
+        for (int i = 0; i < 8; i++) (void)hipEventDestroy(c->ev[i]);
+    if (c->st) (void)hipStreamDestroy(c->st);
+    delete c;
+}
+
+int mdff_create(mdff_ctx **out, const mdff_params *p, int nion, const int *typia, const double *A9,
+                const double *rx, const double *ry, const double *rz, int precision, int device)
+{
+    if (!out || !p || nion < 1 || !typia || !A9 || !rx || !ry || !rz) return MDFF_ERR_ARG;
+    if (p->ntype < 1 || p->ntype > NT) return MDFF_ERR_ARG;
+    if (precision != MDFF_FP64 && precision != MDFF_MIXED) return MDFF_ERR_ARG;
+    if (p->lcoulombic && p->algo_pim != MDFF_PIM_SCF) {
+        int anypol = 0;
+        for (int t = 0; t < p->ntype; t++) anypol |= p->lpolar[t];
+        if (anypol) return MDFF_ERR_UNSUPPORTED;
+    }
+    if (p->integrator == MDFF_NVT_NHCN &&
+        (p->nhc_n < 2 || p->nhc_n > 16 || p->nhc_mults < 1 ||
+         !(p->nhc_yosh_order == 1 || p->nhc_yosh_order == 3 || p->nhc_yosh_order == 5 || p->nhc_yosh_order == 7 ||
+           p->nhc_yosh_order == 9)))
+        return MDFF_ERR_CONFIG;
+    if (p->extrapolate_order < 0 || p->extrapolate_order > MDFF_MAX_EXTRAPOLATE_ORDER) return MDFF_ERR_CONFIG;
+    for (int i = 0; i < nion; i++)
+        if (typia[i] < 0 || typia[i] >= p->ntype) return MDFF_ERR_ARG;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return MDFF_ERR_DEVICE;
+    if (device < 0 || device >= ndev) return MDFF_ERR_ARG;
+    if (hipSetDevice(device) != hipSuccess) return MDFF_ERR_DEVICE;
+
+    mdff_ctx *c = new mdff_ctx();
+    c->p = *p;
+    c->n = nion;
+    c->ntype = p->ntype;
+    c->precision = precision;
+    c->device = device;
+    mdff_units_get(p, &c->u);
+    double A[3][3];
+    for (int i = 0; i < 9; i++) A[i / 3][i % 3] = A9[i];
+    mdff_lattice(A, c->u.press_unit, &c->cell);
+    memcpy(c->cd.A, c->cell.A, sizeof(c->cd.A));
+    memcpy(c->cd.B, c->cell.B, sizeof(c->cd.B));
+    c->typ.assign(typia, typia + nion);
+    // flags (field.c:195-219)
+    for (int t = 0; t < c->ntype; t++) {
+        if (p->qit[t] != 0.0) c->lqch = 1;
+        for (int j = 0; j < 3; j++) {
+            if (p->dipit[t][j] != 0.0) c->ldip_static = 1;
+            for (int k = 0; k < 3; k++)
+                if (p->polit[t][j][k] != 0.0) c->lpol = 1;
+        }
+        if (p->lpolar[t]) c->lpim = 1;
+    }
+    if (!p->lcoulombic) c->lpol = 0;
+    for (int t = 0; t < c->ntype; t++)
+        for (int i = 0; i < c->ntype; i++)
+            for (int k = 0; k < c->ntype; k++)
+                if (p->lpoldamping[t][i][k]) c->ldmp = 1;
+    c->lnonbonded = p->lnmlj || p->lbhmft || p->lbhmftd;
+    c->srcutsq = p->cutshortrange * p->cutshortrange;
+    c->lrcutsq = p->cutlongrange * p->cutlongrange;
+    c->alpha = p->alphaES;
+    memcpy(c->kES, p->kES, sizeof(c->kES));
+    if (p->lcoulombic && p->lautoES) mdff_auto_ewald(p, &c->cell, &c->alpha, c->kES);
+    double a2 = c->alpha * c->alpha;
+    c->selfa = c->alpha / sqrt(MDFF_PI);
+    c->selfa2 = 2.0 * c->selfa * a2 / 3.0;
+    c->ttpiV = 2.0 * MDFF_TPI * c->cell.inveOmega;
+    c->tfpiV = 2.0 * c->ttpiV;
+    // md (check_md md.c:120-141)
+    c->integ = (p->integrator == MDFF_NVE_LF && p->nequil > 0) ? MDFF_NVE_VV : p->integrator;
+    c->rescale_allowed = (c->integ == MDFF_NVE_VV);
+    double tauT = p->tauTberendsen;
+    if (p->nequil > 0 && tauT == 0.0) tauT = p->dt;
+    c->kT = p->temp * c->u.boltz_unit;
+    c->dt = p->dt * c->u.time_unit;
+    c->tauT = tauT * c->u.time_unit;
+    c->timesca = p->timesca_thermo * c->u.time_unit;
+    for (int i = 0; i < 16; i++) c->nhcQ[i] = c->timesca * c->timesca * c->kT;
+    c->nhcQ[0] *= 3.0 * (double)nion;
+    memset(c->yosh, 0, sizeof(c->yosh));
+    switch (p->nhc_yosh_order) {
+    case 1: c->yosh[0] = 1.0; break;
+    case 3:
+        c->yosh[0] = 1.0 / (2.0 - pow(2.0, 1.0 / 3.0));
+        c->yosh[1] = 1.0 - 2.0 * c->yosh[0];
+        c->yosh[2] = c->yosh[0];
+        break;
+    case 5:
+        c->yosh[0] = 1.0 / (4.0 - pow(4, 1.0 / 3.0));
+        c->yosh[1] = c->yosh[2] = c->yosh[3] = c->yosh[0];
+        c->yosh[4] = 1.0 - 4.0 * c->yosh[0];
+        break;
+    case 7:
+        c->yosh[0] = .78451361047756; c->yosh[1] = .235573213359357; c->yosh[2] = -1.17767998417887;
+        c->yosh[3] = 1.0 - 2.0 * (c->yosh[0] + c->yosh[1] + c->yosh[2]);
+        c->yosh[4] = c->yosh[2]; c->yosh[5] = c->yosh[1]; c->yosh[6] = c->yosh[0];
+        break;
+    default: break;
+    }
+    build_tables(c);
+
+    int rc = MDFF_OK;
+    int n = nion;
+#define ALLOC(ptr, cnt) do { if ((rc = dalloc(&(ptr), (cnt))) != MDFF_OK) { mdff_destroy(c); return rc; } } while (0)
+    if (hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess) { delete c; return MDFF_ERR_DEVICE; }
+    ALLOC(c->r, 3 * n); ALLOC(c->v, 3 * n); ALLOC(c->f, 3 * n); ALLOC(c->fold, 3 * n); ALLOC(c->rs, 3 * n);
+    ALLOC(c->s, 3 * n); ALLOC(c->s0, 3 * n); ALLOC(c->type, n); ALLOC(c->invmass, n); ALLOC(c->massd, n);
+    ALLOC(c->q, n); ALLOC(c->qz, n); ALLOC(c->mu_static, 3 * n); ALLOC(c->mu_ind, 3 * n); ALLOC(c->mu_tot, 3 * n);
+    ALLOC(c->hist, 18 * (size_t)n); ALLOC(c->ef_stat, 3 * n); ALLOC(c->ef, 3 * n); ALLOC(c->ef_dir, 3 * n);
+    ALLOC(c->ef_rec, 3 * n); ALLOC(c->efg, 9 * n); ALLOC(c->efg_dir, 9 * n); ALLOC(c->efg_rec, 9 * n);
+    ALLOC(c->f_sr, 3 * n); ALLOC(c->f_dir, 3 * n); ALLOC(c->f_rec, 3 * n); ALLOC(c->acc_sr, 14 * n);
+    ALLOC(c->uw_dir, 7 * n); ALLOC(c->selfbuf, 2 * n); ALLOC(c->ke, n); ALLOC(c->upol_i, n); ALLOC(c->rmsd_i, n);
+    ALLOC(c->tmp_q, n); ALLOC(c->tmp_mu, 3 * n); ALLOC(c->tmp_qraw, n); ALLOC(c->tmp_mur, 3 * n);
+    ALLOC(c->scal, S_COUNT); ALLOC(c->ctl, C_COUNT); ALLOC(c->tab_d, 1);
+    if (hipHostMalloc((void **)&c->ctl_h, sizeof(int) * C_COUNT) != hipSuccess) { mdff_destroy(c); return MDFF_ERR_NOMEM; }
+    if (hipHostMalloc((void **)&c->scal_h, sizeof(double) * S_COUNT) != hipSuccess) { mdff_destroy(c); return MDFF_ERR_NOMEM; }
+    memset(c->ctl_h, 0, sizeof(int) * C_COUNT);
+    memset(c->scal_h, 0, sizeof(double) * S_COUNT);
+    // per-ion host arrays (init_config config.c:124-198)
+    std::vector<double> hm(n), him(n), hq(n), hmu(3 * n, 0.0);
+    std::vector<int> hp;
+    c->mass.resize(n);
+    for (int i = 0; i < n; i++) {
+        int t = typia[i];
+        hm[i] = p->massit[t];
+        him[i] = 1.0 / p->massit[t];
+        c->mass[i] = hm[i];
+        hq[i] = p->lcoulombic ? p->qit[t] : 0.0;
+        if (p->lcoulombic)
+            for (int k = 0; k < 3; k++) hmu[(size_t)k * n + i] = p->dipit[t][k];
+        if (c->lpol && p->lpolar[t]) hp.push_back(i);
+    }
+    c->npol = (int)hp.size();
+    std::vector<double> hr(3 * n);
+    memcpy(hr.data(), rx, sizeof(double) * n);
+    memcpy(hr.data() + n, ry, sizeof(double) * n);
+    memcpy(hr.data() + 2 * n, rz, sizeof(double) * n);
+    if (hipMemcpy(c->r, hr.data(), sizeof(double) * 3 * n, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(c->type, typia, sizeof(int) * n, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(c->massd, hm.data(), sizeof(double) * n, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(c->invmass, him.data(), sizeof(double) * n, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(c->q, hq.data(), sizeof(double) * n, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(c->mu_static, hmu.data(), sizeof(double) * 3 * n, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(c->tab_d, &c->tab, sizeof(TypeTab), hipMemcpyHostToDevice) != hipSuccess) {
+        mdff_destroy(c);
+        return MDFF_ERR_DEVICE;
+    }
+    if (c->npol > 0) {
+        ALLOC(c->pidx, c->npol);
+        if (hipMemcpy(c->pidx, hp.data(), sizeof(int) * c->npol, hipMemcpyHostToDevice) != hipSuccess) {
+            mdff_destroy(c);
+            return MDFF_ERR_DEVICE;
+        }
+    }
+    // k mesh (init_kspace kspace.c:14-41, set_param_kmesh :64-102) on the full
+    // (nx, ny, nz) box with nx >= 0; entries outside the half space get Ak = 0
+    if (p->lcoulombic) {
+        c->Kx = c->kES[0]; c->Ky = c->kES[1]; c->Kz = c->kES[2];
+        if (c->Kx < 0 || c->Ky < 0 || c->Kz < 0) { mdff_destroy(c); return MDFF_ERR_CONFIG; }
+        c->P = (2 * c->Ky + 1) * (2 * c->Kz + 1);
+        c->nkf = (size_t)(c->Kx + 1) * c->P;
+        std::vector<double> kv(3 * c->nkf), ak(c->nkf, 0.0), kc(c->nkf, 0.0);
+        int nk = 0;
+        for (int nx = 0; nx <= c->Kx; nx++)
+            for (int ny = -c->Ky; ny <= c->Ky; ny++)
+                for (int nz = -c->Kz; nz <= c->Kz; nz++) {
+                    size_t idx = (size_t)nx * c->P + (size_t)(ny + c->Ky) * (2 * c->Kz + 1) + (nz + c->Kz);
+                    const double(*B)[3] = c->cell.B;
+                    double kx = MDFF_TPI * ((double)nx * B[0][0] + (double)ny * B[0][1] + (double)nz * B[0][2]);
+                    double ky = MDFF_TPI * ((double)nx * B[1][0] + (double)ny * B[1][1] + (double)nz * B[1][2]);
+                    double kz = MDFF_TPI * ((double)nx * B[2][0] + (double)ny * B[2][1] + (double)nz * B[2][2]);
+                    kv[3 * idx] = kx; kv[3 * idx + 1] = ky; kv[3 * idx + 2] = kz;
+                    bool in_half = (nx > 0) || (nx == 0 && ny > 0) || (nx == 0 && ny == 0 && nz > 0);
+                    double kk = kx * kx + ky * ky + kz * kz;
+                    if (in_half && kk != 0.0) {
+                        ak[idx] = exp(-kk * 0.25 / a2) / kk;
+                        kc[idx] = 2.0 * (1.0 / kk + 1.0 / a2 / 4.0);
+                        nk++;
+                    }
+                }
+        c->nk = nk;
+        ALLOC(c->kvec, 3 * c->nkf); ALLOC(c->Ak, c->nkf); ALLOC(c->kcoe, c->nkf); ALLOC(c->W, 2 * c->nkf);
+        ALLOC(c->ek, 7 * c->nkf);
+        if (hipMemcpy(c->kvec, kv.data(), sizeof(double) * 3 * c->nkf, hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(c->Ak, ak.data(), sizeof(double) * c->nkf, hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(c->kcoe, kc.data(), sizeof(double) * c->nkf, hipMemcpyHostToDevice) != hipSuccess) {
+            mdff_destroy(c);
+            return MDFF_ERR_DEVICE;
+        }
+        size_t rs = rsize(c);
+        size_t nxr = (size_t)(c->Kx + 1), nyr = (size_t)(2 * c->Ky + 1), nzr = (size_t)(2 * c->Kz + 1);
+        if (hipMalloc(&c->tx, 2 * nxr * n * rs) != hipSuccess || hipMalloc(&c->ty, 2 * nyr * n * rs) != hipSuccess ||
+            hipMalloc(&c->tz, 2 * nzr * n * rs) != hipSuccess) {
+            mdff_destroy(c);
+            return MDFF_ERR_NOMEM;
+        }
+        if (c->npol > 0) {
+            size_t np_ = (size_t)c->npol;
+            if (hipMalloc(&c->pxx, 2 * nxr * np_ * rs) != hipSuccess || hipMalloc(&c->pyy, 2 * nyr * np_ * rs) != hipSuccess ||
+                hipMalloc(&c->pzz, 2 * nzr * np_ * rs) != hipSuccess) {
+                mdff_destroy(c);
+                return MDFF_ERR_NOMEM;
+            }
+        }
+        // split-K over ions for the forward sum, split over (nx,ny) rows for
+        // the back-projection: aim at >= 2048 blocks on 256 CUs
+        int ntile = (c->P + 255) / 256;
+        int base = (c->Kx + 1) * ntile;
+        c->nchunk = std::max(1, std::min(64, (2048 + base - 1) / base));
+        c->chunk = std::max(32, ((n + c->nchunk - 1) / c->nchunk + 31) / 32 * 32);
+        c->nchunk = (n + c->chunk - 1) / c->chunk;
+        if (c->npol > 0) {
+            c->nchunk_p = std::max(1, std::min(64, (2048 + base - 1) / base));
+            c->chunk_p = std::max(32, ((c->npol + c->nchunk_p - 1) / c->nchunk_p + 31) / 32 * 32);
+            c->nchunk_p = (c->npol + c->chunk_p - 1) / c->chunk_p;
+        }
+        int nrows = (c->Kx + 1) * (2 * c->Ky + 1);
+        int nblk = (n + 63) / 64;
+        c->nsplit = std::max(1, std::min(nrows, (2048 + nblk - 1) / nblk));
+        c->rows_per_split = (nrows + c->nsplit - 1) / c->nsplit;
+        c->nsplit = (nrows + c->rows_per_split - 1) / c->rows_per_split;
+        int C = 4;
+        ALLOC(c->kpart, (size_t)std::max(c->nchunk, c->nchunk_p) * c->nkf * 2 * C);
+        ALLOC(c->bpart, (size_t)c->nsplit * 9 * n);
+    }
+    // neighbour list capacity: all pairs when no list, else an estimate
+    // from the number density grown on overflow
+    {
+        double rcut = std::max(p->cutshortrange, p->cutlongrange) + p->skindiff;
+        double est = 4.0 / 3.0 * MDFF_PI * rcut * rcut * rcut * ((double)n * c->cell.inveOmega);
+        int capn = p->lverletL ? (int)(est * 1.3) + 64 : n;
+        capn = std::min(capn, ((n + 63) / 64) * 64);
+        capn = std::max(capn, 64);
+        c->cap = (capn + 63) / 64 * 64;
+        ALLOC(c->nbr, (size_t)n * c->cap);
+        ALLOC(c->cnt, n);
+    }
+#undef ALLOC
+    for (int i = 0; i < 8; i++)
+        if (hipEventCreate(&c->ev[i]) != hipSuccess) { mdff_destroy(c); return MDFF_ERR_DEVICE; }
+    c->have_events = 1;
+    // first list build (main calls check_verletlist before MD, mdff20.c:87)
+    set_int(c, C_FIRST, 1);
+    rc = build_list(c, 1);
+    if (rc) { mdff_destroy(c); return rc; }
+    if (hipStreamSynchronize(c->st) != hipSuccess) { mdff_destroy(c); return MDFF_ERR_DEVICE; }
+    *out = c;
+    return MDFF_OK;
+}
+
+int mdff_set_tuning(mdff_ctx *c, int lanes, int katoms, int ksplit)
+{
+    if (!c) return MDFF_ERR_ARG;
+    if (lanes == 16 || lanes == 32 || lanes == 64) c->lanes = lanes;
+    else if (lanes != 0) return MDFF_ERR_ARG;
+    (void)katoms;
+    (void)ksplit;
+    return MDFF_OK;
+}
+
+static int put3(mdff_ctx *c, double *dst, const double *x, const double *y, const double *z)
+{
+    int n = c->n;
+    HIPCHK(hipMemcpyAsync(dst, x, sizeof(double) * n, hipMemcpyHostToDevice, c->st));
+    HIPCHK(hipMemcpyAsync(dst + n, y, sizeof(double) * n, hipMemcpyHostToDevice, c->st));
+    HIPCHK(hipMemcpyAsync(dst + 2 * n, z, sizeof(double) * n, hipMemcpyHostToDevice, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    return MDFF_OK;
+}
+
+static int get3(mdff_ctx *c, const double *src, double *x, double *y, double *z)
+{
+    int n = c->n;
+    if (x) HIPCHK(hipMemcpyAsync(x, src, sizeof(double) * n, hipMemcpyDeviceToHost, c->st));
+    if (y) HIPCHK(hipMemcpyAsync(y, src + n, sizeof(double) * n, hipMemcpyDeviceToHost, c->st));
+    if (z) HIPCHK(hipMemcpyAsync(z, src + 2 * n, sizeof(double) * n, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    return MDFF_OK;
+}
+
+int mdff_set_positions(mdff_ctx *c, const double *rx, const double *ry, const double *rz)
+{
+    if (!c || !rx || !ry || !rz) return MDFF_ERR_ARG;
+    return put3(c, c->r, rx, ry, rz);
+}
+
+int mdff_set_velocities(mdff_ctx *c, const double *vx, const double *vy, const double *vz)
+{
+    if (!c || !vx || !vy || !vz) return MDFF_ERR_ARG;
+    return put3(c, c->v, vx, vy, vz);
+}
+
+int mdff_get_positions(mdff_ctx *c, double *rx, double *ry, double *rz) { return c ? get3(c, c->r, rx, ry, rz) : MDFF_ERR_ARG; }
+int mdff_get_velocities(mdff_ctx *c, double *vx, double *vy, double *vz) { return c ? get3(c, c->v, vx, vy, vz) : MDFF_ERR_ARG; }
+int mdff_get_forces(mdff_ctx *c, double *fx, double *fy, double *fz) { return c ? get3(c, c->f, fx, fy, fz) : MDFF_ERR_ARG; }
+
+static int get_aos3(mdff_ctx *c, const double *src, double *out)
+{
+    int n = c->n;
+    std::vector<double> h(3 * n);
+    HIPCHK(hipMemcpyAsync(h.data(), src, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    for (int i = 0; i < n; i++)
+        for (int k = 0; k < 3; k++) out[3 * i + k] = h[(size_t)k * n + i];
+    return MDFF_OK;
+}
+
+int mdff_get_dipoles(mdff_ctx *c, double *mu) { return (c && mu) ? get_aos3(c, c->mu_tot, mu) : MDFF_ERR_ARG; }
+int mdff_get_field(mdff_ctx *c, double *ef) { return (c && ef) ? get_aos3(c, c->ef, ef) : MDFF_ERR_ARG; }
+
+int mdff_get_thermo(mdff_ctx *c, mdff_thermo *th)
+{
+    if (!c || !th) return MDFF_ERR_ARG;
+    int rc = sync_scal(c);
+    if (rc) return rc;
+    thermo_from_scal(c, th);
+    return MDFF_OK;
+}
+
+int mdff_get_ewald(mdff_ctx *c, double *alpha, int *kES, int *nk)
+{
+    if (!c) return MDFF_ERR_ARG;
+    if (alpha) *alpha = c->alpha;
+    if (kES) memcpy(kES, c->kES, sizeof(int) * 3);
+    if (nk) *nk = c->nk;
+    return MDFF_OK;
+}
+
+int mdff_init_velocities(mdff_ctx *c)
+{
+    if (!c) return MDFF_ERR_ARG;
+    int n = c->n;
+    std::vector<double> vx(n), vy(n), vz(n);
+    int rc = mdff_host_init_velocities(&c->p, n, c->mass.data(), vx.data(), vy.data(), vz.data(), 1);
+    if (rc) return rc;
+    return mdff_set_velocities(c, vx.data(), vy.data(), vz.data());
+}
+
+int mdff_check_verlet(mdff_ctx *c)
+{
+    if (!c) return MDFF_ERR_ARG;
+    return build_list(c, 0);
+}
+
+static int finish_call(mdff_ctx *c, int rc, mdff_thermo *th)
+{
+    int r2 = sync_scal(c);
+    if (r2) return r2;
+    if (c->ctl_h[C_NAN]) rc = MDFF_ERR_NAN;
+    if (c->have_events) {
+        float ms = 0;
+        if (hipEventElapsedTime(&ms, c->ev[0], c->ev[1]) == hipSuccess) c->t_ms[0] = ms;
+        if (hipEventElapsedTime(&ms, c->ev[1], c->ev[2]) == hipSuccess) c->t_ms[1] = ms;
+        if (hipEventElapsedTime(&ms, c->ev[2], c->ev[3]) == hipSuccess) c->t_ms[2] = ms;
+        if (hipEventElapsedTime(&ms, c->ev[3], c->ev[4]) == hipSuccess) c->t_ms[3] = ms;
+        if (hipEventElapsedTime(&ms, c->ev[0], c->ev[4]) == hipSuccess) c->t_ms[5] = ms;
+    }
+    if (th) thermo_from_scal(c, th);
+    return rc;
+}
+
+int mdff_engforce(mdff_ctx *c, int istep, mdff_thermo *th)
+{
+    if (!c) return MDFF_ERR_ARG;
+    int rc = engforce_impl(c, istep);
+    if (rc && rc != MDFF_ERR_SCF) return rc;
+    return finish_call(c, rc, th);
+}
+
+int mdff_pair_short(mdff_ctx *c, double *u, double *pvir, double tau[9])
+{
+    if (!c) return MDFF_ERR_ARG;
+    int n = c->n;
+    hipLaunchKernelGGL(k_frac, dim3(cdiv(n, 256)), dim3(256), 0, c->st, n, c->cd, c->r, c->r + n, c->r + 2 * n, c->s);
+    hipLaunchKernelGGL(k_zero, dim3(cdiv(3 * n, 256)), dim3(256), 0, c->st, c->f_sr, (size_t)3 * n);
+    hipLaunchKernelGGL(k_zero, dim3(1), dim3(14), 0, c->st, c->scal + S_U_BH, (size_t)14);
+    short_range(c);
+    HIPCHK(hipMemcpyAsync(c->f, c->f_sr, sizeof(double) * 3 * n, hipMemcpyDeviceToDevice, c->st));
+    int rc = sync_scal(c);
+    if (rc) return rc;
+    mdff_thermo t;
+    thermo_from_scal(c, &t);
+    const double(*tt)[3] = c->p.lnmlj ? t.tau_lj : t.tau_bhmftd;
+    if (u) *u = c->p.lnmlj ? t.u_lj : t.u_bhmftd;
+    if (pvir) *pvir = c->p.lnmlj ? t.pvir_lj : t.pvir_bhmftd;
+    if (tau)
+        for (int i = 0; i < 9; i++) tau[i] = tt[i / 3][i % 3];
+    return MDFF_OK;
+}
+
+int mdff_get_dipoles_scf(mdff_ctx *c, int istep, double *mu, double *upol)
+{
+    if (!c) return MDFF_ERR_ARG;
+    int n = c->n, rc = MDFF_OK;
+    hipLaunchKernelGGL(k_frac, dim3(cdiv(n, 256)), dim3(256), 0, c->st, n, c->cd, c->r, c->r + n, c->r + 2 * n, c->s);
+    if (c->lpol) {
+        launch_eikr(c, 0, nullptr);
+        launch_eikr(c, 1, nullptr);
+        static_field(c);
+        rc = scf_loop(c, istep);
+        if (rc && rc != MDFF_ERR_SCF) return rc;
+        c->ldip_sticky = 1;
+        hipLaunchKernelGGL(k_axpy3, dim3(cdiv(3 * n, 256)), dim3(256), 0, c->st, 3 * n, c->mu_static, c->mu_ind, c->mu_tot);
+    } else {
+        HIPCHK(hipMemcpyAsync(c->mu_tot, c->mu_static, sizeof(double) * 3 * n, hipMemcpyDeviceToDevice, c->st));
+    }
+    int r2 = sync_scal(c);
+    if (r2) return r2;
+    if (upol) *upol = c->lpol ? c->scal_h[S_UPOL] * c->u.coul_unit : 0.0;
+    if (mu) {
+        r2 = get_aos3(c, c->mu_tot, mu);
+        if (r2) return r2;
+    }
+    return rc;
+}
+
+int mdff_multipole_es(mdff_ctx *c, const double *qh, const double *muh, const double *theta, int flags, double *u,
+                      double *pvir, double tau[9], double *efh, double *efgh, double *fx, double *fy, double *fz)
+{
+    (void)theta;
+    if (!c || !qh || !muh) return MDFF_ERR_ARG;
+    if (!c->p.lcoulombic) return MDFF_ERR_CONFIG;
+    int n = c->n;
+    const int lq = flags & MDFF_ES_LQ, ld = flags & MDFF_ES_LD, ldamp = flags & MDFF_ES_LDAMP;
+    const int do_f = flags & MDFF_ES_FORCES, do_s = flags & MDFF_ES_STRESS, do_ef = flags & MDFF_ES_EF,
+              do_efg = flags & MDFF_ES_EFG, do_dir = flags & MDFF_ES_DIR, do_rec = flags & MDFF_ES_REC;
+    // terms masked by lq/ld; the self terms and the reciprocal charge force
+    // use the arrays as passed (ewald.c:193-226, :984)
+    std::vector<double> qm(n), qr(n), mum(3 * n), mur(3 * n);
+    for (int i = 0; i < n; i++) {
+        qr[i] = qh[i];
+        qm[i] = lq ? qh[i] : 0.0;
+        for (int k = 0; k < 3; k++) {
+            mur[(size_t)k * n + i] = muh[3 * i + k];
+            mum[(size_t)k * n + i] = ld ? muh[3 * i + k] : 0.0;
+        }
+    }
+    double *d_qm = c->tmp_q, *d_qr = c->tmp_qraw, *d_mum = c->tmp_mu, *d_mur = c->tmp_mur;
+    HIPCHK(hipMemcpyAsync(d_qm, qm.data(), sizeof(double) * n, hipMemcpyHostToDevice, c->st));
+    HIPCHK(hipMemcpyAsync(d_qr, qr.data(), sizeof(double) * n, hipMemcpyHostToDevice, c->st));
+    HIPCHK(hipMemcpyAsync(d_mum, mum.data(), sizeof(double) * 3 * n, hipMemcpyHostToDevice, c->st));
+    HIPCHK(hipMemcpyAsync(d_mur, mur.data(), sizeof(double) * 3 * n, hipMemcpyHostToDevice, c->st));
+    hipLaunchKernelGGL(k_frac, dim3(cdiv(n, 256)), dim3(256), 0, c->st, n, c->cd, c->r, c->r + n, c->r + 2 * n, c->s);
+    hipLaunchKernelGGL(k_zero, dim3(cdiv(3 * n, 256)), dim3(256), 0, c->st, c->ef_dir, (size_t)3 * n);
+    hipLaunchKernelGGL(k_zero, dim3(cdiv(9 * n, 256)), dim3(256), 0, c->st, c->efg_dir, (size_t)9 * n);
+    hipLaunchKernelGGL(k_zero, dim3(cdiv(3 * n, 256)), dim3(256), 0, c->st, c->f_dir, (size_t)3 * n);
+    hipLaunchKernelGGL(k_zero, dim3(cdiv(7 * n, 256)), dim3(256), 0, c->st, c->uw_dir, (size_t)7 * n);
+    hipLaunchKernelGGL(k_zero, dim3(cdiv(3 * n, 256)), dim3(256), 0, c->st, c->ef_rec, (size_t)3 * n);
+    hipLaunchKernelGGL(k_zero, dim3(cdiv(9 * n, 256)), dim3(256), 0, c->st, c->efg_rec, (size_t)9 * n);
+    hipLaunchKernelGGL(k_zero, dim3(cdiv(3 * n, 256)), dim3(256), 0, c->st, c->f_rec, (size_t)3 * n);
+    hipLaunchKernelGGL(k_zero, dim3(1), dim3(64), 0, c->st, c->scal + S_U_DIR, (size_t)14);
+    if (do_dir) {
+        DirArgs a = dir_args(c, d_qm, d_mum, ldamp ? 1 : 0, 0, nullptr);
+        a.efg = c->efg_dir;
+        launch_dir(c, a, do_ef, do_efg, do_efg || do_f, do_f);
+        colsum(c, c->uw_dir, n, n, 7, c->scal + S_U_DIR);
+    }
+    if (do_rec) {
+        launch_eikr(c, 0, nullptr);
+        int C = ld ? 4 : 1;
+        int NCH = (do_ef || (do_f && ld)) ? 9 : 3;
+        rec_part(c, 0, C, NCH, d_qm, d_mum, d_qr, d_mum, do_ef, do_f, do_s, c->efg_rec, nullptr);
+        colsum(c, c->ek, (int)c->nkf, (int)c->nkf, 7, c->scal + S_U_REC);
+    }
+    hipLaunchKernelGGL(k_self_terms, dim3(cdiv(n, 256)), dim3(256), 0, c->st, n, d_qr, d_mur, c->selfbuf);
+    colsum(c, c->selfbuf, n, n, 2, c->scal + S_QSQ);
+    std::vector<double> hdir(3 * n), hrec(3 * n), gdir(9 * n), grec(9 * n), fdir(3 * n), frec(3 * n);
+    HIPCHK(hipMemcpyAsync(hdir.data(), c->ef_dir, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipMemcpyAsync(hrec.data(), c->ef_rec, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipMemcpyAsync(gdir.data(), c->efg_dir, sizeof(double) * 9 * n, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipMemcpyAsync(grec.data(), c->efg_rec, sizeof(double) * 9 * n, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipMemcpyAsync(fdir.data(), c->f_dir, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipMemcpyAsync(frec.data(), c->f_rec, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, c->st));
+    int rc = sync_scal(c);
+    if (rc) return rc;
+    const double *S = c->scal_h;
+    double u_self = -c->selfa * S[S_QSQ] - c->selfa2 * S[S_MUSQ];
+    double udir = do_dir ? S[S_U_DIR] : 0.0, urec = do_rec ? S[S_U_REC] * c->ttpiV : 0.0;
+    if (u) *u = (udir + urec + u_self) * c->u.coul_unit;
+    double iv = c->cell.inveOmegaPU;
+    if (do_s && tau) {
+        const double *wd = S + S_W_DIR, *wr = S + S_W_REC;
+        int map[9] = {0, 1, 2, 1, 3, 4, 2, 4, 5};
+        for (int k = 0; k < 9; k++) {
+            double t = 0.0;
+            if (do_dir && do_f) t += 0.5 * wd[map[k]] * iv;
+            if (do_rec) t += wr[map[k]] * c->ttpiV * iv;
+            tau[k] = t * c->u.coul_unit;
+        }
+    }
+    if (pvir && tau) *pvir = (tau[0] + tau[4] + tau[8]) * (1.0 / 3.0);
+    for (int i = 0; i < n; i++) {
+        for (int a = 0; a < 3; a++) {
+            double v = 2.0 * c->selfa2 * muh[3 * i + a];
+            if (do_dir && do_ef) v += hdir[(size_t)a * n + i];
+            if (do_rec && do_ef) v += hrec[(size_t)a * n + i];
+            if (efh) efh[3 * i + a] = v;
+            for (int b = 0; b < 3; b++) {
+                double g = (a == b) ? -2.0 * c->selfa2 * qh[i] : 0.0;
+                if (do_dir) g += gdir[(size_t)(3 * a + b) * n + i];
+                if (do_rec && do_ef) g += grec[(size_t)(3 * a + b) * n + i];
+                if (efgh) efgh[9 * i + 3 * a + b] = g;
+            }
+        }
+        if (do_f) {
+            double fxi = 0, fyi = 0, fzi = 0;
+            if (do_dir) { fxi += fdir[i]; fyi += fdir[n + i]; fzi += fdir[2 * n + i]; }
+            if (do_rec) { fxi += frec[i]; fyi += frec[n + i]; fzi += frec[2 * n + i]; }
+            if (fx) fx[i] += fxi * c->u.coul_unit;
+            if (fy) fy[i] += fyi * c->u.coul_unit;
+            if (fz) fz[i] += fzi * c->u.coul_unit;
+        }
+    }
+    return MDFF_OK;
+}
+
+int mdff_md_init(mdff_ctx *c, mdff_thermo *th)
+{
+    if (!c) return MDFF_ERR_ARG;
+    int n = c->n;
+    hipLaunchKernelGGL(k_ke, dim3(cdiv(n, 256)), dim3(256), 0, c->st, n, c->v, c->massd, c->ke);
+    kin_reduce(c);
+    int rc = engforce_impl(c, 0);
+    if (rc && rc != MDFF_ERR_SCF) return rc;
+    rc = finish_call(c, rc, th);
+    if (th) th->istep = 0;
+    return rc;
+}
+
+int mdff_md_step(mdff_ctx *c, int istep, mdff_thermo *th)
+{
+    if (!c) return MDFF_ERR_ARG;
+    int rc = md_step_impl(c, istep);
+    if (rc && rc != MDFF_ERR_SCF) return rc;
+    rc = finish_call(c, rc, th);
+    if (th) th->istep = istep;
+    return rc;
+}
+
+int mdff_md_run(mdff_ctx *c, int first_step, int nsteps, mdff_thermo *th)
+{
+    if (!c) return MDFF_ERR_ARG;
+    int worst = MDFF_OK;
+    for (int k = 0; k < nsteps; k++) {
+        int rc = md_step_impl(c, first_step + k);
+        if (rc == MDFF_ERR_SCF) worst = rc;
+        else if (rc) return rc;
+    }
+    int rc = finish_call(c, worst, th);
+    if (th) th->istep = first_step + nsteps - 1;
+    return rc;
+}
+
+int mdff_get_timing(mdff_ctx *c, double t[6])
+{
+    if (!c || !t) return MDFF_ERR_ARG;
+    for (int i = 0; i < 6; i++) t[i] = c->t_ms[i];
+    return MDFF_OK;
+}
+
+}  // extern "C"

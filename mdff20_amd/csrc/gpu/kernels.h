@@ -1,0 +1,798 @@
+// kernels.h -- HIP kernels of the engine (included once, by engine.hip).
+#pragma once
+#include "dev.h"
+
+// ---------------------------------------------------------------- reductions
+// out[k] = sum_i in[k*ld + i], i < n; one 1024-thread block per column k,
+// fixed summation order (bitwise reproducible).
+__global__ void __launch_bounds__(1024) k_colsum(const double *__restrict__ in, int n, int ld,
+                                                 double *__restrict__ out, const int *active)
+{
+    if (active && !*active) return;
+    __shared__ double sh[1024];
+    const double *col = in + (size_t)blockIdx.x * ld;
+    double s = 0.0;
+    for (int i = threadIdx.x; i < n; i += 1024) s += col[i];
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 512; o > 0; o >>= 1) {
+        if (threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[blockIdx.x] = sh[0];
+}
+
+// ------------------------------------------------------------ fractional coords
+// kardir cell.c:125-143 on a copy
+__global__ void k_frac(int n, CellD c, const double *__restrict__ rx, const double *__restrict__ ry,
+                       const double *__restrict__ rz, double *__restrict__ s)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double x = rx[i], y = ry[i], z = rz[i];
+    s[i] = x * c.B[0][0] + y * c.B[0][1] + z * c.B[0][2];
+    s[n + i] = x * c.B[1][0] + y * c.B[1][1] + z * c.B[1][2];
+    s[2 * n + i] = x * c.B[2][0] + y * c.B[2][1] + z * c.B[2][2];
+}
+
+// ------------------------------------------------------------------ Verlet
+// check_verletlist verlet.c:124-170: the two largest displacements since the
+// last build; one block.
+__global__ void __launch_bounds__(1024) k_vcheck(int n, CellD c, const double *__restrict__ s,
+                                                 const double *__restrict__ s0, double skin, int *ctl)
+{
+    __shared__ double m1s[1024], m2s[1024];
+    double m1 = 0.0, m2 = 0.0;
+    for (int i = threadIdx.x; i < n; i += 1024) {
+        double x, y, z;
+        min_image(c, s[i] - s0[i], s[n + i] - s0[n + i], s[2 * n + i] - s0[2 * n + i], x, y, z);
+        double d = sqrt(x * x + y * y + z * z);
+        if (d > m1) { m2 = m1; m1 = d; }
+        else if (d > m2) m2 = d;
+    }
+    m1s[threadIdx.x] = m1;
+    m2s[threadIdx.x] = m2;
+    __syncthreads();
+    for (int o = 512; o > 0; o >>= 1) {
+        if (threadIdx.x < o) {
+            double a1 = m1s[threadIdx.x], a2 = m2s[threadIdx.x];
+            double b1 = m1s[threadIdx.x + o], b2 = m2s[threadIdx.x + o];
+            double t1, t2;
+            if (a1 >= b1) { t1 = a1; t2 = fmax(a2, b1); }
+            else { t1 = b1; t2 = fmax(b2, a1); }
+            m1s[threadIdx.x] = t1;
+            m2s[threadIdx.x] = t2;
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        int rebuild = ctl[C_FIRST] || (m1s[0] + m2s[0] > skin);
+        ctl[C_REBUILD] = rebuild;
+        if (rebuild) ctl[C_NUPD] += 1;
+        ctl[C_FIRST] = 0;
+    }
+}
+
+// full neighbour list, all pairs scanned (one wavefront per ion, ordered by j):
+// entry j | NBR_SR if d <= rc_sr+skin | NBR_LR if d <= rc_lr+skin
+// (gen_pbc_verletlist verlet.c:54-122 keeps each pair once; both orders are
+// stored here so each ion owns its sums).  all=1 lists every j != i (no list,
+// lverletL false: the kernels then apply only the cutoffs).
+__global__ void __launch_bounds__(256) k_build_allpairs(int n, CellD c, const double *__restrict__ s,
+                                                        double cut_sr2, double cut_lr2, int all, int cap,
+                                                        unsigned *__restrict__ nbr, int *__restrict__ cnt,
+                                                        double *__restrict__ s0, int *ctl)
+{
+    if (!ctl[C_REBUILD]) return;
+    int lane = threadIdx.x & 63;
+    int i = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (i >= n) return;
+    double six = s[i], siy = s[n + i], siz = s[2 * n + i];
+    int count = 0;
+    for (int j0 = 0; j0 < n; j0 += 64) {
+        int j = j0 + lane;
+        unsigned e = 0;
+        if (j < n && j != i) {
+            if (all) {
+                e = (unsigned)j | NBR_SR | NBR_LR;
+            } else {
+                double x, y, z;
+                min_image(c, six - s[j], siy - s[n + j], siz - s[2 * n + j], x, y, z);
+                double d2 = x * x + y * y + z * z;
+                if (d2 <= cut_sr2) e |= NBR_SR;
+                if (d2 <= cut_lr2) e |= NBR_LR;
+                if (e) e |= (unsigned)j;
+            }
+        }
+        unsigned long long m = __ballot(e != 0);
+        int pos = count + __popcll(m & ((1ull << lane) - 1ull));
+        if (e != 0 && pos < cap) nbr[(size_t)i * cap + pos] = e;
+        count += __popcll(m);
+    }
+    if (lane == 0) {
+        cnt[i] = count < cap ? count : cap;
+        if (count > cap) atomicMax(&ctl[C_OVERFLOW], count);
+        atomicMax(&ctl[C_MAXCNT], count);
+        s0[i] = six;
+        s0[n + i] = siy;
+        s0[2 * n + i] = siz;
+    }
+}
+
+// ------------------------------------------------------------- short range
+struct ShortArgs {
+    int n, cap;
+    CellD c;
+    const double *s;
+    const int *type;
+    const unsigned *nbr;
+    const int *cnt;
+    const TypeTab *T;
+    double rc2;
+    double *f;            // 3n: force (overwritten)
+    double *acc;          // 14n: per-ion halves u_bh, w_bh[6], u_lj, w_lj[6]
+};
+
+// engforce_bhmftd_pbc bhmftd.c:117-271 and engforce_nmlj_pbc nmlj.c:254-385,
+// LANES lanes per ion over its full list.
+template <typename real, int LANES>
+__global__ void __launch_bounds__(256) k_short(ShortArgs a)
+{
+    const int lane = threadIdx.x % LANES;
+    const int i = (blockIdx.x * blockDim.x + threadIdx.x) / LANES;
+    if (i >= a.n) return;
+    const TypeTab &T = *a.T;
+    const int n = a.n;
+    const int ti = a.type[i];
+    const double six = a.s[i], siy = a.s[n + i], siz = a.s[2 * n + i];
+    double fx = 0, fy = 0, fz = 0, ub = 0, ul = 0;
+    double wb[6] = {0, 0, 0, 0, 0, 0}, wl[6] = {0, 0, 0, 0, 0, 0};
+    const int m = a.cnt[i];
+    const unsigned *nb = a.nbr + (size_t)i * a.cap;
+    for (int e = lane; e < m; e += LANES) {
+        unsigned ent = nb[e];
+        if (!(ent & NBR_SR)) continue;
+        int j = ent & NBR_IDX;
+        int tj = a.type[j];
+        double x, y, z;
+        min_image(a.c, six - a.s[j], siy - a.s[n + j], siz - a.s[2 * n + j], x, y, z);
+        double d2 = x * x + y * y + z * z;
+        if (!(d2 < a.rc2)) continue;
+        if (T.lbh && T.A[ti][tj] != 0.0) {
+            real rd2 = (real)d2;
+            real ir2 = (real)1.0 / rd2, d = sqrt(rd2);
+            real B = (real)T.B[ti][tj];
+            real erh = (real)T.A[ti][tj] * exp(-B * d);
+            real ir6 = ir2 * ir2 * ir2;
+            real ir8 = ir6 * ir2;
+            ir6 *= (real)T.C[ti][tj];
+            ir8 *= (real)T.D[ti][tj];
+            real f6 = 1, f8 = 1, fd6 = 0, fd8 = 0;
+            if (T.lbhd) {
+                tt_damp<real>(T.E_TT, (real)T.BD[ti][tj], (real)1.0, d, f6, fd6, 6);
+                tt_damp<real>(T.E_TT, (real)T.BD[ti][tj], (real)1.0, d, f8, fd8, 8);
+            }
+            real ir6d = ir6 * f6, ir8d = ir8 * f8;
+            real ir7 = (real)6.0 * ir6d / d, ir9 = (real)8.0 * ir8d / d;
+            double w = (double)((B * erh - ir7 - ir9 + ir6 * fd6 + ir8 * fd8) / d);
+            ub += 0.5 * (double)(erh - ir6d - ir8d);
+            double px = w * x, py = w * y, pz = w * z;
+            fx += px; fy += py; fz += pz;
+            wb[0] += 0.5 * x * px; wb[1] += 0.25 * (x * py + y * px); wb[2] += 0.25 * (x * pz + z * px);
+            wb[3] += 0.5 * y * py; wb[4] += 0.25 * (y * pz + z * py); wb[5] += 0.5 * z * pz;
+        }
+        if (T.llj) {
+            double sr2 = T.sigsq[ti][tj] / d2;
+            double srp = pow(sr2, T.ptwo[ti][tj]);
+            double srq = pow(sr2, T.qtwo[ti][tj]);
+            double u = 0.0;
+            if (T.trunc == MDFF_TRUNC_NO) u = T.epsqp[ti][tj] * (T.plj[ti][tj] * srq - T.qlj[ti][tj] * srp);
+            else if (T.trunc == MDFF_TRUNC_LINEAR)
+                u = T.epsqp[ti][tj] * (T.plj[ti][tj] * srq - T.qlj[ti][tj] * srp) - T.uc[ti][tj];
+            ul += 0.5 * u;
+            double w = T.fc[ti][tj] * (srq - srp) * sr2;
+            double px = w * x, py = w * y, pz = w * z;
+            fx += px; fy += py; fz += pz;
+            wl[0] += 0.5 * x * px; wl[1] += 0.25 * (x * py + y * px); wl[2] += 0.25 * (x * pz + z * px);
+            wl[3] += 0.5 * y * py; wl[4] += 0.25 * (y * pz + z * py); wl[5] += 0.5 * z * pz;
+        }
+    }
+    fx = group_sum<LANES>(fx); fy = group_sum<LANES>(fy); fz = group_sum<LANES>(fz);
+    ub = group_sum<LANES>(ub); ul = group_sum<LANES>(ul);
+#pragma unroll
+    for (int k = 0; k < 6; k++) { wb[k] = group_sum<LANES>(wb[k]); wl[k] = group_sum<LANES>(wl[k]); }
+    if (lane == 0) {
+        a.f[i] = fx; a.f[n + i] = fy; a.f[2 * n + i] = fz;
+        a.acc[i] = ub;
+        a.acc[(size_t)7 * n + i] = ul;
+#pragma unroll
+        for (int k = 0; k < 6; k++) { a.acc[(size_t)(1 + k) * n + i] = wb[k]; a.acc[(size_t)(8 + k) * n + i] = wl[k]; }
+    }
+}
+
+// ------------------------------------------------------- Ewald real space
+struct DirArgs {
+    int n, cap;
+    CellD c;
+    const double *s;
+    const int *type;
+    const unsigned *nbr;
+    const int *cnt;
+    const TypeTab *T;
+    const double *q;      // n
+    const double *mu;     // 3n SoA
+    double alpha, rc2;
+    int lcouldamp;
+    const int *active;    // when non-null, skip unless *active (SCF loop gate)
+    double *ef;           // 3n (written when DO_EF)
+    double *efg;          // 9n (written when EFG_Q||EFG_MM)
+    double *f;            // 3n force, coulombic units excluded (written when DO_F)
+    double *uw;           // 7n: energy halves then 6 virial halves (DO_F)
+    int q_zero;           // all charges zero: ions without dipole get nothing
+};
+
+// multipole_ES_dir ewald.c:353-871 from ion i's side of its full list.
+// Contributions to ion i of pair (i,j), r = r_i - r_j:
+//   q-q   (ewald.c:703-731): u qi qj F0/d; ef_i -qj T1 (+qj T1D_i if damped);
+//                            efg_i -qj T2; f_ij qi qj T1
+//   mu-mu (ewald.c:737-761), only when both ions carry a dipole: u -mui.T2.muj;
+//                            ef_i T2 muj; efg_i muj.T3; f_ij -mui.T3.muj
+//   q-mu  (ewald.c:767-812): u -qi T1.muj + qj T1.mui (+ damped);
+//                            f_ij -qi T2 muj + qj T2 mui (+ damped)
+// with F_i -= f_ij, stress -(r f_ij + f_ij r) (ewald.c:815-830).
+template <typename real, int LANES, bool DO_EF, bool EFG_Q, bool EFG_MM, bool DO_F>
+__global__ void __launch_bounds__(256) k_ewald_dir(DirArgs a)
+{
+    if (a.active && !*a.active) return;
+    const int lane = threadIdx.x % LANES;
+    const int i = (blockIdx.x * blockDim.x + threadIdx.x) / LANES;
+    if (i >= a.n) return;
+    const TypeTab &T = *a.T;
+    const int n = a.n;
+    const int ti = a.type[i];
+    const double six = a.s[i], siy = a.s[n + i], siz = a.s[2 * n + i];
+    const real qi = (real)a.q[i];
+    const real mix = (real)a.mu[i], miy = (real)a.mu[n + i], miz = (real)a.mu[2 * n + i];
+    const bool dip_i = (a.mu[i] != 0.0) || (a.mu[n + i] != 0.0) || (a.mu[2 * n + i] != 0.0);
+    const real al = (real)a.alpha, al2 = al * al, al3 = al2 * al, al5 = al3 * al2;
+    const real rpi = (real)(1.0 / 1.7724538509055160273);
+    double ex = 0, ey = 0, ez = 0;
+    double g[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    double fx = 0, fy = 0, fz = 0, uu = 0;
+    double w6[6] = {0, 0, 0, 0, 0, 0};
+    const int m = (a.q_zero && !dip_i) ? 0 : a.cnt[i];
+    const unsigned *nb = a.nbr + (size_t)i * a.cap;
+    for (int e = lane; e < m; e += LANES) {
+        unsigned ent = nb[e];
+        if (!(ent & NBR_LR)) continue;
+        int j = ent & NBR_IDX;
+        double dx, dy, dz;
+        min_image(a.c, six - a.s[j], siy - a.s[n + j], siz - a.s[2 * n + j], dx, dy, dz);
+        double dd2 = dx * dx + dy * dy + dz * dz;
+        if (dd2 > a.rc2) continue;
+        const int tj = a.type[j];
+        const real qj = (real)a.q[j];
+        const double mjxd = a.mu[j], mjyd = a.mu[n + j], mjzd = a.mu[2 * n + j];
+        const bool dip_j = (mjxd != 0.0) || (mjyd != 0.0) || (mjzd != 0.0);
+        const real mjx = (real)mjxd, mjy = (real)mjyd, mjz = (real)mjzd;
+        const real x = (real)dx, y = (real)dy, z = (real)dz, d2 = (real)dd2;
+        const real d = sqrt(d2);
+        const real dm1 = (real)1.0 / d, dm3 = dm1 / d2, dm5 = dm3 / d2, dm7 = dm5 / d2;
+        const bool ldamp = a.lcouldamp && T.ldamp[ti][tj];
+        real f1 = 1, fd1 = 0, f2 = 1, fd2 = 0;
+        if (ldamp) {
+            tt_damp<real>(T.E_TT, (real)T.db[ti][tj], (real)T.dc[ti][tj], d, f1, fd1, T.dk[ti][tj]);
+            tt_damp<real>(T.E_TT, (real)T.db[tj][ti], (real)T.dc[tj][ti], d, f2, fd2, T.dk[tj][ti]);
+        }
+        // screening functions ewald.c:524-532
+        const real expon = exp(-al2 * d2) * rpi;
+        const real F0 = fast_erfc(al * d);
+        const real F1 = F0 + (real)2.0 * al * d * expon;
+        const real d3 = d2 * d;
+        const real F2 = F1 + (real)4.0 * al3 * d3 * expon / (real)3.0;
+        const real F3 = F2 + (real)8.0 * al5 * d3 * d2 * expon / (real)15.0;
+        const real F1d1 = (real)1.0 - f1, F1d2 = (real)1.0 - f2;
+        const real F2d1 = F1d1 + (d / (real)3.0) * fd1, F2d2 = F1d2 + (d / (real)3.0) * fd2;
+        // T1 = -r F1/d^3 ; T2 = a2 r r - b2 I ; T3 = -c3 rrr + a2 (delta r)
+        const real t1 = dm3 * F1, t1d1 = dm3 * F1d1, t1d2 = dm3 * F1d2;
+        const real a2 = (real)3.0 * F2 * dm5, b2 = F1 * dm3;
+        const real a2d1 = (real)3.0 * F2d1 * dm5, b2d1 = F1d1 * dm3;
+        const real a2d2 = (real)3.0 * F2d2 * dm5, b2d2 = F1d2 * dm3;
+        const real c3 = (real)15.0 * F3 * dm7;
+        const real rmi = x * mix + y * miy + z * miz;
+        const real rmj = x * mjx + y * mjy + z * mjz;
+        real fij0 = 0, fij1 = 0, fij2 = 0;
+        // charge-charge
+        const bool qq = (qi != (real)0) || (qj != (real)0);
+        if (qq) {
+            uu += 0.5 * (double)(qi * qj * F0 * dm1);
+            if (DO_EF) {
+                // ef_i += -qj T1 (+ qj T1D1)
+                real sc = qj * t1 - (ldamp ? qj * t1d1 : (real)0);
+                ex += (double)(sc * x); ey += (double)(sc * y); ez += (double)(sc * z);
+            }
+            if (DO_F) {
+                real sc = -qi * qj * t1;
+                fij0 = sc * x; fij1 = sc * y; fij2 = sc * z;
+            }
+            if (EFG_Q) {
+                // efg_i += -qj T2
+                real A2 = -qj * a2, B2 = qj * b2;
+                g[0] += (double)(A2 * x * x + B2); g[1] += (double)(A2 * x * y); g[2] += (double)(A2 * x * z);
+                g[3] += (double)(A2 * y * x); g[4] += (double)(A2 * y * y + B2); g[5] += (double)(A2 * y * z);
+                g[6] += (double)(A2 * z * x); g[7] += (double)(A2 * z * y); g[8] += (double)(A2 * z * z + B2);
+            }
+        }
+        // dipole-dipole (both ions must carry a dipole)
+        if (dip_i && dip_j) {
+            const real mimj = mix * mjx + miy * mjy + miz * mjz;
+            uu += 0.5 * (double)(-(a2 * rmi * rmj - b2 * mimj));
+            if (DO_EF) {
+                ex += (double)(a2 * x * rmj - b2 * mjx);
+                ey += (double)(a2 * y * rmj - b2 * mjy);
+                ez += (double)(a2 * z * rmj - b2 * mjz);
+            }
+            if (EFG_MM) {
+                // efg_i[a][b] += sum_c muj_c T3[a][b][c]
+                const real r_[3] = {x, y, z}, mj_[3] = {mjx, mjy, mjz};
+#pragma unroll
+                for (int p = 0; p < 3; p++)
+#pragma unroll
+                    for (int q = 0; q < 3; q++) {
+                        real t = -c3 * r_[p] * r_[q] * rmj + a2 * (r_[q] * mj_[p] + r_[p] * mj_[q]);
+                        if (p == q) t += a2 * rmj;
+                        g[3 * p + q] += (double)t;
+                    }
+            }
+            if (DO_F) {
+                // f_ij,k = -sum mui_a T3[a][k][b] muj_b
+                real cc = c3 * rmi * rmj - a2 * mimj;
+                fij0 += cc * x - a2 * (mix * rmj + mjx * rmi);
+                fij1 += cc * y - a2 * (miy * rmj + mjy * rmi);
+                fij2 += cc * z - a2 * (miz * rmj + mjz * rmi);
+            }
+        }
+        // charge-dipole
+        if (dip_i || dip_j) {
+            uu += 0.5 * (double)(qi * t1 * rmj - qj * t1 * rmi);
+            if (ldamp) uu += 0.5 * (double)(-qi * t1d2 * rmj + qj * t1d1 * rmi);
+            if (DO_F) {
+                // -qi T2 muj + qj T2 mui  (+ -qj T2D1 mui + qi T2D2 muj)
+                real cj = -qi * a2 * rmj, ci = qj * a2 * rmi;
+                real lj = qi * b2, li = -qj * b2;
+                if (ldamp) {
+                    cj += qi * a2d2 * rmj; lj -= qi * b2d2;
+                    ci -= qj * a2d1 * rmi; li += qj * b2d1;
+                }
+                fij0 += (cj + ci) * x + lj * mjx + li * mix;
+                fij1 += (cj + ci) * y + lj * mjy + li * miy;
+                fij2 += (cj + ci) * z + lj * mjz + li * miz;
+            }
+        }
+        if (DO_F) {
+            fx -= (double)fij0; fy -= (double)fij1; fz -= (double)fij2;
+            double X = dx, Y = dy, Z = dz, f0 = fij0, f1_ = fij1, f2_ = fij2;
+            w6[0] -= 0.5 * (X * f0 + X * f0);
+            w6[1] -= 0.5 * (X * f1_ + Y * f0);
+            w6[2] -= 0.5 * (X * f2_ + Z * f0);
+            w6[3] -= 0.5 * (Y * f1_ + Y * f1_);
+            w6[4] -= 0.5 * (Y * f2_ + Z * f1_);
+            w6[5] -= 0.5 * (Z * f2_ + Z * f2_);
+        }
+    }
+    if (DO_EF) { ex = group_sum<LANES>(ex); ey = group_sum<LANES>(ey); ez = group_sum<LANES>(ez); }
+    if (EFG_Q || EFG_MM) {
+#pragma unroll
+        for (int k = 0; k < 9; k++) g[k] = group_sum<LANES>(g[k]);
+    }
+    if (DO_F) {
+        fx = group_sum<LANES>(fx); fy = group_sum<LANES>(fy); fz = group_sum<LANES>(fz);
+        uu = group_sum<LANES>(uu);
+#pragma unroll
+        for (int k = 0; k < 6; k++) w6[k] = group_sum<LANES>(w6[k]);
+    }
+    if (lane == 0) {
+        if (DO_EF) { a.ef[i] = ex; a.ef[n + i] = ey; a.ef[2 * n + i] = ez; }
+        if (EFG_Q || EFG_MM) {
+#pragma unroll
+            for (int k = 0; k < 9; k++) a.efg[(size_t)k * n + i] = g[k];
+        }
+        if (DO_F) {
+            a.f[i] = fx; a.f[n + i] = fy; a.f[2 * n + i] = fz;
+            a.uw[i] = uu;
+#pragma unroll
+            for (int k = 0; k < 6; k++) a.uw[(size_t)(1 + k) * n + i] = w6[k];
+        }
+    }
+}
+
+// ------------------------------------------------------------- k space
+// Separable structure factor: e^{ik.r_j} = ex_j[nx] ey_j[ny] ez_j[nz] with
+// k = 2 pi sum_b n_b B[.][b] (kspace.c:87-89) so k.r = 2 pi sum_b n_b t_b,
+// t_b = sum_a B[a][b] r_a.  Tables hold n = 0..Kx for x and -K..K for y, z.
+template <typename real>
+__global__ void k_eikr(int n, CellD c, int Kx, int Ky, int Kz, const double *__restrict__ rx,
+                       const double *__restrict__ ry, const double *__restrict__ rz, const int *__restrict__ idx,
+                       int nsel, real *__restrict__ tx, real *__restrict__ ty, real *__restrict__ tz,
+                       const int *active)
+{
+    if (active && !*active) return;
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nsel) return;
+    int i = idx ? idx[k] : k;
+    double x = rx[i], y = ry[i], z = rz[i];
+    double t[3];
+    for (int b = 0; b < 3; b++) {
+        double v = x * c.B[0][b] + y * c.B[1][b] + z * c.B[2][b];
+        t[b] = v - rint(v);
+    }
+    double th = MDFF_TPI * t[0];
+    for (int m = 0; m <= Kx; m++) {
+        double sn, cs;
+        sincos(th * (double)m, &sn, &cs);
+        tx[2 * ((size_t)m * nsel + k)] = (real)cs;
+        tx[2 * ((size_t)m * nsel + k) + 1] = (real)sn;
+    }
+    th = MDFF_TPI * t[1];
+    for (int m = -Ky; m <= Ky; m++) {
+        double sn, cs;
+        sincos(th * (double)m, &sn, &cs);
+        size_t o = 2 * ((size_t)(m + Ky) * nsel + k);
+        ty[o] = (real)cs;
+        ty[o + 1] = (real)sn;
+    }
+    th = MDFF_TPI * t[2];
+    for (int m = -Kz; m <= Kz; m++) {
+        double sn, cs;
+        sincos(th * (double)m, &sn, &cs);
+        size_t o = 2 * ((size_t)(m + Kz) * nsel + k);
+        tz[o] = (real)cs;
+        tz[o + 1] = (real)sn;
+    }
+}
+
+struct SkArgs {
+    int nsel;                  // ions in the tables
+    int Kx, Ky, Kz, P;         // P = (2Ky+1)(2Kz+1)
+    const void *tx, *ty, *tz;  // tables (real)
+    const int *idx;            // table row -> ion (nullptr = identity)
+    const double *q, *mu;      // weights (q: n, mu: 3n SoA), full ion indexing
+    int n;
+    int chunk;                 // ions per block in y
+    double *part;              // [nchunk][(Kx+1)*P][C][2]
+    const int *active;
+};
+
+// forward S_c(k) = sum_j w_cj e^{ik.r_j}: block = (nx, 256 plane points),
+// grid.y = ion chunks; partial sums per chunk (reduced in a fixed order).
+// C = 1: q; 3: mu; 4: q and mu.
+template <typename real, int C>
+__global__ void __launch_bounds__(256) k_sk_fwd(SkArgs a)
+{
+    if (a.active && !*a.active) return;
+    constexpr int SUB = 32;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    const int NY = 2 * a.Ky + 1, NZ = 2 * a.Kz + 1;
+    real *shx = (real *)smem_raw;                 // SUB complex
+    real *shy = shx + 2 * SUB;                    // NY*SUB complex
+    real *shz = shy + 2 * NY * SUB;               // NZ*SUB complex
+    real *shw = shz + 2 * NZ * SUB;               // C*SUB
+    const int ntile = (a.P + 255) / 256;
+    const int nx = blockIdx.x / ntile;
+    const int p = (blockIdx.x % ntile) * 256 + threadIdx.x;
+    const int j0 = blockIdx.y * a.chunk;
+    const int j1 = min(j0 + a.chunk, a.nsel);
+    const int iy = p / NZ, iz = p % NZ;
+    const bool valid = p < a.P;
+    const real *tx = (const real *)a.tx, *ty = (const real *)a.ty, *tz = (const real *)a.tz;
+    double acc[C][2];
+#pragma unroll
+    for (int c = 0; c < C; c++) acc[c][0] = acc[c][1] = 0.0;
+    for (int jb = j0; jb < j1; jb += SUB) {
+        const int nj = min(SUB, j1 - jb);
+        __syncthreads();
+        for (int t = threadIdx.x; t < SUB; t += 256) {
+            if (t < nj) {
+                size_t o = 2 * ((size_t)nx * a.nsel + jb + t);
+                shx[2 * t] = tx[o]; shx[2 * t + 1] = tx[o + 1];
+                int ion = a.idx ? a.idx[jb + t] : jb + t;
+                if (C == 1) shw[t] = (real)a.q[ion];
+                if (C == 3) { shw[t] = (real)a.mu[ion]; shw[SUB + t] = (real)a.mu[a.n + ion]; shw[2 * SUB + t] = (real)a.mu[2 * a.n + ion]; }
+                if (C == 4) {
+                    shw[t] = (real)a.q[ion];
+                    shw[SUB + t] = (real)a.mu[ion]; shw[2 * SUB + t] = (real)a.mu[a.n + ion]; shw[3 * SUB + t] = (real)a.mu[2 * a.n + ion];
+                }
+            }
+        }
+        for (int t = threadIdx.x; t < NY * SUB; t += 256) {
+            int m = t / SUB, jj = t % SUB;
+            if (jj < nj) {
+                size_t o = 2 * ((size_t)m * a.nsel + jb + jj);
+                shy[2 * t] = ty[o]; shy[2 * t + 1] = ty[o + 1];
+            }
+        }
+        for (int t = threadIdx.x; t < NZ * SUB; t += 256) {
+            int m = t / SUB, jj = t % SUB;
+            if (jj < nj) {
+                size_t o = 2 * ((size_t)m * a.nsel + jb + jj);
+                shz[2 * t] = tz[o]; shz[2 * t + 1] = tz[o + 1];
+            }
+        }
+        __syncthreads();
+        if (valid) {
+            real s[C][2];
+#pragma unroll
+            for (int c = 0; c < C; c++) s[c][0] = s[c][1] = (real)0;
+            for (int jj = 0; jj < nj; jj++) {
+                real xr = shx[2 * jj], xi = shx[2 * jj + 1];
+                real yr = shy[2 * (iy * SUB + jj)], yi = shy[2 * (iy * SUB + jj) + 1];
+                real zr = shz[2 * (iz * SUB + jj)], zi = shz[2 * (iz * SUB + jj) + 1];
+                real pr = xr * yr - xi * yi, pi = xr * yi + xi * yr;
+                real er = pr * zr - pi * zi, ei = pr * zi + pi * zr;
+#pragma unroll
+                for (int c = 0; c < C; c++) {
+                    real w = shw[c * SUB + jj];
+                    s[c][0] += w * er;
+                    s[c][1] += w * ei;
+                }
+            }
+#pragma unroll
+            for (int c = 0; c < C; c++) { acc[c][0] += (double)s[c][0]; acc[c][1] += (double)s[c][1]; }
+        }
+    }
+    if (valid) {
+        size_t kf = (size_t)nx * a.P + p;
+        size_t nkf = (size_t)(a.Kx + 1) * a.P;
+        double *out = a.part + ((size_t)blockIdx.y * nkf + kf) * (2 * C);
+#pragma unroll
+        for (int c = 0; c < C; c++) { out[2 * c] = acc[c][0]; out[2 * c + 1] = acc[c][1]; }
+    }
+}
+
+struct KRedArgs {
+    int Kx, Ky, Kz, P, nchunk, C;
+    const double *part;
+    const double *kvec;   // [nkf][3]
+    const double *Ak;     // [nkf], 0 outside the half space
+    const double *kcoe;   // [nkf]
+    double *W;            // [nkf][2] = Ak conj(rho)
+    double *ek;           // [7][nkf]: str and 6 stress terms
+    int do_stress;
+    const int *active;
+};
+
+// rho(k) (kspace.c:189-302) from the channel sums, str = |rho|^2 Ak and the
+// stress terms of multipole_ES_rec ewald.c:949-1006; W = Ak conj(rho).
+__global__ void k_sk_reduce(KRedArgs a)
+{
+    if (a.active && !*a.active) return;
+    size_t nkf = (size_t)(a.Kx + 1) * a.P;
+    size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (k >= nkf) return;
+    double Ak = a.Ak[k];
+    if (Ak == 0.0) {
+        a.W[2 * k] = 0.0; a.W[2 * k + 1] = 0.0;
+        for (int t = 0; t < 7; t++) a.ek[t * nkf + k] = 0.0;
+        return;
+    }
+    double S[4][2] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
+    for (int ch = 0; ch < a.nchunk; ch++) {
+        const double *pp = a.part + ((size_t)ch * nkf + k) * (2 * a.C);
+        for (int c = 0; c < a.C; c++) { S[c][0] += pp[2 * c]; S[c][1] += pp[2 * c + 1]; }
+    }
+    double kx = a.kvec[3 * k], ky = a.kvec[3 * k + 1], kz = a.kvec[3 * k + 2];
+    double rR = 0, rI = 0;
+    int c0 = 0;
+    if (a.C == 1 || a.C == 4) { rR += S[0][0]; rI += S[0][1]; c0 = 1; }
+    if (a.C >= 3) {
+        // + i (k.S_mu)
+        double kr = kx * S[c0][0] + ky * S[c0 + 1][0] + kz * S[c0 + 2][0];
+        double ki = kx * S[c0][1] + ky * S[c0 + 1][1] + kz * S[c0 + 2][1];
+        rR += -ki;
+        rI += kr;
+    }
+    double str = (rR * rR + rI * rI) * Ak;
+    a.W[2 * k] = Ak * rR;
+    a.W[2 * k + 1] = -Ak * rI;
+    double *e = a.ek;
+    e[k] = str;
+    if (a.do_stress) {
+        double kc = a.kcoe[k];
+        e[1 * nkf + k] = (1.0 - kc * kx * kx) * str;
+        e[2 * nkf + k] = -kc * kx * ky * str;
+        e[3 * nkf + k] = -kc * kx * kz * str;
+        e[4 * nkf + k] = (1.0 - kc * ky * ky) * str;
+        e[5 * nkf + k] = -kc * ky * kz * str;
+        e[6 * nkf + k] = (1.0 - kc * kz * kz) * str;
+    } else {
+        for (int t = 1; t < 7; t++) e[t * nkf + k] = 0.0;
+    }
+}
+
+struct BackArgs {
+    int nsel, Kx, Ky, Kz, P;
+    const void *tx, *ty, *tz;
+    const double *W;        // [nkf][2]
+    int rows_per_split;     // (nx,ny) rows per grid.y split
+    double *part;           // [nsplit][NCH][nsel]
+    const int *active;
+};
+
+// back-projection Z_j(k) = W_k e^{ik.r_j}; with k linear in n the sums over
+// k of n_b Im Z and n_b n_c Re Z are formed per (nx,ny) row from the nz sums
+//   V = sum_nz W ez, V1 = sum_nz nz W ez, V2 = sum_nz nz^2 W ez
+// and the row factor P = ex[nx] ey[ny]:
+//   H_x += nx Im(PV), H_y += ny Im(PV), H_z += Im(PV1)          (field)
+//   M_xx += nx^2 Re(PV) ... M_zz += Re(PV2)                       (gradient)
+// NCH = 3 (field) or 9 (field + gradient).
+template <typename real, int NCH>
+__global__ void __launch_bounds__(64) k_sk_back(BackArgs a)
+{
+    if (a.active && !*a.active) return;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    const int NZ = 2 * a.Kz + 1, NY = 2 * a.Ky + 1;
+    real *shz = (real *)smem_raw;             // NZ*64 complex
+    const int j = blockIdx.x * 64 + threadIdx.x;
+    const bool valid = j < a.nsel;
+    const real *tx = (const real *)a.tx, *ty = (const real *)a.ty, *tz = (const real *)a.tz;
+    for (int m = 0; m < NZ; m++) {
+        real zr = 0, zi = 0;
+        if (valid) {
+            size_t o = 2 * ((size_t)m * a.nsel + j);
+            zr = tz[o]; zi = tz[o + 1];
+        }
+        shz[2 * (m * 64 + threadIdx.x)] = zr;
+        shz[2 * (m * 64 + threadIdx.x) + 1] = zi;
+    }
+    __syncthreads();
+    const int nrows = (a.Kx + 1) * NY;
+    const int r0 = blockIdx.y * a.rows_per_split;
+    const int r1 = min(r0 + a.rows_per_split, nrows);
+    double H[3] = {0, 0, 0};
+    double M[6] = {0, 0, 0, 0, 0, 0};
+    for (int row = r0; row < r1; row++) {
+        const int nx = row / NY, iy = row % NY, ny = iy - a.Ky;
+        const double *Wr = a.W + 2 * ((size_t)nx * a.P + (size_t)iy * NZ);
+        real v0r = 0, v0i = 0, v1r = 0, v1i = 0, v2r = 0, v2i = 0;
+        for (int m = 0; m < NZ; m++) {
+            const real wr = (real)Wr[2 * m], wi = (real)Wr[2 * m + 1];
+            const real zr = shz[2 * (m * 64 + threadIdx.x)], zi = shz[2 * (m * 64 + threadIdx.x) + 1];
+            const real ur = wr * zr - wi * zi, ui = wr * zi + wi * zr;
+            const real nz = (real)(m - a.Kz);
+            v0r += ur; v0i += ui;
+            v1r += nz * ur; v1i += nz * ui;
+            if (NCH == 9) { v2r += nz * nz * ur; v2i += nz * nz * ui; }
+        }
+        if (!valid) continue;
+        size_t ox = 2 * ((size_t)nx * a.nsel + j), oy = 2 * ((size_t)iy * a.nsel + j);
+        const real xr = tx[ox], xi = tx[ox + 1], yr = ty[oy], yi = ty[oy + 1];
+        const double pr = (double)(xr * yr - xi * yi), pi = (double)(xr * yi + xi * yr);
+        const double im0 = pr * v0i + pi * v0r, im1 = pr * v1i + pi * v1r;
+        const double fnx = nx, fny = ny;
+        H[0] += fnx * im0;
+        H[1] += fny * im0;
+        H[2] += im1;
+        if (NCH == 9) {
+            const double re0 = pr * v0r - pi * v0i, re1 = pr * v1r - pi * v1i, re2 = pr * v2r - pi * v2i;
+            M[0] += fnx * fnx * re0;
+            M[1] += fnx * fny * re0;
+            M[2] += fnx * re1;
+            M[3] += fny * fny * re0;
+            M[4] += fny * re1;
+            M[5] += re2;
+        }
+    }
+    if (!valid) return;
+    double *out = a.part + (size_t)blockIdx.y * NCH * a.nsel;
+    out[j] = H[0];
+    out[a.nsel + j] = H[1];
+    out[2 * a.nsel + j] = H[2];
+    if (NCH == 9)
+        for (int t = 0; t < 6; t++) out[(size_t)(3 + t) * a.nsel + j] = M[t];
+}
+
+struct BackFinArgs {
+    int n, nsel, nsplit, NCH;
+    const int *idx;
+    CellD c;
+    double tfpiV;
+    const double *part;
+    const double *q, *mu;
+    double *ef;       // 3n (DO_EF)
+    double *efg;      // 9n (DO_EF: upper triangle + diagonal, ewald.c:966-973)
+    double *f;        // 3n (DO_F)
+    int do_ef, do_f;
+    const int *active;
+};
+
+// sum the splits, map n-moments to k-moments and write field, gradient and
+// force of multipole_ES_rec ewald.c:957-992 (times 8 pi / V, :1011-1034)
+__global__ void k_sk_back_fin(BackFinArgs a)
+{
+    if (a.active && !*a.active) return;
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= a.nsel) return;
+    int i = a.idx ? a.idx[t] : t;
+    double H[3] = {0, 0, 0}, M[6] = {0, 0, 0, 0, 0, 0};
+    for (int s = 0; s < a.nsplit; s++) {
+        const double *p = a.part + (size_t)s * a.NCH * a.nsel;
+        for (int c = 0; c < 3; c++) H[c] += p[(size_t)c * a.nsel + t];
+        if (a.NCH == 9)
+            for (int c = 0; c < 6; c++) M[c] += p[(size_t)(3 + c) * a.nsel + t];
+    }
+    // E'_a = 2 pi sum_b B[a][b] H_b ; G_ab = (2 pi)^2 sum_cd B[a][c] B[b][d] M_cd
+    double E[3];
+    for (int aa = 0; aa < 3; aa++)
+        E[aa] = MDFF_TPI * (a.c.B[aa][0] * H[0] + a.c.B[aa][1] * H[1] + a.c.B[aa][2] * H[2]);
+    double Mf[3][3] = {{M[0], M[1], M[2]}, {M[1], M[3], M[4]}, {M[2], M[4], M[5]}};
+    double G[3][3];
+    for (int aa = 0; aa < 3; aa++)
+        for (int bb = 0; bb < 3; bb++) {
+            double sum = 0.0;
+            for (int cc = 0; cc < 3; cc++)
+                for (int dd = 0; dd < 3; dd++) sum += a.c.B[aa][cc] * a.c.B[bb][dd] * Mf[cc][dd];
+            G[aa][bb] = MDFF_TPI * MDFF_TPI * sum;
+        }
+    const int n = a.n;
+    if (a.do_ef) {
+        a.ef[i] = a.tfpiV * E[0];
+        a.ef[n + i] = a.tfpiV * E[1];
+        a.ef[2 * n + i] = a.tfpiV * E[2];
+        if (a.efg && a.NCH == 9) {
+            a.efg[(size_t)0 * n + i] = a.tfpiV * G[0][0];
+            a.efg[(size_t)1 * n + i] = a.tfpiV * G[0][1];
+            a.efg[(size_t)2 * n + i] = a.tfpiV * G[0][2];
+            a.efg[(size_t)3 * n + i] = 0.0;
+            a.efg[(size_t)4 * n + i] = a.tfpiV * G[1][1];
+            a.efg[(size_t)5 * n + i] = a.tfpiV * G[1][2];
+            a.efg[(size_t)6 * n + i] = 0.0;
+            a.efg[(size_t)7 * n + i] = 0.0;
+            a.efg[(size_t)8 * n + i] = a.tfpiV * G[2][2];
+        }
+    }
+    if (a.do_f) {
+        double qi = a.q[i];
+        double m0 = a.mu[i], m1 = a.mu[n + i], m2 = a.mu[2 * n + i];
+        a.f[i] = a.tfpiV * (qi * E[0] + (a.NCH == 9 ? G[0][0] * m0 + G[0][1] * m1 + G[0][2] * m2 : 0.0));
+        a.f[n + i] = a.tfpiV * (qi * E[1] + (a.NCH == 9 ? G[1][0] * m0 + G[1][1] * m1 + G[1][2] * m2 : 0.0));
+        a.f[2 * n + i] = a.tfpiV * (qi * E[2] + (a.NCH == 9 ? G[2][0] * m0 + G[2][1] * m1 + G[2][2] * m2 : 0.0));
+    }
+}
+
+// ----------------------------------------------------------- elementwise
+__global__ void k_zero(double *p, size_t n)
+{
+    size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0.0;
+}
+
+__global__ void k_zero_int(int *p, int n)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+// per-ion q^2 and |mu|^2 for the self energy (ewald.c:193-212)
+__global__ void k_self_terms(int n, const double *q, const double *mu, double *out2n)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double m0 = mu[i], m1 = mu[n + i], m2 = mu[2 * n + i];
+    out2n[i] = q[i] * q[i];
+    out2n[n + i] = m0 * m0 + m1 * m1 + m2 * m2;
+}
+
+// ef = self + dir + rec for the selected channels (ewald.c:235-270); the dir
+// and rec arrays may be null when that part was not computed.
+__global__ void k_field_sum(int n, double selfa2, const double *mu, const double *ef_dir, const double *ef_rec,
+                            const double *base, double *ef, const int *active)
+{
+    if (active && !*active) return;
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= 3 * n) return;
+    double v = 2.0 * selfa2 * mu[i];
+    if (ef_dir) v += ef_dir[i];
+    if (ef_rec) v += ef_rec[i];
+    if (base) v += base[i];
+    ef[i] = v;
+}

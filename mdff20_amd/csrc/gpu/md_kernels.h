@@ -1,0 +1,304 @@
+// md_kernels.h -- SCF control, integrator and small bookkeeping kernels.
+#pragma once
+#include "dev.h"
+
+__global__ void k_set_int(int *p, int v) { *p = v; }
+__global__ void k_add_int(int *p, int v) { *p += v; }
+
+__global__ void k_scf_begin(int *ctl, int active)
+{
+    ctl[C_SCF_ACTIVE] = active;
+    ctl[C_ISCF] = 0;
+}
+
+struct AspcArgs {
+    int n, eo, shift;
+    double B[6];
+    double W;
+};
+
+// one SCF update (pim.c:240-253): iteration 0 takes the ASPC predictor
+// (pim.c:382-400) and shifts the history of each ion; later iterations take
+// mu = pol^T E on polarisable ions (induced_moment pim.c:617-641); every
+// ion also gets its share of u_pol = 1/2 mu.pol^-1.mu (get_upol pim.c:145-161)
+__global__ void k_scf_mu(int n, int first, AspcArgs asp, const int *__restrict__ type, const TypeTab *__restrict__ T,
+                         double *__restrict__ hist, const double *__restrict__ ef, double *__restrict__ mu,
+                         double *__restrict__ upol, const int *active)
+{
+    if (!*active) return;
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int t = type[i];
+    double m[3] = {0.0, 0.0, 0.0};
+    const size_t n3 = (size_t)3 * n;
+    if (first) {
+        for (int a = 0; a < 3; a++) {
+            double v = 0.0;
+            for (int k = 0; k <= asp.eo; k++) v += asp.B[k] * hist[k * n3 + (size_t)a * n + i];
+            m[a] = v;
+        }
+        for (int k = asp.shift; k > 0; k--)
+            for (int a = 0; a < 3; a++) hist[k * n3 + (size_t)a * n + i] = hist[(k - 1) * n3 + (size_t)a * n + i];
+    } else if (T->polar[t]) {
+        const double *P = T->pol[t];
+        double e0 = ef[i], e1 = ef[n + i], e2 = ef[2 * n + i];
+        for (int a = 0; a < 3; a++) m[a] = P[0 * 3 + a] * e0 + P[1 * 3 + a] * e1 + P[2 * 3 + a] * e2;
+    }
+    mu[i] = m[0];
+    mu[n + i] = m[1];
+    mu[2 * n + i] = m[2];
+    double up = 0.0;
+    if (T->polar[t]) {
+        const double *I = T->invpol[t];
+        for (int a = 0; a < 3; a++)
+            for (int b = 0; b < 3; b++) up += m[a] * I[3 * a + b] * m[b];
+    }
+    upol[i] = 0.5 * up;
+}
+
+// ASPC corrector (pim.c:403-420): mu = W pol^T E + (1-W) mu on iteration 0
+__global__ void k_scf_correct(int n, AspcArgs asp, const int *__restrict__ type, const TypeTab *__restrict__ T,
+                              const double *__restrict__ ef, double *__restrict__ mu, const int *active)
+{
+    if (!*active) return;
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int t = type[i];
+    double ms[3] = {mu[i], mu[n + i], mu[2 * n + i]};
+    double m[3] = {ms[0], ms[1], ms[2]};
+    if (T->polar[t]) {
+        const double *P = T->pol[t];
+        double e0 = ef[i], e1 = ef[n + i], e2 = ef[2 * n + i];
+        for (int a = 0; a < 3; a++) m[a] = P[0 * 3 + a] * e0 + P[1 * 3 + a] * e1 + P[2 * 3 + a] * e2;
+    }
+    for (int a = 0; a < 3; a++) mu[(size_t)a * n + i] = m[a] * asp.W + (1.0 - asp.W) * ms[a];
+}
+
+// get_rmsd_mu pim.c:431-449 per ion (diagonal polarisability only)
+__global__ void k_scf_rmsd(int n, const int *__restrict__ type, const TypeTab *__restrict__ T,
+                           const double *__restrict__ mu, const double *__restrict__ ef, double *__restrict__ out,
+                           const int *active)
+{
+    if (!*active) return;
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int t = type[i];
+    double r = 0.0;
+    if (T->polar[t]) {
+        for (int a = 0; a < 3; a++) {
+            double paa = T->pol[t][4 * a];
+            if (paa != 0.0) {
+                double e = mu[(size_t)a * n + i] / paa - ef[(size_t)a * n + i];
+                r += e * e;
+            }
+        }
+    }
+    out[i] = r;
+}
+
+// loop condition of momentpolaSCF pim.c:236
+__global__ void k_scf_control(int *ctl, const double *scal, int npol, int maxit, int minit, double tol)
+{
+    if (!ctl[C_SCF_ACTIVE]) return;
+    int iscf = ctl[C_ISCF] + 1;
+    double rmsd = sqrt(scal[S_RMSD2] / (double)(npol > 0 ? npol : 1));
+    ctl[C_ISCF] = iscf;
+    ctl[C_SCF_ACTIVE] = ((iscf < maxit && rmsd > tol) || iscf < minit) ? 1 : 0;
+}
+
+__global__ void k_store_hist(int n, const double *mu, double *hist)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < 3 * n) hist[i] = mu[i];
+}
+
+__global__ void k_axpy3(int m, const double *a, const double *b, double *out)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < m) out[i] = a[i] + b[i];
+}
+
+// F = F_short + coul (F_dir + F_rec)  (field.c:352-407, ewald.c:275-288)
+__global__ void k_assemble_force(int m, const double *fsr, const double *fdir, const double *frec, double coul,
+                                 double *f, int *ctl)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    double v = fsr[i];
+    if (coul != 0.0) v += fdir[i] * coul + frec[i] * coul;
+    f[i] = v;
+    if (!isfinite(v)) ctl[C_NAN] = 1;
+}
+
+// ------------------------------------------------------------ integrators
+__device__ __forceinline__ double ke_of(double vx, double vy, double vz, double m)
+{
+    return 0.5 * ((vx * vx + vy * vy + vz * vz) * m);
+}
+
+// prop_velocity_verlet integration.c:34-40
+__global__ void k_vv_first(int n, double dt, double *r, const double *v, const double *f, double *fold,
+                           const double *invm)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double dtsq2 = dt * dt * 0.5;
+    for (int a = 0; a < 3; a++) {
+        size_t k = (size_t)a * n + i;
+        fold[k] = f[k];
+        r[k] += v[k] * dt + (f[k] * dtsq2) * invm[i];
+    }
+}
+
+// prop_velocity_verlet integration.c:49-54
+__global__ void k_vv_second(int n, double dt, double *v, const double *f, const double *fold, const double *invm,
+                            const double *m, double *ke)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double dt2 = dt * 0.5;
+    for (int a = 0; a < 3; a++) {
+        size_t k = (size_t)a * n + i;
+        v[k] += (fold[k] + f[k]) * dt2 * invm[i];
+    }
+    ke[i] = ke_of(v[i], v[n + i], v[2 * n + i], m[i]);
+}
+
+// prop_pos_vel_verlet integration.c:70-78
+__global__ void k_kick_drift(int n, double dt, double *r, double *v, const double *f, const double *invm)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double dt2 = dt * 0.5;
+    for (int a = 0; a < 3; a++) {
+        size_t k = (size_t)a * n + i;
+        v[k] += f[k] * dt2 * invm[i];
+    }
+    for (int a = 0; a < 3; a++) {
+        size_t k = (size_t)a * n + i;
+        r[k] += v[k] * dt;
+    }
+}
+
+// prop_pos_vel_verlet integration.c:83-88
+__global__ void k_kick(int n, double dt, double *v, const double *f, const double *invm, const double *m, double *ke)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double dt2 = dt * 0.5;
+    for (int a = 0; a < 3; a++) {
+        size_t k = (size_t)a * n + i;
+        v[k] += f[k] * dt2 * invm[i];
+    }
+    ke[i] = ke_of(v[i], v[n + i], v[2 * n + i], m[i]);
+}
+
+__global__ void k_ke(int n, const double *v, const double *m, double *ke)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    ke[i] = ke_of(v[i], v[n + i], v[2 * n + i], m[i]);
+}
+
+__global__ void k_scale_v(int n, const double *s, double *v, const double *m, double *ke)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double f = *s;
+    v[i] *= f;
+    v[n + i] *= f;
+    v[2 * n + i] *= f;
+    ke[i] = ke_of(v[i], v[n + i], v[2 * n + i], m[i]);
+}
+
+struct NhcArgs {
+    int nh, mults, yo;
+    double dt, kT, L;
+    double Q[16];
+    double w[7];
+};
+
+// chain_nhcn integration.c:189-294 on the thermostat scalars (one thread;
+// the forward sweep leaves vxi[n-2] unchanged, integration.c:269-277)
+__global__ void k_nhc_chain(NhcArgs a, double *scal)
+{
+    double *vxi = scal + S_VXI, *xi = scal + S_XI;
+    const int nh = a.nh;
+    double G[16];
+    for (int i = 0; i < nh; i++) G[i] = 0.0;
+    double kin = scal[S_KIN];
+    double s = 1.0;
+    G[0] = 2.0 * kin - a.L * a.kT;
+    for (int k = 0; k < a.mults; k++) {
+        for (int j = 0; j < a.yo; j++) {
+            double dts = (j < 7 ? a.w[j] : 0.0) * a.dt / ((double)a.mults);
+            double dts2 = dts * 0.5, dts4 = dts2 * 0.5, dts8 = dts4 * 0.5;
+            G[nh - 1] = (vxi[nh - 2] * vxi[nh - 2] / a.Q[nh - 2] - a.kT);
+            vxi[nh - 1] += G[nh - 1] * dts4;
+            for (int ih = nh - 2; ih > -1; ih--) {
+                vxi[ih] *= exp(-vxi[ih + 1] * dts8 / a.Q[ih + 1]);
+                vxi[ih] += G[ih] * dts4;
+                vxi[ih] *= exp(-vxi[ih + 1] * dts8 / a.Q[ih + 1]);
+            }
+            s *= exp(-vxi[0] * dts2 / a.Q[0]);
+            for (int i = 0; i < nh; i++) xi[i] += vxi[i] * dts2 / a.Q[i];
+            G[0] = 2.0 * kin * s * s - a.L * a.kT;
+            for (int ih = 0; ih < nh - 2; ih++) {
+                vxi[ih] *= exp(-vxi[ih + 1] * dts8 / a.Q[ih + 1]);
+                vxi[ih] += G[ih] * dts4;
+                vxi[ih] *= exp(-vxi[ih + 1] * dts8 / a.Q[ih + 1]);
+                G[ih + 1] = (vxi[ih] * vxi[ih] / a.Q[ih] - a.kT);
+            }
+            vxi[nh - 1] += G[nh - 1] * dts4;
+        }
+    }
+    scal[S_NHC_S] = s;
+}
+
+// conserved-quantity term of nhcn integration.c:161-176
+__global__ void k_nhc_envt(NhcArgs a, double *scal)
+{
+    const double *vxi = scal + S_VXI, *xi = scal + S_XI;
+    double e = a.L * a.kT * xi[0] + vxi[0] * vxi[0] * 0.5 / a.Q[0];
+    double e3 = 0.0, e4 = 0.0;
+    for (int i = 1; i < a.nh; i++) {
+        e3 += vxi[i] * vxi[i] * 0.5 / a.Q[i];
+        e4 += a.kT * xi[i];
+    }
+    scal[S_ENVT] = e + e3 + e4;
+}
+
+// prop_leap_frog integration.c:101-128
+__global__ void k_leapfrog(int n, double dt, double *r, double *rs, double *v, const double *f, const double *invm,
+                           const double *m, double *ke)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double dtsq = dt * dt, idt = 0.5 / dt;
+    for (int a = 0; a < 3; a++) {
+        size_t k = (size_t)a * n + i;
+        double u = 2.0 * r[k] - rs[k] + f[k] * dtsq * invm[i];
+        v[k] = idt * (u - rs[k]);
+        rs[k] = r[k];
+        r[k] = u;
+    }
+    ke[i] = ke_of(v[i], v[n + i], v[2 * n + i], m[i]);
+}
+
+// rescale_velocities kinetic.c:87-95; ratio = dt/tauT, or < 0 for NVT
+__global__ void k_rescale_lambda(double *scal, int n, double kT, double boltz, double ratio)
+{
+    double ekin = scal[S_KIN];
+    double T = (2.0 * ekin) / (3.0 * (double)n * boltz);
+    double lam = ratio >= 0.0 ? sqrt(1.0 + ratio * ((kT / T / boltz) - 1.0)) : sqrt(1.0 + ((kT / T / boltz) - 1.0));
+    scal[S_LAMBDA] = lam;
+}
+
+__global__ void k_sub_mean(int n, const double *sums, double *v)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= 3 * n) return;
+    int a = i / n;
+    double on = 1.0 / (double)n;
+    v[i] -= sums[a] * on;
+}

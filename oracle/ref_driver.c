@@ -1,0 +1,214 @@
+/*
+ * ref_driver.c -- fixture and timing driver for the reference build in
+ * oracle/_ref (TEST INFRASTRUCTURE).  Linked against libmdff20_ref.so, which
+ * holds the reference's own src/*.c compiled unchanged except for
+ * inverse_matrix.c (needs lapacke.h, absent) and mdff20.c (its main).
+ *
+ * It repeats main()'s initialisation order (mdff20.c:59-92) and then either
+ * one static force call (mdff20.c:94-107) or the run_md loop (md.c:193-248)
+ * without its file output, writing every observable at %.17g.
+ *
+ * init_config (config.c:124-198) calls the LAPACK inverse for polarisable
+ * types, so this driver fills the per-ion arrays itself; the inverse is
+ * only used by the polarisation energy (pim.c:145-161) and is formed here
+ * for diagonal polarisability tensors, the only kind the fixtures use.
+ *
+ * usage: mdff20_ref_driver <dir with control and POSFF> static
+ *        mdff20_ref_driver <dir> md <nsteps> <state-dump-period>
+ *        mdff20_ref_driver <dir> time <n_induced_calls>
+ */
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <unistd.h>
+#include <omp.h>
+
+#include "cell.h"
+#include "config.h"
+#include "constants.h"
+#include "coulombic.h"
+#include "ewald.h"
+#include "field.h"
+#include "global.h"
+#include "integration.h"
+#include "io.h"
+#include "kinetic.h"
+#include "kspace.h"
+#include "md.h"
+#include "mpi_mdff.h"
+#include "thermo.h"
+#include "verlet.h"
+#include "bhmftd.h"
+#include "nmlj.h"
+
+static FILE *out;
+
+static void drv_init_config(void)
+{
+    int cc = 0;
+    typia = malloc(nion * sizeof(*typia));
+    for (int it = 0; it < ntype; it++) {
+        invenionit[it] = 1. / nionit[it];
+        int ccs = cc;
+        cc += nionit[it];
+        for (int ia = ccs; ia < cc; ia++) {
+            typia[ia] = it;
+            massia[ia] = massit[it];
+            invemassia[ia] = 1.0 / massit[it];
+            if (lcoulombic) {
+                qia[ia] = qit[it];
+                for (int i = 0; i < 3; i++) {
+                    dipia[ia][i] = dipit[it][i];
+                    for (int j = 0; j < 3; j++) quadia[ia][i][j] = quadit[it][i][j];
+                }
+            }
+            if (lcoulombic && lpolar[it]) {
+                for (int i = 0; i < 3; i++)
+                    for (int j = 0; j < 3; j++) {
+                        polia[ia][i][j] = polit[it][i][j];
+                        invepolia[ia][i][j] = 0.0;
+                    }
+                for (int i = 0; i < 3; i++)
+                    if (polit[it][i][i] != 0.0) invepolia[ia][i][i] = 1.0 / polit[it][i][i];
+            }
+        }
+    }
+    nionit[ntype] = nion;
+    invenionit[ntype] = 1. / nionit[ntype];
+}
+
+static void dump_vec(const char *key, const double *v, int n)
+{
+    fprintf(out, "%s %d", key, n);
+    for (int i = 0; i < n; i++) fprintf(out, " %.17g", v[i]);
+    fprintf(out, "\n");
+}
+
+static void dump_scalars(int step, int n_scf)
+{
+    fprintf(out, "step %d\n", step);
+    fprintf(out, "n_scf %d\n", n_scf);
+    fprintf(out, "verlet_updates %d\n", updatevl);
+    double sc[10] = {e_kin, e_nvt, u_lj, u_bhmftd, u_coul, u_pol, pvir_lj, pvir_bhmftd, pvir_coul, 0.0};
+    dump_vec("scalars", sc, 9);
+    dump_vec("tau_lj", &tau_lj[0][0], 9);
+    dump_vec("tau_bhmftd", &tau_bhmftd[0][0], 9);
+    dump_vec("tau_coul", &tau_coul[0][0], 9);
+}
+
+static void dump_state(void)
+{
+    dump_vec("rx", rx, nion); dump_vec("ry", ry, nion); dump_vec("rz", rz, nion);
+    dump_vec("vx", vx, nion); dump_vec("vy", vy, nion); dump_vec("vz", vz, nion);
+    dump_vec("fx", fx, nion); dump_vec("fy", fy, nion); dump_vec("fz", fz, nion);
+    if (lcoulombic) {
+        double *mu = malloc(3 * nion * sizeof(double));
+        for (int ia = 0; ia < nion; ia++)
+            for (int i = 0; i < 3; i++)
+                mu[3 * ia + i] = dipia[ia][i] + (lpim ? dipia_ind[ia][0][i] : 0.0);
+        dump_vec("mu", mu, 3 * nion);
+        free(mu);
+    }
+}
+
+static int scf_calls(void) { return cc_multipole_ES_dir_pim; }
+
+static double wall(void) { return omp_get_wtime(); }
+
+int main(int argc, char **argv)
+{
+    if (argc < 3) {
+        fprintf(stderr, "usage: %s dir static|md|time ...\n", argv[0]);
+        return 2;
+    }
+    if (chdir(argv[1]) != 0) {
+        perror("chdir");
+        return 2;
+    }
+    const char *mode = argv[2];
+    out = fopen("ref_out.txt", "w");
+    if (!out) return 3;
+    if (!freopen("ref_stdout.txt", "w", stdout)) return 3;
+    myrank = 0;
+    numprocs = 1;
+    init_io();
+    gen_constants();
+    controlfn = "control";
+    init_global(controlfn);
+    read_config();
+    init_md(controlfn);
+    init_field(controlfn);
+    drv_init_config();
+    do_split(nion, numprocs, myrank, &atomDec, "atoms");
+    do_split(kcoul.nk, numprocs, myrank, &kcoul.kptDec, "k-points");
+    if (egrator == 2) {
+        for (int i = 0; i < nhc_n; i++) { vxi[i] = 0.0; xi[i] = 0.0; }
+    }
+    fprintf(out, "alphaES %.17g\n", alphaES);
+    fprintf(out, "kES %d %d %d\n", kES[0], kES[1], kES[2]);
+    fprintf(out, "nk %d\n", kcoul.nk);
+    if (lverletL) check_verletlist();
+
+    if (!strcmp(mode, "static")) {
+        e_kin = calc_kin();
+        int c0 = scf_calls();
+        engforce();
+        dump_scalars(0, lpim ? scf_calls() - c0 - 1 : 0);
+        dump_state();
+    } else if (!strcmp(mode, "md")) {
+        int nsteps = argc > 3 ? atoi(argv[3]) : npas;
+        int period = argc > 4 ? atoi(argv[4]) : nsteps;
+        init_velocities();
+        dump_vec("v0x", vx, nion); dump_vec("v0y", vy, nion); dump_vec("v0z", vz, nion);
+        e_kin = calc_kin();
+        int c0 = scf_calls();
+        engforce();
+        dump_scalars(0, lpim ? scf_calls() - c0 - 1 : 0);
+        if (period > 0 && 0 % period == 0) dump_state();
+        for (istep = 1; istep < nsteps + 1; istep++) {
+            c0 = scf_calls();
+            prop_agate(istep);
+            int ns = lpim ? scf_calls() - c0 - 1 : 0;
+            if (lverletL) check_verletlist();
+            if (rescale_allowed && (istep < nequil) && (istep % nequilT == 0)) rescale_velocities(1);
+            dump_scalars(istep, ns);
+            if ((period > 0 && istep % period == 0) || istep == nsteps) dump_state();
+        }
+    } else if (!strcmp(mode, "time")) {
+        /* one force call split into the pieces of a PIM step, each timed */
+        int nind = argc > 3 ? atoi(argv[3]) : 4;
+        double (*ef)[3] = malloc(nion * sizeof(*ef));
+        double (*efg)[3][3] = malloc(nion * sizeof(*efg));
+        double (*mu)[3] = malloc(nion * sizeof(*mu));
+        double *zq = calloc(nion, sizeof(double));
+        double u, pv, tau[3][3] = {{0}};
+        for (int i = 0; i < nion; i++) fx[i] = fy[i] = fz[i] = 0.0;
+        double t0 = wall();
+        if (lbhmft || lbhmftd) engforce_bhmftd_pbc(&u_bhmftd, &pvir_bhmftd, tau_bhmftd);
+        if (lnmlj) engforce_nmlj_pbc(&u_lj, &pvir_lj, tau_lj);
+        double t1 = wall();
+        multipole_ES(qia, dipia, quadia, &u, &pv, tau, ef, efg, lqch, ldip, lqua, ldmp,
+                     false, false, true, false, true, true, true, true);
+        double t2 = wall();
+        for (int ia = 0; ia < nion; ia++)
+            for (int i = 0; i < 3; i++) mu[ia][i] = lpolar[typia[ia]] ? polia[ia][i][i] * ef[ia][i] : 0.0;
+        for (int k = 0; k < nind; k++)
+            multipole_ES(zq, mu, quadia, &u, &pv, tau, ef, efg, false, true, false, false,
+                         false, false, true, false, true, true, true, false);
+        double t3 = wall();
+        multipole_ES(qia, mu, quadia, &u_coul, &pvir_coul, tau_coul, ef, efg, lqch, true, lqua, ldmp,
+                     true, true, false, false, true, true, false, !lpim);
+        double t4 = wall();
+        fprintf(out, "threads %d\n", omp_get_max_threads());
+        fprintf(out, "t_short %.9g\n", t1 - t0);
+        fprintf(out, "t_static %.9g\n", t2 - t1);
+        fprintf(out, "t_induced %.9g\n", nind > 0 ? (t3 - t2) / nind : 0.0);
+        fprintf(out, "t_force %.9g\n", t4 - t3);
+        fprintf(out, "n_induced %d\n", nind);
+    } else {
+        fprintf(stderr, "unknown mode %s\n", mode);
+        return 2;
+    }
+    fclose(out);
+    return 0;
+}

@@ -1,0 +1,301 @@
+"""Write the golden vectors of tests/golden/ from the reference build.
+
+TEST INFRASTRUCTURE, run in the development container only (it needs the
+reference tree for oracle/_ref and the reference's data files):
+
+    make -C oracle all ref && python tests/golden/make_fixtures.py
+
+Every case directory holds the inputs (`control`, `POSFF`) and `ref.npz`,
+the outputs of oracle/_ref/mdff20_ref_driver (the reference's own routines,
+see oracle/ref_driver.c) at %.17g.  Inputs are either data files of the
+reference (src/*.s, src/POSFF.*, examples/ex0, ex1) or synthetic melts made
+here from a fixed seed (SiO2-like: cations q=+2.4, anions q=-1.2, 2.2 g/cm^3,
+uniform positions with 1.6 A minimum-image rejection).  `ccd_lj.npz` holds
+the Cambridge Cluster Database energies and structures that the reference's
+example 2 compares against (examples/ex2/config/REFERENCE and <N> files).
+"""
+import os
+import shutil
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+REF = "/root/reference"
+sys.path.insert(0, ROOT)
+
+from oracle import refrun  # noqa: E402
+
+M_O, M_SI = 15.99994, 28.084
+RHO_UNIT = 1.660538782
+
+
+def sio2_box(n_units, rho=2.2):
+    mtot = n_units * (M_SI + 2 * M_O)
+    return (mtot / (rho / RHO_UNIT)) ** (1.0 / 3.0)
+
+
+def random_positions(n, L, rng, dmin=1.6, max_tries=200000):
+    """uniform positions, minimum-image rejection at dmin"""
+    pos = np.zeros((n, 3))
+    k = 0
+    tries = 0
+    while k < n:
+        tries += 1
+        if tries > max_tries:
+            raise RuntimeError("packing failed")
+        p = rng.uniform(-0.5 * L, 0.5 * L, 3)
+        if k:
+            d = pos[:k] - p
+            d -= L * np.rint(d / L)
+            if np.min(np.einsum("ij,ij->i", d, d)) < dmin * dmin:
+                continue
+        pos[k] = p
+        k += 1
+    return pos
+
+
+def write_posff(path, name, L, types, counts, pos, vel=None):
+    with open(path, "w") as fh:
+        fh.write(f"{len(pos)}\n{name}\n")
+        fh.write(f"{L:.12f} 0.0 0.0\n0.0 {L:.12f} 0.0\n0.0 0.0 {L:.12f}\n")
+        fh.write(f"{len(types)}\n{' '.join(types)}\n{' '.join(str(c) for c in counts)}\nCartesian\n")
+        ti = np.repeat(np.arange(len(types)), counts)
+        for i, p in enumerate(pos):
+            v = vel[i] if vel is not None else (0.0, 0.0, 0.0)
+            fh.write(f"{types[ti[i]]} {p[0]:.15f} {p[1]:.15f} {p[2]:.15f} "
+                     f"{v[0]:.15f} {v[1]:.15f} {v[2]:.15f} 0.0 0.0 0.0\n")
+
+
+def pair_params(seed):
+    """BHMFT per-type-pair parameters A in [1e3,2e4] eV, B in [2.5,5] 1/A,
+    C in [0,150] eV A^6, D = 0, drawn from `seed`; a draw is kept only if
+    every pair (with its Coulomb term) is repulsive on [0.6, 1.5] A, so no
+    pair can collapse in a 3000 K melt.  Returns (seed used, A, B, C) with
+    upper triangles in the order O-O, O-Si, Si-Si."""
+    q = {0: -1.2, 1: 2.4}
+    pairs = [(0, 0), (0, 1), (1, 1)]
+    r = np.linspace(0.6, 1.5, 400)
+    while True:
+        rng = np.random.default_rng(seed)
+        A = rng.uniform(1e3, 2e4, 3)
+        B = rng.uniform(2.5, 5.0, 3)
+        Cc = rng.uniform(0.0, 150.0, 3)
+        ok = True
+        for k, (i, j) in enumerate(pairs):
+            dV = -A[k] * B[k] * np.exp(-B[k] * r) + 6 * Cc[k] / r ** 7 - 14.3996441494161 * q[i] * q[j] / r ** 2
+            if np.any(dV >= 0):
+                ok = False
+        if ok:
+            return seed, A, B, Cc
+        seed += 1
+
+
+def control_text(**kw):
+    lines = []
+    for k, v in kw.items():
+        if isinstance(v, bool):
+            v = "true" if v else "false"
+        if isinstance(v, (list, tuple)):
+            v = " ".join(str(x) for x in v)
+        lines.append(f"{k} {v}")
+    return "\n".join(lines) + "\n"
+
+
+def sio2_field(seed_used, A, B, Cc, pim, alpha, kmax, rc, extra=""):
+    t = [
+        "# SiO2-like field: type 0 = O (anion), type 1 = Si (cation)",
+        f"# BHMFT parameters drawn with seed {seed_used} (see tests/golden/make_fixtures.py)",
+        "massit 15.99994", "massit 28.084", "qit -1.2", "qit 2.4",
+        "lbhmft true",
+        "Abhmftd " + " ".join(repr(float(x)) for x in A),
+        "Bbhmftd " + " ".join(repr(float(x)) for x in B),
+        "Cbhmftd " + " ".join(repr(float(x)) for x in Cc),
+        "Dbhmftd 0.0 0.0 0.0",
+        "lcoulombic true", "lautoES false", f"alphaES {alpha}", f"kES {kmax} {kmax} {kmax}",
+        f"cutshortrange {float(rc)!r}", f"cutlongrange {float(rc)!r}",
+    ]
+    if pim:
+        t += ["polit 1.2 0.0 0.0 0.0 1.2 0.0 0.0 0.0 1.2",
+              "polit 0.0 0.0 0.0 0.0 0.0 0.0 0.0 0.0 0.0",
+              "lpoldamping 0 0 0 true", "pol_damp_b 0 0 0 2.0", "pol_damp_c 0 0 0 1.5", "pol_damp_k 0 0 0 4",
+              "lpoldamping 0 0 1 true", "pol_damp_b 0 0 1 2.0", "pol_damp_c 0 0 1 1.5", "pol_damp_k 0 0 1 4",
+              "algo_pim scf"]
+    return "\n".join(t) + "\n" + extra
+
+
+def run_case(name, control, posff_src=None, posff_text=None, mode="static", args=(), posff_writer=None):
+    d = os.path.join(HERE, name)
+    os.makedirs(d, exist_ok=True)
+    with open(os.path.join(d, "control"), "w") as fh:
+        fh.write(control)
+    if posff_src:
+        shutil.copy(posff_src, os.path.join(d, "POSFF"))
+    elif posff_writer:
+        posff_writer(os.path.join(d, "POSFF"))
+    with tempfile.TemporaryDirectory() as tmp:
+        shutil.copy(os.path.join(d, "control"), os.path.join(tmp, "control"))
+        shutil.copy(os.path.join(d, "POSFF"), os.path.join(tmp, "POSFF"))
+        res = refrun.run(tmp, mode, *args, threads=1)
+    out = {"mode": np.array(mode), "alphaES": np.array(res["alphaES"]), "kES": np.array(res["kES"]),
+           "nk": np.array(res["nk"])}
+    for key in ("v0x", "v0y", "v0z"):
+        if key in res:
+            out[key] = res[key]
+    steps = res["steps"]
+    out["step"] = np.array([s["step"] for s in steps])
+    out["n_scf"] = np.array([s["n_scf"] for s in steps])
+    out["verlet_updates"] = np.array([s["verlet_updates"] for s in steps])
+    out["scalars"] = np.stack([s["scalars"] for s in steps])
+    for k in ("tau_lj", "tau_bhmftd", "tau_coul"):
+        out[k] = np.stack([s[k].reshape(3, 3) for s in steps])
+    dumped = [s for s in steps if "rx" in s]
+    out["state_step"] = np.array([s["step"] for s in dumped])
+    for k in ("rx", "ry", "rz", "vx", "vy", "vz", "fx", "fy", "fz", "mu"):
+        if dumped and k in dumped[0]:
+            out[k] = np.stack([s[k] for s in dumped])
+    for k, v in out.items():
+        if np.issubdtype(np.asarray(v).dtype, np.floating) and not np.all(np.isfinite(v)):
+            print(f"  warning: non-finite values in {name}:{k}")
+    np.savez_compressed(os.path.join(d, "ref.npz"), **out)
+    print(f"{name}: {len(steps)} records, n_scf {out['n_scf'][:5]}")
+    return out
+
+
+def melt(n_units, seed, pos_seed):
+    L = sio2_box(n_units)
+    rng = np.random.default_rng(pos_seed)
+    # O first, then Si
+    pos = random_positions(3 * n_units, L, rng)
+    return L, pos
+
+
+def equilibrate(L, pos, field, steps=300, temp=3000.0):
+    """rigid-ion pre-equilibration of a random packing (input generation
+    only) with the oracle restatement: NVE velocity Verlet with a Berendsen
+    rescale every step, dt 1 fs"""
+    sys.path.insert(0, ROOT)
+    import mdff20_amd as m
+    from oracle.oracle import Oracle
+    with tempfile.TemporaryDirectory() as tmp:
+        ctl = os.path.join(tmp, "control")
+        with open(ctl, "w") as fh:
+            fh.write(field + control_text(integrator="nve-vv", dt=0.001, temp=temp, tauTberendsen=0.001,
+                                          nequil=steps + 1, nequilT=1, lverletL=True, skindiff=0.5))
+        p = m.read_control(ctl, 2)
+    n = len(pos)
+    nu = n // 3
+    typ = np.array([0] * (2 * nu) + [1] * nu, dtype=np.int32)
+    A = np.diag([L, L, L])
+    o = Oracle(p, typ, A, pos.T.copy(), threads=8)
+    o.init_velocities()
+    o.md_init()
+    for s in range(1, steps + 1):
+        o.md_step(s)
+    r = o.positions().T
+    r -= L * np.rint(r / L)
+    o.close()
+    return r
+
+
+def main():
+    only = set(sys.argv[1:])
+    want = lambda n: not only or n in only
+    src = os.path.join(REF, "src")
+    # ---- tiny analytic inputs of the reference (src/)
+    if want("twocharges"):
+        run_case("twocharges", open(os.path.join(src, "control_twocharges.s")).read(),
+                 os.path.join(src, "POSFF.twocharges"))
+    if want("onedipole"):
+        run_case("onedipole", open(os.path.join(src, "control_onedipoles.s")).read(),
+                 os.path.join(src, "POSFF.onedipole"))
+    if want("twocharges_twopol"):
+        run_case("twocharges_twopol", open(os.path.join(src, "control_twocharges_twopol.s")).read(),
+                 os.path.join(src, "POSFF.twocharges_twopol"))
+    # ---- SiO2 102 ions and BSiO 200 ions melts of src/ with the scf algorithm
+    if want("sio2_102_static") or want("sio2_102_nvt"):
+        base = open(os.path.join(src, "control_SiO.s")).read().replace("algo_pim scfkO", "algo_pim scf")
+        if want("sio2_102_static"):
+            run_case("sio2_102_static", base.replace("lstatic false", "lstatic true"),
+                     os.path.join(src, "POSFF.SiO2"))
+        if want("sio2_102_nvt"):
+            run_case("sio2_102_nvt", base, os.path.join(src, "POSFF.SiO2"), mode="md", args=(12, 6))
+    if want("bsio_200_nvt"):
+        base = open(os.path.join(src, "control_BSiO.s")).read().replace("algo_pim scfkO", "algo_pim scf")
+        run_case("bsio_200_nvt", base, os.path.join(src, "POSFF.BSiO"), mode="md", args=(6, 3))
+    # ---- Lennard-Jones FCC of examples/ex0 (static) and ex1 (NVE with rescale)
+    if want("lj_fcc_static"):
+        run_case("lj_fcc_static", open(os.path.join(REF, "examples/ex0/config/control.s")).read(),
+                 os.path.join(REF, "examples/ex0/config/POSFF"))
+    if want("lj_fcc_nve"):
+        run_case("lj_fcc_nve", open(os.path.join(REF, "examples/ex1/config/control.s")).read(),
+                 os.path.join(REF, "examples/ex1/config/POSFF"), mode="md", args=(40, 20))
+    # ---- synthetic SiO2-like melts
+    seed_used, A, B, Cc = pair_params(20260)
+    if want("melt216_nve") or want("melt216_raw"):
+        # rigid ions, N=216, Ewald alpha 0.3, rc = L/2, kmax 8, NVE VV 1 fs, 3000 K
+        nu = 72
+        L = sio2_box(nu)
+        _, pos = melt(nu, seed_used, 11)
+        field = sio2_field(seed_used, A, B, Cc, False, 0.3, 8, 0.5 * L)
+        ctl = field + control_text(lverletL=True, skindiff=0.5, integrator="nve-vv", dt=0.001, temp=3000.0,
+                                   tauTberendsen=0.001, npas=200, nprint=1000, fprint=1000, cprint=1000,
+                                   tprint=1000)
+        if want("melt216_raw"):
+            # the raw 1.6 A packing is far from equilibrium and the trajectory
+            # is strongly chaotic: 40 steps only
+            run_case("melt216_raw", ctl, mode="md", args=(40, 20),
+                     posff_writer=lambda p: write_posff(p, "melt216raw", L, ["O", "Si"], [2 * nu, nu], pos))
+        if want("melt216_nve"):
+            eq = equilibrate(L, pos, field, steps=300)
+            run_case("melt216_nve", ctl, mode="md", args=(200, 100),
+                     posff_writer=lambda p: write_posff(p, "melt216", L, ["O", "Si"], [2 * nu, nu], eq))
+    if want("pim192_static") or want("pim192_nvt") or want("pim192_fixed"):
+        # polarisable ions, N=192, alpha_O 1.2 A^3, TT b=2 c=1.5 k=4, rc 7 A,
+        # kmax 6, pre-equilibrated rigid-ion start
+        nu = 64
+        L = sio2_box(nu)
+        _, pos = melt(nu, seed_used, 23)
+        rig = sio2_field(seed_used, A, B, Cc, False, 0.35, 6, 7.0)
+        pos = equilibrate(L, pos, rig, steps=300)
+        field = sio2_field(seed_used, A, B, Cc, True, 0.35, 6, 7.0)
+        wr = lambda p: write_posff(p, "pim192", L, ["O", "Si"], [2 * nu, nu], pos)
+        if want("pim192_static"):
+            ctl = field + control_text(lverletL=True, skindiff=1.0, lstatic=True, conv_tol_ind=1e-8,
+                                       min_scf_pim_iter=3, max_scf_pim_iter=400)
+            run_case("pim192_static", ctl, posff_writer=wr)
+        if want("pim192_fixed"):
+            # fixed SCF schedule: exactly 12 iterations, independent of round-off
+            ctl = field + control_text(lverletL=True, skindiff=1.0, lstatic=True, conv_tol_ind=1e-30,
+                                       min_scf_pim_iter=12, max_scf_pim_iter=12)
+            run_case("pim192_fixed", ctl, posff_writer=wr)
+        if want("pim192_nvt"):
+            ctl = field + control_text(lverletL=True, skindiff=1.0, integrator="nvt-nhcn", nhc_n=3,
+                                       nhc_yosh_order=3, nhc_mults=2, timesca_thermo=0.1, temp=2500.0, dt=0.001,
+                                       conv_tol_ind=1e-8, min_scf_pim_iter=3, max_scf_pim_iter=400,
+                                       algo_extrapolate_dipole="aspc", extrapolate_order=2, nprint=1000)
+            run_case("pim192_nvt", ctl, mode="md", args=(10, 5), posff_writer=wr)
+    # ---- CCD LJ clusters (examples/ex2): reference energies and structures
+    if want("ccd"):
+        cdir = os.path.join(REF, "examples/ex2/config")
+        names, energies, coords = [], [], {}
+        with open(os.path.join(cdir, "REFERENCE")) as fh:
+            for line in fh:
+                if line.startswith("#") or not line.strip():
+                    continue
+                t = line.split()
+                names.append(t[0])
+                energies.append(float(t[1]))
+        for nm in names:
+            f = os.path.join(cdir, nm)
+            if os.path.exists(f):
+                coords[nm] = np.loadtxt(f).reshape(-1, 3)
+        np.savez_compressed(os.path.join(HERE, "ccd_lj.npz"), names=np.array(names), energies=np.array(energies),
+                            **{f"xyz_{k}": v for k, v in coords.items()})
+        print(f"ccd: {len(names)} energies, {len(coords)} structures")
+
+
+if __name__ == "__main__":
+    main()
