@@ -1,0 +1,285 @@
+"""Checks that pin the oracle where no golden vector does: known answers of
+the reference's own example 2 (CCD Lennard-Jones clusters), Madelung
+constants, closed forms, finite differences and integrator invariants."""
+import os
+import tempfile
+
+import numpy as np
+import pytest
+
+import mdff20_amd as m
+from conftest import GOLDEN, load_case, rel
+
+Oracle = pytest.importorskip("oracle.oracle").Oracle
+
+
+def params_from(text, ntype):
+    with tempfile.TemporaryDirectory() as d:
+        f = os.path.join(d, "control")
+        open(f, "w").write(text)
+        return m.read_control(f, ntype)
+
+
+CCD_CONTROL = """lstatic true
+cutshortrange 100.0
+lnmlj true
+trunc linear
+epslj 1.0
+sigmalj 1.0
+massit 1.0
+"""
+
+
+@pytest.fixture(scope="module")
+def ccd():
+    return np.load(os.path.join(GOLDEN, "ccd_lj.npz"), allow_pickle=False)
+
+
+def test_ccd_lj_cluster_energies(ccd):
+    """examples/ex2: NMLJ (p=6, q=12, sigma*=2^(1/6) sigma) energies of the
+    Cambridge Cluster Database global minima, 6 decimals."""
+    p = params_from(CCD_CONTROL, 1)
+    names = list(ccd["names"])
+    energies = ccd["energies"]
+    checked = 0
+    for nm, e in zip(names, energies):
+        key = f"xyz_{nm}"
+        if key not in ccd.files:
+            continue
+        xyz = ccd[key]
+        n = len(xyz)
+        o = Oracle(p, np.zeros(n, np.int32), np.diag([100.0] * 3), xyz.T.copy(), threads=4)
+        t = o.engforce(0)
+        assert t["u_lj"] == pytest.approx(e, abs=5e-6), nm
+        checked += 1
+    assert checked >= 150
+
+
+def madelung_setup(kind, reps, alpha_l, K):
+    """NaCl (rock salt) or CsCl lattices, unit charges, reduced units"""
+    if kind == "nacl":
+        base = np.array([[0, 0, 0], [0, .5, .5], [.5, 0, .5], [.5, .5, 0]])
+        cat = base
+        an = base + np.array([.5, 0, 0])
+        a = 2.0
+        r0 = a / 2
+    else:
+        cat = np.array([[0, 0, 0]])
+        an = np.array([[.5, .5, .5]])
+        a = 2.0 / np.sqrt(3.0)
+        r0 = 1.0
+    cells = np.array([[i, j, k] for i in range(reps) for j in range(reps) for k in range(reps)])
+    pc = (cat[None, :, :] + cells[:, None, :]).reshape(-1, 3) * a
+    pa = (an[None, :, :] + cells[:, None, :]).reshape(-1, 3) * a
+    L = a * reps
+    pos = np.vstack([pc, pa])
+    pos -= L * np.rint(pos / L)
+    n = len(pos)
+    rc = 0.5 * L
+    ctl = f"""lstatic true
+lreduced true
+lverletL false
+cutlongrange {float(rc)!r}
+lcoulombic true
+alphaES {float(alpha_l / L)!r}
+kES {K} {K} {K}
+massit 1.0
+massit 1.0
+qit 1.0
+qit -1.0
+"""
+    p = params_from(ctl, 2)
+    typ = np.array([0] * len(pc) + [1] * len(pa), np.int32)
+    return p, typ, np.diag([L] * 3), pos.T.copy(), n, r0
+
+
+@pytest.mark.parametrize("kind,M", [("nacl", 1.747564594633), ("cscl", 1.762674773070)])
+def test_madelung_constants(kind, M):
+    p, typ, A, r, n, r0 = madelung_setup(kind, 2, 9.0, 14)
+    o = Oracle(p, typ, A, r, threads=8)
+    t = o.engforce(0)
+    npair = n // 2
+    assert t["u_coul"] == pytest.approx(-M * npair / r0, rel=1e-9)
+    # every ion sits on an inversion centre: zero force
+    assert np.max(np.abs(o.forces())) < 1e-8  # erfc(alpha L/2) ~ 2e-10 leftovers at |r| = L/2
+
+
+def test_ewald_alpha_independence():
+    """the converged total does not depend on the splitting parameter"""
+    p, cfg, _ = load_case("melt216_nve")
+    us = []
+    for alpha, K in [(0.62, 15), (0.72, 18)]:
+        p.alphaES = alpha
+        p.kES[0] = p.kES[1] = p.kES[2] = K
+        p.lnmlj = p.lbhmft = p.lbhmftd = 0
+        o = Oracle(p, cfg.typia, cfg.A, cfg.r, threads=8)
+        us.append(o.engforce(0)["u_coul"])
+    assert us[0] == pytest.approx(us[1], rel=1e-9)
+
+
+def test_one_dipole_closed_form():
+    """one dipole in a periodic cube with tin-foil boundary: U = -2 pi mu^2 / (3V)"""
+    p, cfg, ref = load_case("onedipole")
+    o = Oracle(p, cfg.typia, cfg.A, cfg.r)
+    u = o.engforce(0)["u_coul"]
+    V = cfg.A[0, 0] ** 3
+    assert u == pytest.approx(-2 * np.pi / (3 * V), rel=1e-7)
+
+
+def test_two_charges_vs_spherical_image_sum():
+    """+1/-1 at distance 1 in a 10 A cube (reduced units): Ewald (tin-foil)
+    equals the spherical-shell image sum minus 2 pi M^2 / 3V"""
+    p, cfg, ref = load_case("twocharges")
+    o = Oracle(p, cfg.typia, cfg.A, cfg.r)
+    u = o.engforce(0)["u_coul"]
+    L = 10.0
+    nmax = 28
+    g = np.arange(-nmax, nmax + 1)
+    X, Y, Z = np.meshgrid(g, g, g, indexing="ij")
+    cells = np.stack([X.ravel(), Y.ravel(), Z.ravel()], 1) * L
+    cells = cells[np.einsum("ij,ij->i", cells, cells) <= (nmax * L) ** 2]
+    d = np.array([1.0, 0, 0])
+    r_like = np.linalg.norm(cells, axis=1)
+    r_like = r_like[r_like > 0]
+    r_unlike = np.linalg.norm(cells + d, axis=1)
+    e_sphere = np.sum(1.0 / r_like) - np.sum(1.0 / r_unlike)  # per cell: 2 * (1/2) * like - unlike
+    V = L ** 3
+    assert u == pytest.approx(e_sphere - 2 * np.pi / (3 * V), abs=2e-4)
+
+
+def test_induced_dipole_next_to_charge():
+    """a polarisable neutral ion at distance r from a unit charge in a large
+    box: mu = alpha E with E close to q / r^2"""
+    ctl = """lstatic true
+lreduced true
+lverletL false
+cutlongrange 15.0
+lcoulombic true
+alphaES 0.25
+kES 12 12 12
+massit 1.0
+massit 1.0
+qit 1.0
+qit 0.0
+polit 0.0 0.0 0.0 0.0 0.0 0.0 0.0 0.0 0.0
+polit 0.01 0.0 0.0 0.0 0.01 0.0 0.0 0.0 0.01
+conv_tol_ind 1e-14
+min_scf_pim_iter 3
+max_scf_pim_iter 50
+"""
+    p = params_from(ctl, 2)
+    L, r0 = 30.0, 2.0
+    pos = np.array([[0.0, 0, 0], [0, 0, r0]]).T.copy()
+    o = Oracle(p, np.array([0, 1], np.int32), np.diag([L] * 3), pos)
+    o.engforce(0)
+    mu = o.dipoles()[1]
+    assert mu[2] == pytest.approx(0.01 / r0 ** 2, rel=5e-3)
+    assert abs(mu[0]) < 1e-12 and abs(mu[1]) < 1e-12
+
+
+def _energy(o, r):
+    o.set_positions(r)
+    t = o.engforce(0)
+    return t["u_tot"]
+
+
+@pytest.mark.parametrize("case,tol", [("melt216_nve", 1e-6), ("pim192_static", 1e-5)])
+def test_forces_are_minus_energy_gradient(case, tol):
+    p, cfg, _ = load_case(case)
+    p.lverletL = 0
+    p.conv_tol_ind = 1e-13
+    p.max_scf_pim_iter = 400
+    o = Oracle(p, cfg.typia, cfg.A, cfg.r, threads=8)
+    o.engforce(0)
+    F = o.forces()
+    h = 1e-5
+    rng = np.random.default_rng(3)
+    for i in rng.choice(cfg.nion, 4, replace=False):
+        for a in range(3):
+            r = cfg.r.copy()
+            r[a, i] += h
+            ep = _energy(o, r)
+            r[a, i] -= 2 * h
+            em = _energy(o, r)
+            fd = -(ep - em) / (2 * h)
+            assert fd == pytest.approx(F[a, i], abs=tol * max(1.0, abs(F[a, i])) * 10, rel=tol), (i, a)
+
+
+def test_velocity_verlet_time_reversible():
+    p, cfg, _ = load_case("melt216_nve")
+    o = Oracle(p, cfg.typia, cfg.A, cfg.r, threads=8)
+    o.init_velocities()
+    r0, v0 = o.positions(), o.velocities()
+    o.md_init()
+    for s in range(1, 21):
+        o.md_step(s)
+    o.set_velocities(-o.velocities())
+    o.md_init()
+    for s in range(1, 21):
+        o.md_step(s)
+    r1 = o.positions()
+    assert rel(r1, r0) < 1e-11
+    assert rel(-o.velocities(), v0) < 1e-8
+
+
+def test_nve_energy_error_scales_with_dt_squared():
+    """velocity Verlet: the total-energy fluctuation over a fixed time falls
+    by ~4 when dt is halved (converged Ewald so the cutoff does not add
+    noise)"""
+    p, cfg, _ = load_case("melt216_nve")
+    p.alphaES = 0.65
+    p.kES[0] = p.kES[1] = p.kES[2] = 15
+    spread = []
+    for dt, nsteps in [(0.002, 40), (0.001, 80)]:
+        p.dt = dt
+        o = Oracle(p, cfg.typia, cfg.A, cfg.r, threads=8)
+        o.init_velocities()
+        e = [o.md_init()["e_tot"]]
+        for s in range(1, nsteps + 1):
+            e.append(o.md_step(s)["e_tot"])
+        spread.append(np.ptp(e))
+        o.close()
+    assert 3.0 < spread[0] / spread[1] < 5.0
+
+
+def test_nhc_conserved_quantity():
+    """NVT Nose-Hoover chain: the extended energy H = E + e_nvt drifts much
+    less than E itself, and its error falls when dt is halved"""
+    p, cfg, _ = load_case("pim192_nvt")
+    p.alphaES = 0.75
+    p.kES[0] = p.kES[1] = p.kES[2] = 12
+    p.timesca_thermo = 0.02
+    spread_h, spread_e = [], []
+    for dt, nsteps in [(0.002, 20), (0.001, 40)]:
+        p.dt = dt
+        o = Oracle(p, cfg.typia, cfg.A, cfg.r, threads=8)
+        o.init_velocities()
+        t = o.md_init()
+        h, e = [t["h_tot"]], [t["e_tot"]]
+        for s in range(1, nsteps + 1):
+            t = o.md_step(s)
+            h.append(t["h_tot"])
+            e.append(t["e_tot"])
+        spread_h.append(np.ptp(h))
+        spread_e.append(np.ptp(e))
+        o.close()
+    assert spread_h[1] < 0.1 * spread_e[1]
+    assert spread_h[0] / spread_h[1] > 2.0
+
+
+@pytest.mark.parametrize("order", [1, 3, 5, 7])
+def test_yoshida_weights_sum_to_one(order):
+    """the Yoshida-Suzuki weights of chain_nhcn integrate one full step"""
+    import math
+    if order == 1:
+        w = [1.0]
+    elif order == 3:
+        w0 = 1.0 / (2.0 - 2.0 ** (1.0 / 3.0))
+        w = [w0, 1 - 2 * w0, w0]
+    elif order == 5:
+        w0 = 1.0 / (4.0 - 4.0 ** (1.0 / 3.0))
+        w = [w0] * 4 + [1 - 4 * w0]
+    else:
+        w = [.78451361047756, .235573213359357, -1.17767998417887]
+        w = w + [1 - 2 * sum(w)] + w[::-1]
+    assert math.fsum(w) == pytest.approx(1.0, abs=1e-14)
