@@ -1,0 +1,295 @@
+#!/usr/bin/env python3
+"""Benchmark: MD steps/s of the 12,288-ion polarisable-ion (PIM) SiO2-like
+melt on one MI355X, mixed form vs FP64 form.
+
+Workload (synthetic, seeded): N = 12288 ions (4096 cations q=+2.4, 8192
+anions q=-1.2, anion polarisability 1.2 A^3, Tang-Toennies b=2 c=1.5 k=4),
+2.2 g/cm^3 (L = 57.06 A), BHMFT pair potential (the seeded parameter set of
+tests/golden), multipole Ewald alpha = 0.3 1/A, rc = 9 A, kmax = 16 (17968
+k-vectors), Verlet skin 1 A, SCF to 1e-8 on the field, NVT Nose-Hoover chain
+(length 3) at 2500 K, dt = 1 fs.  The random 1.6 A packing is relaxed with
+300 rigid-ion steps on the GPU before anything is timed.
+
+Prints one JSON line.  `value` is the mixed form's steps/s; the FP64 form's
+steps/s, the speed ratio and the force/dipole/energy/stress errors of the
+mixed form against the FP64 form are in the same line.
+"""
+import argparse
+import json
+import os
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+M_O, M_SI = 15.99994, 28.084
+RHO_UNIT = 1.660538782
+PEAK = {"f64": 78.6e12, "f32": 157.3e12, "bf16": 2.5e15, "hbm": 8.0e12}
+# BHMFT parameters of tests/golden (make_fixtures.pair_params, seed 20349):
+# upper triangles O-O, O-Si, Si-Si
+PAIR_A = (13843.775422359393, 9907.395718764405, 13891.629698472487)
+PAIR_B = (2.6317230798523927, 2.977352637018226, 2.7189376696105843)
+PAIR_C = (13.401132428691126, 18.44105648347189, 31.384716111068446)
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def packing(n, L, seed, dmin=1.6):
+    """uniform positions with minimum-image rejection, on a cell grid"""
+    rng = np.random.default_rng(seed)
+    nc = max(1, int(L // dmin))
+    cs = L / nc
+    grid = {}
+    pos = np.zeros((n, 3))
+    k = 0
+    while k < n:
+        p = rng.uniform(0.0, L, 3)
+        c = (p // cs).astype(int) % nc
+        ok = True
+        for dx in (-1, 0, 1):
+            for dy in (-1, 0, 1):
+                for dz in (-1, 0, 1):
+                    key = ((c[0] + dx) % nc, (c[1] + dy) % nc, (c[2] + dz) % nc)
+                    for j in grid.get(key, ()):
+                        d = pos[j] - p
+                        d -= L * np.rint(d / L)
+                        if d @ d < dmin * dmin:
+                            ok = False
+                            break
+                    if not ok:
+                        break
+                if not ok:
+                    break
+            if not ok:
+                break
+        if not ok:
+            continue
+        pos[k] = p
+        grid.setdefault(tuple(c), []).append(k)
+        k += 1
+    pos -= 0.5 * L
+    return pos
+
+
+def control(pim, alpha=0.3, kmax=16, rc=9.0, extra=""):
+    t = ["massit 15.99994", "massit 28.084", "qit -1.2", "qit 2.4", "lbhmft true",
+         "Abhmftd " + " ".join(repr(x) for x in PAIR_A),
+         "Bbhmftd " + " ".join(repr(x) for x in PAIR_B),
+         "Cbhmftd " + " ".join(repr(x) for x in PAIR_C),
+         "Dbhmftd 0.0 0.0 0.0", "lcoulombic true", f"alphaES {alpha}", f"kES {kmax} {kmax} {kmax}",
+         f"cutshortrange {rc}", f"cutlongrange {rc}", "lverletL true", "skindiff 1.0"]
+    if pim:
+        t += ["polit 1.2 0.0 0.0 0.0 1.2 0.0 0.0 0.0 1.2", "polit 0.0 0.0 0.0 0.0 0.0 0.0 0.0 0.0 0.0",
+              "lpoldamping 0 0 0 true", "pol_damp_b 0 0 0 2.0", "pol_damp_c 0 0 0 1.5", "pol_damp_k 0 0 0 4",
+              "lpoldamping 0 0 1 true", "pol_damp_b 0 0 1 2.0", "pol_damp_c 0 0 1 1.5", "pol_damp_k 0 0 1 4",
+              "algo_pim scf", "conv_tol_ind 1e-8", "min_scf_pim_iter 1", "max_scf_pim_iter 400",
+              "algo_extrapolate_dipole aspc", "extrapolate_order 2",
+              "integrator nvt-nhcn", "nhc_n 3", "nhc_yosh_order 3", "nhc_mults 2", "timesca_thermo 0.1",
+              "temp 2500.0", "dt 0.001"]
+    else:
+        t += ["integrator nve-vv", "temp 2500.0", "dt 0.001", "tauTberendsen 0.001", "nequil 100000",
+              "nequilT 1"]
+    return "\n".join(t) + "\n" + extra
+
+
+def params_of(text):
+    import mdff20_amd as m
+    with tempfile.TemporaryDirectory() as d:
+        f = os.path.join(d, "control")
+        open(f, "w").write(text)
+        return m.read_control(f, 2)
+
+
+def system(nunits, seed):
+    L = (nunits * (M_SI + 2 * M_O) / (2.2 / RHO_UNIT)) ** (1.0 / 3.0)
+    pos = packing(3 * nunits, L, seed)
+    typ = np.array([0] * (2 * nunits) + [1] * nunits, dtype=np.int32)
+    return L, pos, typ
+
+
+def equilibrate(L, pos, typ, steps):
+    import mdff20_amd as m
+    p = params_of(control(False))
+    A = np.diag([L, L, L])
+    e = m.Engine(p, typ, A, pos.T.copy(), precision=m.FP64)
+    e.init_velocities()
+    e.md_init()
+    t0 = time.time()
+    th = e.md_run(1, steps)
+    r = e.positions()
+    e.close()
+    log(f"rigid-ion relaxation: {steps} steps in {time.time() - t0:.1f} s, T = {th['temp_r']:.0f} K")
+    return r
+
+
+def timed_run(p, typ, A, r, precision, steps, warmup):
+    import mdff20_amd as m
+    e = m.Engine(p, typ, A, r, precision=precision)
+    e.init_velocities()
+    e.md_init()
+    if warmup:
+        e.md_run(1, warmup)
+    c0 = e.counters()
+    t0 = time.perf_counter()
+    th = e.md_run(warmup + 1, steps)
+    t1 = time.perf_counter()
+    c1 = e.counters()
+    stats = dict(scf_fp64_per_step=(c1["scf_fp64"] - c0["scf_fp64"]) / steps,
+                 scf_inner_per_step=(c1["scf_inner"] - c0["scf_inner"]) / steps,
+                 rmsd=th["rmsd"], temp=th["temp_r"], h_tot=th["h_tot"])
+    return e, (t1 - t0), stats
+
+
+def cpu_baseline(p_text, L, pos, typ, n_scf, threads):
+    """the reference's own routines (oracle/_ref) on the host cores: one
+    static field, n_ind induced-dipole Ewald calls and the final force call
+    at this N; steps/s from the SCF iteration count of the FP64 GPU run"""
+    from oracle import refrun
+    if not refrun.available():
+        return None
+    with tempfile.TemporaryDirectory() as d:
+        open(os.path.join(d, "control"), "w").write(p_text + "lstatic true\n")
+        nu = len(typ) // 3
+        with open(os.path.join(d, "POSFF"), "w") as fh:
+            fh.write(f"{len(typ)}\nbench\n{L!r} 0.0 0.0\n0.0 {L!r} 0.0\n0.0 0.0 {L!r}\n2\nO Si\n{2 * nu} {nu}\n"
+                     "Cartesian\n")
+            for i in range(len(typ)):
+                fh.write(f"{'O' if typ[i] == 0 else 'Si'} {pos[0, i]!r} {pos[1, i]!r} {pos[2, i]!r}\n")
+        n_ind = 2
+        t0 = time.time()
+        res = refrun.run(d, "time", n_ind, threads=threads, timeout=900)
+        wall = time.time() - t0
+    t_step = res["t_short"] + res["t_static"] + n_scf * res["t_induced"] + res["t_force"]
+    return dict(value=1.0 / t_step, unit="steps/s", cores=int(res["threads"]), kind="reference",
+                sample=(f"reference build (shipped -O0 flags, OpenMP) at N={len(typ)}: short range, static field, "
+                        f"{n_ind} induced-dipole Ewald calls and the force call timed once ({wall:.1f} s wall); "
+                        f"step = t_short + t_static + {n_scf:.1f} x t_induced + t_force "
+                        f"(SCF iterations per step of the FP64 GPU run)"),
+                t_short=res["t_short"], t_static=res["t_static"], t_induced=res["t_induced"],
+                t_force=res["t_force"])
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--units", type=int, default=4096, help="SiO2 units (N = 3 x units)")
+    ap.add_argument("--relax", type=int, default=300)
+    ap.add_argument("--seed", type=int, default=2026)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "16")))
+    ap.add_argument("--forms", default="fp64,mixed")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (every box of this build has one GPU); --gpus {args.gpus} "
+              f"is not supported: the multi-GPU sharding of this path is described in DESIGN.md", file=sys.stderr)
+        sys.exit(2)
+    import mdff20_amd as m
+    if m.device_count() < 1:
+        raise RuntimeError("no HIP device")
+    L, pos, typ = system(args.units, args.seed)
+    n = len(typ)
+    log(f"N = {n}, L = {L:.3f} A")
+    r = equilibrate(L, pos, typ, args.relax)
+    A = np.diag([L, L, L])
+    p_text = control(True)
+    p = params_of(p_text)
+    forms = [f.strip() for f in args.forms.split(",") if f.strip()]
+    res = {}
+    eng = {}
+    for f in forms:
+        prec = m.FP64 if f == "fp64" else m.MIXED
+        e, dt, stats = timed_run(p, typ, A, r, prec, args.steps, args.warmup)
+        res[f] = dict(seconds=dt, steps_per_s=args.steps / dt, **stats)
+        eng[f] = e
+        log(f"{f}: {args.steps} steps in {dt:.3f} s = {args.steps / dt:.2f} steps/s, "
+            f"SCF FP64 evaluations/step {stats['scf_fp64_per_step']:.1f}, inner {stats['scf_inner_per_step']:.1f}")
+    # accuracy of the mixed form against the FP64 form on one configuration
+    acc = None
+    if "fp64" in forms and "mixed" in forms:
+        ef, em = m.Engine(p, typ, A, r, precision=m.FP64), m.Engine(p, typ, A, r, precision=m.MIXED)
+        tf, tm = ef.engforce(0), em.engforce(0)
+        Ff, Fm = ef.forces(), em.forces()
+        muf, mum = ef.dipoles(), em.dipoles()
+        acc = dict(
+            force_rms_rel=float(np.sqrt(np.mean((Fm - Ff) ** 2)) / np.sqrt(np.mean(Ff ** 2))),
+            dipole_rms_rel=float(np.linalg.norm(mum - muf) / np.linalg.norm(muf)),
+            energy_rel=float(abs(tm["u_tot"] - tf["u_tot"]) / abs(tf["u_tot"])),
+            stress_rel=float(np.linalg.norm(np.array(tm["tau_coul"]) - np.array(tf["tau_coul"]))
+                             / np.linalg.norm(np.array(tf["tau_coul"]))))
+        log(f"mixed vs fp64: {acc}")
+        ef.close()
+        em.close()
+    # roofline of the dominant kernel: the S(k) forward contraction of the
+    # SCF (dipoles of the polarisable ions), timed live with HIP events
+    head = "mixed" if "mixed" in forms else forms[0]
+    e = eng[head]
+    lo = 1 if head == "mixed" else 0
+    a_, k_, nk = e.ewald()
+    npol = e.counters()["npol"]
+    kern = {}
+    for which, name, ch in [(0, "sk_forward_scf", 3), (1, "sk_back_scf", 3), (2, "dir_induced", 0),
+                            (3, "sk_forward_force", 4), (4, "sk_back_force", 9)]:
+        try:
+            ms = e.time_kernel(which, lo=lo, iters=10)
+        except Exception as ex:  # noqa
+            log(f"kernel {name}: {ex}")
+            continue
+        nsel = npol if which in (0, 1, 2) else n
+        flops = 8.0 * nsel * nk * ch if ch else None
+        kern[name] = dict(ms=ms, flops=flops, tflops=(flops / (ms * 1e-3) / 1e12) if flops else None)
+    log(f"kernels ({head}): {json.dumps(kern)}")
+    fwd = kern.get("sk_forward_scf")
+    dtype_peak = PEAK["f32"] if lo else PEAK["f64"]
+    roof = None
+    if fwd:
+        ach = fwd["flops"] / (fwd["ms"] * 1e-3)
+        roof = dict(bound="mfma", achieved=ach / 1e12, peak=dtype_peak / 1e12, unit="TFLOP/s",
+                    frac=ach / dtype_peak, traffic=None, kernel="k_sk_fwd<%s,3>" % ("float" if lo else "double"),
+                    work=f"8 x {npol} polarisable ions x {nk} k-vectors x 3 dipole channels per launch")
+    cpu = None
+    if not args.no_cpu and "fp64" in res:
+        try:
+            cpu = cpu_baseline(p_text, L, r, typ, res["fp64"]["scf_fp64_per_step"] + 1, args.cpu_threads)
+        except Exception as ex:  # noqa
+            log(f"cpu baseline failed: {ex}")
+    head_res = res[head]
+    out = {
+        "metric": "MD steps/s, 12k-ion PIM melt: mixed vs FP64 GPU at force err<=1e-5",
+        "value": head_res["steps_per_s"],
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 / head_res["steps_per_s"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32/f64" if head == "mixed" else "f64",
+        "data": "synthetic (seeded SiO2-like PIM melt, rigid-ion relaxed)",
+        "config": {"workload": f"PIM multipole-Ewald NVT MD, N={n}, kmax=16 ({nk} k), alpha=0.3, rc=9 A, "
+                               f"skin 1 A, SCF tol 1e-8, NHC-3 2500 K, dt 1 fs",
+                   "form": head},
+        "ns_per_day": head_res["steps_per_s"] * 1.0 * 0.0864,
+        "forms": res,
+        "mixed_over_fp64": (res["mixed"]["steps_per_s"] / res["fp64"]["steps_per_s"])
+        if ("mixed" in res and "fp64" in res) else None,
+        "accuracy_mixed_vs_fp64": acc,
+        "kernels": kern,
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    for e in eng.values():
+        e.close()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

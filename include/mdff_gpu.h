@@ -131,6 +131,28 @@ int mdff_set_tuning(mdff_ctx *ctx, int lanes, int katoms, int ksplit);
  * [3] final Ewald force, [4] neighbour list, [5] total. */
 int mdff_get_timing(mdff_ctx *ctx, double t[6]);
 
+/* Time one kernel of the engine at the current state: `iters` launches
+ * back to back on the context stream between HIP events; ms = average per
+ * launch.  lo = 1 times the FP32 variant (mixed contexts only).  which:
+ *   0 S(k) forward, SCF form (dipoles of the polarisable ions, 3 channels)
+ *   1 back-projection, SCF form (field at the polarisable ions)
+ *   2 real-space induced dipole field
+ *   3 S(k) forward, force form (charges + dipoles, all ions)
+ *   4 back-projection, force form (field + field gradient, all ions)
+ *   5 short-range pair kernel
+ *   6 real-space Ewald force kernel
+ * No counterpart in the reference (measurement support). */
+int mdff_time_kernel(mdff_ctx *ctx, int which, int lo, int iters, double *ms);
+
+/* Counters since creation: [0] force calls, [1] FP64 SCF field
+ * evaluations, [2] FP32 correction sweeps (mixed form), [3] polarisable
+ * ions.  No counterpart in the reference (measurement support). */
+int mdff_get_counters(mdff_ctx *ctx, long long out[4]);
+
+/* Mixed form: FP32 Jacobi sweeps of the SCF correction per FP64 field
+ * evaluation (default 6). */
+int mdff_set_scf_inner(mdff_ctx *ctx, int inner);
+
 #ifdef __cplusplus
 }
 #endif

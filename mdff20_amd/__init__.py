@@ -78,6 +78,9 @@ def lib():
         "mdff_md_run": (C.c_int, [vp, C.c_int, C.c_int, P(Thermo)]),
         "mdff_set_tuning": (C.c_int, [vp, C.c_int, C.c_int, C.c_int]),
         "mdff_get_timing": (C.c_int, [vp, dp]),
+        "mdff_time_kernel": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, dp]),
+        "mdff_get_counters": (C.c_int, [vp, P(C.c_longlong)]),
+        "mdff_set_scf_inner": (C.c_int, [vp, C.c_int]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -272,6 +275,19 @@ class Engine:
         t = np.zeros(6)
         _check(self._L.mdff_get_timing(self._h, _dp(t)), "get_timing", False)
         return t
+
+    def time_kernel(self, which, lo=0, iters=10):
+        ms = C.c_double()
+        _check(self._L.mdff_time_kernel(self._h, which, lo, iters, C.byref(ms)), "time_kernel", False)
+        return ms.value
+
+    def counters(self):
+        c = (C.c_longlong * 4)()
+        _check(self._L.mdff_get_counters(self._h, c), "get_counters", False)
+        return dict(force_calls=c[0], scf_fp64=c[1], scf_inner=c[2], npol=c[3])
+
+    def set_scf_inner(self, k):
+        _check(self._L.mdff_set_scf_inner(self._h, k), "set_scf_inner", False)
 
     def set_tuning(self, lanes=0, katoms=0, ksplit=0):
         _check(self._L.mdff_set_tuning(self._h, lanes, katoms, ksplit), "set_tuning", False)

@@ -81,6 +81,7 @@ struct mdff_ctx {
     double *ke = nullptr;      // n
     double *upol_i = nullptr, *rmsd_i = nullptr;  // n
     double *tmp_q = nullptr, *tmp_mu = nullptr, *tmp_qraw = nullptr, *tmp_mur = nullptr;  // drop-in scratch
+    double *dc_R = nullptr, *dc_d = nullptr, *dc_g = nullptr;  // mixed SCF correction
     double *scal = nullptr;
     int *ctl = nullptr;
     TypeTab *tab_d = nullptr;
@@ -94,8 +95,10 @@ struct mdff_ctx {
     size_t nkf = 0;
     int nk = 0;
     double *kvec = nullptr, *Ak = nullptr, *kcoe = nullptr, *W = nullptr, *ek = nullptr;
-    void *tx = nullptr, *ty = nullptr, *tz = nullptr;     // all ions
-    void *pxx = nullptr, *pyy = nullptr, *pzz = nullptr;  // polarisable ions
+    // e^{i n theta} tables [lo][polar][axis]: lo = FP32 copy (mixed form),
+    // polar = polarisable-ion subset (SCF)
+    void *tab3[2][2][3] = {{{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}},
+                           {{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}}};
     double *kpart = nullptr, *bpart = nullptr;
     int nchunk = 0, nsplit = 0, chunk = 0, rows_per_split = 0;
     int nchunk_p = 0, chunk_p = 0;
@@ -114,10 +117,12 @@ struct mdff_ctx {
     int last_nscf = 0;
     double last_rmsd = 0;
     int scf_chunk = 4;
+    int inner_iters = 6;      // mixed form: FP32 Jacobi sweeps per FP64 refinement
+    long long n_force = 0, n_scf_fp64 = 0, n_scf_inner = 0;
 };
 
 // ------------------------------------------------------------------- helpers
-static size_t rsize(const mdff_ctx *c) { return c->precision == MDFF_MIXED ? sizeof(float) : sizeof(double); }
+static inline int lo_of(const mdff_ctx *c) { return c->precision == MDFF_MIXED ? 1 : 0; }
 
 static int sync_ctl(mdff_ctx *c)
 {
@@ -276,9 +281,9 @@ static void launch_dir_r(mdff_ctx *c, const DirArgs &a, int ef, int eq, int em, 
     }
 }
 
-static void launch_dir(mdff_ctx *c, const DirArgs &a, int ef, int eq, int em, int f)
+static void launch_dir(mdff_ctx *c, const DirArgs &a, int lo, int ef, int eq, int em, int f)
 {
-    if (c->precision == MDFF_MIXED) launch_dir_r<float>(c, a, ef, eq, em, f);
+    if (lo) launch_dir_r<float>(c, a, ef, eq, em, f);
     else launch_dir_r<double>(c, a, ef, eq, em, f);
 }
 
@@ -316,15 +321,13 @@ static void launch_eikr_t(mdff_ctx *c, const int *idx, int nsel, void *tx, void 
                        c->r, c->r + c->n, c->r + 2 * c->n, idx, nsel, (real *)tx, (real *)ty, (real *)tz, active);
 }
 
-static void launch_eikr(mdff_ctx *c, int polar, const int *active)
+static void launch_eikr(mdff_ctx *c, int polar, int lo, const int *active)
 {
-    if (polar) {
-        if (c->precision == MDFF_MIXED) launch_eikr_t<float>(c, c->pidx, c->npol, c->pxx, c->pyy, c->pzz, active);
-        else launch_eikr_t<double>(c, c->pidx, c->npol, c->pxx, c->pyy, c->pzz, active);
-    } else {
-        if (c->precision == MDFF_MIXED) launch_eikr_t<float>(c, nullptr, c->n, c->tx, c->ty, c->tz, active);
-        else launch_eikr_t<double>(c, nullptr, c->n, c->tx, c->ty, c->tz, active);
-    }
+    void **t = c->tab3[lo][polar];
+    const int *idx = polar ? c->pidx : nullptr;
+    int nsel = polar ? c->npol : c->n;
+    if (lo) launch_eikr_t<float>(c, idx, nsel, t[0], t[1], t[2], active);
+    else launch_eikr_t<double>(c, idx, nsel, t[0], t[1], t[2], active);
 }
 
 template <typename real, int C>
@@ -347,7 +350,7 @@ static void launch_back_t(mdff_ctx *c, const BackArgs &a, int nsel)
 // reciprocal part of multipole_ES (ewald.c:876-1051) on the selected ions.
 // polar: use the polarisable-ion tables (SCF); C: channels of S(k);
 // NCH: 3 (field) or 9 (field + gradient); do_ef/do_f outputs; efg optional.
-static void rec_part(mdff_ctx *c, int polar, int C, int NCH, const double *q_w, const double *mu_w,
+static void rec_part(mdff_ctx *c, int polar, int lo, int C, int NCH, const double *q_w, const double *mu_w,
                      const double *q_f, const double *mu_f, int do_ef, int do_f, int do_stress, double *efg_out,
                      const int *active)
 {
@@ -357,15 +360,15 @@ static void rec_part(mdff_ctx *c, int polar, int C, int NCH, const double *q_w, 
     SkArgs a;
     a.nsel = nsel;
     a.Kx = c->Kx; a.Ky = c->Ky; a.Kz = c->Kz; a.P = c->P;
-    a.tx = polar ? c->pxx : c->tx;
-    a.ty = polar ? c->pyy : c->ty;
-    a.tz = polar ? c->pzz : c->tz;
+    a.tx = c->tab3[lo][polar][0];
+    a.ty = c->tab3[lo][polar][1];
+    a.tz = c->tab3[lo][polar][2];
     a.idx = polar ? c->pidx : nullptr;
     a.q = q_w; a.mu = mu_w; a.n = c->n;
     a.chunk = chunk;
     a.part = c->kpart;
     a.active = active;
-    bool mixed = c->precision == MDFF_MIXED;
+    bool mixed = lo != 0;
     if (C == 1) { if (mixed) launch_fwd_t<float, 1>(c, a, nchunk); else launch_fwd_t<double, 1>(c, a, nchunk); }
     else if (C == 3) { if (mixed) launch_fwd_t<float, 3>(c, a, nchunk); else launch_fwd_t<double, 3>(c, a, nchunk); }
     else { if (mixed) launch_fwd_t<float, 4>(c, a, nchunk); else launch_fwd_t<double, 4>(c, a, nchunk); }
@@ -403,7 +406,7 @@ static void short_range(mdff_ctx *c)
     a.T = c->tab_d; a.rc2 = c->srcutsq; a.f = c->f_sr; a.acc = c->acc_sr;
     int L = c->lanes;
     unsigned blocks = cdiv((size_t)n * L, 256);
-    bool mixed = c->precision == MDFF_MIXED;
+    bool mixed = lo_of(c);
     if (L == 16) {
         if (mixed) hipLaunchKernelGGL((k_short<float, 16>), dim3(blocks), dim3(256), 0, c->st, a);
         else hipLaunchKernelGGL((k_short<double, 16>), dim3(blocks), dim3(256), 0, c->st, a);
@@ -419,26 +422,20 @@ static void short_range(mdff_ctx *c)
 
 // static field of momentpolaSCF pim.c:216-218: charges (+ static dipoles)
 // with damping, field only
-static void static_field(mdff_ctx *c)
+static void static_field(mdff_ctx *c, int lo)
 {
     int n = c->n;
     DirArgs a = dir_args(c, c->q, c->mu_static, c->ldmp, 0, nullptr);
-    launch_dir(c, a, 1, 0, 0, 0);
+    launch_dir(c, a, lo, 1, 0, 0, 0);
     int C = c->ldip_static ? 4 : 1;
-    rec_part(c, 0, C, 3, c->q, c->mu_static, c->q, c->mu_static, 1, 0, 0, nullptr, nullptr);
+    rec_part(c, 0, lo, C, 3, c->q, c->mu_static, c->q, c->mu_static, 1, 0, 0, nullptr, nullptr);
     hipLaunchKernelGGL(k_field_sum, dim3(cdiv(3 * n, 256)), dim3(256), 0, c->st, n, c->selfa2, c->mu_static,
                        c->ef_dir, c->ef_rec, (const double *)nullptr, c->ef_stat, (const int *)nullptr);
 }
 
-// the SCF loop of momentpolaSCF pim.c:236-278 on the device, gated by the
-// convergence flag; host polls the flag every scf_chunk iterations.
-static int scf_loop(mdff_ctx *c, int istep)
+// ASPC order and coefficients of extrapolate_dipole_aspc pim.c:324-379
+static int aspc_args(mdff_ctx *c, int istep, AspcArgs *asp)
 {
-    int n = c->n;
-    const int *act = c->ctl + C_SCF_ACTIVE;
-    int maxit = c->p.max_scf_pim_iter, minit = c->p.min_scf_pim_iter;
-    hipLaunchKernelGGL(k_scf_begin, dim3(1), dim3(1), 0, c->st, c->ctl, (maxit > 0 || minit > 0) ? 1 : 0);
-    // ASPC order and coefficients (pim.c:324-379)
     int order = c->p.extrapolate_order;
     int eo = (istep >= order + 1) ? order : (istep ? istep - 1 : 0);
     double B[6] = {0, 0, 0, 0, 0, 0}, Wc = 0.0;
@@ -454,10 +451,46 @@ static int scf_loop(mdff_ctx *c, int istep)
         break;
     default: return MDFF_ERR_CONFIG;
     }
-    int shift = (eo != order) ? eo + 1 : eo;
+    asp->n = c->n;
+    asp->eo = eo;
+    asp->shift = (eo != order) ? eo + 1 : eo;
+    asp->W = Wc;
+    for (int k = 0; k < 6; k++) asp->B[k] = B[k];
+    return MDFF_OK;
+}
+
+// induced field of dipoles mu (no charges, no damping, pim.c:258-260) at
+// the polarisable ions: ef = base + self + dir + rec
+static void induced_field(mdff_ctx *c, int lo, const double *mu, const double *base, double *ef, const int *act)
+{
+    int n = c->n;
+    DirArgs a = dir_args(c, c->qz, mu, 0, 1, act);
+    launch_dir(c, a, lo, 1, 0, 0, 0);
+    rec_part(c, 1, lo, 3, 3, c->qz, mu, c->qz, mu, 1, 0, 0, nullptr, act);
+    hipLaunchKernelGGL(k_field_sum, dim3(cdiv(3 * n, 256)), dim3(256), 0, c->st, n, c->selfa2, mu, c->ef_dir,
+                       c->ef_rec, base, ef, act);
+}
+
+static void scf_finish(mdff_ctx *c)
+{
+    int n = c->n;
+    // dipia_ind[ia][0] = mu_ind (pim.c:281-285)
+    hipLaunchKernelGGL(k_store_hist, dim3(cdiv(3 * n, 256)), dim3(256), 0, c->st, n, c->mu_ind, c->hist);
+    c->last_nscf = c->ctl_h[C_ISCF];
+    c->last_rmsd = sqrt(c->scal_h[S_RMSD2] / (double)(c->npol > 0 ? c->npol : 1));
+}
+
+// the SCF loop of momentpolaSCF pim.c:236-278 on the device (FP64 form,
+// the reference's Jacobi schedule), gated by the convergence flag; the host
+// polls the flag every scf_chunk iterations.
+static int scf_loop(mdff_ctx *c, int istep)
+{
+    int n = c->n;
+    const int *act = c->ctl + C_SCF_ACTIVE;
+    int maxit = c->p.max_scf_pim_iter, minit = c->p.min_scf_pim_iter;
+    hipLaunchKernelGGL(k_scf_begin, dim3(1), dim3(1), 0, c->st, c->ctl, (maxit > 0 || minit > 0) ? 1 : 0);
     AspcArgs asp;
-    asp.n = n; asp.eo = eo; asp.shift = shift; asp.W = Wc;
-    for (int k = 0; k < 6; k++) asp.B[k] = B[k];
+    if (aspc_args(c, istep, &asp)) return MDFF_ERR_CONFIG;
     int launched = 0;
     int limit = std::max(maxit, minit) + 1;
     while (launched < limit) {
@@ -467,12 +500,7 @@ static int scf_loop(mdff_ctx *c, int istep)
             hipLaunchKernelGGL(k_scf_mu, dim3(cdiv(n, 256)), dim3(256), 0, c->st, n, first, asp, c->type, c->tab_d,
                                c->hist, c->ef, c->mu_ind, c->upol_i, act);
             colsum(c, c->upol_i, n, n, 1, c->scal + S_UPOL, act);
-            // induced field: dipoles only, no damping (pim.c:258-260)
-            DirArgs a = dir_args(c, c->qz, c->mu_ind, 0, 1, act);
-            launch_dir(c, a, 1, 0, 0, 0);
-            rec_part(c, 1, 3, 3, c->qz, c->mu_ind, c->qz, c->mu_ind, 1, 0, 0, nullptr, act);
-            hipLaunchKernelGGL(k_field_sum, dim3(cdiv(3 * n, 256)), dim3(256), 0, c->st, n, c->selfa2, c->mu_ind,
-                               c->ef_dir, c->ef_rec, c->ef_stat, c->ef, act);
+            induced_field(c, 0, c->mu_ind, c->ef_stat, c->ef, act);
             if (first)
                 hipLaunchKernelGGL(k_scf_correct, dim3(cdiv(n, 256)), dim3(256), 0, c->st, n, asp, c->type,
                                    c->tab_d, c->ef, c->mu_ind, act);
@@ -486,25 +514,76 @@ static int scf_loop(mdff_ctx *c, int istep)
         if (rc) return rc;
         if (!c->ctl_h[C_SCF_ACTIVE]) break;
     }
-    // dipia_ind[ia][0] = mu_ind (pim.c:281-285)
-    hipLaunchKernelGGL(k_store_hist, dim3(cdiv(3 * n, 256)), dim3(256), 0, c->st, n, c->mu_ind, c->hist);
-    c->last_nscf = c->ctl_h[C_ISCF];
-    c->last_rmsd = sqrt(c->scal_h[S_RMSD2] / (double)(c->npol > 0 ? c->npol : 1));
+    scf_finish(c);
+    c->n_scf_fp64 += c->last_nscf;
+    if (c->last_rmsd > c->p.conv_tol_ind) return MDFF_ERR_SCF;
+    return MDFF_OK;
+}
+
+// Mixed form of the SCF: defect correction.  Each outer iteration evaluates
+// the field E = E_stat + T mu of the current dipoles in FP64 and the
+// residual rmsd exactly as get_rmsd_mu (pim.c:431-449); while it is above
+// conv_tol_ind, the Jacobi correction d = R + pol^T (T d) with
+// R = pol^T E - mu is iterated inner_iters times with the FP32 operator and
+// added to mu.  The fixed point is the FP64 one; FP32 only limits how much
+// each outer step gains.  The start is the ASPC predictor (pim.c:382-400).
+static int scf_loop_mixed(mdff_ctx *c, int istep)
+{
+    int n = c->n;
+    const int *act = c->ctl + C_SCF_ACTIVE;
+    int maxit = c->p.max_scf_pim_iter, minit = c->p.min_scf_pim_iter;
+    hipLaunchKernelGGL(k_scf_begin, dim3(1), dim3(1), 0, c->st, c->ctl, (maxit > 0 || minit > 0) ? 1 : 0);
+    AspcArgs asp;
+    if (aspc_args(c, istep, &asp)) return MDFF_ERR_CONFIG;
+    hipLaunchKernelGGL(k_scf_mu, dim3(cdiv(n, 256)), dim3(256), 0, c->st, n, 1, asp, c->type, c->tab_d, c->hist, c->ef,
+                       c->mu_ind, c->upol_i, act);
+    int launched = 0;
+    int limit = std::max(maxit, minit) + 1;
+    while (launched < limit) {
+        int chunk = launched == 0 ? std::max(1, std::min(minit, 2)) : 2;
+        for (int it = 0; it < chunk && launched < limit; it++, launched++) {
+            induced_field(c, 0, c->mu_ind, c->ef_stat, c->ef, act);
+            hipLaunchKernelGGL(k_scf_rmsd, dim3(cdiv(n, 256)), dim3(256), 0, c->st, n, c->type, c->tab_d, c->mu_ind,
+                               c->ef, c->rmsd_i, act);
+            colsum(c, c->rmsd_i, n, n, 1, c->scal + S_RMSD2, act);
+            hipLaunchKernelGGL(k_scf_control, dim3(1), dim3(1), 0, c->st, c->ctl, c->scal, c->npol, maxit, minit,
+                               c->p.conv_tol_ind);
+            // correction (skipped once converged)
+            hipLaunchKernelGGL(k_dc_residual, dim3(cdiv(n, 256)), dim3(256), 0, c->st, n, c->type, c->tab_d, c->ef,
+                               c->mu_ind, c->dc_R, c->dc_d, act);
+            for (int k = 0; k < c->inner_iters; k++) {
+                induced_field(c, 1, c->dc_d, nullptr, c->dc_g, act);
+                hipLaunchKernelGGL(k_dc_update, dim3(cdiv(n, 256)), dim3(256), 0, c->st, n, c->type, c->tab_d,
+                                   c->dc_R, c->dc_g, c->dc_d, act);
+            }
+            hipLaunchKernelGGL(k_axpy_gated, dim3(cdiv(3 * n, 256)), dim3(256), 0, c->st, 3 * n, c->dc_d, c->mu_ind,
+                               act);
+        }
+        int rc = sync_scal(c);
+        if (rc) return rc;
+        if (!c->ctl_h[C_SCF_ACTIVE]) break;
+    }
+    // u_pol of the dipoles whose field was evaluated last (pim.c:253,299)
+    hipLaunchKernelGGL(k_upol, dim3(cdiv(n, 256)), dim3(256), 0, c->st, n, c->type, c->tab_d, c->mu_ind, c->upol_i);
+    colsum(c, c->upol_i, n, n, 1, c->scal + S_UPOL);
+    scf_finish(c);
+    c->n_scf_fp64 += c->last_nscf;
+    c->n_scf_inner += (long long)c->inner_iters * std::max(0, c->last_nscf - 1);
     if (c->last_rmsd > c->p.conv_tol_ind) return MDFF_ERR_SCF;
     return MDFF_OK;
 }
 
 // final multipole_ES of engforce (field.c:381-383): forces and stress
-static void coulomb_force(mdff_ctx *c)
+static void coulomb_force(mdff_ctx *c, int lo)
 {
     int n = c->n;
     int ld = c->ldip_sticky || c->ldip_static;
     DirArgs a = dir_args(c, c->q, c->mu_tot, c->ldmp, 0, nullptr);
-    launch_dir(c, a, 0, 0, 0, 1);
+    launch_dir(c, a, lo, 0, 0, 0, 1);
     colsum(c, c->uw_dir, n, n, 7, c->scal + S_U_DIR);
     int C = ld ? 4 : 1;
     int NCH = ld ? 9 : 3;
-    rec_part(c, 0, C, NCH, c->q, c->mu_tot, c->q, c->mu_tot, 0, 1, 1, nullptr, nullptr);
+    rec_part(c, 0, lo, C, NCH, c->q, c->mu_tot, c->q, c->mu_tot, 0, 1, 1, nullptr, nullptr);
     colsum(c, c->ek, (int)c->nkf, (int)c->nkf, 7, c->scal + S_U_REC);
     hipLaunchKernelGGL(k_self_terms, dim3(cdiv(n, 256)), dim3(256), 0, c->st, n, c->q, c->mu_tot, c->selfbuf);
     colsum(c, c->selfbuf, n, n, 2, c->scal + S_QSQ);
@@ -522,13 +601,17 @@ static int engforce_impl(mdff_ctx *c, int istep)
     c->last_nscf = 0;
     if (c->p.lcoulombic) {
         // k-space tables at the current positions
-        launch_eikr(c, 0, nullptr);
-        if (c->lpol && c->npol > 0) launch_eikr(c, 1, nullptr);
+        int lo = lo_of(c);
+        launch_eikr(c, 0, lo, nullptr);
+        if (c->lpol && c->npol > 0) {
+            launch_eikr(c, 1, 0, nullptr);
+            if (lo) launch_eikr(c, 1, 1, nullptr);
+        }
         // get_dipoles coulombic.c:67-130
         if (c->lpol) {
-            static_field(c);
+            static_field(c, lo);
             if (c->have_events) HIPCHK(hipEventRecord(c->ev[2], c->st));
-            rc = scf_loop(c, istep);
+            rc = lo ? scf_loop_mixed(c, istep) : scf_loop(c, istep);
             if (rc != MDFF_OK && rc != MDFF_ERR_SCF) return rc;
             c->ldip_sticky = 1;
             hipLaunchKernelGGL(k_axpy3, dim3(cdiv(3 * n, 256)), dim3(256), 0, c->st, 3 * n, c->mu_static, c->mu_ind,
@@ -539,7 +622,7 @@ static int engforce_impl(mdff_ctx *c, int istep)
             hipLaunchKernelGGL(k_zero, dim3(1), dim3(1), 0, c->st, c->scal + S_UPOL, (size_t)1);
         }
         if (c->have_events) HIPCHK(hipEventRecord(c->ev[3], c->st));
-        coulomb_force(c);
+        coulomb_force(c, lo);
     } else {
         if (c->have_events) {
             HIPCHK(hipEventRecord(c->ev[2], c->st));
@@ -548,6 +631,7 @@ static int engforce_impl(mdff_ctx *c, int istep)
     }
     hipLaunchKernelGGL(k_assemble_force, dim3(cdiv(3 * n, 256)), dim3(256), 0, c->st, 3 * n, c->f_sr, c->f_dir,
                        c->f_rec, c->p.lcoulombic ? c->u.coul_unit : 0.0, c->f, c->ctl);
+    c->n_force++;
     if (c->have_events) HIPCHK(hipEventRecord(c->ev[4], c->st));
     return rc;
 }
@@ -737,7 +821,7 @@ void mdff_destroy(mdff_ctx *c)
     double **dp[] = {&c->r, &c->v, &c->f, &c->fold, &c->rs, &c->s, &c->s0, &c->invmass, &c->massd, &c->q, &c->qz,
                      &c->mu_static, &c->mu_ind, &c->mu_tot, &c->hist, &c->ef_stat, &c->ef, &c->ef_dir, &c->ef_rec,
                      &c->efg, &c->efg_dir, &c->efg_rec, &c->f_sr, &c->f_dir, &c->f_rec, &c->acc_sr, &c->uw_dir,
-                     &c->selfbuf, &c->ke, &c->upol_i, &c->rmsd_i, &c->tmp_q, &c->tmp_mu, &c->tmp_qraw, &c->tmp_mur, &c->scal,
+                     &c->selfbuf, &c->ke, &c->upol_i, &c->rmsd_i, &c->tmp_q, &c->tmp_mu, &c->tmp_qraw, &c->tmp_mur, &c->dc_R, &c->dc_d, &c->dc_g, &c->scal,
                      &c->kvec, &c->Ak, &c->kcoe, &c->W, &c->ek, &c->kpart, &c->bpart};
     for (auto pp : dp) dfree(*pp);
     dfree(c->type);
@@ -746,12 +830,10 @@ void mdff_destroy(mdff_ctx *c)
     dfree(c->pidx);
     dfree(c->nbr);
     dfree(c->cnt);
-    if (c->tx) (void)hipFree(c->tx);
-    if (c->ty) (void)hipFree(c->ty);
-    if (c->tz) (void)hipFree(c->tz);
-    if (c->pxx) (void)hipFree(c->pxx);
-    if (c->pyy) (void)hipFree(c->pyy);
-    if (c->pzz) (void)hipFree(c->pzz);
+    for (int a = 0; a < 2; a++)
+        for (int b = 0; b < 2; b++)
+            for (int d = 0; d < 3; d++)
+                if (c->tab3[a][b][d]) (void)hipFree(c->tab3[a][b][d]);
     if (c->ctl_h) (void)hipHostFree(c->ctl_h);
     if (c->scal_h) (void)hipHostFree(c->scal_h);
     if (c->have_events)
@@ -868,6 +950,7 @@ int mdff_create(mdff_ctx **out, const mdff_params *p, int nion, const int *typia
     ALLOC(c->f_sr, 3 * n); ALLOC(c->f_dir, 3 * n); ALLOC(c->f_rec, 3 * n); ALLOC(c->acc_sr, 14 * n);
     ALLOC(c->uw_dir, 7 * n); ALLOC(c->selfbuf, 2 * n); ALLOC(c->ke, n); ALLOC(c->upol_i, n); ALLOC(c->rmsd_i, n);
     ALLOC(c->tmp_q, n); ALLOC(c->tmp_mu, 3 * n); ALLOC(c->tmp_qraw, n); ALLOC(c->tmp_mur, 3 * n);
+    ALLOC(c->dc_R, 3 * n); ALLOC(c->dc_d, 3 * n); ALLOC(c->dc_g, 3 * n);
     ALLOC(c->scal, S_COUNT); ALLOC(c->ctl, C_COUNT); ALLOC(c->tab_d, 1);
     if (hipHostMalloc((void **)&c->ctl_h, sizeof(int) * C_COUNT) != hipSuccess) { mdff_destroy(c); return MDFF_ERR_NOMEM; }
     if (hipHostMalloc((void **)&c->scal_h, sizeof(double) * S_COUNT) != hipSuccess) { mdff_destroy(c); return MDFF_ERR_NOMEM; }
@@ -944,19 +1027,18 @@ int mdff_create(mdff_ctx **out, const mdff_params *p, int nion, const int *typia
             mdff_destroy(c);
             return MDFF_ERR_DEVICE;
         }
-        size_t rs = rsize(c);
-        size_t nxr = (size_t)(c->Kx + 1), nyr = (size_t)(2 * c->Ky + 1), nzr = (size_t)(2 * c->Kz + 1);
-        if (hipMalloc(&c->tx, 2 * nxr * n * rs) != hipSuccess || hipMalloc(&c->ty, 2 * nyr * n * rs) != hipSuccess ||
-            hipMalloc(&c->tz, 2 * nzr * n * rs) != hipSuccess) {
-            mdff_destroy(c);
-            return MDFF_ERR_NOMEM;
-        }
-        if (c->npol > 0) {
-            size_t np_ = (size_t)c->npol;
-            if (hipMalloc(&c->pxx, 2 * nxr * np_ * rs) != hipSuccess || hipMalloc(&c->pyy, 2 * nyr * np_ * rs) != hipSuccess ||
-                hipMalloc(&c->pzz, 2 * nzr * np_ * rs) != hipSuccess) {
-                mdff_destroy(c);
-                return MDFF_ERR_NOMEM;
+        size_t dims[3] = {(size_t)(c->Kx + 1), (size_t)(2 * c->Ky + 1), (size_t)(2 * c->Kz + 1)};
+        for (int lo = 0; lo < 2; lo++) {
+            if (lo && !lo_of(c)) continue;
+            size_t rs = lo ? sizeof(float) : sizeof(double);
+            for (int polar = 0; polar < 2; polar++) {
+                size_t ns = polar ? (size_t)c->npol : (size_t)n;
+                if (ns == 0) continue;
+                for (int d = 0; d < 3; d++)
+                    if (hipMalloc(&c->tab3[lo][polar][d], 2 * dims[d] * ns * rs) != hipSuccess) {
+                        mdff_destroy(c);
+                        return MDFF_ERR_NOMEM;
+                    }
             }
         }
         // split-K over ions for the forward sum, split over (nx,ny) rows for
@@ -1151,10 +1233,12 @@ int mdff_get_dipoles_scf(mdff_ctx *c, int istep, double *mu, double *upol)
     int n = c->n, rc = MDFF_OK;
     hipLaunchKernelGGL(k_frac, dim3(cdiv(n, 256)), dim3(256), 0, c->st, n, c->cd, c->r, c->r + n, c->r + 2 * n, c->s);
     if (c->lpol) {
-        launch_eikr(c, 0, nullptr);
-        launch_eikr(c, 1, nullptr);
-        static_field(c);
-        rc = scf_loop(c, istep);
+        int lo = lo_of(c);
+        launch_eikr(c, 0, lo, nullptr);
+        launch_eikr(c, 1, 0, nullptr);
+        if (lo) launch_eikr(c, 1, 1, nullptr);
+        static_field(c, lo);
+        rc = lo ? scf_loop_mixed(c, istep) : scf_loop(c, istep);
         if (rc && rc != MDFF_ERR_SCF) return rc;
         c->ldip_sticky = 1;
         hipLaunchKernelGGL(k_axpy3, dim3(cdiv(3 * n, 256)), dim3(256), 0, c->st, 3 * n, c->mu_static, c->mu_ind, c->mu_tot);
@@ -1209,14 +1293,14 @@ int mdff_multipole_es(mdff_ctx *c, const double *qh, const double *muh, const do
     if (do_dir) {
         DirArgs a = dir_args(c, d_qm, d_mum, ldamp ? 1 : 0, 0, nullptr);
         a.efg = c->efg_dir;
-        launch_dir(c, a, do_ef, do_efg, do_efg || do_f, do_f);
+        launch_dir(c, a, 0, do_ef, do_efg, do_efg || do_f, do_f);
         colsum(c, c->uw_dir, n, n, 7, c->scal + S_U_DIR);
     }
     if (do_rec) {
-        launch_eikr(c, 0, nullptr);
+        launch_eikr(c, 0, 0, nullptr);
         int C = ld ? 4 : 1;
         int NCH = (do_ef || (do_f && ld)) ? 9 : 3;
-        rec_part(c, 0, C, NCH, d_qm, d_mum, d_qr, d_mum, do_ef, do_f, do_s, c->efg_rec, nullptr);
+        rec_part(c, 0, 0, C, NCH, d_qm, d_mum, d_qr, d_mum, do_ef, do_f, do_s, c->efg_rec, nullptr);
         colsum(c, c->ek, (int)c->nkf, (int)c->nkf, 7, c->scal + S_U_REC);
     }
     hipLaunchKernelGGL(k_self_terms, dim3(cdiv(n, 256)), dim3(256), 0, c->st, n, d_qr, d_mur, c->selfbuf);
@@ -1306,6 +1390,75 @@ int mdff_md_run(mdff_ctx *c, int first_step, int nsteps, mdff_thermo *th)
     int rc = finish_call(c, worst, th);
     if (th) th->istep = first_step + nsteps - 1;
     return rc;
+}
+
+int mdff_time_kernel(mdff_ctx *c, int which, int lo, int iters, double *ms)
+{
+    if (!c || !ms || iters < 1 || !c->p.lcoulombic) return MDFF_ERR_ARG;
+    if (lo && !lo_of(c)) return MDFF_ERR_ARG;
+    int n = c->n;
+    int polar = (which == 0 || which == 1 || which == 2);
+    if (polar && c->npol == 0) return MDFF_ERR_ARG;
+    // state for the kernels: tables, a dipole field and W from one full pass
+    hipLaunchKernelGGL(k_frac, dim3(cdiv(n, 256)), dim3(256), 0, c->st, n, c->cd, c->r, c->r + n, c->r + 2 * n, c->s);
+    launch_eikr(c, polar, lo, nullptr);
+    const double *mu = polar ? c->mu_ind : c->mu_tot;
+    int C = polar ? 3 : 4, NCH = polar ? 3 : 9;
+    rec_part(c, polar, lo, C, NCH, polar ? c->qz : c->q, mu, polar ? c->qz : c->q, mu, 1, !polar, 0, nullptr, nullptr);
+    int nsel = polar ? c->npol : n;
+    SkArgs a;
+    a.nsel = nsel; a.Kx = c->Kx; a.Ky = c->Ky; a.Kz = c->Kz; a.P = c->P;
+    a.tx = c->tab3[lo][polar][0]; a.ty = c->tab3[lo][polar][1]; a.tz = c->tab3[lo][polar][2];
+    a.idx = polar ? c->pidx : nullptr; a.q = polar ? c->qz : c->q; a.mu = mu; a.n = n;
+    a.chunk = polar ? c->chunk_p : c->chunk; a.part = c->kpart; a.active = nullptr;
+    int nchunk = polar ? c->nchunk_p : c->nchunk;
+    BackArgs b;
+    b.nsel = nsel; b.Kx = c->Kx; b.Ky = c->Ky; b.Kz = c->Kz; b.P = c->P;
+    b.tx = a.tx; b.ty = a.ty; b.tz = a.tz; b.W = c->W; b.rows_per_split = c->rows_per_split; b.part = c->bpart;
+    b.active = nullptr;
+    DirArgs d = dir_args(c, polar ? c->qz : c->q, mu, polar ? 0 : c->ldmp, polar ? 1 : 0, nullptr);
+    auto once = [&]() {
+        switch (which) {
+        case 0: case 3:
+            if (lo) { if (C == 3) launch_fwd_t<float, 3>(c, a, nchunk); else launch_fwd_t<float, 4>(c, a, nchunk); }
+            else { if (C == 3) launch_fwd_t<double, 3>(c, a, nchunk); else launch_fwd_t<double, 4>(c, a, nchunk); }
+            break;
+        case 1: case 4:
+            if (lo) { if (NCH == 3) launch_back_t<float, 3>(c, b, nsel); else launch_back_t<float, 9>(c, b, nsel); }
+            else { if (NCH == 3) launch_back_t<double, 3>(c, b, nsel); else launch_back_t<double, 9>(c, b, nsel); }
+            break;
+        case 2: launch_dir(c, d, lo, 1, 0, 0, 0); break;
+        case 5: short_range(c); break;
+        case 6: launch_dir(c, d, lo, 0, 0, 0, 1); break;
+        default: break;
+        }
+    };
+    once();
+    HIPCHK(hipEventRecord(c->ev[6], c->st));
+    for (int k = 0; k < iters; k++) once();
+    HIPCHK(hipEventRecord(c->ev[7], c->st));
+    HIPCHK(hipEventSynchronize(c->ev[7]));
+    float t = 0.0f;
+    HIPCHK(hipEventElapsedTime(&t, c->ev[6], c->ev[7]));
+    *ms = (double)t / iters;
+    return MDFF_OK;
+}
+
+int mdff_get_counters(mdff_ctx *c, long long out[4])
+{
+    if (!c || !out) return MDFF_ERR_ARG;
+    out[0] = c->n_force;
+    out[1] = c->n_scf_fp64;
+    out[2] = c->n_scf_inner;
+    out[3] = c->npol;
+    return MDFF_OK;
+}
+
+int mdff_set_scf_inner(mdff_ctx *c, int inner)
+{
+    if (!c || inner < 0 || inner > 64) return MDFF_ERR_ARG;
+    c->inner_iters = inner;
+    return MDFF_OK;
 }
 
 int mdff_get_timing(mdff_ctx *c, double t[6])

@@ -302,3 +302,66 @@ __global__ void k_sub_mean(int n, const double *sums, double *v)
     double on = 1.0 / (double)n;
     v[i] -= sums[a] * on;
 }
+
+// mixed-form SCF correction: R = pol^T E - mu, d = R on polarisable ions
+__global__ void k_dc_residual(int n, const int *__restrict__ type, const TypeTab *__restrict__ T,
+                              const double *__restrict__ ef, const double *__restrict__ mu, double *__restrict__ R,
+                              double *__restrict__ d, const int *active)
+{
+    if (!*active) return;
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int t = type[i];
+    for (int a = 0; a < 3; a++) {
+        double v = 0.0;
+        if (T->polar[t]) {
+            const double *P = T->pol[t];
+            v = P[0 * 3 + a] * ef[i] + P[1 * 3 + a] * ef[n + i] + P[2 * 3 + a] * ef[2 * n + i] - mu[(size_t)a * n + i];
+        }
+        R[(size_t)a * n + i] = v;
+        d[(size_t)a * n + i] = v;
+    }
+}
+
+// d = R + pol^T g (one Jacobi sweep of the correction equation)
+__global__ void k_dc_update(int n, const int *__restrict__ type, const TypeTab *__restrict__ T,
+                            const double *__restrict__ R, const double *__restrict__ g, double *__restrict__ d,
+                            const int *active)
+{
+    if (!*active) return;
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int t = type[i];
+    for (int a = 0; a < 3; a++) {
+        double v = 0.0;
+        if (T->polar[t]) {
+            const double *P = T->pol[t];
+            v = R[(size_t)a * n + i] + P[0 * 3 + a] * g[i] + P[1 * 3 + a] * g[n + i] + P[2 * 3 + a] * g[2 * n + i];
+        }
+        d[(size_t)a * n + i] = v;
+    }
+}
+
+__global__ void k_axpy_gated(int m, const double *x, double *y, const int *active)
+{
+    if (!*active) return;
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < m) y[i] += x[i];
+}
+
+// get_upol pim.c:145-161 per ion
+__global__ void k_upol(int n, const int *__restrict__ type, const TypeTab *__restrict__ T,
+                       const double *__restrict__ mu, double *__restrict__ out)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int t = type[i];
+    double up = 0.0;
+    if (T->polar[t]) {
+        double m[3] = {mu[i], mu[n + i], mu[2 * n + i]};
+        const double *I = T->invpol[t];
+        for (int a = 0; a < 3; a++)
+            for (int b = 0; b < 3; b++) up += m[a] * I[3 * a + b] * m[b];
+    }
+    out[i] = 0.5 * up;
+}
