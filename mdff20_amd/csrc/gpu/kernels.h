@@ -227,7 +227,7 @@ struct DirArgs {
     double *ef;           // 3n (written when DO_EF)
     double *efg;          // 9n (written when EFG_Q||EFG_MM)
     double *f;            // 3n force, coulombic units excluded (written when DO_F)
-    double *uw;           // 7n: energy halves then 6 virial halves (DO_F)
+    double *uw;           // 7n: energy halves (always) then 6 virial halves (DO_F)
     int q_zero;           // all charges zero: ions without dipole get nothing
 };
 
@@ -385,9 +385,9 @@ __global__ void __launch_bounds__(256) k_ewald_dir(DirArgs a)
 #pragma unroll
         for (int k = 0; k < 9; k++) g[k] = group_sum<LANES>(g[k]);
     }
+    uu = group_sum<LANES>(uu);
     if (DO_F) {
         fx = group_sum<LANES>(fx); fy = group_sum<LANES>(fy); fz = group_sum<LANES>(fz);
-        uu = group_sum<LANES>(uu);
 #pragma unroll
         for (int k = 0; k < 6; k++) w6[k] = group_sum<LANES>(w6[k]);
     }
@@ -397,9 +397,10 @@ __global__ void __launch_bounds__(256) k_ewald_dir(DirArgs a)
 #pragma unroll
             for (int k = 0; k < 9; k++) a.efg[(size_t)k * n + i] = g[k];
         }
+        // the energy is formed in every call, as multipole_ES_dir does
+        a.uw[i] = uu;
         if (DO_F) {
             a.f[i] = fx; a.f[n + i] = fy; a.f[2 * n + i] = fz;
-            a.uw[i] = uu;
 #pragma unroll
             for (int k = 0; k < 6; k++) a.uw[(size_t)(1 + k) * n + i] = w6[k];
         }

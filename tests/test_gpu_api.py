@@ -1,0 +1,86 @@
+"""C-ABI entry points beyond engforce/MD: multipole_ES with its flag
+combinations, get_dipoles, the short-range pair part, and the mixed form's
+accuracy against the FP64 form."""
+import numpy as np
+import pytest
+
+import mdff20_amd as m
+from conftest import load_case, rel
+
+pytestmark = pytest.mark.gpu
+Oracle = pytest.importorskip("oracle.oracle").Oracle
+ES = m.ES
+
+COMBOS = [
+    ES["LQ"] | ES["EF"] | ES["DIR"] | ES["REC"] | ES["LDAMP"],
+    ES["LD"] | ES["EF"] | ES["DIR"] | ES["REC"],
+    ES["LQ"] | ES["LD"] | ES["LDAMP"] | ES["FORCES"] | ES["STRESS"] | ES["DIR"] | ES["REC"],
+    ES["LQ"] | ES["LD"] | ES["EF"] | ES["EFG"] | ES["FORCES"] | ES["STRESS"] | ES["DIR"] | ES["REC"],
+    ES["LQ"] | ES["LD"] | ES["EF"] | ES["EFG"] | ES["DIR"],
+    ES["LQ"] | ES["LD"] | ES["EF"] | ES["FORCES"] | ES["REC"],
+]
+
+
+@pytest.mark.parametrize("flags", COMBOS)
+def test_multipole_es_matches_oracle(flags):
+    print(f"[gpu] multipole_es flags {flags}", flush=True)
+    p, cfg, _ = load_case("pim192_static")
+    rng = np.random.default_rng(flags)
+    q = np.array([p.qit[t] for t in cfg.typia])
+    mu = rng.normal(0, 0.05, (cfg.nion, 3))
+    mu[cfg.typia == 1] = 0.0  # cations carry no dipole
+    o = Oracle(p, cfg.typia, cfg.A, cfg.r)
+    with m.Engine(p, cfg.typia, cfg.A, cfg.r) as e:
+        a = e.multipole_es(q, mu, flags)
+        b = o.multipole_es(q, mu, flags)
+    assert a["u"] == pytest.approx(b["u"], rel=1e-10)
+    assert rel(a["ef"], b["ef"], 1e-12) < 1e-10
+    assert rel(a["efg"], b["efg"], 1e-12) < 1e-10
+    assert rel(a["f"], b["f"], 1e-12) < 1e-10
+    if flags & ES["STRESS"]:
+        assert rel(a["tau"], b["tau"], 1e-12) < 1e-10
+        assert a["pvir"] == pytest.approx(b["pvir"], rel=1e-10)
+
+
+def test_pair_short_and_dipoles_match_oracle():
+    p, cfg, _ = load_case("pim192_static")
+    o = Oracle(p, cfg.typia, cfg.A, cfg.r)
+    to = o.engforce(0)
+    with m.Engine(p, cfg.typia, cfg.A, cfg.r) as e:
+        u, pv, tau = e.pair_short()
+        assert u == pytest.approx(to["u_bhmftd"], rel=1e-12)
+        assert pv == pytest.approx(to["pvir_bhmftd"], rel=1e-12)
+        mu, upol = e.dipoles_scf(0)
+        assert rel(mu, o.dipoles()) < 1e-10
+        assert upol == pytest.approx(to["u_pol"], rel=1e-10)
+
+
+@pytest.mark.parametrize("lanes", [16, 64])
+def test_lane_widths_agree(lanes):
+    p, cfg, _ = load_case("pim192_static")
+    with m.Engine(p, cfg.typia, cfg.A, cfg.r) as e0, m.Engine(p, cfg.typia, cfg.A, cfg.r) as e1:
+        e1.set_tuning(lanes=lanes)
+        t0, t1 = e0.engforce(0), e1.engforce(0)
+        assert rel(e0.forces(), e1.forces()) < 1e-12
+        assert t0["u_tot"] == pytest.approx(t1["u_tot"], rel=1e-13)
+
+
+@pytest.mark.parametrize("case", ["pim192_static", "sio2_102_static", "bsio_200_nvt"])
+def test_mixed_form_accuracy(case):
+    """mixed form (FP32 real space and S(k), FP64 SCF residual) against the
+    FP64 form: force RMS <= 1e-5 relative, dipoles <= 1e-5, energy <= 1e-6"""
+    print(f"[gpu] mixed {case}", flush=True)
+    p, cfg, _ = load_case(case)
+    p.conv_tol_ind = 1e-9
+    p.max_scf_pim_iter = 400
+    p.min_scf_pim_iter = 1
+    with m.Engine(p, cfg.typia, cfg.A, cfg.r, precision=m.FP64) as ef, \
+            m.Engine(p, cfg.typia, cfg.A, cfg.r, precision=m.MIXED) as em:
+        tf, tm = ef.engforce(0), em.engforce(0)
+        assert tf["status"] == 0 and tm["status"] == 0
+        Ff, Fm = ef.forces(), em.forces()
+        frms = np.sqrt(np.mean((Fm - Ff) ** 2)) / np.sqrt(np.mean(Ff ** 2))
+        assert frms < 1e-5, frms
+        assert rel(em.dipoles(), ef.dipoles()) < 1e-5
+        assert tm["u_tot"] == pytest.approx(tf["u_tot"], rel=1e-6)
+        assert tm["rmsd"] <= 1e-9
