@@ -121,9 +121,11 @@ int mdff_md_run(mdff_ctx *ctx, int first_step, int nsteps, mdff_thermo *th);
  * tt_damp.c:16-32), host evaluation of the exact device formula. */
 void mdff_tt_damping(double b, double c, double r, double *f, double *fd, int order);
 
-/* Tuning: lanes per ion in the neighbour kernels (16, 32 or 64; 0 = auto),
- * atoms per block and k-split of the reciprocal kernels.  Chosen by the
- * autotuner in mdff20_amd/autotune.py when not set. */
+/* Tuning: lanes per ion in the neighbour kernels (16, 32 or 64; 0 keeps),
+ * reciprocal-space kernel family katoms (-1 vector ALU; 1 matrix cores with
+ * split-BF16 SCF correction sweeps, the default; 2 matrix cores with FP32
+ * sweeps; 0 keeps) and ksplit, the target block count of the matrix-core
+ * kernels (0 keeps).  No counterpart in the reference. */
 int mdff_set_tuning(mdff_ctx *ctx, int lanes, int katoms, int ksplit);
 
 /* Device timing of the last engforce by phase, in ms (HIP events on the
@@ -133,7 +135,8 @@ int mdff_get_timing(mdff_ctx *ctx, double t[6]);
 
 /* Time one kernel of the engine at the current state: `iters` launches
  * back to back on the context stream between HIP events; ms = average per
- * launch.  lo = 1 times the FP32 variant (mixed contexts only).  which:
+ * launch.  lo = 1 times the FP32 variant, lo = 2 the split-BF16 matrix-core
+ * variant (mixed contexts only).  which:
  *   0 S(k) forward, SCF form (dipoles of the polarisable ions, 3 channels)
  *   1 back-projection, SCF form (field at the polarisable ions)
  *   2 real-space induced dipole field

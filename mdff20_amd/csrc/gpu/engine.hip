@@ -14,6 +14,7 @@
 
 #include "kernels.h"
 #include "md_kernels.h"
+#include "mfma_kernels.h"
 #include "mdff_gpu.h"
 #include "mdff_host.h"
 
@@ -104,6 +105,11 @@ struct mdff_ctx {
     int nchunk_p = 0, chunk_p = 0;
     // tuning
     int lanes = 32;
+    int use_mfma = 1;         // S(k) forward and back-projection on the matrix cores
+    int mf_lo = KM_F32;       // matrix-core mode of the FP32-table evaluations
+    int mf_inner = KM_BF16X3; // mode of the mixed-form SCF correction sweeps
+    int mf_blocks = 512;      // target blocks of the MFMA kernels
+    int nsplit_max = 1;       // (ny,nz) splits the bpart buffer holds
     // host pinned mirrors
     int *ctl_h = nullptr;
     double *scal_h = nullptr;
@@ -347,6 +353,74 @@ static void launch_back_t(mdff_ctx *c, const BackArgs &a, int nsel)
     hipLaunchKernelGGL((k_sk_back<real, NCH>), dim3(cdiv(nsel, 64), c->nsplit), dim3(64), sh, c->st, a);
 }
 
+// forward S(k) on the matrix cores (FP64, or split BF16 for the mixed form);
+// returns 0 when the shape is outside what the kernel is built for
+template <int MODE>
+static int fwd_mfma_t(mdff_ctx *c, SkMArgs &m, int nsel)
+{
+    constexpr int BC = 64 * MfTraits<MODE>::CW, SA = MfTraits<MODE>::SA;
+    int gx = (c->P + BC - 1) / BC;
+    int nchunk = std::max(1, std::min(32, (c->mf_blocks + gx - 1) / gx));
+    int chunk = ((nsel + nchunk - 1) / nchunk + SA - 1) / SA * SA;
+    nchunk = (nsel + chunk - 1) / chunk;
+    m.chunk = chunk;
+    hipLaunchKernelGGL(k_sk_fwd_mfma<MODE>, dim3(gx, nchunk), dim3(256), mf_fwd_lds_bytes<MODE>(), c->st, m);
+    return nchunk;
+}
+
+static int launch_fwd_mfma(mdff_ctx *c, int polar, int lo, int C, const double *q_w, const double *mu_w,
+                           const int *active, int *nchunk_out)
+{
+    int M = (c->Kx + 1) * C * 2;
+    if (!c->use_mfma || M > MF_MT * 16) return 0;
+    int nsel = polar ? c->npol : c->n;
+    SkMArgs m;
+    m.nsel = nsel; m.Kx = c->Kx; m.Ky = c->Ky; m.Kz = c->Kz; m.P = c->P; m.C = C; m.M = M;
+    m.tx = c->tab3[lo][polar][0]; m.ty = c->tab3[lo][polar][1]; m.tz = c->tab3[lo][polar][2];
+    m.idx = polar ? c->pidx : nullptr;
+    m.q = q_w; m.mu = mu_w; m.n = c->n;
+    m.part = c->kpart;
+    m.active = active;
+    int mode = lo ? c->mf_lo : KM_F64;
+    if (mode == KM_F64) *nchunk_out = fwd_mfma_t<KM_F64>(c, m, nsel);
+    else if (mode == KM_F32) *nchunk_out = fwd_mfma_t<KM_F32>(c, m, nsel);
+    else *nchunk_out = fwd_mfma_t<KM_BF16X3>(c, m, nsel);
+    return 1;
+}
+
+template <int MODE>
+static int back_mfma_t(mdff_ctx *c, BkMArgs &b, int nsel)
+{
+    constexpr int SP = MbTraits<MODE>::SP;
+    int gx = (nsel + 63) / 64;
+    int ns = std::max(1, std::min(c->nsplit_max, (c->mf_blocks + gx - 1) / gx));
+    int psplit = ((c->P + ns - 1) / ns + SP - 1) / SP * SP;
+    ns = (c->P + psplit - 1) / psplit;
+    b.psplit = psplit;
+    hipLaunchKernelGGL(k_sk_back_mfma<MODE>, dim3(gx, ns), dim3(256), mf_back_lds_bytes<MODE>(), c->st, b);
+    return ns;
+}
+
+// back-projection on the matrix cores; returns 0 when outside the kernel's shapes
+static int launch_back_mfma(mdff_ctx *c, int polar, int lo, int NCH, const int *active, int *nsplit_out)
+{
+    int CH = NCH == 9 ? 6 : 3;
+    int NC = (c->Kx + 1) * CH * 2;
+    if (!c->use_mfma || NC > MB_CT * 16) return 0;
+    int nsel = polar ? c->npol : c->n;
+    BkMArgs b;
+    b.nsel = nsel; b.Kx = c->Kx; b.Ky = c->Ky; b.Kz = c->Kz; b.P = c->P; b.CH = CH; b.NC = NC;
+    b.tx = c->tab3[lo][polar][0]; b.ty = c->tab3[lo][polar][1]; b.tz = c->tab3[lo][polar][2];
+    b.W = c->W;
+    b.part = c->bpart;
+    b.active = active;
+    int mode = lo ? c->mf_lo : KM_F64;
+    if (mode == KM_F64) *nsplit_out = back_mfma_t<KM_F64>(c, b, nsel);
+    else if (mode == KM_F32) *nsplit_out = back_mfma_t<KM_F32>(c, b, nsel);
+    else *nsplit_out = back_mfma_t<KM_BF16X3>(c, b, nsel);
+    return 1;
+}
+
 // reciprocal part of multipole_ES (ewald.c:876-1051) on the selected ions.
 // polar: use the polarisable-ion tables (SCF); C: channels of S(k);
 // NCH: 3 (field) or 9 (field + gradient); do_ef/do_f outputs; efg optional.
@@ -369,9 +443,11 @@ static void rec_part(mdff_ctx *c, int polar, int lo, int C, int NCH, const doubl
     a.part = c->kpart;
     a.active = active;
     bool mixed = lo != 0;
-    if (C == 1) { if (mixed) launch_fwd_t<float, 1>(c, a, nchunk); else launch_fwd_t<double, 1>(c, a, nchunk); }
-    else if (C == 3) { if (mixed) launch_fwd_t<float, 3>(c, a, nchunk); else launch_fwd_t<double, 3>(c, a, nchunk); }
-    else { if (mixed) launch_fwd_t<float, 4>(c, a, nchunk); else launch_fwd_t<double, 4>(c, a, nchunk); }
+    if (!launch_fwd_mfma(c, polar, lo, C, q_w, mu_w, active, &nchunk)) {
+        if (C == 1) { if (mixed) launch_fwd_t<float, 1>(c, a, nchunk); else launch_fwd_t<double, 1>(c, a, nchunk); }
+        else if (C == 3) { if (mixed) launch_fwd_t<float, 3>(c, a, nchunk); else launch_fwd_t<double, 3>(c, a, nchunk); }
+        else { if (mixed) launch_fwd_t<float, 4>(c, a, nchunk); else launch_fwd_t<double, 4>(c, a, nchunk); }
+    }
     KRedArgs r;
     r.Kx = c->Kx; r.Ky = c->Ky; r.Kz = c->Kz; r.P = c->P; r.nchunk = nchunk; r.C = C;
     r.part = c->kpart; r.kvec = c->kvec; r.Ak = c->Ak; r.kcoe = c->kcoe; r.W = c->W; r.ek = c->ek;
@@ -385,10 +461,13 @@ static void rec_part(mdff_ctx *c, int polar, int lo, int C, int NCH, const doubl
         b.rows_per_split = c->rows_per_split;
         b.part = c->bpart;
         b.active = active;
-        if (NCH == 9) { if (mixed) launch_back_t<float, 9>(c, b, nsel); else launch_back_t<double, 9>(c, b, nsel); }
-        else { if (mixed) launch_back_t<float, 3>(c, b, nsel); else launch_back_t<double, 3>(c, b, nsel); }
+        int nsplit = c->nsplit;
+        if (!launch_back_mfma(c, polar, lo, NCH, active, &nsplit)) {
+            if (NCH == 9) { if (mixed) launch_back_t<float, 9>(c, b, nsel); else launch_back_t<double, 9>(c, b, nsel); }
+            else { if (mixed) launch_back_t<float, 3>(c, b, nsel); else launch_back_t<double, 3>(c, b, nsel); }
+        }
         BackFinArgs fa;
-        fa.n = c->n; fa.nsel = nsel; fa.nsplit = c->nsplit; fa.NCH = NCH;
+        fa.n = c->n; fa.nsel = nsel; fa.nsplit = nsplit; fa.NCH = NCH;
         fa.idx = a.idx; fa.c = c->cd; fa.tfpiV = c->tfpiV; fa.part = c->bpart;
         fa.q = q_f; fa.mu = mu_f;
         fa.ef = c->ef_rec; fa.efg = efg_out; fa.f = c->f_rec;
@@ -551,11 +630,16 @@ static int scf_loop_mixed(mdff_ctx *c, int istep)
             // correction (skipped once converged)
             hipLaunchKernelGGL(k_dc_residual, dim3(cdiv(n, 256)), dim3(256), 0, c->st, n, c->type, c->tab_d, c->ef,
                                c->mu_ind, c->dc_R, c->dc_d, act);
+            // the correction sweeps only steer the FP64-residual iteration,
+            // so they may run at split-BF16 accuracy
+            int keep = c->mf_lo;
+            c->mf_lo = c->mf_inner;
             for (int k = 0; k < c->inner_iters; k++) {
                 induced_field(c, 1, c->dc_d, nullptr, c->dc_g, act);
                 hipLaunchKernelGGL(k_dc_update, dim3(cdiv(n, 256)), dim3(256), 0, c->st, n, c->type, c->tab_d,
                                    c->dc_R, c->dc_g, c->dc_d, act);
             }
+            c->mf_lo = keep;
             hipLaunchKernelGGL(k_axpy_gated, dim3(cdiv(3 * n, 256)), dim3(256), 0, c->st, 3 * n, c->dc_d, c->mu_ind,
                                act);
         }
@@ -1059,8 +1143,9 @@ int mdff_create(mdff_ctx **out, const mdff_params *p, int nion, const int *typia
         c->rows_per_split = (nrows + c->nsplit - 1) / c->nsplit;
         c->nsplit = (nrows + c->rows_per_split - 1) / c->rows_per_split;
         int C = 4;
-        ALLOC(c->kpart, (size_t)std::max(c->nchunk, c->nchunk_p) * c->nkf * 2 * C);
-        ALLOC(c->bpart, (size_t)c->nsplit * 9 * n);
+        ALLOC(c->kpart, (size_t)std::max(64, std::max(c->nchunk, c->nchunk_p)) * c->nkf * 2 * C);
+        c->nsplit_max = std::max(c->nsplit, 16);
+        ALLOC(c->bpart, (size_t)c->nsplit_max * 9 * n);
     }
     // neighbour list capacity: all pairs when no list, else an estimate
     // from the number density grown on overflow
@@ -1077,6 +1162,21 @@ int mdff_create(mdff_ctx **out, const mdff_params *p, int nion, const int *typia
 #undef ALLOC
     for (int i = 0; i < 8; i++)
         if (hipEventCreate(&c->ev[i]) != hipSuccess) { mdff_destroy(c); return MDFF_ERR_DEVICE; }
+    if (hipFuncSetAttribute((const void *)k_sk_fwd_mfma<KM_F64>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)mf_fwd_lds_bytes<KM_F64>()) != hipSuccess ||
+        hipFuncSetAttribute((const void *)k_sk_fwd_mfma<KM_BF16X3>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)mf_fwd_lds_bytes<KM_BF16X3>()) != hipSuccess ||
+        hipFuncSetAttribute((const void *)k_sk_back_mfma<KM_F64>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)mf_back_lds_bytes<KM_F64>()) != hipSuccess ||
+        hipFuncSetAttribute((const void *)k_sk_back_mfma<KM_BF16X3>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)mf_back_lds_bytes<KM_BF16X3>()) != hipSuccess ||
+        hipFuncSetAttribute((const void *)k_sk_fwd_mfma<KM_F32>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)mf_fwd_lds_bytes<KM_F32>()) != hipSuccess ||
+        hipFuncSetAttribute((const void *)k_sk_back_mfma<KM_F32>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)mf_back_lds_bytes<KM_F32>()) != hipSuccess) {
+        mdff_destroy(c);
+        return MDFF_ERR_DEVICE;
+    }
     c->have_events = 1;
     // first list build (main calls check_verletlist before MD, mdff20.c:87)
     set_int(c, C_FIRST, 1);
@@ -1092,8 +1192,13 @@ int mdff_set_tuning(mdff_ctx *c, int lanes, int katoms, int ksplit)
     if (!c) return MDFF_ERR_ARG;
     if (lanes == 16 || lanes == 32 || lanes == 64) c->lanes = lanes;
     else if (lanes != 0) return MDFF_ERR_ARG;
-    (void)katoms;
-    (void)ksplit;
+    // katoms: reciprocal-space kernel family.  -1 vector ALU; 1 matrix cores
+    // with split-BF16 SCF correction sweeps; 2 matrix cores, FP32 sweeps.
+    if (katoms == 1) { c->use_mfma = 1; c->mf_inner = KM_BF16X3; }
+    else if (katoms == 2) { c->use_mfma = 1; c->mf_inner = KM_F32; }
+    else if (katoms == -1) c->use_mfma = 0;
+    else if (katoms != 0) return MDFF_ERR_ARG;
+    if (ksplit > 0) c->mf_blocks = ksplit;
     return MDFF_OK;
 }
 
@@ -1394,8 +1499,14 @@ int mdff_md_run(mdff_ctx *c, int first_step, int nsteps, mdff_thermo *th)
 
 int mdff_time_kernel(mdff_ctx *c, int which, int lo, int iters, double *ms)
 {
-    if (!c || !ms || iters < 1 || !c->p.lcoulombic) return MDFF_ERR_ARG;
+    if (!c || !ms || iters < 1 || !c->p.lcoulombic || lo < 0 || lo > 2) return MDFF_ERR_ARG;
     if (lo && !lo_of(c)) return MDFF_ERR_ARG;
+    // lo = 2: FP32 tables with the split-BF16 matrix-core mode
+    struct ModeGuard {
+        mdff_ctx *c; int keep;
+        ~ModeGuard() { c->mf_lo = keep; }
+    } guard{c, c->mf_lo};
+    if (lo == 2) { c->mf_lo = KM_BF16X3; lo = 1; }
     int n = c->n;
     int polar = (which == 0 || which == 1 || which == 2);
     if (polar && c->npol == 0) return MDFF_ERR_ARG;
@@ -1418,12 +1529,15 @@ int mdff_time_kernel(mdff_ctx *c, int which, int lo, int iters, double *ms)
     b.active = nullptr;
     DirArgs d = dir_args(c, polar ? c->qz : c->q, mu, polar ? 0 : c->ldmp, polar ? 1 : 0, nullptr);
     auto once = [&]() {
+        int dummy = 0;
         switch (which) {
         case 0: case 3:
+            if (launch_fwd_mfma(c, polar, lo, C, a.q, mu, nullptr, &dummy)) break;
             if (lo) { if (C == 3) launch_fwd_t<float, 3>(c, a, nchunk); else launch_fwd_t<float, 4>(c, a, nchunk); }
             else { if (C == 3) launch_fwd_t<double, 3>(c, a, nchunk); else launch_fwd_t<double, 4>(c, a, nchunk); }
             break;
         case 1: case 4:
+            if (launch_back_mfma(c, polar, lo, NCH, nullptr, &dummy)) break;
             if (lo) { if (NCH == 3) launch_back_t<float, 3>(c, b, nsel); else launch_back_t<float, 9>(c, b, nsel); }
             else { if (NCH == 3) launch_back_t<double, 3>(c, b, nsel); else launch_back_t<double, 9>(c, b, nsel); }
             break;

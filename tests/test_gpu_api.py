@@ -65,17 +65,21 @@ def test_lane_widths_agree(lanes):
         assert t0["u_tot"] == pytest.approx(t1["u_tot"], rel=1e-13)
 
 
+@pytest.mark.parametrize("katoms", [1, 2, -1])
 @pytest.mark.parametrize("case", ["pim192_static", "sio2_102_static", "bsio_200_nvt"])
-def test_mixed_form_accuracy(case):
-    """mixed form (FP32 real space and S(k), FP64 SCF residual) against the
-    FP64 form: force RMS <= 1e-5 relative, dipoles <= 1e-5, energy <= 1e-6"""
-    print(f"[gpu] mixed {case}", flush=True)
+def test_mixed_form_accuracy(case, katoms):
+    """mixed form (FP32 real space and S(k), FP64 SCF residual; correction
+    sweeps split-BF16 (katoms 1) or FP32 (2) on the matrix cores, or FP32
+    vector ALU (-1)) against the FP64 form: force RMS <= 1e-5 relative,
+    dipoles <= 1e-5, energy <= 1e-6"""
+    print(f"[gpu] mixed {case} katoms={katoms}", flush=True)
     p, cfg, _ = load_case(case)
     p.conv_tol_ind = 1e-9
     p.max_scf_pim_iter = 400
     p.min_scf_pim_iter = 1
     with m.Engine(p, cfg.typia, cfg.A, cfg.r, precision=m.FP64) as ef, \
             m.Engine(p, cfg.typia, cfg.A, cfg.r, precision=m.MIXED) as em:
+        em.set_tuning(katoms=katoms)
         tf, tm = ef.engforce(0), em.engforce(0)
         assert tf["status"] == 0 and tm["status"] == 0
         Ff, Fm = ef.forces(), em.forces()
@@ -84,3 +88,41 @@ def test_mixed_form_accuracy(case):
         assert rel(em.dipoles(), ef.dipoles()) < 1e-5
         assert tm["u_tot"] == pytest.approx(tf["u_tot"], rel=1e-6)
         assert tm["rmsd"] <= 1e-9
+
+
+@pytest.mark.parametrize("case", ["pim192_static", "bsio_200_nvt", "melt216_nve"])
+def test_mfma_kspace_matches_vector_kernels_fp32(case):
+    """FP32 S(k) forward and back-projection on the matrix cores against the
+    FP32 vector-ALU kernels in the mixed form (FP32 rounding level)"""
+    print(f"[gpu] mfma vs valu fp32 {case}", flush=True)
+    p, cfg, _ = load_case(case)
+    p.conv_tol_ind = 1e-10
+    p.max_scf_pim_iter = 400
+    p.min_scf_pim_iter = 1
+    with m.Engine(p, cfg.typia, cfg.A, cfg.r, precision=m.MIXED) as e0, \
+            m.Engine(p, cfg.typia, cfg.A, cfg.r, precision=m.MIXED) as e1:
+        e0.set_tuning(katoms=-1)
+        e1.set_tuning(katoms=2)
+        t0, t1 = e0.engforce(0), e1.engforce(0)
+        assert rel(e1.forces(), e0.forces()) < 2e-6
+        assert t1["u_coul"] == pytest.approx(t0["u_coul"], rel=1e-7)
+        if t0["u_pol"]:
+            assert rel(e1.dipoles(), e0.dipoles()) < 2e-6
+
+
+@pytest.mark.parametrize("case", ["pim192_static", "bsio_200_nvt", "melt216_nve"])
+def test_mfma_kspace_matches_vector_kernels(case):
+    """S(k) forward and back-projection on the matrix cores (FP64) against
+    the vector-ALU kernels"""
+    print(f"[gpu] mfma vs valu {case}", flush=True)
+    p, cfg, _ = load_case(case)
+    with m.Engine(p, cfg.typia, cfg.A, cfg.r) as e0, m.Engine(p, cfg.typia, cfg.A, cfg.r) as e1:
+        e0.set_tuning(katoms=-1)
+        e1.set_tuning(katoms=1)
+        t0, t1 = e0.engforce(0), e1.engforce(0)
+        assert rel(e1.forces(), e0.forces()) < 1e-12
+        assert t1["u_coul"] == pytest.approx(t0["u_coul"], rel=1e-13)
+        assert t1["n_scf"] == t0["n_scf"]
+        assert rel(np.array(t1["tau_coul"]), np.array(t0["tau_coul"])) < 1e-12
+        if t0["u_pol"]:
+            assert rel(e1.dipoles(), e0.dipoles()) < 1e-12
