@@ -350,3 +350,6 @@ class Engine:
         t = Thermo()
         _check(self._L.mdff_md_run(self._h, first_step, nsteps, C.byref(t)), "md_run")
         return t.as_dict()
+
+
+from . import batch  # noqa: E402  (batched engine and adjoint, include/mdff_batch.h)

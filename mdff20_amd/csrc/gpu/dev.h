@@ -18,6 +18,17 @@ struct CellD {
     double B[3][3];  // reciprocal / Omega, kardir uses B[i][*].r
 };
 
+// one configuration of a batch: its cell, Ewald and cutoff settings and its
+// ion and k-vector ranges in the concatenated arrays
+struct BConfD {
+    CellD c;
+    double alpha, rc2s, rc2l;             // Ewald alpha, short and Coulomb cutoffs squared
+    double selfa, selfa2, ttpiV, tfpiV;   // alpha/sqrt(pi), 2 selfa alpha^2/3, 4 pi/V, 8 pi/V
+    double iv;                            // 1 / (Omega press_unit)
+    int off, n, npol;                     // ions [off, off + n), polarisable count
+    int Kx, Ky, Kz, koff, nk;             // k box and half-space k-vectors [koff, koff + nk)
+};
+
 // per-type and per-type-pair tables, device resident
 struct TypeTab {
     int ntype;

@@ -131,16 +131,21 @@ struct ShortArgs {
     double rc2;
     double *f;            // 3n: force (overwritten)
     double *acc;          // 14n: per-ion halves u_bh, w_bh[6], u_lj, w_lj[6]
+    const BConfD *conf;   // BATCH: per-configuration cell and cutoffs
+    const int *cfg;       // BATCH: configuration of each ion
 };
 
 // engforce_bhmftd_pbc bhmftd.c:117-271 and engforce_nmlj_pbc nmlj.c:254-385,
-// LANES lanes per ion over its full list.
-template <typename real, int LANES>
+// LANES lanes per ion over its full list.  BATCH: the cell and cutoff are
+// those of the ion's configuration.
+template <typename real, int LANES, bool BATCH = false>
 __global__ void __launch_bounds__(256) k_short(ShortArgs a)
 {
     const int lane = threadIdx.x % LANES;
     const int i = (blockIdx.x * blockDim.x + threadIdx.x) / LANES;
     if (i >= a.n) return;
+    const CellD &cell = BATCH ? a.conf[a.cfg[i]].c : a.c;
+    const double rc2 = BATCH ? a.conf[a.cfg[i]].rc2s : a.rc2;
     const TypeTab &T = *a.T;
     const int n = a.n;
     const int ti = a.type[i];
@@ -155,9 +160,9 @@ __global__ void __launch_bounds__(256) k_short(ShortArgs a)
         int j = ent & NBR_IDX;
         int tj = a.type[j];
         double x, y, z;
-        min_image(a.c, six - a.s[j], siy - a.s[n + j], siz - a.s[2 * n + j], x, y, z);
+        min_image(cell, six - a.s[j], siy - a.s[n + j], siz - a.s[2 * n + j], x, y, z);
         double d2 = x * x + y * y + z * z;
-        if (!(d2 < a.rc2)) continue;
+        if (!(d2 < rc2)) continue;
         if (T.lbh && T.A[ti][tj] != 0.0) {
             real rd2 = (real)d2;
             real ir2 = (real)1.0 / rd2, d = sqrt(rd2);
@@ -229,6 +234,8 @@ struct DirArgs {
     double *f;            // 3n force, coulombic units excluded (written when DO_F)
     double *uw;           // 7n: energy halves (always) then 6 virial halves (DO_F)
     int q_zero;           // all charges zero: ions without dipole get nothing
+    const BConfD *conf;   // BATCH: per-configuration cell, alpha and cutoff
+    const int *cfg;       // BATCH: configuration of each ion; active is then per configuration
 };
 
 // multipole_ES_dir ewald.c:353-871 from ion i's side of its full list.
@@ -240,13 +247,18 @@ struct DirArgs {
 //   q-mu  (ewald.c:767-812): u -qi T1.muj + qj T1.mui (+ damped);
 //                            f_ij -qi T2 muj + qj T2 mui (+ damped)
 // with F_i -= f_ij, stress -(r f_ij + f_ij r) (ewald.c:815-830).
-template <typename real, int LANES, bool DO_EF, bool EFG_Q, bool EFG_MM, bool DO_F>
+// BATCH: cell, alpha, cutoff and the active gate are per configuration.
+template <typename real, int LANES, bool DO_EF, bool EFG_Q, bool EFG_MM, bool DO_F, bool BATCH = false>
 __global__ void __launch_bounds__(256) k_ewald_dir(DirArgs a)
 {
-    if (a.active && !*a.active) return;
+    if (!BATCH && a.active && !*a.active) return;
     const int lane = threadIdx.x % LANES;
     const int i = (blockIdx.x * blockDim.x + threadIdx.x) / LANES;
     if (i >= a.n) return;
+    if (BATCH && a.active && !a.active[a.cfg[i]]) return;
+    const CellD &cell = BATCH ? a.conf[a.cfg[i]].c : a.c;
+    const double alpha = BATCH ? a.conf[a.cfg[i]].alpha : a.alpha;
+    const double rc2 = BATCH ? a.conf[a.cfg[i]].rc2l : a.rc2;
     const TypeTab &T = *a.T;
     const int n = a.n;
     const int ti = a.type[i];
@@ -254,7 +266,7 @@ __global__ void __launch_bounds__(256) k_ewald_dir(DirArgs a)
     const real qi = (real)a.q[i];
     const real mix = (real)a.mu[i], miy = (real)a.mu[n + i], miz = (real)a.mu[2 * n + i];
     const bool dip_i = (a.mu[i] != 0.0) || (a.mu[n + i] != 0.0) || (a.mu[2 * n + i] != 0.0);
-    const real al = (real)a.alpha, al2 = al * al, al3 = al2 * al, al5 = al3 * al2;
+    const real al = (real)alpha, al2 = al * al, al3 = al2 * al, al5 = al3 * al2;
     const real rpi = (real)(1.0 / 1.7724538509055160273);
     double ex = 0, ey = 0, ez = 0;
     double g[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
@@ -267,9 +279,9 @@ __global__ void __launch_bounds__(256) k_ewald_dir(DirArgs a)
         if (!(ent & NBR_LR)) continue;
         int j = ent & NBR_IDX;
         double dx, dy, dz;
-        min_image(a.c, six - a.s[j], siy - a.s[n + j], siz - a.s[2 * n + j], dx, dy, dz);
+        min_image(cell, six - a.s[j], siy - a.s[n + j], siz - a.s[2 * n + j], dx, dy, dz);
         double dd2 = dx * dx + dy * dy + dz * dz;
-        if (dd2 > a.rc2) continue;
+        if (dd2 > rc2) continue;
         const int tj = a.type[j];
         const real qj = (real)a.q[j];
         const double mjxd = a.mu[j], mjyd = a.mu[n + j], mjzd = a.mu[2 * n + j];

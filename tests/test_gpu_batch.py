@@ -1,0 +1,230 @@
+"""Batched engine and force-matching adjoint (include/mdff_batch.h) through
+the C ABI.
+
+Parity: every configuration of a batch of uneven size against the oracle run
+on that configuration alone (FP64, 1e-10 relative on energies, forces,
+dipoles and stress, equal SCF iteration counts).  Mixed form: force RMS
+<= 1e-5 relative to the FP64 batch.  Adjoint: the gradient of
+L = sum |F - F_ref|^2 against central finite differences of L (step 1e-6
+relative) to 1e-6 relative, the batch forces against -dU/dr by finite
+differences, and 50 Adam steps through the torch.autograd.Function."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import mdff20_amd as m
+from conftest import load_case, rel
+
+pytestmark = pytest.mark.gpu
+
+
+def _supercell(cfg, reps, noise, seed):
+    """reps (nx, ny, nz) copies of a fixture cell, positions jittered"""
+    A = np.asarray(cfg.A)
+    r = np.asarray(cfg.r)
+    typ = np.asarray(cfg.typia)
+    rs, ts = [], []
+    for ix in range(reps[0]):
+        for iy in range(reps[1]):
+            for iz in range(reps[2]):
+                sh = ix * A[0] + iy * A[1] + iz * A[2]
+                rs.append(r + sh[:, None])
+                ts.append(typ)
+    rr = np.concatenate(rs, axis=1)
+    tt = np.concatenate(ts)
+    # keep the type-ordered layout POSFF gives (types by blocks)
+    order = np.argsort(tt, kind="stable")
+    rr, tt = rr[:, order], tt[order]
+    rng = np.random.default_rng(seed)
+    rr = rr + noise * rng.standard_normal(rr.shape)
+    AA = A * np.array(reps)[:, None]
+    return tt.astype(np.int32), AA, rr
+
+
+def _configs():
+    """uneven batch built from the pim192 fixture: itself, jittered copies,
+    a 2x1x1 and a 2x2x1 supercell and a stretched cell"""
+    p, cfg, _ = load_case("pim192_static")
+    base = dict(alpha=0.35, kES=(6, 6, 6), rcut_long=7.0, rcut_short=7.0)
+    out = []
+    t, A, r = _supercell(cfg, (1, 1, 1), 0.0, 0)
+    out.append(((t, A, r), base))
+    t, A, r = _supercell(cfg, (1, 1, 1), 0.03, 1)
+    out.append(((t, A, r), dict(base, alpha=0.4, kES=(7, 7, 7))))
+    t, A, r = _supercell(cfg, (2, 1, 1), 0.03, 2)
+    out.append(((t, A, r), dict(base, kES=(12, 6, 6))))
+    t, A, r = _supercell(cfg, (1, 1, 1), 0.02, 3)
+    s = 1.03
+    out.append(((t, A * s, r * s), dict(base, alpha=0.38, kES=(6, 6, 6), rcut_long=7.2, rcut_short=6.5)))
+    t, A, r = _supercell(cfg, (2, 2, 1), 0.03, 4)
+    out.append(((t, A, r), dict(base, kES=(12, 12, 6))))
+    return p, out
+
+
+def _oracle_params(p, e):
+    q = m.Params()
+    C.pointer(q)[0] = p
+    q.alphaES = e["alpha"]
+    for k in range(3):
+        q.kES[k] = int(e["kES"][k])
+    q.cutlongrange = e["rcut_long"]
+    q.cutshortrange = e["rcut_short"]
+    q.lautoES = 0
+    return q
+
+
+KEYS = ["u_bhmftd", "u_coul", "u_pol", "u_tot"]
+
+
+def test_batch_fp64_matches_oracle_per_configuration():
+    from oracle.oracle import Oracle
+    p, confs = _configs()
+    print(f"[gpu] batch fp64 parity: sizes {[len(c[0][0]) for c in confs]}", flush=True)
+    with m.batch.Batch(p, [c[0] for c in confs], [c[1] for c in confs], precision=m.FP64) as b:
+        th = b.engforce()
+        F = b.split(b.forces())
+        mu = b.dipoles()
+        for c, ((t, A, r), e) in enumerate(confs):
+            q = _oracle_params(p, e)
+            o = Oracle(q, t, A, r)
+            to = o.engforce(0)
+            for k in KEYS:
+                assert th[c][k] == pytest.approx(to[k], rel=1e-10, abs=1e-12), (c, k, th[c][k], to[k])
+            assert th[c]["n_scf"] == to["n_scf"], (c, th[c]["n_scf"], to["n_scf"])
+            assert rel(F[c], o.forces()) < 1e-10, c
+            mc = mu[b.off[c]:b.off[c + 1]]
+            assert rel(mc, o.dipoles()) < 1e-10, c
+            assert rel(np.array(th[c]["tau_coul"]), np.array(to["tau_coul"])) < 1e-10, c
+            assert rel(np.array(th[c]["tau_bhmftd"]), np.array(to["tau_bhmftd"])) < 1e-10, c
+            o.close()
+
+
+def test_batch_autotune_keeps_results():
+    p, confs = _configs()
+    print("[gpu] batch autotune", flush=True)
+    with m.batch.Batch(p, [c[0] for c in confs], [c[1] for c in confs]) as b:
+        b.engforce()
+        F0 = b.forces()
+        tune = b.autotune()
+        assert tune["lanes"] in (16, 32, 64) and tune["kpb"] in (32, 64, 128) and tune["ipb"] in (64, 128, 256)
+        b.reset_dipoles()
+        b.engforce()
+        assert rel(b.forces(), F0) < 1e-12
+        for lanes, kpb, ipb, bpc in [(32, 32, 128, 2), (64, 128, 256, 8)]:
+            b.set_tuning(lanes, kpb, ipb, bpc)
+            b.reset_dipoles()
+            b.engforce()
+            assert rel(b.forces(), F0) < 1e-12
+
+
+def test_batch_mixed_accuracy():
+    p, confs = _configs()
+    print("[gpu] batch mixed", flush=True)
+    q = m.Params()
+    C.pointer(q)[0] = p
+    q.conv_tol_ind = 1e-10
+    with m.batch.Batch(q, [c[0] for c in confs], [c[1] for c in confs], precision=m.FP64) as bf, \
+            m.batch.Batch(q, [c[0] for c in confs], [c[1] for c in confs], precision=m.MIXED) as bm:
+        tf, tm = bf.engforce(), bm.engforce()
+        Ff, Fm = bf.split(bf.forces()), bm.split(bm.forces())
+        for c in range(len(confs)):
+            frms = np.sqrt(np.mean((Fm[c] - Ff[c]) ** 2)) / np.sqrt(np.mean(Ff[c] ** 2))
+            assert frms < 1e-5, (c, frms)
+            assert tm[c]["u_tot"] == pytest.approx(tf[c]["u_tot"], rel=1e-6)
+
+
+def _tight(p):
+    q = m.Params()
+    C.pointer(q)[0] = p
+    q.conv_tol_ind = 1e-14
+    q.max_scf_pim_iter = 2000
+    q.min_scf_pim_iter = 1
+    return q
+
+
+def _perturbed(spec, seed):
+    rng = np.random.default_rng(seed)
+    v = spec.values()
+    return spec.params(v + np.log(rng.uniform(0.9, 1.1, v.size)))
+
+
+def test_adjoint_matches_finite_differences():
+    p, confs = _configs()
+    confs = confs[:3]
+    p = _tight(p)
+    print("[gpu] adjoint vs finite differences", flush=True)
+    with m.batch.Batch(p, [c[0] for c in confs], [c[1] for c in confs]) as b:
+        spec = m.batch.ParamSpec.default(p, p.ntype, with_D=False)
+        kinds = {e[0] for e in spec.entries}
+        assert {"A", "B", "C", "pol", "damp_b", "damp_c"} <= kinds, spec.entries
+        # targets: FP64 forces under a perturbed parameter set
+        b.set_params(_perturbed(spec, 7))
+        b.engforce()
+        fref = b.forces()
+        b.set_params(p)
+        g = b.adjoint(fref)
+        assert g["adjoint_residual"] < 1e-12
+        spec_lin = m.batch.ParamSpec(p, spec.entries, log=False)
+        th0 = spec_lin.values()
+        grad = spec_lin.gather(g)
+        for k, e in enumerate(spec.entries):
+            h = 1e-6 * abs(th0[k])
+            L = []
+            for sgn in (1, -1):
+                th = th0.copy()
+                th[k] += sgn * h
+                b.set_params(spec_lin.params(th))
+                b.engforce()
+                d = b.forces() - fref
+                L.append(float(np.sum(d * d)))
+            fd = (L[0] - L[1]) / (2 * h)
+            print(f"  {e}: adjoint {grad[k]:.10e} fd {fd:.10e}", flush=True)
+            assert abs(grad[k] - fd) <= 1e-6 * max(abs(fd), 1e-8 * abs(g['loss']) / max(abs(th0[k]), 1e-30)), (e, grad[k], fd)
+        b.set_params(p)
+
+
+def test_batch_forces_are_minus_energy_gradient():
+    p, confs = _configs()
+    p = _tight(p)
+    (t, A, r), e = confs[1]
+    print("[gpu] batch forces vs -dU/dr", flush=True)
+    with m.batch.Batch(p, [(t, A, r)], [e]) as b:
+        b.engforce()
+        F = b.forces()
+        h = 1e-5
+        for i in (0, 17, 150):
+            for a in range(3):
+                U = []
+                for sgn in (1, -1):
+                    rr = r.copy()
+                    rr[a, i] += sgn * h
+                    b.set_positions(rr)
+                    U.append(b.engforce()[0]["u_tot"])
+                fd = -(U[0] - U[1]) / (2 * h)
+                assert abs(fd - F[a, i]) <= 2e-6 * max(1.0, abs(F[a, i])), (i, a, fd, F[a, i])
+
+
+def test_adam_through_autograd_function():
+    import torch  # noqa: F401
+    from mdff20_amd.adjoint import ForceMatchLoss, fit
+    p, confs = _configs()
+    confs = confs[:2]
+    q = m.Params()
+    C.pointer(q)[0] = p
+    q.conv_tol_ind = 1e-10
+    print("[gpu] 50 Adam steps", flush=True)
+    with m.batch.Batch(q, [c[0] for c in confs], [c[1] for c in confs]) as b:
+        spec = m.batch.ParamSpec.default(q, q.ntype)
+        b.set_params(_perturbed(spec, 11))
+        b.engforce()
+        fref = b.forces()
+        b.set_params(q)
+        theta = torch.tensor(spec.values(), dtype=torch.float64, requires_grad=True)
+        loss = ForceMatchLoss.apply(theta, b, spec, fref)
+        loss.backward()
+        g = theta.grad.numpy().copy()
+        assert np.all(np.isfinite(g)) and np.linalg.norm(g) > 0
+        th, hist = fit(b, spec, fref, steps=50, lr=0.01)
+        print(f"  loss {hist[0]:.6e} -> {hist[-1]:.6e}", flush=True)
+        assert len(hist) == 50 and hist[-1] < 0.5 * hist[0]
