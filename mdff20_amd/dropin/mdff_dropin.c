@@ -1,0 +1,362 @@
+/*
+ * mdff_dropin.c -- same-name drop-ins for mdff20's force path
+ * (declarations and contract in include/mdff_dropin.h).
+ *
+ * Build: compiled against the reference's headers, which define its globals
+ * (compile with -fcommon as the reference itself needs), and linked with
+ * libmdff_gpu.so into the reference's executable in place of the functions
+ * it replaces, e.g.
+ *   cc -c mdff_dropin.c -I<mdff20>/src -I<repo>/include -fcommon
+ * Each entry point maps the reference's global state onto the handle API of
+ * mdff_gpu.h; no computation happens here.
+ */
+#include <math.h>
+#include <stdbool.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* the reference's globals (its own headers, from its source tree) */
+#include "bhmftd.h"
+#include "cell.h"
+#include "config.h"
+#include "constants.h"
+#include "coulombic.h"
+#include "ewald.h"
+#include "field.h"
+#include "global.h"
+#include "md.h"
+#include "nmlj.h"
+#include "pim.h"
+#include "thermo.h"
+#include "verlet.h"
+
+#include "mdff_dropin.h"
+#include "mdff_gpu.h"
+#include "mdff_host.h"
+
+static mdff_ctx *g_ctx = NULL;
+static int g_nion = 0;
+static int g_atexit = 0;
+
+/* The reference draws its initial velocities from the C library's rand()
+ * (kinetic.c, rand.c), and the HIP runtime may draw from the same generator.
+ * Every call into the device library therefore runs on a private generator
+ * state; the reference's sequence is left where it was. */
+static char g_rng_state[256];
+static int g_rng_ready = 0;
+
+static char *rng_enter(void)
+{
+    if (!g_rng_ready) {
+        g_rng_ready = 1;
+        return initstate(1u, g_rng_state, sizeof(g_rng_state));
+    }
+    return setstate(g_rng_state);
+}
+
+static void rng_leave(char *prev) { setstate(prev); }
+
+static void fail(const char *where, int rc)
+{
+    fprintf(stderr, "mdff drop-in: %s failed with status %d\n", where, rc);
+    exit(-1);
+}
+
+/* mdff_params from the parsed globals of global.c, field.c, bhmftd.c,
+ * nmlj.c, pim.c and config.c; alphaES/kES already hold what set_autoES chose */
+static void params_from_globals(mdff_params *p)
+{
+    mdff_params_default(p);
+    p->ntype = ntype;
+    p->Fposff = Fposff;
+    p->lverletL = lverletL;
+    p->skindiff = skindiff;
+    p->lstatic = lstatic;
+    p->lrdf = lrdf;
+    p->lpstress = lpstress;
+    p->lreduced = lreduced;
+    /* the cutoffs the force routines use: lrcutsq is set before set_autoES
+     * overwrites cutlongrange (field.c:329-330) */
+    p->cutshortrange = srcutsq > 0.0 ? sqrt(srcutsq) : cutshortrange;
+    p->cutlongrange = lrcutsq > 0.0 ? sqrt(lrcutsq) : cutlongrange;
+    /* the context only evaluates forces here; the integrator stays the
+     * reference's own */
+    p->integrator = MDFF_NVE_VV;
+    p->nequil = 0;
+    p->lnmlj = lnmlj;
+    p->lbhmft = lbhmft;
+    p->lbhmftd = lbhmftd;
+    p->lcoulombic = lcoulombic;
+    p->lsymmetric = lsymmetric;
+    for (int t = 0; t < NTYPEMAX && t < MDFF_NTYPEMAX; t++) {
+        p->massit[t] = massit[t];
+        p->qit[t] = qit[t];
+        p->lpolar[t] = lpolar[t];
+        for (int i = 0; i < 3; i++) {
+            p->dipit[t][i] = dipit[t][i];
+            for (int j = 0; j < 3; j++) {
+                p->quadit[t][i][j] = quadit[t][i][j];
+                p->polit[t][i][j] = polit[t][i][j];
+            }
+        }
+        for (int i = 0; i < NTYPEMAX && i < MDFF_NTYPEMAX; i++) {
+            p->Abhmftd[t][i] = Abhmftd[t][i];
+            p->Bbhmftd[t][i] = Bbhmftd[t][i];
+            p->Cbhmftd[t][i] = Cbhmftd[t][i];
+            p->Dbhmftd[t][i] = Dbhmftd[t][i];
+            p->BDbhmftd[t][i] = BDbhmftd[t][i];
+            p->sigmalj[t][i] = sigmalj[t][i];
+            p->epslj[t][i] = epslj[t][i];
+            p->plj[t][i] = plj[t][i];
+            p->qlj[t][i] = qlj[t][i];
+            for (int k = 0; k < NTYPEMAX && k < MDFF_NTYPEMAX; k++) {
+                p->lpoldamping[t][i][k] = lpoldamping[t][i][k];
+                p->pol_damp_b[t][i][k] = pol_damp_b[t][i][k];
+                p->pol_damp_c[t][i][k] = pol_damp_c[t][i][k];
+                p->pol_damp_k[t][i][k] = pol_damp_k[t][i][k];
+            }
+        }
+    }
+    p->trunc = trunctype;
+    p->alphaES = alphaES;
+    for (int k = 0; k < 3; k++) p->kES[k] = kES[k];
+    p->epsw = epsw;
+    p->lautoES = 0;
+    p->algo_pim = algo_pim;
+    p->algo_extrapolate_dipole = algo_extrapolate_dipole;
+    p->extrapolate_order = extrapolate_order;
+    p->min_scf_pim_iter = min_scf_pim_iter;
+    p->max_scf_pim_iter = max_scf_pim_iter;
+    p->conv_tol_ind = conv_tol_ind;
+    p->omegakO = omegakO;
+}
+
+int mdff_dropin_params(mdff_params *p)
+{
+    if (!p) return MDFF_ERR_ARG;
+    params_from_globals(p);
+    return MDFF_OK;
+}
+
+void mdff_dropin_release(void)
+{
+    if (g_ctx) {
+        char *rs = rng_enter();
+        mdff_destroy(g_ctx);
+        rng_leave(rs);
+    }
+    g_ctx = NULL;
+    g_nion = 0;
+}
+
+/* the device context for the current system; positions uploaded */
+static mdff_ctx *ctx_sync(void)
+{
+    if (!g_ctx || g_nion != nion) {
+        mdff_dropin_release();
+        mdff_params p;
+        params_from_globals(&p);
+        double A9[9];
+        for (int i = 0; i < 9; i++) A9[i] = simuCell.A[i / 3][i % 3];
+        const char *pr = getenv("MDFF_PRECISION");
+        int prec = (pr && !strcmp(pr, "mixed")) ? MDFF_MIXED : MDFF_FP64;
+        const char *dv = getenv("MDFF_DEVICE");
+        int dev = dv ? atoi(dv) : 0;
+        int rc = mdff_create(&g_ctx, &p, nion, typia, A9, rx, ry, rz, prec, dev);
+        if (rc) fail("mdff_create", rc);
+        g_nion = nion;
+        if (!g_atexit) {
+            atexit(mdff_dropin_release);
+            g_atexit = 1;
+        }
+        return g_ctx;
+    }
+    int rc = mdff_set_positions(g_ctx, rx, ry, rz);
+    if (rc) fail("mdff_set_positions", rc);
+    return g_ctx;
+}
+
+static void copy_tau(double dst[3][3], const double src[3][3])
+{
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) dst[i][j] = src[i][j];
+}
+
+/* the induced dipoles into the reference's history slot 0, after the shift
+ * the predictor of extrapolate_dipole_aspc performs (pim.c:393-398) */
+static void store_induced(mdff_ctx *c)
+{
+    if (!lpim || !dipia_ind) return;
+    double *mu = (double *)malloc(sizeof(double) * 3 * (size_t)nion);
+    if (!mu) fail("malloc", MDFF_ERR_NOMEM);
+    int rc = mdff_get_dipoles(c, mu);
+    if (rc) fail("mdff_get_dipoles", rc);
+    for (int ia = 0; ia < nion; ia++) {
+        for (int k = extrapolate_order; k > 0; k--)
+            for (int i = 0; i < 3; i++) dipia_ind[ia][k][i] = dipia_ind[ia][k - 1][i];
+        for (int i = 0; i < 3; i++) dipia_ind[ia][0][i] = mu[3 * ia + i] - dipia[ia][i];
+    }
+    free(mu);
+}
+
+static void engforce_impl(void)
+{
+    mdff_ctx *c = ctx_sync();
+    mdff_thermo th;
+    int rc = mdff_engforce(c, istep, &th);
+    if (rc && rc != MDFF_ERR_SCF) fail("mdff_engforce", rc);
+    rc = mdff_get_forces(c, fx, fy, fz);
+    if (rc) fail("mdff_get_forces", rc);
+    if (lnmlj) {
+        u_lj = th.u_lj;
+        pvir_lj = th.pvir_lj;
+        copy_tau(tau_lj, th.tau_lj);
+    }
+    if (lbhmft || lbhmftd) {
+        u_bhmftd = th.u_bhmftd;
+        pvir_bhmftd = th.pvir_bhmftd;
+        copy_tau(tau_bhmftd, th.tau_bhmftd);
+    }
+    if (lcoulombic) {
+        u_coul = th.u_coul;
+        pvir_coul = th.pvir_coul;
+        copy_tau(tau_coul, th.tau_coul);
+        u_pol = th.u_pol;
+        if (lpol) {
+            ldip = true;
+            /* the dipole Ewald calls the reference's SCF would have made */
+            cc_multipole_ES_dir_pim += th.n_scf + 1;
+            cc_multipole_ES_rec_pim += th.n_scf + 1;
+            store_induced(c);
+        }
+        for (int it = 0; it < ntype; it++)
+            for (int j = 0; j < 3; j++)
+                if (dipit[it][j] != 0.0) ldip = true;
+    }
+}
+
+static void multipole_ES_impl(double *q, double (*mu)[3], double (*theta)[3][3], double *u, double *pvir,
+                              double tau[3][3], double (*ef)[3], double (*efg)[3][3], bool lq, bool ld, bool lt,
+                              bool ldamp, bool do_forces, bool do_stress, bool do_ef, bool do_efg, bool do_dir,
+                              bool do_rec, bool inpim, bool update_sf)
+{
+    mdff_ctx *c = ctx_sync();
+    int flags = (lq ? MDFF_ES_LQ : 0) | (ld ? MDFF_ES_LD : 0) | (lt ? MDFF_ES_LT : 0) | (ldamp ? MDFF_ES_LDAMP : 0) |
+                (do_forces ? MDFF_ES_FORCES : 0) | (do_stress ? MDFF_ES_STRESS : 0) | (do_ef ? MDFF_ES_EF : 0) |
+                (do_efg ? MDFF_ES_EFG : 0) | (do_dir ? MDFF_ES_DIR : 0) | (do_rec ? MDFF_ES_REC : 0) |
+                (inpim ? MDFF_ES_INPIM : 0) | (update_sf ? MDFF_ES_UPDATE_SF : 0);
+    double t9[9];
+    int rc = mdff_multipole_es(c, q, &mu[0][0], theta ? &theta[0][0][0] : NULL, flags, u, pvir, t9,
+                               ef ? &ef[0][0] : NULL, efg ? &efg[0][0][0] : NULL, do_forces ? fx : NULL,
+                               do_forces ? fy : NULL, do_forces ? fz : NULL);
+    if (rc) fail("mdff_multipole_es", rc);
+    for (int i = 0; i < 9; i++) tau[i / 3][i % 3] = t9[i];
+}
+
+static void get_dipoles_impl(double (*mu)[3], double *upol)
+{
+    mdff_ctx *c = ctx_sync();
+    int rc = mdff_get_dipoles_scf(c, istep, &mu[0][0], upol);
+    if (rc && rc != MDFF_ERR_SCF) fail("mdff_get_dipoles_scf", rc);
+    if (lpol) ldip = true;
+}
+
+static void momentpolaSCF_impl(double (*mu)[3], double *upol)
+{
+    mdff_ctx *c = ctx_sync();
+    int rc = mdff_get_dipoles_scf(c, istep, &mu[0][0], upol);
+    if (rc && rc != MDFF_ERR_SCF) fail("mdff_get_dipoles_scf", rc);
+    for (int ia = 0; ia < nion; ia++)
+        for (int i = 0; i < 3; i++) mu[ia][i] -= dipia[ia][i];
+}
+
+static void pair_part(double *u, double *pvir, double tau[3][3])
+{
+    mdff_ctx *c = ctx_sync();
+    double t9[9];
+    int rc = mdff_pair_short(c, u, pvir, t9);
+    if (rc) fail("mdff_pair_short", rc);
+    for (int i = 0; i < 9; i++) tau[i / 3][i % 3] = t9[i];
+    double *f = (double *)malloc(sizeof(double) * 3 * (size_t)nion);
+    if (!f) fail("malloc", MDFF_ERR_NOMEM);
+    rc = mdff_get_forces(c, f, f + nion, f + 2 * nion);
+    if (rc) fail("mdff_get_forces", rc);
+    for (int ia = 0; ia < nion; ia++) {
+        fx[ia] += f[ia];
+        fy[ia] += f[nion + ia];
+        fz[ia] += f[2 * nion + ia];
+    }
+    free(f);
+}
+
+void engforce_bhmftd_pbc(double *u, double *pvir, double tau[3][3])
+{
+    char *rs = rng_enter();
+    pair_part(u, pvir, tau);
+    rng_leave(rs);
+}
+
+void engforce_nmlj_pbc(double *u, double *pvir, double tau[3][3])
+{
+    char *rs = rng_enter();
+    pair_part(u, pvir, tau);
+    rng_leave(rs);
+}
+
+void TT_damping_functions(double b, double c, double r, double *f, double *fd, int order)
+{
+    mdff_tt_damping(b, c, r, f, fd, order);
+}
+
+static void check_verletlist_impl(void)
+{
+    mdff_ctx *c = ctx_sync();
+    int rc = mdff_check_verlet(c);
+    if (rc) fail("mdff_check_verlet", rc);
+    mdff_thermo th;
+    rc = mdff_get_thermo(c, &th);
+    if (rc) fail("mdff_get_thermo", rc);
+    updatevl = th.verlet_updates;
+}
+
+void gen_pbc_verletlist(void) { check_verletlist(); }
+
+/* ---- the same-name entry points: the bodies above on a private rand() state */
+void engforce(void)
+{
+    char *rs = rng_enter();
+    engforce_impl();
+    rng_leave(rs);
+}
+
+void multipole_ES(double *q, double (*mu)[3], double (*theta)[3][3], double *u, double *pvir, double tau[3][3],
+                  double (*ef)[3], double (*efg)[3][3], bool lq, bool ld, bool lt, bool ldamp, bool do_forces,
+                  bool do_stress, bool do_ef, bool do_efg, bool do_dir, bool do_rec, bool inpim, bool update_sf)
+{
+    char *rs = rng_enter();
+    multipole_ES_impl(q, mu, theta, u, pvir, tau, ef, efg, lq, ld, lt, ldamp, do_forces, do_stress, do_ef, do_efg,
+                      do_dir, do_rec, inpim, update_sf);
+    rng_leave(rs);
+}
+
+void get_dipoles(double (*mu)[3], double *upol)
+{
+    char *rs = rng_enter();
+    get_dipoles_impl(mu, upol);
+    rng_leave(rs);
+}
+
+void momentpolaSCF(double (*mu)[3], double *upol)
+{
+    char *rs = rng_enter();
+    momentpolaSCF_impl(mu, upol);
+    rng_leave(rs);
+}
+
+void check_verletlist(void)
+{
+    char *rs = rng_enter();
+    check_verletlist_impl();
+    rng_leave(rs);
+}

@@ -113,6 +113,9 @@ static void dump_state(void)
 
 static int scf_calls(void) { return cc_multipole_ES_dir_pim; }
 
+/* present when the driver is linked with the same-name drop-ins */
+extern int mdff_dropin_params(void *p) __attribute__((weak));
+
 static double wall(void) { return omp_get_wtime(); }
 
 int main(int argc, char **argv)
@@ -147,6 +150,23 @@ int main(int argc, char **argv)
     fprintf(out, "alphaES %.17g\n", alphaES);
     fprintf(out, "kES %d %d %d\n", kES[0], kES[1], kES[2]);
     fprintf(out, "nk %d\n", kcoul.nk);
+    if (!strcmp(mode, "params")) {
+        /* the drop-ins' engine parameters, as raw bytes in hex */
+        if (!mdff_dropin_params) {
+            fprintf(stderr, "params: not linked with the drop-ins\n");
+            return 2;
+        }
+        unsigned char buf[1 << 20];
+        memset(buf, 0, sizeof(buf));
+        if (mdff_dropin_params(buf) != 0) return 4;
+        size_t nb = argc > 3 ? (size_t)atol(argv[3]) : 0;
+        if (nb == 0 || nb > sizeof(buf)) return 2;
+        fprintf(out, "params %zu ", nb);
+        for (size_t i = 0; i < nb; i++) fprintf(out, "%02x", buf[i]);
+        fprintf(out, "\n");
+        fclose(out);
+        return 0;
+    }
     if (lverletL) check_verletlist();
 
     if (!strcmp(mode, "static")) {

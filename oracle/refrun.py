@@ -14,16 +14,20 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 DRIVER = os.path.join(_HERE, "_ref", "mdff20_ref_driver")
 DRIVER_O3 = os.path.join(_HERE, "_ref", "mdff20_ref_driver_O3")
+# the same driver with the force path replaced by the same-name drop-ins
+DRIVER_DROPIN = os.path.join(_HERE, "_ref", "mdff20_dropin_driver")
 
 
 def available(o3=False):
     return os.path.exists(DRIVER_O3 if o3 else DRIVER)
 
 
-def run(workdir, mode, *args, threads=1, o3=False, timeout=3600):
+def run(workdir, mode, *args, threads=1, o3=False, timeout=3600, exe=None, env_extra=None):
     env = dict(os.environ)
     env["OMP_NUM_THREADS"] = str(threads)
-    exe = DRIVER_O3 if o3 else DRIVER
+    if env_extra:
+        env.update(env_extra)
+    exe = exe or (DRIVER_O3 if o3 else DRIVER)
     subprocess.run([exe, workdir, mode] + [str(a) for a in args], check=True, env=env, timeout=timeout)
     return parse(os.path.join(workdir, "ref_out.txt"))
 
@@ -42,7 +46,9 @@ def parse(path):
                 cur = {"step": int(t[1])}
                 steps.append(cur)
                 continue
-            if key in ("n_scf", "verlet_updates", "nk", "threads", "n_induced"):
+            if key == "params":
+                cur[key] = t[2]
+            elif key in ("n_scf", "verlet_updates", "nk", "threads", "n_induced"):
                 cur[key] = int(t[1])
             elif key == "kES":
                 cur[key] = [int(x) for x in t[1:4]]
