@@ -122,11 +122,22 @@ int mdff_md_run(mdff_ctx *ctx, int first_step, int nsteps, mdff_thermo *th);
 void mdff_tt_damping(double b, double c, double r, double *f, double *fd, int order);
 
 /* Tuning: lanes per ion in the neighbour kernels (16, 32 or 64; 0 keeps),
- * reciprocal-space kernel family katoms (-1 vector ALU; 1 matrix cores with
- * split-BF16 SCF correction sweeps, the default; 2 matrix cores with FP32
- * sweeps; 0 keeps) and ksplit, the target block count of the matrix-core
- * kernels (0 keeps).  No counterpart in the reference. */
+ * reciprocal-space kernel family katoms (-1 vector ALU; 1 matrix-core sums
+ * over operands stored once per force call, split-BF16 SCF correction
+ * sweeps, the default; 2 the same with FP32 sweeps; 3 matrix cores with
+ * operands formed in LDS on every call; 0 keeps) and ksplit, the target
+ * block count of the matrix-core kernels (0 keeps).  No counterpart in the
+ * reference. */
 int mdff_set_tuning(mdff_ctx *ctx, int lanes, int katoms, int ksplit);
+
+/* Shape of the matrix-core k-space sums over stored operands: column tiles
+ * of 16 points per wavefront in the forward sum for the FP64 mode (cw64) and
+ * the FP32 / split-BF16 modes (cw32), each 1, 2 or 4, and the target block
+ * counts of the forward and back-projection sums; 0 keeps a value, and a
+ * block count of 0 (the default) sizes each grid to one round of the blocks
+ * the device holds at once.  Default column tiles: 1 (FP64), 2 (others).
+ * No counterpart in the reference (tuning). */
+int mdff_set_kspace_shape(mdff_ctx *ctx, int cw64, int cw32, int fwd_blocks, int back_blocks);
 
 /* Device timing of the last engforce by phase, in ms (HIP events on the
  * context stream): [0] short range, [1] static field, [2] SCF loop,

@@ -77,6 +77,7 @@ def lib():
         "mdff_md_step": (C.c_int, [vp, C.c_int, P(Thermo)]),
         "mdff_md_run": (C.c_int, [vp, C.c_int, C.c_int, P(Thermo)]),
         "mdff_set_tuning": (C.c_int, [vp, C.c_int, C.c_int, C.c_int]),
+        "mdff_set_kspace_shape": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, C.c_int]),
         "mdff_get_timing": (C.c_int, [vp, dp]),
         "mdff_time_kernel": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, dp]),
         "mdff_get_counters": (C.c_int, [vp, P(C.c_longlong)]),
@@ -291,6 +292,10 @@ class Engine:
 
     def set_tuning(self, lanes=0, katoms=0, ksplit=0):
         _check(self._L.mdff_set_tuning(self._h, lanes, katoms, ksplit), "set_tuning", False)
+
+    def set_kspace_shape(self, cw64=0, cw32=0, fwd_blocks=0, back_blocks=0):
+        _check(self._L.mdff_set_kspace_shape(self._h, cw64, cw32, fwd_blocks, back_blocks), "set_kspace_shape",
+               False)
 
     # ---- compute
     def init_velocities(self):

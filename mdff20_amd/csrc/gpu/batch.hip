@@ -259,8 +259,11 @@ static int build_list(mdff_batch *b)
         if (!ov) return MDFF_OK;
         bfree(b->nbr);
         b->cap = (ov + ov / 4 + 63) / 64 * 64;
-        int rc = balloc(&b->nbr, (size_t)n * b->cap);
-        if (rc) return rc;
+        // no zero fill: the build writes every entry below each ion's count
+        if (hipMalloc((void **)&b->nbr, sizeof(unsigned) * (size_t)n * b->cap) != hipSuccess) {
+            b->nbr = nullptr;
+            return MDFF_ERR_NOMEM;
+        }
     }
     return MDFF_ERR_LIST_OVERFLOW;
 }
@@ -679,6 +682,8 @@ int mdff_batch_create(mdff_batch **out, const mdff_params *p, int nconf, const i
         return MDFF_ERR_DEVICE;
     }
     if ((rc = build_items(b)) != MDFF_OK) { mdff_batch_destroy(b); return rc; }
+    // every zero fill of the set-up (null stream) is done before the batch stream runs
+    if (hipDeviceSynchronize() != hipSuccess) { mdff_batch_destroy(b); return MDFF_ERR_DEVICE; }
     *out = b;
     return MDFF_OK;
 }
@@ -899,7 +904,10 @@ int mdff_batch_set_tuning(mdff_batch *b, int lanes, int kpb, int ipb, int bpc)
     if (ipb) { if (ipb != 64 && ipb != 128 && ipb != 256) return MDFF_ERR_ARG; b->ipb = ipb; }
     if (bpc) { if (bpc < 1 || bpc > 32) return MDFF_ERR_ARG; b->bpc = bpc; }
     b->tuned = 1;
-    return build_items(b);
+    int rc = build_items(b);
+    if (rc) return rc;
+    BCHK(hipDeviceSynchronize());
+    return MDFF_OK;
 }
 
 // time `iters` runs of fn between events on the batch stream

@@ -15,6 +15,7 @@
 #include "kernels.h"
 #include "md_kernels.h"
 #include "mfma_kernels.h"
+#include "gemm_kernels.h"
 #include "typetab.h"
 #include "mdff_gpu.h"
 #include "mdff_host.h"
@@ -88,6 +89,11 @@ struct mdff_ctx {
     int *ctl = nullptr;
     TypeTab *tab_d = nullptr;
     int *pidx = nullptr;
+    // stored dipole tensors of the SCF real space (k_t2_build / k_t2_field)
+    int *pinv = nullptr, *cntp = nullptr, *idxp = nullptr, *t2_ovf = nullptr;
+    double *t2 = nullptr;
+    float *t2f = nullptr;
+    int capP = 0, t2_ok = 0;
     // neighbour list
     unsigned *nbr = nullptr;
     int *cnt = nullptr;
@@ -102,11 +108,27 @@ struct mdff_ctx {
     void *tab3[2][2][3] = {{{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}},
                            {{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}}};
     double *kpart = nullptr, *bpart = nullptr;
+    double *ksum = nullptr;   // [nkf][8] chunk-summed forward partials of the matrix-core path
+    int fwd_part = 0;         // last forward launch: 0 partials in kpart (k_sk_reduce sums them), 1 summed into ksum
     int nchunk = 0, nsplit = 0, chunk = 0, rows_per_split = 0;
     int nchunk_p = 0, chunk_p = 0;
     // tuning
     int lanes = 32;
     int use_mfma = 1;         // S(k) forward and back-projection on the matrix cores
+    int gemm = 1;             // matrix-core sums over stored operands (gemm_kernels.h)
+    int g_cw[2] = {1, 2};     // column tiles per wave of the forward GEMM (FP64, FP32/BF16)
+    int g_fblocks = 0;        // target blocks of the forward GEMM (0: one round of resident blocks)
+    int g_bblocks = 0;        // target blocks of the back-projection GEMM (0: one round)
+    int ncu = 256;            // compute units of the device
+    struct OccEntry { const void *fn; size_t lds; int slots; };
+    std::vector<OccEntry> occ_cache;
+    int nselp[2] = {0, 0};    // ions padded to 128 (all, polarisable)
+    int Pp = 0, Pk = 0;       // (ny,nz) points padded: forward rows, back-projection K
+    void *yz[3][2][2] = {};   // [mode][polar][hi,lo]: YZ[p][2j+ri]
+    void *yzt[3][2][2] = {};  // [mode][polar][hi,lo]: YZt[j][2p+ri]
+    int yz_ok[3][2] = {};     // stored operands match the current tables
+    void *aw[3][2] = {};      // [mode][hi,lo]: rows (nx,c,re/im) of the forward sum
+    void *wm[3][2] = {};      // [mode][hi,lo]: columns (nx,m,re/im) of the back sum
     int mf_lo = KM_F32;       // matrix-core mode of the FP32-table evaluations
     int mf_inner = KM_BF16X3; // mode of the mixed-form SCF correction sweeps
     int mf_blocks = 512;      // target blocks of the MFMA kernels
@@ -190,7 +212,11 @@ static int build_list(mdff_ctx *c, int force)
         int ncap = ((need * 5) / 4 + 64 + 63) / 64 * 64;
         if (ncap > ((n + 63) / 64) * 64) ncap = ((n + 63) / 64) * 64;
         dfree(c->nbr);
-        if (dalloc(&c->nbr, (size_t)n * ncap)) return MDFF_ERR_NOMEM;
+        // no zero fill: the build writes every entry below each ion's count
+        if (hipMalloc((void **)&c->nbr, sizeof(unsigned) * (size_t)n * ncap) != hipSuccess) {
+            c->nbr = nullptr;
+            return MDFF_ERR_NOMEM;
+        }
         c->cap = ncap;
     }
     return MDFF_ERR_LIST_OVERFLOW;
@@ -270,6 +296,9 @@ static void launch_eikr_t(mdff_ctx *c, const int *idx, int nsel, void *tx, void 
 
 static void launch_eikr(mdff_ctx *c, int polar, int lo, const int *active)
 {
+    // stored operands built from these tables are stale now
+    if (lo) { c->yz_ok[KM_F32][polar] = 0; c->yz_ok[KM_BF16X3][polar] = 0; }
+    else c->yz_ok[KM_F64][polar] = 0;
     void **t = c->tab3[lo][polar];
     const int *idx = polar ? c->pidx : nullptr;
     int nsel = polar ? c->npol : c->n;
@@ -309,11 +338,16 @@ static int fwd_mfma_t(mdff_ctx *c, SkMArgs &m, int nsel)
     return nchunk;
 }
 
+static int launch_fwd_gemm(mdff_ctx *c, int polar, int lo, int C, const double *q_w, const double *mu_w,
+                           const int *active, int *nchunk_out);
+static int launch_back_gemm(mdff_ctx *c, int polar, int lo, int NCH, const int *active, int *nsplit_out);
+
 static int launch_fwd_mfma(mdff_ctx *c, int polar, int lo, int C, const double *q_w, const double *mu_w,
                            const int *active, int *nchunk_out)
 {
     int M = (c->Kx + 1) * C * 2;
     if (!c->use_mfma || M > MF_MT * 16) return 0;
+    if (c->gemm && launch_fwd_gemm(c, polar, lo, C, q_w, mu_w, active, nchunk_out)) return 1;
     int nsel = polar ? c->npol : c->n;
     SkMArgs m;
     m.nsel = nsel; m.Kx = c->Kx; m.Ky = c->Ky; m.Kz = c->Kz; m.P = c->P; m.C = C; m.M = M;
@@ -348,6 +382,7 @@ static int launch_back_mfma(mdff_ctx *c, int polar, int lo, int NCH, const int *
     int CH = NCH == 9 ? 6 : 3;
     int NC = (c->Kx + 1) * CH * 2;
     if (!c->use_mfma || NC > MB_CT * 16) return 0;
+    if (c->gemm && launch_back_gemm(c, polar, lo, NCH, active, nsplit_out)) return 1;
     int nsel = polar ? c->npol : c->n;
     BkMArgs b;
     b.nsel = nsel; b.Kx = c->Kx; b.Ky = c->Ky; b.Kz = c->Kz; b.P = c->P; b.CH = CH; b.NC = NC;
@@ -362,12 +397,264 @@ static int launch_back_mfma(mdff_ctx *c, int polar, int lo, int NCH, const int *
     return 1;
 }
 
+// ------------------------------------------------ stored-operand matrix-core sums
+static size_t g_elt(int mode) { return mode == KM_F64 ? 8 : (mode == KM_F32 ? 4 : 2); }
+
+// allocation made while the context works: zeroed in stream order, since a
+// null-stream memset is not ordered with the non-blocking context stream
+static int galloc(mdff_ctx *c, void **p, size_t bytes)
+{
+    if (*p) return MDFF_OK;
+    if (hipMalloc(p, bytes) != hipSuccess) { *p = nullptr; return MDFF_ERR_NOMEM; }
+    if (hipMemsetAsync(*p, 0, bytes, c->st) != hipSuccess) return MDFF_ERR_DEVICE;
+    return MDFF_OK;
+}
+
+// YZ and YZt of one table set in one mode, built once per table rebuild
+static int ensure_yz(mdff_ctx *c, int mode, int polar, int lo)
+{
+    if (c->yz_ok[mode][polar]) return MDFF_OK;
+    const int nsel = polar ? c->npol : c->n;
+    const int nselp = c->nselp[polar];
+    const size_t e = g_elt(mode);
+    const size_t nb1 = (size_t)c->Pp * 2 * nselp * e, nb2 = (size_t)nselp * 2 * c->Pk * e;
+    int rc;
+    if ((rc = galloc(c, &c->yz[mode][polar][0], nb1)) || (rc = galloc(c, &c->yzt[mode][polar][0], nb2))) return rc;
+    if (mode == KM_BF16X3 &&
+        ((rc = galloc(c, &c->yz[mode][polar][1], nb1)) || (rc = galloc(c, &c->yzt[mode][polar][1], nb2))))
+        return rc;
+    const int NZ = 2 * c->Kz + 1;
+    void **t = c->tab3[lo][polar];
+    void *y0 = c->yz[mode][polar][0], *y1 = c->yz[mode][polar][1];
+    void *z0 = c->yzt[mode][polar][0], *z1 = c->yzt[mode][polar][1];
+    dim3 g1(nselp / 64, (std::max(c->Pp, c->Pk) + 31) / 32);
+#define YZL(M, TT)                                                                                            \
+    hipLaunchKernelGGL((k_yz_build<M, TT>), g1, dim3(256), 0, c->st, nsel, nselp, c->P, c->Pp, c->Pk, NZ,   \
+                       (const TT *)t[1], (const TT *)t[2], y0, y1, z0, z1)
+    if (mode == KM_F64) YZL(KM_F64, double);
+    else if (mode == KM_F32) YZL(KM_F32, float);
+    else YZL(KM_BF16X3, float);
+#undef YZL
+    c->yz_ok[mode][polar] = 1;
+    return MDFF_OK;
+}
+
+static size_t g_fwd_lds_bytes_m(int mode, int MT)
+{
+    return mode == KM_F64 ? g_fwd_lds_bytes<KM_F64>(MT)
+                          : (mode == KM_F32 ? g_fwd_lds_bytes<KM_F32>(MT) : g_fwd_lds_bytes<KM_BF16X3>(MT));
+}
+
+static size_t g_back_lds_bytes_m(int mode, int CT)
+{
+    return mode == KM_F64 ? g_back_lds_bytes<KM_F64>(CT)
+                          : (mode == KM_F32 ? g_back_lds_bytes<KM_F32>(CT) : g_back_lds_bytes<KM_BF16X3>(CT));
+}
+
+template <int MODE, int CW>
+static const void *gfwd_kernel_t(int MT)
+{
+    if (MT <= 3) return (const void *)k_gemm_fwd<MODE, CW, 3>;
+    if (MT <= 5) return (const void *)k_gemm_fwd<MODE, CW, 5>;
+    if (MT <= 7) return (const void *)k_gemm_fwd<MODE, CW, 7>;
+    return (const void *)k_gemm_fwd<MODE, CW, MF_MT>;
+}
+
+template <int MODE>
+static const void *gfwd_kernel_m(int CW, int MT)
+{
+    if (CW == 1) return gfwd_kernel_t<MODE, 1>(MT);
+    if (CW == 4) return gfwd_kernel_t<MODE, 4>(MT);
+    return gfwd_kernel_t<MODE, 2>(MT);
+}
+
+static const void *gfwd_kernel(int mode, int CW, int MT)
+{
+    if (mode == KM_F64) return gfwd_kernel_m<KM_F64>(CW, MT);
+    if (mode == KM_F32) return gfwd_kernel_m<KM_F32>(CW, MT);
+    return gfwd_kernel_m<KM_BF16X3>(CW, MT);
+}
+
+static const void *gback_kernel(int mode, int CT)
+{
+    if (mode == KM_F64)
+        return CT <= 7 ? (const void *)k_gemm_back<KM_F64, 7, double> : (const void *)k_gemm_back<KM_F64, MB_CT, double>;
+    if (mode == KM_F32)
+        return CT <= 7 ? (const void *)k_gemm_back<KM_F32, 7, float> : (const void *)k_gemm_back<KM_F32, MB_CT, float>;
+    return CT <= 7 ? (const void *)k_gemm_back<KM_BF16X3, 7, float> : (const void *)k_gemm_back<KM_BF16X3, MB_CT, float>;
+}
+
+// number of K splits for gx blocks per split: the smallest that fills at
+// least one round of the resident blocks at >= 95% (else the fullest)
+static int g_split(int gx, int slots, int maxs)
+{
+    int best = 1;
+    double beste = -1.0;
+    for (int ns = 1; ns <= maxs; ns++) {
+        long long b = (long long)gx * ns;
+        long long rounds = (b + slots - 1) / slots;
+        double eff = (double)b / (double)(rounds * slots);
+        if (b >= slots && eff >= 0.95) return ns;
+        if (eff > beste + 1e-9) { beste = eff; best = ns; }
+    }
+    return best;
+}
+
+// resident blocks of a kernel over the whole device (compute units times
+// blocks per unit), cached per kernel
+static int g_slots(mdff_ctx *c, const void *fn, int threads, size_t lds)
+{
+    for (auto &e : c->occ_cache)
+        if (e.fn == fn && e.lds == lds) return e.slots;
+    int per = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, fn, threads, lds) != hipSuccess || per < 1) per = 1;
+    int slots = per * c->ncu;
+    c->occ_cache.push_back({fn, lds, slots});
+    return slots;
+}
+
+// launch with the smallest instantiated tile count that holds the shape
+template <int MODE, int CW>
+static void gfwd_mt(mdff_ctx *c, dim3 grid, int MT, const GFwdArgs &a)
+{
+    const size_t lds = g_fwd_lds_bytes<MODE>(MT);
+    if (MT <= 3) hipLaunchKernelGGL((k_gemm_fwd<MODE, CW, 3>), grid, dim3(256), lds, c->st, a);
+    else if (MT <= 5) hipLaunchKernelGGL((k_gemm_fwd<MODE, CW, 5>), grid, dim3(256), lds, c->st, a);
+    else if (MT <= 7) hipLaunchKernelGGL((k_gemm_fwd<MODE, CW, 7>), grid, dim3(256), lds, c->st, a);
+    else hipLaunchKernelGGL((k_gemm_fwd<MODE, CW, MF_MT>), grid, dim3(256), lds, c->st, a);
+}
+
+template <int MODE>
+static void gfwd_cw(mdff_ctx *c, dim3 grid, int CW, int MT, const GFwdArgs &a)
+{
+    if (CW == 1) gfwd_mt<MODE, 1>(c, grid, MT, a);
+    else if (CW == 4) gfwd_mt<MODE, 4>(c, grid, MT, a);
+    else gfwd_mt<MODE, 2>(c, grid, MT, a);
+}
+
+template <int MODE, typename TT>
+static void gback_ct(mdff_ctx *c, dim3 grid, int CT, const GBackArgs &b)
+{
+    const size_t lds = g_back_lds_bytes<MODE>(CT);
+    if (CT <= 7) hipLaunchKernelGGL((k_gemm_back<MODE, 7, TT>), grid, dim3(512), lds, c->st, b);
+    else hipLaunchKernelGGL((k_gemm_back<MODE, MB_CT, TT>), grid, dim3(512), lds, c->st, b);
+}
+
+// forward sum over stored operands; 0 when outside the kernel's shapes
+static int launch_fwd_gemm(mdff_ctx *c, int polar, int lo, int C, const double *q_w, const double *mu_w,
+                           const int *active, int *nchunk_out)
+{
+    const int M = (c->Kx + 1) * C * 2;
+    const int MT = (M + 15) / 16;
+    if (MT > MF_MT) return 0;
+    const int mode = lo ? c->mf_lo : KM_F64;
+    if (ensure_yz(c, mode, polar, lo)) return 0;
+    const int nsel = polar ? c->npol : c->n;
+    const int nselp = c->nselp[polar];
+    const size_t e = g_elt(mode);
+    const size_t awb = (size_t)16 * MF_MT * 2 * std::max(c->nselp[0], c->nselp[1]) * e;
+    if (galloc(c, &c->aw[mode][0], awb)) return 0;
+    if (mode == KM_BF16X3 && galloc(c, &c->aw[mode][1], awb)) return 0;
+    void **t = c->tab3[lo][polar];
+    const int *idx = polar ? c->pidx : nullptr;
+    dim3 ga(cdiv(nselp, 256), c->Kx + 1);
+    if (mode == KM_F64)
+        hipLaunchKernelGGL((k_aw_build<KM_F64, double>), ga, dim3(256), 0, c->st, nsel, nselp, C, (const double *)t[0],
+                           idx, q_w, mu_w, c->n, c->aw[mode][0], c->aw[mode][1], active);
+    else if (mode == KM_F32)
+        hipLaunchKernelGGL((k_aw_build<KM_F32, float>), ga, dim3(256), 0, c->st, nsel, nselp, C, (const float *)t[0],
+                           idx, q_w, mu_w, c->n, c->aw[mode][0], c->aw[mode][1], active);
+    else
+        hipLaunchKernelGGL((k_aw_build<KM_BF16X3, float>), ga, dim3(256), 0, c->st, nsel, nselp, C,
+                           (const float *)t[0], idx, q_w, mu_w, c->n, c->aw[mode][0], c->aw[mode][1], active);
+    GFwdArgs a;
+    a.aw = c->aw[mode][0]; a.aw_lo = c->aw[mode][1];
+    a.yz = c->yz[mode][polar][0]; a.yz_lo = c->yz[mode][polar][1];
+    a.MT = MT; a.M = M; a.P = c->P; a.C = C; a.ldk = 2 * nselp;
+    a.part = c->kpart; a.active = active;
+    const int CW = c->g_cw[mode == KM_F64 ? 0 : 1];
+    const int gx = (c->P + 64 * CW - 1) / (64 * CW);
+    // split K so that the grid fills whole rounds of the resident blocks
+    int slots = g_slots(c, (const void *)gfwd_kernel(mode, CW, MT), 256, g_fwd_lds_bytes_m(mode, MT));
+    int nchunk = c->g_fblocks > 0 ? std::max(1, std::min(64, c->g_fblocks / gx)) : g_split(gx, slots, 64);
+    a.kchunk = ((a.ldk + nchunk - 1) / nchunk + GK - 1) / GK * GK;
+    nchunk = (a.ldk + a.kchunk - 1) / a.kchunk;
+    dim3 grid(gx, nchunk);
+    if (mode == KM_F64) gfwd_cw<KM_F64>(c, grid, CW, MT, a);
+    else if (mode == KM_F32) gfwd_cw<KM_F32>(c, grid, CW, MT, a);
+    else gfwd_cw<KM_BF16X3>(c, grid, CW, MT, a);
+    // chunk sums, all threads busy (the per-k reduction then reads one chunk)
+    const size_t nval = c->nkf * 2 * C;
+    if (mode == KM_F64)
+        hipLaunchKernelGGL(k_part_sum<double>, dim3(cdiv(nval, 256)), dim3(256), 0, c->st, (const double *)c->kpart,
+                           nchunk, nval, c->ksum, active);
+    else
+        hipLaunchKernelGGL(k_part_sum<float>, dim3(cdiv(nval, 256)), dim3(256), 0, c->st, (const float *)c->kpart,
+                           nchunk, nval, c->ksum, active);
+    c->fwd_part = 1;
+    nchunk = 1;
+    *nchunk_out = nchunk;
+    return 1;
+}
+
+// back-projection over stored operands; 0 when outside the kernel's shapes
+static int launch_back_gemm(mdff_ctx *c, int polar, int lo, int NCH, const int *active, int *nsplit_out)
+{
+    const int CH = NCH == 9 ? 6 : 3;
+    const int NC = (c->Kx + 1) * CH * 2;
+    const int CT = (NC + 15) / 16;
+    if (CT > MB_CT) return 0;
+    const int mode = lo ? c->mf_lo : KM_F64;
+    if (ensure_yz(c, mode, polar, lo)) return 0;
+    const int nsel = polar ? c->npol : c->n;
+    const int nselp = c->nselp[polar];
+    const size_t e = g_elt(mode);
+    const size_t wmb = (size_t)16 * MB_CT * 2 * c->Pk * e;
+    if (galloc(c, &c->wm[mode][0], wmb)) return 0;
+    if (mode == KM_BF16X3 && galloc(c, &c->wm[mode][1], wmb)) return 0;
+    dim3 gw(cdiv(c->Pk, 256), (c->Kx + 1) * CH);
+    if (mode == KM_F64)
+        hipLaunchKernelGGL(k_wm_build<KM_F64>, gw, dim3(256), 0, c->st, c->P, c->Pk, c->Ky, c->Kz, CH, c->W,
+                           c->wm[mode][0], c->wm[mode][1], active);
+    else if (mode == KM_F32)
+        hipLaunchKernelGGL(k_wm_build<KM_F32>, gw, dim3(256), 0, c->st, c->P, c->Pk, c->Ky, c->Kz, CH, c->W,
+                           c->wm[mode][0], c->wm[mode][1], active);
+    else
+        hipLaunchKernelGGL(k_wm_build<KM_BF16X3>, gw, dim3(256), 0, c->st, c->P, c->Pk, c->Ky, c->Kz, CH, c->W,
+                           c->wm[mode][0], c->wm[mode][1], active);
+    GBackArgs b;
+    b.yzt = c->yzt[mode][polar][0]; b.yzt_lo = c->yzt[mode][polar][1];
+    b.wm = c->wm[mode][0]; b.wm_lo = c->wm[mode][1];
+    b.tx = c->tab3[lo][polar][0];
+    b.nsel = nsel; b.CH = CH; b.NC = NC; b.CT = CT; b.ldk = 2 * c->Pk;
+    b.part = c->bpart; b.active = active;
+    const int gx = nselp / 128;
+    int slots = g_slots(c, (const void *)gback_kernel(mode, CT), 512, g_back_lds_bytes_m(mode, CT));
+    int ns = c->g_bblocks > 0 ? std::max(1, std::min(c->nsplit_max, c->g_bblocks / gx))
+                              : g_split(gx, slots, c->nsplit_max);
+    b.kchunk = ((b.ldk + ns - 1) / ns + GK - 1) / GK * GK;
+    ns = (b.ldk + b.kchunk - 1) / b.kchunk;
+    dim3 grid(gx, ns);
+    if (mode == KM_F64) gback_ct<KM_F64, double>(c, grid, CT, b);
+    else if (mode == KM_F32) gback_ct<KM_F32, float>(c, grid, CT, b);
+    else gback_ct<KM_BF16X3, float>(c, grid, CT, b);
+    *nsplit_out = ns;
+    return 1;
+}
+
 // reciprocal part of multipole_ES (ewald.c:876-1051) on the selected ions.
 // polar: use the polarisable-ion tables (SCF); C: channels of S(k);
 // NCH: 3 (field) or 9 (field + gradient); do_ef/do_f outputs; efg optional.
+// total field formed in the back-projection's last kernel (self + dir +
+// rec + base), instead of a separate k_field_sum
+struct FieldFuse {
+    double *ef;
+    const double *mu, *base;
+};
+
 static void rec_part(mdff_ctx *c, int polar, int lo, int C, int NCH, const double *q_w, const double *mu_w,
                      const double *q_f, const double *mu_f, int do_ef, int do_f, int do_stress, double *efg_out,
-                     const int *active)
+                     const int *active, const FieldFuse *ff = nullptr)
 {
     int nsel = polar ? c->npol : c->n;
     int chunk = polar ? c->chunk_p : c->chunk;
@@ -384,6 +671,7 @@ static void rec_part(mdff_ctx *c, int polar, int lo, int C, int NCH, const doubl
     a.part = c->kpart;
     a.active = active;
     bool mixed = lo != 0;
+    c->fwd_part = 0;
     if (!launch_fwd_mfma(c, polar, lo, C, q_w, mu_w, active, &nchunk)) {
         if (C == 1) { if (mixed) launch_fwd_t<float, 1>(c, a, nchunk); else launch_fwd_t<double, 1>(c, a, nchunk); }
         else if (C == 3) { if (mixed) launch_fwd_t<float, 3>(c, a, nchunk); else launch_fwd_t<double, 3>(c, a, nchunk); }
@@ -391,7 +679,8 @@ static void rec_part(mdff_ctx *c, int polar, int lo, int C, int NCH, const doubl
     }
     KRedArgs r;
     r.Kx = c->Kx; r.Ky = c->Ky; r.Kz = c->Kz; r.P = c->P; r.nchunk = nchunk; r.C = C;
-    r.part = c->kpart; r.kvec = c->kvec; r.Ak = c->Ak; r.kcoe = c->kcoe; r.W = c->W; r.ek = c->ek;
+    r.part = c->fwd_part ? c->ksum : c->kpart; r.kvec = c->kvec;
+    c->fwd_part = 0; r.Ak = c->Ak; r.kcoe = c->kcoe; r.W = c->W; r.ek = c->ek;
     r.do_stress = do_stress; r.active = active;
     hipLaunchKernelGGL(k_sk_reduce, dim3(cdiv(c->nkf, 256)), dim3(256), 0, c->st, r);
     if (do_ef || do_f) {
@@ -413,6 +702,11 @@ static void rec_part(mdff_ctx *c, int polar, int lo, int C, int NCH, const doubl
         fa.q = q_f; fa.mu = mu_f;
         fa.ef = c->ef_rec; fa.efg = efg_out; fa.f = c->f_rec;
         fa.do_ef = do_ef; fa.do_f = do_f; fa.active = active;
+        fa.ef_total = ff ? ff->ef : nullptr;
+        fa.mu_self = ff ? ff->mu : nullptr;
+        fa.ef_dir = c->ef_dir;
+        fa.base = ff ? ff->base : nullptr;
+        fa.selfa2 = c->selfa2;
         hipLaunchKernelGGL(k_sk_back_fin, dim3(cdiv(nsel, 256)), dim3(256), 0, c->st, fa);
     }
 }
@@ -481,9 +775,49 @@ static int aspc_args(mdff_ctx *c, int istep, AspcArgs *asp)
 
 // induced field of dipoles mu (no charges, no damping, pim.c:258-260) at
 // the polarisable ions: ef = base + self + dir + rec
+// dipole tensors of all polarisable pairs at the current positions (per
+// force call, after k_frac); rows hold the list capacity, so they never
+// overflow
+static int build_t2(mdff_ctx *c)
+{
+    c->t2_ok = 0;
+    if (c->npol == 0) return MDFF_OK;
+    if (c->capP != c->cap) {
+        dfree(c->idxp);
+        dfree(c->t2);
+        if (c->t2f) { (void)hipFree(c->t2f); c->t2f = nullptr; }
+        size_t ne = (size_t)c->npol * c->cap;
+        if (hipMalloc((void **)&c->idxp, sizeof(int) * ne) != hipSuccess ||
+            hipMalloc((void **)&c->t2, sizeof(double) * 6 * ne) != hipSuccess)
+            return MDFF_ERR_NOMEM;
+        if (lo_of(c) && hipMalloc((void **)&c->t2f, sizeof(float) * 6 * ne) != hipSuccess) return MDFF_ERR_NOMEM;
+        c->capP = c->cap;
+    }
+    T2Args a;
+    a.n = c->n; a.cap = c->cap; a.capP = c->capP; a.c = c->cd; a.s = c->s; a.nbr = c->nbr; a.cnt = c->cnt;
+    a.pidx = c->pidx; a.pinv = c->pinv; a.alpha = c->alpha; a.rc2 = c->lrcutsq; a.npol = c->npol;
+    a.cntp = c->cntp; a.idxp = c->idxp; a.t2 = c->t2; a.t2f = lo_of(c) ? c->t2f : nullptr; a.overflow = c->t2_ovf;
+    hipLaunchKernelGGL(k_t2_build, dim3(cdiv(c->npol, 4)), dim3(256), 0, c->st, a);
+    c->t2_ok = 1;
+    return MDFF_OK;
+}
+
 static void induced_field(mdff_ctx *c, int lo, const double *mu, const double *base, double *ef, const int *act)
 {
     int n = c->n;
+    if (c->t2_ok) {
+        unsigned blocks = cdiv((size_t)n * 16, 256);
+        if (lo)
+            hipLaunchKernelGGL((k_t2_field<float, 16>), dim3(blocks), dim3(256), 0, c->st, n, c->capP, c->npol,
+                               c->pinv, c->cntp, c->idxp, c->t2f, mu, c->ef_dir, act);
+        else
+            hipLaunchKernelGGL((k_t2_field<double, 16>), dim3(blocks), dim3(256), 0, c->st, n, c->capP, c->npol,
+                               c->pinv, c->cntp, c->idxp, c->t2, mu, c->ef_dir, act);
+        // the total field at the polarisable ions comes out of the back-projection
+        FieldFuse ff{ef, mu, base};
+        rec_part(c, 1, lo, 3, 3, c->qz, mu, c->qz, mu, 1, 0, 0, nullptr, act, &ff);
+        return;
+    }
     DirArgs a = dir_args(c, c->qz, mu, 0, 1, act);
     launch_dir(c, a, lo, 1, 0, 0, 0);
     rec_part(c, 1, lo, 3, 3, c->qz, mu, c->qz, mu, 1, 0, 0, nullptr, act);
@@ -519,16 +853,13 @@ static int scf_loop(mdff_ctx *c, int istep)
             int first = launched == 0;
             hipLaunchKernelGGL(k_scf_mu, dim3(cdiv(n, 256)), dim3(256), 0, c->st, n, first, asp, c->type, c->tab_d,
                                c->hist, c->ef, c->mu_ind, c->upol_i, act);
-            colsum(c, c->upol_i, n, n, 1, c->scal + S_UPOL, act);
             induced_field(c, 0, c->mu_ind, c->ef_stat, c->ef, act);
             if (first)
                 hipLaunchKernelGGL(k_scf_correct, dim3(cdiv(n, 256)), dim3(256), 0, c->st, n, asp, c->type,
                                    c->tab_d, c->ef, c->mu_ind, act);
-            hipLaunchKernelGGL(k_scf_rmsd, dim3(cdiv(n, 256)), dim3(256), 0, c->st, n, c->type, c->tab_d, c->mu_ind,
-                               c->ef, c->rmsd_i, act);
-            colsum(c, c->rmsd_i, n, n, 1, c->scal + S_RMSD2, act);
-            hipLaunchKernelGGL(k_scf_control, dim3(1), dim3(1), 0, c->st, c->ctl, c->scal, c->npol, maxit, minit,
-                               c->p.conv_tol_ind);
+            // rmsd, u_pol of this iteration's dipoles and the loop condition
+            hipLaunchKernelGGL(k_scf_check, dim3(1), dim3(1024), 0, c->st, n, c->type, c->tab_d, c->mu_ind, c->ef,
+                               c->upol_i, c->ctl, c->scal, c->npol, maxit, minit, c->p.conv_tol_ind);
         }
         int rc = sync_scal(c);
         if (rc) return rc;
@@ -563,11 +894,8 @@ static int scf_loop_mixed(mdff_ctx *c, int istep)
         int chunk = launched == 0 ? std::max(1, std::min(minit, 2)) : 2;
         for (int it = 0; it < chunk && launched < limit; it++, launched++) {
             induced_field(c, 0, c->mu_ind, c->ef_stat, c->ef, act);
-            hipLaunchKernelGGL(k_scf_rmsd, dim3(cdiv(n, 256)), dim3(256), 0, c->st, n, c->type, c->tab_d, c->mu_ind,
-                               c->ef, c->rmsd_i, act);
-            colsum(c, c->rmsd_i, n, n, 1, c->scal + S_RMSD2, act);
-            hipLaunchKernelGGL(k_scf_control, dim3(1), dim3(1), 0, c->st, c->ctl, c->scal, c->npol, maxit, minit,
-                               c->p.conv_tol_ind);
+            hipLaunchKernelGGL(k_scf_check, dim3(1), dim3(1024), 0, c->st, n, c->type, c->tab_d, c->mu_ind, c->ef,
+                               (const double *)nullptr, c->ctl, c->scal, c->npol, maxit, minit, c->p.conv_tol_ind);
             // correction (skipped once converged)
             hipLaunchKernelGGL(k_dc_residual, dim3(cdiv(n, 256)), dim3(256), 0, c->st, n, c->type, c->tab_d, c->ef,
                                c->mu_ind, c->dc_R, c->dc_d, act);
@@ -634,6 +962,7 @@ static int engforce_impl(mdff_ctx *c, int istep)
         }
         // get_dipoles coulombic.c:67-130
         if (c->lpol) {
+            if ((rc = build_t2(c)) != MDFF_OK) return rc;
             static_field(c, lo);
             if (c->have_events) HIPCHK(hipEventRecord(c->ev[2], c->st));
             rc = lo ? scf_loop_mixed(c, istep) : scf_loop(c, istep);
@@ -847,18 +1176,33 @@ void mdff_destroy(mdff_ctx *c)
                      &c->mu_static, &c->mu_ind, &c->mu_tot, &c->hist, &c->ef_stat, &c->ef, &c->ef_dir, &c->ef_rec,
                      &c->efg, &c->efg_dir, &c->efg_rec, &c->f_sr, &c->f_dir, &c->f_rec, &c->acc_sr, &c->uw_dir,
                      &c->selfbuf, &c->ke, &c->upol_i, &c->rmsd_i, &c->tmp_q, &c->tmp_mu, &c->tmp_qraw, &c->tmp_mur, &c->dc_R, &c->dc_d, &c->dc_g, &c->scal,
-                     &c->kvec, &c->Ak, &c->kcoe, &c->W, &c->ek, &c->kpart, &c->bpart};
+                     &c->kvec, &c->Ak, &c->kcoe, &c->W, &c->ek, &c->kpart, &c->bpart, &c->ksum};
     for (auto pp : dp) dfree(*pp);
     dfree(c->type);
     dfree(c->ctl);
     dfree(c->tab_d);
     dfree(c->pidx);
+    dfree(c->pinv);
+    dfree(c->cntp);
+    dfree(c->idxp);
+    dfree(c->t2_ovf);
+    dfree(c->t2);
+    if (c->t2f) (void)hipFree(c->t2f);
     dfree(c->nbr);
     dfree(c->cnt);
     for (int a = 0; a < 2; a++)
         for (int b = 0; b < 2; b++)
             for (int d = 0; d < 3; d++)
                 if (c->tab3[a][b][d]) (void)hipFree(c->tab3[a][b][d]);
+    for (int m = 0; m < 3; m++)
+        for (int h = 0; h < 2; h++) {
+            for (int pl = 0; pl < 2; pl++) {
+                if (c->yz[m][pl][h]) (void)hipFree(c->yz[m][pl][h]);
+                if (c->yzt[m][pl][h]) (void)hipFree(c->yzt[m][pl][h]);
+            }
+            if (c->aw[m][h]) (void)hipFree(c->aw[m][h]);
+            if (c->wm[m][h]) (void)hipFree(c->wm[m][h]);
+        }
     if (c->ctl_h) (void)hipHostFree(c->ctl_h);
     if (c->scal_h) (void)hipHostFree(c->scal_h);
     if (c->have_events)
@@ -893,6 +1237,11 @@ int mdff_create(mdff_ctx **out, const mdff_params *p, int nion, const int *typia
 
     mdff_ctx *c = new mdff_ctx();
     c->p = *p;
+    {
+        hipDeviceProp_t prop;
+        if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
+            c->ncu = prop.multiProcessorCount;
+    }
     c->n = nion;
     c->ntype = p->ntype;
     c->precision = precision;
@@ -1016,6 +1365,15 @@ int mdff_create(mdff_ctx **out, const mdff_params *p, int nion, const int *typia
             mdff_destroy(c);
             return MDFF_ERR_DEVICE;
         }
+        std::vector<int> hinv(n, -1);
+        for (int k = 0; k < c->npol; k++) hinv[hp[k]] = k;
+        ALLOC(c->pinv, n);
+        ALLOC(c->cntp, c->npol);
+        ALLOC(c->t2_ovf, 1);
+        if (hipMemcpy(c->pinv, hinv.data(), sizeof(int) * n, hipMemcpyHostToDevice) != hipSuccess) {
+            mdff_destroy(c);
+            return MDFF_ERR_DEVICE;
+        }
     }
     // k mesh (init_kspace kspace.c:14-41, set_param_kmesh :64-102) on the full
     // (nx, ny, nz) box with nx >= 0; entries outside the half space get Ak = 0
@@ -1085,8 +1443,14 @@ int mdff_create(mdff_ctx **out, const mdff_params *p, int nion, const int *typia
         c->nsplit = (nrows + c->rows_per_split - 1) / c->rows_per_split;
         int C = 4;
         ALLOC(c->kpart, (size_t)std::max(64, std::max(c->nchunk, c->nchunk_p)) * c->nkf * 2 * C);
+        ALLOC(c->ksum, c->nkf * 2 * C);
         c->nsplit_max = std::max(c->nsplit, 16);
         ALLOC(c->bpart, (size_t)c->nsplit_max * 9 * n);
+        // padded sizes of the stored-operand sums (gemm_kernels.h)
+        c->nselp[0] = (n + 127) / 128 * 128;
+        c->nselp[1] = (c->npol + 127) / 128 * 128;
+        c->Pp = (c->P + 255) / 256 * 256;
+        c->Pk = (c->P + 15) / 16 * 16;
     }
     // neighbour list capacity: all pairs when no list, else an estimate
     // from the number density grown on overflow
@@ -1114,7 +1478,16 @@ int mdff_create(mdff_ctx **out, const mdff_params *p, int nion, const int *typia
         hipFuncSetAttribute((const void *)k_sk_fwd_mfma<KM_F32>, hipFuncAttributeMaxDynamicSharedMemorySize,
                             (int)mf_fwd_lds_bytes<KM_F32>()) != hipSuccess ||
         hipFuncSetAttribute((const void *)k_sk_back_mfma<KM_F32>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                            (int)mf_back_lds_bytes<KM_F32>()) != hipSuccess) {
+                            (int)mf_back_lds_bytes<KM_F32>()) != hipSuccess ||
+        hipFuncSetAttribute((const void *)k_gemm_back<KM_F64, MB_CT, double>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)g_back_lds_bytes<KM_F64>(MB_CT)) != hipSuccess ||
+        hipFuncSetAttribute((const void *)k_gemm_back<KM_F32, MB_CT, float>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)g_back_lds_bytes<KM_F32>(MB_CT)) != hipSuccess ||
+        hipFuncSetAttribute((const void *)k_gemm_back<KM_BF16X3, MB_CT, float>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)g_back_lds_bytes<KM_BF16X3>(MB_CT)) != hipSuccess) {
         mdff_destroy(c);
         return MDFF_ERR_DEVICE;
     }
@@ -1123,8 +1496,23 @@ int mdff_create(mdff_ctx **out, const mdff_params *p, int nion, const int *typia
     set_int(c, C_FIRST, 1);
     rc = build_list(c, 1);
     if (rc) { mdff_destroy(c); return rc; }
-    if (hipStreamSynchronize(c->st) != hipSuccess) { mdff_destroy(c); return MDFF_ERR_DEVICE; }
+    // every zero fill of the set-up (null stream) and the first build are done
+    if (hipDeviceSynchronize() != hipSuccess) { mdff_destroy(c); return MDFF_ERR_DEVICE; }
     *out = c;
+    return MDFF_OK;
+}
+
+int mdff_set_kspace_shape(mdff_ctx *c, int cw64, int cw32, int fwd_blocks, int back_blocks)
+{
+    if (!c) return MDFF_ERR_ARG;
+    int cw[2] = {cw64, cw32};
+    for (int k = 0; k < 2; k++)
+        if (cw[k] && cw[k] != 1 && cw[k] != 2 && cw[k] != 4) return MDFF_ERR_ARG;
+    if (fwd_blocks < 0 || back_blocks < 0) return MDFF_ERR_ARG;
+    for (int k = 0; k < 2; k++)
+        if (cw[k]) c->g_cw[k] = cw[k];
+    if (fwd_blocks) c->g_fblocks = fwd_blocks;
+    if (back_blocks) c->g_bblocks = back_blocks;
     return MDFF_OK;
 }
 
@@ -1133,10 +1521,13 @@ int mdff_set_tuning(mdff_ctx *c, int lanes, int katoms, int ksplit)
     if (!c) return MDFF_ERR_ARG;
     if (lanes == 16 || lanes == 32 || lanes == 64) c->lanes = lanes;
     else if (lanes != 0) return MDFF_ERR_ARG;
-    // katoms: reciprocal-space kernel family.  -1 vector ALU; 1 matrix cores
-    // with split-BF16 SCF correction sweeps; 2 matrix cores, FP32 sweeps.
-    if (katoms == 1) { c->use_mfma = 1; c->mf_inner = KM_BF16X3; }
-    else if (katoms == 2) { c->use_mfma = 1; c->mf_inner = KM_F32; }
+    // katoms: reciprocal-space kernel family.  -1 vector ALU; 1 matrix-core
+    // sums over stored operands with split-BF16 SCF correction sweeps; 2 the
+    // same with FP32 sweeps; 3 matrix cores with operands formed in LDS per
+    // call (split-BF16 sweeps).
+    if (katoms == 1) { c->use_mfma = 1; c->gemm = 1; c->mf_inner = KM_BF16X3; }
+    else if (katoms == 2) { c->use_mfma = 1; c->gemm = 1; c->mf_inner = KM_F32; }
+    else if (katoms == 3) { c->use_mfma = 1; c->gemm = 0; c->mf_inner = KM_BF16X3; }
     else if (katoms == -1) c->use_mfma = 0;
     else if (katoms != 0) return MDFF_ERR_ARG;
     if (ksplit > 0) c->mf_blocks = ksplit;
@@ -1283,6 +1674,7 @@ int mdff_get_dipoles_scf(mdff_ctx *c, int istep, double *mu, double *upol)
         launch_eikr(c, 0, lo, nullptr);
         launch_eikr(c, 1, 0, nullptr);
         if (lo) launch_eikr(c, 1, 1, nullptr);
+        if ((rc = build_t2(c)) != MDFF_OK) return rc;
         static_field(c, lo);
         rc = lo ? scf_loop_mixed(c, istep) : scf_loop(c, istep);
         if (rc && rc != MDFF_ERR_SCF) return rc;
@@ -1469,6 +1861,7 @@ int mdff_time_kernel(mdff_ctx *c, int which, int lo, int iters, double *ms)
     b.tx = a.tx; b.ty = a.ty; b.tz = a.tz; b.W = c->W; b.rows_per_split = c->rows_per_split; b.part = c->bpart;
     b.active = nullptr;
     DirArgs d = dir_args(c, polar ? c->qz : c->q, mu, polar ? 0 : c->ldmp, polar ? 1 : 0, nullptr);
+    if (which == 2 && build_t2(c) != MDFF_OK) return MDFF_ERR_NOMEM;
     auto once = [&]() {
         int dummy = 0;
         switch (which) {
@@ -1482,13 +1875,26 @@ int mdff_time_kernel(mdff_ctx *c, int which, int lo, int iters, double *ms)
             if (lo) { if (NCH == 3) launch_back_t<float, 3>(c, b, nsel); else launch_back_t<float, 9>(c, b, nsel); }
             else { if (NCH == 3) launch_back_t<double, 3>(c, b, nsel); else launch_back_t<double, 9>(c, b, nsel); }
             break;
-        case 2: launch_dir(c, d, lo, 1, 0, 0, 0); break;
+        case 2:
+            if (c->t2_ok) {
+                unsigned blocks = cdiv((size_t)n * 16, 256);
+                if (lo)
+                    hipLaunchKernelGGL((k_t2_field<float, 16>), dim3(blocks), dim3(256), 0, c->st, n, c->capP,
+                                       c->npol, c->pinv, c->cntp, c->idxp, c->t2f, mu, c->ef_dir, nullptr);
+                else
+                    hipLaunchKernelGGL((k_t2_field<double, 16>), dim3(blocks), dim3(256), 0, c->st, n, c->capP,
+                                       c->npol, c->pinv, c->cntp, c->idxp, c->t2, mu, c->ef_dir, nullptr);
+            } else {
+                launch_dir(c, d, lo, 1, 0, 0, 0);
+            }
+            break;
         case 5: short_range(c); break;
         case 6: launch_dir(c, d, lo, 0, 0, 0, 1); break;
         default: break;
         }
     };
     once();
+    c->fwd_part = 0;
     HIPCHK(hipEventRecord(c->ev[6], c->st));
     for (int k = 0; k < iters; k++) once();
     HIPCHK(hipEventRecord(c->ev[7], c->st));
@@ -1496,6 +1902,7 @@ int mdff_time_kernel(mdff_ctx *c, int which, int lo, int iters, double *ms)
     float t = 0.0f;
     HIPCHK(hipEventElapsedTime(&t, c->ev[6], c->ev[7]));
     *ms = (double)t / iters;
+    c->fwd_part = 0;
     return MDFF_OK;
 }
 

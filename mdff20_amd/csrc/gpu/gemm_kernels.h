@@ -1,0 +1,487 @@
+// gemm_kernels.h -- the structure-factor sum and its back-projection as
+// streaming matrix-core GEMMs over operands stored in HBM.
+//
+// The ey*ez products of every ion and (ny, nz) point depend only on the
+// positions, so they are formed once per force call in two layouts:
+//   YZ [p][2j + ri]   (point-major: B operand of the forward sum)
+//   YZt[j][2p + ri]   (ion-major:   A operand of the back-projection)
+// with K = (ion, re/im) or (point, re/im) contiguous, zero-padded to the
+// padded sizes.  Per evaluation only the small operands change:
+//   AW[(nx, c, re/im)][2j + ri]   = w_cj ex_j[nx] in the real embedding of
+//                                   mfma_kernels.h (rows of the forward sum)
+//   WM[(nx, m, re/im)][2p + ri]   = W(nx, p) mom_m(p) (columns of the back sum)
+// The big operand is loaded straight into registers, 8 consecutive K values
+// per lane; the small one is staged through LDS and shared by the block's
+// waves.  For the 16x16x4 FP64/FP32 instructions a lane group g = lane/16
+// takes K = 8g + s at step s of an 8-step block, for both operands (any
+// permutation of K leaves the sum unchanged); the 16x16x32 BF16 instruction
+// takes K = 8g..8g+7 in one step.
+#pragma once
+#include "mfma_kernels.h"
+
+template <int MODE>
+struct GT;
+template <>
+struct GT<KM_F64> { using T = double; };
+template <>
+struct GT<KM_F32> { using T = float; };
+template <>
+struct GT<KM_BF16X3> { using T = __bf16; };
+
+typedef double g_d8 __attribute__((ext_vector_type(8)));
+typedef float g_f8 __attribute__((ext_vector_type(8)));
+
+// store x into the operand arrays of a mode (hi and, for split BF16, lo)
+template <int MODE>
+__device__ __forceinline__ void g_store(void *hi, void *lo, size_t i, double x)
+{
+    if constexpr (MODE == KM_F64) {
+        ((double *)hi)[i] = x;
+    } else if constexpr (MODE == KM_F32) {
+        ((float *)hi)[i] = (float)x;
+    } else {
+        __bf16 h = (__bf16)(float)x;
+        ((__bf16 *)hi)[i] = h;
+        ((__bf16 *)lo)[i] = (__bf16)(float)(x - (double)(float)h);
+    }
+}
+
+// YZ[p][2j + ri] and YZt[j][2p + ri] = (Re, Im)(ey_j[iy] ez_j[iz]),
+// p = iy NZ + iz, from one 64-ion x 32-point tile per block: the products go
+// through LDS so that both layouts are written in contiguous rows.  Zero
+// for j >= nsel or p >= P (padding).  grid (nselp/64, ceil(max(Pp, Pk)/32)).
+template <int MODE, typename TT>
+__global__ void __launch_bounds__(256) k_yz_build(int nsel, int nselp, int P, int Pp_rows, int Pk, int NZ,
+                                                  const TT *__restrict__ ty, const TT *__restrict__ tz, void *yz,
+                                                  void *yz_lo, void *yzt, void *yzt_lo)
+{
+    __shared__ double sr[32][65], si[32][65];   // [point][ion]
+    const int j0 = blockIdx.x * 64, p0 = blockIdx.y * 32;
+    const int tj = threadIdx.x & 63, tq = threadIdx.x >> 6;
+    const int j = j0 + tj;
+    for (int pp = tq; pp < 32; pp += 4) {
+        const int p = p0 + pp;
+        double br = 0.0, bi = 0.0;
+        if (j < nsel && p < P) {
+            const int iy = p / NZ, iz = p % NZ;
+            size_t oy = 2 * ((size_t)iy * nsel + j), oz = 2 * ((size_t)iz * nsel + j);
+            double yr = ty[oy], yi = ty[oy + 1], zr = tz[oz], zi = tz[oz + 1];
+            br = yr * zr - yi * zi;
+            bi = yr * zi + yi * zr;
+        }
+        sr[pp][tj] = br;
+        si[pp][tj] = bi;
+    }
+    __syncthreads();
+    // point-major rows: lanes along ions
+    for (int pp = tq; pp < 32; pp += 4) {
+        const int p = p0 + pp;
+        if (p >= Pp_rows || j >= nselp) continue;
+        const size_t o = (size_t)p * 2 * nselp + 2 * (size_t)j;
+        g_store<MODE>(yz, yz_lo, o, sr[pp][tj]);
+        g_store<MODE>(yz, yz_lo, o + 1, si[pp][tj]);
+    }
+    // ion-major rows: 32 lanes along points, two ions per wavefront pass
+    const int tp = threadIdx.x & 31, tr = threadIdx.x >> 5;
+    for (int jj = tr; jj < 64; jj += 8) {
+        const int jr = j0 + jj, p = p0 + tp;
+        if (jr >= nselp || p >= Pk) continue;
+        const size_t o = (size_t)jr * 2 * Pk + 2 * (size_t)p;
+        g_store<MODE>(yzt, yzt_lo, o, sr[tp][jj]);
+        g_store<MODE>(yzt, yzt_lo, o + 1, si[tp][jj]);
+    }
+}
+
+// AW rows (nx, c, re/im) over K = (j, ri): row (nx,c,0) = (w exr, -w exi),
+// row (nx,c,1) = (w exi, w exr).  grid (nselp/256, Kx+1).  Rows >= M are
+// never written (zeroed at allocation).
+template <int MODE, typename TT>
+__global__ void k_aw_build(int nsel, int nselp, int C, const TT *__restrict__ tx, const int *__restrict__ idx,
+                           const double *__restrict__ q, const double *__restrict__ mu, int n, void *aw, void *aw_lo,
+                           const int *active)
+{
+    if (active && !*active) return;
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nx = blockIdx.y;
+    if (j >= nselp) return;
+    double xr = 0.0, xi = 0.0;
+    int ion = 0;
+    if (j < nsel) {
+        size_t ox = 2 * ((size_t)nx * nsel + j);
+        xr = tx[ox];
+        xi = tx[ox + 1];
+        ion = idx ? idx[j] : j;
+    }
+    const size_t ld = 2 * (size_t)nselp;
+    for (int c = 0; c < C; c++) {
+        double w = 0.0;
+        if (j < nsel) {
+            if (C == 1) w = q[ion];
+            else if (C == 3) w = mu[(size_t)c * n + ion];
+            else w = c == 0 ? q[ion] : mu[(size_t)(c - 1) * n + ion];
+        }
+        const size_t r0 = (size_t)(nx * C + c) * 2;
+        g_store<MODE>(aw, aw_lo, r0 * ld + 2 * (size_t)j, w * xr);
+        g_store<MODE>(aw, aw_lo, r0 * ld + 2 * (size_t)j + 1, -w * xi);
+        g_store<MODE>(aw, aw_lo, (r0 + 1) * ld + 2 * (size_t)j, w * xi);
+        g_store<MODE>(aw, aw_lo, (r0 + 1) * ld + 2 * (size_t)j + 1, w * xr);
+    }
+}
+
+// WM columns (nx, m, re/im) over K = (p, ri): with Wm = W(nx, p) mom_m(p),
+// column (nx,m,0) = (Re Wm, -Im Wm), column (nx,m,1) = (Im Wm, Re Wm).
+// grid (Pp/256, (Kx+1) CH), one thread per point.
+template <int MODE>
+__global__ void k_wm_build(int P, int Pp, int Ky, int Kz, int CH, const double *__restrict__ W, void *wm,
+                           void *wm_lo, const int *active)
+{
+    if (active && !*active) return;
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nx = blockIdx.y / CH, m = blockIdx.y % CH;
+    if (p >= Pp) return;
+    double wr = 0.0, wi = 0.0;
+    if (p < P) {
+        const int NZ = 2 * Kz + 1;
+        const int iy = p / NZ, iz = p % NZ;
+        const double mom = mb_moment(m, iy - Ky, iz - Kz);
+        const size_t kf = (size_t)nx * P + p;
+        wr = W[2 * kf] * mom;
+        wi = W[2 * kf + 1] * mom;
+    }
+    const size_t ld = 2 * (size_t)Pp;
+    const size_t c0 = (size_t)(nx * CH + m) * 2;
+    g_store<MODE>(wm, wm_lo, c0 * ld + 2 * (size_t)p, wr);
+    g_store<MODE>(wm, wm_lo, c0 * ld + 2 * (size_t)p + 1, -wi);
+    g_store<MODE>(wm, wm_lo, (c0 + 1) * ld + 2 * (size_t)p, wi);
+    g_store<MODE>(wm, wm_lo, (c0 + 1) * ld + 2 * (size_t)p + 1, wr);
+}
+
+// 8 consecutive operand values of one lane
+template <int MODE>
+struct GRun {
+    using T = typename GT<MODE>::T;
+    typename std::conditional<MODE == KM_F64, g_d8, typename std::conditional<MODE == KM_F32, g_f8, mf_bf8>::type>::type
+        v;
+};
+
+template <int MODE>
+__device__ __forceinline__ GRun<MODE> g_load(const void *base, size_t off)
+{
+    GRun<MODE> r;
+    if constexpr (MODE == KM_F64) r.v = *(const g_d8 *)((const double *)base + off);
+    else if constexpr (MODE == KM_F32) r.v = *(const g_f8 *)((const float *)base + off);
+    else r.v = *(const mf_bf8 *)((const __bf16 *)base + off);
+    return r;
+}
+
+#define GK 32          // K values per block step
+#define GLDS_PAD 8     // LDS row padding (elements)
+
+struct GFwdArgs {
+    const void *aw, *aw_lo;   // [16 MT][ldk]
+    const void *yz, *yz_lo;   // [Pp][ldk]
+    int MT, M, P, C, ldk, kchunk;
+    double *part;             // [nchunk][(Kx+1) P][2C]
+    const int *active;
+};
+
+template <int MODE>
+inline size_t g_fwd_lds_bytes(int MT)
+{
+    size_t e = (size_t)MT * 16 * (GK + GLDS_PAD);
+    return MODE == KM_F64 ? e * 8 : (MODE == KM_F32 ? e * 4 : e * 2 * 2);
+}
+
+// forward sum: block = 4 waves x CW column tiles of 16 points, all MT row
+// tiles, one K range (grid.y); partial sums per K range in the layout of
+// k_sk_reduce.  The next step's A rows and B runs are loaded while the
+// current step computes; consecutive instructions update different
+// accumulators (no back-to-back dependence on one accumulator).
+#define GF_SLOTS 3   // A runs a thread stages per step: ceil(16 MT GK/8 / 256) for MT <= 12
+template <int MODE, int CW, int MTMAX>
+__global__ void __launch_bounds__(256) k_gemm_fwd(GFwdArgs a)
+{
+    if (a.active && !*a.active) return;
+    using T = typename GT<MODE>::T;
+    constexpr int LD = GK + GLDS_PAD;
+    constexpr bool SPLIT = MODE == KM_BF16X3;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    const int MT = a.MT;
+    T *As = (T *)smem_raw;
+    T *Asl = As + MT * 16 * LD;   // split-BF16 low parts (the LDS size holds MT rows of each)
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int g = lane >> 4, cl = lane & 15;
+    const int ct0 = (blockIdx.x * 4 + wave) * CW;
+    const int k0 = blockIdx.y * a.kchunk;
+    const int k1 = min(k0 + a.kchunk, a.ldk);
+    const int nslot = MT * 16 * (GK / 8);
+    using acc_t = typename std::conditional<MODE == KM_F64, mf_d4, mf_f4>::type;
+    acc_t acc[MTMAX][CW];
+#pragma unroll
+    for (int t = 0; t < MTMAX; t++)
+#pragma unroll
+        for (int c = 0; c < CW; c++) acc[t][c] = acc_t{0, 0, 0, 0};
+    size_t brow[CW];
+#pragma unroll
+    for (int c = 0; c < CW; c++) brow[c] = (size_t)((ct0 + c) * 16 + cl) * a.ldk;
+    GRun<MODE> bh[CW], bl[CW], ra[GF_SLOTS], rl[GF_SLOTS];
+    auto load_step = [&](int kb) {
+#pragma unroll
+        for (int c = 0; c < CW; c++) {
+            bh[c] = g_load<MODE>(a.yz, brow[c] + kb + 8 * g);
+            if constexpr (SPLIT) bl[c] = g_load<MODE>(a.yz_lo, brow[c] + kb + 8 * g);
+        }
+#pragma unroll
+        for (int q = 0; q < GF_SLOTS; q++) {
+            const int e = threadIdx.x + 256 * q;
+            if (e < nslot) {
+                const int r = e / (GK / 8), q8 = (e % (GK / 8)) * 8;
+                ra[q] = g_load<MODE>(a.aw, (size_t)r * a.ldk + kb + q8);
+                if constexpr (SPLIT) rl[q] = g_load<MODE>(a.aw_lo, (size_t)r * a.ldk + kb + q8);
+            }
+        }
+    };
+    if (k0 < k1) load_step(k0);
+    for (int kb = k0; kb < k1; kb += GK) {
+        // this step's A rows into LDS; its B runs stay in registers
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < GF_SLOTS; q++) {
+            const int e = threadIdx.x + 256 * q;
+            if (e < nslot) {
+                const int r = e / (GK / 8), q8 = (e % (GK / 8)) * 8;
+                *(decltype(ra[q].v) *)(As + r * LD + q8) = ra[q].v;
+                if constexpr (SPLIT) *(decltype(rl[q].v) *)(Asl + r * LD + q8) = rl[q].v;
+            }
+        }
+        GRun<MODE> ch[CW], cl_[CW];
+#pragma unroll
+        for (int c = 0; c < CW; c++) {
+            ch[c] = bh[c];
+            if constexpr (SPLIT) cl_[c] = bl[c];
+        }
+        __syncthreads();
+        if (kb + GK < k1) load_step(kb + GK);
+        if constexpr (SPLIT) {
+            mf_bf8 ah[MTMAX], al[MTMAX];
+#pragma unroll
+            for (int t = 0; t < MTMAX; t++) {
+                if (t >= MT) continue;
+                ah[t] = *(const mf_bf8 *)(As + (t * 16 + cl) * LD + 8 * g);
+                al[t] = *(const mf_bf8 *)(Asl + (t * 16 + cl) * LD + 8 * g);
+            }
+#pragma unroll
+            for (int t = 0; t < MTMAX; t++)
+#pragma unroll
+                for (int c = 0; c < CW; c++)
+                    if (t < MT) acc[t][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[t], ch[c].v, acc[t][c], 0, 0, 0);
+#pragma unroll
+            for (int t = 0; t < MTMAX; t++)
+#pragma unroll
+                for (int c = 0; c < CW; c++)
+                    if (t < MT) acc[t][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[t], cl_[c].v, acc[t][c], 0, 0, 0);
+#pragma unroll
+            for (int t = 0; t < MTMAX; t++)
+#pragma unroll
+                for (int c = 0; c < CW; c++)
+                    if (t < MT) acc[t][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[t], ch[c].v, acc[t][c], 0, 0, 0);
+        } else {
+#pragma unroll
+            for (int s = 0; s < 8; s++) {
+#pragma unroll
+                for (int t = 0; t < MTMAX; t++) {
+                    if (t >= MT) continue;
+                    const T av = As[(t * 16 + cl) * LD + 8 * g + s];
+#pragma unroll
+                    for (int c = 0; c < CW; c++) acc[t][c] = mf_mma<MODE>(av, ch[c].v[s], acc[t][c]);
+                }
+            }
+        }
+    }
+    // partial sums of this K range (FP32 accumulators are stored as FP32)
+    using part_t = typename std::conditional<MODE == KM_F64, double, float>::type;
+    const size_t nkf = (size_t)(a.M / (2 * a.C)) * a.P;
+    part_t *out = (part_t *)a.part + (size_t)blockIdx.y * nkf * (2 * a.C);
+#pragma unroll
+    for (int t = 0; t < MTMAX; t++) {
+        if (t >= MT) continue;
+#pragma unroll
+        for (int c = 0; c < CW; c++) {
+            const int p = (ct0 + c) * 16 + cl;
+            if (p >= a.P) continue;
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const int r = (MODE == KM_F64) ? t * 16 + g + 4 * i : t * 16 + 4 * g + i;
+                if (r >= a.M) continue;
+                const int nx = r / (2 * a.C), cc = (r >> 1) % a.C, ri = r & 1;
+                out[((size_t)nx * a.P + p) * (2 * a.C) + 2 * cc + ri] = (part_t)acc[t][c][i];
+            }
+        }
+    }
+}
+
+// sum of the K-range partials of the forward sum, one thread per value,
+// chunks in a fixed order (bitwise reproducible)
+template <typename part_t>
+__global__ void k_part_sum(const part_t *__restrict__ part, int nchunk, size_t nval, double *__restrict__ out,
+                           const int *active)
+{
+    if (active && !*active) return;
+    const size_t v = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (v >= nval) return;
+    double s = 0.0;
+    for (int ch = 0; ch < nchunk; ch++) s += (double)part[(size_t)ch * nval + v];
+    out[v] = s;
+}
+
+#define GB_SLOTS 2   // B runs a thread stages per step: ceil(16 CT GK/8 / 512) for CT <= 16
+
+struct GBackArgs {
+    const void *yzt, *yzt_lo;  // [nselp][ldk]
+    const void *wm, *wm_lo;    // [16 CT][ldk]
+    const void *tx;            // ex tables [Kx+1][nsel][2] (TT)
+    int nsel, CH, NC, CT, ldk, kchunk;
+    double *part;              // [nsplit][3 or 9][nsel]
+    const int *active;
+};
+
+template <int MODE>
+inline size_t g_back_lds_bytes(int CT)
+{
+    size_t e = (size_t)CT * 16 * (GK + GLDS_PAD);
+    return MODE == KM_F64 ? e * 8 : (MODE == KM_F32 ? e * 4 : e * 2 * 2);
+}
+
+// back-projection: block = 8 waves x 16 ions, all CT column tiles, one K
+// range (grid.y); epilogue as k_sk_back_mfma (x ex[nx], moments)
+template <int MODE, int CTMAX, typename TT>
+__global__ void __launch_bounds__(512) k_gemm_back(GBackArgs a)
+{
+    if (a.active && !*a.active) return;
+    using T = typename GT<MODE>::T;
+    constexpr int LD = GK + GLDS_PAD;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    const int CT = a.CT;
+    T *Bs = (T *)smem_raw;
+    T *Bsl = Bs + CT * 16 * LD;   // split-BF16 low parts (the LDS size holds CT rows of each)
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int g = lane >> 4, cl = lane & 15;
+    const int jrow0 = blockIdx.x * 128 + wave * 16;
+    const int k0 = blockIdx.y * a.kchunk;
+    const int k1 = min(k0 + a.kchunk, a.ldk);
+    using acc_t = typename std::conditional<MODE == KM_F64, mf_d4, mf_f4>::type;
+    acc_t acc[CTMAX];
+#pragma unroll
+    for (int c = 0; c < CTMAX; c++) acc[c] = acc_t{0, 0, 0, 0};
+    const size_t arow = (size_t)(jrow0 + cl) * a.ldk;
+    constexpr bool SPLIT = MODE == KM_BF16X3;
+    const int nslot = CT * 16 * (GK / 8);
+    GRun<MODE> ah, al, rb[GB_SLOTS], rbl[GB_SLOTS];   // this thread's staged B runs
+    auto load_step = [&](int kb) {
+        ah = g_load<MODE>(a.yzt, arow + kb + 8 * g);
+        if constexpr (SPLIT) al = g_load<MODE>(a.yzt_lo, arow + kb + 8 * g);
+#pragma unroll
+        for (int q = 0; q < GB_SLOTS; q++) {
+            const int e = threadIdx.x + 512 * q;
+            if (e < nslot) {
+                const int r = e / (GK / 8), q8 = (e % (GK / 8)) * 8;
+                rb[q] = g_load<MODE>(a.wm, (size_t)r * a.ldk + kb + q8);
+                if constexpr (SPLIT) rbl[q] = g_load<MODE>(a.wm_lo, (size_t)r * a.ldk + kb + q8);
+            }
+        }
+    };
+    if (k0 < k1) load_step(k0);
+    for (int kb = k0; kb < k1; kb += GK) {
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < GB_SLOTS; q++) {
+            const int e = threadIdx.x + 512 * q;
+            if (e < nslot) {
+                const int r = e / (GK / 8), q8 = (e % (GK / 8)) * 8;
+                *(decltype(rb[q].v) *)(Bs + r * LD + q8) = rb[q].v;
+                if constexpr (SPLIT) *(decltype(rbl[q].v) *)(Bsl + r * LD + q8) = rbl[q].v;
+            }
+        }
+        GRun<MODE> ch = ah, cl_ = al;
+        __syncthreads();
+        if (kb + GK < k1) load_step(kb + GK);
+        if constexpr (SPLIT) {
+#pragma unroll
+            for (int ct = 0; ct < CTMAX; ct++)
+                if (ct < CT)
+                    acc[ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                        ch.v, *(const mf_bf8 *)(Bs + (ct * 16 + cl) * LD + 8 * g), acc[ct], 0, 0, 0);
+#pragma unroll
+            for (int ct = 0; ct < CTMAX; ct++)
+                if (ct < CT)
+                    acc[ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                        ch.v, *(const mf_bf8 *)(Bsl + (ct * 16 + cl) * LD + 8 * g), acc[ct], 0, 0, 0);
+#pragma unroll
+            for (int ct = 0; ct < CTMAX; ct++)
+                if (ct < CT)
+                    acc[ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                        cl_.v, *(const mf_bf8 *)(Bs + (ct * 16 + cl) * LD + 8 * g), acc[ct], 0, 0, 0);
+        } else {
+#pragma unroll
+            for (int s = 0; s < 8; s++)
+#pragma unroll
+                for (int ct = 0; ct < CTMAX; ct++)
+                    if (ct < CT) acc[ct] = mf_mma<MODE>(ch.v[s], Bs[(ct * 16 + cl) * LD + 8 * g + s], acc[ct]);
+        }
+    }
+    // epilogue: Z_m = ex[nx] T_m on the even (real-part) columns, moments,
+    // then the sum over the 8 even lanes that share this lane's ions
+    double H[4][3], Mm[4][6];
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        H[i][0] = H[i][1] = H[i][2] = 0.0;
+#pragma unroll
+        for (int c = 0; c < 6; c++) Mm[i][c] = 0.0;
+    }
+#pragma unroll
+    for (int ct = 0; ct < CTMAX; ct++) {
+        if (ct >= CT) continue;
+        const int col = ct * 16 + cl;
+        const int nx = col / (2 * a.CH), m = (col >> 1) % a.CH;
+        const bool even = (cl & 1) == 0 && col < a.NC;
+        const double fnx = (double)nx;
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            double tre = (double)acc[ct][i];
+            double tim = __shfl_xor(tre, 1, 64);
+            const int r = (MODE == KM_F64) ? g + 4 * i : 4 * g + i;
+            const int j = jrow0 + r;
+            if (!even || j >= a.nsel) continue;
+            size_t ox = 2 * ((size_t)nx * a.nsel + j);
+            double xr = (double)((const TT *)a.tx)[ox], xi = (double)((const TT *)a.tx)[ox + 1];
+            double zr = xr * tre - xi * tim, zi = xr * tim + xi * tre;
+            switch (m) {
+            case 0: H[i][0] += fnx * zi; Mm[i][0] += fnx * fnx * zr; break;
+            case 1: H[i][1] += zi; Mm[i][1] += fnx * zr; break;
+            case 2: H[i][2] += zi; Mm[i][2] += fnx * zr; break;
+            case 3: Mm[i][3] += zr; break;
+            case 4: Mm[i][4] += zr; break;
+            default: Mm[i][5] += zr; break;
+            }
+        }
+    }
+    const int nch = (a.CH == 6) ? 9 : 3;
+    double *out = a.part + (size_t)blockIdx.y * nch * a.nsel;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        double v[9] = {H[i][0], H[i][1], H[i][2], Mm[i][0], Mm[i][1], Mm[i][2], Mm[i][3], Mm[i][4], Mm[i][5]};
+#pragma unroll
+        for (int c = 0; c < 9; c++) {
+            double s = v[c];
+            s += __shfl_xor(s, 2, 64);
+            s += __shfl_xor(s, 4, 64);
+            s += __shfl_xor(s, 8, 64);
+            v[c] = s;
+        }
+        const int r = (MODE == KM_F64) ? g + 4 * i : 4 * g + i;
+        const int j = jrow0 + r;
+        if (cl == 0 && j < a.nsel) {
+            for (int c = 0; c < nch; c++) out[(size_t)c * a.nsel + j] = v[c];
+        }
+    }
+}

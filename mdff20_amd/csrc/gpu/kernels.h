@@ -716,6 +716,11 @@ struct BackFinArgs {
     double *f;        // 3n (DO_F)
     int do_ef, do_f;
     const int *active;
+    // when ef_total is set: ef_total = 2 selfa2 mu_self + ef_dir + rec (+ base)
+    // at the selected ions, the sum k_field_sum forms (ewald.c:235-270)
+    double *ef_total;
+    const double *mu_self, *ef_dir, *base;
+    double selfa2;
 };
 
 // sum the splits, map n-moments to k-moments and write field, gradient and
@@ -747,6 +752,14 @@ __global__ void k_sk_back_fin(BackFinArgs a)
             G[aa][bb] = MDFF_TPI * MDFF_TPI * sum;
         }
     const int n = a.n;
+    if (a.ef_total) {
+        for (int c = 0; c < 3; c++) {
+            const size_t o = (size_t)c * n + i;
+            double v = 2.0 * a.selfa2 * a.mu_self[o] + a.ef_dir[o] + a.tfpiV * E[c];
+            if (a.base) v += a.base[o];
+            a.ef_total[o] = v;
+        }
+    }
     if (a.do_ef) {
         a.ef[i] = a.tfpiV * E[0];
         a.ef[n + i] = a.tfpiV * E[1];
@@ -808,4 +821,127 @@ __global__ void k_field_sum(int n, double selfa2, const double *mu, const double
     if (ef_rec) v += ef_rec[i];
     if (base) v += base[i];
     ef[i] = v;
+}
+
+// ------------------------------------------ SCF real space over stored T2
+// The dipole field of the SCF (multipole_ES_dir with charges off,
+// ewald.c:737-761) needs, per pair of polarisable ions, only the screened
+// tensor T2 = a2 r r - b2 I, which depends on the positions alone.  It is
+// stored once per force call, per polarisable ion row (SoA: neighbour ion,
+// then the 6 components xx xy xz yy yz zz), and each SCF evaluation is a
+// gather sum E_i = sum_j T2_ij mu_j.
+struct T2Args {
+    int n, cap, capP;
+    CellD c;
+    const double *s;
+    const unsigned *nbr;
+    const int *cnt;
+    const int *pidx;     // polarisable row -> ion
+    const int *pinv;     // ion -> polarisable row or -1
+    double alpha, rc2;
+    int npol;
+    int *cntp;           // [npol]
+    int *idxp;           // [npol][capP] neighbour ion
+    double *t2;          // [6][npol][capP]
+    float *t2f;          // [6][npol][capP] (optional FP32 copy)
+    int *overflow;
+};
+
+// one wavefront per polarisable ion, ballot-compacted rows
+__global__ void __launch_bounds__(256) k_t2_build(T2Args a)
+{
+    const int lane = threadIdx.x & 63;
+    const int pr = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (pr >= a.npol) return;
+    const int i = a.pidx[pr];
+    const int n = a.n;
+    const double six = a.s[i], siy = a.s[n + i], siz = a.s[2 * n + i];
+    const double al = a.alpha, al2 = al * al, al3 = al2 * al;
+    const double rpi = 1.0 / 1.7724538509055160273;
+    const int m = a.cnt[i];
+    const unsigned *nb = a.nbr + (size_t)i * a.cap;
+    const size_t plane = (size_t)a.npol * a.capP;
+    int count = 0;
+    for (int e0 = 0; e0 < m; e0 += 64) {
+        const int e = e0 + lane;
+        bool keep = false;
+        int j = 0;
+        double T[6] = {0, 0, 0, 0, 0, 0};
+        if (e < m) {
+            unsigned ent = nb[e];
+            j = ent & NBR_IDX;
+            if ((ent & NBR_LR) && a.pinv[j] >= 0) {
+                double x, y, z;
+                min_image(a.c, six - a.s[j], siy - a.s[n + j], siz - a.s[2 * n + j], x, y, z);
+                double d2 = x * x + y * y + z * z;
+                if (d2 <= a.rc2) {
+                    keep = true;
+                    const double d = sqrt(d2), dm1 = 1.0 / d, dm3 = dm1 / d2, dm5 = dm3 / d2;
+                    const double expon = exp(-al2 * d2) * rpi;
+                    const double F0 = erfc(al * d);
+                    const double F1 = F0 + 2.0 * al * d * expon;
+                    const double F2 = F1 + 4.0 * al3 * d2 * d * expon / 3.0;
+                    const double a2 = 3.0 * F2 * dm5, b2 = F1 * dm3;
+                    T[0] = a2 * x * x - b2; T[1] = a2 * x * y; T[2] = a2 * x * z;
+                    T[3] = a2 * y * y - b2; T[4] = a2 * y * z; T[5] = a2 * z * z - b2;
+                }
+            }
+        }
+        unsigned long long bm = __ballot(keep);
+        int pos = count + __popcll(bm & ((1ull << lane) - 1ull));
+        if (keep && pos < a.capP) {
+            const size_t o = (size_t)pr * a.capP + pos;
+            a.idxp[o] = j;
+#pragma unroll
+            for (int k = 0; k < 6; k++) a.t2[k * plane + o] = T[k];
+            if (a.t2f) {
+#pragma unroll
+                for (int k = 0; k < 6; k++) a.t2f[k * plane + o] = (float)T[k];
+            }
+        }
+        count += __popcll(bm);
+    }
+    if (lane == 0) {
+        a.cntp[pr] = count < a.capP ? count : a.capP;
+        if (count > a.capP) atomicMax(a.overflow, count);
+    }
+}
+
+// E_i = sum_j T2_ij mu_j at every ion (zero where mu_i = 0 or i is not
+// polarisable: the dipole-dipole field of the reference needs both ions to
+// carry a dipole, ewald.c:737)
+template <typename real, int LANES>
+__global__ void __launch_bounds__(256) k_t2_field(int n, int capP, int npol, const int *__restrict__ pinv,
+                                                   const int *__restrict__ cntp, const int *__restrict__ idxp,
+                                                   const real *__restrict__ t2, const double *__restrict__ mu,
+                                                   double *__restrict__ ef, const int *active)
+{
+    if (active && !*active) return;
+    const int lane = threadIdx.x % LANES;
+    const int i = (blockIdx.x * blockDim.x + threadIdx.x) / LANES;
+    if (i >= n) return;
+    const int pr = pinv[i];
+    const bool dip_i = (mu[i] != 0.0) || (mu[n + i] != 0.0) || (mu[2 * n + i] != 0.0);
+    real ex = 0, ey = 0, ez = 0;
+    if (pr >= 0 && dip_i) {
+        const size_t plane = (size_t)npol * capP;
+        const size_t row = (size_t)pr * capP;
+        const int m = cntp[pr];
+        for (int e = lane; e < m; e += LANES) {
+            const size_t o = row + e;
+            const int j = idxp[o];
+            const real mx = (real)mu[j], my = (real)mu[n + j], mz = (real)mu[2 * n + j];
+            const real txx = t2[o], txy = t2[plane + o], txz = t2[2 * plane + o];
+            const real tyy = t2[3 * plane + o], tyz = t2[4 * plane + o], tzz = t2[5 * plane + o];
+            ex += txx * mx + txy * my + txz * mz;
+            ey += txy * mx + tyy * my + tyz * mz;
+            ez += txz * mx + tyz * my + tzz * mz;
+        }
+    }
+    double dx = group_sum<LANES>((double)ex), dy = group_sum<LANES>((double)ey), dz = group_sum<LANES>((double)ez);
+    if (lane == 0) {
+        ef[i] = dx;
+        ef[n + i] = dy;
+        ef[2 * n + i] = dz;
+    }
 }

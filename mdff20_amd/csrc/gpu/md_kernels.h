@@ -96,6 +96,52 @@ __global__ void k_scf_rmsd(int n, const int *__restrict__ type, const TypeTab *_
     out[i] = r;
 }
 
+// one block: get_rmsd_mu pim.c:431-449 summed over the ions (and u_pol
+// when upol_i is given), then the loop condition of momentpolaSCF pim.c:236;
+// the sums take the fixed order of k_colsum
+__global__ void __launch_bounds__(1024) k_scf_check(int n, const int *__restrict__ type,
+                                                    const TypeTab *__restrict__ T, const double *__restrict__ mu,
+                                                    const double *__restrict__ ef, const double *__restrict__ upol_i,
+                                                    int *ctl, double *scal, int npol, int maxit, int minit, double tol)
+{
+    if (!ctl[C_SCF_ACTIVE]) return;
+    __shared__ double sr[1024], su[1024];
+    double r = 0.0, u = 0.0;
+    for (int i = threadIdx.x; i < n; i += 1024) {
+        const int t = type[i];
+        if (T->polar[t]) {
+            double ri = 0.0;
+            for (int c = 0; c < 3; c++) {
+                double paa = T->pol[t][4 * c];
+                if (paa != 0.0) {
+                    double e = mu[(size_t)c * n + i] / paa - ef[(size_t)c * n + i];
+                    ri += e * e;
+                }
+            }
+            r += ri;
+        }
+        if (upol_i) u += upol_i[i];
+    }
+    sr[threadIdx.x] = r;
+    su[threadIdx.x] = u;
+    __syncthreads();
+    for (int o = 512; o > 0; o >>= 1) {
+        if (threadIdx.x < o) {
+            sr[threadIdx.x] += sr[threadIdx.x + o];
+            su[threadIdx.x] += su[threadIdx.x + o];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        scal[S_RMSD2] = sr[0];
+        if (upol_i) scal[S_UPOL] = su[0];
+        int iscf = ctl[C_ISCF] + 1;
+        double rmsd = sqrt(sr[0] / (double)(npol > 0 ? npol : 1));
+        ctl[C_ISCF] = iscf;
+        ctl[C_SCF_ACTIVE] = ((iscf < maxit && rmsd > tol) || iscf < minit) ? 1 : 0;
+    }
+}
+
 // loop condition of momentpolaSCF pim.c:236
 __global__ void k_scf_control(int *ctl, const double *scal, int npol, int maxit, int minit, double tol)
 {

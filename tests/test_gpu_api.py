@@ -93,7 +93,9 @@ def test_mixed_form_accuracy(case, katoms):
 @pytest.mark.parametrize("case", ["pim192_static", "bsio_200_nvt", "melt216_nve"])
 def test_mfma_kspace_matches_vector_kernels_fp32(case):
     """FP32 S(k) forward and back-projection on the matrix cores against the
-    FP32 vector-ALU kernels in the mixed form (FP32 rounding level)"""
+    FP32 vector-ALU kernels in the mixed form: both are FP32 roundings of the
+    FP64 sums (the matrix cores accumulate a K range of up to a few hundred
+    ions in FP32 before the FP64 reduction), so they agree to a few 1e-6"""
     print(f"[gpu] mfma vs valu fp32 {case}", flush=True)
     p, cfg, _ = load_case(case)
     p.conv_tol_ind = 1e-10
@@ -104,10 +106,10 @@ def test_mfma_kspace_matches_vector_kernels_fp32(case):
         e0.set_tuning(katoms=-1)
         e1.set_tuning(katoms=2)
         t0, t1 = e0.engforce(0), e1.engforce(0)
-        assert rel(e1.forces(), e0.forces()) < 2e-6
+        assert rel(e1.forces(), e0.forces()) < 5e-6
         assert t1["u_coul"] == pytest.approx(t0["u_coul"], rel=1e-7)
         if t0["u_pol"]:
-            assert rel(e1.dipoles(), e0.dipoles()) < 2e-6
+            assert rel(e1.dipoles(), e0.dipoles()) < 5e-6
 
 
 @pytest.mark.parametrize("case", ["pim192_static", "bsio_200_nvt", "melt216_nve"])
