@@ -163,9 +163,18 @@ int mdff_time_kernel(mdff_ctx *ctx, int which, int lo, int iters, double *ms);
  * ions.  No counterpart in the reference (measurement support). */
 int mdff_get_counters(mdff_ctx *ctx, long long out[4]);
 
-/* Mixed form: FP32 Jacobi sweeps of the SCF correction per FP64 field
- * evaluation (default 6). */
+/* Mixed form: low-precision iterations of the SCF correction per FP64
+ * field evaluation (default 8). */
 int mdff_set_scf_inner(mdff_ctx *ctx, int inner);
+
+/* Mixed form: the solver of the SCF correction equation
+ * (pol^-T - T) d = E - pol^-T mu: 0 Jacobi sweeps (the reference's update,
+ * pim.c:262-268), 1 conjugate gradients preconditioned by the
+ * polarisability (default; used only when every polarisability is
+ * symmetric positive definite, else Jacobi), with inner iterations per FP64
+ * field evaluation.  No counterpart in the reference: the FP64 form keeps
+ * the reference's Jacobi schedule. */
+int mdff_set_scf_solver(mdff_ctx *ctx, int solver, int inner);
 
 #ifdef __cplusplus
 }

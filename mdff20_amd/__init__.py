@@ -82,6 +82,7 @@ def lib():
         "mdff_time_kernel": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, dp]),
         "mdff_get_counters": (C.c_int, [vp, P(C.c_longlong)]),
         "mdff_set_scf_inner": (C.c_int, [vp, C.c_int]),
+        "mdff_set_scf_solver": (C.c_int, [vp, C.c_int, C.c_int]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -289,6 +290,10 @@ class Engine:
 
     def set_scf_inner(self, k):
         _check(self._L.mdff_set_scf_inner(self._h, k), "set_scf_inner", False)
+
+    def set_scf_solver(self, solver, inner):
+        """Mixed form: 0 Jacobi sweeps, 1 preconditioned CG (inner iterations per FP64 evaluation)."""
+        _check(self._L.mdff_set_scf_solver(self._h, solver, inner), "set_scf_solver", False)
 
     def set_tuning(self, lanes=0, katoms=0, ksplit=0):
         _check(self._L.mdff_set_tuning(self._h, lanes, katoms, ksplit), "set_tuning", False)

@@ -85,6 +85,7 @@ struct mdff_ctx {
     double *upol_i = nullptr, *rmsd_i = nullptr;  // n
     double *tmp_q = nullptr, *tmp_mu = nullptr, *tmp_qraw = nullptr, *tmp_mur = nullptr;  // drop-in scratch
     double *dc_R = nullptr, *dc_d = nullptr, *dc_g = nullptr;  // mixed SCF correction
+    double *cg_z = nullptr, *cg_p = nullptr, *cg_part = nullptr; // its conjugate-gradient form
     double *scal = nullptr;
     int *ctl = nullptr;
     TypeTab *tab_d = nullptr;
@@ -144,9 +145,12 @@ struct mdff_ctx {
     int have_events = 0;
     double t_ms[6] = {0, 0, 0, 0, 0, 0};
     int last_nscf = 0;
+    int hint_fp64 = 0, hint_mixed = 0; // field evaluations of the last converged SCF of each form
     double last_rmsd = 0;
     int scf_chunk = 4;
-    int inner_iters = 6;      // mixed form: FP32 Jacobi sweeps per FP64 refinement
+    int inner_iters = 8;      // mixed form: low-precision correction iterations per FP64 refinement
+    int scf_solver = 1;       // mixed-form correction: 0 Jacobi sweeps, 1 preconditioned CG
+    int cg_ok = 0;            // every polarisability symmetric positive definite (CG applies)
     long long n_force = 0, n_scf_fp64 = 0, n_scf_inner = 0;
 };
 
@@ -179,7 +183,23 @@ static void colsum(mdff_ctx *c, const double *in, int n, int ld, int ncols, doub
 }
 
 // ------------------------------------------------------------- type tables
-static void build_tables(mdff_ctx *c) { typetab_build(c->p, c->ntype, c->srcutsq, c->tab); }
+static void build_tables(mdff_ctx *c)
+{
+    typetab_build(c->p, c->ntype, c->srcutsq, c->tab);
+    // the CG correction needs pol^-T - T symmetric positive definite: each
+    // polarisability symmetric with positive leading minors
+    c->cg_ok = 1;
+    for (int t = 0; t < c->ntype; t++) {
+        if (!c->tab.polar[t]) continue;
+        const double *m = c->tab.pol[t];
+        double sc = fabs(m[0]) + fabs(m[4]) + fabs(m[8]);
+        bool sym = fabs(m[1] - m[3]) <= 1e-14 * sc && fabs(m[2] - m[6]) <= 1e-14 * sc && fabs(m[5] - m[7]) <= 1e-14 * sc;
+        double m2 = m[0] * m[4] - m[1] * m[3];
+        double m3 = m[0] * (m[4] * m[8] - m[5] * m[7]) - m[1] * (m[3] * m[8] - m[5] * m[6]) +
+                    m[2] * (m[3] * m[7] - m[4] * m[6]);
+        if (!(sym && m[0] > 0.0 && m2 > 0.0 && m3 > 0.0)) c->cg_ok = 0;
+    }
+}
 
 // ------------------------------------------------------------ neighbour list
 static int build_list(mdff_ctx *c, int force)
@@ -835,8 +855,11 @@ static void scf_finish(mdff_ctx *c)
 }
 
 // the SCF loop of momentpolaSCF pim.c:236-278 on the device (FP64 form,
-// the reference's Jacobi schedule), gated by the convergence flag; the host
-// polls the flag every scf_chunk iterations.
+// the reference's Jacobi schedule), gated by the convergence flag.  The
+// host first launches as many iterations as the last SCF took (consecutive
+// MD steps need about the same number), then polls the flag every
+// scf_chunk iterations; iterations past convergence are skipped on the
+// device, so the schedule never changes the result.
 static int scf_loop(mdff_ctx *c, int istep)
 {
     int n = c->n;
@@ -848,7 +871,8 @@ static int scf_loop(mdff_ctx *c, int istep)
     int launched = 0;
     int limit = std::max(maxit, minit) + 1;
     while (launched < limit) {
-        int chunk = launched == 0 ? std::max(1, std::min(minit, c->scf_chunk)) : c->scf_chunk;
+        int chunk = launched == 0 ? std::max(std::max(1, std::min(minit, c->scf_chunk)), c->hint_fp64)
+                                  : c->scf_chunk;
         for (int it = 0; it < chunk && launched < limit; it++, launched++) {
             int first = launched == 0;
             hipLaunchKernelGGL(k_scf_mu, dim3(cdiv(n, 256)), dim3(256), 0, c->st, n, first, asp, c->type, c->tab_d,
@@ -867,6 +891,7 @@ static int scf_loop(mdff_ctx *c, int istep)
     }
     scf_finish(c);
     c->n_scf_fp64 += c->last_nscf;
+    c->hint_fp64 = c->last_nscf;
     if (c->last_rmsd > c->p.conv_tol_ind) return MDFF_ERR_SCF;
     return MDFF_OK;
 }
@@ -874,10 +899,49 @@ static int scf_loop(mdff_ctx *c, int istep)
 // Mixed form of the SCF: defect correction.  Each outer iteration evaluates
 // the field E = E_stat + T mu of the current dipoles in FP64 and the
 // residual rmsd exactly as get_rmsd_mu (pim.c:431-449); while it is above
-// conv_tol_ind, the Jacobi correction d = R + pol^T (T d) with
-// R = pol^T E - mu is iterated inner_iters times with the FP32 operator and
-// added to mu.  The fixed point is the FP64 one; FP32 only limits how much
-// each outer step gains.  The start is the ASPC predictor (pim.c:382-400).
+// conv_tol_ind, the correction equation (pol^-T - T) d = E - pol^-T mu is
+// solved approximately with the low-precision operator and d is added to
+// mu.  The fixed point is the FP64 one; the low precision only limits how
+// much each outer step gains.  The start is the ASPC predictor
+// (pim.c:382-400).  The correction is inner_iters steps of conjugate
+// gradients preconditioned by pol^T (scf_solver 1, when every
+// polarisability is symmetric positive definite), else inner_iters Jacobi
+// sweeps d = R + pol^T (T d) with R = pol^T E - mu.
+static void scf_correct_jacobi(mdff_ctx *c, const int *act)
+{
+    int n = c->n;
+    hipLaunchKernelGGL(k_dc_residual, dim3(cdiv(n, 256)), dim3(256), 0, c->st, n, c->type, c->tab_d, c->ef, c->mu_ind,
+                       c->dc_R, c->dc_d, act);
+    for (int k = 0; k < c->inner_iters; k++) {
+        induced_field(c, 1, c->dc_d, nullptr, c->dc_g, act);
+        hipLaunchKernelGGL(k_dc_update, dim3(cdiv(n, 256)), dim3(256), 0, c->st, n, c->type, c->tab_d, c->dc_R,
+                           c->dc_g, c->dc_d, act);
+    }
+    hipLaunchKernelGGL(k_axpy_gated, dim3(cdiv(3 * n, 256)), dim3(256), 0, c->st, 3 * n, c->dc_d, c->mu_ind, act);
+}
+
+static void scf_correct_cg(mdff_ctx *c, const int *act)
+{
+    int n = c->n, np = c->npol;
+    int nb = cdiv(np, CG_BLOCK);
+    double *rz[2] = {c->cg_part, c->cg_part + nb}, *pq = c->cg_part + 2 * nb;
+    hipLaunchKernelGGL(k_cg_init, dim3(nb), dim3(CG_BLOCK), 0, c->st, n, np, c->pidx, c->type, c->tab_d, c->ef,
+                       c->mu_ind, c->dc_R, c->cg_z, c->cg_p, c->dc_d, rz[0], act);
+    int cur = 0;
+    for (int k = 0; k < c->inner_iters; k++) {
+        induced_field(c, 1, c->cg_p, nullptr, c->dc_g, act);
+        hipLaunchKernelGGL(k_cg_pq, dim3(nb), dim3(CG_BLOCK), 0, c->st, n, np, c->pidx, c->type, c->tab_d, c->cg_p,
+                           c->dc_g, pq, act);
+        hipLaunchKernelGGL(k_cg_update, dim3(nb), dim3(CG_BLOCK), 0, c->st, n, np, nb, c->pidx, c->type, c->tab_d,
+                           c->cg_p, c->dc_g, c->dc_d, c->dc_R, c->cg_z, rz[cur], pq, rz[cur ^ 1], act);
+        if (k + 1 < c->inner_iters)
+            hipLaunchKernelGGL(k_cg_next, dim3(nb), dim3(CG_BLOCK), 0, c->st, n, np, nb, c->pidx, c->cg_z, c->cg_p,
+                               rz[cur], rz[cur ^ 1], act);
+        cur ^= 1;
+    }
+    hipLaunchKernelGGL(k_cg_apply, dim3(nb), dim3(CG_BLOCK), 0, c->st, n, np, c->pidx, c->dc_d, c->mu_ind, act);
+}
+
 static int scf_loop_mixed(mdff_ctx *c, int istep)
 {
     int n = c->n;
@@ -888,39 +952,48 @@ static int scf_loop_mixed(mdff_ctx *c, int istep)
     if (aspc_args(c, istep, &asp)) return MDFF_ERR_CONFIG;
     hipLaunchKernelGGL(k_scf_mu, dim3(cdiv(n, 256)), dim3(256), 0, c->st, n, 1, asp, c->type, c->tab_d, c->hist, c->ef,
                        c->mu_ind, c->upol_i, act);
+    bool cg = c->scf_solver == 1 && c->cg_ok && c->npol > 0;
+    // FP64 evaluation of the field and residual check, and the correction
+    // between two of them; the host polls the flag after as many
+    // evaluations as the last SCF took (at least two: one correction
+    // usually converges), then after every further one, so a converged SCF
+    // leaves no skipped launches behind
     int launched = 0;
     int limit = std::max(maxit, minit) + 1;
-    while (launched < limit) {
-        int chunk = launched == 0 ? std::max(1, std::min(minit, 2)) : 2;
-        for (int it = 0; it < chunk && launched < limit; it++, launched++) {
-            induced_field(c, 0, c->mu_ind, c->ef_stat, c->ef, act);
-            hipLaunchKernelGGL(k_scf_check, dim3(1), dim3(1024), 0, c->st, n, c->type, c->tab_d, c->mu_ind, c->ef,
-                               (const double *)nullptr, c->ctl, c->scal, c->npol, maxit, minit, c->p.conv_tol_ind);
-            // correction (skipped once converged)
-            hipLaunchKernelGGL(k_dc_residual, dim3(cdiv(n, 256)), dim3(256), 0, c->st, n, c->type, c->tab_d, c->ef,
-                               c->mu_ind, c->dc_R, c->dc_d, act);
-            // the correction sweeps only steer the FP64-residual iteration,
-            // so they may run at split-BF16 accuracy
-            int keep = c->mf_lo;
-            c->mf_lo = c->mf_inner;
-            for (int k = 0; k < c->inner_iters; k++) {
-                induced_field(c, 1, c->dc_d, nullptr, c->dc_g, act);
-                hipLaunchKernelGGL(k_dc_update, dim3(cdiv(n, 256)), dim3(256), 0, c->st, n, c->type, c->tab_d,
-                                   c->dc_R, c->dc_g, c->dc_d, act);
-            }
-            c->mf_lo = keep;
-            hipLaunchKernelGGL(k_axpy_gated, dim3(cdiv(3 * n, 256)), dim3(256), 0, c->st, 3 * n, c->dc_d, c->mu_ind,
-                               act);
-        }
+    auto eval = [&]() {
+        induced_field(c, 0, c->mu_ind, c->ef_stat, c->ef, act);
+        hipLaunchKernelGGL(k_scf_check, dim3(1), dim3(1024), 0, c->st, n, c->type, c->tab_d, c->mu_ind, c->ef,
+                           (const double *)nullptr, c->ctl, c->scal, c->npol, maxit, minit, c->p.conv_tol_ind);
+        launched++;
+    };
+    auto correct = [&]() {
+        // the correction only steers the FP64-residual iteration, so it may
+        // run at split-BF16 accuracy
+        int keep = c->mf_lo;
+        c->mf_lo = c->mf_inner;
+        if (cg) scf_correct_cg(c, act);
+        else scf_correct_jacobi(c, act);
+        c->mf_lo = keep;
+    };
+    eval();
+    int first = std::min(limit, std::max(std::max(2, minit), c->hint_mixed));
+    while (launched < first) {
+        correct();
+        eval();
+    }
+    for (;;) {
         int rc = sync_scal(c);
         if (rc) return rc;
-        if (!c->ctl_h[C_SCF_ACTIVE]) break;
+        if (!c->ctl_h[C_SCF_ACTIVE] || launched >= limit) break;
+        correct();
+        eval();
     }
     // u_pol of the dipoles whose field was evaluated last (pim.c:253,299)
     hipLaunchKernelGGL(k_upol, dim3(cdiv(n, 256)), dim3(256), 0, c->st, n, c->type, c->tab_d, c->mu_ind, c->upol_i);
     colsum(c, c->upol_i, n, n, 1, c->scal + S_UPOL);
     scf_finish(c);
     c->n_scf_fp64 += c->last_nscf;
+    c->hint_mixed = c->last_nscf;
     c->n_scf_inner += (long long)c->inner_iters * std::max(0, c->last_nscf - 1);
     if (c->last_rmsd > c->p.conv_tol_ind) return MDFF_ERR_SCF;
     return MDFF_OK;
@@ -1175,7 +1248,7 @@ void mdff_destroy(mdff_ctx *c)
     double **dp[] = {&c->r, &c->v, &c->f, &c->fold, &c->rs, &c->s, &c->s0, &c->invmass, &c->massd, &c->q, &c->qz,
                      &c->mu_static, &c->mu_ind, &c->mu_tot, &c->hist, &c->ef_stat, &c->ef, &c->ef_dir, &c->ef_rec,
                      &c->efg, &c->efg_dir, &c->efg_rec, &c->f_sr, &c->f_dir, &c->f_rec, &c->acc_sr, &c->uw_dir,
-                     &c->selfbuf, &c->ke, &c->upol_i, &c->rmsd_i, &c->tmp_q, &c->tmp_mu, &c->tmp_qraw, &c->tmp_mur, &c->dc_R, &c->dc_d, &c->dc_g, &c->scal,
+                     &c->selfbuf, &c->ke, &c->upol_i, &c->rmsd_i, &c->tmp_q, &c->tmp_mu, &c->tmp_qraw, &c->tmp_mur, &c->dc_R, &c->dc_d, &c->dc_g, &c->cg_z, &c->cg_p, &c->cg_part, &c->scal,
                      &c->kvec, &c->Ak, &c->kcoe, &c->W, &c->ek, &c->kpart, &c->bpart, &c->ksum};
     for (auto pp : dp) dfree(*pp);
     dfree(c->type);
@@ -1325,6 +1398,7 @@ int mdff_create(mdff_ctx **out, const mdff_params *p, int nion, const int *typia
     ALLOC(c->uw_dir, 7 * n); ALLOC(c->selfbuf, 2 * n); ALLOC(c->ke, n); ALLOC(c->upol_i, n); ALLOC(c->rmsd_i, n);
     ALLOC(c->tmp_q, n); ALLOC(c->tmp_mu, 3 * n); ALLOC(c->tmp_qraw, n); ALLOC(c->tmp_mur, 3 * n);
     ALLOC(c->dc_R, 3 * n); ALLOC(c->dc_d, 3 * n); ALLOC(c->dc_g, 3 * n);
+    ALLOC(c->cg_z, 3 * n); ALLOC(c->cg_p, 3 * n); ALLOC(c->cg_part, 3 * (size_t)cdiv(n, CG_BLOCK));
     ALLOC(c->scal, S_COUNT); ALLOC(c->ctl, C_COUNT); ALLOC(c->tab_d, 1);
     if (hipHostMalloc((void **)&c->ctl_h, sizeof(int) * C_COUNT) != hipSuccess) { mdff_destroy(c); return MDFF_ERR_NOMEM; }
     if (hipHostMalloc((void **)&c->scal_h, sizeof(double) * S_COUNT) != hipSuccess) { mdff_destroy(c); return MDFF_ERR_NOMEM; }
@@ -1919,6 +1993,14 @@ int mdff_get_counters(mdff_ctx *c, long long out[4])
 int mdff_set_scf_inner(mdff_ctx *c, int inner)
 {
     if (!c || inner < 0 || inner > 64) return MDFF_ERR_ARG;
+    c->inner_iters = inner;
+    return MDFF_OK;
+}
+
+int mdff_set_scf_solver(mdff_ctx *c, int solver, int inner)
+{
+    if (!c || solver < 0 || solver > 1 || inner < 0 || inner > 64) return MDFF_ERR_ARG;
+    c->scf_solver = solver;
     c->inner_iters = inner;
     return MDFF_OK;
 }

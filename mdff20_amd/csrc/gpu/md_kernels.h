@@ -411,3 +411,155 @@ __global__ void k_upol(int n, const int *__restrict__ type, const TypeTab *__res
     }
     out[i] = 0.5 * up;
 }
+
+// ---------------------------------------------------------------------------
+// Mixed-form SCF correction by preconditioned conjugate gradients on
+//   (pol^-T - T) d = E - pol^-T mu      (mu = pol^T E at the fixed point;
+// SPD for a symmetric, positive polarisability and a converging SCF)
+// with pol^T as preconditioner.  Vectors live on the
+// polarisable rows (ion pidx[k]); dot products are per-block partials that
+// every block of the next kernel sums in the same fixed order, so the
+// whole iteration stays on the device.
+#define CG_BLOCK 256
+
+__device__ __forceinline__ double cg_block_sum(double v, double *sh)
+{
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int o = CG_BLOCK / 2; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+        __syncthreads();
+    }
+    double r = sh[0];
+    __syncthreads();
+    return r;
+}
+
+__device__ __forceinline__ double cg_sum_partials(const double *part, int nb, double *sh)
+{
+    double v = 0.0;
+    for (int b = threadIdx.x; b < nb; b += CG_BLOCK) v += part[b];
+    return cg_block_sum(v, sh);
+}
+
+// r = E - pol^-T mu, z = p = pol^T r (= pol^T E - mu), d = 0; partials of r.z
+__global__ void __launch_bounds__(CG_BLOCK) k_cg_init(int n, int npol, const int *__restrict__ pidx,
+                                                      const int *__restrict__ type, const TypeTab *__restrict__ T,
+                                                      const double *__restrict__ ef, const double *__restrict__ mu,
+                                                      double *r, double *z, double *p, double *d, double *rz_part,
+                                                      const int *active)
+{
+    if (!*active) return;
+    __shared__ double sh[CG_BLOCK];
+    const int k = blockIdx.x * CG_BLOCK + threadIdx.x;
+    double acc = 0.0;
+    if (k < npol) {
+        const int i = pidx[k], t = type[i];
+        const double *P = T->pol[t], *I = T->invpol[t];
+        double m[3] = {mu[i], mu[n + i], mu[2 * n + i]};
+        double rr[3], zz[3];
+        for (int a = 0; a < 3; a++) rr[a] = ef[(size_t)a * n + i] - (I[a] * m[0] + I[3 + a] * m[1] + I[6 + a] * m[2]);
+        for (int a = 0; a < 3; a++) zz[a] = P[0 * 3 + a] * rr[0] + P[1 * 3 + a] * rr[1] + P[2 * 3 + a] * rr[2];
+        for (int a = 0; a < 3; a++) {
+            const size_t o = (size_t)a * n + i;
+            r[o] = rr[a]; z[o] = zz[a]; p[o] = zz[a]; d[o] = 0.0;
+            acc += rr[a] * zz[a];
+        }
+    }
+    double s = cg_block_sum(acc, sh);
+    if (threadIdx.x == 0) rz_part[blockIdx.x] = s;
+}
+
+// partials of p.q, q = pol^-T p - Tp (Tp: field of the dipoles p)
+__global__ void __launch_bounds__(CG_BLOCK) k_cg_pq(int n, int npol, const int *__restrict__ pidx,
+                                                    const int *__restrict__ type, const TypeTab *__restrict__ T,
+                                                    const double *__restrict__ p, const double *__restrict__ Tp,
+                                                    double *pq_part, const int *active)
+{
+    if (!*active) return;
+    __shared__ double sh[CG_BLOCK];
+    const int k = blockIdx.x * CG_BLOCK + threadIdx.x;
+    double acc = 0.0;
+    if (k < npol) {
+        const int i = pidx[k], t = type[i];
+        const double *I = T->invpol[t];
+        double pv[3] = {p[i], p[n + i], p[2 * n + i]};
+        for (int a = 0; a < 3; a++) {
+            double q = I[a] * pv[0] + I[3 + a] * pv[1] + I[6 + a] * pv[2] - Tp[(size_t)a * n + i];
+            acc += pv[a] * q;
+        }
+    }
+    double s = cg_block_sum(acc, sh);
+    if (threadIdx.x == 0) pq_part[blockIdx.x] = s;
+}
+
+// gamma = r.z / p.q; d += gamma p; r -= gamma q; z = pol^T r; partials of
+// the new r.z
+__global__ void __launch_bounds__(CG_BLOCK) k_cg_update(int n, int npol, int nb, const int *__restrict__ pidx,
+                                                        const int *__restrict__ type, const TypeTab *__restrict__ T,
+                                                        const double *__restrict__ p, const double *__restrict__ Tp,
+                                                        double *d, double *r, double *z,
+                                                        const double *__restrict__ rz_old,
+                                                        const double *__restrict__ pq_part, double *rz_new,
+                                                        const int *active)
+{
+    if (!*active) return;
+    __shared__ double sh[CG_BLOCK];
+    const double rz = cg_sum_partials(rz_old, nb, sh);
+    const double pq = cg_sum_partials(pq_part, nb, sh);
+    const double g = (pq != 0.0) ? rz / pq : 0.0;
+    const int k = blockIdx.x * CG_BLOCK + threadIdx.x;
+    double acc = 0.0;
+    if (k < npol) {
+        const int i = pidx[k], t = type[i];
+        const double *P = T->pol[t], *I = T->invpol[t];
+        double pv[3] = {p[i], p[n + i], p[2 * n + i]};
+        double rr[3];
+        for (int a = 0; a < 3; a++) {
+            const size_t o = (size_t)a * n + i;
+            double q = I[a] * pv[0] + I[3 + a] * pv[1] + I[6 + a] * pv[2] - Tp[o];
+            d[o] += g * pv[a];
+            rr[a] = r[o] - g * q;
+            r[o] = rr[a];
+        }
+        for (int a = 0; a < 3; a++) {
+            double zz = P[0 * 3 + a] * rr[0] + P[1 * 3 + a] * rr[1] + P[2 * 3 + a] * rr[2];
+            z[(size_t)a * n + i] = zz;
+            acc += rr[a] * zz;
+        }
+    }
+    double s = cg_block_sum(acc, sh);
+    if (threadIdx.x == 0) rz_new[blockIdx.x] = s;
+}
+
+// beta = r.z (new) / r.z (old); p = z + beta p
+__global__ void __launch_bounds__(CG_BLOCK) k_cg_next(int n, int npol, int nb, const int *__restrict__ pidx,
+                                                      const double *__restrict__ z, double *p,
+                                                      const double *__restrict__ rz_old,
+                                                      const double *__restrict__ rz_new, const int *active)
+{
+    if (!*active) return;
+    __shared__ double sh[CG_BLOCK];
+    const double ro = cg_sum_partials(rz_old, nb, sh);
+    const double rn = cg_sum_partials(rz_new, nb, sh);
+    const double b = (ro != 0.0) ? rn / ro : 0.0;
+    const int k = blockIdx.x * CG_BLOCK + threadIdx.x;
+    if (k < npol) {
+        const int i = pidx[k];
+        for (int a = 0; a < 3; a++) {
+            const size_t o = (size_t)a * n + i;
+            p[o] = z[o] + b * p[o];
+        }
+    }
+}
+
+// mu += d on the polarisable rows
+__global__ void k_cg_apply(int n, int npol, const int *__restrict__ pidx, const double *__restrict__ d, double *mu,
+                           const int *active)
+{
+    if (!*active) return;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= npol) return;
+    const int i = pidx[k];
+    for (int a = 0; a < 3; a++) mu[(size_t)a * n + i] += d[(size_t)a * n + i];
+}

@@ -91,6 +91,32 @@ def test_mixed_form_accuracy(case, katoms):
 
 
 @pytest.mark.parametrize("case", ["pim192_static", "bsio_200_nvt", "melt216_nve"])
+def test_mixed_scf_solvers_reach_the_fp64_fixed_point(case):
+    """the mixed-form correction solved by preconditioned CG (default) and by
+    Jacobi sweeps: both stop on the same FP64 residual, so their dipoles
+    agree with each other and with the FP64 form to the SCF tolerance, and
+    CG needs no more FP64 field evaluations than Jacobi"""
+    print(f"[gpu] mixed solvers {case}", flush=True)
+    p, cfg, _ = load_case(case)
+    p.conv_tol_ind = 1e-10
+    p.max_scf_pim_iter = 400
+    p.min_scf_pim_iter = 1
+    with m.Engine(p, cfg.typia, cfg.A, cfg.r, precision=m.FP64) as ef, \
+            m.Engine(p, cfg.typia, cfg.A, cfg.r, precision=m.MIXED) as ej, \
+            m.Engine(p, cfg.typia, cfg.A, cfg.r, precision=m.MIXED) as ec:
+        ej.set_scf_solver(0, 6)
+        ec.set_scf_solver(1, 6)
+        tf, tj, tc = ef.engforce(0), ej.engforce(0), ec.engforce(0)
+        assert tf["status"] == 0 and tj["status"] == 0 and tc["status"] == 0
+        assert tj["rmsd"] <= 1e-10 and tc["rmsd"] <= 1e-10
+        muf = ef.dipoles()
+        assert rel(ec.dipoles(), muf) < 1e-5
+        assert rel(ec.dipoles(), ej.dipoles()) < 1e-5
+        assert rel(ec.forces(), ef.forces()) < 1e-5
+        assert ec.counters()["scf_fp64"] <= ej.counters()["scf_fp64"]
+
+
+@pytest.mark.parametrize("case", ["pim192_static", "bsio_200_nvt", "melt216_nve"])
 def test_mfma_kspace_matches_vector_kernels_fp32(case):
     """FP32 S(k) forward and back-projection on the matrix cores against the
     FP32 vector-ALU kernels in the mixed form: both are FP32 roundings of the
