@@ -175,6 +175,110 @@ def cpu_baseline(p_text, L, pos, typ, n_scf, threads):
                 t_force=res["t_force"])
 
 
+# the S(k) GEMM kernels behind mdff_time_kernel's (which, lo): kernel-name
+# prefix (the template's first argument is the arithmetic mode: 0 FP64,
+# 1 split-BF16, 2 FP32)
+TIMED = {"sk_forward_scf": (0, "k_gemm_fwd"), "sk_back_scf": (1, "k_gemm_back"),
+         "sk_forward_force": (3, "k_gemm_fwd"), "sk_back_force": (4, "k_gemm_back")}
+VARIANT = {"f64": (0, 0), "f32": (1, 2), "bf16x3": (2, 1)}   # name -> (lo, template mode)
+CALIB_BYTES = 512 << 20
+
+
+def kernel_prefix(kname):
+    name, var = kname.split("/")
+    which, pre = TIMED[name]
+    lo, mode = VARIANT[var]
+    return which, lo, f"{pre}<{mode},"
+
+
+def traffic_file(kname):
+    name, var = kname.split("/")
+    return os.path.join(ROOT, "profiles", f"{name}_{var}_pim12288_traffic.json")
+
+
+def pmc_run(args, kname):
+    """one plain device copy of CALIB_BYTES (the counters' scale) and then
+    `--pmc-iters` launches of one S(k) GEMM kernel at the benchmark state;
+    run under rocprofv3 --pmc FETCH_SIZE and again with WRITE_SIZE"""
+    import ctypes
+    import mdff20_amd as m
+    which, lo, _ = kernel_prefix(kname)
+    L, pos, typ = system(args.units, args.seed)
+    r = equilibrate(L, pos, typ, args.relax)
+    A = np.diag([L, L, L])
+    p = params_of(control(True))
+    e = m.Engine(p, typ, A, r, precision=m.MIXED)
+    e.engforce(0)
+    # the runtime's own device-to-device copy kernel over CALIB_BYTES
+    hip = ctypes.CDLL("libamdhip64.so")
+    src, dst = ctypes.c_void_p(), ctypes.c_void_p()
+    for ptr in (src, dst):
+        if hip.hipMalloc(ctypes.byref(ptr), ctypes.c_size_t(CALIB_BYTES)) != 0:
+            raise RuntimeError("hipMalloc failed")
+    if (hip.hipMemset(src, 0, ctypes.c_size_t(CALIB_BYTES)) != 0 or hip.hipDeviceSynchronize() != 0 or
+            hip.hipMemcpy(dst, src, ctypes.c_size_t(CALIB_BYTES), 3) != 0 or hip.hipDeviceSynchronize() != 0):
+        raise RuntimeError("calibration copy failed")
+    hip.hipFree(src)
+    hip.hipFree(dst)
+    log(f"calibration copy: {CALIB_BYTES} bytes read and written")
+    ms = e.time_kernel(which, lo=lo, iters=args.pmc_iters)
+    log(f"{kname}: {args.pmc_iters} launches, {ms:.4f} ms each")
+    e.close()
+
+
+def read_counters(d):
+    import csv
+    import glob
+    rows = []
+    for f in glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True):
+        rows += list(csv.DictReader(open(f)))
+    return rows
+
+
+def pmc_summarize(args, kname):
+    """traffic of one kernel launch from the two counter runs: per-dispatch
+    FETCH_SIZE and WRITE_SIZE (KiB) of the kernel's timed launches, each
+    divided by what the same counter reads for the calibration copy per byte
+    it moves"""
+    _, _, prefix = kernel_prefix(kname)
+    out = {"kernel": kname, "kernel_prefix": prefix}
+    for counter, d in (("FETCH_SIZE", args.pmc_summarize[0]), ("WRITE_SIZE", args.pmc_summarize[1])):
+        rows = [x for x in read_counters(d) if x.get("Counter_Name") == counter]
+        by = {}
+        for x in rows:
+            by.setdefault((x["Dispatch_Id"], x["Kernel_Name"]), 0.0)
+            by[(x["Dispatch_Id"], x["Kernel_Name"])] += float(x["Counter_Value"])
+        disp = sorted(((int(k[0]), k[1], v) for k, v in by.items()), key=lambda t: t[0])
+        copies = [v for _, nm, v in disp if "opy" in nm and v * 1024 > 0.5 * CALIB_BYTES]
+        if not copies:
+            raise RuntimeError(f"{counter}: no calibration copy dispatch found in {d}")
+        factor = max(copies) * 1024.0 / CALIB_BYTES
+        ks = [v for _, nm, v in disp if prefix in nm][-args.pmc_iters:]
+        if not ks:
+            raise RuntimeError(f"{counter}: no dispatch of {prefix} in {d}")
+        kib = sum(ks) / len(ks)
+        out[counter] = dict(kib_per_dispatch=kib, dispatches=len(ks), copy_factor=factor,
+                            bytes=kib * 1024.0 / factor)
+    out["traffic_bytes"] = out["FETCH_SIZE"]["bytes"] + out["WRITE_SIZE"]["bytes"]
+    with open(traffic_file(kname), "w") as fh:
+        json.dump(out, fh, indent=1)
+    print(json.dumps(out, indent=1))
+
+
+def trace_summarize(d, top=40):
+    """markdown table of a rocprofv3 --kernel-trace --stats run"""
+    import csv
+    import glob
+    f = glob.glob(os.path.join(d, "**", "*kernel_stats.csv"), recursive=True)[0]
+    rows = list(csv.DictReader(open(f)))
+    print("| kernel | calls | total ms | % | avg us | min us | max us |")
+    print("|---|---:|---:|---:|---:|---:|---:|")
+    for x in rows[:top]:
+        print(f"| `{x['Name'][:100]}` | {x['Calls']} | {float(x['TotalDurationNs']) / 1e6:.2f} | "
+              f"{float(x['Percentage']):.1f} | {float(x['AverageNs']) / 1e3:.2f} | {float(x['MinNs']) / 1e3:.2f} | "
+              f"{float(x['MaxNs']) / 1e3:.2f} |")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -186,7 +290,17 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "16")))
     ap.add_argument("--forms", default="fp64,mixed")
+    ap.add_argument("--pmc-kernel", default=None, help="e.g. sk_forward_scf/bf16x3 (counter runs)")
+    ap.add_argument("--pmc-iters", type=int, default=5)
+    ap.add_argument("--pmc-summarize", nargs=2, default=None, metavar=("FETCH_DIR", "WRITE_DIR"))
+    ap.add_argument("--trace-summarize", default=None, metavar="DIR")
     args = ap.parse_args()
+    if args.trace_summarize:
+        trace_summarize(args.trace_summarize)
+        return
+    if args.pmc_summarize:
+        pmc_summarize(args, args.pmc_kernel)
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (every box of this build has one GPU); --gpus {args.gpus} "
               f"is not supported: the multi-GPU sharding of this path is described in DESIGN.md", file=sys.stderr)
@@ -194,6 +308,9 @@ def main():
     import mdff20_amd as m
     if m.device_count() < 1:
         raise RuntimeError("no HIP device")
+    if args.pmc_kernel:
+        pmc_run(args, args.pmc_kernel)
+        return
     L, pos, typ = system(args.units, args.seed)
     n = len(typ)
     log(f"N = {n}, L = {L:.3f} A")
@@ -273,8 +390,13 @@ def main():
         kv = kern[kname]
         ach = kv["flops"] / (kv["ms"] * 1e-3)
         name, vname = kname.split("/")
+        traffic, tsrc = None, None
+        tf = traffic_file(kname)
+        if os.path.exists(tf):
+            traffic = json.load(open(tf))["traffic_bytes"]
+            tsrc = os.path.relpath(tf, ROOT)
         roof = dict(bound="mfma", achieved=ach / 1e12, peak=kv["peak"] / 1e12, unit="TFLOP/s",
-                    frac=ach / kv["peak"], traffic=None, kernel=kname,
+                    frac=ach / kv["peak"], traffic=traffic, traffic_source=tsrc, kernel=kname,
                     work=(f"8 flops x {npol if 'scf' in name else n} ions x {nk} k-vectors x "
                           f"{3 if 'scf' in name else (4 if 'forward' in name else 9)} channels per launch"))
     cpu = None

@@ -145,9 +145,12 @@ int mdff_set_kspace_shape(mdff_ctx *ctx, int cw64, int cw32, int fwd_blocks, int
 int mdff_get_timing(mdff_ctx *ctx, double t[6]);
 
 /* Time one kernel of the engine at the current state: `iters` launches
- * back to back on the context stream between HIP events; ms = average per
- * launch.  lo = 1 times the FP32 variant, lo = 2 the split-BF16 matrix-core
- * variant (mixed contexts only).  which:
+ * back to back on the context stream; ms = average per launch.  The
+ * matrix-core S(k) GEMM kernels (which 0, 1, 3, 4 on the matrix-core path)
+ * are timed alone, each launch between its own pair of HIP events (operand
+ * builds and the partial-sum pass excluded); the others between two events
+ * around all launches.  lo = 1 times the FP32 variant, lo = 2 the
+ * split-BF16 matrix-core variant (mixed contexts only).  which:
  *   0 S(k) forward, SCF form (dipoles of the polarisable ions, 3 channels)
  *   1 back-projection, SCF form (field at the polarisable ions)
  *   2 real-space induced dipole field

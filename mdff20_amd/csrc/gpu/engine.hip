@@ -142,6 +142,8 @@ struct mdff_ctx {
     double yosh[7];
     // timing
     hipEvent_t ev[8];
+    hipEvent_t *probe_ev = nullptr; // mdff_time_kernel: events around each k-space GEMM launch
+    int probe_k = 0, probe_max = 0;
     int have_events = 0;
     double t_ms[6] = {0, 0, 0, 0, 0, 0};
     int last_nscf = 0;
@@ -170,6 +172,14 @@ static int sync_scal(mdff_ctx *c)
     HIPCHK(hipMemcpyAsync(c->ctl_h, c->ctl, sizeof(int) * C_COUNT, hipMemcpyDeviceToHost, c->st));
     HIPCHK(hipStreamSynchronize(c->st));
     return MDFF_OK;
+}
+
+// mdff_time_kernel brackets each k-space GEMM launch with a pair of events
+static void probe_mark(mdff_ctx *c, int end)
+{
+    if (!c->probe_ev || c->probe_k >= c->probe_max) return;
+    (void)hipEventRecord(c->probe_ev[2 * c->probe_k + end], c->st);
+    if (end) c->probe_k++;
 }
 
 static void set_int(mdff_ctx *c, int slot, int val)
@@ -600,9 +610,11 @@ static int launch_fwd_gemm(mdff_ctx *c, int polar, int lo, int C, const double *
     a.kchunk = ((a.ldk + nchunk - 1) / nchunk + GK - 1) / GK * GK;
     nchunk = (a.ldk + a.kchunk - 1) / a.kchunk;
     dim3 grid(gx, nchunk);
+    probe_mark(c, 0);
     if (mode == KM_F64) gfwd_cw<KM_F64>(c, grid, CW, MT, a);
     else if (mode == KM_F32) gfwd_cw<KM_F32>(c, grid, CW, MT, a);
     else gfwd_cw<KM_BF16X3>(c, grid, CW, MT, a);
+    probe_mark(c, 1);
     // chunk sums, all threads busy (the per-k reduction then reads one chunk)
     const size_t nval = c->nkf * 2 * C;
     if (mode == KM_F64)
@@ -655,9 +667,11 @@ static int launch_back_gemm(mdff_ctx *c, int polar, int lo, int NCH, const int *
     b.kchunk = ((b.ldk + ns - 1) / ns + GK - 1) / GK * GK;
     ns = (b.ldk + b.kchunk - 1) / b.kchunk;
     dim3 grid(gx, ns);
+    probe_mark(c, 0);
     if (mode == KM_F64) gback_ct<KM_F64, double>(c, grid, CT, b);
     else if (mode == KM_F32) gback_ct<KM_F32, float>(c, grid, CT, b);
     else gback_ct<KM_BF16X3, float>(c, grid, CT, b);
+    probe_mark(c, 1);
     *nsplit_out = ns;
     return 1;
 }
@@ -1969,6 +1983,17 @@ int mdff_time_kernel(mdff_ctx *c, int which, int lo, int iters, double *ms)
     };
     once();
     c->fwd_part = 0;
+    // the k-space GEMM kernels are timed alone (operand builds and the
+    // partial-sum pass excluded), each launch between its own two events
+    std::vector<hipEvent_t> pe;
+    if (which == 0 || which == 1 || which == 3 || which == 4) {
+        pe.resize(2 * (size_t)iters);
+        for (auto &e : pe)
+            if (hipEventCreate(&e) != hipSuccess) e = nullptr;
+        bool ok = true;
+        for (auto e : pe) ok = ok && e != nullptr;
+        if (ok) { c->probe_ev = pe.data(); c->probe_k = 0; c->probe_max = iters; }
+    }
     HIPCHK(hipEventRecord(c->ev[6], c->st));
     for (int k = 0; k < iters; k++) once();
     HIPCHK(hipEventRecord(c->ev[7], c->st));
@@ -1976,6 +2001,18 @@ int mdff_time_kernel(mdff_ctx *c, int which, int lo, int iters, double *ms)
     float t = 0.0f;
     HIPCHK(hipEventElapsedTime(&t, c->ev[6], c->ev[7]));
     *ms = (double)t / iters;
+    if (c->probe_ev && c->probe_k == iters) {
+        double sum = 0.0;
+        for (int k = 0; k < iters; k++) {
+            float tk = 0.0f;
+            if (hipEventElapsedTime(&tk, pe[2 * k], pe[2 * k + 1]) == hipSuccess) sum += tk;
+        }
+        *ms = sum / iters;
+    }
+    c->probe_ev = nullptr;
+    c->probe_k = c->probe_max = 0;
+    for (auto e : pe)
+        if (e) (void)hipEventDestroy(e);
     c->fwd_part = 0;
     return MDFF_OK;
 }
