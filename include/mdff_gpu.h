@@ -170,13 +170,23 @@ int mdff_get_counters(mdff_ctx *ctx, long long out[4]);
  * field evaluation (default 8). */
 int mdff_set_scf_inner(mdff_ctx *ctx, int inner);
 
-/* Mixed form: the solver of the SCF correction equation
- * (pol^-T - T) d = E - pol^-T mu: 0 Jacobi sweeps (the reference's update,
- * pim.c:262-268), 1 conjugate gradients preconditioned by the
- * polarisability (default; used only when every polarisability is
- * symmetric positive definite, else Jacobi), with inner iterations per FP64
- * field evaluation.  No counterpart in the reference: the FP64 form keeps
- * the reference's Jacobi schedule. */
+/* Mixed form: the SCF solver.
+ *   0  defect correction: FP64 field and rmsd of the current dipoles, then
+ *      `inner` low-precision Jacobi sweeps (the reference's update,
+ *      pim.c:262-268) on the correction equation
+ *      (pol^-T - T) d = E - pol^-T mu; repeated until the FP64 rmsd passes
+ *   1  the same with `inner` steps of conjugate gradients preconditioned by
+ *      the polarisability
+ *   2  anchored conjugate gradients (default): one FP64 field evaluation of
+ *      the ASPC predictor, then preconditioned CG steps that apply the
+ *      low-precision T to the search direction only and carry the field
+ *      along; the rmsd of get_rmsd_mu (pim.c:431-449) is formed from the
+ *      updated dipoles and field after every step, one SCF iteration per
+ *      step; `inner` is the number of steps launched before the first
+ *      convergence poll of a context (later calls use the last count)
+ * Solvers 1 and 2 need every polarisability symmetric positive definite;
+ * otherwise solver 0 runs.  No counterpart in the reference: the FP64 form
+ * keeps the reference's Jacobi schedule. */
 int mdff_set_scf_solver(mdff_ctx *ctx, int solver, int inner);
 
 #ifdef __cplusplus

@@ -292,7 +292,8 @@ class Engine:
         _check(self._L.mdff_set_scf_inner(self._h, k), "set_scf_inner", False)
 
     def set_scf_solver(self, solver, inner):
-        """Mixed form: 0 Jacobi sweeps, 1 preconditioned CG (inner iterations per FP64 evaluation)."""
+        """Mixed form: 0 / 1 Jacobi / CG defect correction (inner steps per FP64 evaluation),
+        2 anchored CG (one FP64 evaluation per SCF; the default)."""
         _check(self._L.mdff_set_scf_solver(self._h, solver, inner), "set_scf_solver", False)
 
     def set_tuning(self, lanes=0, katoms=0, ksplit=0):

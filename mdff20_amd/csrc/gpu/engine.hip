@@ -147,11 +147,11 @@ struct mdff_ctx {
     int have_events = 0;
     double t_ms[6] = {0, 0, 0, 0, 0, 0};
     int last_nscf = 0;
-    int hint_fp64 = 0, hint_mixed = 0; // field evaluations of the last converged SCF of each form
+    int hint_fp64 = 0, hint_mixed = 0, hint_cg = 0; // iterations of the last converged SCF of each form
     double last_rmsd = 0;
     int scf_chunk = 4;
     int inner_iters = 8;      // mixed form: low-precision correction iterations per FP64 refinement
-    int scf_solver = 1;       // mixed-form correction: 0 Jacobi sweeps, 1 preconditioned CG
+    int scf_solver = 2;       // mixed form: 0/1 Jacobi/CG defect correction, 2 anchored CG
     int cg_ok = 0;            // every polarisability symmetric positive definite (CG applies)
     long long n_force = 0, n_scf_fp64 = 0, n_scf_inner = 0;
 };
@@ -966,7 +966,7 @@ static int scf_loop_mixed(mdff_ctx *c, int istep)
     if (aspc_args(c, istep, &asp)) return MDFF_ERR_CONFIG;
     hipLaunchKernelGGL(k_scf_mu, dim3(cdiv(n, 256)), dim3(256), 0, c->st, n, 1, asp, c->type, c->tab_d, c->hist, c->ef,
                        c->mu_ind, c->upol_i, act);
-    bool cg = c->scf_solver == 1 && c->cg_ok && c->npol > 0;
+    bool cg = c->scf_solver >= 1 && c->cg_ok && c->npol > 0;
     // FP64 evaluation of the field and residual check, and the correction
     // between two of them; the host polls the flag after as many
     // evaluations as the last SCF took (at least two: one correction
@@ -1013,6 +1013,77 @@ static int scf_loop_mixed(mdff_ctx *c, int istep)
     return MDFF_OK;
 }
 
+// Mixed form, anchored conjugate gradients (the default): one FP64
+// evaluation of the field of the ASPC predictor (pim.c:382-400), then
+// conjugate-gradient steps on (pol^-T - T) mu = E_stat preconditioned by
+// pol^T, each applying the low-precision T to the search direction only and
+// carrying the field along (E <- E + gamma T p).  The rmsd of get_rmsd_mu
+// (pim.c:431-449) is formed from the updated dipoles and field after every
+// step and the loop condition of pim.c:236 applied to it on the device, one
+// SCF iteration per step.  The low precision enters only through T d, where
+// d is the change from the predictor, so the field error it leaves is of
+// the order of that precision times |T d|, far below the tolerance for a
+// predictor that is close.  The host launches as many steps as the last
+// SCF took before it polls.
+static int scf_loop_anchored(mdff_ctx *c, int istep)
+{
+    int n = c->n, np = c->npol;
+    int *ctl = c->ctl;
+    const int *act = c->ctl + C_SCF_ACTIVE;
+    int maxit = c->p.max_scf_pim_iter, minit = c->p.min_scf_pim_iter;
+    hipLaunchKernelGGL(k_scf_begin, dim3(1), dim3(1), 0, c->st, ctl, (maxit > 0 || minit > 0) ? 1 : 0);
+    AspcArgs asp;
+    if (aspc_args(c, istep, &asp)) return MDFF_ERR_CONFIG;
+    hipLaunchKernelGGL(k_scf_mu, dim3(cdiv(n, 256)), dim3(256), 0, c->st, n, 1, asp, c->type, c->tab_d, c->hist, c->ef,
+                       c->mu_ind, c->upol_i, act);
+    induced_field(c, 0, c->mu_ind, c->ef_stat, c->ef, act);
+    hipLaunchKernelGGL(k_scf_check, dim3(1), dim3(1024), 0, c->st, n, c->type, c->tab_d, c->mu_ind, c->ef,
+                       (const double *)nullptr, ctl, c->scal, np, maxit, minit, c->p.conv_tol_ind);
+    const int nb = cdiv(np, CG_BLOCK);
+    double *rz[2] = {c->cg_part, c->cg_part + nb}, *pq = c->cg_part + 2 * nb, *rm = c->cg_part + 3 * nb;
+    hipLaunchKernelGGL(k_cg_init, dim3(nb), dim3(CG_BLOCK), 0, c->st, n, np, c->pidx, c->type, c->tab_d, c->ef,
+                       c->mu_ind, c->dc_R, c->cg_z, c->cg_p, c->dc_d, rz[0], act);
+    int cur = 0, launched = 1;
+    const int limit = std::max(maxit, minit) + 1;
+    auto step = [&]() {
+        int keep = c->mf_lo;
+        c->mf_lo = c->mf_inner;
+        induced_field(c, 1, c->cg_p, nullptr, c->dc_g, act);
+        c->mf_lo = keep;
+        hipLaunchKernelGGL(k_cg_pq, dim3(nb), dim3(CG_BLOCK), 0, c->st, n, np, c->pidx, c->type, c->tab_d, c->cg_p,
+                           c->dc_g, pq, act);
+        hipLaunchKernelGGL(k_cga_update, dim3(nb), dim3(CG_BLOCK), 0, c->st, n, np, nb, c->pidx, c->type, c->tab_d,
+                           c->cg_p, c->dc_g, c->mu_ind, c->ef, c->dc_R, c->cg_z, rz[cur], pq, rz[cur ^ 1], rm, act);
+        hipLaunchKernelGGL(k_cga_next, dim3(nb), dim3(CG_BLOCK), 0, c->st, n, np, nb, c->pidx, c->cg_z, c->cg_p,
+                           rz[cur], rz[cur ^ 1], rm, ctl, c->scal, maxit, minit, c->p.conv_tol_ind);
+        cur ^= 1;
+        launched++;
+    };
+    int first = std::min(limit, 1 + std::max(1, c->hint_cg > 0 ? c->hint_cg : c->inner_iters));
+    while (launched < first) step();
+    for (;;) {
+        int rc = sync_scal(c);
+        if (rc) return rc;
+        if (!c->ctl_h[C_SCF_ACTIVE] || launched >= limit) break;
+        step();
+    }
+    // u_pol of the final dipoles (pim.c:253,299)
+    hipLaunchKernelGGL(k_upol, dim3(cdiv(n, 256)), dim3(256), 0, c->st, n, c->type, c->tab_d, c->mu_ind, c->upol_i);
+    colsum(c, c->upol_i, n, n, 1, c->scal + S_UPOL);
+    scf_finish(c);
+    c->n_scf_fp64 += 1;
+    c->n_scf_inner += std::max(0, c->last_nscf - 1);
+    c->hint_cg = std::max(1, c->last_nscf - 1);
+    if (c->last_rmsd > c->p.conv_tol_ind) return MDFF_ERR_SCF;
+    return MDFF_OK;
+}
+
+static int scf_mixed(mdff_ctx *c, int istep)
+{
+    if (c->scf_solver == 2 && c->cg_ok && c->npol > 0) return scf_loop_anchored(c, istep);
+    return scf_loop_mixed(c, istep);
+}
+
 // final multipole_ES of engforce (field.c:381-383): forces and stress
 static void coulomb_force(mdff_ctx *c, int lo)
 {
@@ -1052,7 +1123,7 @@ static int engforce_impl(mdff_ctx *c, int istep)
             if ((rc = build_t2(c)) != MDFF_OK) return rc;
             static_field(c, lo);
             if (c->have_events) HIPCHK(hipEventRecord(c->ev[2], c->st));
-            rc = lo ? scf_loop_mixed(c, istep) : scf_loop(c, istep);
+            rc = lo ? scf_mixed(c, istep) : scf_loop(c, istep);
             if (rc != MDFF_OK && rc != MDFF_ERR_SCF) return rc;
             c->ldip_sticky = 1;
             hipLaunchKernelGGL(k_axpy3, dim3(cdiv(3 * n, 256)), dim3(256), 0, c->st, 3 * n, c->mu_static, c->mu_ind,
@@ -1412,7 +1483,7 @@ int mdff_create(mdff_ctx **out, const mdff_params *p, int nion, const int *typia
     ALLOC(c->uw_dir, 7 * n); ALLOC(c->selfbuf, 2 * n); ALLOC(c->ke, n); ALLOC(c->upol_i, n); ALLOC(c->rmsd_i, n);
     ALLOC(c->tmp_q, n); ALLOC(c->tmp_mu, 3 * n); ALLOC(c->tmp_qraw, n); ALLOC(c->tmp_mur, 3 * n);
     ALLOC(c->dc_R, 3 * n); ALLOC(c->dc_d, 3 * n); ALLOC(c->dc_g, 3 * n);
-    ALLOC(c->cg_z, 3 * n); ALLOC(c->cg_p, 3 * n); ALLOC(c->cg_part, 3 * (size_t)cdiv(n, CG_BLOCK));
+    ALLOC(c->cg_z, 3 * n); ALLOC(c->cg_p, 3 * n); ALLOC(c->cg_part, 4 * (size_t)cdiv(n, CG_BLOCK));
     ALLOC(c->scal, S_COUNT); ALLOC(c->ctl, C_COUNT); ALLOC(c->tab_d, 1);
     if (hipHostMalloc((void **)&c->ctl_h, sizeof(int) * C_COUNT) != hipSuccess) { mdff_destroy(c); return MDFF_ERR_NOMEM; }
     if (hipHostMalloc((void **)&c->scal_h, sizeof(double) * S_COUNT) != hipSuccess) { mdff_destroy(c); return MDFF_ERR_NOMEM; }
@@ -1764,7 +1835,7 @@ int mdff_get_dipoles_scf(mdff_ctx *c, int istep, double *mu, double *upol)
         if (lo) launch_eikr(c, 1, 1, nullptr);
         if ((rc = build_t2(c)) != MDFF_OK) return rc;
         static_field(c, lo);
-        rc = lo ? scf_loop_mixed(c, istep) : scf_loop(c, istep);
+        rc = lo ? scf_mixed(c, istep) : scf_loop(c, istep);
         if (rc && rc != MDFF_ERR_SCF) return rc;
         c->ldip_sticky = 1;
         hipLaunchKernelGGL(k_axpy3, dim3(cdiv(3 * n, 256)), dim3(256), 0, c->st, 3 * n, c->mu_static, c->mu_ind, c->mu_tot);
@@ -2036,9 +2107,10 @@ int mdff_set_scf_inner(mdff_ctx *c, int inner)
 
 int mdff_set_scf_solver(mdff_ctx *c, int solver, int inner)
 {
-    if (!c || solver < 0 || solver > 1 || inner < 0 || inner > 64) return MDFF_ERR_ARG;
+    if (!c || solver < 0 || solver > 2 || inner < 0 || inner > 64) return MDFF_ERR_ARG;
     c->scf_solver = solver;
     c->inner_iters = inner;
+    c->hint_mixed = c->hint_cg = 0;
     return MDFF_OK;
 }
 

@@ -563,3 +563,90 @@ __global__ void k_cg_apply(int n, int npol, const int *__restrict__ pidx, const 
     const int i = pidx[k];
     for (int a = 0; a < 3; a++) mu[(size_t)a * n + i] += d[(size_t)a * n + i];
 }
+
+// Anchored form of the CG solve: the iterate is the dipoles themselves and
+// the field is carried along, E <- E + gamma T p, from one FP64 evaluation
+// of the predictor (the low-precision T only acts on the increments).  The
+// residual r = E - pol^-T mu and the rmsd of get_rmsd_mu (pim.c:431-449)
+// are formed from the updated mu and E on every iteration.
+__global__ void __launch_bounds__(CG_BLOCK) k_cga_update(int n, int npol, int nb, const int *__restrict__ pidx,
+                                                         const int *__restrict__ type, const TypeTab *__restrict__ T,
+                                                         const double *__restrict__ p, const double *__restrict__ Tp,
+                                                         double *mu, double *ef, double *r, double *z,
+                                                         const double *__restrict__ rz_old,
+                                                         const double *__restrict__ pq_part, double *rz_new,
+                                                         double *rm_part, const int *active)
+{
+    if (!*active) return;
+    __shared__ double sh[CG_BLOCK];
+    const double rz = cg_sum_partials(rz_old, nb, sh);
+    const double pq = cg_sum_partials(pq_part, nb, sh);
+    const double g = (pq != 0.0) ? rz / pq : 0.0;
+    const int k = blockIdx.x * CG_BLOCK + threadIdx.x;
+    double acc = 0.0, rms = 0.0;
+    if (k < npol) {
+        const int i = pidx[k], t = type[i];
+        const double *P = T->pol[t], *I = T->invpol[t];
+        double m[3], e[3], rr[3];
+        for (int a = 0; a < 3; a++) {
+            const size_t o = (size_t)a * n + i;
+            m[a] = mu[o] + g * p[o];
+            e[a] = ef[o] + g * Tp[o];
+            mu[o] = m[a];
+            ef[o] = e[a];
+        }
+        for (int a = 0; a < 3; a++) {
+            rr[a] = e[a] - (I[a] * m[0] + I[3 + a] * m[1] + I[6 + a] * m[2]);
+            r[(size_t)a * n + i] = rr[a];
+            const double paa = P[4 * a];
+            if (paa != 0.0) {
+                const double d = m[a] / paa - e[a];
+                rms += d * d;
+            }
+        }
+        for (int a = 0; a < 3; a++) {
+            double zz = P[0 * 3 + a] * rr[0] + P[1 * 3 + a] * rr[1] + P[2 * 3 + a] * rr[2];
+            z[(size_t)a * n + i] = zz;
+            acc += rr[a] * zz;
+        }
+    }
+    double s = cg_block_sum(acc, sh);
+    double sr = cg_block_sum(rms, sh);
+    if (threadIdx.x == 0) {
+        rz_new[blockIdx.x] = s;
+        rm_part[blockIdx.x] = sr;
+    }
+}
+
+// beta = r.z (new) / r.z (old); p = z + beta p; block 0 records the rmsd
+// and applies the loop condition of momentpolaSCF pim.c:236 (one SCF
+// iteration per CG step)
+__global__ void __launch_bounds__(CG_BLOCK) k_cga_next(int n, int npol, int nb, const int *__restrict__ pidx,
+                                                       const double *__restrict__ z, double *p,
+                                                       const double *__restrict__ rz_old,
+                                                       const double *__restrict__ rz_new,
+                                                       const double *__restrict__ rm_part, int *ctl, double *scal,
+                                                       int maxit, int minit, double tol)
+{
+    if (!ctl[C_SCF_ACTIVE]) return;
+    __shared__ double sh[CG_BLOCK];
+    const double ro = cg_sum_partials(rz_old, nb, sh);
+    const double rn = cg_sum_partials(rz_new, nb, sh);
+    const double rm = cg_sum_partials(rm_part, nb, sh);
+    const double b = (ro != 0.0) ? rn / ro : 0.0;
+    const int k = blockIdx.x * CG_BLOCK + threadIdx.x;
+    if (k < npol) {
+        const int i = pidx[k];
+        for (int a = 0; a < 3; a++) {
+            const size_t o = (size_t)a * n + i;
+            p[o] = z[o] + b * p[o];
+        }
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        scal[S_RMSD2] = rm;
+        int iscf = ctl[C_ISCF] + 1;
+        double rmsd = sqrt(rm / (double)(npol > 0 ? npol : 1));
+        ctl[C_ISCF] = iscf;
+        ctl[C_SCF_ACTIVE] = ((iscf < maxit && rmsd > tol) || iscf < minit) ? 1 : 0;
+    }
+}

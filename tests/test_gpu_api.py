@@ -92,10 +92,11 @@ def test_mixed_form_accuracy(case, katoms):
 
 @pytest.mark.parametrize("case", ["pim192_static", "bsio_200_nvt", "melt216_nve"])
 def test_mixed_scf_solvers_reach_the_fp64_fixed_point(case):
-    """the mixed-form correction solved by preconditioned CG (default) and by
-    Jacobi sweeps: both stop on the same FP64 residual, so their dipoles
-    agree with each other and with the FP64 form to the SCF tolerance, and
-    CG needs no more FP64 field evaluations than Jacobi"""
+    """the three mixed-form SCF solvers: defect correction by Jacobi sweeps
+    and by preconditioned CG (both stop on an FP64 residual) and anchored CG
+    (one FP64 field evaluation, the field carried along the CG steps): all
+    reach the FP64 form's dipoles and forces to the SCF tolerance, CG needs
+    no more FP64 evaluations than Jacobi, and the anchored solve needs one"""
     print(f"[gpu] mixed solvers {case}", flush=True)
     p, cfg, _ = load_case(case)
     p.conv_tol_ind = 1e-10
@@ -103,17 +104,23 @@ def test_mixed_scf_solvers_reach_the_fp64_fixed_point(case):
     p.min_scf_pim_iter = 1
     with m.Engine(p, cfg.typia, cfg.A, cfg.r, precision=m.FP64) as ef, \
             m.Engine(p, cfg.typia, cfg.A, cfg.r, precision=m.MIXED) as ej, \
-            m.Engine(p, cfg.typia, cfg.A, cfg.r, precision=m.MIXED) as ec:
+            m.Engine(p, cfg.typia, cfg.A, cfg.r, precision=m.MIXED) as ec, \
+            m.Engine(p, cfg.typia, cfg.A, cfg.r, precision=m.MIXED) as ea:
         ej.set_scf_solver(0, 6)
         ec.set_scf_solver(1, 6)
-        tf, tj, tc = ef.engforce(0), ej.engforce(0), ec.engforce(0)
-        assert tf["status"] == 0 and tj["status"] == 0 and tc["status"] == 0
-        assert tj["rmsd"] <= 1e-10 and tc["rmsd"] <= 1e-10
+        ea.set_scf_solver(2, 6)
+        tf, tj, tc, ta = ef.engforce(0), ej.engforce(0), ec.engforce(0), ea.engforce(0)
+        assert tf["status"] == 0 and tj["status"] == 0 and tc["status"] == 0 and ta["status"] == 0
+        assert tj["rmsd"] <= 1e-10 and tc["rmsd"] <= 1e-10 and ta["rmsd"] <= 1e-10
         muf = ef.dipoles()
-        assert rel(ec.dipoles(), muf) < 1e-5
+        for e in (ej, ec, ea):
+            assert rel(e.dipoles(), muf) < 1e-5
+            assert rel(e.forces(), ef.forces()) < 1e-5
         assert rel(ec.dipoles(), ej.dipoles()) < 1e-5
-        assert rel(ec.forces(), ef.forces()) < 1e-5
+        assert rel(ea.dipoles(), ec.dipoles()) < 1e-5
         assert ec.counters()["scf_fp64"] <= ej.counters()["scf_fp64"]
+        # the anchored solve evaluates the FP64 field once (when there is an SCF)
+        assert ea.counters()["scf_fp64"] == (1 if ea.counters()["npol"] > 0 else 0)
 
 
 @pytest.mark.parametrize("case", ["pim192_static", "bsio_200_nvt", "melt216_nve"])
