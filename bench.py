@@ -131,6 +131,8 @@ def equilibrate(L, pos, typ, steps):
 def timed_run(p, typ, A, r, precision, steps, warmup):
     import mdff20_amd as m
     e = m.Engine(p, typ, A, r, precision=precision)
+    if precision == m.MIXED:
+        e.set_kspace_modes(m.KS_F16X3, m.KS_BF16X3)
     e.init_velocities()
     e.md_init()
     if warmup:
@@ -180,7 +182,10 @@ def cpu_baseline(p_text, L, pos, typ, n_scf, threads):
 # 1 split-BF16, 2 FP32)
 TIMED = {"sk_forward_scf": (0, "k_gemm_fwd"), "sk_back_scf": (1, "k_gemm_back"),
          "sk_forward_force": (3, "k_gemm_fwd"), "sk_back_force": (4, "k_gemm_back")}
-VARIANT = {"f64": (0, 0), "f32": (1, 2), "bf16x3": (2, 1)}   # name -> (lo, template mode)
+VARIANT = {"f64": (0, 0), "f32": (1, 2), "bf16x3": (2, 1), "f16x3": (3, 3)}   # name -> (lo, template mode)
+# arithmetic of the mixed form's matrix-core sums: static field and force
+# call (eval), low-precision SCF steps (inner)
+MIXED_EVAL, MIXED_INNER = "f16x3", "bf16x3"
 CALIB_BYTES = 512 << 20
 
 
@@ -208,6 +213,7 @@ def pmc_run(args, kname):
     A = np.diag([L, L, L])
     p = params_of(control(True))
     e = m.Engine(p, typ, A, r, precision=m.MIXED)
+    e.set_kspace_modes(m.KS_F16X3, m.KS_BF16X3)
     e.engforce(0)
     # the runtime's own device-to-device copy kernel over CALIB_BYTES
     hip = ctypes.CDLL("libamdhip64.so")
@@ -352,13 +358,13 @@ def main():
     a_, k_, nk = e.ewald()
     npol = e.counters()["npol"]
     st = res[head]
-    variants = [(0, "f64", PEAK["f64"])] + ([(1, "f32", PEAK["f32"]), (2, "bf16x3", PEAK["bf16"])]
-                                           if head == "mixed" else [])
+    variants = [(0, "f64", PEAK["f64"])] + ([(1, "f32", PEAK["f32"]), (2, "bf16x3", PEAK["bf16"]),
+                                            (3, "f16x3", PEAK["bf16"])] if head == "mixed" else [])
     kern = {}
     for lo, vname, peak in variants:
         for which, name, ch in [(0, "sk_forward_scf", 3), (1, "sk_back_scf", 3), (2, "dir_induced", 0),
                                 (3, "sk_forward_force", 4), (4, "sk_back_force", 9)]:
-            if which == 2 and lo == 2:
+            if which == 2 and lo >= 2:
                 continue
             try:
                 ms = e.time_kernel(which, lo=lo, iters=10)
@@ -367,19 +373,19 @@ def main():
                 continue
             nsel = npol if which in (0, 1, 2) else n
             flops = 8.0 * nsel * nk * ch if ch else None
-            # launches per MD step of this kernel in this variant
-            # (SCF: FP64 evaluations, and FP32 real space + split-BF16
-            # k-space correction sweeps in the mixed form; force form: the
-            # final force call)
+            # launches per MD step of this kernel in this variant (SCF:
+            # FP64 evaluations, and in the mixed form FP32 real space and
+            # MIXED_INNER k-space for the low-precision SCF steps; force
+            # form: the final force call, MIXED_EVAL k-space in the mixed form)
             if which in (0, 1, 2):
                 if head == "mixed":
-                    inner_lo = 1 if which == 2 else 2
+                    inner_lo = 1 if which == 2 else VARIANT[MIXED_INNER][0]
                     per = (st["scf_fp64_per_step"] if lo == 0 else
                            st["scf_inner_per_step"] if lo == inner_lo else 0.0)
                 else:
                     per = st["scf_fp64_per_step"]
             else:
-                per = 1.0 if (head == "mixed") == (lo == 1) else 0.0
+                per = (1.0 if lo == VARIANT[MIXED_EVAL][0] else 0.0) if head == "mixed" else 1.0
             kern[f"{name}/{vname}"] = dict(ms=ms, flops=flops, per_step=per, peak=peak,
                                            tflops=(flops / (ms * 1e-3) / 1e12) if flops else None)
     log(f"kernels ({head}): {json.dumps(kern)}")

@@ -130,6 +130,16 @@ void mdff_tt_damping(double b, double c, double r, double *f, double *fd, int or
  * reference. */
 int mdff_set_tuning(mdff_ctx *ctx, int lanes, int katoms, int ksplit);
 
+/* Arithmetic of the mixed form's matrix-core k-space sums (stored-operand
+ * path): MDFF_KS_F32 (FP32 operands), MDFF_KS_BF16X3 (split BF16, three
+ * products, about 16 bits), MDFF_KS_F16X3 (split FP16 with power-of-two
+ * row scaling, three products, about 22 bits).  eval_mode applies to the
+ * static field and the force call, inner_mode to the low-precision SCF
+ * steps; 0 keeps a setting.  Defaults: MDFF_KS_F16X3, MDFF_KS_BF16X3.  No
+ * counterpart in the reference. */
+enum { MDFF_KS_BF16X3 = 1, MDFF_KS_F32 = 2, MDFF_KS_F16X3 = 3 };
+int mdff_set_kspace_modes(mdff_ctx *ctx, int eval_mode, int inner_mode);
+
 /* Shape of the matrix-core k-space sums over stored operands: column tiles
  * of 16 points per wavefront in the forward sum for the FP64 mode (cw64) and
  * the FP32 / split-BF16 modes (cw32), each 1, 2 or 4, and the target block
@@ -149,8 +159,8 @@ int mdff_get_timing(mdff_ctx *ctx, double t[6]);
  * matrix-core S(k) GEMM kernels (which 0, 1, 3, 4 on the matrix-core path)
  * are timed alone, each launch between its own pair of HIP events (operand
  * builds and the partial-sum pass excluded); the others between two events
- * around all launches.  lo = 1 times the FP32 variant, lo = 2 the
- * split-BF16 matrix-core variant (mixed contexts only).  which:
+ * around all launches.  lo = 0 the FP64 variant, 1 FP32, 2 split-BF16, 3
+ * split-FP16 (1-3 in mixed contexts only).  which:
  *   0 S(k) forward, SCF form (dipoles of the polarisable ions, 3 channels)
  *   1 back-projection, SCF form (field at the polarisable ions)
  *   2 real-space induced dipole field

@@ -10,7 +10,7 @@ import os
 
 import numpy as np
 
-from ._types import (ES, ERR_SCF, FP64, MIXED, NVE_LF, NVE_VV, NVT_NHCN, PIM_PCG, PIM_SCF,
+from ._types import (ES, ERR_SCF, FP64, KS_BF16X3, KS_F16X3, KS_F32, MIXED, NVE_LF, NVE_VV, NVT_NHCN, PIM_PCG, PIM_SCF,
                      PIM_SCFKO, STATUS, Config, Params, Thermo)
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
@@ -18,7 +18,7 @@ LIB_PATH = os.path.join(_HERE, "libmdff_gpu.so")
 _lib = None
 
 __all__ = ["lib", "MdffError", "read_control", "read_posff", "Engine", "Params", "Thermo",
-           "FP64", "MIXED", "ES", "NVE_LF", "NVE_VV", "NVT_NHCN", "PIM_SCF", "PIM_SCFKO", "PIM_PCG",
+           "FP64", "MIXED", "KS_BF16X3", "KS_F32", "KS_F16X3", "ES", "NVE_LF", "NVE_VV", "NVT_NHCN", "PIM_SCF", "PIM_SCFKO", "PIM_PCG",
            "device_count", "tt_damping", "init_velocities_host"]
 
 
@@ -78,6 +78,7 @@ def lib():
         "mdff_md_run": (C.c_int, [vp, C.c_int, C.c_int, P(Thermo)]),
         "mdff_set_tuning": (C.c_int, [vp, C.c_int, C.c_int, C.c_int]),
         "mdff_set_kspace_shape": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, C.c_int]),
+        "mdff_set_kspace_modes": (C.c_int, [vp, C.c_int, C.c_int]),
         "mdff_get_timing": (C.c_int, [vp, dp]),
         "mdff_time_kernel": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, dp]),
         "mdff_get_counters": (C.c_int, [vp, P(C.c_longlong)]),
@@ -298,6 +299,11 @@ class Engine:
 
     def set_tuning(self, lanes=0, katoms=0, ksplit=0):
         _check(self._L.mdff_set_tuning(self._h, lanes, katoms, ksplit), "set_tuning", False)
+
+    def set_kspace_modes(self, eval_mode=0, inner_mode=0):
+        """mixed form's matrix-core arithmetic: KS_F32, KS_BF16X3 or KS_F16X3 for the static field and
+        force call (eval_mode) and the low-precision SCF steps (inner_mode); 0 keeps"""
+        _check(self._L.mdff_set_kspace_modes(self._h, eval_mode, inner_mode), "set_kspace_modes", False)
 
     def set_kspace_shape(self, cw64=0, cw32=0, fwd_blocks=0, back_blocks=0):
         _check(self._L.mdff_set_kspace_shape(self._h, cw64, cw32, fwd_blocks, back_blocks), "set_kspace_shape",

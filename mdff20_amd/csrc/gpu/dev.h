@@ -69,6 +69,7 @@ enum {
 enum {
     C_REBUILD = 0, C_NUPD = 1, C_OVERFLOW = 2, C_MAXCNT = 3,
     C_SCF_ACTIVE = 4, C_ISCF = 5, C_NAN = 6, C_FIRST = 7,
+    C_KO_ACTIVE = 8, C_KO_ITER = 9, C_KO_TOTAL = 10,
     C_COUNT = 16
 };
 

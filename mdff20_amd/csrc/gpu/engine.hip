@@ -86,6 +86,8 @@ struct mdff_ctx {
     double *tmp_q = nullptr, *tmp_mu = nullptr, *tmp_qraw = nullptr, *tmp_mur = nullptr;  // drop-in scratch
     double *dc_R = nullptr, *dc_d = nullptr, *dc_g = nullptr;  // mixed SCF correction
     double *cg_z = nullptr, *cg_p = nullptr, *cg_part = nullptr; // its conjugate-gradient form
+    double *ko_f = nullptr, *ko_dmu = nullptr; // scfkO: residual field and inner-loop dipoles
+    double *t2ko = nullptr;   // scfkO: dipole tensors of the polarisable pairs at alpha 0.001
     double *scal = nullptr;
     int *ctl = nullptr;
     TypeTab *tab_d = nullptr;
@@ -125,13 +127,14 @@ struct mdff_ctx {
     std::vector<OccEntry> occ_cache;
     int nselp[2] = {0, 0};    // ions padded to 128 (all, polarisable)
     int Pp = 0, Pk = 0;       // (ny,nz) points padded: forward rows, back-projection K
-    void *yz[3][2][2] = {};   // [mode][polar][hi,lo]: YZ[p][2j+ri]
-    void *yzt[3][2][2] = {};  // [mode][polar][hi,lo]: YZt[j][2p+ri]
-    int yz_ok[3][2] = {};     // stored operands match the current tables
-    void *aw[3][2] = {};      // [mode][hi,lo]: rows (nx,c,re/im) of the forward sum
-    void *wm[3][2] = {};      // [mode][hi,lo]: columns (nx,m,re/im) of the back sum
-    int mf_lo = KM_F32;       // matrix-core mode of the FP32-table evaluations
-    int mf_inner = KM_BF16X3; // mode of the mixed-form SCF correction sweeps
+    void *yz[4][2][2] = {};   // [mode][polar][hi,lo]: YZ[p][2j+ri]
+    void *yzt[4][2][2] = {};  // [mode][polar][hi,lo]: YZt[j][2p+ri]
+    int yz_ok[4][2] = {};     // stored operands match the current tables
+    void *aw[4][2] = {};      // [mode][hi,lo]: rows (nx,c,re/im) of the forward sum
+    void *wm[4][2] = {};      // [mode][hi,lo]: columns (nx,m,re/im) of the back sum
+    double *gsc = nullptr;    // split-FP16 scales: weights [0,8) and inverses [8,16), WM columns from 16
+    int mf_lo = KM_F16X3;     // matrix-core mode of the FP32-table evaluations (static field, forces)
+    int mf_inner = KM_BF16X3; // mode of the mixed-form low-precision SCF steps
     int mf_blocks = 512;      // target blocks of the MFMA kernels
     int nsplit_max = 1;       // (ny,nz) splits the bpart buffer holds
     // host pinned mirrors
@@ -147,7 +150,7 @@ struct mdff_ctx {
     int have_events = 0;
     double t_ms[6] = {0, 0, 0, 0, 0, 0};
     int last_nscf = 0;
-    int hint_fp64 = 0, hint_mixed = 0, hint_cg = 0; // iterations of the last converged SCF of each form
+    int hint_fp64 = 0, hint_mixed = 0, hint_cg = 0, hint_ko = 0; // iterations of the last SCF of each form
     double last_rmsd = 0;
     int scf_chunk = 4;
     int inner_iters = 8;      // mixed form: low-precision correction iterations per FP64 refinement
@@ -327,7 +330,7 @@ static void launch_eikr_t(mdff_ctx *c, const int *idx, int nsel, void *tx, void 
 static void launch_eikr(mdff_ctx *c, int polar, int lo, const int *active)
 {
     // stored operands built from these tables are stale now
-    if (lo) { c->yz_ok[KM_F32][polar] = 0; c->yz_ok[KM_BF16X3][polar] = 0; }
+    if (lo) { c->yz_ok[KM_F32][polar] = 0; c->yz_ok[KM_BF16X3][polar] = 0; c->yz_ok[KM_F16X3][polar] = 0; }
     else c->yz_ok[KM_F64][polar] = 0;
     void **t = c->tab3[lo][polar];
     const int *idx = polar ? c->pidx : nullptr;
@@ -387,8 +390,9 @@ static int launch_fwd_mfma(mdff_ctx *c, int polar, int lo, int C, const double *
     m.part = c->kpart;
     m.active = active;
     int mode = lo ? c->mf_lo : KM_F64;
+    // the operand-generating kernels have no split-FP16 form: FP32 instead
     if (mode == KM_F64) *nchunk_out = fwd_mfma_t<KM_F64>(c, m, nsel);
-    else if (mode == KM_F32) *nchunk_out = fwd_mfma_t<KM_F32>(c, m, nsel);
+    else if (mode == KM_F32 || mode == KM_F16X3) *nchunk_out = fwd_mfma_t<KM_F32>(c, m, nsel);
     else *nchunk_out = fwd_mfma_t<KM_BF16X3>(c, m, nsel);
     return 1;
 }
@@ -422,13 +426,14 @@ static int launch_back_mfma(mdff_ctx *c, int polar, int lo, int NCH, const int *
     b.active = active;
     int mode = lo ? c->mf_lo : KM_F64;
     if (mode == KM_F64) *nsplit_out = back_mfma_t<KM_F64>(c, b, nsel);
-    else if (mode == KM_F32) *nsplit_out = back_mfma_t<KM_F32>(c, b, nsel);
+    else if (mode == KM_F32 || mode == KM_F16X3) *nsplit_out = back_mfma_t<KM_F32>(c, b, nsel);
     else *nsplit_out = back_mfma_t<KM_BF16X3>(c, b, nsel);
     return 1;
 }
 
 // ------------------------------------------------ stored-operand matrix-core sums
 static size_t g_elt(int mode) { return mode == KM_F64 ? 8 : (mode == KM_F32 ? 4 : 2); }
+static bool g_split_mode(int mode) { return mode == KM_BF16X3 || mode == KM_F16X3; }
 
 // allocation made while the context works: zeroed in stream order, since a
 // null-stream memset is not ordered with the non-blocking context stream
@@ -450,7 +455,7 @@ static int ensure_yz(mdff_ctx *c, int mode, int polar, int lo)
     const size_t nb1 = (size_t)c->Pp * 2 * nselp * e, nb2 = (size_t)nselp * 2 * c->Pk * e;
     int rc;
     if ((rc = galloc(c, &c->yz[mode][polar][0], nb1)) || (rc = galloc(c, &c->yzt[mode][polar][0], nb2))) return rc;
-    if (mode == KM_BF16X3 &&
+    if (g_split_mode(mode) &&
         ((rc = galloc(c, &c->yz[mode][polar][1], nb1)) || (rc = galloc(c, &c->yzt[mode][polar][1], nb2))))
         return rc;
     const int NZ = 2 * c->Kz + 1;
@@ -463,31 +468,39 @@ static int ensure_yz(mdff_ctx *c, int mode, int polar, int lo)
                        (const TT *)t[1], (const TT *)t[2], y0, y1, z0, z1)
     if (mode == KM_F64) YZL(KM_F64, double);
     else if (mode == KM_F32) YZL(KM_F32, float);
+    else if (mode == KM_F16X3) YZL(KM_F16X3, float);
     else YZL(KM_BF16X3, float);
 #undef YZL
     c->yz_ok[mode][polar] = 1;
     return MDFF_OK;
 }
 
+// LDS of the instantiated forward / back kernels (MT, CT: the shape's tile
+// counts; the instantiated counts follow from them)
 static size_t g_fwd_lds_bytes_m(int mode, int MT)
 {
-    return mode == KM_F64 ? g_fwd_lds_bytes<KM_F64>(MT)
-                          : (mode == KM_F32 ? g_fwd_lds_bytes<KM_F32>(MT) : g_fwd_lds_bytes<KM_BF16X3>(MT));
+    const int MTI = g_mt_inst(MT);
+    return mode == KM_F64 ? g_fwd_lds_bytes<KM_F64>(MTI)
+                          : (mode == KM_F32 ? g_fwd_lds_bytes<KM_F32>(MTI) : g_fwd_lds_bytes<KM_BF16X3>(MTI));
+    // (the split modes share one LDS size)
 }
 
-static size_t g_back_lds_bytes_m(int mode, int CT)
+static size_t g_back_lds_bytes_m(int mode, int CT, int KX1)
 {
-    return mode == KM_F64 ? g_back_lds_bytes<KM_F64>(CT)
-                          : (mode == KM_F32 ? g_back_lds_bytes<KM_F32>(CT) : g_back_lds_bytes<KM_BF16X3>(CT));
+    const int CTI = g_ct_inst(CT);
+    return mode == KM_F64 ? g_back_lds_bytes<KM_F64>(CTI, KX1)
+                          : (mode == KM_F32 ? g_back_lds_bytes<KM_F32>(CTI, KX1) : g_back_lds_bytes<KM_BF16X3>(CTI, KX1));
 }
 
 template <int MODE, int CW>
 static const void *gfwd_kernel_t(int MT)
 {
-    if (MT <= 3) return (const void *)k_gemm_fwd<MODE, CW, 3>;
-    if (MT <= 5) return (const void *)k_gemm_fwd<MODE, CW, 5>;
-    if (MT <= 7) return (const void *)k_gemm_fwd<MODE, CW, 7>;
-    return (const void *)k_gemm_fwd<MODE, CW, MF_MT>;
+    switch (g_mt_inst(MT)) {
+    case 3: return (const void *)k_gemm_fwd<MODE, CW, 3>;
+    case 5: return (const void *)k_gemm_fwd<MODE, CW, 5>;
+    case 7: return (const void *)k_gemm_fwd<MODE, CW, 7>;
+    default: return (const void *)k_gemm_fwd<MODE, CW, MF_MT>;
+    }
 }
 
 template <int MODE>
@@ -502,16 +515,27 @@ static const void *gfwd_kernel(int mode, int CW, int MT)
 {
     if (mode == KM_F64) return gfwd_kernel_m<KM_F64>(CW, MT);
     if (mode == KM_F32) return gfwd_kernel_m<KM_F32>(CW, MT);
+    if (mode == KM_F16X3) return gfwd_kernel_m<KM_F16X3>(CW, MT);
     return gfwd_kernel_m<KM_BF16X3>(CW, MT);
+}
+
+template <int MODE, typename TT>
+static const void *gback_kernel_t(int CT)
+{
+    switch (g_ct_inst(CT)) {
+    case 4: return (const void *)k_gemm_back<MODE, 4, TT>;
+    case 7: return (const void *)k_gemm_back<MODE, 7, TT>;
+    case 13: return (const void *)k_gemm_back<MODE, 13, TT>;
+    default: return (const void *)k_gemm_back<MODE, MB_CT, TT>;
+    }
 }
 
 static const void *gback_kernel(int mode, int CT)
 {
-    if (mode == KM_F64)
-        return CT <= 7 ? (const void *)k_gemm_back<KM_F64, 7, double> : (const void *)k_gemm_back<KM_F64, MB_CT, double>;
-    if (mode == KM_F32)
-        return CT <= 7 ? (const void *)k_gemm_back<KM_F32, 7, float> : (const void *)k_gemm_back<KM_F32, MB_CT, float>;
-    return CT <= 7 ? (const void *)k_gemm_back<KM_BF16X3, 7, float> : (const void *)k_gemm_back<KM_BF16X3, MB_CT, float>;
+    if (mode == KM_F64) return gback_kernel_t<KM_F64, double>(CT);
+    if (mode == KM_F32) return gback_kernel_t<KM_F32, float>(CT);
+    if (mode == KM_F16X3) return gback_kernel_t<KM_F16X3, float>(CT);
+    return gback_kernel_t<KM_BF16X3, float>(CT);
 }
 
 // number of K splits for gx blocks per split: the smallest that fills at
@@ -543,14 +567,15 @@ static int g_slots(mdff_ctx *c, const void *fn, int threads, size_t lds)
     return slots;
 }
 
-// launch with the smallest instantiated tile count that holds the shape
+// launch the instantiation for the shape's tile count
 template <int MODE, int CW>
 static void gfwd_mt(mdff_ctx *c, dim3 grid, int MT, const GFwdArgs &a)
 {
-    const size_t lds = g_fwd_lds_bytes<MODE>(MT);
-    if (MT <= 3) hipLaunchKernelGGL((k_gemm_fwd<MODE, CW, 3>), grid, dim3(256), lds, c->st, a);
-    else if (MT <= 5) hipLaunchKernelGGL((k_gemm_fwd<MODE, CW, 5>), grid, dim3(256), lds, c->st, a);
-    else if (MT <= 7) hipLaunchKernelGGL((k_gemm_fwd<MODE, CW, 7>), grid, dim3(256), lds, c->st, a);
+    const int MTI = g_mt_inst(MT);
+    const size_t lds = g_fwd_lds_bytes<MODE>(MTI);
+    if (MTI == 3) hipLaunchKernelGGL((k_gemm_fwd<MODE, CW, 3>), grid, dim3(256), lds, c->st, a);
+    else if (MTI == 5) hipLaunchKernelGGL((k_gemm_fwd<MODE, CW, 5>), grid, dim3(256), lds, c->st, a);
+    else if (MTI == 7) hipLaunchKernelGGL((k_gemm_fwd<MODE, CW, 7>), grid, dim3(256), lds, c->st, a);
     else hipLaunchKernelGGL((k_gemm_fwd<MODE, CW, MF_MT>), grid, dim3(256), lds, c->st, a);
 }
 
@@ -565,9 +590,36 @@ static void gfwd_cw(mdff_ctx *c, dim3 grid, int CW, int MT, const GFwdArgs &a)
 template <int MODE, typename TT>
 static void gback_ct(mdff_ctx *c, dim3 grid, int CT, const GBackArgs &b)
 {
-    const size_t lds = g_back_lds_bytes<MODE>(CT);
-    if (CT <= 7) hipLaunchKernelGGL((k_gemm_back<MODE, 7, TT>), grid, dim3(512), lds, c->st, b);
+    const int CTI = g_ct_inst(CT);
+    const size_t lds = g_back_lds_bytes<MODE>(CTI, c->Kx + 1);
+    if (CTI == 4) hipLaunchKernelGGL((k_gemm_back<MODE, 4, TT>), grid, dim3(512), lds, c->st, b);
+    else if (CTI == 7) hipLaunchKernelGGL((k_gemm_back<MODE, 7, TT>), grid, dim3(512), lds, c->st, b);
+    else if (CTI == 13) hipLaunchKernelGGL((k_gemm_back<MODE, 13, TT>), grid, dim3(512), lds, c->st, b);
     else hipLaunchKernelGGL((k_gemm_back<MODE, MB_CT, TT>), grid, dim3(512), lds, c->st, b);
+}
+
+// dynamic LDS limits of every back-projection instantiation (the largest
+// exceed the 64 KB default)
+template <int MODE, typename TT>
+static bool g_back_attrs_t(int KX1)
+{
+    const void *fn[4] = {(const void *)k_gemm_back<MODE, 4, TT>, (const void *)k_gemm_back<MODE, 7, TT>,
+                         (const void *)k_gemm_back<MODE, 13, TT>, (const void *)k_gemm_back<MODE, MB_CT, TT>};
+    const int cti[4] = {4, 7, 13, MB_CT};
+    for (int k = 0; k < 4; k++)
+        if (hipFuncSetAttribute(fn[k], hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)g_back_lds_bytes<MODE>(cti[k], KX1)) != hipSuccess)
+            return false;
+    return true;
+}
+
+static bool g_back_attrs(mdff_ctx *)
+{
+    // the function attribute is process-wide: size it for the largest k box
+    // the stored-operand path takes, (Kx+1) 6 2 <= 16 MB_CT
+    const int KX1 = 16 * MB_CT / 12;
+    return g_back_attrs_t<KM_F64, double>(KX1) && g_back_attrs_t<KM_F32, float>(KX1) &&
+           g_back_attrs_t<KM_BF16X3, float>(KX1) && g_back_attrs_t<KM_F16X3, float>(KX1);
 }
 
 // forward sum over stored operands; 0 when outside the kernel's shapes
@@ -584,19 +636,29 @@ static int launch_fwd_gemm(mdff_ctx *c, int polar, int lo, int C, const double *
     const size_t e = g_elt(mode);
     const size_t awb = (size_t)16 * MF_MT * 2 * std::max(c->nselp[0], c->nselp[1]) * e;
     if (galloc(c, &c->aw[mode][0], awb)) return 0;
-    if (mode == KM_BF16X3 && galloc(c, &c->aw[mode][1], awb)) return 0;
+    if (g_split_mode(mode) && galloc(c, &c->aw[mode][1], awb)) return 0;
     void **t = c->tab3[lo][polar];
     const int *idx = polar ? c->pidx : nullptr;
     dim3 ga(cdiv(nselp, 256), c->Kx + 1);
+    const double *wsc = nullptr, *winv = nullptr;
+    if (mode == KM_F16X3) {
+        // weights scaled into [-1, 1] per channel
+        hipLaunchKernelGGL(k_w_scale, dim3(1), dim3(1024), 0, c->st, nsel, C, idx, q_w, mu_w, c->n, c->gsc, active);
+        wsc = c->gsc;
+        winv = c->gsc + 8;
+    }
     if (mode == KM_F64)
         hipLaunchKernelGGL((k_aw_build<KM_F64, double>), ga, dim3(256), 0, c->st, nsel, nselp, C, (const double *)t[0],
-                           idx, q_w, mu_w, c->n, c->aw[mode][0], c->aw[mode][1], active);
+                           idx, q_w, mu_w, c->n, c->aw[mode][0], c->aw[mode][1], wsc, active);
     else if (mode == KM_F32)
         hipLaunchKernelGGL((k_aw_build<KM_F32, float>), ga, dim3(256), 0, c->st, nsel, nselp, C, (const float *)t[0],
-                           idx, q_w, mu_w, c->n, c->aw[mode][0], c->aw[mode][1], active);
+                           idx, q_w, mu_w, c->n, c->aw[mode][0], c->aw[mode][1], wsc, active);
+    else if (mode == KM_F16X3)
+        hipLaunchKernelGGL((k_aw_build<KM_F16X3, float>), ga, dim3(256), 0, c->st, nsel, nselp, C,
+                           (const float *)t[0], idx, q_w, mu_w, c->n, c->aw[mode][0], c->aw[mode][1], wsc, active);
     else
         hipLaunchKernelGGL((k_aw_build<KM_BF16X3, float>), ga, dim3(256), 0, c->st, nsel, nselp, C,
-                           (const float *)t[0], idx, q_w, mu_w, c->n, c->aw[mode][0], c->aw[mode][1], active);
+                           (const float *)t[0], idx, q_w, mu_w, c->n, c->aw[mode][0], c->aw[mode][1], wsc, active);
     GFwdArgs a;
     a.aw = c->aw[mode][0]; a.aw_lo = c->aw[mode][1];
     a.yz = c->yz[mode][polar][0]; a.yz_lo = c->yz[mode][polar][1];
@@ -613,16 +675,17 @@ static int launch_fwd_gemm(mdff_ctx *c, int polar, int lo, int C, const double *
     probe_mark(c, 0);
     if (mode == KM_F64) gfwd_cw<KM_F64>(c, grid, CW, MT, a);
     else if (mode == KM_F32) gfwd_cw<KM_F32>(c, grid, CW, MT, a);
+    else if (mode == KM_F16X3) gfwd_cw<KM_F16X3>(c, grid, CW, MT, a);
     else gfwd_cw<KM_BF16X3>(c, grid, CW, MT, a);
     probe_mark(c, 1);
     // chunk sums, all threads busy (the per-k reduction then reads one chunk)
-    const size_t nval = c->nkf * 2 * C;
+    const size_t nval = (size_t)M * c->P;
     if (mode == KM_F64)
         hipLaunchKernelGGL(k_part_sum<double>, dim3(cdiv(nval, 256)), dim3(256), 0, c->st, (const double *)c->kpart,
-                           nchunk, nval, c->ksum, active);
+                           nchunk, M, c->P, c->ksum, C, winv, active);
     else
         hipLaunchKernelGGL(k_part_sum<float>, dim3(cdiv(nval, 256)), dim3(256), 0, c->st, (const float *)c->kpart,
-                           nchunk, nval, c->ksum, active);
+                           nchunk, M, c->P, c->ksum, C, winv, active);
     c->fwd_part = 1;
     nchunk = 1;
     *nchunk_out = nchunk;
@@ -643,9 +706,12 @@ static int launch_back_gemm(mdff_ctx *c, int polar, int lo, int NCH, const int *
     const size_t e = g_elt(mode);
     const size_t wmb = (size_t)16 * MB_CT * 2 * c->Pk * e;
     if (galloc(c, &c->wm[mode][0], wmb)) return 0;
-    if (mode == KM_BF16X3 && galloc(c, &c->wm[mode][1], wmb)) return 0;
+    if (g_split_mode(mode) && galloc(c, &c->wm[mode][1], wmb)) return 0;
     dim3 gw(cdiv(c->Pk, 256), (c->Kx + 1) * CH);
-    if (mode == KM_F64)
+    if (mode == KM_F16X3)
+        hipLaunchKernelGGL(k_wm_build_scaled<KM_F16X3>, dim3((c->Kx + 1) * CH), dim3(256), 0, c->st, c->P, c->Pk,
+                           c->Ky, c->Kz, CH, c->W, c->wm[mode][0], c->wm[mode][1], c->gsc + 16, active);
+    else if (mode == KM_F64)
         hipLaunchKernelGGL(k_wm_build<KM_F64>, gw, dim3(256), 0, c->st, c->P, c->Pk, c->Ky, c->Kz, CH, c->W,
                            c->wm[mode][0], c->wm[mode][1], active);
     else if (mode == KM_F32)
@@ -658,10 +724,11 @@ static int launch_back_gemm(mdff_ctx *c, int polar, int lo, int NCH, const int *
     b.yzt = c->yzt[mode][polar][0]; b.yzt_lo = c->yzt[mode][polar][1];
     b.wm = c->wm[mode][0]; b.wm_lo = c->wm[mode][1];
     b.tx = c->tab3[lo][polar][0];
+    b.csc = mode == KM_F16X3 ? c->gsc + 16 : nullptr;
     b.nsel = nsel; b.CH = CH; b.NC = NC; b.CT = CT; b.ldk = 2 * c->Pk;
     b.part = c->bpart; b.active = active;
     const int gx = nselp / 128;
-    int slots = g_slots(c, (const void *)gback_kernel(mode, CT), 512, g_back_lds_bytes_m(mode, CT));
+    int slots = g_slots(c, (const void *)gback_kernel(mode, CT), 512, g_back_lds_bytes_m(mode, CT, c->Kx + 1));
     int ns = c->g_bblocks > 0 ? std::max(1, std::min(c->nsplit_max, c->g_bblocks / gx))
                               : g_split(gx, slots, c->nsplit_max);
     b.kchunk = ((b.ldk + ns - 1) / ns + GK - 1) / GK * GK;
@@ -670,6 +737,7 @@ static int launch_back_gemm(mdff_ctx *c, int polar, int lo, int NCH, const int *
     probe_mark(c, 0);
     if (mode == KM_F64) gback_ct<KM_F64, double>(c, grid, CT, b);
     else if (mode == KM_F32) gback_ct<KM_F32, float>(c, grid, CT, b);
+    else if (mode == KM_F16X3) gback_ct<KM_F16X3, float>(c, grid, CT, b);
     else gback_ct<KM_BF16X3, float>(c, grid, CT, b);
     probe_mark(c, 1);
     *nsplit_out = ns;
@@ -807,24 +875,29 @@ static int aspc_args(mdff_ctx *c, int istep, AspcArgs *asp)
     return MDFF_OK;
 }
 
-// induced field of dipoles mu (no charges, no damping, pim.c:258-260) at
-// the polarisable ions: ef = base + self + dir + rec
+// alphaES of the inner real-space sweeps of scfkO (pim.c:693-694: the value
+// computed from epsw just before is overwritten by this constant)
+#define KO_ALPHA 0.001
+
 // dipole tensors of all polarisable pairs at the current positions (per
-// force call, after k_frac); rows hold the list capacity, so they never
-// overflow
+// force call, after k_frac), and for scfkO the same at KO_ALPHA; rows hold
+// the list capacity, so they never overflow
 static int build_t2(mdff_ctx *c)
 {
     c->t2_ok = 0;
     if (c->npol == 0) return MDFF_OK;
-    if (c->capP != c->cap) {
+    const bool ko = c->p.algo_pim == MDFF_PIM_SCFKO;
+    if (c->capP != c->cap || (ko && !c->t2ko)) {
         dfree(c->idxp);
         dfree(c->t2);
+        dfree(c->t2ko);
         if (c->t2f) { (void)hipFree(c->t2f); c->t2f = nullptr; }
         size_t ne = (size_t)c->npol * c->cap;
         if (hipMalloc((void **)&c->idxp, sizeof(int) * ne) != hipSuccess ||
             hipMalloc((void **)&c->t2, sizeof(double) * 6 * ne) != hipSuccess)
             return MDFF_ERR_NOMEM;
         if (lo_of(c) && hipMalloc((void **)&c->t2f, sizeof(float) * 6 * ne) != hipSuccess) return MDFF_ERR_NOMEM;
+        if (ko && hipMalloc((void **)&c->t2ko, sizeof(double) * 6 * ne) != hipSuccess) return MDFF_ERR_NOMEM;
         c->capP = c->cap;
     }
     T2Args a;
@@ -832,10 +905,20 @@ static int build_t2(mdff_ctx *c)
     a.pidx = c->pidx; a.pinv = c->pinv; a.alpha = c->alpha; a.rc2 = c->lrcutsq; a.npol = c->npol;
     a.cntp = c->cntp; a.idxp = c->idxp; a.t2 = c->t2; a.t2f = lo_of(c) ? c->t2f : nullptr; a.overflow = c->t2_ovf;
     hipLaunchKernelGGL(k_t2_build, dim3(cdiv(c->npol, 4)), dim3(256), 0, c->st, a);
+    if (ko) {
+        // the same pairs at the alpha of induced_moment_inner_kO (pim.c:693-694);
+        // the pair lists come out identical
+        a.alpha = KO_ALPHA;
+        a.t2 = c->t2ko;
+        a.t2f = nullptr;
+        hipLaunchKernelGGL(k_t2_build, dim3(cdiv(c->npol, 4)), dim3(256), 0, c->st, a);
+    }
     c->t2_ok = 1;
     return MDFF_OK;
 }
 
+// induced field of dipoles mu (no charges, no damping, pim.c:258-260) at
+// the polarisable ions: ef = base + self + dir + rec
 static void induced_field(mdff_ctx *c, int lo, const double *mu, const double *base, double *ef, const int *act)
 {
     int n = c->n;
@@ -1084,6 +1167,88 @@ static int scf_mixed(mdff_ctx *c, int istep)
     return scf_loop_mixed(c, istep);
 }
 
+// scfkO, momentpolaSCFkO pim.c:457-603 (FP64 in both forms): the SCF loop
+// of momentpolaSCF whose update is mu += dmu from the inner loop of
+// induced_moment_inner_kO (pim.c:647-748) on the residual field
+// f = E - mu diag(pol^-1), f = E_stat to start; the ASPC predictor only
+// when istep > 1, else the first update starts from zero dipoles.  The
+// inner sweeps run gated on the device (at most 21 per update); the host
+// polls the outer loop condition as scf_loop does.
+static void ko_update(mdff_ctx *c, const int *act)
+{
+    int n = c->n, np = c->npol;
+    const int nb = cdiv(np, CG_BLOCK);
+    double *in_part = c->cg_part, *ext_part = c->cg_part + nb;
+    const int *kact = c->ctl + C_KO_ACTIVE;
+    const double om = c->p.omegakO;
+    const double a2 = KO_ALPHA * KO_ALPHA;
+    const double sself = 2.0 * (2.0 * (KO_ALPHA / sqrt(MDFF_PI)) * a2 / 3.0);   // 2 selfa2 at alpha 0.001
+    hipLaunchKernelGGL(k_ko_start, dim3(nb), dim3(CG_BLOCK), 0, c->st, n, np, c->pidx, c->type, c->tab_d, c->ko_f,
+                       c->ko_dmu, om, ext_part, c->ctl);
+    unsigned blocks = cdiv((size_t)n * 16, 256);
+    for (int k = 0; k < 21; k++) {
+        hipLaunchKernelGGL((k_t2_field<double, 16>), dim3(blocks), dim3(256), 0, c->st, n, c->capP, np, c->pinv,
+                           c->cntp, c->idxp, c->t2ko, c->ko_dmu, c->dc_g, kact);
+        hipLaunchKernelGGL(k_ko_sweep, dim3(nb), dim3(CG_BLOCK), 0, c->st, n, np, c->pidx, c->type, c->tab_d,
+                           c->ko_f, c->dc_g, sself, om, c->ko_dmu, in_part, c->ctl);
+        hipLaunchKernelGGL(k_ko_check, dim3(1), dim3(CG_BLOCK), 0, c->st, nb, np, in_part, ext_part, c->ctl);
+    }
+    hipLaunchKernelGGL(k_cg_apply, dim3(nb), dim3(CG_BLOCK), 0, c->st, n, np, c->pidx, c->ko_dmu, c->mu_ind, act);
+}
+
+static int scf_loop_ko(mdff_ctx *c, int istep)
+{
+    int n = c->n;
+    const int *act = c->ctl + C_SCF_ACTIVE;
+    int maxit = c->p.max_scf_pim_iter, minit = c->p.min_scf_pim_iter;
+    if (!c->t2_ok) return MDFF_ERR_UNSUPPORTED;
+    hipLaunchKernelGGL(k_scf_begin, dim3(1), dim3(1), 0, c->st, c->ctl, (maxit > 0 || minit > 0) ? 1 : 0);
+    set_int(c, C_KO_TOTAL, 0);
+    AspcArgs asp;
+    if (aspc_args(c, istep, &asp)) return MDFF_ERR_CONFIG;
+    const bool predict = istep > 1;
+    // get_dipoles hands over zero dipoles (coulombic.c:90-96); the residual
+    // field of the first update is the static field (pim.c:508-514)
+    if (!predict) hipLaunchKernelGGL(k_zero, dim3(cdiv(3 * n, 256)), dim3(256), 0, c->st, c->mu_ind, (size_t)3 * n);
+    HIPCHK(hipMemcpyAsync(c->ko_f, c->ef_stat, sizeof(double) * 3 * n, hipMemcpyDeviceToDevice, c->st));
+    HIPCHK(hipMemcpyAsync(c->ef, c->ef_stat, sizeof(double) * 3 * n, hipMemcpyDeviceToDevice, c->st));
+    int launched = 0;
+    const int limit = std::max(maxit, minit) + 1;
+    while (launched < limit) {
+        int chunk = launched == 0 ? std::max(std::max(1, std::min(minit, c->scf_chunk)), c->hint_ko) : c->scf_chunk;
+        for (int it = 0; it < chunk && launched < limit; it++, launched++) {
+            const int first = launched == 0;
+            if (first && predict) {
+                hipLaunchKernelGGL(k_scf_mu, dim3(cdiv(n, 256)), dim3(256), 0, c->st, n, 1, asp, c->type, c->tab_d,
+                                   c->hist, c->ef, c->mu_ind, c->upol_i, act);
+            } else {
+                ko_update(c, act);
+                hipLaunchKernelGGL(k_upol, dim3(cdiv(n, 256)), dim3(256), 0, c->st, n, c->type, c->tab_d,
+                                   c->mu_ind, c->upol_i);
+            }
+            induced_field(c, 0, c->mu_ind, c->ef_stat, c->ef, act);
+            if (first)
+                hipLaunchKernelGGL(k_scf_correct, dim3(cdiv(n, 256)), dim3(256), 0, c->st, n, asp, c->type,
+                                   c->tab_d, c->ef, c->mu_ind, act);
+            // rmsd, u_pol of this iteration's dipoles and the loop condition;
+            // then the residual field of the next update
+            hipLaunchKernelGGL(k_scf_check, dim3(1), dim3(1024), 0, c->st, n, c->type, c->tab_d, c->mu_ind, c->ef,
+                               c->upol_i, c->ctl, c->scal, c->npol, maxit, minit, c->p.conv_tol_ind);
+            hipLaunchKernelGGL(k_ko_resid, dim3(cdiv(c->npol, 256)), dim3(256), 0, c->st, n, c->npol, c->pidx,
+                               c->type, c->tab_d, c->ef, c->mu_ind, c->ko_f, act);
+        }
+        int rc = sync_scal(c);
+        if (rc) return rc;
+        if (!c->ctl_h[C_SCF_ACTIVE]) break;
+    }
+    scf_finish(c);
+    c->n_scf_fp64 += c->last_nscf;
+    c->n_scf_inner += c->ctl_h[C_KO_TOTAL];
+    c->hint_ko = c->last_nscf;
+    if (c->last_rmsd > c->p.conv_tol_ind) return MDFF_ERR_SCF;
+    return MDFF_OK;
+}
+
 // final multipole_ES of engforce (field.c:381-383): forces and stress
 static void coulomb_force(mdff_ctx *c, int lo)
 {
@@ -1123,7 +1288,7 @@ static int engforce_impl(mdff_ctx *c, int istep)
             if ((rc = build_t2(c)) != MDFF_OK) return rc;
             static_field(c, lo);
             if (c->have_events) HIPCHK(hipEventRecord(c->ev[2], c->st));
-            rc = lo ? scf_mixed(c, istep) : scf_loop(c, istep);
+            rc = c->p.algo_pim == MDFF_PIM_SCFKO ? scf_loop_ko(c, istep) : (lo ? scf_mixed(c, istep) : scf_loop(c, istep));
             if (rc != MDFF_OK && rc != MDFF_ERR_SCF) return rc;
             c->ldip_sticky = 1;
             hipLaunchKernelGGL(k_axpy3, dim3(cdiv(3 * n, 256)), dim3(256), 0, c->st, 3 * n, c->mu_static, c->mu_ind,
@@ -1333,8 +1498,8 @@ void mdff_destroy(mdff_ctx *c)
     double **dp[] = {&c->r, &c->v, &c->f, &c->fold, &c->rs, &c->s, &c->s0, &c->invmass, &c->massd, &c->q, &c->qz,
                      &c->mu_static, &c->mu_ind, &c->mu_tot, &c->hist, &c->ef_stat, &c->ef, &c->ef_dir, &c->ef_rec,
                      &c->efg, &c->efg_dir, &c->efg_rec, &c->f_sr, &c->f_dir, &c->f_rec, &c->acc_sr, &c->uw_dir,
-                     &c->selfbuf, &c->ke, &c->upol_i, &c->rmsd_i, &c->tmp_q, &c->tmp_mu, &c->tmp_qraw, &c->tmp_mur, &c->dc_R, &c->dc_d, &c->dc_g, &c->cg_z, &c->cg_p, &c->cg_part, &c->scal,
-                     &c->kvec, &c->Ak, &c->kcoe, &c->W, &c->ek, &c->kpart, &c->bpart, &c->ksum};
+                     &c->selfbuf, &c->ke, &c->upol_i, &c->rmsd_i, &c->tmp_q, &c->tmp_mu, &c->tmp_qraw, &c->tmp_mur, &c->dc_R, &c->dc_d, &c->dc_g, &c->cg_z, &c->cg_p, &c->cg_part, &c->ko_f, &c->ko_dmu, &c->t2ko, &c->scal,
+                     &c->kvec, &c->Ak, &c->kcoe, &c->W, &c->ek, &c->kpart, &c->bpart, &c->ksum, &c->gsc};
     for (auto pp : dp) dfree(*pp);
     dfree(c->type);
     dfree(c->ctl);
@@ -1352,7 +1517,7 @@ void mdff_destroy(mdff_ctx *c)
         for (int b = 0; b < 2; b++)
             for (int d = 0; d < 3; d++)
                 if (c->tab3[a][b][d]) (void)hipFree(c->tab3[a][b][d]);
-    for (int m = 0; m < 3; m++)
+    for (int m = 0; m < 4; m++)
         for (int h = 0; h < 2; h++) {
             for (int pl = 0; pl < 2; pl++) {
                 if (c->yz[m][pl][h]) (void)hipFree(c->yz[m][pl][h]);
@@ -1375,7 +1540,7 @@ int mdff_create(mdff_ctx **out, const mdff_params *p, int nion, const int *typia
     if (!out || !p || nion < 1 || !typia || !A9 || !rx || !ry || !rz) return MDFF_ERR_ARG;
     if (p->ntype < 1 || p->ntype > NT) return MDFF_ERR_ARG;
     if (precision != MDFF_FP64 && precision != MDFF_MIXED) return MDFF_ERR_ARG;
-    if (p->lcoulombic && p->algo_pim != MDFF_PIM_SCF) {
+    if (p->lcoulombic && p->algo_pim != MDFF_PIM_SCF && p->algo_pim != MDFF_PIM_SCFKO) {
         int anypol = 0;
         for (int t = 0; t < p->ntype; t++) anypol |= p->lpolar[t];
         if (anypol) return MDFF_ERR_UNSUPPORTED;
@@ -1484,6 +1649,7 @@ int mdff_create(mdff_ctx **out, const mdff_params *p, int nion, const int *typia
     ALLOC(c->tmp_q, n); ALLOC(c->tmp_mu, 3 * n); ALLOC(c->tmp_qraw, n); ALLOC(c->tmp_mur, 3 * n);
     ALLOC(c->dc_R, 3 * n); ALLOC(c->dc_d, 3 * n); ALLOC(c->dc_g, 3 * n);
     ALLOC(c->cg_z, 3 * n); ALLOC(c->cg_p, 3 * n); ALLOC(c->cg_part, 4 * (size_t)cdiv(n, CG_BLOCK));
+    ALLOC(c->ko_f, 3 * n); ALLOC(c->ko_dmu, 3 * n);
     ALLOC(c->scal, S_COUNT); ALLOC(c->ctl, C_COUNT); ALLOC(c->tab_d, 1);
     if (hipHostMalloc((void **)&c->ctl_h, sizeof(int) * C_COUNT) != hipSuccess) { mdff_destroy(c); return MDFF_ERR_NOMEM; }
     if (hipHostMalloc((void **)&c->scal_h, sizeof(double) * S_COUNT) != hipSuccess) { mdff_destroy(c); return MDFF_ERR_NOMEM; }
@@ -1563,6 +1729,7 @@ int mdff_create(mdff_ctx **out, const mdff_params *p, int nion, const int *typia
         c->nk = nk;
         ALLOC(c->kvec, 3 * c->nkf); ALLOC(c->Ak, c->nkf); ALLOC(c->kcoe, c->nkf); ALLOC(c->W, 2 * c->nkf);
         ALLOC(c->ek, 7 * c->nkf);
+        ALLOC(c->gsc, 16 + 6 * (size_t)(c->Kx + 1));
         if (hipMemcpy(c->kvec, kv.data(), sizeof(double) * 3 * c->nkf, hipMemcpyHostToDevice) != hipSuccess ||
             hipMemcpy(c->Ak, ak.data(), sizeof(double) * c->nkf, hipMemcpyHostToDevice) != hipSuccess ||
             hipMemcpy(c->kcoe, kc.data(), sizeof(double) * c->nkf, hipMemcpyHostToDevice) != hipSuccess) {
@@ -1638,15 +1805,7 @@ int mdff_create(mdff_ctx **out, const mdff_params *p, int nion, const int *typia
                             (int)mf_fwd_lds_bytes<KM_F32>()) != hipSuccess ||
         hipFuncSetAttribute((const void *)k_sk_back_mfma<KM_F32>, hipFuncAttributeMaxDynamicSharedMemorySize,
                             (int)mf_back_lds_bytes<KM_F32>()) != hipSuccess ||
-        hipFuncSetAttribute((const void *)k_gemm_back<KM_F64, MB_CT, double>,
-                            hipFuncAttributeMaxDynamicSharedMemorySize,
-                            (int)g_back_lds_bytes<KM_F64>(MB_CT)) != hipSuccess ||
-        hipFuncSetAttribute((const void *)k_gemm_back<KM_F32, MB_CT, float>,
-                            hipFuncAttributeMaxDynamicSharedMemorySize,
-                            (int)g_back_lds_bytes<KM_F32>(MB_CT)) != hipSuccess ||
-        hipFuncSetAttribute((const void *)k_gemm_back<KM_BF16X3, MB_CT, float>,
-                            hipFuncAttributeMaxDynamicSharedMemorySize,
-                            (int)g_back_lds_bytes<KM_BF16X3>(MB_CT)) != hipSuccess) {
+        !g_back_attrs(c)) {
         mdff_destroy(c);
         return MDFF_ERR_DEVICE;
     }
@@ -1672,6 +1831,16 @@ int mdff_set_kspace_shape(mdff_ctx *c, int cw64, int cw32, int fwd_blocks, int b
         if (cw[k]) c->g_cw[k] = cw[k];
     if (fwd_blocks) c->g_fblocks = fwd_blocks;
     if (back_blocks) c->g_bblocks = back_blocks;
+    return MDFF_OK;
+}
+
+int mdff_set_kspace_modes(mdff_ctx *c, int eval_mode, int inner_mode)
+{
+    if (!c) return MDFF_ERR_ARG;
+    auto ok = [](int m) { return m == 0 || m == KM_BF16X3 || m == KM_F32 || m == KM_F16X3; };
+    if (!ok(eval_mode) || !ok(inner_mode)) return MDFF_ERR_ARG;
+    if (eval_mode) c->mf_lo = eval_mode;
+    if (inner_mode) c->mf_inner = inner_mode;
     return MDFF_OK;
 }
 
@@ -1835,7 +2004,7 @@ int mdff_get_dipoles_scf(mdff_ctx *c, int istep, double *mu, double *upol)
         if (lo) launch_eikr(c, 1, 1, nullptr);
         if ((rc = build_t2(c)) != MDFF_OK) return rc;
         static_field(c, lo);
-        rc = lo ? scf_mixed(c, istep) : scf_loop(c, istep);
+        rc = c->p.algo_pim == MDFF_PIM_SCFKO ? scf_loop_ko(c, istep) : (lo ? scf_mixed(c, istep) : scf_loop(c, istep));
         if (rc && rc != MDFF_ERR_SCF) return rc;
         c->ldip_sticky = 1;
         hipLaunchKernelGGL(k_axpy3, dim3(cdiv(3 * n, 256)), dim3(256), 0, c->st, 3 * n, c->mu_static, c->mu_ind, c->mu_tot);
@@ -1991,14 +2160,16 @@ int mdff_md_run(mdff_ctx *c, int first_step, int nsteps, mdff_thermo *th)
 
 int mdff_time_kernel(mdff_ctx *c, int which, int lo, int iters, double *ms)
 {
-    if (!c || !ms || iters < 1 || !c->p.lcoulombic || lo < 0 || lo > 2) return MDFF_ERR_ARG;
+    if (!c || !ms || iters < 1 || !c->p.lcoulombic || lo < 0 || lo > 3) return MDFF_ERR_ARG;
     if (lo && !lo_of(c)) return MDFF_ERR_ARG;
     // lo = 2: FP32 tables with the split-BF16 matrix-core mode
     struct ModeGuard {
         mdff_ctx *c; int keep;
         ~ModeGuard() { c->mf_lo = keep; }
     } guard{c, c->mf_lo};
-    if (lo == 2) { c->mf_lo = KM_BF16X3; lo = 1; }
+    if (lo == 1) c->mf_lo = KM_F32;
+    else if (lo == 2) { c->mf_lo = KM_BF16X3; lo = 1; }
+    else if (lo == 3) { c->mf_lo = KM_F16X3; lo = 1; }
     int n = c->n;
     int polar = (which == 0 || which == 1 || which == 2);
     if (polar && c->npol == 0) return MDFF_ERR_ARG;

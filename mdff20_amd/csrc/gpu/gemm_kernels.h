@@ -14,8 +14,14 @@
 // per lane; the small one is staged through LDS and shared by the block's
 // waves.  For the 16x16x4 FP64/FP32 instructions a lane group g = lane/16
 // takes K = 8g + s at step s of an 8-step block, for both operands (any
-// permutation of K leaves the sum unchanged); the 16x16x32 BF16 instruction
-// takes K = 8g..8g+7 in one step.
+// permutation of K leaves the sum unchanged); the 16x16x32 BF16 and FP16
+// instructions take K = 8g..8g+7 in one step.
+//
+// Split-FP16 (KM_F16X3) keeps 11 + 11 bits of every operand value, but only
+// near 1: values below the normal range keep an absolute resolution of
+// 2^-24.  The ey*ez products are bounded by 1 already; the weight rows of AW
+// (per channel) and the columns of WM (per (nx, m)) are scaled by powers of
+// two into [-1, 1] and the sums scaled back afterwards.
 #pragma once
 #include "mfma_kernels.h"
 
@@ -27,6 +33,22 @@ template <>
 struct GT<KM_F32> { using T = float; };
 template <>
 struct GT<KM_BF16X3> { using T = __bf16; };
+template <>
+struct GT<KM_F16X3> { using T = _Float16; };
+
+// split modes: two arrays (hi, lo) of 16-bit values and three products
+template <int MODE>
+constexpr bool g_is_split() { return MODE == KM_BF16X3 || MODE == KM_F16X3; }
+
+template <int MODE>
+using gvec_t = typename std::conditional<MODE == KM_F16X3, mf_h8, mf_bf8>::type;
+
+template <int MODE>
+__device__ __forceinline__ mf_f4 g_mma_split(gvec_t<MODE> a, gvec_t<MODE> b, mf_f4 c)
+{
+    if constexpr (MODE == KM_F16X3) return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
+    else return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
 
 typedef double g_d8 __attribute__((ext_vector_type(8)));
 typedef float g_f8 __attribute__((ext_vector_type(8)));
@@ -39,6 +61,10 @@ __device__ __forceinline__ void g_store(void *hi, void *lo, size_t i, double x)
         ((double *)hi)[i] = x;
     } else if constexpr (MODE == KM_F32) {
         ((float *)hi)[i] = (float)x;
+    } else if constexpr (MODE == KM_F16X3) {
+        _Float16 h = (_Float16)x;
+        ((_Float16 *)hi)[i] = h;
+        ((_Float16 *)lo)[i] = (_Float16)(x - (double)h);
     } else {
         __bf16 h = (__bf16)(float)x;
         ((__bf16 *)hi)[i] = h;
@@ -93,12 +119,13 @@ __global__ void __launch_bounds__(256) k_yz_build(int nsel, int nselp, int P, in
 }
 
 // AW rows (nx, c, re/im) over K = (j, ri): row (nx,c,0) = (w exr, -w exi),
-// row (nx,c,1) = (w exi, w exr).  grid (nselp/256, Kx+1).  Rows >= M are
-// never written (zeroed at allocation).
+// row (nx,c,1) = (w exi, w exr), w times wsc[c] when wsc is given.
+// grid (nselp/256, Kx+1).  Rows >= M are never written (zeroed at
+// allocation).
 template <int MODE, typename TT>
 __global__ void k_aw_build(int nsel, int nselp, int C, const TT *__restrict__ tx, const int *__restrict__ idx,
                            const double *__restrict__ q, const double *__restrict__ mu, int n, void *aw, void *aw_lo,
-                           const int *active)
+                           const double *__restrict__ wsc, const int *active)
 {
     if (active && !*active) return;
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
@@ -119,6 +146,7 @@ __global__ void k_aw_build(int nsel, int nselp, int C, const TT *__restrict__ tx
             if (C == 1) w = q[ion];
             else if (C == 3) w = mu[(size_t)c * n + ion];
             else w = c == 0 ? q[ion] : mu[(size_t)(c - 1) * n + ion];
+            if (wsc) w *= wsc[c];
         }
         const size_t r0 = (size_t)(nx * C + c) * 2;
         g_store<MODE>(aw, aw_lo, r0 * ld + 2 * (size_t)j, w * xr);
@@ -156,12 +184,93 @@ __global__ void k_wm_build(int P, int Pp, int Ky, int Kz, int CH, const double *
     g_store<MODE>(wm, wm_lo, (c0 + 1) * ld + 2 * (size_t)p + 1, wr);
 }
 
+// power-of-two scale of each weight channel (q, mu components) of the
+// selected ions so that |w| <= 1: sc[c] = 2^-e, sc[8 + c] = 2^e.  One block.
+__global__ void __launch_bounds__(1024) k_w_scale(int nsel, int C, const int *__restrict__ idx,
+                                                  const double *__restrict__ q, const double *__restrict__ mu, int n,
+                                                  double *sc, const int *active)
+{
+    if (active && !*active) return;
+    __shared__ double red[1024];
+    for (int c = 0; c < C; c++) {
+        double mx = 0.0;
+        for (int j = threadIdx.x; j < nsel; j += 1024) {
+            const int ion = idx ? idx[j] : j;
+            double w;
+            if (C == 1) w = q[ion];
+            else if (C == 3) w = mu[(size_t)c * n + ion];
+            else w = c == 0 ? q[ion] : mu[(size_t)(c - 1) * n + ion];
+            mx = fmax(mx, fabs(w));
+        }
+        red[threadIdx.x] = mx;
+        __syncthreads();
+        for (int o = 512; o > 0; o >>= 1) {
+            if ((int)threadIdx.x < o) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + o]);
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) {
+            const double m = red[0];
+            const int e = (m > 0.0 && isfinite(m)) ? ilogb(m) + 1 : 0;
+            sc[c] = ldexp(1.0, -e);
+            sc[8 + c] = ldexp(1.0, e);
+        }
+        __syncthreads();
+    }
+}
+
+// WM columns as k_wm_build, each (nx, m) pair scaled by a power of two
+// into [-1, 1]; cinv[nx CH + m] receives the inverse scale.  One block per
+// (nx, m): grid (Kx+1) CH.
+template <int MODE>
+__global__ void __launch_bounds__(256) k_wm_build_scaled(int P, int Pp, int Ky, int Kz, int CH,
+                                                         const double *__restrict__ W, void *wm, void *wm_lo,
+                                                         double *cinv, const int *active)
+{
+    if (active && !*active) return;
+    __shared__ double red[256];
+    const int nx = blockIdx.x / CH, m = blockIdx.x % CH;
+    const int NZ = 2 * Kz + 1;
+    double mx = 0.0;
+    for (int p = threadIdx.x; p < P; p += 256) {
+        const int iy = p / NZ, iz = p % NZ;
+        const double mom = mb_moment(m, iy - Ky, iz - Kz);
+        const size_t kf = (size_t)nx * P + p;
+        mx = fmax(mx, fmax(fabs(W[2 * kf] * mom), fabs(W[2 * kf + 1] * mom)));
+    }
+    red[threadIdx.x] = mx;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + o]);
+        __syncthreads();
+    }
+    const double amax = red[0];
+    const int e = (amax > 0.0 && isfinite(amax)) ? ilogb(amax) + 1 : 0;
+    const double scl = ldexp(1.0, -e);
+    if (threadIdx.x == 0) cinv[blockIdx.x] = ldexp(1.0, e);
+    const size_t ld = 2 * (size_t)Pp;
+    const size_t c0 = (size_t)(nx * CH + m) * 2;
+    for (int p = threadIdx.x; p < Pp; p += 256) {
+        double wr = 0.0, wi = 0.0;
+        if (p < P) {
+            const int iy = p / NZ, iz = p % NZ;
+            const double mom = mb_moment(m, iy - Ky, iz - Kz) * scl;
+            const size_t kf = (size_t)nx * P + p;
+            wr = W[2 * kf] * mom;
+            wi = W[2 * kf + 1] * mom;
+        }
+        g_store<MODE>(wm, wm_lo, c0 * ld + 2 * (size_t)p, wr);
+        g_store<MODE>(wm, wm_lo, c0 * ld + 2 * (size_t)p + 1, -wi);
+        g_store<MODE>(wm, wm_lo, (c0 + 1) * ld + 2 * (size_t)p, wi);
+        g_store<MODE>(wm, wm_lo, (c0 + 1) * ld + 2 * (size_t)p + 1, wr);
+    }
+}
+
 // 8 consecutive operand values of one lane
 template <int MODE>
 struct GRun {
     using T = typename GT<MODE>::T;
-    typename std::conditional<MODE == KM_F64, g_d8, typename std::conditional<MODE == KM_F32, g_f8, mf_bf8>::type>::type
-        v;
+    typename std::conditional<MODE == KM_F64, g_d8,
+                              typename std::conditional<MODE == KM_F32, g_f8, gvec_t<MODE>>::type>::type v;
 };
 
 template <int MODE>
@@ -170,6 +279,7 @@ __device__ __forceinline__ GRun<MODE> g_load(const void *base, size_t off)
     GRun<MODE> r;
     if constexpr (MODE == KM_F64) r.v = *(const g_d8 *)((const double *)base + off);
     else if constexpr (MODE == KM_F32) r.v = *(const g_f8 *)((const float *)base + off);
+    else if constexpr (MODE == KM_F16X3) r.v = *(const mf_h8 *)((const _Float16 *)base + off);
     else r.v = *(const mf_bf8 *)((const __bf16 *)base + off);
     return r;
 }
@@ -178,12 +288,16 @@ __device__ __forceinline__ GRun<MODE> g_load(const void *base, size_t off)
 #define GLDS_PAD 8     // LDS row padding (elements)
 
 struct GFwdArgs {
-    const void *aw, *aw_lo;   // [16 MT][ldk]
+    const void *aw, *aw_lo;   // [16 MF_MT][ldk], rows >= M zero
     const void *yz, *yz_lo;   // [Pp][ldk]
     int MT, M, P, C, ldk, kchunk;
-    double *part;             // [nchunk][(Kx+1) P][2C]
+    double *part;             // [nchunk][M][P] (FP64, or FP32 in the other modes)
     const int *active;
 };
+
+// row tiles a forward kernel is instantiated for: every launch computes
+// this many (zero rows past M) so that no matrix instruction is predicated
+inline int g_mt_inst(int MT) { return MT <= 3 ? 3 : (MT <= 5 ? 5 : (MT <= 7 ? 7 : 9)); }
 
 template <int MODE>
 inline size_t g_fwd_lds_bytes(int MT)
@@ -198,23 +312,42 @@ inline size_t g_fwd_lds_bytes(int MT)
 // current step computes; consecutive instructions update different
 // accumulators (no back-to-back dependence on one accumulator).
 #define GF_SLOTS 3   // A runs a thread stages per step: ceil(16 MT GK/8 / 256) for MT <= 12
+
+// Block order of the forward sum: consecutive workgroups are dispatched to
+// the 8 XCDs in turn, and blocks of one K range share that range of AW.
+// The dispatch index w is mapped so that each XCD receives a contiguous run
+// of (x fastest) logical blocks, so a K range is fetched into one or two
+// XCD L2s instead of all eight.  A bijection: correctness does not depend
+// on where blocks run.
+__device__ __forceinline__ void g_xcd_order(int gx, int gy, int &bx, int &by)
+{
+    const int w = blockIdx.x + blockIdx.y * gx;
+    const int total = gx * gy;
+    const int q = total >> 3, r = total & 7;
+    const int xcd = w & 7, i = w >> 3;
+    const int L = (xcd < r ? xcd * (q + 1) : r * (q + 1) + (xcd - r) * q) + i;
+    bx = L % gx;
+    by = L / gx;
+}
 template <int MODE, int CW, int MTMAX>
 __global__ void __launch_bounds__(256) k_gemm_fwd(GFwdArgs a)
 {
     if (a.active && !*a.active) return;
     using T = typename GT<MODE>::T;
     constexpr int LD = GK + GLDS_PAD;
-    constexpr bool SPLIT = MODE == KM_BF16X3;
+    constexpr bool SPLIT = g_is_split<MODE>();
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    const int MT = a.MT;
     T *As = (T *)smem_raw;
-    T *Asl = As + MT * 16 * LD;   // split-BF16 low parts (the LDS size holds MT rows of each)
+    T *Asl = As + MTMAX * 16 * LD;   // low parts of the split modes
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int g = lane >> 4, cl = lane & 15;
-    const int ct0 = (blockIdx.x * 4 + wave) * CW;
-    const int k0 = blockIdx.y * a.kchunk;
+    int bx, by;
+    g_xcd_order(gridDim.x, gridDim.y, bx, by);
+    const int ct0 = (bx * 4 + wave) * CW;
+    const int k0 = by * a.kchunk;
     const int k1 = min(k0 + a.kchunk, a.ldk);
-    const int nslot = MT * 16 * (GK / 8);
+    constexpr int nslot = MTMAX * 16 * (GK / 8);
+    static_assert(nslot <= 256 * GF_SLOTS, "A staging slots");
     using acc_t = typename std::conditional<MODE == KM_F64, mf_d4, mf_f4>::type;
     acc_t acc[MTMAX][CW];
 #pragma unroll
@@ -263,34 +396,30 @@ __global__ void __launch_bounds__(256) k_gemm_fwd(GFwdArgs a)
         __syncthreads();
         if (kb + GK < k1) load_step(kb + GK);
         if constexpr (SPLIT) {
-            mf_bf8 ah[MTMAX], al[MTMAX];
+            using V = gvec_t<MODE>;
+            V ah[MTMAX], al[MTMAX];
 #pragma unroll
             for (int t = 0; t < MTMAX; t++) {
-                if (t >= MT) continue;
-                ah[t] = *(const mf_bf8 *)(As + (t * 16 + cl) * LD + 8 * g);
-                al[t] = *(const mf_bf8 *)(Asl + (t * 16 + cl) * LD + 8 * g);
+                ah[t] = *(const V *)(As + (t * 16 + cl) * LD + 8 * g);
+                al[t] = *(const V *)(Asl + (t * 16 + cl) * LD + 8 * g);
             }
 #pragma unroll
             for (int t = 0; t < MTMAX; t++)
 #pragma unroll
-                for (int c = 0; c < CW; c++)
-                    if (t < MT) acc[t][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[t], ch[c].v, acc[t][c], 0, 0, 0);
+                for (int c = 0; c < CW; c++) acc[t][c] = g_mma_split<MODE>(ah[t], ch[c].v, acc[t][c]);
 #pragma unroll
             for (int t = 0; t < MTMAX; t++)
 #pragma unroll
-                for (int c = 0; c < CW; c++)
-                    if (t < MT) acc[t][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[t], cl_[c].v, acc[t][c], 0, 0, 0);
+                for (int c = 0; c < CW; c++) acc[t][c] = g_mma_split<MODE>(ah[t], cl_[c].v, acc[t][c]);
 #pragma unroll
             for (int t = 0; t < MTMAX; t++)
 #pragma unroll
-                for (int c = 0; c < CW; c++)
-                    if (t < MT) acc[t][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[t], ch[c].v, acc[t][c], 0, 0, 0);
+                for (int c = 0; c < CW; c++) acc[t][c] = g_mma_split<MODE>(al[t], ch[c].v, acc[t][c]);
         } else {
 #pragma unroll
             for (int s = 0; s < 8; s++) {
 #pragma unroll
                 for (int t = 0; t < MTMAX; t++) {
-                    if (t >= MT) continue;
                     const T av = As[(t * 16 + cl) * LD + 8 * g + s];
 #pragma unroll
                     for (int c = 0; c < CW; c++) acc[t][c] = mf_mma<MODE>(av, ch[c].v[s], acc[t][c]);
@@ -298,13 +427,12 @@ __global__ void __launch_bounds__(256) k_gemm_fwd(GFwdArgs a)
             }
         }
     }
-    // partial sums of this K range (FP32 accumulators are stored as FP32)
+    // partial sums of this K range, [row][point] (lanes write consecutive
+    // points; FP32 accumulators are stored as FP32)
     using part_t = typename std::conditional<MODE == KM_F64, double, float>::type;
-    const size_t nkf = (size_t)(a.M / (2 * a.C)) * a.P;
-    part_t *out = (part_t *)a.part + (size_t)blockIdx.y * nkf * (2 * a.C);
+    part_t *out = (part_t *)a.part + (size_t)by * a.M * a.P;
 #pragma unroll
     for (int t = 0; t < MTMAX; t++) {
-        if (t >= MT) continue;
 #pragma unroll
         for (int c = 0; c < CW; c++) {
             const int p = (ct0 + c) * 16 + cl;
@@ -312,26 +440,30 @@ __global__ void __launch_bounds__(256) k_gemm_fwd(GFwdArgs a)
 #pragma unroll
             for (int i = 0; i < 4; i++) {
                 const int r = (MODE == KM_F64) ? t * 16 + g + 4 * i : t * 16 + 4 * g + i;
-                if (r >= a.M) continue;
-                const int nx = r / (2 * a.C), cc = (r >> 1) % a.C, ri = r & 1;
-                out[((size_t)nx * a.P + p) * (2 * a.C) + 2 * cc + ri] = (part_t)acc[t][c][i];
+                if (r < a.M) out[(size_t)r * a.P + p] = (part_t)acc[t][c][i];
             }
         }
     }
 }
 
-// sum of the K-range partials of the forward sum, one thread per value,
-// chunks in a fixed order (bitwise reproducible)
+// sum of the K-range partials of the forward sum, one thread per value
+// (row r = (nx, c, re/im), point p), chunks in a fixed order (bitwise
+// reproducible), into the layout [(nx, p)][c][re/im] of k_sk_reduce; values
+// are multiplied by inv[c] when the weights were scaled
 template <typename part_t>
-__global__ void k_part_sum(const part_t *__restrict__ part, int nchunk, size_t nval, double *__restrict__ out,
-                           const int *active)
+__global__ void k_part_sum(const part_t *__restrict__ part, int nchunk, int M, int P, double *__restrict__ out,
+                           int C, const double *__restrict__ inv, const int *active)
 {
     if (active && !*active) return;
     const size_t v = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    const size_t nval = (size_t)M * P;
     if (v >= nval) return;
+    const int r = (int)(v / P), p = (int)(v % P);
     double s = 0.0;
     for (int ch = 0; ch < nchunk; ch++) s += (double)part[(size_t)ch * nval + v];
-    out[v] = s;
+    const int nx = r / (2 * C), cc = (r >> 1) % C, ri = r & 1;
+    if (inv) s *= inv[cc];
+    out[((size_t)nx * P + p) * (2 * C) + 2 * cc + ri] = s;
 }
 
 #define GB_SLOTS 2   // B runs a thread stages per step: ceil(16 CT GK/8 / 512) for CT <= 16
@@ -340,16 +472,24 @@ struct GBackArgs {
     const void *yzt, *yzt_lo;  // [nselp][ldk]
     const void *wm, *wm_lo;    // [16 CT][ldk]
     const void *tx;            // ex tables [Kx+1][nsel][2] (TT)
+    const double *csc;         // inverse column scales [(Kx+1) CH] (scaled WM) or null
     int nsel, CH, NC, CT, ldk, kchunk;
     double *part;              // [nsplit][3 or 9][nsel]
     const int *active;
 };
 
+// column tiles a back-projection kernel is instantiated for (every launch
+// computes this many; columns past NC are zero or ignored)
+inline int g_ct_inst(int CT) { return CT <= 4 ? 4 : (CT <= 7 ? 7 : (CT <= 13 ? 13 : MB_CT)); }
+
+// LDS of the back-projection: CTINST column tiles (x2 for the split modes)
+// and the block's ex table, (Kx+1) x 128 ions x (re, im)
 template <int MODE>
-inline size_t g_back_lds_bytes(int CT)
+inline size_t g_back_lds_bytes(int CTINST, int KX1)
 {
-    size_t e = (size_t)CT * 16 * (GK + GLDS_PAD);
-    return MODE == KM_F64 ? e * 8 : (MODE == KM_F32 ? e * 4 : e * 2 * 2);
+    size_t e = (size_t)CTINST * 16 * (GK + GLDS_PAD);
+    size_t b = MODE == KM_F64 ? e * 8 : (MODE == KM_F32 ? e * 4 : e * 2 * 2);
+    return b + (size_t)KX1 * 128 * 2 * (MODE == KM_F64 ? 8 : 4);
 }
 
 // back-projection: block = 8 waves x 16 ions, all CT column tiles, one K
@@ -363,7 +503,10 @@ __global__ void __launch_bounds__(512) k_gemm_back(GBackArgs a)
     extern __shared__ __align__(16) unsigned char smem_raw[];
     const int CT = a.CT;
     T *Bs = (T *)smem_raw;
-    T *Bsl = Bs + CT * 16 * LD;   // split-BF16 low parts (the LDS size holds CT rows of each)
+    T *Bsl = Bs + CTMAX * 16 * LD;   // low parts of the split modes
+    // the ex values of this block's ions, [nx][128][re, im]
+    TT *exs = (TT *)(smem_raw + (size_t)CTMAX * 16 * LD * sizeof(T) * (g_is_split<MODE>() ? 2 : 1));
+    const int KX1 = a.NC / (2 * a.CH);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int g = lane >> 4, cl = lane & 15;
     const int jrow0 = blockIdx.x * 128 + wave * 16;
@@ -374,8 +517,16 @@ __global__ void __launch_bounds__(512) k_gemm_back(GBackArgs a)
 #pragma unroll
     for (int c = 0; c < CTMAX; c++) acc[c] = acc_t{0, 0, 0, 0};
     const size_t arow = (size_t)(jrow0 + cl) * a.ldk;
-    constexpr bool SPLIT = MODE == KM_BF16X3;
-    const int nslot = CT * 16 * (GK / 8);
+    constexpr bool SPLIT = g_is_split<MODE>();
+    constexpr int nslot = CTMAX * 16 * (GK / 8);
+    static_assert(nslot <= 512 * GB_SLOTS, "B staging slots");
+    {
+        const int jb = blockIdx.x * 128;
+        for (int e = threadIdx.x; e < KX1 * 256; e += 512) {
+            const int nx = e >> 8, q = e & 255, j = jb + (q >> 1);
+            exs[e] = j < a.nsel ? ((const TT *)a.tx)[2 * ((size_t)nx * a.nsel + j) + (q & 1)] : (TT)0;
+        }
+    }
     GRun<MODE> ah, al, rb[GB_SLOTS], rbl[GB_SLOTS];   // this thread's staged B runs
     auto load_step = [&](int kb) {
         ah = g_load<MODE>(a.yzt, arow + kb + 8 * g);
@@ -406,29 +557,24 @@ __global__ void __launch_bounds__(512) k_gemm_back(GBackArgs a)
         __syncthreads();
         if (kb + GK < k1) load_step(kb + GK);
         if constexpr (SPLIT) {
+            using V = gvec_t<MODE>;
 #pragma unroll
-            for (int ct = 0; ct < CTMAX; ct++)
-                if (ct < CT)
-                    acc[ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
-                        ch.v, *(const mf_bf8 *)(Bs + (ct * 16 + cl) * LD + 8 * g), acc[ct], 0, 0, 0);
-#pragma unroll
-            for (int ct = 0; ct < CTMAX; ct++)
-                if (ct < CT)
-                    acc[ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
-                        ch.v, *(const mf_bf8 *)(Bsl + (ct * 16 + cl) * LD + 8 * g), acc[ct], 0, 0, 0);
-#pragma unroll
-            for (int ct = 0; ct < CTMAX; ct++)
-                if (ct < CT)
-                    acc[ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
-                        cl_.v, *(const mf_bf8 *)(Bs + (ct * 16 + cl) * LD + 8 * g), acc[ct], 0, 0, 0);
+            for (int ct = 0; ct < CTMAX; ct++) {
+                const V bh = *(const V *)(Bs + (ct * 16 + cl) * LD + 8 * g);
+                const V bl = *(const V *)(Bsl + (ct * 16 + cl) * LD + 8 * g);
+                acc[ct] = g_mma_split<MODE>(ch.v, bh, acc[ct]);
+                acc[ct] = g_mma_split<MODE>(ch.v, bl, acc[ct]);
+                acc[ct] = g_mma_split<MODE>(cl_.v, bh, acc[ct]);
+            }
         } else {
 #pragma unroll
             for (int s = 0; s < 8; s++)
 #pragma unroll
                 for (int ct = 0; ct < CTMAX; ct++)
-                    if (ct < CT) acc[ct] = mf_mma<MODE>(ch.v[s], Bs[(ct * 16 + cl) * LD + 8 * g + s], acc[ct]);
+                    acc[ct] = mf_mma<MODE>(ch.v[s], Bs[(ct * 16 + cl) * LD + 8 * g + s], acc[ct]);
         }
     }
+    __syncthreads();   // the ex table is complete (also for an empty K range)
     // epilogue: Z_m = ex[nx] T_m on the even (real-part) columns, moments,
     // then the sum over the 8 even lanes that share this lane's ions
     double H[4][3], Mm[4][6];
@@ -445,15 +591,16 @@ __global__ void __launch_bounds__(512) k_gemm_back(GBackArgs a)
         const int nx = col / (2 * a.CH), m = (col >> 1) % a.CH;
         const bool even = (cl & 1) == 0 && col < a.NC;
         const double fnx = (double)nx;
+        const double csc = (a.csc && col < a.NC) ? a.csc[nx * a.CH + m] : 1.0;
 #pragma unroll
         for (int i = 0; i < 4; i++) {
-            double tre = (double)acc[ct][i];
+            double tre = (double)acc[ct][i] * csc;
             double tim = __shfl_xor(tre, 1, 64);
             const int r = (MODE == KM_F64) ? g + 4 * i : 4 * g + i;
             const int j = jrow0 + r;
             if (!even || j >= a.nsel) continue;
-            size_t ox = 2 * ((size_t)nx * a.nsel + j);
-            double xr = (double)((const TT *)a.tx)[ox], xi = (double)((const TT *)a.tx)[ox + 1];
+            const int ox = 2 * (nx * 128 + (j - blockIdx.x * 128));
+            double xr = (double)exs[ox], xi = (double)exs[ox + 1];
             double zr = xr * tre - xi * tim, zi = xr * tim + xi * tre;
             switch (m) {
             case 0: H[i][0] += fnx * zi; Mm[i][0] += fnx * fnx * zr; break;

@@ -650,3 +650,114 @@ __global__ void __launch_bounds__(CG_BLOCK) k_cga_next(int n, int npol, int nb, 
         ctl[C_SCF_ACTIVE] = ((iscf < maxit && rmsd > tol) || iscf < minit) ? 1 : 0;
     }
 }
+
+// ---------------------------------------------------------------------------
+// scfkO (momentpolaSCFkO pim.c:457-603, induced_moment_inner_kO :647-748),
+// per polarisable row.  The inner loop relaxes dmu towards pol^T (f + E(dmu))
+// with the real-space dipole field of alpha 0.001 (self term included),
+// omega-weighted, while its rmsd exceeds a tenth of the rmsd of f, at most
+// 21 sweeps; ctl[C_KO_ACTIVE] gates the sweeps on the device.
+
+// dmu = omega pol^T f; partial sums of f_a^2 (rmsd of zero dipoles in f);
+// block 0 opens the inner loop (closed when the SCF is no longer active)
+__global__ void __launch_bounds__(CG_BLOCK) k_ko_start(int n, int npol, const int *__restrict__ pidx,
+                                                       const int *__restrict__ type, const TypeTab *__restrict__ T,
+                                                       const double *__restrict__ f, double *dmu, double omega,
+                                                       double *ext_part, int *ctl)
+{
+    const bool on = ctl[C_SCF_ACTIVE] != 0;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        ctl[C_KO_ACTIVE] = on ? 1 : 0;
+        ctl[C_KO_ITER] = 0;
+    }
+    if (!on) return;
+    __shared__ double sh[CG_BLOCK];
+    const int k = blockIdx.x * CG_BLOCK + threadIdx.x;
+    double acc = 0.0;
+    if (k < npol) {
+        const int i = pidx[k], t = type[i];
+        const double *P = T->pol[t];
+        const double f0 = f[i], f1 = f[n + i], f2 = f[2 * n + i];
+        const double fv[3] = {f0, f1, f2};
+        for (int a = 0; a < 3; a++)
+            if (P[4 * a] != 0.0) acc += fv[a] * fv[a];
+        for (int a = 0; a < 3; a++) {
+            const double m = P[0 * 3 + a] * f0 + P[1 * 3 + a] * f1 + P[2 * 3 + a] * f2;
+            dmu[(size_t)a * n + i] = m * omega;
+        }
+    }
+    const double s = cg_block_sum(acc, sh);
+    if (threadIdx.x == 0) ext_part[blockIdx.x] = s;
+}
+
+// one sweep: E = f + (sself dmu + E_dir); rmsd partials of (dmu, E);
+// dmu = (1 - omega) dmu + omega pol^T E
+__global__ void __launch_bounds__(CG_BLOCK) k_ko_sweep(int n, int npol, const int *__restrict__ pidx,
+                                                       const int *__restrict__ type, const TypeTab *__restrict__ T,
+                                                       const double *__restrict__ f, const double *__restrict__ efdir,
+                                                       double sself, double omega, double *dmu, double *in_part,
+                                                       const int *ctl)
+{
+    if (!ctl[C_KO_ACTIVE]) return;
+    __shared__ double sh[CG_BLOCK];
+    const int k = blockIdx.x * CG_BLOCK + threadIdx.x;
+    double acc = 0.0;
+    if (k < npol) {
+        const int i = pidx[k], t = type[i];
+        const double *P = T->pol[t];
+        double m[3], e[3];
+        for (int a = 0; a < 3; a++) {
+            const size_t o = (size_t)a * n + i;
+            m[a] = dmu[o];
+            e[a] = f[o] + (sself * m[a] + efdir[o]);
+        }
+        for (int a = 0; a < 3; a++) {
+            const double paa = P[4 * a];
+            if (paa != 0.0) {
+                const double d = m[a] / paa - e[a];
+                acc += d * d;
+            }
+        }
+        for (int a = 0; a < 3; a++) {
+            const double nm = P[0 * 3 + a] * e[0] + P[1 * 3 + a] * e[1] + P[2 * 3 + a] * e[2];
+            dmu[(size_t)a * n + i] = (1.0 - omega) * m[a] + omega * nm;
+        }
+    }
+    const double s = cg_block_sum(acc, sh);
+    if (threadIdx.x == 0) in_part[blockIdx.x] = s;
+}
+
+// loop condition of the inner loop (pim.c:698): another sweep while
+// sweeps <= 20 and rmsd > 0.1 rmsd(f); one block
+__global__ void __launch_bounds__(CG_BLOCK) k_ko_check(int nb, int npol, const double *__restrict__ in_part,
+                                                       const double *__restrict__ ext_part, int *ctl)
+{
+    if (!ctl[C_KO_ACTIVE]) return;
+    __shared__ double sh[CG_BLOCK];
+    const double rin = cg_sum_partials(in_part, nb, sh);
+    const double rext = cg_sum_partials(ext_part, nb, sh);
+    if (threadIdx.x == 0) {
+        const int it = ctl[C_KO_ITER] + 1;
+        ctl[C_KO_ITER] = it;
+        ctl[C_KO_TOTAL] += 1;
+        const double np = (double)(npol > 0 ? npol : 1);
+        const double ri = sqrt(rin / np), re = sqrt(rext / np);
+        ctl[C_KO_ACTIVE] = (it <= 20 && ri > 0.1 * re) ? 1 : 0;
+    }
+}
+
+// residual field of the next update (pim.c:579-586): f = E - mu diag(pol^-1)
+__global__ void k_ko_resid(int n, int npol, const int *__restrict__ pidx, const int *__restrict__ type,
+                           const TypeTab *__restrict__ T, const double *__restrict__ ef,
+                           const double *__restrict__ mu, double *f, const int *active)
+{
+    if (!*active) return;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= npol) return;
+    const int i = pidx[k], t = type[i];
+    const double *I = T->invpol[t];
+    for (int a = 0; a < 3; a++) {
+        const size_t o = (size_t)a * n + i;
+        f[o] = ef[o] - mu[o] * I[4 * a];
+    }
+}

@@ -13,6 +13,9 @@
 //   KM_BF16X3 v_mfma_f32_16x16x32_bf16 on split operands x = hi + lo
 //             (hi = bf16(x), lo = bf16(x - hi)); A.B ~ Ah.Bh + Ah.Bl + Al.Bh,
 //             FP32 accumulation, about 16 significant bits per product.
+//   KM_F16X3  v_mfma_f32_16x16x32_f16 on split operands (hi = f16(x),
+//             lo = f16(x - hi)) scaled into [-1, 1], same three products:
+//             about 22 bits (stored-operand GEMMs of gemm_kernels.h only).
 // Result lane layouts (checked on gfx950): f32/bf16 16x16 D[4(l/16)+i][l%16],
 // f64 16x16x4 D[(l/16)+4i][l%16]; operands A[l%16][k], B[k][l%16] with
 // k = l/16 (x4) or 8(l/16)..+7 (x32).
@@ -21,11 +24,12 @@
 
 #include "dev.h"
 
-enum { KM_F64 = 0, KM_BF16X3 = 1, KM_F32 = 2 };
+enum { KM_F64 = 0, KM_BF16X3 = 1, KM_F32 = 2, KM_F16X3 = 3 };
 
 typedef double mf_d4 __attribute__((ext_vector_type(4)));
 typedef float mf_f4 __attribute__((ext_vector_type(4)));
 typedef __bf16 mf_bf8 __attribute__((ext_vector_type(8)));
+typedef _Float16 mf_h8 __attribute__((ext_vector_type(8)));
 
 #define MF_MT 9       // max row tiles: (Kx+1) * C * 2 <= 144
 #define MF_LDA_PAD 1  // FP64 LDS row padding (in elements)

@@ -78,6 +78,7 @@ struct orc_sys {
     double E_TT[MDFF_MAX_TT_ORDER + 1];
     mdff_thermo th;
     int istep;
+    int ko_inner;   /* scfkO: inner sweeps of the last SCF */
 };
 
 /* ---------------------------------------------------------------- units */
@@ -967,16 +968,112 @@ static int moment_pola_scf(orc_sys *s, double (*mu_ind)[3], double *upol, int is
     return (rmsd > s->p.conv_tol_ind) ? MDFF_ERR_SCF : MDFF_OK;
 }
 
-/* get_dipoles coulombic.c:67-130 (algo scf) */
+/* induced_moment_inner_kO pim.c:647-748: dmu from the external field
+ * f_ext by omega-relaxed Jacobi sweeps on the field of dmu alone, real space
+ * only with alphaES set to 0.001 for the sweeps (the self term of
+ * multipole_ES ewald.c:193-226 at that alpha included); at most 21 sweeps,
+ * until the rmsd of a sweep is below a tenth of the rmsd of f_ext */
+static void induced_moment_inner_ko(orc_sys *s, double (*dmu)[3], const double *f_ext)
+{
+    int n = s->nion;
+    double *ef_tot = (double *)malloc(sizeof(double) * 3 * n);
+    double *ef_ind = (double *)calloc((size_t)3 * n, sizeof(double));
+    double *efg = (double *)calloc((size_t)9 * n, sizeof(double));
+    double *zero = (double *)calloc((size_t)3 * n, sizeof(double));
+    double *zq = (double *)calloc((size_t)n, sizeof(double));
+    double (*prev)[3] = (double(*)[3])malloc(sizeof(double) * 3 * n);
+    double u, pd, td[9] = {0};
+    memcpy(ef_tot, f_ext, sizeof(double) * 3 * n);
+    double rmsd_ext = get_rmsd(s, (const double(*)[3])zero, ef_tot);
+    induced_moment(s, dmu, ef_tot);
+    const double w = s->p.omegakO;
+    for (int ia = 0; ia < n; ia++)
+        for (int i = 0; i < 3; i++) dmu[ia][i] *= w;
+    const double alpha_save = s->alpha;
+    s->alpha = 0.001;   /* pim.c:693-694 (the computed value is overwritten) */
+    int it = 0;
+    double rmsd_inner = DBL_MAX;
+    while (it <= 20 && rmsd_inner > 0.1 * rmsd_ext) {
+        it++;
+        memcpy(prev, dmu, sizeof(double) * 3 * n);
+        orc_multipole_es(s, zq, (const double *)dmu, ORC_LD | ORC_EF | ORC_DIR, &u, &pd, td, ef_ind, efg, zero, zero,
+                         zero);
+        for (int i = 0; i < 3 * n; i++) ef_tot[i] = f_ext[i] + ef_ind[i];
+        rmsd_inner = get_rmsd(s, (const double(*)[3])dmu, ef_tot);
+        s->ko_inner++;
+        induced_moment(s, dmu, ef_tot);
+        for (int ia = 0; ia < n; ia++)
+            for (int i = 0; i < 3; i++) dmu[ia][i] = (1.0 - w) * prev[ia][i] + w * dmu[ia][i];
+    }
+    s->alpha = alpha_save;
+    free(ef_tot); free(ef_ind); free(efg); free(zero); free(zq); free(prev);
+}
+
+/* momentpolaSCFkO pim.c:457-603: the SCF loop of momentpolaSCF with the
+ * update mu += dmu from induced_moment_inner_kO on the residual field
+ * f = E - mu diag(pol^-1); the ASPC predictor only when istep > 1 */
+static int moment_pola_scf_ko(orc_sys *s, double (*mu_ind)[3], double *upol, int istep)
+{
+    int n = s->nion;
+    double *ef = (double *)calloc((size_t)3 * n, sizeof(double));
+    double *ef_stat = (double *)calloc((size_t)3 * n, sizeof(double));
+    double *ef_ind = (double *)calloc((size_t)3 * n, sizeof(double));
+    double *f_ind = (double *)calloc((size_t)3 * n, sizeof(double));
+    double *efg = (double *)calloc((size_t)9 * n, sizeof(double));
+    double *zero = (double *)calloc((size_t)3 * n, sizeof(double));
+    double *zq = (double *)calloc((size_t)n, sizeof(double));
+    double (*dmu)[3] = (double(*)[3])calloc((size_t)3 * n, sizeof(double));
+    double u_stat, u_ind, pd, td[9] = {0};
+    int fl = ORC_EF | ORC_DIR | ORC_REC;
+    if (s->lqch) fl |= ORC_LQ;
+    if (s->ldip) fl |= ORC_LD;
+    if (s->ldmp) fl |= ORC_LDAMP;
+    orc_multipole_es(s, s->q, (const double *)s->dip, fl, &u_stat, &pd, td, ef_stat, efg, zero, zero, zero);
+    memcpy(ef, ef_stat, sizeof(double) * 3 * n);
+    memcpy(f_ind, ef_stat, sizeof(double) * 3 * n);
+    s->ko_inner = 0;
+    int iscf = 0;
+    double rmsd = DBL_MAX, uupol = 0.0;
+    while ((iscf < s->p.max_scf_pim_iter && rmsd > s->p.conv_tol_ind) || iscf < s->p.min_scf_pim_iter) {
+        if (!iscf && istep > 1) {
+            aspc_predict(s, mu_ind, istep);
+        } else {
+            induced_moment_inner_ko(s, dmu, f_ind);
+            for (int ia = 0; ia < n; ia++)
+                for (int i = 0; i < 3; i++) mu_ind[ia][i] += dmu[ia][i];
+        }
+        uupol = get_upol(s, (const double(*)[3])mu_ind);
+        orc_multipole_es(s, zq, (const double *)mu_ind, ORC_LD | ORC_EF | ORC_DIR | ORC_REC, &u_ind, &pd, td,
+                         ef_ind, efg, zero, zero, zero);
+        for (int i = 0; i < 3 * n; i++) ef[i] = ef_stat[i] + ef_ind[i];
+        if (!iscf) aspc_correct(s, mu_ind, ef, istep);
+        rmsd = get_rmsd(s, (const double(*)[3])mu_ind, ef);
+        for (int ia = 0; ia < n; ia++) {
+            if (!s->p.lpolar[s->typ[ia]]) continue;
+            for (int i = 0; i < 3; i++) f_ind[3 * ia + i] = ef[3 * ia + i] - mu_ind[ia][i] * s->invpol[ia][i][i];
+        }
+        iscf++;
+    }
+    for (int ia = 0; ia < n; ia++)
+        for (int i = 0; i < 3; i++) s->hist[ia][0][i] = mu_ind[ia][i];
+    *upol = uupol * s->u.coul;
+    s->th.n_scf = iscf;
+    s->th.rmsd = rmsd;
+    free(ef); free(ef_stat); free(ef_ind); free(f_ind); free(efg); free(zero); free(zq); free(dmu);
+    return (rmsd > s->p.conv_tol_ind) ? MDFF_ERR_SCF : MDFF_OK;
+}
+
+/* get_dipoles coulombic.c:67-130 (algo scf or scfkO) */
 static int get_dipoles(orc_sys *s, double (*mu)[3], double *upol, int istep)
 {
     int rc = MDFF_OK;
     for (int ia = 0; ia < s->nion; ia++)
         for (int i = 0; i < 3; i++) mu[ia][i] = s->dip[ia][i];
     if (s->lpol) {
-        if (s->p.algo_pim != MDFF_PIM_SCF) return MDFF_ERR_UNSUPPORTED;
+        if (s->p.algo_pim != MDFF_PIM_SCF && s->p.algo_pim != MDFF_PIM_SCFKO) return MDFF_ERR_UNSUPPORTED;
         double (*mi)[3] = (double(*)[3])calloc((size_t)3 * s->nion, sizeof(double));
-        rc = moment_pola_scf(s, mi, upol, istep);
+        rc = s->p.algo_pim == MDFF_PIM_SCFKO ? moment_pola_scf_ko(s, mi, upol, istep)
+                                             : moment_pola_scf(s, mi, upol, istep);
         for (int ia = 0; ia < s->nion; ia++)
             for (int i = 0; i < 3; i++) mu[ia][i] += mi[ia][i];
         s->ldip = 1; /* coulombic.c:115 */
@@ -1448,6 +1545,8 @@ void orc_get_thermo(const orc_sys *s, mdff_thermo *th)
     finish_thermo(m);
     *th = m->th;
 }
+
+int orc_ko_inner(const orc_sys *s) { return s->ko_inner; }
 
 void orc_get_ewald(const orc_sys *s, double *alpha, int *kES, int *nk)
 {

@@ -39,6 +39,8 @@ void orc_get_dipoles(const orc_sys *s, double *mu);
 void orc_get_thermo(const orc_sys *s, mdff_thermo *th);
 /* derived Ewald parameters (after lautoES) and k count */
 void orc_get_ewald(const orc_sys *s, double *alpha, int *kES, int *nk);
+/* scfkO: inner real-space sweeps of the last SCF (induced_moment_inner_kO) */
+int orc_ko_inner(const orc_sys *s);
 
 /* initial velocities of an MD start (kinetic.c:15-28) from rand() after
  * srand(1) */

@@ -36,6 +36,8 @@ def load():
         L.orc_get_dipoles.argtypes = [vp, dp]
         L.orc_get_thermo.argtypes = [vp, C.POINTER(Thermo)]
         L.orc_get_ewald.argtypes = [vp, dp, ip, ip]
+        L.orc_ko_inner.argtypes = [vp]
+        L.orc_ko_inner.restype = C.c_int
         L.orc_init_velocities.argtypes = [vp]
         L.orc_engforce.argtypes = [vp, C.c_int]
         L.orc_engforce.restype = C.c_int
@@ -99,6 +101,10 @@ class Oracle:
         t = Thermo()
         self._L.orc_get_thermo(self._h, C.byref(t))
         return t.as_dict()
+
+    def ko_inner(self):
+        """scfkO: inner real-space sweeps of the last SCF"""
+        return self._L.orc_ko_inner(self._h)
 
     def ewald(self):
         a, k, nk = C.c_double(), (C.c_int * 3)(), C.c_int()

@@ -225,6 +225,17 @@ def main():
     if want("bsio_200_nvt"):
         base = open(os.path.join(src, "control_BSiO.s")).read().replace("algo_pim scfkO", "algo_pim scf")
         run_case("bsio_200_nvt", base, os.path.join(src, "POSFF.BSiO"), mode="md", args=(6, 3))
+    # ---- the same two inputs with their own SCF algorithm, scfkO (pim.c:457-748)
+    if want("sio2_102_ko_static") or want("sio2_102_ko_nvt"):
+        base = open(os.path.join(src, "control_SiO.s")).read()
+        if want("sio2_102_ko_static"):
+            run_case("sio2_102_ko_static", base.replace("lstatic false", "lstatic true"),
+                     os.path.join(src, "POSFF.SiO2"))
+        if want("sio2_102_ko_nvt"):
+            run_case("sio2_102_ko_nvt", base, os.path.join(src, "POSFF.SiO2"), mode="md", args=(12, 6))
+    if want("bsio_200_ko_nvt"):
+        base = open(os.path.join(src, "control_BSiO.s")).read()
+        run_case("bsio_200_ko_nvt", base, os.path.join(src, "POSFF.BSiO"), mode="md", args=(6, 3))
     # ---- Lennard-Jones FCC of examples/ex0 (static) and ex1 (NVE with rescale)
     if want("lj_fcc_static"):
         run_case("lj_fcc_static", open(os.path.join(REF, "examples/ex0/config/control.s")).read(),

@@ -90,6 +90,34 @@ def test_mixed_form_accuracy(case, katoms):
         assert tm["rmsd"] <= 1e-9
 
 
+@pytest.mark.parametrize("case", ["pim192_static", "sio2_102_static", "bsio_200_nvt", "melt216_nve"])
+@pytest.mark.parametrize("modes", [(m.KS_F16X3, m.KS_F16X3), (m.KS_F16X3, m.KS_BF16X3)])
+def test_mixed_form_split_fp16(case, modes):
+    """split-FP16 matrix-core sums (power-of-two scaled operands, three
+    products) for the static field and force call, and for the SCF steps:
+    the same bounds against the FP64 form as the FP32 form, and within
+    a few 1e-6 of the FP32 matrix-core form"""
+    print(f"[gpu] split fp16 {case} {modes}", flush=True)
+    p, cfg, _ = load_case(case)
+    p.conv_tol_ind = 1e-9
+    p.max_scf_pim_iter = 400
+    p.min_scf_pim_iter = 1
+    with m.Engine(p, cfg.typia, cfg.A, cfg.r, precision=m.FP64) as ef, \
+            m.Engine(p, cfg.typia, cfg.A, cfg.r, precision=m.MIXED) as e32, \
+            m.Engine(p, cfg.typia, cfg.A, cfg.r, precision=m.MIXED) as e16:
+        e32.set_kspace_modes(m.KS_F32, m.KS_F32)
+        e16.set_kspace_modes(*modes)
+        tf, t32, t16 = ef.engforce(0), e32.engforce(0), e16.engforce(0)
+        assert tf["status"] == 0 and t32["status"] == 0 and t16["status"] == 0
+        Ff, F16 = ef.forces(), e16.forces()
+        frms = np.sqrt(np.mean((F16 - Ff) ** 2)) / np.sqrt(np.mean(Ff ** 2))
+        assert frms < 1e-5, frms
+        assert rel(e16.dipoles(), ef.dipoles()) < 1e-5
+        assert t16["u_tot"] == pytest.approx(tf["u_tot"], rel=1e-6)
+        assert rel(F16, e32.forces()) < 5e-6
+        assert t16["rmsd"] <= 1e-9
+
+
 @pytest.mark.parametrize("case", ["pim192_static", "bsio_200_nvt", "melt216_nve"])
 def test_mixed_scf_solvers_reach_the_fp64_fixed_point(case):
     """the three mixed-form SCF solvers: defect correction by Jacobi sweeps
