@@ -669,7 +669,9 @@ static int launch_fwd_gemm(mdff_ctx *c, int polar, int lo, int C, const double *
     // split K so that the grid fills whole rounds of the resident blocks
     int slots = g_slots(c, (const void *)gfwd_kernel(mode, CW, MT), 256, g_fwd_lds_bytes_m(mode, MT));
     int nchunk = c->g_fblocks > 0 ? std::max(1, std::min(64, c->g_fblocks / gx)) : g_split(gx, slots, 64);
-    a.kchunk = ((a.ldk + nchunk - 1) / nchunk + GK - 1) / GK * GK;
+    // K ranges of an even number of steps (the kernel alternates two
+    // register sets); ldk = 2 nselp is a multiple of 2 GK
+    a.kchunk = ((a.ldk + nchunk - 1) / nchunk + 2 * GK - 1) / (2 * GK) * (2 * GK);
     nchunk = (a.ldk + a.kchunk - 1) / a.kchunk;
     dim3 grid(gx, nchunk);
     probe_mark(c, 0);
@@ -731,7 +733,8 @@ static int launch_back_gemm(mdff_ctx *c, int polar, int lo, int NCH, const int *
     int slots = g_slots(c, (const void *)gback_kernel(mode, CT), 512, g_back_lds_bytes_m(mode, CT, c->Kx + 1));
     int ns = c->g_bblocks > 0 ? std::max(1, std::min(c->nsplit_max, c->g_bblocks / gx))
                               : g_split(gx, slots, c->nsplit_max);
-    b.kchunk = ((b.ldk + ns - 1) / ns + GK - 1) / GK * GK;
+    // even step counts as in the forward sum; ldk = 2 Pk is a multiple of 2 GK
+    b.kchunk = ((b.ldk + ns - 1) / ns + 2 * GK - 1) / (2 * GK) * (2 * GK);
     ns = (b.ldk + b.kchunk - 1) / b.kchunk;
     dim3 grid(gx, ns);
     probe_mark(c, 0);
@@ -1776,7 +1779,7 @@ int mdff_create(mdff_ctx **out, const mdff_params *p, int nion, const int *typia
         c->nselp[0] = (n + 127) / 128 * 128;
         c->nselp[1] = (c->npol + 127) / 128 * 128;
         c->Pp = (c->P + 255) / 256 * 256;
-        c->Pk = (c->P + 15) / 16 * 16;
+        c->Pk = (c->P + 31) / 32 * 32;   // 2 Pk: whole pairs of K steps
     }
     // neighbour list capacity: all pairs when no list, else an estimate
     // from the number density grown on overflow

@@ -357,44 +357,49 @@ __global__ void __launch_bounds__(256) k_gemm_fwd(GFwdArgs a)
     size_t brow[CW];
 #pragma unroll
     for (int c = 0; c < CW; c++) brow[c] = (size_t)((ct0 + c) * 16 + cl) * a.ldk;
-    GRun<MODE> bh[CW], bl[CW], ra[GF_SLOTS], rl[GF_SLOTS];
-    auto load_step = [&](int kb) {
+    // Two register sets alternate (the K range holds an even number of
+    // steps): the loads of step k+2 are issued before step k computes, and
+    // never under a branch, so the compiler keeps them in flight across the
+    // step (a conditional load makes it wait for everything before the
+    // matrix instructions).  Past the end the last step is loaded again.
+    constexpr int NQ = (nslot + 255) / 256;
+    struct Set {
+        GRun<MODE> bh[CW], bl[CW], ra[NQ], rl[NQ];
+    } S0, S1;
+    auto load_step = [&](Set &S, int kb) {
 #pragma unroll
         for (int c = 0; c < CW; c++) {
-            bh[c] = g_load<MODE>(a.yz, brow[c] + kb + 8 * g);
-            if constexpr (SPLIT) bl[c] = g_load<MODE>(a.yz_lo, brow[c] + kb + 8 * g);
+            S.bh[c] = g_load<MODE>(a.yz, brow[c] + kb + 8 * g);
+            if constexpr (SPLIT) S.bl[c] = g_load<MODE>(a.yz_lo, brow[c] + kb + 8 * g);
         }
 #pragma unroll
-        for (int q = 0; q < GF_SLOTS; q++) {
-            const int e = threadIdx.x + 256 * q;
-            if (e < nslot) {
-                const int r = e / (GK / 8), q8 = (e % (GK / 8)) * 8;
-                ra[q] = g_load<MODE>(a.aw, (size_t)r * a.ldk + kb + q8);
-                if constexpr (SPLIT) rl[q] = g_load<MODE>(a.aw_lo, (size_t)r * a.ldk + kb + q8);
-            }
+        for (int q = 0; q < NQ; q++) {
+            const int e0 = threadIdx.x + 256 * q, e = e0 < nslot ? e0 : nslot - 1;
+            const int r = e / (GK / 8), q8 = (e % (GK / 8)) * 8;
+            S.ra[q] = g_load<MODE>(a.aw, (size_t)r * a.ldk + kb + q8);
+            if constexpr (SPLIT) S.rl[q] = g_load<MODE>(a.aw_lo, (size_t)r * a.ldk + kb + q8);
         }
     };
-    if (k0 < k1) load_step(k0);
-    for (int kb = k0; kb < k1; kb += GK) {
+    auto step = [&](Set &S, int kb) {
         // this step's A rows into LDS; its B runs stay in registers
         __syncthreads();
 #pragma unroll
-        for (int q = 0; q < GF_SLOTS; q++) {
+        for (int q = 0; q < NQ; q++) {
             const int e = threadIdx.x + 256 * q;
             if (e < nslot) {
                 const int r = e / (GK / 8), q8 = (e % (GK / 8)) * 8;
-                *(decltype(ra[q].v) *)(As + r * LD + q8) = ra[q].v;
-                if constexpr (SPLIT) *(decltype(rl[q].v) *)(Asl + r * LD + q8) = rl[q].v;
+                *(decltype(S.ra[q].v) *)(As + r * LD + q8) = S.ra[q].v;
+                if constexpr (SPLIT) *(decltype(S.rl[q].v) *)(Asl + r * LD + q8) = S.rl[q].v;
             }
         }
         GRun<MODE> ch[CW], cl_[CW];
 #pragma unroll
         for (int c = 0; c < CW; c++) {
-            ch[c] = bh[c];
-            if constexpr (SPLIT) cl_[c] = bl[c];
+            ch[c] = S.bh[c];
+            if constexpr (SPLIT) cl_[c] = S.bl[c];
         }
         __syncthreads();
-        if (kb + GK < k1) load_step(kb + GK);
+        load_step(S, kb + 2 * GK < k1 ? kb + 2 * GK : kb);
         if constexpr (SPLIT) {
             using V = gvec_t<MODE>;
             V ah[MTMAX], al[MTMAX];
@@ -426,6 +431,14 @@ __global__ void __launch_bounds__(256) k_gemm_fwd(GFwdArgs a)
                 }
             }
         }
+    };
+    if (k0 < k1) {
+        load_step(S0, k0);
+        load_step(S1, k0 + GK < k1 ? k0 + GK : k0);
+    }
+    for (int kb = k0; kb < k1; kb += 2 * GK) {
+        step(S0, kb);
+        step(S1, kb + GK);
     }
     // partial sums of this K range, [row][point] (lanes write consecutive
     // points; FP32 accumulators are stored as FP32)
@@ -482,20 +495,23 @@ struct GBackArgs {
 // computes this many; columns past NC are zero or ignored)
 inline int g_ct_inst(int CT) { return CT <= 4 ? 4 : (CT <= 7 ? 7 : (CT <= 13 ? 13 : MB_CT)); }
 
-// LDS of the back-projection: CTINST column tiles (x2 for the split modes)
-// and the block's ex table, (Kx+1) x 128 ions x (re, im)
+// largest k box of the back-projection: (Kx+1) 6 2 <= 16 MB_CT
+#define GB_KX1_MAX (16 * MB_CT / 12)
+
+// LDS of the back-projection: CTINST column tiles (x2 for the split modes),
+// the block's ex table, (Kx+1) x 128 ions x (re, im), and the column scales
 template <int MODE>
 inline size_t g_back_lds_bytes(int CTINST, int KX1)
 {
     size_t e = (size_t)CTINST * 16 * (GK + GLDS_PAD);
     size_t b = MODE == KM_F64 ? e * 8 : (MODE == KM_F32 ? e * 4 : e * 2 * 2);
-    return b + (size_t)KX1 * 128 * 2 * (MODE == KM_F64 ? 8 : 4);
+    return b + (size_t)KX1 * 128 * 2 * (MODE == KM_F64 ? 8 : 4) + sizeof(double) * GB_KX1_MAX * 6;
 }
 
 // back-projection: block = 8 waves x 16 ions, all CT column tiles, one K
 // range (grid.y); epilogue as k_sk_back_mfma (x ex[nx], moments)
 template <int MODE, int CTMAX, typename TT>
-__global__ void __launch_bounds__(512) k_gemm_back(GBackArgs a)
+__global__ void __launch_bounds__(512, ((MODE == KM_F64 || (MODE == KM_F32 && CTMAX >= 13)) ? 2 : 4)) k_gemm_back(GBackArgs a)
 {
     if (a.active && !*a.active) return;
     using T = typename GT<MODE>::T;
@@ -504,9 +520,11 @@ __global__ void __launch_bounds__(512) k_gemm_back(GBackArgs a)
     const int CT = a.CT;
     T *Bs = (T *)smem_raw;
     T *Bsl = Bs + CTMAX * 16 * LD;   // low parts of the split modes
-    // the ex values of this block's ions, [nx][128][re, im]
+    // the ex values of this block's ions, [nx][128][re, im], and the
+    // column scales [(nx, m)] (1 unless the columns were scaled)
     TT *exs = (TT *)(smem_raw + (size_t)CTMAX * 16 * LD * sizeof(T) * (g_is_split<MODE>() ? 2 : 1));
     const int KX1 = a.NC / (2 * a.CH);
+    double *cscs = (double *)(exs + (size_t)KX1 * 256);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int g = lane >> 4, cl = lane & 15;
     const int jrow0 = blockIdx.x * 128 + wave * 16;
@@ -520,42 +538,54 @@ __global__ void __launch_bounds__(512) k_gemm_back(GBackArgs a)
     constexpr bool SPLIT = g_is_split<MODE>();
     constexpr int nslot = CTMAX * 16 * (GK / 8);
     static_assert(nslot <= 512 * GB_SLOTS, "B staging slots");
-    {
-        const int jb = blockIdx.x * 128;
-        for (int e = threadIdx.x; e < KX1 * 256; e += 512) {
-            const int nx = e >> 8, q = e & 255, j = jb + (q >> 1);
-            exs[e] = j < a.nsel ? ((const TT *)a.tx)[2 * ((size_t)nx * a.nsel + j) + (q & 1)] : (TT)0;
-        }
-    }
-    GRun<MODE> ah, al, rb[GB_SLOTS], rbl[GB_SLOTS];   // this thread's staged B runs
-    auto load_step = [&](int kb) {
-        ah = g_load<MODE>(a.yzt, arow + kb + 8 * g);
-        if constexpr (SPLIT) al = g_load<MODE>(a.yzt_lo, arow + kb + 8 * g);
+    // staged with all loads issued first (clamped addresses, no branch
+    // around a load), then written to LDS
+    constexpr int EXU = (GB_KX1_MAX * 256 + 511) / 512;
+    TT xv[EXU];
+    const int jb = blockIdx.x * 128, nex = KX1 * 256, ncs = KX1 * a.CH;
 #pragma unroll
-        for (int q = 0; q < GB_SLOTS; q++) {
-            const int e = threadIdx.x + 512 * q;
-            if (e < nslot) {
-                const int r = e / (GK / 8), q8 = (e % (GK / 8)) * 8;
-                rb[q] = g_load<MODE>(a.wm, (size_t)r * a.ldk + kb + q8);
-                if constexpr (SPLIT) rbl[q] = g_load<MODE>(a.wm_lo, (size_t)r * a.ldk + kb + q8);
-            }
+    for (int u = 0; u < EXU; u++) {
+        const int e0 = threadIdx.x + 512 * u, e = e0 < nex ? e0 : 0;
+        const int nx = e >> 8, q = e & 255, j0 = jb + (q >> 1), j = j0 < a.nsel ? j0 : 0;
+        xv[u] = ((const TT *)a.tx)[2 * ((size_t)nx * a.nsel + j) + (q & 1)];
+    }
+    const int ics = (int)threadIdx.x < ncs ? (int)threadIdx.x : ncs - 1;
+    const double csv = a.csc ? a.csc[ics] : 1.0;
+    // two register sets alternate over an even number of steps; loads of
+    // step k+2 are issued unconditionally before step k computes (see
+    // k_gemm_fwd).  The FP64 form with 13 or more column tiles keeps one set
+    // (loads of step k+1 in flight during step k): two do not fit the
+    // 256 registers a wave of a 512-thread block can have.
+    constexpr bool TWO = !(MODE == KM_F64 && CTMAX >= 13);
+    constexpr int NQ = (nslot + 511) / 512;
+    struct Set {
+        GRun<MODE> ah, al, rb[NQ], rbl[NQ];
+    } S0, S1;
+    auto load_step = [&](Set &S, int kb) {
+        S.ah = g_load<MODE>(a.yzt, arow + kb + 8 * g);
+        if constexpr (SPLIT) S.al = g_load<MODE>(a.yzt_lo, arow + kb + 8 * g);
+#pragma unroll
+        for (int q = 0; q < NQ; q++) {
+            const int e0 = threadIdx.x + 512 * q, e = e0 < nslot ? e0 : nslot - 1;
+            const int r = e / (GK / 8), q8 = (e % (GK / 8)) * 8;
+            S.rb[q] = g_load<MODE>(a.wm, (size_t)r * a.ldk + kb + q8);
+            if constexpr (SPLIT) S.rbl[q] = g_load<MODE>(a.wm_lo, (size_t)r * a.ldk + kb + q8);
         }
     };
-    if (k0 < k1) load_step(k0);
-    for (int kb = k0; kb < k1; kb += GK) {
+    auto step = [&](Set &S, int kb, int dist) {
         __syncthreads();
 #pragma unroll
-        for (int q = 0; q < GB_SLOTS; q++) {
+        for (int q = 0; q < NQ; q++) {
             const int e = threadIdx.x + 512 * q;
             if (e < nslot) {
                 const int r = e / (GK / 8), q8 = (e % (GK / 8)) * 8;
-                *(decltype(rb[q].v) *)(Bs + r * LD + q8) = rb[q].v;
-                if constexpr (SPLIT) *(decltype(rbl[q].v) *)(Bsl + r * LD + q8) = rbl[q].v;
+                *(decltype(S.rb[q].v) *)(Bs + r * LD + q8) = S.rb[q].v;
+                if constexpr (SPLIT) *(decltype(S.rbl[q].v) *)(Bsl + r * LD + q8) = S.rbl[q].v;
             }
         }
-        GRun<MODE> ch = ah, cl_ = al;
+        GRun<MODE> ch = S.ah, cl_ = S.al;
         __syncthreads();
-        if (kb + GK < k1) load_step(kb + GK);
+        load_step(S, kb + dist * GK < k1 ? kb + dist * GK : kb);
         if constexpr (SPLIT) {
             using V = gvec_t<MODE>;
 #pragma unroll
@@ -573,62 +603,88 @@ __global__ void __launch_bounds__(512) k_gemm_back(GBackArgs a)
                 for (int ct = 0; ct < CTMAX; ct++)
                     acc[ct] = mf_mma<MODE>(ch.v[s], Bs[(ct * 16 + cl) * LD + 8 * g + s], acc[ct]);
         }
+    };
+    if (k0 < k1) {
+        load_step(S0, k0);
+        if constexpr (TWO) load_step(S1, k0 + GK < k1 ? k0 + GK : k0);
     }
-    __syncthreads();   // the ex table is complete (also for an empty K range)
-    // epilogue: Z_m = ex[nx] T_m on the even (real-part) columns, moments,
-    // then the sum over the 8 even lanes that share this lane's ions
-    double H[4][3], Mm[4][6];
+    // the staged tables into LDS (their loads were issued before the
+    // first steps' loads); read in the epilogue only
 #pragma unroll
-    for (int i = 0; i < 4; i++) {
-        H[i][0] = H[i][1] = H[i][2] = 0.0;
-#pragma unroll
-        for (int c = 0; c < 6; c++) Mm[i][c] = 0.0;
+    for (int u = 0; u < EXU; u++) {
+        const int e = threadIdx.x + 512 * u;
+        if (e < nex) exs[e] = (jb + ((e & 255) >> 1)) < a.nsel ? xv[u] : (TT)0;
     }
-#pragma unroll
-    for (int ct = 0; ct < CTMAX; ct++) {
-        if (ct >= CT) continue;
-        const int col = ct * 16 + cl;
-        const int nx = col / (2 * a.CH), m = (col >> 1) % a.CH;
-        const bool even = (cl & 1) == 0 && col < a.NC;
-        const double fnx = (double)nx;
-        const double csc = (a.csc && col < a.NC) ? a.csc[nx * a.CH + m] : 1.0;
-#pragma unroll
-        for (int i = 0; i < 4; i++) {
-            double tre = (double)acc[ct][i] * csc;
-            double tim = __shfl_xor(tre, 1, 64);
-            const int r = (MODE == KM_F64) ? g + 4 * i : 4 * g + i;
-            const int j = jrow0 + r;
-            if (!even || j >= a.nsel) continue;
-            const int ox = 2 * (nx * 128 + (j - blockIdx.x * 128));
-            double xr = (double)exs[ox], xi = (double)exs[ox + 1];
-            double zr = xr * tre - xi * tim, zi = xr * tim + xi * tre;
-            switch (m) {
-            case 0: H[i][0] += fnx * zi; Mm[i][0] += fnx * fnx * zr; break;
-            case 1: H[i][1] += zi; Mm[i][1] += fnx * zr; break;
-            case 2: H[i][2] += zi; Mm[i][2] += fnx * zr; break;
-            case 3: Mm[i][3] += zr; break;
-            case 4: Mm[i][4] += zr; break;
-            default: Mm[i][5] += zr; break;
-            }
+    if ((int)threadIdx.x < ncs) cscs[threadIdx.x] = csv;
+    if constexpr (TWO) {
+        for (int kb = k0; kb < k1; kb += 2 * GK) {
+            step(S0, kb, 2);
+            step(S1, kb + GK, 2);
         }
+    } else {
+        for (int kb = k0; kb < k1; kb += GK) step(S0, kb, 1);
     }
+    __syncthreads();   // the staging area is free; the ex table is complete (also for an empty K range)
+    // Epilogue, two waves at a time: their accumulator tiles (32 ions x all
+    // columns) go to the staging area, then every thread of the block takes
+    // one ion and every 16th nx and forms Z = ex[nx] T for each moment m
+    // (a uniform, unrolled loop: no lane-dependent branch), and the 16 lanes
+    // of an ion are summed.  Outputs: field H (x nx, 1, 1 for m = 0, 1, 2)
+    // and, with the gradient, M (x nx^2, nx, nx, 1, 1, 1 for m = 0..5).
+    using et = typename std::conditional<MODE == KM_F64, double, float>::type;
+    constexpr int TLD = 16 * CTMAX + 1;
+    static_assert(2 * 16 * TLD * sizeof(et) <= (size_t)CTMAX * 16 * LD * sizeof(T) * (g_is_split<MODE>() ? 2 : 1),
+                  "epilogue tiles fit the staging area");
+    et *tile = (et *)smem_raw;
     const int nch = (a.CH == 6) ? 9 : 3;
     double *out = a.part + (size_t)blockIdx.y * nch * a.nsel;
+    const int er = threadIdx.x >> 4, es = threadIdx.x & 15;
+    for (int round = 0; round < 4; round++) {
+        if ((wave >> 1) == round) {
+            et *tw = tile + (wave & 1) * 16 * TLD;
 #pragma unroll
-    for (int i = 0; i < 4; i++) {
-        double v[9] = {H[i][0], H[i][1], H[i][2], Mm[i][0], Mm[i][1], Mm[i][2], Mm[i][3], Mm[i][4], Mm[i][5]};
+            for (int ct = 0; ct < CTMAX; ct++)
+#pragma unroll
+                for (int i = 0; i < 4; i++) {
+                    const int r = (MODE == KM_F64) ? g + 4 * i : 4 * g + i;
+                    tw[r * TLD + ct * 16 + cl] = acc[ct][i];
+                }
+        }
+        __syncthreads();
+        const int jl = round * 32 + er;              // ion within the block
+        double o[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        for (int nx = es; nx < KX1; nx += 16) {
+            const double xr = (double)exs[2 * (nx * 128 + jl)], xi = (double)exs[2 * (nx * 128 + jl) + 1];
+            const double fnx = (double)nx;
+            const et *rowp = tile + er * TLD + nx * a.CH * 2;
+            const double *cs = cscs + nx * a.CH;
+            auto z = [&](int m, double &zr, double &zi) {
+                const double tre = (double)rowp[2 * m] * cs[m], tim = (double)rowp[2 * m + 1] * cs[m];
+                zr = xr * tre - xi * tim;
+                zi = xr * tim + xi * tre;
+            };
+            double zr, zi;
+            z(0, zr, zi); o[0] += fnx * zi; o[3] += fnx * fnx * zr;
+            z(1, zr, zi); o[1] += zi; o[4] += fnx * zr;
+            z(2, zr, zi); o[2] += zi; o[5] += fnx * zr;
+            if (a.CH == 6) {
+                z(3, zr, zi); o[6] += zr;
+                z(4, zr, zi); o[7] += zr;
+                z(5, zr, zi); o[8] += zr;
+            }
+        }
 #pragma unroll
         for (int c = 0; c < 9; c++) {
-            double s = v[c];
-            s += __shfl_xor(s, 2, 64);
-            s += __shfl_xor(s, 4, 64);
-            s += __shfl_xor(s, 8, 64);
-            v[c] = s;
+            double v = o[c];
+            v += __shfl_xor(v, 1, 64);
+            v += __shfl_xor(v, 2, 64);
+            v += __shfl_xor(v, 4, 64);
+            v += __shfl_xor(v, 8, 64);
+            o[c] = v;
         }
-        const int r = (MODE == KM_F64) ? g + 4 * i : 4 * g + i;
-        const int j = jrow0 + r;
-        if (cl == 0 && j < a.nsel) {
-            for (int c = 0; c < nch; c++) out[(size_t)c * a.nsel + j] = v[c];
-        }
+        const int j = blockIdx.x * 128 + jl;
+        if (es == 0 && j < a.nsel)
+            for (int c = 0; c < nch; c++) out[(size_t)c * a.nsel + j] = o[c];
+        __syncthreads();   // the tiles are rewritten in the next round
     }
 }
