@@ -193,7 +193,12 @@ int mdff_set_scf_inner(mdff_ctx *ctx, int inner);
  *      along; the rmsd of get_rmsd_mu (pim.c:431-449) is formed from the
  *      updated dipoles and field after every step, one SCF iteration per
  *      step; `inner` is the number of steps launched before the first
- *      convergence poll of a context (later calls use the last count)
+ *      convergence poll of a context (later calls use the last count).
+ *      Preconditioner pol^T
+ *   3  the same with the preconditioner pol^T + pol^T T_dir pol^T (T_dir:
+ *      the real-space dipole tensors and the self term, FP32); on the
+ *      12288-ion melt it needs as many steps as solver 2 (the slow modes are
+ *      long-wavelength, set by the k-space part) and costs a field pass more
  * Solvers 1 and 2 need every polarisability symmetric positive definite;
  * otherwise solver 0 runs.  No counterpart in the reference: the FP64 form
  * keeps the reference's Jacobi schedule. */

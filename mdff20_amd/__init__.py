@@ -294,7 +294,8 @@ class Engine:
 
     def set_scf_solver(self, solver, inner):
         """Mixed form: 0 / 1 Jacobi / CG defect correction (inner steps per FP64 evaluation),
-        2 anchored CG (one FP64 evaluation per SCF; the default)."""
+        2 anchored CG (one FP64 evaluation per SCF; the default), 3 the same with the
+        real-space dipole tensors in the preconditioner."""
         _check(self._L.mdff_set_scf_solver(self._h, solver, inner), "set_scf_solver", False)
 
     def set_tuning(self, lanes=0, katoms=0, ksplit=0):

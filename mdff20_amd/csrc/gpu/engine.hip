@@ -86,6 +86,8 @@ struct mdff_ctx {
     double *tmp_q = nullptr, *tmp_mu = nullptr, *tmp_qraw = nullptr, *tmp_mur = nullptr;  // drop-in scratch
     double *dc_R = nullptr, *dc_d = nullptr, *dc_g = nullptr;  // mixed SCF correction
     double *cg_z = nullptr, *cg_p = nullptr, *cg_part = nullptr; // its conjugate-gradient form
+    double *cg_w = nullptr;   // pol^T r of the preconditioned anchored CG
+    int cg_precond = 0;       // anchored CG: 0 preconditioner pol^T, 1 pol^T + pol^T T_dir pol^T
     double *ko_f = nullptr, *ko_dmu = nullptr; // scfkO: residual field and inner-loop dipoles
     double *t2ko = nullptr;   // scfkO: dipole tensors of the polarisable pairs at alpha 0.001
     double *scal = nullptr;
@@ -154,7 +156,7 @@ struct mdff_ctx {
     double last_rmsd = 0;
     int scf_chunk = 4;
     int inner_iters = 8;      // mixed form: low-precision correction iterations per FP64 refinement
-    int scf_solver = 2;       // mixed form: 0/1 Jacobi/CG defect correction, 2 anchored CG
+    int scf_solver = 2;       // mixed form: 0/1 Jacobi/CG defect correction, 2/3 anchored CG
     int cg_ok = 0;            // every polarisability symmetric positive definite (CG applies)
     long long n_force = 0, n_scf_fp64 = 0, n_scf_inner = 0;
 };
@@ -1127,8 +1129,26 @@ static int scf_loop_anchored(mdff_ctx *c, int istep)
                        (const double *)nullptr, ctl, c->scal, np, maxit, minit, c->p.conv_tol_ind);
     const int nb = cdiv(np, CG_BLOCK);
     double *rz[2] = {c->cg_part, c->cg_part + nb}, *pq = c->cg_part + 2 * nb, *rm = c->cg_part + 3 * nb;
-    hipLaunchKernelGGL(k_cg_init, dim3(nb), dim3(CG_BLOCK), 0, c->st, n, np, c->pidx, c->type, c->tab_d, c->ef,
-                       c->mu_ind, c->dc_R, c->cg_z, c->cg_p, c->dc_d, rz[0], act);
+    // preconditioner: z = pol^T r, or with c->cg_precond (and the FP32 pair
+    // tensors) z = w + pol^T (T_dir w + 2 selfa2 w), w = pol^T r
+    const bool pc = c->cg_precond && c->t2_ok && c->t2f != nullptr;
+    const unsigned tb = cdiv((size_t)n * 16, 256);
+    const double sself = 2.0 * c->selfa2;
+    auto precond = [&](double *rzout, int set_p) {
+        hipLaunchKernelGGL((k_t2_field<float, 16>), dim3(tb), dim3(256), 0, c->st, n, c->capP, np, c->pinv, c->cntp,
+                           c->idxp, c->t2f, c->cg_w, c->dc_g, act);
+        hipLaunchKernelGGL(k_cg_precond, dim3(nb), dim3(CG_BLOCK), 0, c->st, n, np, c->pidx, c->type, c->tab_d,
+                           c->dc_R, c->cg_w, c->dc_g, sself, c->cg_z, c->cg_p, set_p, rzout, act);
+    };
+    if (pc) {
+        // r and w = pol^T r (k_cg_init's z), then z = P r and p = z
+        hipLaunchKernelGGL(k_cg_init, dim3(nb), dim3(CG_BLOCK), 0, c->st, n, np, c->pidx, c->type, c->tab_d, c->ef,
+                           c->mu_ind, c->dc_R, c->cg_w, c->cg_p, c->dc_d, rz[0], act);
+        precond(rz[0], 1);
+    } else {
+        hipLaunchKernelGGL(k_cg_init, dim3(nb), dim3(CG_BLOCK), 0, c->st, n, np, c->pidx, c->type, c->tab_d, c->ef,
+                           c->mu_ind, c->dc_R, c->cg_z, c->cg_p, c->dc_d, rz[0], act);
+    }
     int cur = 0, launched = 1;
     const int limit = std::max(maxit, minit) + 1;
     auto step = [&]() {
@@ -1138,10 +1158,17 @@ static int scf_loop_anchored(mdff_ctx *c, int istep)
         c->mf_lo = keep;
         hipLaunchKernelGGL(k_cg_pq, dim3(nb), dim3(CG_BLOCK), 0, c->st, n, np, c->pidx, c->type, c->tab_d, c->cg_p,
                            c->dc_g, pq, act);
-        hipLaunchKernelGGL(k_cga_update, dim3(nb), dim3(CG_BLOCK), 0, c->st, n, np, nb, c->pidx, c->type, c->tab_d,
-                           c->cg_p, c->dc_g, c->mu_ind, c->ef, c->dc_R, c->cg_z, rz[cur], pq, rz[cur ^ 1], rm, act);
+        if (pc) {
+            hipLaunchKernelGGL(k_cga_update_pc, dim3(nb), dim3(CG_BLOCK), 0, c->st, n, np, nb, c->pidx, c->type,
+                               c->tab_d, c->cg_p, c->dc_g, c->mu_ind, c->ef, c->dc_R, c->cg_w, rz[cur], pq, rm, act);
+            precond(rz[cur ^ 1], 0);
+        } else {
+            hipLaunchKernelGGL(k_cga_update, dim3(nb), dim3(CG_BLOCK), 0, c->st, n, np, nb, c->pidx, c->type,
+                               c->tab_d, c->cg_p, c->dc_g, c->mu_ind, c->ef, c->dc_R, c->cg_z, rz[cur], pq,
+                               rz[cur ^ 1], rm, act);
+        }
         hipLaunchKernelGGL(k_cga_next, dim3(nb), dim3(CG_BLOCK), 0, c->st, n, np, nb, c->pidx, c->cg_z, c->cg_p,
-                           rz[cur], rz[cur ^ 1], rm, ctl, c->scal, maxit, minit, c->p.conv_tol_ind);
+                           rz[cur], rz[cur ^ 1], rm, c->ctl, c->scal, maxit, minit, c->p.conv_tol_ind);
         cur ^= 1;
         launched++;
     };
@@ -1166,7 +1193,7 @@ static int scf_loop_anchored(mdff_ctx *c, int istep)
 
 static int scf_mixed(mdff_ctx *c, int istep)
 {
-    if (c->scf_solver == 2 && c->cg_ok && c->npol > 0) return scf_loop_anchored(c, istep);
+    if (c->scf_solver >= 2 && c->cg_ok && c->npol > 0) return scf_loop_anchored(c, istep);
     return scf_loop_mixed(c, istep);
 }
 
@@ -1501,7 +1528,7 @@ void mdff_destroy(mdff_ctx *c)
     double **dp[] = {&c->r, &c->v, &c->f, &c->fold, &c->rs, &c->s, &c->s0, &c->invmass, &c->massd, &c->q, &c->qz,
                      &c->mu_static, &c->mu_ind, &c->mu_tot, &c->hist, &c->ef_stat, &c->ef, &c->ef_dir, &c->ef_rec,
                      &c->efg, &c->efg_dir, &c->efg_rec, &c->f_sr, &c->f_dir, &c->f_rec, &c->acc_sr, &c->uw_dir,
-                     &c->selfbuf, &c->ke, &c->upol_i, &c->rmsd_i, &c->tmp_q, &c->tmp_mu, &c->tmp_qraw, &c->tmp_mur, &c->dc_R, &c->dc_d, &c->dc_g, &c->cg_z, &c->cg_p, &c->cg_part, &c->ko_f, &c->ko_dmu, &c->t2ko, &c->scal,
+                     &c->selfbuf, &c->ke, &c->upol_i, &c->rmsd_i, &c->tmp_q, &c->tmp_mu, &c->tmp_qraw, &c->tmp_mur, &c->dc_R, &c->dc_d, &c->dc_g, &c->cg_z, &c->cg_p, &c->cg_part, &c->ko_f, &c->ko_dmu, &c->t2ko, &c->cg_w, &c->scal,
                      &c->kvec, &c->Ak, &c->kcoe, &c->W, &c->ek, &c->kpart, &c->bpart, &c->ksum, &c->gsc};
     for (auto pp : dp) dfree(*pp);
     dfree(c->type);
@@ -1652,7 +1679,7 @@ int mdff_create(mdff_ctx **out, const mdff_params *p, int nion, const int *typia
     ALLOC(c->tmp_q, n); ALLOC(c->tmp_mu, 3 * n); ALLOC(c->tmp_qraw, n); ALLOC(c->tmp_mur, 3 * n);
     ALLOC(c->dc_R, 3 * n); ALLOC(c->dc_d, 3 * n); ALLOC(c->dc_g, 3 * n);
     ALLOC(c->cg_z, 3 * n); ALLOC(c->cg_p, 3 * n); ALLOC(c->cg_part, 4 * (size_t)cdiv(n, CG_BLOCK));
-    ALLOC(c->ko_f, 3 * n); ALLOC(c->ko_dmu, 3 * n);
+    ALLOC(c->ko_f, 3 * n); ALLOC(c->ko_dmu, 3 * n); ALLOC(c->cg_w, 3 * n);
     ALLOC(c->scal, S_COUNT); ALLOC(c->ctl, C_COUNT); ALLOC(c->tab_d, 1);
     if (hipHostMalloc((void **)&c->ctl_h, sizeof(int) * C_COUNT) != hipSuccess) { mdff_destroy(c); return MDFF_ERR_NOMEM; }
     if (hipHostMalloc((void **)&c->scal_h, sizeof(double) * S_COUNT) != hipSuccess) { mdff_destroy(c); return MDFF_ERR_NOMEM; }
@@ -2281,8 +2308,9 @@ int mdff_set_scf_inner(mdff_ctx *c, int inner)
 
 int mdff_set_scf_solver(mdff_ctx *c, int solver, int inner)
 {
-    if (!c || solver < 0 || solver > 2 || inner < 0 || inner > 64) return MDFF_ERR_ARG;
+    if (!c || solver < 0 || solver > 3 || inner < 0 || inner > 64) return MDFF_ERR_ARG;
     c->scf_solver = solver;
+    c->cg_precond = solver == 3;
     c->inner_iters = inner;
     c->hint_mixed = c->hint_cg = 0;
     return MDFF_OK;

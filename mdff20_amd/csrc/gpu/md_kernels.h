@@ -761,3 +761,84 @@ __global__ void k_ko_resid(int n, int npol, const int *__restrict__ pidx, const 
         f[o] = ef[o] - mu[o] * I[4 * a];
     }
 }
+
+// Preconditioned form of the anchored CG: z = P r with
+//   P = pol^T + pol^T T_dir pol^T
+// (the first two terms of (pol^-T - T)^-1 = pol^T (I - T pol^T)^-1 with the
+// real-space dipole tensors and the self term for T), symmetric for a
+// symmetric polarisability.  The update writes w = pol^T r; the field of w
+// comes from k_t2_field; k_cg_precond forms z and the partial sums of r.z.
+__global__ void __launch_bounds__(CG_BLOCK) k_cga_update_pc(int n, int npol, int nb, const int *__restrict__ pidx,
+                                                            const int *__restrict__ type,
+                                                            const TypeTab *__restrict__ T,
+                                                            const double *__restrict__ p,
+                                                            const double *__restrict__ Tp, double *mu, double *ef,
+                                                            double *r, double *w, const double *__restrict__ rz_old,
+                                                            const double *__restrict__ pq_part, double *rm_part,
+                                                            const int *active)
+{
+    if (!*active) return;
+    __shared__ double sh[CG_BLOCK];
+    const double rz = cg_sum_partials(rz_old, nb, sh);
+    const double pq = cg_sum_partials(pq_part, nb, sh);
+    const double g = (pq != 0.0) ? rz / pq : 0.0;
+    const int k = blockIdx.x * CG_BLOCK + threadIdx.x;
+    double rms = 0.0;
+    if (k < npol) {
+        const int i = pidx[k], t = type[i];
+        const double *P = T->pol[t], *I = T->invpol[t];
+        double m[3], e[3], rr[3];
+        for (int a = 0; a < 3; a++) {
+            const size_t o = (size_t)a * n + i;
+            m[a] = mu[o] + g * p[o];
+            e[a] = ef[o] + g * Tp[o];
+            mu[o] = m[a];
+            ef[o] = e[a];
+        }
+        for (int a = 0; a < 3; a++) {
+            rr[a] = e[a] - (I[a] * m[0] + I[3 + a] * m[1] + I[6 + a] * m[2]);
+            r[(size_t)a * n + i] = rr[a];
+            const double paa = P[4 * a];
+            if (paa != 0.0) {
+                const double d = m[a] / paa - e[a];
+                rms += d * d;
+            }
+        }
+        for (int a = 0; a < 3; a++)
+            w[(size_t)a * n + i] = P[0 * 3 + a] * rr[0] + P[1 * 3 + a] * rr[1] + P[2 * 3 + a] * rr[2];
+    }
+    const double sr = cg_block_sum(rms, sh);
+    if (threadIdx.x == 0) rm_part[blockIdx.x] = sr;
+}
+
+// z = w + pol^T (T_dir w + sself w); partial sums of r.z; p = z on the
+// first call of a solve
+__global__ void __launch_bounds__(CG_BLOCK) k_cg_precond(int n, int npol, const int *__restrict__ pidx,
+                                                         const int *__restrict__ type, const TypeTab *__restrict__ T,
+                                                         const double *__restrict__ r, const double *__restrict__ w,
+                                                         const double *__restrict__ tw, double sself, double *z,
+                                                         double *p, int set_p, double *rz_part, const int *active)
+{
+    if (!*active) return;
+    __shared__ double sh[CG_BLOCK];
+    const int k = blockIdx.x * CG_BLOCK + threadIdx.x;
+    double acc = 0.0;
+    if (k < npol) {
+        const int i = pidx[k], t = type[i];
+        const double *P = T->pol[t];
+        double f[3];
+        for (int a = 0; a < 3; a++) {
+            const size_t o = (size_t)a * n + i;
+            f[a] = tw[o] + sself * w[o];
+        }
+        for (int a = 0; a < 3; a++) {
+            const size_t o = (size_t)a * n + i;
+            const double zz = w[o] + (P[0 * 3 + a] * f[0] + P[1 * 3 + a] * f[1] + P[2 * 3 + a] * f[2]);
+            z[o] = zz;
+            if (set_p) p[o] = zz;
+            acc += r[o] * zz;
+        }
+    }
+    const double s = cg_block_sum(acc, sh);
+    if (threadIdx.x == 0) rz_part[blockIdx.x] = s;
+}

@@ -133,22 +133,25 @@ def test_mixed_scf_solvers_reach_the_fp64_fixed_point(case):
     with m.Engine(p, cfg.typia, cfg.A, cfg.r, precision=m.FP64) as ef, \
             m.Engine(p, cfg.typia, cfg.A, cfg.r, precision=m.MIXED) as ej, \
             m.Engine(p, cfg.typia, cfg.A, cfg.r, precision=m.MIXED) as ec, \
-            m.Engine(p, cfg.typia, cfg.A, cfg.r, precision=m.MIXED) as ea:
+            m.Engine(p, cfg.typia, cfg.A, cfg.r, precision=m.MIXED) as ea, \
+            m.Engine(p, cfg.typia, cfg.A, cfg.r, precision=m.MIXED) as e3:
         ej.set_scf_solver(0, 6)
         ec.set_scf_solver(1, 6)
         ea.set_scf_solver(2, 6)
-        tf, tj, tc, ta = ef.engforce(0), ej.engforce(0), ec.engforce(0), ea.engforce(0)
-        assert tf["status"] == 0 and tj["status"] == 0 and tc["status"] == 0 and ta["status"] == 0
-        assert tj["rmsd"] <= 1e-10 and tc["rmsd"] <= 1e-10 and ta["rmsd"] <= 1e-10
+        e3.set_scf_solver(3, 6)
+        tf, tj, tc, ta, t3 = ef.engforce(0), ej.engforce(0), ec.engforce(0), ea.engforce(0), e3.engforce(0)
+        assert all(t["status"] == 0 for t in (tf, tj, tc, ta, t3))
+        assert all(t["rmsd"] <= 1e-10 for t in (tj, tc, ta, t3))
         muf = ef.dipoles()
-        for e in (ej, ec, ea):
+        for e in (ej, ec, ea, e3):
             assert rel(e.dipoles(), muf) < 1e-5
             assert rel(e.forces(), ef.forces()) < 1e-5
         assert rel(ec.dipoles(), ej.dipoles()) < 1e-5
         assert rel(ea.dipoles(), ec.dipoles()) < 1e-5
         assert ec.counters()["scf_fp64"] <= ej.counters()["scf_fp64"]
-        # the anchored solve evaluates the FP64 field once (when there is an SCF)
-        assert ea.counters()["scf_fp64"] == (1 if ea.counters()["npol"] > 0 else 0)
+        # the anchored solves evaluate the FP64 field once (when there is an SCF)
+        for e in (ea, e3):
+            assert e.counters()["scf_fp64"] == (1 if e.counters()["npol"] > 0 else 0)
 
 
 @pytest.mark.parametrize("case", ["pim192_static", "bsio_200_nvt", "melt216_nve"])
