@@ -142,11 +142,12 @@ int mdff_set_kspace_modes(mdff_ctx *ctx, int eval_mode, int inner_mode);
 
 /* Shape of the matrix-core k-space sums over stored operands: column tiles
  * of 16 points per wavefront in the forward sum for the FP64 mode (cw64) and
- * the FP32 / split-BF16 modes (cw32), each 1, 2 or 4, and the target block
- * counts of the forward and back-projection sums; 0 keeps a value, and a
- * block count of 0 (the default) sizes each grid to one round of the blocks
- * the device holds at once.  Default column tiles: 1 (FP64), 2 (others).
- * No counterpart in the reference (tuning). */
+ * the FP32 / split modes (cw32), each 1, 2 or 4, and the target block counts
+ * of the forward and back-projection sums (their K ranges are split to reach
+ * about that many blocks); 0 keeps a value.  Defaults: column tiles 1
+ * (FP64), 2 (others); block targets timed on the first launch of each sum
+ * shape (1/2 to 3 blocks per compute unit, the fastest kept).  No
+ * counterpart in the reference (tuning). */
 int mdff_set_kspace_shape(mdff_ctx *ctx, int cw64, int cw32, int fwd_blocks, int back_blocks);
 
 /* Device timing of the last engforce by phase, in ms (HIP events on the

@@ -125,8 +125,6 @@ struct mdff_ctx {
     int g_fblocks = 0;        // target blocks of the forward GEMM (0: one round of resident blocks)
     int g_bblocks = 0;        // target blocks of the back-projection GEMM (0: one round)
     int ncu = 256;            // compute units of the device
-    struct OccEntry { const void *fn; size_t lds; int slots; };
-    std::vector<OccEntry> occ_cache;
     int nselp[2] = {0, 0};    // ions padded to 128 (all, polarisable)
     int Pp = 0, Pk = 0;       // (ny,nz) points padded: forward rows, back-projection K
     void *yz[4][2][2] = {};   // [mode][polar][hi,lo]: YZ[p][2j+ri]
@@ -135,6 +133,8 @@ struct mdff_ctx {
     void *aw[4][2] = {};      // [mode][hi,lo]: rows (nx,c,re/im) of the forward sum
     void *wm[4][2] = {};      // [mode][hi,lo]: columns (nx,m,re/im) of the back sum
     double *gsc = nullptr;    // split-FP16 scales: weights [0,8) and inverses [8,16), WM columns from 16
+    struct GTune { int mode, dir, tiles, polar, blocks; };
+    std::vector<GTune> gtune;  // block targets of the stored-operand sums, timed on first use
     int mf_lo = KM_F16X3;     // matrix-core mode of the FP32-table evaluations (static field, forces)
     int mf_inner = KM_BF16X3; // mode of the mixed-form low-precision SCF steps
     int mf_blocks = 512;      // target blocks of the MFMA kernels
@@ -477,23 +477,6 @@ static int ensure_yz(mdff_ctx *c, int mode, int polar, int lo)
     return MDFF_OK;
 }
 
-// LDS of the instantiated forward / back kernels (MT, CT: the shape's tile
-// counts; the instantiated counts follow from them)
-static size_t g_fwd_lds_bytes_m(int mode, int MT)
-{
-    const int MTI = g_mt_inst(MT);
-    return mode == KM_F64 ? g_fwd_lds_bytes<KM_F64>(MTI)
-                          : (mode == KM_F32 ? g_fwd_lds_bytes<KM_F32>(MTI) : g_fwd_lds_bytes<KM_BF16X3>(MTI));
-    // (the split modes share one LDS size)
-}
-
-static size_t g_back_lds_bytes_m(int mode, int CT, int KX1)
-{
-    const int CTI = g_ct_inst(CT);
-    return mode == KM_F64 ? g_back_lds_bytes<KM_F64>(CTI, KX1)
-                          : (mode == KM_F32 ? g_back_lds_bytes<KM_F32>(CTI, KX1) : g_back_lds_bytes<KM_BF16X3>(CTI, KX1));
-}
-
 template <int MODE, int CW>
 static const void *gfwd_kernel_t(int MT)
 {
@@ -538,35 +521,6 @@ static const void *gback_kernel(int mode, int CT)
     if (mode == KM_F32) return gback_kernel_t<KM_F32, float>(CT);
     if (mode == KM_F16X3) return gback_kernel_t<KM_F16X3, float>(CT);
     return gback_kernel_t<KM_BF16X3, float>(CT);
-}
-
-// number of K splits for gx blocks per split: the smallest that fills at
-// least one round of the resident blocks at >= 95% (else the fullest)
-static int g_split(int gx, int slots, int maxs)
-{
-    int best = 1;
-    double beste = -1.0;
-    for (int ns = 1; ns <= maxs; ns++) {
-        long long b = (long long)gx * ns;
-        long long rounds = (b + slots - 1) / slots;
-        double eff = (double)b / (double)(rounds * slots);
-        if (b >= slots && eff >= 0.95) return ns;
-        if (eff > beste + 1e-9) { beste = eff; best = ns; }
-    }
-    return best;
-}
-
-// resident blocks of a kernel over the whole device (compute units times
-// blocks per unit), cached per kernel
-static int g_slots(mdff_ctx *c, const void *fn, int threads, size_t lds)
-{
-    for (auto &e : c->occ_cache)
-        if (e.fn == fn && e.lds == lds) return e.slots;
-    int per = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, fn, threads, lds) != hipSuccess || per < 1) per = 1;
-    int slots = per * c->ncu;
-    c->occ_cache.push_back({fn, lds, slots});
-    return slots;
 }
 
 // launch the instantiation for the shape's tile count
@@ -624,6 +578,44 @@ static bool g_back_attrs(mdff_ctx *)
            g_back_attrs_t<KM_BF16X3, float>(KX1) && g_back_attrs_t<KM_F16X3, float>(KX1);
 }
 
+// Block target of one stored-operand sum (mode, direction, tile count,
+// ion set): on first use, candidates of 1/2 .. 3 blocks per compute unit
+// are timed with HIP events (one warm launch, then three) and the fastest
+// is kept.  `run` launches the sum (and, for the forward sum, its
+// partial-sum pass) ungated; the launches only rewrite partial-sum buffers
+// before the real launch writes them again.
+template <typename F>
+static int gtune_blocks(mdff_ctx *c, int mode, int dir, int tiles, int polar, F &&run)
+{
+    for (const auto &t : c->gtune)
+        if (t.mode == mode && t.dir == dir && t.tiles == tiles && t.polar == polar) return t.blocks;
+    const int cand[6] = {c->ncu / 2, 3 * c->ncu / 4, c->ncu, 3 * c->ncu / 2, 2 * c->ncu, 3 * c->ncu};
+    int best = c->ncu;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (hipEventCreate(&e0) == hipSuccess && hipEventCreate(&e1) == hipSuccess) {
+        float bt = 1e30f;
+        int last = -1;
+        for (int k = 0; k < 6; k++) {
+            const int splits = run(cand[k], 0);   // the split count this target gives (0: no launch)
+            if (splits == last) continue;         // same grid as the previous candidate
+            last = splits;
+            run(cand[k], 1);
+            (void)hipEventRecord(e0, c->st);
+            for (int r = 0; r < 3; r++) run(cand[k], 1);
+            (void)hipEventRecord(e1, c->st);
+            float ms = 0.0f;
+            if (hipEventSynchronize(e1) == hipSuccess && hipEventElapsedTime(&ms, e0, e1) == hipSuccess && ms < bt) {
+                bt = ms;
+                best = cand[k];
+            }
+        }
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    c->gtune.push_back({mode, dir, tiles, polar, best});
+    return best;
+}
+
 // forward sum over stored operands; 0 when outside the kernel's shapes
 static int launch_fwd_gemm(mdff_ctx *c, int polar, int lo, int C, const double *q_w, const double *mu_w,
                            const int *active, int *nchunk_out)
@@ -668,31 +660,39 @@ static int launch_fwd_gemm(mdff_ctx *c, int polar, int lo, int C, const double *
     a.part = c->kpart; a.active = active;
     const int CW = c->g_cw[mode == KM_F64 ? 0 : 1];
     const int gx = (c->P + 64 * CW - 1) / (64 * CW);
-    // split K so that the grid fills whole rounds of the resident blocks
-    int slots = g_slots(c, (const void *)gfwd_kernel(mode, CW, MT), 256, g_fwd_lds_bytes_m(mode, MT));
-    int nchunk = c->g_fblocks > 0 ? std::max(1, std::min(64, c->g_fblocks / gx)) : g_split(gx, slots, 64);
-    // K ranges of an even number of steps (the kernel alternates two
-    // register sets); ldk = 2 nselp is a multiple of 2 GK
-    a.kchunk = ((a.ldk + nchunk - 1) / nchunk + 2 * GK - 1) / (2 * GK) * (2 * GK);
-    nchunk = (a.ldk + a.kchunk - 1) / a.kchunk;
-    dim3 grid(gx, nchunk);
-    probe_mark(c, 0);
-    if (mode == KM_F64) gfwd_cw<KM_F64>(c, grid, CW, MT, a);
-    else if (mode == KM_F32) gfwd_cw<KM_F32>(c, grid, CW, MT, a);
-    else if (mode == KM_F16X3) gfwd_cw<KM_F16X3>(c, grid, CW, MT, a);
-    else gfwd_cw<KM_BF16X3>(c, grid, CW, MT, a);
-    probe_mark(c, 1);
-    // chunk sums, all threads busy (the per-k reduction then reads one chunk)
     const size_t nval = (size_t)M * c->P;
-    if (mode == KM_F64)
-        hipLaunchKernelGGL(k_part_sum<double>, dim3(cdiv(nval, 256)), dim3(256), 0, c->st, (const double *)c->kpart,
-                           nchunk, M, c->P, c->ksum, C, winv, active);
-    else
-        hipLaunchKernelGGL(k_part_sum<float>, dim3(cdiv(nval, 256)), dim3(256), 0, c->st, (const float *)c->kpart,
-                           nchunk, M, c->P, c->ksum, C, winv, active);
+    // K split to reach `blocks` blocks (K ranges of an even number of steps:
+    // the kernel alternates two register sets; ldk = 2 nselp is a multiple
+    // of 2 GK), then the chunk sums; with go = 0 only the split count
+    auto run = [&](int blocks, int go, const int *act, bool probe) {
+        int nch = std::max(1, std::min(64, (blocks + gx - 1) / gx));
+        GFwdArgs ga2 = a;
+        ga2.active = act;
+        ga2.kchunk = ((ga2.ldk + nch - 1) / nch + 2 * GK - 1) / (2 * GK) * (2 * GK);
+        nch = (ga2.ldk + ga2.kchunk - 1) / ga2.kchunk;
+        if (!go) return nch;
+        dim3 grid(gx, nch);
+        if (probe) probe_mark(c, 0);
+        if (mode == KM_F64) gfwd_cw<KM_F64>(c, grid, CW, MT, ga2);
+        else if (mode == KM_F32) gfwd_cw<KM_F32>(c, grid, CW, MT, ga2);
+        else if (mode == KM_F16X3) gfwd_cw<KM_F16X3>(c, grid, CW, MT, ga2);
+        else gfwd_cw<KM_BF16X3>(c, grid, CW, MT, ga2);
+        if (probe) probe_mark(c, 1);
+        if (mode == KM_F64)
+            hipLaunchKernelGGL(k_part_sum<double>, dim3(cdiv(nval, 256)), dim3(256), 0, c->st,
+                               (const double *)c->kpart, nch, M, c->P, c->ksum, C, winv, act);
+        else
+            hipLaunchKernelGGL(k_part_sum<float>, dim3(cdiv(nval, 256)), dim3(256), 0, c->st, (const float *)c->kpart,
+                               nch, M, c->P, c->ksum, C, winv, act);
+        return nch;
+    };
+    const int fb = c->g_fblocks > 0 ? c->g_fblocks
+                                    : gtune_blocks(c, mode, 0, MT, polar, [&](int b, int go) {
+                                          return run(b, go, nullptr, false);
+                                      });
+    run(fb, 1, active, true);
     c->fwd_part = 1;
-    nchunk = 1;
-    *nchunk_out = nchunk;
+    *nchunk_out = 1;
     return 1;
 }
 
@@ -732,19 +732,29 @@ static int launch_back_gemm(mdff_ctx *c, int polar, int lo, int NCH, const int *
     b.nsel = nsel; b.CH = CH; b.NC = NC; b.CT = CT; b.ldk = 2 * c->Pk;
     b.part = c->bpart; b.active = active;
     const int gx = nselp / 128;
-    int slots = g_slots(c, (const void *)gback_kernel(mode, CT), 512, g_back_lds_bytes_m(mode, CT, c->Kx + 1));
-    int ns = c->g_bblocks > 0 ? std::max(1, std::min(c->nsplit_max, c->g_bblocks / gx))
-                              : g_split(gx, slots, c->nsplit_max);
-    // even step counts as in the forward sum; ldk = 2 Pk is a multiple of 2 GK
-    b.kchunk = ((b.ldk + ns - 1) / ns + 2 * GK - 1) / (2 * GK) * (2 * GK);
-    ns = (b.ldk + b.kchunk - 1) / b.kchunk;
-    dim3 grid(gx, ns);
-    probe_mark(c, 0);
-    if (mode == KM_F64) gback_ct<KM_F64, double>(c, grid, CT, b);
-    else if (mode == KM_F32) gback_ct<KM_F32, float>(c, grid, CT, b);
-    else if (mode == KM_F16X3) gback_ct<KM_F16X3, float>(c, grid, CT, b);
-    else gback_ct<KM_BF16X3, float>(c, grid, CT, b);
-    probe_mark(c, 1);
+    // K split to reach `blocks` blocks (even step counts as in the forward
+    // sum; ldk = 2 Pk is a multiple of 2 GK); with go = 0 only the count
+    auto run = [&](int blocks, int go, const int *act, bool probe) {
+        int nsp = std::max(1, std::min(c->nsplit_max, (blocks + gx - 1) / gx));
+        GBackArgs gb2 = b;
+        gb2.active = act;
+        gb2.kchunk = ((gb2.ldk + nsp - 1) / nsp + 2 * GK - 1) / (2 * GK) * (2 * GK);
+        nsp = (gb2.ldk + gb2.kchunk - 1) / gb2.kchunk;
+        if (!go) return nsp;
+        dim3 grid(gx, nsp);
+        if (probe) probe_mark(c, 0);
+        if (mode == KM_F64) gback_ct<KM_F64, double>(c, grid, CT, gb2);
+        else if (mode == KM_F32) gback_ct<KM_F32, float>(c, grid, CT, gb2);
+        else if (mode == KM_F16X3) gback_ct<KM_F16X3, float>(c, grid, CT, gb2);
+        else gback_ct<KM_BF16X3, float>(c, grid, CT, gb2);
+        if (probe) probe_mark(c, 1);
+        return nsp;
+    };
+    const int bb = c->g_bblocks > 0 ? c->g_bblocks
+                                    : gtune_blocks(c, mode, 1, CT, polar, [&](int bl, int go) {
+                                          return run(bl, go, nullptr, false);
+                                      });
+    const int ns = run(bb, 1, active, true);
     *nsplit_out = ns;
     return 1;
 }
@@ -1861,6 +1871,7 @@ int mdff_set_kspace_shape(mdff_ctx *c, int cw64, int cw32, int fwd_blocks, int b
         if (cw[k]) c->g_cw[k] = cw[k];
     if (fwd_blocks) c->g_fblocks = fwd_blocks;
     if (back_blocks) c->g_bblocks = back_blocks;
+    c->gtune.clear();   // timed block targets belong to the old shapes
     return MDFF_OK;
 }
 

@@ -517,7 +517,6 @@ __global__ void __launch_bounds__(512, ((MODE == KM_F64 || (MODE == KM_F32 && CT
     using T = typename GT<MODE>::T;
     constexpr int LD = GK + GLDS_PAD;
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    const int CT = a.CT;
     T *Bs = (T *)smem_raw;
     T *Bsl = Bs + CTMAX * 16 * LD;   // low parts of the split modes
     // the ex values of this block's ions, [nx][128][re, im], and the
