@@ -375,7 +375,9 @@ static int fwd_mfma_t(mdff_ctx *c, SkMArgs &m, int nsel)
 
 static int launch_fwd_gemm(mdff_ctx *c, int polar, int lo, int C, const double *q_w, const double *mu_w,
                            const int *active, int *nchunk_out);
-static int launch_back_gemm(mdff_ctx *c, int polar, int lo, int NCH, const int *active, int *nsplit_out);
+static int launch_back_gemm(mdff_ctx *c, int polar, int lo, int NCH, const int *active, int *nsplit_out,
+                            bool wm_built);
+static int wm_alloc(mdff_ctx *c, int mode);
 
 static int launch_fwd_mfma(mdff_ctx *c, int polar, int lo, int C, const double *q_w, const double *mu_w,
                            const int *active, int *nchunk_out)
@@ -413,12 +415,13 @@ static int back_mfma_t(mdff_ctx *c, BkMArgs &b, int nsel)
 }
 
 // back-projection on the matrix cores; returns 0 when outside the kernel's shapes
-static int launch_back_mfma(mdff_ctx *c, int polar, int lo, int NCH, const int *active, int *nsplit_out)
+static int launch_back_mfma(mdff_ctx *c, int polar, int lo, int NCH, const int *active, int *nsplit_out,
+                            bool wm_built = false)
 {
     int CH = NCH == 9 ? 6 : 3;
     int NC = (c->Kx + 1) * CH * 2;
     if (!c->use_mfma || NC > MB_CT * 16) return 0;
-    if (c->gemm && launch_back_gemm(c, polar, lo, NCH, active, nsplit_out)) return 1;
+    if (c->gemm && launch_back_gemm(c, polar, lo, NCH, active, nsplit_out, wm_built)) return 1;
     int nsel = polar ? c->npol : c->n;
     BkMArgs b;
     b.nsel = nsel; b.Kx = c->Kx; b.Ky = c->Ky; b.Kz = c->Kz; b.P = c->P; b.CH = CH; b.NC = NC;
@@ -696,8 +699,20 @@ static int launch_fwd_gemm(mdff_ctx *c, int polar, int lo, int C, const double *
     return 1;
 }
 
-// back-projection over stored operands; 0 when outside the kernel's shapes
-static int launch_back_gemm(mdff_ctx *c, int polar, int lo, int NCH, const int *active, int *nsplit_out)
+// WM columns of a mode (both parts for the split modes)
+static int wm_alloc(mdff_ctx *c, int mode)
+{
+    const size_t wmb = (size_t)16 * MB_CT * 2 * c->Pk * g_elt(mode);
+    int rc;
+    if ((rc = galloc(c, &c->wm[mode][0], wmb)) != MDFF_OK) return rc;
+    if (g_split_mode(mode) && (rc = galloc(c, &c->wm[mode][1], wmb)) != MDFF_OK) return rc;
+    return MDFF_OK;
+}
+
+// back-projection over stored operands (WM columns formed here unless
+// wm_built: k_sk_reduce_wm wrote them); 0 when outside the kernel's shapes
+static int launch_back_gemm(mdff_ctx *c, int polar, int lo, int NCH, const int *active, int *nsplit_out,
+                            bool wm_built)
 {
     const int CH = NCH == 9 ? 6 : 3;
     const int NC = (c->Kx + 1) * CH * 2;
@@ -707,12 +722,11 @@ static int launch_back_gemm(mdff_ctx *c, int polar, int lo, int NCH, const int *
     if (ensure_yz(c, mode, polar, lo)) return 0;
     const int nsel = polar ? c->npol : c->n;
     const int nselp = c->nselp[polar];
-    const size_t e = g_elt(mode);
-    const size_t wmb = (size_t)16 * MB_CT * 2 * c->Pk * e;
-    if (galloc(c, &c->wm[mode][0], wmb)) return 0;
-    if (g_split_mode(mode) && galloc(c, &c->wm[mode][1], wmb)) return 0;
+    if (wm_alloc(c, mode) != MDFF_OK) return 0;
     dim3 gw(cdiv(c->Pk, 256), (c->Kx + 1) * CH);
-    if (mode == KM_F16X3)
+    if (wm_built) {
+        // columns written by k_sk_reduce_wm
+    } else if (mode == KM_F16X3)
         hipLaunchKernelGGL(k_wm_build_scaled<KM_F16X3>, dim3((c->Kx + 1) * CH), dim3(256), 0, c->st, c->P, c->Pk,
                            c->Ky, c->Kz, CH, c->W, c->wm[mode][0], c->wm[mode][1], c->gsc + 16, active);
     else if (mode == KM_F64)
@@ -767,6 +781,7 @@ static int launch_back_gemm(mdff_ctx *c, int polar, int lo, int NCH, const int *
 struct FieldFuse {
     double *ef;
     const double *mu, *base;
+    double *pq_part;   // optional: the CG step's p.q partial sums (k_sk_back_fin)
 };
 
 static void rec_part(mdff_ctx *c, int polar, int lo, int C, int NCH, const double *q_w, const double *mu_w,
@@ -796,10 +811,26 @@ static void rec_part(mdff_ctx *c, int polar, int lo, int C, int NCH, const doubl
     }
     KRedArgs r;
     r.Kx = c->Kx; r.Ky = c->Ky; r.Kz = c->Kz; r.P = c->P; r.nchunk = nchunk; r.C = C;
-    r.part = c->fwd_part ? c->ksum : c->kpart; r.kvec = c->kvec;
+    const bool gemm_fwd = c->fwd_part != 0;
+    r.part = gemm_fwd ? c->ksum : c->kpart; r.kvec = c->kvec;
     c->fwd_part = 0; r.Ak = c->Ak; r.kcoe = c->kcoe; r.W = c->W; r.ek = c->ek;
     r.do_stress = do_stress; r.active = active;
-    hipLaunchKernelGGL(k_sk_reduce, dim3(cdiv(c->nkf, 256)), dim3(256), 0, c->st, r);
+    // the back-projection's WM columns come out of the reduction for the
+    // unscaled stored-operand modes
+    const int bmode = lo ? c->mf_lo : KM_F64;
+    const int CHb = NCH == 9 ? 6 : 3;
+    bool wm_built = false;
+    if ((do_ef || do_f) && gemm_fwd && c->use_mfma && c->gemm && bmode != KM_F16X3 &&
+        (c->Kx + 1) * CHb * 2 <= MB_CT * 16 && wm_alloc(c, bmode) == MDFF_OK) {
+        void *w0 = c->wm[bmode][0], *w1 = c->wm[bmode][1];
+        const dim3 gr(cdiv(c->nkf, 256));
+        if (bmode == KM_F64) hipLaunchKernelGGL(k_sk_reduce_wm<KM_F64>, gr, dim3(256), 0, c->st, r, CHb, c->Pk, w0, w1);
+        else if (bmode == KM_F32) hipLaunchKernelGGL(k_sk_reduce_wm<KM_F32>, gr, dim3(256), 0, c->st, r, CHb, c->Pk, w0, w1);
+        else hipLaunchKernelGGL(k_sk_reduce_wm<KM_BF16X3>, gr, dim3(256), 0, c->st, r, CHb, c->Pk, w0, w1);
+        wm_built = true;
+    } else {
+        hipLaunchKernelGGL(k_sk_reduce, dim3(cdiv(c->nkf, 256)), dim3(256), 0, c->st, r);
+    }
     if (do_ef || do_f) {
         BackArgs b;
         b.nsel = nsel; b.Kx = c->Kx; b.Ky = c->Ky; b.Kz = c->Kz; b.P = c->P;
@@ -809,7 +840,7 @@ static void rec_part(mdff_ctx *c, int polar, int lo, int C, int NCH, const doubl
         b.part = c->bpart;
         b.active = active;
         int nsplit = c->nsplit;
-        if (!launch_back_mfma(c, polar, lo, NCH, active, &nsplit)) {
+        if (!launch_back_mfma(c, polar, lo, NCH, active, &nsplit, wm_built)) {
             if (NCH == 9) { if (mixed) launch_back_t<float, 9>(c, b, nsel); else launch_back_t<double, 9>(c, b, nsel); }
             else { if (mixed) launch_back_t<float, 3>(c, b, nsel); else launch_back_t<double, 3>(c, b, nsel); }
         }
@@ -824,6 +855,9 @@ static void rec_part(mdff_ctx *c, int polar, int lo, int C, int NCH, const doubl
         fa.ef_dir = c->ef_dir;
         fa.base = ff ? ff->base : nullptr;
         fa.selfa2 = c->selfa2;
+        fa.type = c->type;
+        fa.T = c->tab_d;
+        fa.pq_part = ff ? ff->pq_part : nullptr;
         hipLaunchKernelGGL(k_sk_back_fin, dim3(cdiv(nsel, 256)), dim3(256), 0, c->st, fa);
     }
 }
@@ -934,7 +968,10 @@ static int build_t2(mdff_ctx *c)
 
 // induced field of dipoles mu (no charges, no damping, pim.c:258-260) at
 // the polarisable ions: ef = base + self + dir + rec
-static void induced_field(mdff_ctx *c, int lo, const double *mu, const double *base, double *ef, const int *act)
+// with pq_part, the back-projection also writes the partial sums of
+// mu . (pol^-T mu - ef) per 256 polarisable ions and true is returned
+static bool induced_field(mdff_ctx *c, int lo, const double *mu, const double *base, double *ef, const int *act,
+                          double *pq_part = nullptr)
 {
     int n = c->n;
     if (c->t2_ok) {
@@ -946,15 +983,16 @@ static void induced_field(mdff_ctx *c, int lo, const double *mu, const double *b
             hipLaunchKernelGGL((k_t2_field<double, 16>), dim3(blocks), dim3(256), 0, c->st, n, c->capP, c->npol,
                                c->pinv, c->cntp, c->idxp, c->t2, mu, c->ef_dir, act);
         // the total field at the polarisable ions comes out of the back-projection
-        FieldFuse ff{ef, mu, base};
+        FieldFuse ff{ef, mu, base, pq_part};
         rec_part(c, 1, lo, 3, 3, c->qz, mu, c->qz, mu, 1, 0, 0, nullptr, act, &ff);
-        return;
+        return pq_part != nullptr;
     }
     DirArgs a = dir_args(c, c->qz, mu, 0, 1, act);
     launch_dir(c, a, lo, 1, 0, 0, 0);
     rec_part(c, 1, lo, 3, 3, c->qz, mu, c->qz, mu, 1, 0, 0, nullptr, act);
     hipLaunchKernelGGL(k_field_sum, dim3(cdiv(3 * n, 256)), dim3(256), 0, c->st, n, c->selfa2, mu, c->ef_dir,
                        c->ef_rec, base, ef, act);
+    return false;
 }
 
 static void scf_finish(mdff_ctx *c)
@@ -1123,6 +1161,10 @@ static int scf_loop_mixed(mdff_ctx *c, int istep)
 // the order of that precision times |T d|, far below the tolerance for a
 // predictor that is close.  The host launches as many steps as the last
 // SCF took before it polls.
+// k_sk_back_fin forms the CG step's p.q partial sums in its blocks of 256
+// polarisable ions, the blocks of the CG kernels
+static_assert(CG_BLOCK == 256, "CG partial-sum blocks");
+
 static int scf_loop_anchored(mdff_ctx *c, int istep)
 {
     int n = c->n, np = c->npol;
@@ -1164,10 +1206,11 @@ static int scf_loop_anchored(mdff_ctx *c, int istep)
     auto step = [&]() {
         int keep = c->mf_lo;
         c->mf_lo = c->mf_inner;
-        induced_field(c, 1, c->cg_p, nullptr, c->dc_g, act);
+        const bool fused = induced_field(c, 1, c->cg_p, nullptr, c->dc_g, act, pq);
         c->mf_lo = keep;
-        hipLaunchKernelGGL(k_cg_pq, dim3(nb), dim3(CG_BLOCK), 0, c->st, n, np, c->pidx, c->type, c->tab_d, c->cg_p,
-                           c->dc_g, pq, act);
+        if (!fused)
+            hipLaunchKernelGGL(k_cg_pq, dim3(nb), dim3(CG_BLOCK), 0, c->st, n, np, c->pidx, c->type, c->tab_d,
+                               c->cg_p, c->dc_g, pq, act);
         if (pc) {
             hipLaunchKernelGGL(k_cga_update_pc, dim3(nb), dim3(CG_BLOCK), 0, c->st, n, np, nb, c->pidx, c->type,
                                c->tab_d, c->cg_p, c->dc_g, c->mu_ind, c->ef, c->dc_R, c->cg_w, rz[cur], pq, rm, act);

@@ -72,6 +72,30 @@ __device__ __forceinline__ void g_store(void *hi, void *lo, size_t i, double x)
     }
 }
 
+// store the pair (x0, x1) at elements i, i+1 (i even) of a mode's operand
+// arrays, one store per array
+template <int MODE>
+__device__ __forceinline__ void g_store2(void *hi, void *lo, size_t i, double x0, double x1)
+{
+    if constexpr (MODE == KM_F64) {
+        typedef double d2 __attribute__((ext_vector_type(2)));
+        *(d2 *)((double *)hi + i) = d2{x0, x1};
+    } else if constexpr (MODE == KM_F32) {
+        typedef float f2 __attribute__((ext_vector_type(2)));
+        *(f2 *)((float *)hi + i) = f2{(float)x0, (float)x1};
+    } else if constexpr (MODE == KM_F16X3) {
+        typedef _Float16 h2 __attribute__((ext_vector_type(2)));
+        const _Float16 a = (_Float16)x0, b = (_Float16)x1;
+        *(h2 *)((_Float16 *)hi + i) = h2{a, b};
+        *(h2 *)((_Float16 *)lo + i) = h2{(_Float16)(x0 - (double)a), (_Float16)(x1 - (double)b)};
+    } else {
+        typedef __bf16 b2 __attribute__((ext_vector_type(2)));
+        const __bf16 a = (__bf16)(float)x0, b = (__bf16)(float)x1;
+        *(b2 *)((__bf16 *)hi + i) = b2{a, b};
+        *(b2 *)((__bf16 *)lo + i) = b2{(__bf16)(float)(x0 - (double)(float)a), (__bf16)(float)(x1 - (double)(float)b)};
+    }
+}
+
 // YZ[p][2j + ri] and YZt[j][2p + ri] = (Re, Im)(ey_j[iy] ez_j[iz]),
 // p = iy NZ + iz, from one 64-ion x 32-point tile per block: the products go
 // through LDS so that both layouts are written in contiguous rows.  Zero
@@ -104,8 +128,7 @@ __global__ void __launch_bounds__(256) k_yz_build(int nsel, int nselp, int P, in
         const int p = p0 + pp;
         if (p >= Pp_rows || j >= nselp) continue;
         const size_t o = (size_t)p * 2 * nselp + 2 * (size_t)j;
-        g_store<MODE>(yz, yz_lo, o, sr[pp][tj]);
-        g_store<MODE>(yz, yz_lo, o + 1, si[pp][tj]);
+        g_store2<MODE>(yz, yz_lo, o, sr[pp][tj], si[pp][tj]);
     }
     // ion-major rows: 32 lanes along points, two ions per wavefront pass
     const int tp = threadIdx.x & 31, tr = threadIdx.x >> 5;
@@ -113,8 +136,7 @@ __global__ void __launch_bounds__(256) k_yz_build(int nsel, int nselp, int P, in
         const int jr = j0 + jj, p = p0 + tp;
         if (jr >= nselp || p >= Pk) continue;
         const size_t o = (size_t)jr * 2 * Pk + 2 * (size_t)p;
-        g_store<MODE>(yzt, yzt_lo, o, sr[tp][jj]);
-        g_store<MODE>(yzt, yzt_lo, o + 1, si[tp][jj]);
+        g_store2<MODE>(yzt, yzt_lo, o, sr[tp][jj], si[tp][jj]);
     }
 }
 
@@ -149,10 +171,8 @@ __global__ void k_aw_build(int nsel, int nselp, int C, const TT *__restrict__ tx
             if (wsc) w *= wsc[c];
         }
         const size_t r0 = (size_t)(nx * C + c) * 2;
-        g_store<MODE>(aw, aw_lo, r0 * ld + 2 * (size_t)j, w * xr);
-        g_store<MODE>(aw, aw_lo, r0 * ld + 2 * (size_t)j + 1, -w * xi);
-        g_store<MODE>(aw, aw_lo, (r0 + 1) * ld + 2 * (size_t)j, w * xi);
-        g_store<MODE>(aw, aw_lo, (r0 + 1) * ld + 2 * (size_t)j + 1, w * xr);
+        g_store2<MODE>(aw, aw_lo, r0 * ld + 2 * (size_t)j, w * xr, -w * xi);
+        g_store2<MODE>(aw, aw_lo, (r0 + 1) * ld + 2 * (size_t)j, w * xi, w * xr);
     }
 }
 
@@ -178,10 +198,33 @@ __global__ void k_wm_build(int P, int Pp, int Ky, int Kz, int CH, const double *
     }
     const size_t ld = 2 * (size_t)Pp;
     const size_t c0 = (size_t)(nx * CH + m) * 2;
-    g_store<MODE>(wm, wm_lo, c0 * ld + 2 * (size_t)p, wr);
-    g_store<MODE>(wm, wm_lo, c0 * ld + 2 * (size_t)p + 1, -wi);
-    g_store<MODE>(wm, wm_lo, (c0 + 1) * ld + 2 * (size_t)p, wi);
-    g_store<MODE>(wm, wm_lo, (c0 + 1) * ld + 2 * (size_t)p + 1, wr);
+    g_store2<MODE>(wm, wm_lo, c0 * ld + 2 * (size_t)p, wr, -wi);
+    g_store2<MODE>(wm, wm_lo, (c0 + 1) * ld + 2 * (size_t)p, wi, wr);
+}
+
+// k_sk_reduce and the unscaled WM columns of k_wm_build in one pass, one
+// thread per k point (nx, p): W(k), the energy and stress terms, then the
+// columns (nx, m, re/im) at point p for the CH moments
+template <int MODE>
+__global__ void k_sk_reduce_wm(KRedArgs a, int CH, int Pk, void *wm, void *wm_lo)
+{
+    if (a.active && !*a.active) return;
+    const size_t nkf = (size_t)(a.Kx + 1) * a.P;
+    const size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (k >= nkf) return;
+    double Wr, Wi;
+    sk_reduce_one(a, k, Wr, Wi);
+    const int nx = (int)(k / a.P), p = (int)(k % a.P);
+    const int NZ = 2 * a.Kz + 1;
+    const int iy = p / NZ, iz = p % NZ;
+    const size_t ld = 2 * (size_t)Pk;
+    for (int m = 0; m < CH; m++) {
+        const double mom = mb_moment(m, iy - a.Ky, iz - a.Kz);
+        const double wr = Wr * mom, wi = Wi * mom;
+        const size_t c0 = (size_t)(nx * CH + m) * 2;
+        g_store2<MODE>(wm, wm_lo, c0 * ld + 2 * (size_t)p, wr, -wi);
+        g_store2<MODE>(wm, wm_lo, (c0 + 1) * ld + 2 * (size_t)p, wi, wr);
+    }
 }
 
 // power-of-two scale of each weight channel (q, mu components) of the
@@ -258,10 +301,8 @@ __global__ void __launch_bounds__(256) k_wm_build_scaled(int P, int Pp, int Ky, 
             wr = W[2 * kf] * mom;
             wi = W[2 * kf + 1] * mom;
         }
-        g_store<MODE>(wm, wm_lo, c0 * ld + 2 * (size_t)p, wr);
-        g_store<MODE>(wm, wm_lo, c0 * ld + 2 * (size_t)p + 1, -wi);
-        g_store<MODE>(wm, wm_lo, (c0 + 1) * ld + 2 * (size_t)p, wi);
-        g_store<MODE>(wm, wm_lo, (c0 + 1) * ld + 2 * (size_t)p + 1, wr);
+        g_store2<MODE>(wm, wm_lo, c0 * ld + 2 * (size_t)p, wr, -wi);
+        g_store2<MODE>(wm, wm_lo, (c0 + 1) * ld + 2 * (size_t)p, wi, wr);
     }
 }
 

@@ -576,16 +576,14 @@ struct KRedArgs {
 
 // rho(k) (kspace.c:189-302) from the channel sums, str = |rho|^2 Ak and the
 // stress terms of multipole_ES_rec ewald.c:949-1006; W = Ak conj(rho).
-__global__ void k_sk_reduce(KRedArgs a)
+__device__ __forceinline__ void sk_reduce_one(const KRedArgs &a, size_t k, double &Wr, double &Wi)
 {
-    if (a.active && !*a.active) return;
     size_t nkf = (size_t)(a.Kx + 1) * a.P;
-    size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
-    if (k >= nkf) return;
     double Ak = a.Ak[k];
     if (Ak == 0.0) {
         a.W[2 * k] = 0.0; a.W[2 * k + 1] = 0.0;
         for (int t = 0; t < 7; t++) a.ek[t * nkf + k] = 0.0;
+        Wr = Wi = 0.0;
         return;
     }
     double S[4][2] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
@@ -605,8 +603,10 @@ __global__ void k_sk_reduce(KRedArgs a)
         rI += kr;
     }
     double str = (rR * rR + rI * rI) * Ak;
-    a.W[2 * k] = Ak * rR;
-    a.W[2 * k + 1] = -Ak * rI;
+    Wr = Ak * rR;
+    Wi = -Ak * rI;
+    a.W[2 * k] = Wr;
+    a.W[2 * k + 1] = Wi;
     double *e = a.ek;
     e[k] = str;
     if (a.do_stress) {
@@ -620,6 +620,16 @@ __global__ void k_sk_reduce(KRedArgs a)
     } else {
         for (int t = 1; t < 7; t++) e[t * nkf + k] = 0.0;
     }
+}
+
+__global__ void k_sk_reduce(KRedArgs a)
+{
+    if (a.active && !*a.active) return;
+    size_t nkf = (size_t)(a.Kx + 1) * a.P;
+    size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (k >= nkf) return;
+    double Wr, Wi;
+    sk_reduce_one(a, k, Wr, Wi);
 }
 
 struct BackArgs {
@@ -721,15 +731,18 @@ struct BackFinArgs {
     double *ef_total;
     const double *mu_self, *ef_dir, *base;
     double selfa2;
+    // with pq_part (and ef_total): per-block partial sums over the selected
+    // ions of mu_self . (pol^-T mu_self - ef_total), the p.q of a
+    // conjugate-gradient step (blocks of 256 selected ions)
+    const int *type;
+    const TypeTab *T;
+    double *pq_part;
 };
 
 // sum the splits, map n-moments to k-moments and write field, gradient and
 // force of multipole_ES_rec ewald.c:957-992 (times 8 pi / V, :1011-1034)
-__global__ void k_sk_back_fin(BackFinArgs a)
+__device__ __forceinline__ double back_fin_one(const BackFinArgs &a, int t)
 {
-    if (a.active && !*a.active) return;
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= a.nsel) return;
     int i = a.idx ? a.idx[t] : t;
     double H[3] = {0, 0, 0}, M[6] = {0, 0, 0, 0, 0, 0};
     for (int s = 0; s < a.nsplit; s++) {
@@ -752,12 +765,20 @@ __global__ void k_sk_back_fin(BackFinArgs a)
             G[aa][bb] = MDFF_TPI * MDFF_TPI * sum;
         }
     const int n = a.n;
+    double pq = 0.0;
     if (a.ef_total) {
+        double v[3];
         for (int c = 0; c < 3; c++) {
             const size_t o = (size_t)c * n + i;
-            double v = 2.0 * a.selfa2 * a.mu_self[o] + a.ef_dir[o] + a.tfpiV * E[c];
-            if (a.base) v += a.base[o];
-            a.ef_total[o] = v;
+            v[c] = 2.0 * a.selfa2 * a.mu_self[o] + a.ef_dir[o] + a.tfpiV * E[c];
+            if (a.base) v[c] += a.base[o];
+            a.ef_total[o] = v[c];
+        }
+        if (a.pq_part) {
+            const double *I = a.T->invpol[a.type[i]];
+            const double p0 = a.mu_self[i], p1 = a.mu_self[n + i], p2 = a.mu_self[2 * n + i];
+            const double pv[3] = {p0, p1, p2};
+            for (int c = 0; c < 3; c++) pq += pv[c] * (I[c] * p0 + I[3 + c] * p1 + I[6 + c] * p2 - v[c]);
         }
     }
     if (a.do_ef) {
@@ -783,6 +804,23 @@ __global__ void k_sk_back_fin(BackFinArgs a)
         a.f[n + i] = a.tfpiV * (qi * E[1] + (a.NCH == 9 ? G[1][0] * m0 + G[1][1] * m1 + G[1][2] * m2 : 0.0));
         a.f[2 * n + i] = a.tfpiV * (qi * E[2] + (a.NCH == 9 ? G[2][0] * m0 + G[2][1] * m1 + G[2][2] * m2 : 0.0));
     }
+    return pq;
+}
+
+__global__ void __launch_bounds__(256) k_sk_back_fin(BackFinArgs a)
+{
+    if (a.active && !*a.active) return;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const double pq = t < a.nsel ? back_fin_one(a, t) : 0.0;
+    if (!a.pq_part) return;
+    __shared__ double sh[256];
+    sh[threadIdx.x] = pq;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) a.pq_part[blockIdx.x] = sh[0];
 }
 
 // ----------------------------------------------------------- elementwise
