@@ -58,6 +58,8 @@ struct mdff_ctx {
     CellD cd;
     int n = 0, ntype = 0, precision = MDFF_FP64, device = 0;
     hipStream_t st = nullptr;
+    hipStream_t st2 = nullptr;       // real-space kernels overlapping the k-space sums (default priority)
+    hipEvent_t xev[2] = {nullptr, nullptr};   // fork (st -> st2) and join (st2 -> st)
     // derived (field.c:186-335, md.c:120-141)
     int lqch = 0, ldip_static = 0, lpol = 0, ldmp = 0, lpim = 0, lnonbonded = 0;
     int ldip_sticky = 0;  // coulombic.c:115
@@ -196,6 +198,31 @@ static void colsum(mdff_ctx *c, const double *in, int n, int ld, int ncols, doub
 {
     hipLaunchKernelGGL(k_colsum, dim3(ncols), dim3(1024), 0, c->st, in, n, ld, out, active);
 }
+
+// Second stream: the real-space pair kernels (vector ALU) run beside the
+// k-space sums (matrix cores).  aux_fork makes st2 wait for everything
+// issued on st so far; aux_join makes st wait for everything on st2.
+// OnAux sends the launches of its scope (all helpers launch on c->st) to st2.
+static void aux_fork(mdff_ctx *c)
+{
+    if (!c->st2) return;
+    (void)hipEventRecord(c->xev[0], c->st);
+    (void)hipStreamWaitEvent(c->st2, c->xev[0], 0);
+}
+
+static void aux_join(mdff_ctx *c)
+{
+    if (!c->st2) return;
+    (void)hipEventRecord(c->xev[1], c->st2);
+    (void)hipStreamWaitEvent(c->st, c->xev[1], 0);
+}
+
+struct OnAux {
+    mdff_ctx *c;
+    hipStream_t keep;
+    explicit OnAux(mdff_ctx *cc) : c(cc), keep(cc->st) { if (cc->st2) cc->st = cc->st2; }
+    ~OnAux() { c->st = keep; }
+};
 
 // ------------------------------------------------------------- type tables
 static void build_tables(mdff_ctx *c)
@@ -890,10 +917,16 @@ static void short_range(mdff_ctx *c)
 static void static_field(mdff_ctx *c, int lo)
 {
     int n = c->n;
-    DirArgs a = dir_args(c, c->q, c->mu_static, c->ldmp, 0, nullptr);
-    launch_dir(c, a, lo, 1, 0, 0, 0);
+    // real space on the second stream, k space here
+    aux_fork(c);
+    {
+        OnAux g(c);
+        DirArgs a = dir_args(c, c->q, c->mu_static, c->ldmp, 0, nullptr);
+        launch_dir(c, a, lo, 1, 0, 0, 0);
+    }
     int C = c->ldip_static ? 4 : 1;
     rec_part(c, 0, lo, C, 3, c->q, c->mu_static, c->q, c->mu_static, 1, 0, 0, nullptr, nullptr);
+    aux_join(c);
     hipLaunchKernelGGL(k_field_sum, dim3(cdiv(3 * n, 256)), dim3(256), 0, c->st, n, c->selfa2, c->mu_static,
                        c->ef_dir, c->ef_rec, (const double *)nullptr, c->ef_stat, (const int *)nullptr);
 }
@@ -1337,9 +1370,15 @@ static void coulomb_force(mdff_ctx *c, int lo)
 {
     int n = c->n;
     int ld = c->ldip_sticky || c->ldip_static;
-    DirArgs a = dir_args(c, c->q, c->mu_tot, c->ldmp, 0, nullptr);
-    launch_dir(c, a, lo, 0, 0, 0, 1);
-    colsum(c, c->uw_dir, n, n, 7, c->scal + S_U_DIR);
+    // real space on the second stream (after mu_tot), k space here; joined
+    // before the forces are assembled
+    aux_fork(c);
+    {
+        OnAux g(c);
+        DirArgs a = dir_args(c, c->q, c->mu_tot, c->ldmp, 0, nullptr);
+        launch_dir(c, a, lo, 0, 0, 0, 1);
+        colsum(c, c->uw_dir, n, n, 7, c->scal + S_U_DIR);
+    }
     int C = ld ? 4 : 1;
     int NCH = ld ? 9 : 3;
     rec_part(c, 0, lo, C, NCH, c->q, c->mu_tot, c->q, c->mu_tot, 0, 1, 1, nullptr, nullptr);
@@ -1354,8 +1393,14 @@ static int engforce_impl(mdff_ctx *c, int istep)
     int rc = MDFF_OK;
     if (c->have_events) HIPCHK(hipEventRecord(c->ev[0], c->st));
     hipLaunchKernelGGL(k_frac, dim3(cdiv(n, 256)), dim3(256), 0, c->st, n, c->cd, c->r, c->r + n, c->r + 2 * n, c->s);
-    hipLaunchKernelGGL(k_zero, dim3(cdiv(3 * n, 256)), dim3(256), 0, c->st, c->f_sr, (size_t)3 * n);
-    short_range(c);
+    // the short-range pair forces on the second stream, beside the k-space
+    // tables and sums; joined before the forces are assembled
+    aux_fork(c);
+    {
+        OnAux g(c);
+        hipLaunchKernelGGL(k_zero, dim3(cdiv(3 * n, 256)), dim3(256), 0, c->st, c->f_sr, (size_t)3 * n);
+        short_range(c);
+    }
     if (c->have_events) HIPCHK(hipEventRecord(c->ev[1], c->st));
     c->last_nscf = 0;
     if (c->p.lcoulombic) {
@@ -1389,6 +1434,7 @@ static int engforce_impl(mdff_ctx *c, int istep)
             HIPCHK(hipEventRecord(c->ev[3], c->st));
         }
     }
+    aux_join(c);
     hipLaunchKernelGGL(k_assemble_force, dim3(cdiv(3 * n, 256)), dim3(256), 0, c->st, 3 * n, c->f_sr, c->f_dir,
                        c->f_rec, c->p.lcoulombic ? c->u.coul_unit : 0.0, c->f, c->ctl);
     c->n_force++;
@@ -1613,6 +1659,10 @@ void mdff_destroy(mdff_ctx *c)
     if (c->scal_h) (void)hipHostFree(c->scal_h);
     if (c->have_events)
         for (int i = 0; i < 8; i++) (void)hipEventDestroy(c->ev[i]);
+    if (c->st2) (void)hipStreamSynchronize(c->st2);
+    for (auto &e : c->xev)
+        if (e) (void)hipEventDestroy(e);
+    if (c->st2) (void)hipStreamDestroy(c->st2);
     if (c->st) (void)hipStreamDestroy(c->st);
     delete c;
 }
@@ -1722,6 +1772,12 @@ int mdff_create(mdff_ctx **out, const mdff_params *p, int nion, const int *typia
     int n = nion;
 #define ALLOC(ptr, cnt) do { if ((rc = dalloc(&(ptr), (cnt))) != MDFF_OK) { mdff_destroy(c); return rc; } } while (0)
     if (hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess) { delete c; return MDFF_ERR_DEVICE; }
+    if (hipStreamCreateWithFlags(&c->st2, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&c->xev[0], hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c->xev[1], hipEventDisableTiming) != hipSuccess) {
+        mdff_destroy(c);
+        return MDFF_ERR_DEVICE;
+    }
     ALLOC(c->r, 3 * n); ALLOC(c->v, 3 * n); ALLOC(c->f, 3 * n); ALLOC(c->fold, 3 * n); ALLOC(c->rs, 3 * n);
     ALLOC(c->s, 3 * n); ALLOC(c->s0, 3 * n); ALLOC(c->type, n); ALLOC(c->invmass, n); ALLOC(c->massd, n);
     ALLOC(c->q, n); ALLOC(c->qz, n); ALLOC(c->mu_static, 3 * n); ALLOC(c->mu_ind, 3 * n); ALLOC(c->mu_tot, 3 * n);
