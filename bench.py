@@ -423,7 +423,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32/f64" if head == "mixed" else "f64",
+        "dtype": "f16x3/bf16x3/f32/f64" if head == "mixed" else "f64",
         "data": "synthetic (seeded SiO2-like PIM melt, rigid-ion relaxed)",
         "config": {"workload": f"PIM multipole-Ewald NVT MD, N={n}, kmax=16 ({nk} k), alpha=0.3, rc=9 A, "
                                f"skin 1 A, SCF tol 1e-8, NHC-3 2500 K, dt 1 fs",
