@@ -132,7 +132,8 @@ def timed_run(p, typ, A, r, precision, steps, warmup):
     import mdff20_amd as m
     e = m.Engine(p, typ, A, r, precision=precision)
     if precision == m.MIXED:
-        e.set_kspace_modes(m.KS_F16X3, m.KS_BF16X3)
+        e.set_kspace_modes(VARIANT[MIXED_EVAL][1], VARIANT[MIXED_INNER][1])
+        e.set_scf_anchor(VARIANT[MIXED_ANCHOR][1])
     e.init_velocities()
     e.md_init()
     if warmup:
@@ -179,13 +180,14 @@ def cpu_baseline(p_text, L, pos, typ, n_scf, threads):
 
 # the S(k) GEMM kernels behind mdff_time_kernel's (which, lo): kernel-name
 # prefix (the template's first argument is the arithmetic mode: 0 FP64,
-# 1 split-BF16, 2 FP32)
+# 1 split-BF16, 2 FP32, 3 split-FP16)
 TIMED = {"sk_forward_scf": (0, "k_gemm_fwd"), "sk_back_scf": (1, "k_gemm_back"),
          "sk_forward_force": (3, "k_gemm_fwd"), "sk_back_force": (4, "k_gemm_back")}
 VARIANT = {"f64": (0, 0), "f32": (1, 2), "bf16x3": (2, 1), "f16x3": (3, 3)}   # name -> (lo, template mode)
 # arithmetic of the mixed form's matrix-core sums: static field and force
-# call (eval), low-precision SCF steps (inner)
-MIXED_EVAL, MIXED_INNER = "f16x3", "bf16x3"
+# call (eval), SCF steps (inner) and the anchor evaluation of the SCF
+# (booked with the inner evaluations when it is not FP64)
+MIXED_EVAL, MIXED_INNER, MIXED_ANCHOR = "f16x3", "f16x3", "f16x3"
 CALIB_BYTES = 512 << 20
 
 
@@ -213,7 +215,8 @@ def pmc_run(args, kname):
     A = np.diag([L, L, L])
     p = params_of(control(True))
     e = m.Engine(p, typ, A, r, precision=m.MIXED)
-    e.set_kspace_modes(m.KS_F16X3, m.KS_BF16X3)
+    e.set_kspace_modes(VARIANT[MIXED_EVAL][1], VARIANT[MIXED_INNER][1])
+    e.set_scf_anchor(VARIANT[MIXED_ANCHOR][1])
     e.engforce(0)
     # the runtime's own device-to-device copy kernel over CALIB_BYTES
     hip = ctypes.CDLL("libamdhip64.so")
@@ -423,7 +426,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f16x3/bf16x3/f32/f64" if head == "mixed" else "f64",
+        "dtype": "f16x3/f32/f64" if head == "mixed" else "f64",
         "data": "synthetic (seeded SiO2-like PIM melt, rigid-ion relaxed)",
         "config": {"workload": f"PIM multipole-Ewald NVT MD, N={n}, kmax=16 ({nk} k), alpha=0.3, rc=9 A, "
                                f"skin 1 A, SCF tol 1e-8, NHC-3 2500 K, dt 1 fs",

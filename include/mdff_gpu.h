@@ -135,10 +135,18 @@ int mdff_set_tuning(mdff_ctx *ctx, int lanes, int katoms, int ksplit);
  * products, about 16 bits), MDFF_KS_F16X3 (split FP16 with power-of-two
  * row scaling, three products, about 22 bits).  eval_mode applies to the
  * static field and the force call, inner_mode to the low-precision SCF
- * steps; 0 keeps a setting.  Defaults: MDFF_KS_F16X3, MDFF_KS_BF16X3.  No
+ * steps; 0 keeps a setting.  Defaults: MDFF_KS_F16X3 for both.  No
  * counterpart in the reference. */
 enum { MDFF_KS_BF16X3 = 1, MDFF_KS_F32 = 2, MDFF_KS_F16X3 = 3 };
 int mdff_set_kspace_modes(mdff_ctx *ctx, int eval_mode, int inner_mode);
+
+/* Anchored-CG SCF (solver 2): the anchor, the one field evaluation at the
+ * predicted dipoles that the CG steps carry along, in FP64 (anchor_mode 0)
+ * or over the FP32 tables in a matrix-core mode MDFF_KS_* (then the SCF
+ * makes no FP64 field evaluation; the counters book it as a low-precision
+ * one).  Default: MDFF_KS_F16X3.  Split-FP16 steps take the weight scales from the CG kernels that
+ * write the search direction.  No counterpart in the reference. */
+int mdff_set_scf_anchor(mdff_ctx *ctx, int anchor_mode);
 
 /* Shape of the matrix-core k-space sums over stored operands: column tiles
  * of 16 points per wavefront in the forward sum for the FP64 mode (cw64) and

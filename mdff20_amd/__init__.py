@@ -79,6 +79,7 @@ def lib():
         "mdff_set_tuning": (C.c_int, [vp, C.c_int, C.c_int, C.c_int]),
         "mdff_set_kspace_shape": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, C.c_int]),
         "mdff_set_kspace_modes": (C.c_int, [vp, C.c_int, C.c_int]),
+        "mdff_set_scf_anchor": (C.c_int, [vp, C.c_int]),
         "mdff_get_timing": (C.c_int, [vp, dp]),
         "mdff_time_kernel": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, dp]),
         "mdff_get_counters": (C.c_int, [vp, P(C.c_longlong)]),
@@ -305,6 +306,11 @@ class Engine:
         """mixed form's matrix-core arithmetic: KS_F32, KS_BF16X3 or KS_F16X3 for the static field and
         force call (eval_mode) and the low-precision SCF steps (inner_mode); 0 keeps"""
         _check(self._L.mdff_set_kspace_modes(self._h, eval_mode, inner_mode), "set_kspace_modes", False)
+
+    def set_scf_anchor(self, anchor_mode=0):
+        """anchored CG (SCF solver 2): the anchor field evaluation in FP64 (0) or in a matrix-core mode
+        (KS_F32, KS_BF16X3, KS_F16X3) over the FP32 tables"""
+        _check(self._L.mdff_set_scf_anchor(self._h, anchor_mode), "set_scf_anchor", False)
 
     def set_kspace_shape(self, cw64=0, cw32=0, fwd_blocks=0, back_blocks=0):
         _check(self._L.mdff_set_kspace_shape(self._h, cw64, cw32, fwd_blocks, back_blocks), "set_kspace_shape",

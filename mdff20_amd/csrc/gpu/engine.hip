@@ -134,11 +134,14 @@ struct mdff_ctx {
     int yz_ok[4][2] = {};     // stored operands match the current tables
     void *aw[4][2] = {};      // [mode][hi,lo]: rows (nx,c,re/im) of the forward sum
     void *wm[4][2] = {};      // [mode][hi,lo]: columns (nx,m,re/im) of the back sum
-    double *gsc = nullptr;    // split-FP16 scales: weights [0,8) and inverses [8,16), WM columns from 16
+    double *gsc = nullptr;    // split-FP16 scales: weight inverses [8,16), WM column inverses from 16
+    double *gwp = nullptr;    // split-FP16 per-block weight channel maxima [4][n/256]
     struct GTune { int mode, dir, tiles, polar, blocks; };
     std::vector<GTune> gtune;  // block targets of the stored-operand sums, timed on first use
     int mf_lo = KM_F16X3;     // matrix-core mode of the FP32-table evaluations (static field, forces)
-    int mf_inner = KM_BF16X3; // mode of the mixed-form low-precision SCF steps
+    int mf_inner = KM_F16X3;  // mode of the mixed-form low-precision SCF steps
+    int mf_anchor = KM_F16X3; // anchored CG: anchor evaluation in FP64 (0) or this mode over the FP32 tables
+    int wpart_ready = 0;      // gwp holds the channel maxima of the next split-FP16 forward sum's weights
     int mf_blocks = 512;      // target blocks of the MFMA kernels
     int nsplit_max = 1;       // (ny,nz) splits the bpart buffer holds
     // host pinned mirrors
@@ -665,24 +668,28 @@ static int launch_fwd_gemm(mdff_ctx *c, int polar, int lo, int C, const double *
     const int *idx = polar ? c->pidx : nullptr;
     dim3 ga(cdiv(nselp, 256), c->Kx + 1);
     const double *wsc = nullptr, *winv = nullptr;
+    const int nwb = cdiv(nsel, 256);
     if (mode == KM_F16X3) {
-        // weights scaled into [-1, 1] per channel
-        hipLaunchKernelGGL(k_w_scale, dim3(1), dim3(1024), 0, c->st, nsel, C, idx, q_w, mu_w, c->n, c->gsc, active);
-        wsc = c->gsc;
+        // weights scaled into [-1, 1] per channel (maxima already in gwp
+        // when the kernel that wrote the weights produced them)
+        if (!c->wpart_ready)
+            hipLaunchKernelGGL(k_w_max, dim3(nwb), dim3(256), 0, c->st, nsel, C, idx, q_w, mu_w, c->n, c->gwp,
+                               active);
+        wsc = c->gwp;
         winv = c->gsc + 8;
     }
     if (mode == KM_F64)
         hipLaunchKernelGGL((k_aw_build<KM_F64, double>), ga, dim3(256), 0, c->st, nsel, nselp, C, (const double *)t[0],
-                           idx, q_w, mu_w, c->n, c->aw[mode][0], c->aw[mode][1], wsc, active);
+                           idx, q_w, mu_w, c->n, c->aw[mode][0], c->aw[mode][1], wsc, nwb, c->gsc + 8, active);
     else if (mode == KM_F32)
         hipLaunchKernelGGL((k_aw_build<KM_F32, float>), ga, dim3(256), 0, c->st, nsel, nselp, C, (const float *)t[0],
-                           idx, q_w, mu_w, c->n, c->aw[mode][0], c->aw[mode][1], wsc, active);
+                           idx, q_w, mu_w, c->n, c->aw[mode][0], c->aw[mode][1], wsc, nwb, c->gsc + 8, active);
     else if (mode == KM_F16X3)
         hipLaunchKernelGGL((k_aw_build<KM_F16X3, float>), ga, dim3(256), 0, c->st, nsel, nselp, C,
-                           (const float *)t[0], idx, q_w, mu_w, c->n, c->aw[mode][0], c->aw[mode][1], wsc, active);
+                           (const float *)t[0], idx, q_w, mu_w, c->n, c->aw[mode][0], c->aw[mode][1], wsc, nwb, c->gsc + 8, active);
     else
         hipLaunchKernelGGL((k_aw_build<KM_BF16X3, float>), ga, dim3(256), 0, c->st, nsel, nselp, C,
-                           (const float *)t[0], idx, q_w, mu_w, c->n, c->aw[mode][0], c->aw[mode][1], wsc, active);
+                           (const float *)t[0], idx, q_w, mu_w, c->n, c->aw[mode][0], c->aw[mode][1], wsc, nwb, c->gsc + 8, active);
     GFwdArgs a;
     a.aw = c->aw[mode][0]; a.aw_lo = c->aw[mode][1];
     a.yz = c->yz[mode][polar][0]; a.yz_lo = c->yz[mode][polar][1];
@@ -831,7 +838,9 @@ static void rec_part(mdff_ctx *c, int polar, int lo, int C, int NCH, const doubl
     a.active = active;
     bool mixed = lo != 0;
     c->fwd_part = 0;
-    if (!launch_fwd_mfma(c, polar, lo, C, q_w, mu_w, active, &nchunk)) {
+    const bool fwd_mf = launch_fwd_mfma(c, polar, lo, C, q_w, mu_w, active, &nchunk);
+    c->wpart_ready = 0;
+    if (!fwd_mf) {
         if (C == 1) { if (mixed) launch_fwd_t<float, 1>(c, a, nchunk); else launch_fwd_t<double, 1>(c, a, nchunk); }
         else if (C == 3) { if (mixed) launch_fwd_t<float, 3>(c, a, nchunk); else launch_fwd_t<double, 3>(c, a, nchunk); }
         else { if (mixed) launch_fwd_t<float, 4>(c, a, nchunk); else launch_fwd_t<double, 4>(c, a, nchunk); }
@@ -964,7 +973,17 @@ static int aspc_args(mdff_ctx *c, int istep, AspcArgs *asp)
 // dipole tensors of all polarisable pairs at the current positions (per
 // force call, after k_frac), and for scfkO the same at KO_ALPHA; rows hold
 // the list capacity, so they never overflow
-static int build_t2(mdff_ctx *c)
+// whether the SCF evaluates the induced field in FP64: the FP64 form, scfkO
+// (FP64 in both forms), and the mixed form's solvers but anchored CG with a
+// low-precision anchor
+static bool scf_uses_fp64(const mdff_ctx *c)
+{
+    return !lo_of(c) || c->p.algo_pim == MDFF_PIM_SCFKO || !(c->scf_solver >= 2 && c->cg_ok && c->mf_anchor);
+}
+
+// T2 pair tensors of the polarisable ions; the FP64 copy only when the SCF
+// (or fp64, for kernel timing) needs it
+static int build_t2(mdff_ctx *c, bool fp64 = false)
 {
     c->t2_ok = 0;
     if (c->npol == 0) return MDFF_OK;
@@ -985,7 +1004,8 @@ static int build_t2(mdff_ctx *c)
     T2Args a;
     a.n = c->n; a.cap = c->cap; a.capP = c->capP; a.c = c->cd; a.s = c->s; a.nbr = c->nbr; a.cnt = c->cnt;
     a.pidx = c->pidx; a.pinv = c->pinv; a.alpha = c->alpha; a.rc2 = c->lrcutsq; a.npol = c->npol;
-    a.cntp = c->cntp; a.idxp = c->idxp; a.t2 = c->t2; a.t2f = lo_of(c) ? c->t2f : nullptr; a.overflow = c->t2_ovf;
+    a.cntp = c->cntp; a.idxp = c->idxp; a.t2f = lo_of(c) ? c->t2f : nullptr; a.overflow = c->t2_ovf;
+    a.t2 = (fp64 || !a.t2f || scf_uses_fp64(c)) ? c->t2 : nullptr;
     hipLaunchKernelGGL(k_t2_build, dim3(cdiv(c->npol, 4)), dim3(256), 0, c->st, a);
     if (ko) {
         // the same pairs at the alpha of induced_moment_inner_kO (pim.c:693-694);
@@ -1209,10 +1229,22 @@ static int scf_loop_anchored(mdff_ctx *c, int istep)
     if (aspc_args(c, istep, &asp)) return MDFF_ERR_CONFIG;
     hipLaunchKernelGGL(k_scf_mu, dim3(cdiv(n, 256)), dim3(256), 0, c->st, n, 1, asp, c->type, c->tab_d, c->hist, c->ef,
                        c->mu_ind, c->upol_i, act);
-    induced_field(c, 0, c->mu_ind, c->ef_stat, c->ef, act);
+    // the anchor: the field of the predicted dipoles in FP64, or with
+    // mf_anchor over the FP32 tables in that matrix-core mode
+    if (c->mf_anchor) {
+        const int keep = c->mf_lo;
+        c->mf_lo = c->mf_anchor;
+        induced_field(c, 1, c->mu_ind, c->ef_stat, c->ef, act);
+        c->mf_lo = keep;
+    } else {
+        induced_field(c, 0, c->mu_ind, c->ef_stat, c->ef, act);
+    }
     hipLaunchKernelGGL(k_scf_check, dim3(1), dim3(1024), 0, c->st, n, c->type, c->tab_d, c->mu_ind, c->ef,
                        (const double *)nullptr, ctl, c->scal, np, maxit, minit, c->p.conv_tol_ind);
     const int nb = cdiv(np, CG_BLOCK);
+    // split-FP16 steps: the kernels that write p also give the channel
+    // maxima of p that scale the forward sum's weights
+    double *wp = (c->mf_inner == KM_F16X3 && !(c->cg_precond && c->t2_ok && c->t2f != nullptr)) ? c->gwp : nullptr;
     double *rz[2] = {c->cg_part, c->cg_part + nb}, *pq = c->cg_part + 2 * nb, *rm = c->cg_part + 3 * nb;
     // preconditioner: z = pol^T r, or with c->cg_precond (and the FP32 pair
     // tensors) z = w + pol^T (T_dir w + 2 selfa2 w), w = pol^T r
@@ -1232,13 +1264,14 @@ static int scf_loop_anchored(mdff_ctx *c, int istep)
         precond(rz[0], 1);
     } else {
         hipLaunchKernelGGL(k_cg_init, dim3(nb), dim3(CG_BLOCK), 0, c->st, n, np, c->pidx, c->type, c->tab_d, c->ef,
-                           c->mu_ind, c->dc_R, c->cg_z, c->cg_p, c->dc_d, rz[0], act);
+                           c->mu_ind, c->dc_R, c->cg_z, c->cg_p, c->dc_d, rz[0], act, wp);
     }
     int cur = 0, launched = 1;
     const int limit = std::max(maxit, minit) + 1;
     auto step = [&]() {
         int keep = c->mf_lo;
         c->mf_lo = c->mf_inner;
+        c->wpart_ready = wp != nullptr;
         const bool fused = induced_field(c, 1, c->cg_p, nullptr, c->dc_g, act, pq);
         c->mf_lo = keep;
         if (!fused)
@@ -1254,7 +1287,7 @@ static int scf_loop_anchored(mdff_ctx *c, int istep)
                                rz[cur ^ 1], rm, act);
         }
         hipLaunchKernelGGL(k_cga_next, dim3(nb), dim3(CG_BLOCK), 0, c->st, n, np, nb, c->pidx, c->cg_z, c->cg_p,
-                           rz[cur], rz[cur ^ 1], rm, c->ctl, c->scal, maxit, minit, c->p.conv_tol_ind);
+                           rz[cur], rz[cur ^ 1], rm, c->ctl, c->scal, maxit, minit, c->p.conv_tol_ind, wp);
         cur ^= 1;
         launched++;
     };
@@ -1270,7 +1303,8 @@ static int scf_loop_anchored(mdff_ctx *c, int istep)
     hipLaunchKernelGGL(k_upol, dim3(cdiv(n, 256)), dim3(256), 0, c->st, n, c->type, c->tab_d, c->mu_ind, c->upol_i);
     colsum(c, c->upol_i, n, n, 1, c->scal + S_UPOL);
     scf_finish(c);
-    c->n_scf_fp64 += 1;
+    if (c->mf_anchor) c->n_scf_inner += 1;
+    else c->n_scf_fp64 += 1;
     c->n_scf_inner += std::max(0, c->last_nscf - 1);
     c->hint_cg = std::max(1, c->last_nscf - 1);
     if (c->last_rmsd > c->p.conv_tol_ind) return MDFF_ERR_SCF;
@@ -1408,7 +1442,7 @@ static int engforce_impl(mdff_ctx *c, int istep)
         int lo = lo_of(c);
         launch_eikr(c, 0, lo, nullptr);
         if (c->lpol && c->npol > 0) {
-            launch_eikr(c, 1, 0, nullptr);
+            if (scf_uses_fp64(c)) launch_eikr(c, 1, 0, nullptr);
             if (lo) launch_eikr(c, 1, 1, nullptr);
         }
         // get_dipoles coulombic.c:67-130
@@ -1628,7 +1662,7 @@ void mdff_destroy(mdff_ctx *c)
                      &c->mu_static, &c->mu_ind, &c->mu_tot, &c->hist, &c->ef_stat, &c->ef, &c->ef_dir, &c->ef_rec,
                      &c->efg, &c->efg_dir, &c->efg_rec, &c->f_sr, &c->f_dir, &c->f_rec, &c->acc_sr, &c->uw_dir,
                      &c->selfbuf, &c->ke, &c->upol_i, &c->rmsd_i, &c->tmp_q, &c->tmp_mu, &c->tmp_qraw, &c->tmp_mur, &c->dc_R, &c->dc_d, &c->dc_g, &c->cg_z, &c->cg_p, &c->cg_part, &c->ko_f, &c->ko_dmu, &c->t2ko, &c->cg_w, &c->scal,
-                     &c->kvec, &c->Ak, &c->kcoe, &c->W, &c->ek, &c->kpart, &c->bpart, &c->ksum, &c->gsc};
+                     &c->kvec, &c->Ak, &c->kcoe, &c->W, &c->ek, &c->kpart, &c->bpart, &c->ksum, &c->gsc, &c->gwp};
     for (auto pp : dp) dfree(*pp);
     dfree(c->type);
     dfree(c->ctl);
@@ -1869,6 +1903,7 @@ int mdff_create(mdff_ctx **out, const mdff_params *p, int nion, const int *typia
         ALLOC(c->kvec, 3 * c->nkf); ALLOC(c->Ak, c->nkf); ALLOC(c->kcoe, c->nkf); ALLOC(c->W, 2 * c->nkf);
         ALLOC(c->ek, 7 * c->nkf);
         ALLOC(c->gsc, 16 + 6 * (size_t)(c->Kx + 1));
+        ALLOC(c->gwp, 4 * (size_t)cdiv(c->n, 256));
         if (hipMemcpy(c->kvec, kv.data(), sizeof(double) * 3 * c->nkf, hipMemcpyHostToDevice) != hipSuccess ||
             hipMemcpy(c->Ak, ak.data(), sizeof(double) * c->nkf, hipMemcpyHostToDevice) != hipSuccess ||
             hipMemcpy(c->kcoe, kc.data(), sizeof(double) * c->nkf, hipMemcpyHostToDevice) != hipSuccess) {
@@ -1974,6 +2009,15 @@ int mdff_set_kspace_shape(mdff_ctx *c, int cw64, int cw32, int fwd_blocks, int b
     return MDFF_OK;
 }
 
+int mdff_set_scf_anchor(mdff_ctx *c, int anchor_mode)
+{
+    if (!c) return MDFF_ERR_ARG;
+    if (!(anchor_mode == 0 || anchor_mode == KM_BF16X3 || anchor_mode == KM_F32 || anchor_mode == KM_F16X3))
+        return MDFF_ERR_ARG;
+    c->mf_anchor = anchor_mode;
+    return MDFF_OK;
+}
+
 int mdff_set_kspace_modes(mdff_ctx *c, int eval_mode, int inner_mode)
 {
     if (!c) return MDFF_ERR_ARG;
@@ -1993,7 +2037,7 @@ int mdff_set_tuning(mdff_ctx *c, int lanes, int katoms, int ksplit)
     // sums over stored operands with split-BF16 SCF correction sweeps; 2 the
     // same with FP32 sweeps; 3 matrix cores with operands formed in LDS per
     // call (split-BF16 sweeps).
-    if (katoms == 1) { c->use_mfma = 1; c->gemm = 1; c->mf_inner = KM_BF16X3; }
+    if (katoms == 1) { c->use_mfma = 1; c->gemm = 1; c->mf_inner = KM_F16X3; }
     else if (katoms == 2) { c->use_mfma = 1; c->gemm = 1; c->mf_inner = KM_F32; }
     else if (katoms == 3) { c->use_mfma = 1; c->gemm = 0; c->mf_inner = KM_BF16X3; }
     else if (katoms == -1) c->use_mfma = 0;
@@ -2140,7 +2184,7 @@ int mdff_get_dipoles_scf(mdff_ctx *c, int istep, double *mu, double *upol)
     if (c->lpol) {
         int lo = lo_of(c);
         launch_eikr(c, 0, lo, nullptr);
-        launch_eikr(c, 1, 0, nullptr);
+        if (scf_uses_fp64(c)) launch_eikr(c, 1, 0, nullptr);
         if (lo) launch_eikr(c, 1, 1, nullptr);
         if ((rc = build_t2(c)) != MDFF_OK) return rc;
         static_field(c, lo);
@@ -2331,7 +2375,7 @@ int mdff_time_kernel(mdff_ctx *c, int which, int lo, int iters, double *ms)
     b.tx = a.tx; b.ty = a.ty; b.tz = a.tz; b.W = c->W; b.rows_per_split = c->rows_per_split; b.part = c->bpart;
     b.active = nullptr;
     DirArgs d = dir_args(c, polar ? c->qz : c->q, mu, polar ? 0 : c->ldmp, polar ? 1 : 0, nullptr);
-    if (which == 2 && build_t2(c) != MDFF_OK) return MDFF_ERR_NOMEM;
+    if (which == 2 && build_t2(c, true) != MDFF_OK) return MDFF_ERR_NOMEM;
     auto once = [&]() {
         int dummy = 0;
         switch (which) {

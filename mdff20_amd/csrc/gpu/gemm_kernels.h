@@ -62,9 +62,11 @@ __device__ __forceinline__ void g_store(void *hi, void *lo, size_t i, double x)
     } else if constexpr (MODE == KM_F32) {
         ((float *)hi)[i] = (float)x;
     } else if constexpr (MODE == KM_F16X3) {
-        _Float16 h = (_Float16)x;
+        // through FP32: a direct double-to-half conversion is a long
+        // software sequence; the low part absorbs any double rounding of hi
+        _Float16 h = (_Float16)(float)x;
         ((_Float16 *)hi)[i] = h;
-        ((_Float16 *)lo)[i] = (_Float16)(x - (double)h);
+        ((_Float16 *)lo)[i] = (_Float16)(float)(x - (double)h);
     } else {
         __bf16 h = (__bf16)(float)x;
         ((__bf16 *)hi)[i] = h;
@@ -84,10 +86,11 @@ __device__ __forceinline__ void g_store2(void *hi, void *lo, size_t i, double x0
         typedef float f2 __attribute__((ext_vector_type(2)));
         *(f2 *)((float *)hi + i) = f2{(float)x0, (float)x1};
     } else if constexpr (MODE == KM_F16X3) {
+        // through FP32 (see g_store)
         typedef _Float16 h2 __attribute__((ext_vector_type(2)));
-        const _Float16 a = (_Float16)x0, b = (_Float16)x1;
+        const _Float16 a = (_Float16)(float)x0, b = (_Float16)(float)x1;
         *(h2 *)((_Float16 *)hi + i) = h2{a, b};
-        *(h2 *)((_Float16 *)lo + i) = h2{(_Float16)(x0 - (double)a), (_Float16)(x1 - (double)b)};
+        *(h2 *)((_Float16 *)lo + i) = h2{(_Float16)(float)(x0 - (double)a), (_Float16)(float)(x1 - (double)b)};
     } else {
         typedef __bf16 b2 __attribute__((ext_vector_type(2)));
         const __bf16 a = (__bf16)(float)x0, b = (__bf16)(float)x1;
@@ -141,15 +144,35 @@ __global__ void __launch_bounds__(256) k_yz_build(int nsel, int nselp, int P, in
 }
 
 // AW rows (nx, c, re/im) over K = (j, ri): row (nx,c,0) = (w exr, -w exi),
-// row (nx,c,1) = (w exi, w exr), w times wsc[c] when wsc is given.
-// grid (nselp/256, Kx+1).  Rows >= M are never written (zeroed at
-// allocation).
+// row (nx,c,1) = (w exi, w exr).  With the per-block channel maxima wpart
+// of k_w_max (nwb blocks), w is scaled by a power of two 2^-e into [-1, 1]
+// per channel, and block (0, 0) writes 2^e to winv[c].  grid (nselp/256,
+// Kx+1), 256 threads (one wave per channel, C <= 4).  Rows >= M are never
+// written (zeroed at allocation).
 template <int MODE, typename TT>
-__global__ void k_aw_build(int nsel, int nselp, int C, const TT *__restrict__ tx, const int *__restrict__ idx,
-                           const double *__restrict__ q, const double *__restrict__ mu, int n, void *aw, void *aw_lo,
-                           const double *__restrict__ wsc, const int *active)
+__global__ void __launch_bounds__(256) k_aw_build(int nsel, int nselp, int C, const TT *__restrict__ tx,
+                                                  const int *__restrict__ idx, const double *__restrict__ q,
+                                                  const double *__restrict__ mu, int n, void *aw, void *aw_lo,
+                                                  const double *__restrict__ wpart, int nwb, double *winv,
+                                                  const int *active)
 {
     if (active && !*active) return;
+    __shared__ double wsc[4];
+    if (wpart) {
+        const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+        if (wv < C) {
+            double m = 0.0;
+            for (int l = lane; l < nwb; l += 64) m = fmax(m, wpart[(size_t)wv * nwb + l]);
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o));
+            if (lane == 0) {
+                const int e = m > 0.0 ? ilogb(m) + 1 : 0;
+                wsc[wv] = ldexp(1.0, -e);
+                if (blockIdx.x == 0 && blockIdx.y == 0) winv[wv] = ldexp(1.0, e);
+            }
+        }
+        __syncthreads();
+    }
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     const int nx = blockIdx.y;
     if (j >= nselp) return;
@@ -168,7 +191,7 @@ __global__ void k_aw_build(int nsel, int nselp, int C, const TT *__restrict__ tx
             if (C == 1) w = q[ion];
             else if (C == 3) w = mu[(size_t)c * n + ion];
             else w = c == 0 ? q[ion] : mu[(size_t)(c - 1) * n + ion];
-            if (wsc) w *= wsc[c];
+            if (wpart) w *= wsc[c];
         }
         const size_t r0 = (size_t)(nx * C + c) * 2;
         g_store2<MODE>(aw, aw_lo, r0 * ld + 2 * (size_t)j, w * xr, -w * xi);
@@ -227,38 +250,33 @@ __global__ void k_sk_reduce_wm(KRedArgs a, int CH, int Pk, void *wm, void *wm_lo
     }
 }
 
-// power-of-two scale of each weight channel (q, mu components) of the
-// selected ions so that |w| <= 1: sc[c] = 2^-e, sc[8 + c] = 2^e.  One block.
-__global__ void __launch_bounds__(1024) k_w_scale(int nsel, int C, const int *__restrict__ idx,
-                                                  const double *__restrict__ q, const double *__restrict__ mu, int n,
-                                                  double *sc, const int *active)
+// largest |w| of each weight channel (q, mu components) over the block's
+// 256 selected ions: wpart[c nb + block], nb = gridDim.x.  Non-finite
+// weights count as zero.
+__global__ void __launch_bounds__(256) k_w_max(int nsel, int C, const int *__restrict__ idx,
+                                               const double *__restrict__ q, const double *__restrict__ mu, int n,
+                                               double *__restrict__ wpart, const int *active)
 {
     if (active && !*active) return;
-    __shared__ double red[1024];
+    __shared__ double red[4][256];
+    const int j = blockIdx.x * 256 + threadIdx.x;
+    const int ion = (j < nsel) ? (idx ? idx[j] : j) : 0;
     for (int c = 0; c < C; c++) {
-        double mx = 0.0;
-        for (int j = threadIdx.x; j < nsel; j += 1024) {
-            const int ion = idx ? idx[j] : j;
-            double w;
+        double w = 0.0;
+        if (j < nsel) {
             if (C == 1) w = q[ion];
             else if (C == 3) w = mu[(size_t)c * n + ion];
             else w = c == 0 ? q[ion] : mu[(size_t)(c - 1) * n + ion];
-            mx = fmax(mx, fabs(w));
         }
-        red[threadIdx.x] = mx;
-        __syncthreads();
-        for (int o = 512; o > 0; o >>= 1) {
-            if ((int)threadIdx.x < o) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + o]);
-            __syncthreads();
-        }
-        if (threadIdx.x == 0) {
-            const double m = red[0];
-            const int e = (m > 0.0 && isfinite(m)) ? ilogb(m) + 1 : 0;
-            sc[c] = ldexp(1.0, -e);
-            sc[8 + c] = ldexp(1.0, e);
-        }
+        red[c][threadIdx.x] = isfinite(w) ? fabs(w) : 0.0;
+    }
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o)
+            for (int c = 0; c < C; c++) red[c][threadIdx.x] = fmax(red[c][threadIdx.x], red[c][threadIdx.x + o]);
         __syncthreads();
     }
+    if ((int)threadIdx.x < C) wpart[(size_t)threadIdx.x * gridDim.x + blockIdx.x] = red[threadIdx.x][0];
 }
 
 // WM columns as k_wm_build, each (nx, m) pair scaled by a power of two

@@ -9,17 +9,27 @@ __global__ void __launch_bounds__(1024) k_colsum(const double *__restrict__ in, 
                                                  double *__restrict__ out, const int *active)
 {
     if (active && !*active) return;
-    __shared__ double sh[1024];
+    __shared__ double sh[16];
     const double *col = in + (size_t)blockIdx.x * ld;
-    double s = 0.0;
-    for (int i = threadIdx.x; i < n; i += 1024) s += col[i];
-    sh[threadIdx.x] = s;
-    __syncthreads();
-    for (int o = 512; o > 0; o >>= 1) {
-        if (threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
-        __syncthreads();
+    // two independent accumulators, then wave and block sums in a fixed order
+    double s0 = 0.0, s1 = 0.0;
+    int i = threadIdx.x;
+    for (; i + 1024 < n; i += 2048) {
+        s0 += col[i];
+        s1 += col[i + 1024];
     }
-    if (threadIdx.x == 0) out[blockIdx.x] = sh[0];
+    if (i < n) s0 += col[i];
+    double s = s0 + s1;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x < 64) {
+        double t = threadIdx.x < 16 ? sh[threadIdx.x] : 0.0;
+#pragma unroll
+        for (int o = 8; o > 0; o >>= 1) t += __shfl_xor(t, o);
+        if (threadIdx.x == 0) out[blockIdx.x] = t;
+    }
 }
 
 // ------------------------------------------------------------ fractional coords
@@ -930,8 +940,10 @@ __global__ void __launch_bounds__(256) k_t2_build(T2Args a)
         if (keep && pos < a.capP) {
             const size_t o = (size_t)pr * a.capP + pos;
             a.idxp[o] = j;
+            if (a.t2) {
 #pragma unroll
-            for (int k = 0; k < 6; k++) a.t2[k * plane + o] = T[k];
+                for (int k = 0; k < 6; k++) a.t2[k * plane + o] = T[k];
+            }
             if (a.t2f) {
 #pragma unroll
                 for (int k = 0; k < 6; k++) a.t2f[k * plane + o] = (float)T[k];

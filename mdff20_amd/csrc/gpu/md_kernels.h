@@ -424,15 +424,39 @@ __global__ void k_upol(int n, const int *__restrict__ type, const TypeTab *__res
 
 __device__ __forceinline__ double cg_block_sum(double v, double *sh)
 {
-    sh[threadIdx.x] = v;
+    // wave sums, then the four wave sums in order (fixed order throughout)
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
     __syncthreads();
-    for (int o = CG_BLOCK / 2; o > 0; o >>= 1) {
-        if ((int)threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
-        __syncthreads();
-    }
-    double r = sh[0];
+    double r = 0.0;
+#pragma unroll
+    for (int w = 0; w < CG_BLOCK / 64; w++) r += sh[w];
     __syncthreads();
     return r;
+}
+
+// the block's largest |v[a]| of each of three channels into
+// wpart[a nb + block] (non-finite values count as zero): the channel maxima
+// that scale the weights of a split-FP16 k-space sum over these dipoles
+__device__ __forceinline__ void cg_block_max3(const double v[3], double *sh, double *wpart, int nb)
+{
+    double m[3];
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        m[a] = isfinite(v[a]) ? fabs(v[a]) : 0.0;
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) m[a] = fmax(m[a], __shfl_xor(m[a], o));
+    }
+    if ((threadIdx.x & 63) == 0)
+        for (int a = 0; a < 3; a++) sh[(threadIdx.x >> 6) * 3 + a] = m[a];
+    __syncthreads();
+    if (threadIdx.x < 3) {
+        double r = 0.0;
+        for (int w = 0; w < CG_BLOCK / 64; w++) r = fmax(r, sh[w * 3 + threadIdx.x]);
+        wpart[(size_t)threadIdx.x * nb + blockIdx.x] = r;
+    }
+    __syncthreads();
 }
 
 __device__ __forceinline__ double cg_sum_partials(const double *part, int nb, double *sh)
@@ -442,17 +466,19 @@ __device__ __forceinline__ double cg_sum_partials(const double *part, int nb, do
     return cg_block_sum(v, sh);
 }
 
-// r = E - pol^-T mu, z = p = pol^T r (= pol^T E - mu), d = 0; partials of r.z
+// r = E - pol^-T mu, z = p = pol^T r (= pol^T E - mu), d = 0; partials of
+// r.z; with wpart, the block's channel maxima of |p| (cg_block_max3)
 __global__ void __launch_bounds__(CG_BLOCK) k_cg_init(int n, int npol, const int *__restrict__ pidx,
                                                       const int *__restrict__ type, const TypeTab *__restrict__ T,
                                                       const double *__restrict__ ef, const double *__restrict__ mu,
                                                       double *r, double *z, double *p, double *d, double *rz_part,
-                                                      const int *active)
+                                                      const int *active, double *wpart = nullptr)
 {
     if (!*active) return;
     __shared__ double sh[CG_BLOCK];
     const int k = blockIdx.x * CG_BLOCK + threadIdx.x;
     double acc = 0.0;
+    double pv[3] = {0.0, 0.0, 0.0};
     if (k < npol) {
         const int i = pidx[k], t = type[i];
         const double *P = T->pol[t], *I = T->invpol[t];
@@ -464,10 +490,12 @@ __global__ void __launch_bounds__(CG_BLOCK) k_cg_init(int n, int npol, const int
             const size_t o = (size_t)a * n + i;
             r[o] = rr[a]; z[o] = zz[a]; p[o] = zz[a]; d[o] = 0.0;
             acc += rr[a] * zz[a];
+            pv[a] = zz[a];
         }
     }
     double s = cg_block_sum(acc, sh);
     if (threadIdx.x == 0) rz_part[blockIdx.x] = s;
+    if (wpart) cg_block_max3(pv, sh, wpart, gridDim.x);
 }
 
 // partials of p.q, q = pol^-T p - Tp (Tp: field of the dipoles p)
@@ -626,7 +654,7 @@ __global__ void __launch_bounds__(CG_BLOCK) k_cga_next(int n, int npol, int nb, 
                                                        const double *__restrict__ rz_old,
                                                        const double *__restrict__ rz_new,
                                                        const double *__restrict__ rm_part, int *ctl, double *scal,
-                                                       int maxit, int minit, double tol)
+                                                       int maxit, int minit, double tol, double *wpart = nullptr)
 {
     if (!ctl[C_SCF_ACTIVE]) return;
     __shared__ double sh[CG_BLOCK];
@@ -635,13 +663,16 @@ __global__ void __launch_bounds__(CG_BLOCK) k_cga_next(int n, int npol, int nb, 
     const double rm = cg_sum_partials(rm_part, nb, sh);
     const double b = (ro != 0.0) ? rn / ro : 0.0;
     const int k = blockIdx.x * CG_BLOCK + threadIdx.x;
+    double pv[3] = {0.0, 0.0, 0.0};
     if (k < npol) {
         const int i = pidx[k];
         for (int a = 0; a < 3; a++) {
             const size_t o = (size_t)a * n + i;
-            p[o] = z[o] + b * p[o];
+            pv[a] = z[o] + b * p[o];
+            p[o] = pv[a];
         }
     }
+    if (wpart) cg_block_max3(pv, sh, wpart, nb);
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         scal[S_RMSD2] = rm;
         int iscf = ctl[C_ISCF] + 1;

@@ -122,9 +122,10 @@ def test_mixed_form_split_fp16(case, modes):
 def test_mixed_scf_solvers_reach_the_fp64_fixed_point(case):
     """the three mixed-form SCF solvers: defect correction by Jacobi sweeps
     and by preconditioned CG (both stop on an FP64 residual) and anchored CG
-    (one FP64 field evaluation, the field carried along the CG steps): all
-    reach the FP64 form's dipoles and forces to the SCF tolerance, CG needs
-    no more FP64 evaluations than Jacobi, and the anchored solve needs one"""
+    (one split-FP16 anchor evaluation by default, the field carried along
+    the CG steps): all reach the FP64 form's dipoles and forces to the SCF
+    tolerance, CG needs no more FP64 evaluations than Jacobi, and the
+    anchored solve needs none"""
     print(f"[gpu] mixed solvers {case}", flush=True)
     p, cfg, _ = load_case(case)
     p.conv_tol_ind = 1e-10
@@ -149,9 +150,39 @@ def test_mixed_scf_solvers_reach_the_fp64_fixed_point(case):
         assert rel(ec.dipoles(), ej.dipoles()) < 1e-5
         assert rel(ea.dipoles(), ec.dipoles()) < 1e-5
         assert ec.counters()["scf_fp64"] <= ej.counters()["scf_fp64"]
-        # the anchored solves evaluate the FP64 field once (when there is an SCF)
+        # the anchored solves make no FP64 field evaluation
         for e in (ea, e3):
-            assert e.counters()["scf_fp64"] == (1 if e.counters()["npol"] > 0 else 0)
+            assert e.counters()["scf_fp64"] == 0
+
+
+@pytest.mark.parametrize("case", ["pim192_static", "bsio_200_nvt", "melt216_nve"])
+@pytest.mark.parametrize("anchor,inner", [(0, 3), (3, 1), (3, 3)])
+def test_anchored_cg_anchor_and_step_modes(case, anchor, inner):
+    """anchored CG with the anchor evaluation in FP64 (0) or split FP16 over
+    the FP32 tables (3), and split-BF16 (1) or split-FP16 (3) steps (the
+    latter scaled by the channel maxima that the CG kernels write beside p):
+    dipoles and forces reach the FP64 form's to the SCF tolerance, and a
+    split-FP16 anchor needs no FP64 field evaluation"""
+    print(f"[gpu] anchor {case} {anchor} {inner}", flush=True)
+    p, cfg, _ = load_case(case)
+    p.conv_tol_ind = 1e-9
+    p.max_scf_pim_iter = 400
+    p.min_scf_pim_iter = 1
+    with m.Engine(p, cfg.typia, cfg.A, cfg.r, precision=m.FP64) as ef, \
+            m.Engine(p, cfg.typia, cfg.A, cfg.r, precision=m.MIXED) as ea:
+        ea.set_scf_solver(2, 6)
+        ea.set_kspace_modes(0, inner)
+        ea.set_scf_anchor(anchor)
+        tf, ta = ef.engforce(0), ea.engforce(0)
+        assert tf["status"] == 0 and ta["status"] == 0
+        assert ta["rmsd"] <= 1e-9
+        Ff, Fa = ef.forces(), ea.forces()
+        frms = np.sqrt(np.mean((Fa - Ff) ** 2)) / np.sqrt(np.mean(Ff ** 2))
+        assert frms < 1e-5, frms
+        assert rel(ea.dipoles(), ef.dipoles()) < 1e-5
+        assert ta["u_tot"] == pytest.approx(tf["u_tot"], rel=1e-6)
+        cnt = ea.counters()
+        assert cnt["scf_fp64"] == (1 if anchor == 0 and cnt["npol"] > 0 else 0)
 
 
 @pytest.mark.parametrize("case", ["pim192_static", "bsio_200_nvt", "melt216_nve"])
