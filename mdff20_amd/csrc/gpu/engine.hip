@@ -488,11 +488,12 @@ static int ensure_yz(mdff_ctx *c, int mode, int polar, int lo)
     const int nselp = c->nselp[polar];
     const size_t e = g_elt(mode);
     const size_t nb1 = (size_t)c->Pp * 2 * nselp * e, nb2 = (size_t)nselp * 2 * c->Pk * e;
+    // YZt only for the FP64 mode's back-projection (the others form it)
+    const bool wt = mode == KM_F64;
     int rc;
-    if ((rc = galloc(c, &c->yz[mode][polar][0], nb1)) || (rc = galloc(c, &c->yzt[mode][polar][0], nb2))) return rc;
-    if (g_split_mode(mode) &&
-        ((rc = galloc(c, &c->yz[mode][polar][1], nb1)) || (rc = galloc(c, &c->yzt[mode][polar][1], nb2))))
+    if ((rc = galloc(c, &c->yz[mode][polar][0], nb1)) || (wt && (rc = galloc(c, &c->yzt[mode][polar][0], nb2))))
         return rc;
+    if (g_split_mode(mode) && (rc = galloc(c, &c->yz[mode][polar][1], nb1))) return rc;
     const int NZ = 2 * c->Kz + 1;
     void **t = c->tab3[lo][polar];
     void *y0 = c->yz[mode][polar][0], *y1 = c->yz[mode][polar][1];
@@ -501,11 +502,16 @@ static int ensure_yz(mdff_ctx *c, int mode, int polar, int lo)
 #define YZL(M, TT)                                                                                            \
     hipLaunchKernelGGL((k_yz_build<M, TT>), g1, dim3(256), 0, c->st, nsel, nselp, c->P, c->Pp, c->Pk, NZ,   \
                        (const TT *)t[1], (const TT *)t[2], y0, y1, z0, z1)
+    dim3 g2(nselp / 64, (std::max(c->Pp, c->Pk) + 63) / 64);
+#define YZ16(M)                                                                                               \
+    hipLaunchKernelGGL(k_yz_build16<M>, g2, dim3(256), 0, c->st, nsel, nselp, c->P, c->Pp, c->Pk, NZ,          \
+                       (const float *)t[1], (const float *)t[2], y0, y1, z0, z1)
     if (mode == KM_F64) YZL(KM_F64, double);
-    else if (mode == KM_F32) YZL(KM_F32, float);
-    else if (mode == KM_F16X3) YZL(KM_F16X3, float);
-    else YZL(KM_BF16X3, float);
+    else if (mode == KM_F32) YZ16(KM_F32);
+    else if (mode == KM_F16X3) YZ16(KM_F16X3);
+    else YZ16(KM_BF16X3);
 #undef YZL
+#undef YZ16
     c->yz_ok[mode][polar] = 1;
     return MDFF_OK;
 }
@@ -753,7 +759,8 @@ static int launch_back_gemm(mdff_ctx *c, int polar, int lo, int NCH, const int *
     const int CT = (NC + 15) / 16;
     if (CT > MB_CT) return 0;
     const int mode = lo ? c->mf_lo : KM_F64;
-    if (ensure_yz(c, mode, polar, lo)) return 0;
+    // stored YZt in the FP64 mode only (the others form it from the tables)
+    if (mode == KM_F64 && ensure_yz(c, mode, polar, lo)) return 0;
     const int nsel = polar ? c->npol : c->n;
     const int nselp = c->nselp[polar];
     if (wm_alloc(c, mode) != MDFF_OK) return 0;
@@ -774,6 +781,8 @@ static int launch_back_gemm(mdff_ctx *c, int polar, int lo, int NCH, const int *
                            c->wm[mode][0], c->wm[mode][1], active);
     GBackArgs b;
     b.yzt = c->yzt[mode][polar][0]; b.yzt_lo = c->yzt[mode][polar][1];
+    b.gty = (const float *)c->tab3[lo][polar][1]; b.gtz = (const float *)c->tab3[lo][polar][2];
+    b.NY = 2 * c->Ky + 1; b.NZ = 2 * c->Kz + 1;
     b.wm = c->wm[mode][0]; b.wm_lo = c->wm[mode][1];
     b.tx = c->tab3[lo][polar][0];
     b.csc = mode == KM_F16X3 ? c->gsc + 16 : nullptr;

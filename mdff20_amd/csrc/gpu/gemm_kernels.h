@@ -143,6 +143,101 @@ __global__ void __launch_bounds__(256) k_yz_build(int nsel, int nselp, int P, in
     }
 }
 
+// the 16 bytes a lane stores per operand array: V values (8 for the 16-bit
+// modes, 4 for FP32) of one row, from FP32 values; the split modes write
+// hi = round(x) and lo = round(x - hi)
+template <int MODE>
+struct GPack {
+    static constexpr int V = MODE == KM_F32 ? 4 : 8;
+    __device__ static void store(void *hi, void *lo, size_t i, const float *x)
+    {
+        typedef unsigned u4 __attribute__((ext_vector_type(4)));
+        if constexpr (MODE == KM_F32) {
+            *(u4 *)((float *)hi + i) = u4{__float_as_uint(x[0]), __float_as_uint(x[1]), __float_as_uint(x[2]),
+                                          __float_as_uint(x[3])};
+        } else {
+            using T = typename GT<MODE>::T;
+            T h[8], l[8];
+#pragma unroll
+            for (int k = 0; k < 8; k++) {
+                h[k] = (T)x[k];
+                l[k] = (T)(x[k] - (float)h[k]);
+            }
+            u4 vh, vl;
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                vh[k] = (unsigned)__builtin_bit_cast(unsigned short, h[2 * k]) |
+                        ((unsigned)__builtin_bit_cast(unsigned short, h[2 * k + 1]) << 16);
+                vl[k] = (unsigned)__builtin_bit_cast(unsigned short, l[2 * k]) |
+                        ((unsigned)__builtin_bit_cast(unsigned short, l[2 * k + 1]) << 16);
+            }
+            *(u4 *)((T *)hi + i) = vh;
+            *(u4 *)((T *)lo + i) = vl;
+        }
+    }
+};
+
+// k_yz_build for the FP32 and split modes: one 64-ion x 64-point tile per
+// block, the products rounded to FP32 in LDS (below the resolution of these
+// modes), written in 16-byte stores: rows of 64 ions (point-major) and,
+// when yzt is given, of 64 points (ion-major).  nselp is a multiple of 64, Pk of 32
+// and Pp_rows of 64.  grid (nselp/64, ceil(max(Pp_rows, Pk)/64)).
+template <int MODE>
+__global__ void __launch_bounds__(256) k_yz_build16(int nsel, int nselp, int P, int Pp_rows, int Pk, int NZ,
+                                                    const float *__restrict__ ty, const float *__restrict__ tz,
+                                                    void *yz, void *yz_lo, void *yzt, void *yzt_lo)
+{
+    __shared__ float sr[64][65], si[64][65];   // [point][ion]
+    const int j0 = blockIdx.x * 64, p0 = blockIdx.y * 64;
+    {
+        const int tj = threadIdx.x & 63, tq = threadIdx.x >> 6;
+        const int j = j0 + tj;
+        for (int pp = tq; pp < 64; pp += 4) {
+            const int p = p0 + pp;
+            double br = 0.0, bi = 0.0;
+            if (j < nsel && p < P) {
+                const int iy = p / NZ, iz = p % NZ;
+                const size_t oy = 2 * ((size_t)iy * nsel + j), oz = 2 * ((size_t)iz * nsel + j);
+                const double yr = ty[oy], yi = ty[oy + 1], zr = tz[oz], zi = tz[oz + 1];
+                br = yr * zr - yi * zi;
+                bi = yr * zi + yi * zr;
+            }
+            sr[pp][tj] = (float)br;
+            si[pp][tj] = (float)bi;
+        }
+    }
+    __syncthreads();
+    constexpr int V = GPack<MODE>::V;   // values per lane store
+    constexpr int E = V / 2;            // complex entries per lane store
+    constexpr int LPR = 64 / E;         // lanes per 64-entry row
+    constexpr int RPP = 256 / LPR;      // rows per pass of the block
+    const int q = threadIdx.x % LPR, rsub = threadIdx.x / LPR;
+    float x[V];
+    // point-major rows YZ[p][2j + ri]: lanes along ions
+    for (int pp = rsub; pp < 64; pp += RPP) {
+        const int p = p0 + pp;
+        if (p >= Pp_rows) break;
+#pragma unroll
+        for (int e = 0; e < E; e++) {
+            x[2 * e] = sr[pp][q * E + e];
+            x[2 * e + 1] = si[pp][q * E + e];
+        }
+        GPack<MODE>::store(yz, yz_lo, (size_t)p * 2 * nselp + 2 * (size_t)(j0 + q * E), x);
+    }
+    // ion-major rows YZt[j][2p + ri]: lanes along points (when yzt is given)
+    const int pq = p0 + q * E;
+    if (yzt && pq < Pk) {
+        for (int jj = rsub; jj < 64; jj += RPP) {
+#pragma unroll
+            for (int e = 0; e < E; e++) {
+                x[2 * e] = sr[q * E + e][jj];
+                x[2 * e + 1] = si[q * E + e][jj];
+            }
+            GPack<MODE>::store(yzt, yzt_lo, (size_t)(j0 + jj) * 2 * Pk + 2 * (size_t)pq, x);
+        }
+    }
+}
+
 // AW rows (nx, c, re/im) over K = (j, ri): row (nx,c,0) = (w exr, -w exi),
 // row (nx,c,1) = (w exi, w exr).  With the per-block channel maxima wpart
 // of k_w_max (nwb blocks), w is scaled by a power of two 2^-e into [-1, 1]
@@ -343,6 +438,32 @@ __device__ __forceinline__ GRun<MODE> g_load(const void *base, size_t off)
     return r;
 }
 
+// the run of 8 operand values (re, im of 4 entries y_i z_i) formed from
+// FP32 table values: products in FP32, then for the split modes
+// hi = round(x) and lo = round(x - hi)
+template <int MODE>
+__device__ __forceinline__ void g_gen_run(const float2 *y, const float2 *z, GRun<MODE> &hi, GRun<MODE> &lo)
+{
+    float x[8];
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        x[2 * i] = fmaf(y[i].x, z[i].x, -(y[i].y * z[i].y));
+        x[2 * i + 1] = fmaf(y[i].x, z[i].y, y[i].y * z[i].x);
+    }
+    if constexpr (MODE == KM_F32) {
+#pragma unroll
+        for (int k = 0; k < 8; k++) hi.v[k] = x[k];
+    } else {
+        using T = typename GT<MODE>::T;
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            const T h = (T)x[k];
+            hi.v[k] = h;
+            lo.v[k] = (T)(x[k] - (float)h);
+        }
+    }
+}
+
 #define GK 32          // K values per block step
 #define GLDS_PAD 8     // LDS row padding (elements)
 
@@ -541,7 +662,11 @@ __global__ void k_part_sum(const part_t *__restrict__ part, int nchunk, int M, i
 #define GB_SLOTS 2   // B runs a thread stages per step: ceil(16 CT GK/8 / 512) for CT <= 16
 
 struct GBackArgs {
-    const void *yzt, *yzt_lo;  // [nselp][ldk]
+    const void *yzt, *yzt_lo;  // [nselp][ldk] (FP64 mode)
+    // the other modes form YZt[j][2p + ri] = ey_j[iy] ez_j[iz] from the FP32
+    // tables [NY][nsel][2], [NZ][nsel][2] of the selected ions
+    const float *gty, *gtz;
+    int NY, NZ;
     const void *wm, *wm_lo;    // [16 CT][ldk]
     const void *tx;            // ex tables [Kx+1][nsel][2] (TT)
     const double *csc;         // inverse column scales [(Kx+1) CH] (scaled WM) or null
@@ -570,7 +695,7 @@ inline size_t g_back_lds_bytes(int CTINST, int KX1)
 // back-projection: block = 8 waves x 16 ions, all CT column tiles, one K
 // range (grid.y); epilogue as k_sk_back_mfma (x ex[nx], moments)
 template <int MODE, int CTMAX, typename TT>
-__global__ void __launch_bounds__(512, ((MODE == KM_F64 || (MODE == KM_F32 && CTMAX >= 13)) ? 2 : 4)) k_gemm_back(GBackArgs a)
+__global__ void __launch_bounds__(512, ((MODE == KM_F64 || CTMAX >= 13) ? 2 : 4)) k_gemm_back(GBackArgs a)
 {
     if (a.active && !*a.active) return;
     using T = typename GT<MODE>::T;
@@ -616,12 +741,37 @@ __global__ void __launch_bounds__(512, ((MODE == KM_F64 || (MODE == KM_F32 && CT
     // 256 registers a wave of a 512-thread block can have.
     constexpr bool TWO = !(MODE == KM_F64 && CTMAX >= 13);
     constexpr int NQ = (nslot + 511) / 512;
-    struct Set {
+    // the A runs: loaded (FP64) or formed from the table values of the
+    // lane's ion and 4 points (the other modes; ions past nsel repeat the
+    // last, points past P the last row, against zero WM columns)
+    constexpr bool GEN = MODE != KM_F64;
+    struct SetS {
         GRun<MODE> ah, al, rb[NQ], rbl[NQ];
-    } S0, S1;
+    };
+    struct SetG {
+        float2 y[4], z[4];
+        GRun<MODE> rb[NQ], rbl[NQ];
+    };
+    using Set = typename std::conditional<GEN, SetG, SetS>::type;
+    Set S0, S1;
+    const int jc = jrow0 + cl < a.nsel ? jrow0 + cl : a.nsel - 1;
     auto load_step = [&](Set &S, int kb) {
-        S.ah = g_load<MODE>(a.yzt, arow + kb + 8 * g);
-        if constexpr (SPLIT) S.al = g_load<MODE>(a.yzt_lo, arow + kb + 8 * g);
+        if constexpr (GEN) {
+            const int pb = (kb + 8 * g) >> 1;
+            int iy = pb / a.NZ, iz = pb - iy * a.NZ;
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const int yy = iy < a.NY ? iy : a.NY - 1;
+                S.y[i] = *(const float2 *)(a.gty + 2 * ((size_t)yy * a.nsel + jc));
+                S.z[i] = *(const float2 *)(a.gtz + 2 * ((size_t)iz * a.nsel + jc));
+                iz++;
+                iy = iz == a.NZ ? iy + 1 : iy;
+                iz = iz == a.NZ ? 0 : iz;
+            }
+        } else {
+            S.ah = g_load<MODE>(a.yzt, arow + kb + 8 * g);
+            if constexpr (SPLIT) S.al = g_load<MODE>(a.yzt_lo, arow + kb + 8 * g);
+        }
 #pragma unroll
         for (int q = 0; q < NQ; q++) {
             const int e0 = threadIdx.x + 512 * q, e = e0 < nslot ? e0 : nslot - 1;
@@ -641,7 +791,9 @@ __global__ void __launch_bounds__(512, ((MODE == KM_F64 || (MODE == KM_F32 && CT
                 if constexpr (SPLIT) *(decltype(S.rbl[q].v) *)(Bsl + r * LD + q8) = S.rbl[q].v;
             }
         }
-        GRun<MODE> ch = S.ah, cl_ = S.al;
+        GRun<MODE> ch, cl_;
+        if constexpr (GEN) g_gen_run<MODE>(S.y, S.z, ch, cl_);
+        else { ch = S.ah; cl_ = S.al; }
         __syncthreads();
         load_step(S, kb + dist * GK < k1 ? kb + dist * GK : kb);
         if constexpr (SPLIT) {
