@@ -562,24 +562,25 @@ static const void *gback_kernel(int mode, int CT)
     return gback_kernel_t<KM_BF16X3, float>(CT);
 }
 
-// launch the instantiation for the shape's tile count
-template <int MODE, int CW>
+// launch the instantiation for the shape's tile count (GEN: B formed from
+// the tables)
+template <int MODE, int CW, bool GEN>
 static void gfwd_mt(mdff_ctx *c, dim3 grid, int MT, const GFwdArgs &a)
 {
     const int MTI = g_mt_inst(MT);
-    const size_t lds = g_fwd_lds_bytes<MODE>(MTI);
-    if (MTI == 3) hipLaunchKernelGGL((k_gemm_fwd<MODE, CW, 3>), grid, dim3(256), lds, c->st, a);
-    else if (MTI == 5) hipLaunchKernelGGL((k_gemm_fwd<MODE, CW, 5>), grid, dim3(256), lds, c->st, a);
-    else if (MTI == 7) hipLaunchKernelGGL((k_gemm_fwd<MODE, CW, 7>), grid, dim3(256), lds, c->st, a);
-    else hipLaunchKernelGGL((k_gemm_fwd<MODE, CW, MF_MT>), grid, dim3(256), lds, c->st, a);
+    const size_t lds = g_fwd_lds_bytes<MODE>(MTI, GEN);
+    if (MTI == 3) hipLaunchKernelGGL((k_gemm_fwd<MODE, CW, 3, GEN>), grid, dim3(256), lds, c->st, a);
+    else if (MTI == 5) hipLaunchKernelGGL((k_gemm_fwd<MODE, CW, 5, GEN>), grid, dim3(256), lds, c->st, a);
+    else if (MTI == 7) hipLaunchKernelGGL((k_gemm_fwd<MODE, CW, 7, GEN>), grid, dim3(256), lds, c->st, a);
+    else hipLaunchKernelGGL((k_gemm_fwd<MODE, CW, MF_MT, GEN>), grid, dim3(256), lds, c->st, a);
 }
 
-template <int MODE>
+template <int MODE, bool GEN = false>
 static void gfwd_cw(mdff_ctx *c, dim3 grid, int CW, int MT, const GFwdArgs &a)
 {
-    if (CW == 1) gfwd_mt<MODE, 1>(c, grid, MT, a);
-    else if (CW == 4) gfwd_mt<MODE, 4>(c, grid, MT, a);
-    else gfwd_mt<MODE, 2>(c, grid, MT, a);
+    if (CW == 1) gfwd_mt<MODE, 1, GEN>(c, grid, MT, a);
+    else if (CW == 4) gfwd_mt<MODE, 4, GEN>(c, grid, MT, a);
+    else gfwd_mt<MODE, 2, GEN>(c, grid, MT, a);
 }
 
 template <int MODE, typename TT>
@@ -663,7 +664,11 @@ static int launch_fwd_gemm(mdff_ctx *c, int polar, int lo, int C, const double *
     const int MT = (M + 15) / 16;
     if (MT > MF_MT) return 0;
     const int mode = lo ? c->mf_lo : KM_F64;
-    if (ensure_yz(c, mode, polar, lo)) return 0;
+    // stored YZ for the FP64 mode, and where a block's table slice does not
+    // fit the staging area; the others form it from the tables
+    const int CW = c->g_cw[mode == KM_F64 ? 0 : 1];
+    const bool gen = mode != KM_F64 && g_fwd_gen_fits(CW, 2 * c->Kz + 1);
+    if (!gen && ensure_yz(c, mode, polar, lo)) return 0;
     const int nsel = polar ? c->npol : c->n;
     const int nselp = c->nselp[polar];
     const size_t e = g_elt(mode);
@@ -699,9 +704,10 @@ static int launch_fwd_gemm(mdff_ctx *c, int polar, int lo, int C, const double *
     GFwdArgs a;
     a.aw = c->aw[mode][0]; a.aw_lo = c->aw[mode][1];
     a.yz = c->yz[mode][polar][0]; a.yz_lo = c->yz[mode][polar][1];
+    a.gty = (const float *)t[1]; a.gtz = (const float *)t[2];
+    a.nsel = nsel; a.NY = 2 * c->Ky + 1; a.NZ = 2 * c->Kz + 1;
     a.MT = MT; a.M = M; a.P = c->P; a.C = C; a.ldk = 2 * nselp;
     a.part = c->kpart; a.active = active;
-    const int CW = c->g_cw[mode == KM_F64 ? 0 : 1];
     const int gx = (c->P + 64 * CW - 1) / (64 * CW);
     const size_t nval = (size_t)M * c->P;
     // K split to reach `blocks` blocks (K ranges of an even number of steps:
@@ -717,9 +723,9 @@ static int launch_fwd_gemm(mdff_ctx *c, int polar, int lo, int C, const double *
         dim3 grid(gx, nch);
         if (probe) probe_mark(c, 0);
         if (mode == KM_F64) gfwd_cw<KM_F64>(c, grid, CW, MT, ga2);
-        else if (mode == KM_F32) gfwd_cw<KM_F32>(c, grid, CW, MT, ga2);
-        else if (mode == KM_F16X3) gfwd_cw<KM_F16X3>(c, grid, CW, MT, ga2);
-        else gfwd_cw<KM_BF16X3>(c, grid, CW, MT, ga2);
+        else if (mode == KM_F32) { if (gen) gfwd_cw<KM_F32, true>(c, grid, CW, MT, ga2); else gfwd_cw<KM_F32>(c, grid, CW, MT, ga2); }
+        else if (mode == KM_F16X3) { if (gen) gfwd_cw<KM_F16X3, true>(c, grid, CW, MT, ga2); else gfwd_cw<KM_F16X3>(c, grid, CW, MT, ga2); }
+        else { if (gen) gfwd_cw<KM_BF16X3, true>(c, grid, CW, MT, ga2); else gfwd_cw<KM_BF16X3>(c, grid, CW, MT, ga2); }
         if (probe) probe_mark(c, 1);
         if (mode == KM_F64)
             hipLaunchKernelGGL(k_part_sum<double>, dim3(cdiv(nval, 256)), dim3(256), 0, c->st,

@@ -469,7 +469,11 @@ __device__ __forceinline__ void g_gen_run(const float2 *y, const float2 *z, GRun
 
 struct GFwdArgs {
     const void *aw, *aw_lo;   // [16 MF_MT][ldk], rows >= M zero
-    const void *yz, *yz_lo;   // [Pp][ldk]
+    const void *yz, *yz_lo;   // [Pp][ldk] (FP64 mode)
+    // the other modes form YZ[p][2j + ri] = ey_j[iy] ez_j[iz] from the FP32
+    // tables [NY][nsel][2], [NZ][nsel][2] of the selected ions
+    const float *gty, *gtz;
+    int nsel, NY, NZ;
     int MT, M, P, C, ldk, kchunk;
     double *part;             // [nchunk][M][P] (FP64, or FP32 in the other modes)
     const int *active;
@@ -479,12 +483,24 @@ struct GFwdArgs {
 // this many (zero rows past M) so that no matrix instruction is predicated
 inline int g_mt_inst(int MT) { return MT <= 3 ? 3 : (MT <= 5 ? 5 : (MT <= 7 ? 7 : 9)); }
 
+// table slice of a forward step formed from the tables (GEN): GT_ROWS rows
+// (the ey rows of the block's points, then all ez rows) x 16 ions, rows of
+// GT_LD complex values (padded: the 16 rows a fragment reads fall in
+// distinct banks)
+#define GT_ROWS 64
+#define GT_LD 18
+
 template <int MODE>
-inline size_t g_fwd_lds_bytes(int MT)
+inline size_t g_fwd_lds_bytes(int MT, bool gen = false)
 {
     size_t e = (size_t)MT * 16 * (GK + GLDS_PAD);
-    return MODE == KM_F64 ? e * 8 : (MODE == KM_F32 ? e * 4 : e * 2 * 2);
+    e = MODE == KM_F64 ? e * 8 : (MODE == KM_F32 ? e * 4 : e * 2 * 2);
+    return e + (gen ? (size_t)GT_ROWS * GT_LD * sizeof(float2) : 0);
 }
+
+// whether a forward block's table slice fits GT_ROWS rows: at most
+// (64 CW - 1)/NZ + 2 ey rows and NZ ez rows
+inline bool g_fwd_gen_fits(int CW, int NZ) { return (64 * CW - 1) / NZ + 2 + NZ <= GT_ROWS; }
 
 // forward sum: block = 4 waves x CW column tiles of 16 points, all MT row
 // tiles, one K range (grid.y); partial sums per K range in the layout of
@@ -509,9 +525,10 @@ __device__ __forceinline__ void g_xcd_order(int gx, int gy, int &bx, int &by)
     bx = L % gx;
     by = L / gx;
 }
-template <int MODE, int CW, int MTMAX>
+template <int MODE, int CW, int MTMAX, bool GEN = false>
 __global__ void __launch_bounds__(256) k_gemm_fwd(GFwdArgs a)
 {
+    static_assert(!GEN || MODE != KM_F64, "the FP64 mode loads stored operands");
     if (a.active && !*a.active) return;
     using T = typename GT<MODE>::T;
     constexpr int LD = GK + GLDS_PAD;
@@ -534,23 +551,64 @@ __global__ void __launch_bounds__(256) k_gemm_fwd(GFwdArgs a)
     for (int t = 0; t < MTMAX; t++)
 #pragma unroll
         for (int c = 0; c < CW; c++) acc[t][c] = acc_t{0, 0, 0, 0};
+    // the B runs: loaded (stored operands) or, with GEN, formed from a table
+    // slice staged in LDS per step: rows 0..nyr-1 the ey rows iy0.. of the
+    // block's points, then the NZ ez rows, each over the step's 16 ions
+    // (points past P use the last point, ions past nsel the last ion,
+    // against zero AW columns)
     size_t brow[CW];
 #pragma unroll
     for (int c = 0; c < CW; c++) brow[c] = (size_t)((ct0 + c) * 16 + cl) * a.ldk;
+    float2 *ts = (float2 *)(smem_raw + (size_t)MTMAX * 16 * LD * sizeof(T) * (SPLIT ? 2 : 1));
+    int iy0 = 0, nyr = 0, nrow = 0;
+    int yr[CW], zr[CW];
+    if constexpr (GEN) {
+        const int pb0 = bx * 4 * CW * 16;
+        const int pl = min(pb0 + 64 * CW - 1, a.P - 1);
+        iy0 = min(pb0, a.P - 1) / a.NZ;
+        nyr = pl / a.NZ - iy0 + 1;
+        nrow = nyr + a.NZ;
+#pragma unroll
+        for (int c = 0; c < CW; c++) {
+            const int p = min((ct0 + c) * 16 + cl, a.P - 1);
+            const int iy = p / a.NZ;
+            yr[c] = (iy - iy0) * GT_LD + 4 * g;
+            zr[c] = (nyr + p - iy * a.NZ) * GT_LD + 4 * g;
+        }
+    }
     // Two register sets alternate (the K range holds an even number of
     // steps): the loads of step k+2 are issued before step k computes, and
     // never under a branch, so the compiler keeps them in flight across the
     // step (a conditional load makes it wait for everything before the
     // matrix instructions).  Past the end the last step is loaded again.
     constexpr int NQ = (nslot + 255) / 256;
-    struct Set {
+    constexpr int TSL = GT_ROWS * 16 / 256;   // table entries a thread stages per step
+    struct SetS {
         GRun<MODE> bh[CW], bl[CW], ra[NQ], rl[NQ];
-    } S0, S1;
+    };
+    struct SetG {
+        float2 t[TSL];
+        GRun<MODE> ra[NQ], rl[NQ];
+    };
+    using Set = typename std::conditional<GEN, SetG, SetS>::type;
+    Set S0, S1;
     auto load_step = [&](Set &S, int kb) {
+        if constexpr (GEN) {
+            // entry e = (row, ion): rows past nrow repeat the last row
 #pragma unroll
-        for (int c = 0; c < CW; c++) {
-            S.bh[c] = g_load<MODE>(a.yz, brow[c] + kb + 8 * g);
-            if constexpr (SPLIT) S.bl[c] = g_load<MODE>(a.yz_lo, brow[c] + kb + 8 * g);
+            for (int u = 0; u < TSL; u++) {
+                const int e = threadIdx.x + 256 * u;
+                const int row = min(e >> 4, nrow - 1), jj = min((kb >> 1) + (e & 15), a.nsel - 1);
+                const float *src = row < nyr ? a.gty + 2 * ((size_t)(iy0 + row) * a.nsel + jj)
+                                             : a.gtz + 2 * ((size_t)(row - nyr) * a.nsel + jj);
+                S.t[u] = *(const float2 *)src;
+            }
+        } else {
+#pragma unroll
+            for (int c = 0; c < CW; c++) {
+                S.bh[c] = g_load<MODE>(a.yz, brow[c] + kb + 8 * g);
+                if constexpr (SPLIT) S.bl[c] = g_load<MODE>(a.yz_lo, brow[c] + kb + 8 * g);
+            }
         }
 #pragma unroll
         for (int q = 0; q < NQ; q++) {
@@ -561,7 +619,8 @@ __global__ void __launch_bounds__(256) k_gemm_fwd(GFwdArgs a)
         }
     };
     auto step = [&](Set &S, int kb) {
-        // this step's A rows into LDS; its B runs stay in registers
+        // this step's A rows (and table slice) into LDS; stored B runs stay
+        // in registers
         __syncthreads();
 #pragma unroll
         for (int q = 0; q < NQ; q++) {
@@ -573,13 +632,35 @@ __global__ void __launch_bounds__(256) k_gemm_fwd(GFwdArgs a)
             }
         }
         GRun<MODE> ch[CW], cl_[CW];
+        if constexpr (GEN) {
 #pragma unroll
-        for (int c = 0; c < CW; c++) {
-            ch[c] = S.bh[c];
-            if constexpr (SPLIT) cl_[c] = S.bl[c];
+            for (int u = 0; u < TSL; u++) {
+                const int e = threadIdx.x + 256 * u;
+                if ((e >> 4) < nrow) ts[(e >> 4) * GT_LD + (e & 15)] = S.t[u];
+            }
+        } else {
+#pragma unroll
+            for (int c = 0; c < CW; c++) {
+                ch[c] = S.bh[c];
+                if constexpr (SPLIT) cl_[c] = S.bl[c];
+            }
         }
         __syncthreads();
         load_step(S, kb + 2 * GK < k1 ? kb + 2 * GK : kb);
+        if constexpr (GEN) {
+#pragma unroll
+            for (int c = 0; c < CW; c++) {
+                typedef float f4 __attribute__((ext_vector_type(4)));
+                float2 y[4], z[4];
+                const f4 y01 = *(const f4 *)(ts + yr[c]), y23 = *(const f4 *)(ts + yr[c] + 2);
+                const f4 z01 = *(const f4 *)(ts + zr[c]), z23 = *(const f4 *)(ts + zr[c] + 2);
+                y[0] = float2{y01[0], y01[1]}; y[1] = float2{y01[2], y01[3]};
+                y[2] = float2{y23[0], y23[1]}; y[3] = float2{y23[2], y23[3]};
+                z[0] = float2{z01[0], z01[1]}; z[1] = float2{z01[2], z01[3]};
+                z[2] = float2{z23[0], z23[1]}; z[3] = float2{z23[2], z23[3]};
+                g_gen_run<MODE>(y, z, ch[c], cl_[c]);
+            }
+        }
         if constexpr (SPLIT) {
             using V = gvec_t<MODE>;
             V ah[MTMAX], al[MTMAX];
