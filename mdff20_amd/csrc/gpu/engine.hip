@@ -852,6 +852,8 @@ static void rec_part(mdff_ctx *c, int polar, int lo, int C, int NCH, const doubl
     a.part = c->kpart;
     a.active = active;
     bool mixed = lo != 0;
+    const int bmode = lo ? c->mf_lo : KM_F64;
+    const int CHb = NCH == 9 ? 6 : 3;
     c->fwd_part = 0;
     const bool fwd_mf = launch_fwd_mfma(c, polar, lo, C, q_w, mu_w, active, &nchunk);
     c->wpart_ready = 0;
@@ -866,12 +868,20 @@ static void rec_part(mdff_ctx *c, int polar, int lo, int C, int NCH, const doubl
     r.part = gemm_fwd ? c->ksum : c->kpart; r.kvec = c->kvec;
     c->fwd_part = 0; r.Ak = c->Ak; r.kcoe = c->kcoe; r.W = c->W; r.ek = c->ek;
     r.do_stress = do_stress; r.active = active;
-    // the back-projection's WM columns come out of the reduction for the
-    // unscaled stored-operand modes
-    const int bmode = lo ? c->mf_lo : KM_F64;
-    const int CHb = NCH == 9 ? 6 : 3;
+    // the back-projection's WM columns come out of the reduction: for split
+    // FP16 (column-scaled) one block per nx plane (k_sk_scaled_wm), the other
+    // modes one thread per k-vector (k_sk_reduce_wm)
     bool wm_built = false;
-    if ((do_ef || do_f) && gemm_fwd && c->use_mfma && c->gemm && bmode != KM_F16X3 &&
+    if ((do_ef || do_f) && gemm_fwd && c->use_mfma && c->gemm && bmode == KM_F16X3 &&
+        (c->Kx + 1) * CHb * 2 <= MB_CT * 16 && c->Pk <= SKF_PPT * 1024 && wm_alloc(c, bmode) == MDFF_OK) {
+        void *w0 = c->wm[bmode][0], *w1 = c->wm[bmode][1];
+        double *ci = c->gsc + 16;
+        if (CHb == 6)
+            hipLaunchKernelGGL((k_sk_scaled_wm<KM_F16X3, 6>), dim3(c->Kx + 1), dim3(1024), 0, c->st, r, c->Pk, w0, w1, ci);
+        else
+            hipLaunchKernelGGL((k_sk_scaled_wm<KM_F16X3, 3>), dim3(c->Kx + 1), dim3(1024), 0, c->st, r, c->Pk, w0, w1, ci);
+        wm_built = true;
+    } else if ((do_ef || do_f) && gemm_fwd && c->use_mfma && c->gemm && bmode != KM_F16X3 &&
         (c->Kx + 1) * CHb * 2 <= MB_CT * 16 && wm_alloc(c, bmode) == MDFF_OK) {
         void *w0 = c->wm[bmode][0], *w1 = c->wm[bmode][1];
         const dim3 gr(cdiv(c->nkf, 256));

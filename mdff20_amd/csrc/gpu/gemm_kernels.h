@@ -345,6 +345,77 @@ __global__ void k_sk_reduce_wm(KRedArgs a, int CH, int Pk, void *wm, void *wm_lo
     }
 }
 
+// the reduction (sk_reduce_one, channel sums of the forward sum in part)
+// fused with the split-FP16 WM columns of the back-projection, one
+// 1024-thread block per nx plane: the CH column pairs (nx, m) over the Pk
+// points (zero past P), each scaled by a power of two into [-1, 1] with 2^e
+// in cinv[nx CH + m].  P <= Pk <= SKF_PPT 1024.
+#define SKF_PPT 4
+template <int MODE, int CH>
+__global__ void __launch_bounds__(1024) k_sk_scaled_wm(KRedArgs a, int Pk, void *wm, void *wm_lo, double *cinv)
+{
+    if (a.active && !*a.active) return;
+    const int nx = blockIdx.x;
+    const int NZ = 2 * a.Kz + 1;
+    double Wr[SKF_PPT], Wi[SKF_PPT], mx[CH];
+#pragma unroll
+    for (int m = 0; m < CH; m++) mx[m] = 0.0;
+#pragma unroll
+    for (int u = 0; u < SKF_PPT; u++) {
+        const int p = threadIdx.x + 1024 * u;
+        Wr[u] = 0.0;
+        Wi[u] = 0.0;
+        if (p < a.P) {
+            sk_reduce_one(a, (size_t)nx * a.P + p, Wr[u], Wi[u]);
+            const int iy = p / NZ, iz = p % NZ;
+#pragma unroll
+            for (int m = 0; m < CH; m++) {
+                const double mom = mb_moment(m, iy - a.Ky, iz - a.Kz);
+                mx[m] = fmax(mx[m], fmax(fabs(Wr[u] * mom), fabs(Wi[u] * mom)));
+            }
+        }
+    }
+    __shared__ double sc[CH];
+    {
+        __shared__ double red[16][CH];
+#pragma unroll
+        for (int m = 0; m < CH; m++) {
+            double v = mx[m];
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o));
+            if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6][m] = v;
+        }
+        __syncthreads();
+        if ((int)threadIdx.x < CH) {
+            double amax = 0.0;
+            for (int w = 0; w < 16; w++) amax = fmax(amax, red[w][threadIdx.x]);
+            const int e = (amax > 0.0 && isfinite(amax)) ? ilogb(amax) + 1 : 0;
+            sc[threadIdx.x] = ldexp(1.0, -e);
+            cinv[nx * CH + threadIdx.x] = ldexp(1.0, e);
+        }
+        __syncthreads();
+    }
+    const size_t ld = 2 * (size_t)Pk;
+#pragma unroll
+    for (int u = 0; u < SKF_PPT; u++) {
+        const int p = threadIdx.x + 1024 * u;
+        if (p >= Pk) continue;
+        const int iy = p < a.P ? p / NZ : 0, iz = p < a.P ? p % NZ : 0;
+#pragma unroll
+        for (int m = 0; m < CH; m++) {
+            double wr = 0.0, wi = 0.0;
+            if (p < a.P) {
+                const double mom = mb_moment(m, iy - a.Ky, iz - a.Kz) * sc[m];
+                wr = Wr[u] * mom;
+                wi = Wi[u] * mom;
+            }
+            const size_t c0 = (size_t)(nx * CH + m) * 2;
+            g_store2<MODE>(wm, wm_lo, c0 * ld + 2 * (size_t)p, wr, -wi);
+            g_store2<MODE>(wm, wm_lo, (c0 + 1) * ld + 2 * (size_t)p, wi, wr);
+        }
+    }
+}
+
 // largest |w| of each weight channel (q, mu components) over the block's
 // 256 selected ions: wpart[c nb + block], nb = gridDim.x.  Non-finite
 // weights count as zero.
@@ -834,7 +905,10 @@ __global__ void __launch_bounds__(512, ((MODE == KM_F64 || CTMAX >= 13) ? 2 : 4)
         GRun<MODE> rb[NQ], rbl[NQ];
     };
     using Set = typename std::conditional<GEN, SetG, SetS>::type;
-    Set S0, S1;
+    // register sets in flight: loads of step k+NS are issued before step k
+    // computes
+    constexpr int NS = TWO ? 2 : 1;
+    Set S[NS];
     const int jc = jrow0 + cl < a.nsel ? jrow0 + cl : a.nsel - 1;
     auto load_step = [&](Set &S, int kb) {
         if constexpr (GEN) {
@@ -896,8 +970,8 @@ __global__ void __launch_bounds__(512, ((MODE == KM_F64 || CTMAX >= 13) ? 2 : 4)
         }
     };
     if (k0 < k1) {
-        load_step(S0, k0);
-        if constexpr (TWO) load_step(S1, k0 + GK < k1 ? k0 + GK : k0);
+        load_step(S[0], k0);
+        if constexpr (NS > 1) load_step(S[1], k0 + GK < k1 ? k0 + GK : k0);
     }
     // the staged tables into LDS (their loads were issued before the
     // first steps' loads); read in the epilogue only
@@ -907,13 +981,9 @@ __global__ void __launch_bounds__(512, ((MODE == KM_F64 || CTMAX >= 13) ? 2 : 4)
         if (e < nex) exs[e] = (jb + ((e & 255) >> 1)) < a.nsel ? xv[u] : (TT)0;
     }
     if ((int)threadIdx.x < ncs) cscs[threadIdx.x] = csv;
-    if constexpr (TWO) {
-        for (int kb = k0; kb < k1; kb += 2 * GK) {
-            step(S0, kb, 2);
-            step(S1, kb + GK, 2);
-        }
-    } else {
-        for (int kb = k0; kb < k1; kb += GK) step(S0, kb, 1);
+    for (int kb = k0; kb < k1; kb += NS * GK) {
+        step(S[0], kb, NS);
+        if constexpr (NS > 1) step(S[1], kb + GK, NS);
     }
     __syncthreads();   // the staging area is free; the ex table is complete (also for an empty K range)
     // Epilogue, two waves at a time: their accumulator tiles (32 ions x all
@@ -974,8 +1044,11 @@ __global__ void __launch_bounds__(512, ((MODE == KM_F64 || CTMAX >= 13) ? 2 : 4)
             o[c] = v;
         }
         const int j = blockIdx.x * 128 + jl;
-        if (es == 0 && j < a.nsel)
-            for (int c = 0; c < nch; c++) out[(size_t)c * a.nsel + j] = o[c];
+        if (es == 0 && j < a.nsel) {
+#pragma unroll
+            for (int c = 0; c < 9; c++)
+                if (c < nch) out[(size_t)c * a.nsel + j] = o[c];
+        }
         __syncthreads();   // the tiles are rewritten in the next round
     }
 }

@@ -586,7 +586,9 @@ struct KRedArgs {
 
 // rho(k) (kspace.c:189-302) from the channel sums, str = |rho|^2 Ak and the
 // stress terms of multipole_ES_rec ewald.c:949-1006; W = Ak conj(rho).
-__device__ __forceinline__ void sk_reduce_one(const KRedArgs &a, size_t k, double &Wr, double &Wi)
+// rho(k) from the channel sums S, then str, W and the stress terms
+__device__ __forceinline__ void sk_reduce_S(const KRedArgs &a, size_t k, const double (&S)[4][2], double &Wr,
+                                            double &Wi)
 {
     size_t nkf = (size_t)(a.Kx + 1) * a.P;
     double Ak = a.Ak[k];
@@ -595,11 +597,6 @@ __device__ __forceinline__ void sk_reduce_one(const KRedArgs &a, size_t k, doubl
         for (int t = 0; t < 7; t++) a.ek[t * nkf + k] = 0.0;
         Wr = Wi = 0.0;
         return;
-    }
-    double S[4][2] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
-    for (int ch = 0; ch < a.nchunk; ch++) {
-        const double *pp = a.part + ((size_t)ch * nkf + k) * (2 * a.C);
-        for (int c = 0; c < a.C; c++) { S[c][0] += pp[2 * c]; S[c][1] += pp[2 * c + 1]; }
     }
     double kx = a.kvec[3 * k], ky = a.kvec[3 * k + 1], kz = a.kvec[3 * k + 2];
     double rR = 0, rI = 0;
@@ -630,6 +627,19 @@ __device__ __forceinline__ void sk_reduce_one(const KRedArgs &a, size_t k, doubl
     } else {
         for (int t = 1; t < 7; t++) e[t * nkf + k] = 0.0;
     }
+}
+
+__device__ __forceinline__ void sk_reduce_one(const KRedArgs &a, size_t k, double &Wr, double &Wi)
+{
+    size_t nkf = (size_t)(a.Kx + 1) * a.P;
+    double S[4][2] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
+    if (a.Ak[k] != 0.0) {
+        for (int ch = 0; ch < a.nchunk; ch++) {
+            const double *pp = a.part + ((size_t)ch * nkf + k) * (2 * a.C);
+            for (int c = 0; c < a.C; c++) { S[c][0] += pp[2 * c]; S[c][1] += pp[2 * c + 1]; }
+        }
+    }
+    sk_reduce_S(a, k, S, Wr, Wi);
 }
 
 __global__ void k_sk_reduce(KRedArgs a)
