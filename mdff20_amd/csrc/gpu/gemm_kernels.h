@@ -834,14 +834,32 @@ inline int g_ct_inst(int CT) { return CT <= 4 ? 4 : (CT <= 7 ? 7 : (CT <= 13 ? 1
 // largest k box of the back-projection: (Kx+1) 6 2 <= 16 MB_CT
 #define GB_KX1_MAX (16 * MB_CT / 12)
 
-// LDS of the back-projection: CTINST column tiles (x2 for the split modes),
-// the block's ex table, (Kx+1) x 128 ions x (re, im), and the column scales
+// epilogue of the back-projection: waves whose accumulator tiles one round
+// stages (all 8 where the LDS allows), threads per ion (each takes every
+// TPI-th nx), and the padded row of the block's ex table (elements: the TPI
+// lanes of one ion read rows in distinct banks)
+constexpr int gb_rw(int mode, int ct) { return mode == KM_F64 ? 2 : (ct >= 13 ? 4 : 8); }
+constexpr int gb_tpi(int mode, int ct) { return 512 / (gb_rw(mode, ct) * 16); }
+constexpr int gb_exld(int mode, int ct) { return 256 + 32 / gb_tpi(mode, ct); }
+
+// LDS of the back-projection: the staging area (CTINST column tiles, x2 for
+// the split modes, or the epilogue's accumulator tiles), the block's ex
+// table, (Kx+1) rows of 128 ions x (re, im) padded, and the column scales
+template <int MODE>
+constexpr size_t g_back_sa_bytes(int CTINST)
+{
+    return (((size_t)CTINST * 16 * (GK + GLDS_PAD) * (MODE == KM_F64 ? 8 : 4) >
+                     (size_t)gb_rw(MODE, CTINST) * 16 * (16 * CTINST + 1) * (MODE == KM_F64 ? 8 : 4)
+                 ? (size_t)CTINST * 16 * (GK + GLDS_PAD) * (MODE == KM_F64 ? 8 : 4)
+                 : (size_t)gb_rw(MODE, CTINST) * 16 * (16 * CTINST + 1) * (MODE == KM_F64 ? 8 : 4)) +
+            15) / 16 * 16;
+}
+
 template <int MODE>
 inline size_t g_back_lds_bytes(int CTINST, int KX1)
 {
-    size_t e = (size_t)CTINST * 16 * (GK + GLDS_PAD);
-    size_t b = MODE == KM_F64 ? e * 8 : (MODE == KM_F32 ? e * 4 : e * 2 * 2);
-    return b + (size_t)KX1 * 128 * 2 * (MODE == KM_F64 ? 8 : 4) + sizeof(double) * GB_KX1_MAX * 6;
+    return g_back_sa_bytes<MODE>(CTINST) + (size_t)KX1 * gb_exld(MODE, CTINST) * (MODE == KM_F64 ? 8 : 4) +
+           sizeof(double) * GB_KX1_MAX * 6;
 }
 
 // back-projection: block = 8 waves x 16 ions, all CT column tiles, one K
@@ -857,9 +875,10 @@ __global__ void __launch_bounds__(512, ((MODE == KM_F64 || CTMAX >= 13) ? 2 : 4)
     T *Bsl = Bs + CTMAX * 16 * LD;   // low parts of the split modes
     // the ex values of this block's ions, [nx][128][re, im], and the
     // column scales [(nx, m)] (1 unless the columns were scaled)
-    TT *exs = (TT *)(smem_raw + (size_t)CTMAX * 16 * LD * sizeof(T) * (g_is_split<MODE>() ? 2 : 1));
+    constexpr int EXLD = gb_exld(MODE, CTMAX);
+    TT *exs = (TT *)(smem_raw + g_back_sa_bytes<MODE>(CTMAX));
     const int KX1 = a.NC / (2 * a.CH);
-    double *cscs = (double *)(exs + (size_t)KX1 * 256);
+    double *cscs = (double *)(exs + (size_t)KX1 * EXLD);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int g = lane >> 4, cl = lane & 15;
     const int jrow0 = blockIdx.x * 128 + wave * 16;
@@ -978,7 +997,7 @@ __global__ void __launch_bounds__(512, ((MODE == KM_F64 || CTMAX >= 13) ? 2 : 4)
 #pragma unroll
     for (int u = 0; u < EXU; u++) {
         const int e = threadIdx.x + 512 * u;
-        if (e < nex) exs[e] = (jb + ((e & 255) >> 1)) < a.nsel ? xv[u] : (TT)0;
+        if (e < nex) exs[(e >> 8) * EXLD + (e & 255)] = (jb + ((e & 255) >> 1)) < a.nsel ? xv[u] : (TT)0;
     }
     if ((int)threadIdx.x < ncs) cscs[threadIdx.x] = csv;
     for (int kb = k0; kb < k1; kb += NS * GK) {
@@ -986,23 +1005,23 @@ __global__ void __launch_bounds__(512, ((MODE == KM_F64 || CTMAX >= 13) ? 2 : 4)
         if constexpr (NS > 1) step(S[1], kb + GK, NS);
     }
     __syncthreads();   // the staging area is free; the ex table is complete (also for an empty K range)
-    // Epilogue, two waves at a time: their accumulator tiles (32 ions x all
-    // columns) go to the staging area, then every thread of the block takes
-    // one ion and every 16th nx and forms Z = ex[nx] T for each moment m
-    // (a uniform, unrolled loop: no lane-dependent branch), and the 16 lanes
-    // of an ion are summed.  Outputs: field H (x nx, 1, 1 for m = 0, 1, 2)
-    // and, with the gradient, M (x nx^2, nx, nx, 1, 1, 1 for m = 0..5).
+    // Epilogue, RW waves at a time (all 8 where the LDS allows): their
+    // accumulator tiles (16 ions x all columns each) go to the staging area,
+    // then TPI threads per ion take every TPI-th nx and form Z = ex[nx] T for
+    // each moment m (a uniform, unrolled loop: no lane-dependent branch), and
+    // the TPI lanes of an ion are summed.  Outputs: field H (x nx, 1, 1 for
+    // m = 0, 1, 2) and, with the gradient, M (x nx^2, nx, nx, 1, 1, 1 for
+    // m = 0..5).
     using et = typename std::conditional<MODE == KM_F64, double, float>::type;
     constexpr int TLD = 16 * CTMAX + 1;
-    static_assert(2 * 16 * TLD * sizeof(et) <= (size_t)CTMAX * 16 * LD * sizeof(T) * (g_is_split<MODE>() ? 2 : 1),
-                  "epilogue tiles fit the staging area");
+    constexpr int RW = gb_rw(MODE, CTMAX), TPI = gb_tpi(MODE, CTMAX);
     et *tile = (et *)smem_raw;
     const int nch = (a.CH == 6) ? 9 : 3;
     double *out = a.part + (size_t)blockIdx.y * nch * a.nsel;
-    const int er = threadIdx.x >> 4, es = threadIdx.x & 15;
-    for (int round = 0; round < 4; round++) {
-        if ((wave >> 1) == round) {
-            et *tw = tile + (wave & 1) * 16 * TLD;
+    const int er = threadIdx.x / TPI, es = threadIdx.x % TPI;
+    for (int round = 0; round < 8 / RW; round++) {
+        if (wave / RW == round) {
+            et *tw = tile + (wave % RW) * 16 * TLD;
 #pragma unroll
             for (int ct = 0; ct < CTMAX; ct++)
 #pragma unroll
@@ -1012,10 +1031,10 @@ __global__ void __launch_bounds__(512, ((MODE == KM_F64 || CTMAX >= 13) ? 2 : 4)
                 }
         }
         __syncthreads();
-        const int jl = round * 32 + er;              // ion within the block
+        const int jl = round * RW * 16 + er;         // ion within the block
         double o[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-        for (int nx = es; nx < KX1; nx += 16) {
-            const double xr = (double)exs[2 * (nx * 128 + jl)], xi = (double)exs[2 * (nx * 128 + jl) + 1];
+        for (int nx = es; nx < KX1; nx += TPI) {
+            const double xr = (double)exs[nx * EXLD + 2 * jl], xi = (double)exs[nx * EXLD + 2 * jl + 1];
             const double fnx = (double)nx;
             const et *rowp = tile + er * TLD + nx * a.CH * 2;
             const double *cs = cscs + nx * a.CH;
@@ -1037,10 +1056,8 @@ __global__ void __launch_bounds__(512, ((MODE == KM_F64 || CTMAX >= 13) ? 2 : 4)
 #pragma unroll
         for (int c = 0; c < 9; c++) {
             double v = o[c];
-            v += __shfl_xor(v, 1, 64);
-            v += __shfl_xor(v, 2, 64);
-            v += __shfl_xor(v, 4, 64);
-            v += __shfl_xor(v, 8, 64);
+#pragma unroll
+            for (int w = 1; w < TPI; w <<= 1) v += __shfl_xor(v, w, 64);
             o[c] = v;
         }
         const int j = blockIdx.x * 128 + jl;
