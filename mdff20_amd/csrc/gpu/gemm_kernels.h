@@ -804,8 +804,17 @@ __global__ void k_part_sum(const part_t *__restrict__ part, int nchunk, int M, i
     const size_t nval = (size_t)M * P;
     if (v >= nval) return;
     const int r = (int)(v / P), p = (int)(v % P);
+    // loads issued eight at a time, summed in chunk order
     double s = 0.0;
-    for (int ch = 0; ch < nchunk; ch++) s += (double)part[(size_t)ch * nval + v];
+    int ch = 0;
+    for (; ch + 8 <= nchunk; ch += 8) {
+        part_t x[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) x[u] = part[(size_t)(ch + u) * nval + v];
+#pragma unroll
+        for (int u = 0; u < 8; u++) s += (double)x[u];
+    }
+    for (; ch < nchunk; ch++) s += (double)part[(size_t)ch * nval + v];
     const int nx = r / (2 * C), cc = (r >> 1) % C, ri = r & 1;
     if (inv) s *= inv[cc];
     out[((size_t)nx * P + p) * (2 * C) + 2 * cc + ri] = s;
