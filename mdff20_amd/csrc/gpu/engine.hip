@@ -136,6 +136,7 @@ struct mdff_ctx {
     void *wm[4][2] = {};      // [mode][hi,lo]: columns (nx,m,re/im) of the back sum
     double *gsc = nullptr;    // split-FP16 scales: weight inverses [8,16), WM column inverses from 16
     double *gwp = nullptr;    // split-FP16 per-block weight channel maxima [4][n/256]
+    unsigned long long *gcm = nullptr;   // split-FP16 WM column maxima [(Kx+1) 6] (double bits)
     struct GTune { int mode, dir, tiles, polar, blocks; };
     std::vector<GTune> gtune;  // block targets of the stored-operand sums, timed on first use
     int mf_lo = KM_F16X3;     // matrix-core mode of the FP32-table evaluations (static field, forces)
@@ -693,14 +694,17 @@ static int launch_fwd_gemm(mdff_ctx *c, int polar, int lo, int C, const double *
         hipLaunchKernelGGL((k_aw_build<KM_F64, double>), ga, dim3(256), 0, c->st, nsel, nselp, C, (const double *)t[0],
                            idx, q_w, mu_w, c->n, c->aw[mode][0], c->aw[mode][1], wsc, nwb, c->gsc + 8, active);
     else if (mode == KM_F32)
-        hipLaunchKernelGGL((k_aw_build<KM_F32, float>), ga, dim3(256), 0, c->st, nsel, nselp, C, (const float *)t[0],
-                           idx, q_w, mu_w, c->n, c->aw[mode][0], c->aw[mode][1], wsc, nwb, c->gsc + 8, active);
+        hipLaunchKernelGGL(k_aw_build16<KM_F32>, dim3(cdiv(nselp, 256 * 2), c->Kx + 1, C), dim3(256), 0, c->st, nsel,
+                           nselp, C, (const float *)t[0], idx, q_w, mu_w, c->n, c->aw[mode][0], c->aw[mode][1], wsc,
+                           nwb, c->gsc + 8, active);
     else if (mode == KM_F16X3)
-        hipLaunchKernelGGL((k_aw_build<KM_F16X3, float>), ga, dim3(256), 0, c->st, nsel, nselp, C,
-                           (const float *)t[0], idx, q_w, mu_w, c->n, c->aw[mode][0], c->aw[mode][1], wsc, nwb, c->gsc + 8, active);
+        hipLaunchKernelGGL(k_aw_build16<KM_F16X3>, dim3(cdiv(nselp, 256 * 4), c->Kx + 1, C), dim3(256), 0, c->st, nsel,
+                           nselp, C, (const float *)t[0], idx, q_w, mu_w, c->n, c->aw[mode][0], c->aw[mode][1], wsc,
+                           nwb, c->gsc + 8, active);
     else
-        hipLaunchKernelGGL((k_aw_build<KM_BF16X3, float>), ga, dim3(256), 0, c->st, nsel, nselp, C,
-                           (const float *)t[0], idx, q_w, mu_w, c->n, c->aw[mode][0], c->aw[mode][1], wsc, nwb, c->gsc + 8, active);
+        hipLaunchKernelGGL(k_aw_build16<KM_BF16X3>, dim3(cdiv(nselp, 256 * 4), c->Kx + 1, C), dim3(256), 0, c->st, nsel,
+                           nselp, C, (const float *)t[0], idx, q_w, mu_w, c->n, c->aw[mode][0], c->aw[mode][1], wsc,
+                           nwb, c->gsc + 8, active);
     GFwdArgs a;
     a.aw = c->aw[mode][0]; a.aw_lo = c->aw[mode][1];
     a.yz = c->yz[mode][polar][0]; a.yz_lo = c->yz[mode][polar][1];
@@ -732,7 +736,7 @@ static int launch_fwd_gemm(mdff_ctx *c, int polar, int lo, int C, const double *
                                (const double *)c->kpart, nch, M, c->P, c->ksum, C, winv, act);
         else
             hipLaunchKernelGGL(k_part_sum<float>, dim3(cdiv(nval, 256)), dim3(256), 0, c->st, (const float *)c->kpart,
-                               nch, M, c->P, c->ksum, C, winv, act);
+                               nch, M, c->P, c->ksum, C, winv, act, c->gcm, 6 * (c->Kx + 1));
         return nch;
     };
     const int fb = c->g_fblocks > 0 ? c->g_fblocks
@@ -869,17 +873,18 @@ static void rec_part(mdff_ctx *c, int polar, int lo, int C, int NCH, const doubl
     c->fwd_part = 0; r.Ak = c->Ak; r.kcoe = c->kcoe; r.W = c->W; r.ek = c->ek;
     r.do_stress = do_stress; r.active = active;
     // the back-projection's WM columns come out of the reduction: for split
-    // FP16 (column-scaled) one block per nx plane (k_sk_scaled_wm), the other
-    // modes one thread per k-vector (k_sk_reduce_wm)
+    // FP16 (column-scaled) the reduction gives the column maxima (zeroed by
+    // the partial sums) and k_wm_scaled the columns; the other modes form
+    // them in the reduction (k_sk_reduce_wm)
     bool wm_built = false;
     if ((do_ef || do_f) && gemm_fwd && c->use_mfma && c->gemm && bmode == KM_F16X3 &&
-        (c->Kx + 1) * CHb * 2 <= MB_CT * 16 && c->Pk <= SKF_PPT * 1024 && wm_alloc(c, bmode) == MDFF_OK) {
+        (c->Kx + 1) * CHb * 2 <= MB_CT * 16 && wm_alloc(c, bmode) == MDFF_OK) {
         void *w0 = c->wm[bmode][0], *w1 = c->wm[bmode][1];
-        double *ci = c->gsc + 16;
-        if (CHb == 6)
-            hipLaunchKernelGGL((k_sk_scaled_wm<KM_F16X3, 6>), dim3(c->Kx + 1), dim3(1024), 0, c->st, r, c->Pk, w0, w1, ci);
-        else
-            hipLaunchKernelGGL((k_sk_scaled_wm<KM_F16X3, 3>), dim3(c->Kx + 1), dim3(1024), 0, c->st, r, c->Pk, w0, w1, ci);
+        const dim3 gr(cdiv(c->nkf, 256));
+        if (CHb == 6) hipLaunchKernelGGL(k_sk_reduce_cm<6>, gr, dim3(256), 0, c->st, r, c->gcm);
+        else hipLaunchKernelGGL(k_sk_reduce_cm<3>, gr, dim3(256), 0, c->st, r, c->gcm);
+        hipLaunchKernelGGL(k_wm_scaled, dim3(cdiv(c->Pk, 256), (c->Kx + 1) * CHb), dim3(256), 0, c->st, c->P, c->Pk,
+                           c->Ky, c->Kz, CHb, c->W, c->gcm, w0, w1, c->gsc + 16, active);
         wm_built = true;
     } else if ((do_ef || do_f) && gemm_fwd && c->use_mfma && c->gemm && bmode != KM_F16X3 &&
         (c->Kx + 1) * CHb * 2 <= MB_CT * 16 && wm_alloc(c, bmode) == MDFF_OK) {
@@ -1694,6 +1699,7 @@ void mdff_destroy(mdff_ctx *c)
     dfree(c->tab_d);
     dfree(c->pidx);
     dfree(c->pinv);
+    if (c->gcm) (void)hipFree(c->gcm);
     dfree(c->cntp);
     dfree(c->idxp);
     dfree(c->t2_ovf);
@@ -1929,6 +1935,10 @@ int mdff_create(mdff_ctx **out, const mdff_params *p, int nion, const int *typia
         ALLOC(c->ek, 7 * c->nkf);
         ALLOC(c->gsc, 16 + 6 * (size_t)(c->Kx + 1));
         ALLOC(c->gwp, 4 * (size_t)cdiv(c->n, 256));
+        if (hipMalloc((void **)&c->gcm, sizeof(unsigned long long) * 6 * (size_t)(c->Kx + 1)) != hipSuccess) {
+            mdff_destroy(c);
+            return MDFF_ERR_NOMEM;
+        }
         if (hipMemcpy(c->kvec, kv.data(), sizeof(double) * 3 * c->nkf, hipMemcpyHostToDevice) != hipSuccess ||
             hipMemcpy(c->Ak, ak.data(), sizeof(double) * c->nkf, hipMemcpyHostToDevice) != hipSuccess ||
             hipMemcpy(c->kcoe, kc.data(), sizeof(double) * c->nkf, hipMemcpyHostToDevice) != hipSuccess) {

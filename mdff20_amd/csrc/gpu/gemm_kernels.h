@@ -294,6 +294,77 @@ __global__ void __launch_bounds__(256) k_aw_build(int nsel, int nselp, int C, co
     }
 }
 
+// k_aw_build for the FP32 and split modes: E = GPack::V/2 consecutive ions
+// per thread (one 16-byte store per row and operand array), one channel
+// per block.  grid (ceil(nselp / (256 E)), Kx+1, C), 256 threads.
+template <int MODE>
+__global__ void __launch_bounds__(256) k_aw_build16(int nsel, int nselp, int C, const float *__restrict__ tx,
+                                                    const int *__restrict__ idx, const double *__restrict__ q,
+                                                    const double *__restrict__ mu, int n, void *aw, void *aw_lo,
+                                                    const double *__restrict__ wpart, int nwb, double *winv,
+                                                    const int *active)
+{
+    if (active && !*active) return;
+    constexpr int V = GPack<MODE>::V, E = V / 2;
+    __shared__ double wsc[4];
+    if (wpart) {
+        const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+        if (wv < C) {
+            double m = 0.0;
+            for (int l = lane; l < nwb; l += 64) m = fmax(m, wpart[(size_t)wv * nwb + l]);
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o));
+            if (lane == 0) {
+                const int e = m > 0.0 ? ilogb(m) + 1 : 0;
+                wsc[wv] = ldexp(1.0, -e);
+                if (blockIdx.x == 0 && blockIdx.y == 0 && blockIdx.z == 0) winv[wv] = ldexp(1.0, e);
+            }
+        }
+        __syncthreads();
+    }
+    const int j0 = (blockIdx.x * 256 + threadIdx.x) * E;
+    const int nx = blockIdx.y;
+    if (j0 >= nselp) return;
+    double xr[E], xi[E];
+    int ion[E];
+#pragma unroll
+    for (int e = 0; e < E; e++) {
+        const int j = j0 + e;
+        xr[e] = 0.0;
+        xi[e] = 0.0;
+        ion[e] = -1;
+        if (j < nsel) {
+            const size_t ox = 2 * ((size_t)nx * nsel + j);
+            xr[e] = tx[ox];
+            xi[e] = tx[ox + 1];
+            ion[e] = idx ? idx[j] : j;
+        }
+    }
+    const size_t ld = 2 * (size_t)nselp;
+    {
+        const int c = blockIdx.z;
+        float r0v[V], r1v[V];
+#pragma unroll
+        for (int e = 0; e < E; e++) {
+            double w = 0.0;
+            if (ion[e] >= 0) {
+                const int i = ion[e];
+                if (C == 1) w = q[i];
+                else if (C == 3) w = mu[(size_t)c * n + i];
+                else w = c == 0 ? q[i] : mu[(size_t)(c - 1) * n + i];
+                if (wpart) w *= wsc[c];
+            }
+            r0v[2 * e] = (float)(w * xr[e]);
+            r0v[2 * e + 1] = (float)(-w * xi[e]);
+            r1v[2 * e] = (float)(w * xi[e]);
+            r1v[2 * e + 1] = (float)(w * xr[e]);
+        }
+        const size_t r0 = (size_t)(nx * C + c) * 2;
+        GPack<MODE>::store(aw, aw_lo, r0 * ld + 2 * (size_t)j0, r0v);
+        GPack<MODE>::store(aw, aw_lo, (r0 + 1) * ld + 2 * (size_t)j0, r1v);
+    }
+}
+
 // WM columns (nx, m, re/im) over K = (p, ri): with Wm = W(nx, p) mom_m(p),
 // column (nx,m,0) = (Re Wm, -Im Wm), column (nx,m,1) = (Im Wm, Re Wm).
 // grid (Pp/256, (Kx+1) CH), one thread per point.
@@ -345,75 +416,82 @@ __global__ void k_sk_reduce_wm(KRedArgs a, int CH, int Pk, void *wm, void *wm_lo
     }
 }
 
-// the reduction (sk_reduce_one, channel sums of the forward sum in part)
-// fused with the split-FP16 WM columns of the back-projection, one
-// 1024-thread block per nx plane: the CH column pairs (nx, m) over the Pk
-// points (zero past P), each scaled by a power of two into [-1, 1] with 2^e
-// in cinv[nx CH + m].  P <= Pk <= SKF_PPT 1024.
-#define SKF_PPT 4
-template <int MODE, int CH>
-__global__ void __launch_bounds__(1024) k_sk_scaled_wm(KRedArgs a, int Pk, void *wm, void *wm_lo, double *cinv)
+// the reduction (sk_reduce_one) of a split-FP16 evaluation, one thread per
+// k-vector, with the column maxima of its WM columns: for every nx plane in
+// the block and moment m < CH, the block maximum of |W mom_m| goes into
+// cm[nx CH + m] by atomicMax on the bits of the non-negative double (cm
+// zeroed by k_part_sum beforehand; non-finite values count as zero)
+template <int CH>
+__global__ void __launch_bounds__(256) k_sk_reduce_cm(KRedArgs a, unsigned long long *cm)
 {
     if (a.active && !*a.active) return;
-    const int nx = blockIdx.x;
+    __shared__ double red[4];
+    const size_t nkf = (size_t)(a.Kx + 1) * a.P;
+    const size_t k0 = (size_t)blockIdx.x * 256, k = k0 + threadIdx.x;
     const int NZ = 2 * a.Kz + 1;
-    double Wr[SKF_PPT], Wi[SKF_PPT], mx[CH];
+    double v[CH];
 #pragma unroll
-    for (int m = 0; m < CH; m++) mx[m] = 0.0;
-#pragma unroll
-    for (int u = 0; u < SKF_PPT; u++) {
-        const int p = threadIdx.x + 1024 * u;
-        Wr[u] = 0.0;
-        Wi[u] = 0.0;
-        if (p < a.P) {
-            sk_reduce_one(a, (size_t)nx * a.P + p, Wr[u], Wi[u]);
-            const int iy = p / NZ, iz = p % NZ;
-#pragma unroll
-            for (int m = 0; m < CH; m++) {
-                const double mom = mb_moment(m, iy - a.Ky, iz - a.Kz);
-                mx[m] = fmax(mx[m], fmax(fabs(Wr[u] * mom), fabs(Wi[u] * mom)));
-            }
-        }
-    }
-    __shared__ double sc[CH];
-    {
-        __shared__ double red[16][CH];
+    for (int m = 0; m < CH; m++) v[m] = 0.0;
+    int nx = -1;
+    if (k < nkf) {
+        double Wr, Wi;
+        sk_reduce_one(a, k, Wr, Wi);
+        nx = (int)(k / a.P);
+        const int p = (int)(k % a.P), iy = p / NZ, iz = p % NZ;
 #pragma unroll
         for (int m = 0; m < CH; m++) {
-            double v = mx[m];
+            const double mom = mb_moment(m, iy - a.Ky, iz - a.Kz);
+            const double x = fmax(fabs(Wr * mom), fabs(Wi * mom));
+            v[m] = isfinite(x) ? x : 0.0;
+        }
+    }
+    const size_t kl = k0 + 255 < nkf ? k0 + 255 : nkf - 1;
+    const int nlo = (int)(k0 / a.P), nhi = (int)(kl / a.P);
+    for (int nxv = nlo; nxv <= nhi; nxv++) {
 #pragma unroll
-            for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o));
-            if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6][m] = v;
+        for (int m = 0; m < CH; m++) {
+            double x = nx == nxv ? v[m] : 0.0;
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) x = fmax(x, __shfl_xor(x, o));
+            if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = x;
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                const double b = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
+                if (b > 0.0) atomicMax(cm + nxv * CH + m, (unsigned long long)__double_as_longlong(b));
+            }
+            __syncthreads();
         }
-        __syncthreads();
-        if ((int)threadIdx.x < CH) {
-            double amax = 0.0;
-            for (int w = 0; w < 16; w++) amax = fmax(amax, red[w][threadIdx.x]);
-            const int e = (amax > 0.0 && isfinite(amax)) ? ilogb(amax) + 1 : 0;
-            sc[threadIdx.x] = ldexp(1.0, -e);
-            cinv[nx * CH + threadIdx.x] = ldexp(1.0, e);
-        }
-        __syncthreads();
+    }
+}
+
+// split-FP16 WM columns (nx, m, re/im) over the Pk points (zero past P),
+// each column scaled by 2^-e into [-1, 1] from its maximum cm[nx CH + m],
+// 2^e into cinv.  grid (Pk/256, (Kx+1) CH).
+__global__ void __launch_bounds__(256) k_wm_scaled(int P, int Pk, int Ky, int Kz, int CH, const double *__restrict__ W,
+                                                   const unsigned long long *__restrict__ cm, void *wm, void *wm_lo,
+                                                   double *cinv, const int *active)
+{
+    if (active && !*active) return;
+    const int nx = blockIdx.y / CH, m = blockIdx.y % CH;
+    const double amax = __longlong_as_double((long long)cm[blockIdx.y]);
+    const int e = amax > 0.0 ? ilogb(amax) + 1 : 0;
+    const double scl = ldexp(1.0, -e);
+    if (blockIdx.x == 0 && threadIdx.x == 0) cinv[blockIdx.y] = ldexp(1.0, e);
+    const int p = blockIdx.x * 256 + threadIdx.x;
+    if (p >= Pk) return;
+    const int NZ = 2 * Kz + 1;
+    double wr = 0.0, wi = 0.0;
+    if (p < P) {
+        const int iy = p / NZ, iz = p % NZ;
+        const double mom = mb_moment(m, iy - Ky, iz - Kz) * scl;
+        const size_t kf = (size_t)nx * P + p;
+        wr = W[2 * kf] * mom;
+        wi = W[2 * kf + 1] * mom;
     }
     const size_t ld = 2 * (size_t)Pk;
-#pragma unroll
-    for (int u = 0; u < SKF_PPT; u++) {
-        const int p = threadIdx.x + 1024 * u;
-        if (p >= Pk) continue;
-        const int iy = p < a.P ? p / NZ : 0, iz = p < a.P ? p % NZ : 0;
-#pragma unroll
-        for (int m = 0; m < CH; m++) {
-            double wr = 0.0, wi = 0.0;
-            if (p < a.P) {
-                const double mom = mb_moment(m, iy - a.Ky, iz - a.Kz) * sc[m];
-                wr = Wr[u] * mom;
-                wi = Wi[u] * mom;
-            }
-            const size_t c0 = (size_t)(nx * CH + m) * 2;
-            g_store2<MODE>(wm, wm_lo, c0 * ld + 2 * (size_t)p, wr, -wi);
-            g_store2<MODE>(wm, wm_lo, (c0 + 1) * ld + 2 * (size_t)p, wi, wr);
-        }
-    }
+    const size_t c0 = (size_t)(nx * CH + m) * 2;
+    g_store2<KM_F16X3>(wm, wm_lo, c0 * ld + 2 * (size_t)p, wr, -wi);
+    g_store2<KM_F16X3>(wm, wm_lo, (c0 + 1) * ld + 2 * (size_t)p, wi, wr);
 }
 
 // largest |w| of each weight channel (q, mu components) over the block's
@@ -794,12 +872,17 @@ __global__ void __launch_bounds__(256) k_gemm_fwd(GFwdArgs a)
 // sum of the K-range partials of the forward sum, one thread per value
 // (row r = (nx, c, re/im), point p), chunks in a fixed order (bitwise
 // reproducible), into the layout [(nx, p)][c][re/im] of k_sk_reduce; values
-// are multiplied by inv[c] when the weights were scaled
+// are multiplied by inv[c] when the weights were scaled.  Block 0 also
+// zeroes the ncm column maxima cm of the reduction that follows
+// (k_sk_reduce_cm), if given.
 template <typename part_t>
 __global__ void k_part_sum(const part_t *__restrict__ part, int nchunk, int M, int P, double *__restrict__ out,
-                           int C, const double *__restrict__ inv, const int *active)
+                           int C, const double *__restrict__ inv, const int *active,
+                           unsigned long long *cm = nullptr, int ncm = 0)
 {
     if (active && !*active) return;
+    if (cm && blockIdx.x == 0)
+        for (int i = threadIdx.x; i < ncm; i += blockDim.x) cm[i] = 0ull;
     const size_t v = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
     const size_t nval = (size_t)M * P;
     if (v >= nval) return;
