@@ -122,10 +122,10 @@ int mdff_md_run(mdff_ctx *ctx, int first_step, int nsteps, mdff_thermo *th);
 void mdff_tt_damping(double b, double c, double r, double *f, double *fd, int order);
 
 /* Tuning: lanes per ion in the neighbour kernels (16, 32 or 64; 0 keeps),
- * reciprocal-space kernel family katoms (-1 vector ALU; 1 matrix-core sums
- * over operands stored once per force call, split-BF16 SCF correction
- * sweeps, the default; 2 the same with FP32 sweeps; 3 matrix cores with
- * operands formed in LDS on every call; 0 keeps) and ksplit, the target
+ * reciprocal-space kernel family katoms (-1 vector ALU; 1 the streaming
+ * matrix-core sums, split-FP16 SCF steps in the mixed form, the default; 2
+ * the same with FP32 SCF steps; 3 the older matrix-core kernels that form
+ * whole operand tiles in LDS on every call; 0 keeps) and ksplit, the target
  * block count of the matrix-core kernels (0 keeps).  No counterpart in the
  * reference. */
 int mdff_set_tuning(mdff_ctx *ctx, int lanes, int katoms, int ksplit);
