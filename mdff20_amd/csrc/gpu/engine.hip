@@ -137,6 +137,7 @@ struct mdff_ctx {
     double *gsc = nullptr;    // split-FP16 scales: weight inverses [8,16), WM column inverses from 16
     double *gwp = nullptr;    // split-FP16 per-block weight channel maxima [4][n/256]
     unsigned long long *gcm = nullptr;   // split-FP16 WM column maxima [(Kx+1) 6] (double bits)
+    double *chk_part = nullptr;          // SCF rmsd / u_pol partial sums [2][n/256]
     struct GTune { int mode, dir, tiles, polar, blocks; };
     std::vector<GTune> gtune;  // block targets of the stored-operand sums, timed on first use
     int mf_lo = KM_F16X3;     // matrix-core mode of the FP32-table evaluations (static field, forces)
@@ -1078,6 +1079,17 @@ static bool induced_field(mdff_ctx *c, int lo, const double *mu, const double *b
     return false;
 }
 
+// rmsd of mu_ind against ef (and u_pol when upol is given) and the loop
+// condition of momentpolaSCF pim.c:236, in two passes
+static void scf_check(mdff_ctx *c, const double *upol, int maxit, int minit)
+{
+    const int nb = cdiv(c->n, 256);
+    hipLaunchKernelGGL(k_scf_check_part, dim3(nb), dim3(256), 0, c->st, c->n, c->type, c->tab_d, c->mu_ind, c->ef,
+                       upol, c->ctl, c->chk_part);
+    hipLaunchKernelGGL(k_scf_check_fin, dim3(1), dim3(256), 0, c->st, nb, c->chk_part, upol ? 1 : 0, c->ctl, c->scal,
+                       c->npol, maxit, minit, c->p.conv_tol_ind);
+}
+
 static void scf_finish(mdff_ctx *c)
 {
     int n = c->n;
@@ -1115,8 +1127,7 @@ static int scf_loop(mdff_ctx *c, int istep)
                 hipLaunchKernelGGL(k_scf_correct, dim3(cdiv(n, 256)), dim3(256), 0, c->st, n, asp, c->type,
                                    c->tab_d, c->ef, c->mu_ind, act);
             // rmsd, u_pol of this iteration's dipoles and the loop condition
-            hipLaunchKernelGGL(k_scf_check, dim3(1), dim3(1024), 0, c->st, n, c->type, c->tab_d, c->mu_ind, c->ef,
-                               c->upol_i, c->ctl, c->scal, c->npol, maxit, minit, c->p.conv_tol_ind);
+            scf_check(c, c->upol_i, maxit, minit);
         }
         int rc = sync_scal(c);
         if (rc) return rc;
@@ -1195,8 +1206,7 @@ static int scf_loop_mixed(mdff_ctx *c, int istep)
     int limit = std::max(maxit, minit) + 1;
     auto eval = [&]() {
         induced_field(c, 0, c->mu_ind, c->ef_stat, c->ef, act);
-        hipLaunchKernelGGL(k_scf_check, dim3(1), dim3(1024), 0, c->st, n, c->type, c->tab_d, c->mu_ind, c->ef,
-                           (const double *)nullptr, c->ctl, c->scal, c->npol, maxit, minit, c->p.conv_tol_ind);
+        scf_check(c, nullptr, maxit, minit);
         launched++;
     };
     auto correct = [&]() {
@@ -1269,8 +1279,7 @@ static int scf_loop_anchored(mdff_ctx *c, int istep)
     } else {
         induced_field(c, 0, c->mu_ind, c->ef_stat, c->ef, act);
     }
-    hipLaunchKernelGGL(k_scf_check, dim3(1), dim3(1024), 0, c->st, n, c->type, c->tab_d, c->mu_ind, c->ef,
-                       (const double *)nullptr, ctl, c->scal, np, maxit, minit, c->p.conv_tol_ind);
+    scf_check(c, nullptr, maxit, minit);
     const int nb = cdiv(np, CG_BLOCK);
     // split-FP16 steps: the kernels that write p also give the channel
     // maxima of p that scale the forward sum's weights
@@ -1412,8 +1421,7 @@ static int scf_loop_ko(mdff_ctx *c, int istep)
                                    c->tab_d, c->ef, c->mu_ind, act);
             // rmsd, u_pol of this iteration's dipoles and the loop condition;
             // then the residual field of the next update
-            hipLaunchKernelGGL(k_scf_check, dim3(1), dim3(1024), 0, c->st, n, c->type, c->tab_d, c->mu_ind, c->ef,
-                               c->upol_i, c->ctl, c->scal, c->npol, maxit, minit, c->p.conv_tol_ind);
+            scf_check(c, c->upol_i, maxit, minit);
             hipLaunchKernelGGL(k_ko_resid, dim3(cdiv(c->npol, 256)), dim3(256), 0, c->st, n, c->npol, c->pidx,
                                c->type, c->tab_d, c->ef, c->mu_ind, c->ko_f, act);
         }
@@ -1691,7 +1699,7 @@ void mdff_destroy(mdff_ctx *c)
     double **dp[] = {&c->r, &c->v, &c->f, &c->fold, &c->rs, &c->s, &c->s0, &c->invmass, &c->massd, &c->q, &c->qz,
                      &c->mu_static, &c->mu_ind, &c->mu_tot, &c->hist, &c->ef_stat, &c->ef, &c->ef_dir, &c->ef_rec,
                      &c->efg, &c->efg_dir, &c->efg_rec, &c->f_sr, &c->f_dir, &c->f_rec, &c->acc_sr, &c->uw_dir,
-                     &c->selfbuf, &c->ke, &c->upol_i, &c->rmsd_i, &c->tmp_q, &c->tmp_mu, &c->tmp_qraw, &c->tmp_mur, &c->dc_R, &c->dc_d, &c->dc_g, &c->cg_z, &c->cg_p, &c->cg_part, &c->ko_f, &c->ko_dmu, &c->t2ko, &c->cg_w, &c->scal,
+                     &c->selfbuf, &c->ke, &c->upol_i, &c->rmsd_i, &c->tmp_q, &c->tmp_mu, &c->tmp_qraw, &c->tmp_mur, &c->dc_R, &c->dc_d, &c->dc_g, &c->cg_z, &c->cg_p, &c->cg_part, &c->chk_part, &c->ko_f, &c->ko_dmu, &c->t2ko, &c->cg_w, &c->scal,
                      &c->kvec, &c->Ak, &c->kcoe, &c->W, &c->ek, &c->kpart, &c->bpart, &c->ksum, &c->gsc, &c->gwp};
     for (auto pp : dp) dfree(*pp);
     dfree(c->type);
@@ -1853,6 +1861,7 @@ int mdff_create(mdff_ctx **out, const mdff_params *p, int nion, const int *typia
     ALLOC(c->tmp_q, n); ALLOC(c->tmp_mu, 3 * n); ALLOC(c->tmp_qraw, n); ALLOC(c->tmp_mur, 3 * n);
     ALLOC(c->dc_R, 3 * n); ALLOC(c->dc_d, 3 * n); ALLOC(c->dc_g, 3 * n);
     ALLOC(c->cg_z, 3 * n); ALLOC(c->cg_p, 3 * n); ALLOC(c->cg_part, 4 * (size_t)cdiv(n, CG_BLOCK));
+    ALLOC(c->chk_part, 2 * (size_t)cdiv(n, 256));
     ALLOC(c->ko_f, 3 * n); ALLOC(c->ko_dmu, 3 * n); ALLOC(c->cg_w, 3 * n);
     ALLOC(c->scal, S_COUNT); ALLOC(c->ctl, C_COUNT); ALLOC(c->tab_d, 1);
     if (hipHostMalloc((void **)&c->ctl_h, sizeof(int) * C_COUNT) != hipSuccess) { mdff_destroy(c); return MDFF_ERR_NOMEM; }

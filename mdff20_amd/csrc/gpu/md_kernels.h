@@ -466,6 +466,59 @@ __device__ __forceinline__ double cg_sum_partials(const double *part, int nb, do
     return cg_block_sum(v, sh);
 }
 
+// k_scf_check in two passes: per-block partial sums of the rmsd terms (and
+// of u_pol_i when given) over 256 ions each, then the finishing block sums
+// the partials in block order and applies the loop condition
+__global__ void __launch_bounds__(256) k_scf_check_part(int n, const int *__restrict__ type,
+                                                        const TypeTab *__restrict__ T, const double *__restrict__ mu,
+                                                        const double *__restrict__ ef,
+                                                        const double *__restrict__ upol_i, const int *ctl,
+                                                        double *part)
+{
+    if (!ctl[C_SCF_ACTIVE]) return;
+    __shared__ double sh[CG_BLOCK];
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    double r = 0.0, u = 0.0;
+    if (i < n) {
+        const int t = type[i];
+        if (T->polar[t]) {
+            for (int c = 0; c < 3; c++) {
+                const double paa = T->pol[t][4 * c];
+                if (paa != 0.0) {
+                    const double e = mu[(size_t)c * n + i] / paa - ef[(size_t)c * n + i];
+                    r += e * e;
+                }
+            }
+        }
+        if (upol_i) u = upol_i[i];
+    }
+    const double sr = cg_block_sum(r, sh);
+    const double su = upol_i ? cg_block_sum(u, sh) : 0.0;
+    if (threadIdx.x == 0) {
+        part[blockIdx.x] = sr;
+        part[gridDim.x + blockIdx.x] = su;
+    }
+}
+
+__global__ void __launch_bounds__(256) k_scf_check_fin(int nb, const double *__restrict__ part, int with_upol,
+                                                       int *ctl, double *scal, int npol, int maxit, int minit,
+                                                       double tol)
+{
+    if (!ctl[C_SCF_ACTIVE]) return;
+    __shared__ double sh[CG_BLOCK];
+    const double r = cg_sum_partials(part, nb, sh);
+    const double u = with_upol ? cg_sum_partials(part + nb, nb, sh) : 0.0;
+    if (threadIdx.x == 0) {
+        scal[S_RMSD2] = r;
+        if (with_upol) scal[S_UPOL] = u;
+        const int iscf = ctl[C_ISCF] + 1;
+        const double rmsd = sqrt(r / (double)(npol > 0 ? npol : 1));
+        ctl[C_ISCF] = iscf;
+        ctl[C_SCF_ACTIVE] = ((iscf < maxit && rmsd > tol) || iscf < minit) ? 1 : 0;
+    }
+}
+
+
 // r = E - pol^-T mu, z = p = pol^T r (= pol^T E - mu), d = 0; partials of
 // r.z; with wpart, the block's channel maxima of |p| (cg_block_max3)
 __global__ void __launch_bounds__(CG_BLOCK) k_cg_init(int n, int npol, const int *__restrict__ pidx,
