@@ -138,6 +138,7 @@ struct mdff_ctx {
     double *gwp = nullptr;    // split-FP16 per-block weight channel maxima [4][n/256]
     unsigned long long *gcm = nullptr;   // split-FP16 WM column maxima [(Kx+1) 6] (double bits)
     double *chk_part = nullptr;          // SCF rmsd / u_pol partial sums [2][n/256]
+    double *pq_buf = nullptr;            // CG p.q partial sums [n/16]
     struct GTune { int mode, dir, tiles, polar, blocks; };
     std::vector<GTune> gtune;  // block targets of the stored-operand sums, timed on first use
     int mf_lo = KM_F16X3;     // matrix-core mode of the FP32-table evaluations (static field, forces)
@@ -836,6 +837,8 @@ struct FieldFuse {
     double *ef;
     const double *mu, *base;
     double *pq_part;   // optional: the CG step's p.q partial sums (k_sk_back_fin)
+    const void *t2;    // optional: pair tensors whose field of mu the finish forms (k_sk_back_fin_t2)
+    int t2f32;         // t2 in FP32
 };
 
 static void rec_part(mdff_ctx *c, int polar, int lo, int C, int NCH, const double *q_w, const double *mu_w,
@@ -925,7 +928,17 @@ static void rec_part(mdff_ctx *c, int polar, int lo, int C, int NCH, const doubl
         fa.type = c->type;
         fa.T = c->tab_d;
         fa.pq_part = ff ? ff->pq_part : nullptr;
-        hipLaunchKernelGGL(k_sk_back_fin, dim3(cdiv(nsel, 256)), dim3(256), 0, c->st, fa);
+        if (ff && ff->t2) {
+            const dim3 gf(cdiv((size_t)nsel * 16, 256));
+            if (ff->t2f32)
+                hipLaunchKernelGGL(k_sk_back_fin_t2<float>, gf, dim3(256), 0, c->st, fa, c->capP, c->npol, c->cntp,
+                                   c->idxp, (const float *)ff->t2);
+            else
+                hipLaunchKernelGGL(k_sk_back_fin_t2<double>, gf, dim3(256), 0, c->st, fa, c->capP, c->npol, c->cntp,
+                                   c->idxp, (const double *)ff->t2);
+        } else {
+            hipLaunchKernelGGL(k_sk_back_fin, dim3(cdiv(nsel, 256)), dim3(256), 0, c->st, fa);
+        }
     }
 }
 
@@ -1059,15 +1072,10 @@ static bool induced_field(mdff_ctx *c, int lo, const double *mu, const double *b
 {
     int n = c->n;
     if (c->t2_ok) {
-        unsigned blocks = cdiv((size_t)n * 16, 256);
-        if (lo)
-            hipLaunchKernelGGL((k_t2_field<float, 16>), dim3(blocks), dim3(256), 0, c->st, n, c->capP, c->npol,
-                               c->pinv, c->cntp, c->idxp, c->t2f, mu, c->ef_dir, act);
-        else
-            hipLaunchKernelGGL((k_t2_field<double, 16>), dim3(blocks), dim3(256), 0, c->st, n, c->capP, c->npol,
-                               c->pinv, c->cntp, c->idxp, c->t2, mu, c->ef_dir, act);
-        // the total field at the polarisable ions comes out of the back-projection
-        FieldFuse ff{ef, mu, base, pq_part};
+        // the total field at the polarisable ions comes out of the back-
+        // projection's finish, which also forms the real-space field of mu
+        // from the pair tensors (16 lanes per polarisable ion)
+        FieldFuse ff{ef, mu, base, pq_part, lo ? (const void *)c->t2f : (const void *)c->t2, lo ? 1 : 0};
         rec_part(c, 1, lo, 3, 3, c->qz, mu, c->qz, mu, 1, 0, 0, nullptr, act, &ff);
         return pq_part != nullptr;
     }
@@ -1284,7 +1292,9 @@ static int scf_loop_anchored(mdff_ctx *c, int istep)
     // split-FP16 steps: the kernels that write p also give the channel
     // maxima of p that scale the forward sum's weights
     double *wp = (c->mf_inner == KM_F16X3 && !(c->cg_precond && c->t2_ok && c->t2f != nullptr)) ? c->gwp : nullptr;
-    double *rz[2] = {c->cg_part, c->cg_part + nb}, *pq = c->cg_part + 2 * nb, *rm = c->cg_part + 3 * nb;
+    double *rz[2] = {c->cg_part, c->cg_part + nb}, *rm = c->cg_part + 3 * nb;
+    // p.q partials: per 16 ions from the fused finish, else per CG block
+    double *pq = c->pq_buf;
     // preconditioner: z = pol^T r, or with c->cg_precond (and the FP32 pair
     // tensors) z = w + pol^T (T_dir w + 2 selfa2 w), w = pol^T r
     const bool pc = c->cg_precond && c->t2_ok && c->t2f != nullptr;
@@ -1313,15 +1323,16 @@ static int scf_loop_anchored(mdff_ctx *c, int istep)
         c->wpart_ready = wp != nullptr;
         const bool fused = induced_field(c, 1, c->cg_p, nullptr, c->dc_g, act, pq);
         c->mf_lo = keep;
+        const int nbq = fused ? cdiv(np, 16) : nb;
         if (!fused)
             hipLaunchKernelGGL(k_cg_pq, dim3(nb), dim3(CG_BLOCK), 0, c->st, n, np, c->pidx, c->type, c->tab_d,
                                c->cg_p, c->dc_g, pq, act);
         if (pc) {
-            hipLaunchKernelGGL(k_cga_update_pc, dim3(nb), dim3(CG_BLOCK), 0, c->st, n, np, nb, c->pidx, c->type,
+            hipLaunchKernelGGL(k_cga_update_pc, dim3(nb), dim3(CG_BLOCK), 0, c->st, n, np, nb, nbq, c->pidx, c->type,
                                c->tab_d, c->cg_p, c->dc_g, c->mu_ind, c->ef, c->dc_R, c->cg_w, rz[cur], pq, rm, act);
             precond(rz[cur ^ 1], 0);
         } else {
-            hipLaunchKernelGGL(k_cga_update, dim3(nb), dim3(CG_BLOCK), 0, c->st, n, np, nb, c->pidx, c->type,
+            hipLaunchKernelGGL(k_cga_update, dim3(nb), dim3(CG_BLOCK), 0, c->st, n, np, nb, nbq, c->pidx, c->type,
                                c->tab_d, c->cg_p, c->dc_g, c->mu_ind, c->ef, c->dc_R, c->cg_z, rz[cur], pq,
                                rz[cur ^ 1], rm, act);
         }
@@ -1699,7 +1710,7 @@ void mdff_destroy(mdff_ctx *c)
     double **dp[] = {&c->r, &c->v, &c->f, &c->fold, &c->rs, &c->s, &c->s0, &c->invmass, &c->massd, &c->q, &c->qz,
                      &c->mu_static, &c->mu_ind, &c->mu_tot, &c->hist, &c->ef_stat, &c->ef, &c->ef_dir, &c->ef_rec,
                      &c->efg, &c->efg_dir, &c->efg_rec, &c->f_sr, &c->f_dir, &c->f_rec, &c->acc_sr, &c->uw_dir,
-                     &c->selfbuf, &c->ke, &c->upol_i, &c->rmsd_i, &c->tmp_q, &c->tmp_mu, &c->tmp_qraw, &c->tmp_mur, &c->dc_R, &c->dc_d, &c->dc_g, &c->cg_z, &c->cg_p, &c->cg_part, &c->chk_part, &c->ko_f, &c->ko_dmu, &c->t2ko, &c->cg_w, &c->scal,
+                     &c->selfbuf, &c->ke, &c->upol_i, &c->rmsd_i, &c->tmp_q, &c->tmp_mu, &c->tmp_qraw, &c->tmp_mur, &c->dc_R, &c->dc_d, &c->dc_g, &c->cg_z, &c->cg_p, &c->cg_part, &c->chk_part, &c->pq_buf, &c->ko_f, &c->ko_dmu, &c->t2ko, &c->cg_w, &c->scal,
                      &c->kvec, &c->Ak, &c->kcoe, &c->W, &c->ek, &c->kpart, &c->bpart, &c->ksum, &c->gsc, &c->gwp};
     for (auto pp : dp) dfree(*pp);
     dfree(c->type);
@@ -1862,6 +1873,7 @@ int mdff_create(mdff_ctx **out, const mdff_params *p, int nion, const int *typia
     ALLOC(c->dc_R, 3 * n); ALLOC(c->dc_d, 3 * n); ALLOC(c->dc_g, 3 * n);
     ALLOC(c->cg_z, 3 * n); ALLOC(c->cg_p, 3 * n); ALLOC(c->cg_part, 4 * (size_t)cdiv(n, CG_BLOCK));
     ALLOC(c->chk_part, 2 * (size_t)cdiv(n, 256));
+    ALLOC(c->pq_buf, (size_t)cdiv(n, 16));
     ALLOC(c->ko_f, 3 * n); ALLOC(c->ko_dmu, 3 * n); ALLOC(c->cg_w, 3 * n);
     ALLOC(c->scal, S_COUNT); ALLOC(c->ctl, C_COUNT); ALLOC(c->tab_d, 1);
     if (hipHostMalloc((void **)&c->ctl_h, sizeof(int) * C_COUNT) != hipSuccess) { mdff_destroy(c); return MDFF_ERR_NOMEM; }

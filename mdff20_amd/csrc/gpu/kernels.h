@@ -761,7 +761,7 @@ struct BackFinArgs {
 
 // sum the splits, map n-moments to k-moments and write field, gradient and
 // force of multipole_ES_rec ewald.c:957-992 (times 8 pi / V, :1011-1034)
-__device__ __forceinline__ double back_fin_one(const BackFinArgs &a, int t)
+__device__ __forceinline__ double back_fin_one(const BackFinArgs &a, int t, const double *dir = nullptr)
 {
     int i = a.idx ? a.idx[t] : t;
     double H[3] = {0, 0, 0}, M[6] = {0, 0, 0, 0, 0, 0};
@@ -790,7 +790,7 @@ __device__ __forceinline__ double back_fin_one(const BackFinArgs &a, int t)
         double v[3];
         for (int c = 0; c < 3; c++) {
             const size_t o = (size_t)c * n + i;
-            v[c] = 2.0 * a.selfa2 * a.mu_self[o] + a.ef_dir[o] + a.tfpiV * E[c];
+            v[c] = 2.0 * a.selfa2 * a.mu_self[o] + (dir ? dir[c] : a.ef_dir[o]) + a.tfpiV * E[c];
             if (a.base) v[c] += a.base[o];
             a.ef_total[o] = v[c];
         }
@@ -832,6 +832,55 @@ __global__ void __launch_bounds__(256) k_sk_back_fin(BackFinArgs a)
     if (a.active && !*a.active) return;
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     const double pq = t < a.nsel ? back_fin_one(a, t) : 0.0;
+    if (!a.pq_part) return;
+    __shared__ double sh[256];
+    sh[threadIdx.x] = pq;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) a.pq_part[blockIdx.x] = sh[0];
+}
+
+// k_sk_back_fin with the real-space field of the dipoles mu_self formed here
+// from the pair tensors of k_t2_build (the sum of k_t2_field): 16 lanes per
+// selected polarisable ion (selection = the polarisable rows in order) sum
+// its tensor row, and lane 0 finishes the ion with that field in place of
+// ef_dir.  pq partials per block of 16 ions.
+template <typename real>
+__global__ void __launch_bounds__(256) k_sk_back_fin_t2(BackFinArgs a, int capP, int npol,
+                                                        const int *__restrict__ cntp, const int *__restrict__ idxp,
+                                                        const real *__restrict__ t2)
+{
+    if (a.active && !*a.active) return;
+    constexpr int L = 16;
+    const int lane = threadIdx.x % L;
+    const int t = (blockIdx.x * blockDim.x + threadIdx.x) / L;
+    const int n = a.n;
+    const double *mu = a.mu_self;
+    real ex = 0, ey = 0, ez = 0;
+    if (t < a.nsel) {
+        const int i = a.idx ? a.idx[t] : t;
+        const bool dip_i = (mu[i] != 0.0) || (mu[n + i] != 0.0) || (mu[2 * n + i] != 0.0);
+        if (dip_i) {
+            const size_t plane = (size_t)npol * capP;
+            const size_t row = (size_t)t * capP;
+            const int m = cntp[t];
+            for (int e = lane; e < m; e += L) {
+                const size_t o = row + e;
+                const int j = idxp[o];
+                const real mx = (real)mu[j], my = (real)mu[n + j], mz = (real)mu[2 * n + j];
+                const real txx = t2[o], txy = t2[plane + o], txz = t2[2 * plane + o];
+                const real tyy = t2[3 * plane + o], tyz = t2[4 * plane + o], tzz = t2[5 * plane + o];
+                ex += txx * mx + txy * my + txz * mz;
+                ey += txy * mx + tyy * my + tyz * mz;
+                ez += txz * mx + tyz * my + tzz * mz;
+            }
+        }
+    }
+    const double d[3] = {group_sum<L>((double)ex), group_sum<L>((double)ey), group_sum<L>((double)ez)};
+    const double pq = (t < a.nsel && lane == 0) ? back_fin_one(a, t, d) : 0.0;
     if (!a.pq_part) return;
     __shared__ double sh[256];
     sh[threadIdx.x] = pq;

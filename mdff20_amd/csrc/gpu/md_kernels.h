@@ -650,7 +650,7 @@ __global__ void k_cg_apply(int n, int npol, const int *__restrict__ pidx, const 
 // of the predictor (the low-precision T only acts on the increments).  The
 // residual r = E - pol^-T mu and the rmsd of get_rmsd_mu (pim.c:431-449)
 // are formed from the updated mu and E on every iteration.
-__global__ void __launch_bounds__(CG_BLOCK) k_cga_update(int n, int npol, int nb, const int *__restrict__ pidx,
+__global__ void __launch_bounds__(CG_BLOCK) k_cga_update(int n, int npol, int nb, int nbq, const int *__restrict__ pidx,
                                                          const int *__restrict__ type, const TypeTab *__restrict__ T,
                                                          const double *__restrict__ p, const double *__restrict__ Tp,
                                                          double *mu, double *ef, double *r, double *z,
@@ -661,7 +661,7 @@ __global__ void __launch_bounds__(CG_BLOCK) k_cga_update(int n, int npol, int nb
     if (!*active) return;
     __shared__ double sh[CG_BLOCK];
     const double rz = cg_sum_partials(rz_old, nb, sh);
-    const double pq = cg_sum_partials(pq_part, nb, sh);
+    const double pq = cg_sum_partials(pq_part, nbq, sh);
     const double g = (pq != 0.0) ? rz / pq : 0.0;
     const int k = blockIdx.x * CG_BLOCK + threadIdx.x;
     double acc = 0.0, rms = 0.0;
@@ -852,7 +852,8 @@ __global__ void k_ko_resid(int n, int npol, const int *__restrict__ pidx, const 
 // real-space dipole tensors and the self term for T), symmetric for a
 // symmetric polarisability.  The update writes w = pol^T r; the field of w
 // comes from k_t2_field; k_cg_precond forms z and the partial sums of r.z.
-__global__ void __launch_bounds__(CG_BLOCK) k_cga_update_pc(int n, int npol, int nb, const int *__restrict__ pidx,
+__global__ void __launch_bounds__(CG_BLOCK) k_cga_update_pc(int n, int npol, int nb, int nbq,
+                                                            const int *__restrict__ pidx,
                                                             const int *__restrict__ type,
                                                             const TypeTab *__restrict__ T,
                                                             const double *__restrict__ p,
@@ -864,7 +865,7 @@ __global__ void __launch_bounds__(CG_BLOCK) k_cga_update_pc(int n, int npol, int
     if (!*active) return;
     __shared__ double sh[CG_BLOCK];
     const double rz = cg_sum_partials(rz_old, nb, sh);
-    const double pq = cg_sum_partials(pq_part, nb, sh);
+    const double pq = cg_sum_partials(pq_part, nbq, sh);
     const double g = (pq != 0.0) ? rz / pq : 0.0;
     const int k = blockIdx.x * CG_BLOCK + threadIdx.x;
     double rms = 0.0;
