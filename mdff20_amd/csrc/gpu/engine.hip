@@ -356,22 +356,31 @@ static DirArgs dir_args(mdff_ctx *c, const double *q, const double *mu, int lcou
 }
 
 template <typename real>
-static void launch_eikr_t(mdff_ctx *c, const int *idx, int nsel, void *tx, void *ty, void *tz, const int *active)
+static void launch_eikr_t(mdff_ctx *c, const int *idx, int nsel, void *tx, void *ty, void *tz, const int *active,
+                          void **tp)
 {
     hipLaunchKernelGGL(k_eikr<real>, dim3(cdiv(nsel, 128)), dim3(128), 0, c->st, c->n, c->cd, c->Kx, c->Ky, c->Kz,
-                       c->r, c->r + c->n, c->r + 2 * c->n, idx, nsel, (real *)tx, (real *)ty, (real *)tz, active);
+                       c->r, c->r + c->n, c->r + 2 * c->n, idx, nsel, (real *)tx, (real *)ty, (real *)tz, active,
+                       tp ? c->pinv : nullptr, c->npol, tp ? (real *)tp[0] : nullptr, tp ? (real *)tp[1] : nullptr,
+                       tp ? (real *)tp[2] : nullptr);
 }
 
-static void launch_eikr(mdff_ctx *c, int polar, int lo, const int *active)
+// the e^{in theta} tables of one ion set; for the all-ion set with
+// with_polar also the polarisable-ion tables of the same precision (one pass)
+static void launch_eikr(mdff_ctx *c, int polar, int lo, const int *active, bool with_polar = false)
 {
+    with_polar = with_polar && !polar && c->npol > 0;
     // stored operands built from these tables are stale now
-    if (lo) { c->yz_ok[KM_F32][polar] = 0; c->yz_ok[KM_BF16X3][polar] = 0; c->yz_ok[KM_F16X3][polar] = 0; }
-    else c->yz_ok[KM_F64][polar] = 0;
+    for (int ps = polar; ps <= (with_polar ? 1 : polar); ps++) {
+        if (lo) { c->yz_ok[KM_F32][ps] = 0; c->yz_ok[KM_BF16X3][ps] = 0; c->yz_ok[KM_F16X3][ps] = 0; }
+        else c->yz_ok[KM_F64][ps] = 0;
+    }
     void **t = c->tab3[lo][polar];
+    void **tp = with_polar ? c->tab3[lo][1] : nullptr;
     const int *idx = polar ? c->pidx : nullptr;
     int nsel = polar ? c->npol : c->n;
-    if (lo) launch_eikr_t<float>(c, idx, nsel, t[0], t[1], t[2], active);
-    else launch_eikr_t<double>(c, idx, nsel, t[0], t[1], t[2], active);
+    if (lo) launch_eikr_t<float>(c, idx, nsel, t[0], t[1], t[2], active, tp);
+    else launch_eikr_t<double>(c, idx, nsel, t[0], t[1], t[2], active, tp);
 }
 
 template <typename real, int C>
@@ -1489,11 +1498,11 @@ static int engforce_impl(mdff_ctx *c, int istep)
     if (c->p.lcoulombic) {
         // k-space tables at the current positions
         int lo = lo_of(c);
-        launch_eikr(c, 0, lo, nullptr);
-        if (c->lpol && c->npol > 0) {
-            if (scf_uses_fp64(c)) launch_eikr(c, 1, 0, nullptr);
-            if (lo) launch_eikr(c, 1, 1, nullptr);
-        }
+        // all ions, and in the same pass the polarisable ions' tables of
+        // this precision; the FP64 ones too when the mixed SCF uses them
+        const bool pol = c->lpol && c->npol > 0;
+        launch_eikr(c, 0, lo, nullptr, pol);
+        if (pol && lo && scf_uses_fp64(c)) launch_eikr(c, 1, 0, nullptr);
         // get_dipoles coulombic.c:67-130
         if (c->lpol) {
             if ((rc = build_t2(c)) != MDFF_OK) return rc;
@@ -2239,9 +2248,9 @@ int mdff_get_dipoles_scf(mdff_ctx *c, int istep, double *mu, double *upol)
     hipLaunchKernelGGL(k_frac, dim3(cdiv(n, 256)), dim3(256), 0, c->st, n, c->cd, c->r, c->r + n, c->r + 2 * n, c->s);
     if (c->lpol) {
         int lo = lo_of(c);
-        launch_eikr(c, 0, lo, nullptr);
-        if (scf_uses_fp64(c)) launch_eikr(c, 1, 0, nullptr);
-        if (lo) launch_eikr(c, 1, 1, nullptr);
+        const bool pol = c->npol > 0;
+        launch_eikr(c, 0, lo, nullptr, pol);
+        if (pol && lo && scf_uses_fp64(c)) launch_eikr(c, 1, 0, nullptr);
         if ((rc = build_t2(c)) != MDFF_OK) return rc;
         static_field(c, lo);
         rc = c->p.algo_pim == MDFF_PIM_SCFKO ? scf_loop_ko(c, istep) : (lo ? scf_mixed(c, istep) : scf_loop(c, istep));

@@ -434,44 +434,52 @@ __global__ void __launch_bounds__(256) k_ewald_dir(DirArgs a)
 // k = 2 pi sum_b n_b B[.][b] (kspace.c:87-89) so k.r = 2 pi sum_b n_b t_b,
 // t_b = sum_a B[a][b] r_a.  Tables hold n = 0..Kx for x and -K..K for y, z.
 template <typename real>
+__device__ __forceinline__ void eikr_store(real *t, size_t o, double cs, double sn)
+{
+    typedef real r2 __attribute__((ext_vector_type(2)));
+    *(r2 *)(t + o) = r2{(real)cs, (real)sn};
+}
+
+template <typename real>
 __global__ void k_eikr(int n, CellD c, int Kx, int Ky, int Kz, const double *__restrict__ rx,
                        const double *__restrict__ ry, const double *__restrict__ rz, const int *__restrict__ idx,
                        int nsel, real *__restrict__ tx, real *__restrict__ ty, real *__restrict__ tz,
-                       const int *active)
+                       const int *active, const int *__restrict__ pinv = nullptr, int npol = 0,
+                       real *__restrict__ px = nullptr, real *__restrict__ py = nullptr, real *__restrict__ pz = nullptr)
 {
     if (active && !*active) return;
     int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= nsel) return;
     int i = idx ? idx[k] : k;
+    // with px: the same rows also into the polarisable-ion tables (row pinv[i])
+    const int pr = px ? pinv[i] : -1;
     double x = rx[i], y = ry[i], z = rz[i];
     double t[3];
     for (int b = 0; b < 3; b++) {
         double v = x * c.B[0][b] + y * c.B[1][b] + z * c.B[2][b];
         t[b] = v - rint(v);
     }
-    double th = MDFF_TPI * t[0];
-    for (int m = 0; m <= Kx; m++) {
-        double sn, cs;
-        sincos(th * (double)m, &sn, &cs);
-        tx[2 * ((size_t)m * nsel + k)] = (real)cs;
-        tx[2 * ((size_t)m * nsel + k) + 1] = (real)sn;
-    }
-    th = MDFF_TPI * t[1];
-    for (int m = -Ky; m <= Ky; m++) {
-        double sn, cs;
-        sincos(th * (double)m, &sn, &cs);
-        size_t o = 2 * ((size_t)(m + Ky) * nsel + k);
-        ty[o] = (real)cs;
-        ty[o + 1] = (real)sn;
-    }
-    th = MDFF_TPI * t[2];
-    for (int m = -Kz; m <= Kz; m++) {
-        double sn, cs;
-        sincos(th * (double)m, &sn, &cs);
-        size_t o = 2 * ((size_t)(m + Kz) * nsel + k);
-        tz[o] = (real)cs;
-        tz[o + 1] = (real)sn;
-    }
+    // e^{i m th} for m = 0..K by the recurrence e^{i(m+1)th} = e^{i m th} e^{i th}
+    // in FP64 (relative error below (m + 1) 2^-52), e^{-i m th} as its conjugate
+    auto axis = [&](double th, int K, int off, real *tk, real *tp) {
+        double s1, c1;
+        sincos(th, &s1, &c1);
+        double cs = 1.0, sn = 0.0;
+        for (int m = 0; m <= K; m++) {
+            eikr_store(tk, 2 * ((size_t)(off + m) * nsel + k), cs, sn);
+            if (pr >= 0) eikr_store(tp, 2 * ((size_t)(off + m) * npol + pr), cs, sn);
+            if (off > 0 && m > 0) {
+                eikr_store(tk, 2 * ((size_t)(off - m) * nsel + k), cs, -sn);
+                if (pr >= 0) eikr_store(tp, 2 * ((size_t)(off - m) * npol + pr), cs, -sn);
+            }
+            const double cn = cs * c1 - sn * s1, snn = sn * c1 + cs * s1;
+            cs = cn;
+            sn = snn;
+        }
+    };
+    axis(MDFF_TPI * t[0], Kx, 0, tx, px);
+    axis(MDFF_TPI * t[1], Ky, Ky, ty, py);
+    axis(MDFF_TPI * t[2], Kz, Kz, tz, pz);
 }
 
 struct SkArgs {
