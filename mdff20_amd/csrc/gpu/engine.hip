@@ -1059,14 +1059,16 @@ static int build_t2(mdff_ctx *c, bool fp64 = false)
     a.pidx = c->pidx; a.pinv = c->pinv; a.alpha = c->alpha; a.rc2 = c->lrcutsq; a.npol = c->npol;
     a.cntp = c->cntp; a.idxp = c->idxp; a.t2f = lo_of(c) ? c->t2f : nullptr; a.overflow = c->t2_ovf;
     a.t2 = (fp64 || !a.t2f || scf_uses_fp64(c)) ? c->t2 : nullptr;
-    hipLaunchKernelGGL(k_t2_build, dim3(cdiv(c->npol, 4)), dim3(256), 0, c->st, a);
+    // FP32 arithmetic when only the FP32 tensors are built
+    if (a.t2) hipLaunchKernelGGL(k_t2_build<double>, dim3(cdiv(c->npol, 4)), dim3(256), 0, c->st, a);
+    else hipLaunchKernelGGL(k_t2_build<float>, dim3(cdiv(c->npol, 4)), dim3(256), 0, c->st, a);
     if (ko) {
         // the same pairs at the alpha of induced_moment_inner_kO (pim.c:693-694);
         // the pair lists come out identical
         a.alpha = KO_ALPHA;
         a.t2 = c->t2ko;
         a.t2f = nullptr;
-        hipLaunchKernelGGL(k_t2_build, dim3(cdiv(c->npol, 4)), dim3(256), 0, c->st, a);
+        hipLaunchKernelGGL(k_t2_build<double>, dim3(cdiv(c->npol, 4)), dim3(256), 0, c->st, a);
     }
     c->t2_ok = 1;
     return MDFF_OK;

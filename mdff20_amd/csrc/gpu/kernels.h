@@ -963,6 +963,9 @@ struct T2Args {
 };
 
 // one wavefront per polarisable ion, ballot-compacted rows
+// real: the arithmetic of the tensor values (float when only t2f is wanted;
+// the pair selection is FP64 either way)
+template <typename real>
 __global__ void __launch_bounds__(256) k_t2_build(T2Args a)
 {
     const int lane = threadIdx.x & 63;
@@ -991,14 +994,16 @@ __global__ void __launch_bounds__(256) k_t2_build(T2Args a)
                 double d2 = x * x + y * y + z * z;
                 if (d2 <= a.rc2) {
                     keep = true;
-                    const double d = sqrt(d2), dm1 = 1.0 / d, dm3 = dm1 / d2, dm5 = dm3 / d2;
-                    const double expon = exp(-al2 * d2) * rpi;
-                    const double F0 = erfc(al * d);
-                    const double F1 = F0 + 2.0 * al * d * expon;
-                    const double F2 = F1 + 4.0 * al3 * d2 * d * expon / 3.0;
-                    const double a2 = 3.0 * F2 * dm5, b2 = F1 * dm3;
-                    T[0] = a2 * x * x - b2; T[1] = a2 * x * y; T[2] = a2 * x * z;
-                    T[3] = a2 * y * y - b2; T[4] = a2 * y * z; T[5] = a2 * z * z - b2;
+                    const real rx = (real)x, ry = (real)y, rz = (real)z, r2 = (real)d2;
+                    const real ral = (real)al, ral2 = (real)al2, ral3 = (real)al3;
+                    const real d = sqrt(r2), dm1 = (real)1 / d, dm3 = dm1 / r2, dm5 = dm3 / r2;
+                    const real expon = exp(-ral2 * r2) * (real)rpi;
+                    const real F0 = fast_erfc(ral * d);
+                    const real F1 = F0 + (real)2 * ral * d * expon;
+                    const real F2 = F1 + (real)4 * ral3 * r2 * d * expon / (real)3;
+                    const real a2 = (real)3 * F2 * dm5, b2 = F1 * dm3;
+                    T[0] = a2 * rx * rx - b2; T[1] = a2 * rx * ry; T[2] = a2 * rx * rz;
+                    T[3] = a2 * ry * ry - b2; T[4] = a2 * ry * rz; T[5] = a2 * rz * rz - b2;
                 }
             }
         }
