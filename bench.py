@@ -291,7 +291,7 @@ def trace_summarize(d, top=40):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--units", type=int, default=4096, help="SiO2 units (N = 3 x units)")
     ap.add_argument("--relax", type=int, default=300)

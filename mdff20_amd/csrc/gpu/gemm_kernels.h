@@ -502,25 +502,34 @@ __global__ void __launch_bounds__(256) k_w_max(int nsel, int C, const int *__res
                                                double *__restrict__ wpart, const int *active)
 {
     if (active && !*active) return;
-    __shared__ double red[4][256];
+    __shared__ double red[4][4];   // [wave][channel]
     const int j = blockIdx.x * 256 + threadIdx.x;
     const int ion = (j < nsel) ? (idx ? idx[j] : j) : 0;
-    for (int c = 0; c < C; c++) {
-        double w = 0.0;
-        if (j < nsel) {
-            if (C == 1) w = q[ion];
-            else if (C == 3) w = mu[(size_t)c * n + ion];
-            else w = c == 0 ? q[ion] : mu[(size_t)(c - 1) * n + ion];
+    // all channels' loads first, then one wave reduction per channel
+    double w[4] = {0.0, 0.0, 0.0, 0.0};
+    if (j < nsel) {
+        if (C == 1) {
+            w[0] = q[ion];
+        } else if (C == 3) {
+#pragma unroll
+            for (int c = 0; c < 3; c++) w[c] = mu[(size_t)c * n + ion];
+        } else {
+            w[0] = q[ion];
+#pragma unroll
+            for (int c = 0; c < 3; c++) w[c + 1] = mu[(size_t)c * n + ion];
         }
-        red[c][threadIdx.x] = isfinite(w) ? fabs(w) : 0.0;
+    }
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+        double x = isfinite(w[c]) ? fabs(w[c]) : 0.0;
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) x = fmax(x, __shfl_xor(x, o));
+        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6][c] = x;
     }
     __syncthreads();
-    for (int o = 128; o > 0; o >>= 1) {
-        if ((int)threadIdx.x < o)
-            for (int c = 0; c < C; c++) red[c][threadIdx.x] = fmax(red[c][threadIdx.x], red[c][threadIdx.x + o]);
-        __syncthreads();
-    }
-    if ((int)threadIdx.x < C) wpart[(size_t)threadIdx.x * gridDim.x + blockIdx.x] = red[threadIdx.x][0];
+    if ((int)threadIdx.x < C)
+        wpart[(size_t)threadIdx.x * gridDim.x + blockIdx.x] =
+            fmax(fmax(red[0][threadIdx.x], red[1][threadIdx.x]), fmax(red[2][threadIdx.x], red[3][threadIdx.x]));
 }
 
 // WM columns as k_wm_build, each (nx, m) pair scaled by a power of two

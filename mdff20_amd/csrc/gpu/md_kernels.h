@@ -265,39 +265,64 @@ struct NhcArgs {
 };
 
 // chain_nhcn integration.c:189-294 on the thermostat scalars (one thread;
-// the forward sweep leaves vxi[n-2] unchanged, integration.c:269-277)
+// the forward sweep leaves vxi[n-2] unchanged, integration.c:269-277).  The
+// chain is held in registers: loops run over the maximum length 16 with
+// constant indices, guarded by the chain length.
 __global__ void k_nhc_chain(NhcArgs a, double *scal)
 {
-    double *vxi = scal + S_VXI, *xi = scal + S_XI;
+    constexpr int NH = 16;
     const int nh = a.nh;
-    double G[16];
-    for (int i = 0; i < nh; i++) G[i] = 0.0;
-    double kin = scal[S_KIN];
+    double v[NH], x[NH], G[NH];
+#pragma unroll
+    for (int i = 0; i < NH; i++) {
+        v[i] = i < nh ? scal[S_VXI + i] : 0.0;
+        x[i] = i < nh ? scal[S_XI + i] : 0.0;
+        G[i] = 0.0;
+    }
+    const double kin = scal[S_KIN];
     double s = 1.0;
     G[0] = 2.0 * kin - a.L * a.kT;
     for (int k = 0; k < a.mults; k++) {
         for (int j = 0; j < a.yo; j++) {
-            double dts = (j < 7 ? a.w[j] : 0.0) * a.dt / ((double)a.mults);
-            double dts2 = dts * 0.5, dts4 = dts2 * 0.5, dts8 = dts4 * 0.5;
-            G[nh - 1] = (vxi[nh - 2] * vxi[nh - 2] / a.Q[nh - 2] - a.kT);
-            vxi[nh - 1] += G[nh - 1] * dts4;
-            for (int ih = nh - 2; ih > -1; ih--) {
-                vxi[ih] *= exp(-vxi[ih + 1] * dts8 / a.Q[ih + 1]);
-                vxi[ih] += G[ih] * dts4;
-                vxi[ih] *= exp(-vxi[ih + 1] * dts8 / a.Q[ih + 1]);
-            }
-            s *= exp(-vxi[0] * dts2 / a.Q[0]);
-            for (int i = 0; i < nh; i++) xi[i] += vxi[i] * dts2 / a.Q[i];
+            const double dts = (j < 7 ? a.w[j] : 0.0) * a.dt / ((double)a.mults);
+            const double dts2 = dts * 0.5, dts4 = dts2 * 0.5, dts8 = dts4 * 0.5;
+#pragma unroll
+            for (int i = 1; i < NH; i++)
+                if (i == nh - 1) {
+                    G[i] = (v[i - 1] * v[i - 1] / a.Q[i - 1] - a.kT);
+                    v[i] += G[i] * dts4;
+                }
+#pragma unroll
+            for (int ih = NH - 2; ih > -1; ih--)
+                if (ih <= nh - 2) {
+                    v[ih] *= exp(-v[ih + 1] * dts8 / a.Q[ih + 1]);
+                    v[ih] += G[ih] * dts4;
+                    v[ih] *= exp(-v[ih + 1] * dts8 / a.Q[ih + 1]);
+                }
+            s *= exp(-v[0] * dts2 / a.Q[0]);
+#pragma unroll
+            for (int i = 0; i < NH; i++)
+                if (i < nh) x[i] += v[i] * dts2 / a.Q[i];
             G[0] = 2.0 * kin * s * s - a.L * a.kT;
-            for (int ih = 0; ih < nh - 2; ih++) {
-                vxi[ih] *= exp(-vxi[ih + 1] * dts8 / a.Q[ih + 1]);
-                vxi[ih] += G[ih] * dts4;
-                vxi[ih] *= exp(-vxi[ih + 1] * dts8 / a.Q[ih + 1]);
-                G[ih + 1] = (vxi[ih] * vxi[ih] / a.Q[ih] - a.kT);
-            }
-            vxi[nh - 1] += G[nh - 1] * dts4;
+#pragma unroll
+            for (int ih = 0; ih < NH - 2; ih++)
+                if (ih < nh - 2) {
+                    v[ih] *= exp(-v[ih + 1] * dts8 / a.Q[ih + 1]);
+                    v[ih] += G[ih] * dts4;
+                    v[ih] *= exp(-v[ih + 1] * dts8 / a.Q[ih + 1]);
+                    G[ih + 1] = (v[ih] * v[ih] / a.Q[ih] - a.kT);
+                }
+#pragma unroll
+            for (int i = 1; i < NH; i++)
+                if (i == nh - 1) v[i] += G[i] * dts4;
         }
     }
+#pragma unroll
+    for (int i = 0; i < NH; i++)
+        if (i < nh) {
+            scal[S_VXI + i] = v[i];
+            scal[S_XI + i] = x[i];
+        }
     scal[S_NHC_S] = s;
 }
 
