@@ -185,6 +185,33 @@ def test_anchored_cg_anchor_and_step_modes(case, anchor, inner):
         assert cnt["scf_fp64"] == (1 if anchor == 0 and cnt["npol"] > 0 else 0)
 
 
+@pytest.mark.parametrize("case", ["pim192_static", "bsio_200_nvt"])
+def test_mixed_form_on_a_flat_k_box(case):
+    """a k box with Kz = 0 (one ez row per ion): a forward-sum block's table
+    slice no longer fits its LDS staging area, so the split-FP16 forward sum
+    takes the stored point-major operand instead; the mixed form still
+    agrees with the FP64 form of the same k box"""
+    print(f"[gpu] flat k box {case}", flush=True)
+    p, cfg, _ = load_case(case)
+    p.lautoES = 0
+    p.kES[0] = max(int(p.kES[0]), 3)
+    p.kES[1] = max(int(p.kES[1]), 3)
+    p.kES[2] = 0
+    p.conv_tol_ind = 1e-9
+    p.max_scf_pim_iter = 400
+    p.min_scf_pim_iter = 1
+    with m.Engine(p, cfg.typia, cfg.A, cfg.r, precision=m.FP64) as ef, \
+            m.Engine(p, cfg.typia, cfg.A, cfg.r, precision=m.MIXED) as em:
+        tf, tm = ef.engforce(0), em.engforce(0)
+        assert tf["status"] == 0 and tm["status"] == 0
+        Ff, Fm = ef.forces(), em.forces()
+        frms = np.sqrt(np.mean((Fm - Ff) ** 2)) / np.sqrt(np.mean(Ff ** 2))
+        assert frms < 1e-5, frms
+        if tf["u_pol"]:
+            assert rel(em.dipoles(), ef.dipoles()) < 1e-5
+        assert tm["u_tot"] == pytest.approx(tf["u_tot"], rel=1e-6)
+
+
 @pytest.mark.parametrize("case", ["pim192_static", "bsio_200_nvt", "melt216_nve"])
 def test_mfma_kspace_matches_vector_kernels_fp32(case):
     """FP32 S(k) forward and back-projection on the matrix cores against the
