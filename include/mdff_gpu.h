@@ -121,7 +121,8 @@ int mdff_md_run(mdff_ctx *ctx, int first_step, int nsteps, mdff_thermo *th);
  * tt_damp.c:16-32), host evaluation of the exact device formula. */
 void mdff_tt_damping(double b, double c, double r, double *f, double *fd, int order);
 
-/* Tuning: lanes per ion in the neighbour kernels (16, 32 or 64; 0 keeps),
+/* Tuning: lanes per ion in the neighbour kernels and the CG steps' pair-tensor
+ * field (8, 16, 32 or 64, default 16; 0 keeps),
  * reciprocal-space kernel family katoms (-1 vector ALU; 1 the streaming
  * matrix-core sums, split-FP16 SCF steps in the mixed form, the default; 2
  * the same with FP32 SCF steps; 3 the older matrix-core kernels that form

@@ -120,7 +120,7 @@ struct mdff_ctx {
     int nchunk = 0, nsplit = 0, chunk = 0, rows_per_split = 0;
     int nchunk_p = 0, chunk_p = 0;
     // tuning
-    int lanes = 32;
+    int lanes = 16;           // lanes per ion of the pair kernels and the pair-tensor field
     int use_mfma = 1;         // S(k) forward and back-projection on the matrix cores
     int gemm = 1;             // matrix-core sums over stored operands (gemm_kernels.h)
     int g_cw[2] = {1, 2};     // column tiles per wave of the forward GEMM (FP64, FP32/BF16)
@@ -138,7 +138,7 @@ struct mdff_ctx {
     double *gwp = nullptr;    // split-FP16 per-block weight channel maxima [4][n/256]
     unsigned long long *gcm = nullptr;   // split-FP16 WM column maxima [(Kx+1) 6] (double bits)
     double *chk_part = nullptr;          // SCF rmsd / u_pol partial sums [2][n/256]
-    double *pq_buf = nullptr;            // CG p.q partial sums [n/16]
+    double *pq_buf = nullptr;            // CG p.q partial sums, one per fin_t2 block [n/4]
     struct GTune { int mode, dir, tiles, polar, blocks; };
     std::vector<GTune> gtune;  // block targets of the stored-operand sums, timed on first use
     int mf_lo = KM_F16X3;     // matrix-core mode of the FP32-table evaluations (static field, forces)
@@ -199,6 +199,10 @@ static void set_int(mdff_ctx *c, int slot, int val)
 {
     hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, c->st, c->ctl + slot, val);
 }
+
+// lanes per polarisable ion of k_sk_back_fin_t2 (8, 16, 32 or 64; at most 64,
+// which sizes pq_buf)
+static int fin_t2_lanes(const mdff_ctx *c) { return c->lanes; }
 
 static void colsum(mdff_ctx *c, const double *in, int n, int ld, int ncols, double *out, const int *active = nullptr)
 {
@@ -296,7 +300,8 @@ static void launch_dir_t(mdff_ctx *c, const DirArgs &a)
 {
     int L = c->lanes;
     unsigned blocks = cdiv((size_t)c->n * L, 256);
-    if (L == 16) hipLaunchKernelGGL((k_ewald_dir<real, 16, EF, EQ, EM, F>), dim3(blocks), dim3(256), 0, c->st, a);
+    if (L == 8) hipLaunchKernelGGL((k_ewald_dir<real, 8, EF, EQ, EM, F>), dim3(blocks), dim3(256), 0, c->st, a);
+    else if (L == 16) hipLaunchKernelGGL((k_ewald_dir<real, 16, EF, EQ, EM, F>), dim3(blocks), dim3(256), 0, c->st, a);
     else if (L == 64) hipLaunchKernelGGL((k_ewald_dir<real, 64, EF, EQ, EM, F>), dim3(blocks), dim3(256), 0, c->st, a);
     else hipLaunchKernelGGL((k_ewald_dir<real, 32, EF, EQ, EM, F>), dim3(blocks), dim3(256), 0, c->st, a);
 }
@@ -643,24 +648,31 @@ static int gtune_blocks(mdff_ctx *c, int mode, int dir, int tiles, int polar, F 
         if (t.mode == mode && t.dir == dir && t.tiles == tiles && t.polar == polar) return t.blocks;
     const int cand[6] = {c->ncu / 2, 3 * c->ncu / 4, c->ncu, 3 * c->ncu / 2, 2 * c->ncu, 3 * c->ncu};
     int best = c->ncu;
+    // time on a quiet device: real-space kernels still running on the second
+    // stream would slow some candidates and not others
+    if (c->st2) (void)hipStreamSynchronize(c->st2);
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (hipEventCreate(&e0) == hipSuccess && hipEventCreate(&e1) == hipSuccess) {
-        float bt = 1e30f;
-        int last = -1;
-        for (int k = 0; k < 6; k++) {
-            const int splits = run(cand[k], 0);   // the split count this target gives (0: no launch)
-            if (splits == last) continue;         // same grid as the previous candidate
-            last = splits;
-            run(cand[k], 1);
-            (void)hipEventRecord(e0, c->st);
-            for (int r = 0; r < 3; r++) run(cand[k], 1);
-            (void)hipEventRecord(e1, c->st);
-            float ms = 0.0f;
-            if (hipEventSynchronize(e1) == hipSuccess && hipEventElapsedTime(&ms, e0, e1) == hipSuccess && ms < bt) {
-                bt = ms;
-                best = cand[k];
+        // two sweeps over the candidates; each candidate keeps its best time
+        float bt[6] = {1e30f, 1e30f, 1e30f, 1e30f, 1e30f, 1e30f};
+        for (int sweep = 0; sweep < 2; sweep++) {
+            int last = -1;
+            for (int k = 0; k < 6; k++) {
+                const int splits = run(cand[k], 0);   // the split count this target gives (0: no launch)
+                if (splits == last) continue;         // same grid as the previous candidate
+                last = splits;
+                run(cand[k], 1);
+                (void)hipEventRecord(e0, c->st);
+                for (int r = 0; r < 3; r++) run(cand[k], 1);
+                (void)hipEventRecord(e1, c->st);
+                float ms = 0.0f;
+                if (hipEventSynchronize(e1) == hipSuccess && hipEventElapsedTime(&ms, e0, e1) == hipSuccess)
+                    bt[k] = std::min(bt[k], ms);
             }
         }
+        float b = 1e30f;
+        for (int k = 0; k < 6; k++)
+            if (bt[k] < b) { b = bt[k]; best = cand[k]; }
     }
     if (e0) (void)hipEventDestroy(e0);
     if (e1) (void)hipEventDestroy(e1);
@@ -938,13 +950,21 @@ static void rec_part(mdff_ctx *c, int polar, int lo, int C, int NCH, const doubl
         fa.T = c->tab_d;
         fa.pq_part = ff ? ff->pq_part : nullptr;
         if (ff && ff->t2) {
-            const dim3 gf(cdiv((size_t)nsel * 16, 256));
-            if (ff->t2f32)
-                hipLaunchKernelGGL(k_sk_back_fin_t2<float>, gf, dim3(256), 0, c->st, fa, c->capP, c->npol, c->cntp,
-                                   c->idxp, (const float *)ff->t2);
-            else
-                hipLaunchKernelGGL(k_sk_back_fin_t2<double>, gf, dim3(256), 0, c->st, fa, c->capP, c->npol, c->cntp,
-                                   c->idxp, (const double *)ff->t2);
+            const int L = fin_t2_lanes(c);
+            const dim3 gf(cdiv((size_t)nsel * L, 256));
+            if (ff->t2f32) {
+                const float *t2 = (const float *)ff->t2;
+                if (L == 8) hipLaunchKernelGGL((k_sk_back_fin_t2<float, 8>), gf, dim3(256), 0, c->st, fa, c->capP, c->npol, c->cntp, c->idxp, t2);
+                else if (L == 64) hipLaunchKernelGGL((k_sk_back_fin_t2<float, 64>), gf, dim3(256), 0, c->st, fa, c->capP, c->npol, c->cntp, c->idxp, t2);
+                else if (L == 32) hipLaunchKernelGGL((k_sk_back_fin_t2<float, 32>), gf, dim3(256), 0, c->st, fa, c->capP, c->npol, c->cntp, c->idxp, t2);
+                else hipLaunchKernelGGL((k_sk_back_fin_t2<float, 16>), gf, dim3(256), 0, c->st, fa, c->capP, c->npol, c->cntp, c->idxp, t2);
+            } else {
+                const double *t2 = (const double *)ff->t2;
+                if (L == 8) hipLaunchKernelGGL((k_sk_back_fin_t2<double, 8>), gf, dim3(256), 0, c->st, fa, c->capP, c->npol, c->cntp, c->idxp, t2);
+                else if (L == 64) hipLaunchKernelGGL((k_sk_back_fin_t2<double, 64>), gf, dim3(256), 0, c->st, fa, c->capP, c->npol, c->cntp, c->idxp, t2);
+                else if (L == 32) hipLaunchKernelGGL((k_sk_back_fin_t2<double, 32>), gf, dim3(256), 0, c->st, fa, c->capP, c->npol, c->cntp, c->idxp, t2);
+                else hipLaunchKernelGGL((k_sk_back_fin_t2<double, 16>), gf, dim3(256), 0, c->st, fa, c->capP, c->npol, c->cntp, c->idxp, t2);
+            }
         } else {
             hipLaunchKernelGGL(k_sk_back_fin, dim3(cdiv(nsel, 256)), dim3(256), 0, c->st, fa);
         }
@@ -961,7 +981,10 @@ static void short_range(mdff_ctx *c)
     int L = c->lanes;
     unsigned blocks = cdiv((size_t)n * L, 256);
     bool mixed = lo_of(c);
-    if (L == 16) {
+    if (L == 8) {
+        if (mixed) hipLaunchKernelGGL((k_short<float, 8>), dim3(blocks), dim3(256), 0, c->st, a);
+        else hipLaunchKernelGGL((k_short<double, 8>), dim3(blocks), dim3(256), 0, c->st, a);
+    } else if (L == 16) {
         if (mixed) hipLaunchKernelGGL((k_short<float, 16>), dim3(blocks), dim3(256), 0, c->st, a);
         else hipLaunchKernelGGL((k_short<double, 16>), dim3(blocks), dim3(256), 0, c->st, a);
     } else if (L == 64) {
@@ -1334,7 +1357,7 @@ static int scf_loop_anchored(mdff_ctx *c, int istep)
         c->wpart_ready = wp != nullptr;
         const bool fused = induced_field(c, 1, c->cg_p, nullptr, c->dc_g, act, pq);
         c->mf_lo = keep;
-        const int nbq = fused ? cdiv(np, 16) : nb;
+        const int nbq = fused ? cdiv(np * fin_t2_lanes(c), 256) : nb;
         if (!fused)
             hipLaunchKernelGGL(k_cg_pq, dim3(nb), dim3(CG_BLOCK), 0, c->st, n, np, c->pidx, c->type, c->tab_d,
                                c->cg_p, c->dc_g, pq, act);
@@ -1884,7 +1907,7 @@ int mdff_create(mdff_ctx **out, const mdff_params *p, int nion, const int *typia
     ALLOC(c->dc_R, 3 * n); ALLOC(c->dc_d, 3 * n); ALLOC(c->dc_g, 3 * n);
     ALLOC(c->cg_z, 3 * n); ALLOC(c->cg_p, 3 * n); ALLOC(c->cg_part, 4 * (size_t)cdiv(n, CG_BLOCK));
     ALLOC(c->chk_part, 2 * (size_t)cdiv(n, 256));
-    ALLOC(c->pq_buf, (size_t)cdiv(n, 16));
+    ALLOC(c->pq_buf, (size_t)cdiv(n, 4));
     ALLOC(c->ko_f, 3 * n); ALLOC(c->ko_dmu, 3 * n); ALLOC(c->cg_w, 3 * n);
     ALLOC(c->scal, S_COUNT); ALLOC(c->ctl, C_COUNT); ALLOC(c->tab_d, 1);
     if (hipHostMalloc((void **)&c->ctl_h, sizeof(int) * C_COUNT) != hipSuccess) { mdff_destroy(c); return MDFF_ERR_NOMEM; }
@@ -2098,7 +2121,7 @@ int mdff_set_kspace_modes(mdff_ctx *c, int eval_mode, int inner_mode)
 int mdff_set_tuning(mdff_ctx *c, int lanes, int katoms, int ksplit)
 {
     if (!c) return MDFF_ERR_ARG;
-    if (lanes == 16 || lanes == 32 || lanes == 64) c->lanes = lanes;
+    if (lanes == 8 || lanes == 16 || lanes == 32 || lanes == 64) c->lanes = lanes;
     else if (lanes != 0) return MDFF_ERR_ARG;
     // katoms: reciprocal-space kernel family.  -1 vector ALU; 1 matrix-core
     // sums over stored operands with split-BF16 SCF correction sweeps; 2 the

@@ -851,18 +851,54 @@ __global__ void __launch_bounds__(256) k_sk_back_fin(BackFinArgs a)
     if (threadIdx.x == 0) a.pq_part[blockIdx.x] = sh[0];
 }
 
+// One ion's T2 row: the field sum_j T2_ij mu_j over its m pair entries,
+// lanes lane, lane + L, ... of L; U entries per lane per pass, so every
+// index, tensor and dipole load of a pass is in flight together.
+template <typename real, int L>
+__device__ __forceinline__ void t2_row_sum(size_t row, int m, int lane, size_t plane, const int *__restrict__ idxp,
+                                           const real *__restrict__ t2, const double *__restrict__ mu, int n,
+                                           real &ex, real &ey, real &ez)
+{
+    constexpr int U = 4;
+    for (int e0 = lane; e0 < m; e0 += U * L) {
+        int j[U];
+        real tt[U][6];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int e = e0 + u * L;
+            j[u] = -1;
+#pragma unroll
+            for (int q = 0; q < 6; q++) tt[u][q] = 0;
+            if (e < m) {
+                const size_t o = row + e;
+                j[u] = idxp[o];
+#pragma unroll
+                for (int q = 0; q < 6; q++) tt[u][q] = t2[q * plane + o];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            if (j[u] < 0) continue;
+            const int jj = j[u];
+            const real mx = (real)mu[jj], my = (real)mu[n + jj], mz = (real)mu[2 * n + jj];
+            ex += tt[u][0] * mx + tt[u][1] * my + tt[u][2] * mz;
+            ey += tt[u][1] * mx + tt[u][3] * my + tt[u][4] * mz;
+            ez += tt[u][2] * mx + tt[u][4] * my + tt[u][5] * mz;
+        }
+    }
+}
+
 // k_sk_back_fin with the real-space field of the dipoles mu_self formed here
-// from the pair tensors of k_t2_build (the sum of k_t2_field): 16 lanes per
+// from the pair tensors of k_t2_build (the sum of k_t2_field): L lanes per
 // selected polarisable ion (selection = the polarisable rows in order) sum
 // its tensor row, and lane 0 finishes the ion with that field in place of
-// ef_dir.  pq partials per block of 16 ions.
-template <typename real>
+// ef_dir.  pq partials per block of 256 / L ions.
+template <typename real, int L>
 __global__ void __launch_bounds__(256) k_sk_back_fin_t2(BackFinArgs a, int capP, int npol,
                                                         const int *__restrict__ cntp, const int *__restrict__ idxp,
                                                         const real *__restrict__ t2)
 {
     if (a.active && !*a.active) return;
-    constexpr int L = 16;
     const int lane = threadIdx.x % L;
     const int t = (blockIdx.x * blockDim.x + threadIdx.x) / L;
     const int n = a.n;
@@ -875,16 +911,7 @@ __global__ void __launch_bounds__(256) k_sk_back_fin_t2(BackFinArgs a, int capP,
             const size_t plane = (size_t)npol * capP;
             const size_t row = (size_t)t * capP;
             const int m = cntp[t];
-            for (int e = lane; e < m; e += L) {
-                const size_t o = row + e;
-                const int j = idxp[o];
-                const real mx = (real)mu[j], my = (real)mu[n + j], mz = (real)mu[2 * n + j];
-                const real txx = t2[o], txy = t2[plane + o], txz = t2[2 * plane + o];
-                const real tyy = t2[3 * plane + o], tyz = t2[4 * plane + o], tzz = t2[5 * plane + o];
-                ex += txx * mx + txy * my + txz * mz;
-                ey += txy * mx + tyy * my + tyz * mz;
-                ez += txz * mx + tyz * my + tzz * mz;
-            }
+            t2_row_sum<real, L>(row, m, lane, plane, idxp, t2, mu, n, ex, ey, ez);
         }
     }
     const double d[3] = {group_sum<L>((double)ex), group_sum<L>((double)ey), group_sum<L>((double)ez)};
@@ -1049,16 +1076,7 @@ __global__ void __launch_bounds__(256) k_t2_field(int n, int capP, int npol, con
         const size_t plane = (size_t)npol * capP;
         const size_t row = (size_t)pr * capP;
         const int m = cntp[pr];
-        for (int e = lane; e < m; e += LANES) {
-            const size_t o = row + e;
-            const int j = idxp[o];
-            const real mx = (real)mu[j], my = (real)mu[n + j], mz = (real)mu[2 * n + j];
-            const real txx = t2[o], txy = t2[plane + o], txz = t2[2 * plane + o];
-            const real tyy = t2[3 * plane + o], tyz = t2[4 * plane + o], tzz = t2[5 * plane + o];
-            ex += txx * mx + txy * my + txz * mz;
-            ey += txy * mx + tyy * my + tyz * mz;
-            ez += txz * mx + tyz * my + tzz * mz;
-        }
+        t2_row_sum<real, LANES>(row, m, lane, plane, idxp, t2, mu, n, ex, ey, ez);
     }
     double dx = group_sum<LANES>((double)ex), dy = group_sum<LANES>((double)ey), dz = group_sum<LANES>((double)ez);
     if (lane == 0) {

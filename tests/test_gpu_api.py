@@ -55,7 +55,7 @@ def test_pair_short_and_dipoles_match_oracle():
         assert upol == pytest.approx(to["u_pol"], rel=1e-10)
 
 
-@pytest.mark.parametrize("lanes", [16, 64])
+@pytest.mark.parametrize("lanes", [8, 16, 32, 64])
 def test_lane_widths_agree(lanes):
     p, cfg, _ = load_case("pim192_static")
     with m.Engine(p, cfg.typia, cfg.A, cfg.r) as e0, m.Engine(p, cfg.typia, cfg.A, cfg.r) as e1:
@@ -63,6 +63,20 @@ def test_lane_widths_agree(lanes):
         t0, t1 = e0.engforce(0), e1.engforce(0)
         assert rel(e0.forces(), e1.forces()) < 1e-12
         assert t0["u_tot"] == pytest.approx(t1["u_tot"], rel=1e-13)
+
+
+@pytest.mark.parametrize("lanes", [8, 16, 32, 64])
+def test_lane_widths_agree_mixed(lanes):
+    """the mixed form's pair kernels and its CG steps' pair-tensor field at
+    every lane width: only the FP32 summation order differs"""
+    p, cfg, _ = load_case("pim192_static")
+    with m.Engine(p, cfg.typia, cfg.A, cfg.r, precision=m.MIXED) as e0, \
+            m.Engine(p, cfg.typia, cfg.A, cfg.r, precision=m.MIXED) as e1:
+        e1.set_tuning(lanes=lanes)
+        t0, t1 = e0.engforce(0), e1.engforce(0)
+        assert rel(e0.forces(), e1.forces()) < 1e-6
+        assert rel(e0.dipoles(), e1.dipoles()) < 1e-6
+        assert t0["u_tot"] == pytest.approx(t1["u_tot"], rel=1e-8)
 
 
 @pytest.mark.parametrize("katoms", [1, 2, -1])
