@@ -139,6 +139,7 @@ struct mdff_ctx {
     unsigned long long *gcm = nullptr;   // split-FP16 WM column maxima [(Kx+1) 6] (double bits)
     double *chk_part = nullptr;          // SCF rmsd / u_pol partial sums [2][n/256]
     double *pq_buf = nullptr;            // CG p.q partial sums, one per fin_t2 block [n/4]
+    double *t2dir = nullptr;             // pair-tensor field of a CG direction (back-projection's extra rows) [3n]
     struct GTune { int mode, dir, tiles, polar, blocks; };
     std::vector<GTune> gtune;  // block targets of the stored-operand sums, timed on first use
     int mf_lo = KM_F16X3;     // matrix-core mode of the FP32-table evaluations (static field, forces)
@@ -422,8 +423,17 @@ static int fwd_mfma_t(mdff_ctx *c, SkMArgs &m, int nsel)
 
 static int launch_fwd_gemm(mdff_ctx *c, int polar, int lo, int C, const double *q_w, const double *mu_w,
                            const int *active, int *nchunk_out);
+// a request to the back-projection for the pair-tensor field of mu at the
+// polarisable ions (FP32 tensors) in extra grid rows of its launch; done is
+// set when the launch took them
+struct T2Rows {
+    const float *t2;
+    const double *mu;
+    double *out;
+    int done;
+};
 static int launch_back_gemm(mdff_ctx *c, int polar, int lo, int NCH, const int *active, int *nsplit_out,
-                            bool wm_built);
+                            bool wm_built, T2Rows *t2r);
 static int wm_alloc(mdff_ctx *c, int mode);
 
 static int launch_fwd_mfma(mdff_ctx *c, int polar, int lo, int C, const double *q_w, const double *mu_w,
@@ -463,12 +473,12 @@ static int back_mfma_t(mdff_ctx *c, BkMArgs &b, int nsel)
 
 // back-projection on the matrix cores; returns 0 when outside the kernel's shapes
 static int launch_back_mfma(mdff_ctx *c, int polar, int lo, int NCH, const int *active, int *nsplit_out,
-                            bool wm_built = false)
+                            bool wm_built = false, T2Rows *t2r = nullptr)
 {
     int CH = NCH == 9 ? 6 : 3;
     int NC = (c->Kx + 1) * CH * 2;
     if (!c->use_mfma || NC > MB_CT * 16) return 0;
-    if (c->gemm && launch_back_gemm(c, polar, lo, NCH, active, nsplit_out, wm_built)) return 1;
+    if (c->gemm && launch_back_gemm(c, polar, lo, NCH, active, nsplit_out, wm_built, t2r)) return 1;
     int nsel = polar ? c->npol : c->n;
     BkMArgs b;
     b.nsel = nsel; b.Kx = c->Kx; b.Ky = c->Ky; b.Kz = c->Kz; b.P = c->P; b.CH = CH; b.NC = NC;
@@ -785,7 +795,7 @@ static int wm_alloc(mdff_ctx *c, int mode)
 // back-projection over stored operands (WM columns formed here unless
 // wm_built: k_sk_reduce_wm wrote them); 0 when outside the kernel's shapes
 static int launch_back_gemm(mdff_ctx *c, int polar, int lo, int NCH, const int *active, int *nsplit_out,
-                            bool wm_built)
+                            bool wm_built, T2Rows *t2r)
 {
     const int CH = NCH == 9 ? 6 : 3;
     const int NC = (c->Kx + 1) * CH * 2;
@@ -812,7 +822,7 @@ static int launch_back_gemm(mdff_ctx *c, int polar, int lo, int NCH, const int *
     else
         hipLaunchKernelGGL(k_wm_build<KM_BF16X3>, gw, dim3(256), 0, c->st, c->P, c->Pk, c->Ky, c->Kz, CH, c->W,
                            c->wm[mode][0], c->wm[mode][1], active);
-    GBackArgs b;
+    GBackArgs b{};
     b.yzt = c->yzt[mode][polar][0]; b.yzt_lo = c->yzt[mode][polar][1];
     b.gty = (const float *)c->tab3[lo][polar][1]; b.gtz = (const float *)c->tab3[lo][polar][2];
     b.NY = 2 * c->Ky + 1; b.NZ = 2 * c->Kz + 1;
@@ -822,6 +832,14 @@ static int launch_back_gemm(mdff_ctx *c, int polar, int lo, int NCH, const int *
     b.nsel = nsel; b.CH = CH; b.NC = NC; b.CT = CT; b.ldk = 2 * c->Pk;
     b.part = c->bpart; b.active = active;
     const int gx = nselp / 128;
+    // pair-tensor rows: 32 polarisable ions per block over gx blocks a row
+    const bool t2rows = t2r && polar && c->t2_ok;
+    const int nt2 = t2rows ? cdiv(c->npol, 32 * gx) : 0;
+    if (t2rows) {
+        b.t2_npol = c->npol; b.t2_capP = c->capP; b.t2_n = c->n;
+        b.t2 = t2r->t2; b.t2_cnt = c->cntp; b.t2_idx = c->idxp; b.t2_pidx = c->pidx;
+        b.t2mu = t2r->mu; b.t2out = t2r->out;
+    }
     // K split to reach `blocks` blocks (even step counts as in the forward
     // sum; ldk = 2 Pk is a multiple of 2 GK); with go = 0 only the count
     auto run = [&](int blocks, int go, const int *act, bool probe) {
@@ -831,7 +849,8 @@ static int launch_back_gemm(mdff_ctx *c, int polar, int lo, int NCH, const int *
         gb2.kchunk = ((gb2.ldk + nsp - 1) / nsp + 2 * GK - 1) / (2 * GK) * (2 * GK);
         nsp = (gb2.ldk + gb2.kchunk - 1) / gb2.kchunk;
         if (!go) return nsp;
-        dim3 grid(gx, nsp);
+        gb2.t2_y0 = nsp;
+        dim3 grid(gx, nsp + nt2);
         if (probe) probe_mark(c, 0);
         if (mode == KM_F64) gback_ct<KM_F64, double>(c, grid, CT, gb2);
         else if (mode == KM_F32) gback_ct<KM_F32, float>(c, grid, CT, gb2);
@@ -846,6 +865,7 @@ static int launch_back_gemm(mdff_ctx *c, int polar, int lo, int NCH, const int *
                                       });
     const int ns = run(bb, 1, active, true);
     *nsplit_out = ns;
+    if (t2rows) t2r->done = 1;
     return 1;
 }
 
@@ -862,10 +882,11 @@ struct FieldFuse {
     int t2f32;         // t2 in FP32
 };
 
-static void rec_part(mdff_ctx *c, int polar, int lo, int C, int NCH, const double *q_w, const double *mu_w,
-                     const double *q_f, const double *mu_f, int do_ef, int do_f, int do_stress, double *efg_out,
-                     const int *active, const FieldFuse *ff = nullptr)
+static int rec_part(mdff_ctx *c, int polar, int lo, int C, int NCH, const double *q_w, const double *mu_w,
+                    const double *q_f, const double *mu_f, int do_ef, int do_f, int do_stress, double *efg_out,
+                    const int *active, const FieldFuse *ff = nullptr)
 {
+    int npq = 0;
     int nsel = polar ? c->npol : c->n;
     int chunk = polar ? c->chunk_p : c->chunk;
     int nchunk = polar ? c->nchunk_p : c->nchunk;
@@ -931,7 +952,10 @@ static void rec_part(mdff_ctx *c, int polar, int lo, int C, int NCH, const doubl
         b.part = c->bpart;
         b.active = active;
         int nsplit = c->nsplit;
-        if (!launch_back_mfma(c, polar, lo, NCH, active, &nsplit, wm_built)) {
+        // the FP32 pair-tensor field of the finish in extra rows of the
+        // back-projection's launch, when it takes them
+        T2Rows t2r{ff && ff->t2 && ff->t2f32 ? (const float *)ff->t2 : nullptr, ff ? ff->mu : nullptr, c->t2dir, 0};
+        if (!launch_back_mfma(c, polar, lo, NCH, active, &nsplit, wm_built, t2r.t2 ? &t2r : nullptr)) {
             if (NCH == 9) { if (mixed) launch_back_t<float, 9>(c, b, nsel); else launch_back_t<double, 9>(c, b, nsel); }
             else { if (mixed) launch_back_t<float, 3>(c, b, nsel); else launch_back_t<double, 3>(c, b, nsel); }
         }
@@ -949,8 +973,14 @@ static void rec_part(mdff_ctx *c, int polar, int lo, int C, int NCH, const doubl
         fa.type = c->type;
         fa.T = c->tab_d;
         fa.pq_part = ff ? ff->pq_part : nullptr;
-        if (ff && ff->t2) {
+        if (t2r.done) {
+            // the pair-tensor field came out of the back-projection's launch
+            fa.ef_dir = c->t2dir;
+            hipLaunchKernelGGL(k_sk_back_fin, dim3(cdiv(nsel, 256)), dim3(256), 0, c->st, fa);
+            if (fa.pq_part) npq = cdiv(nsel, 256);
+        } else if (ff && ff->t2) {
             const int L = fin_t2_lanes(c);
+            if (fa.pq_part) npq = cdiv(nsel * L, 256);
             const dim3 gf(cdiv((size_t)nsel * L, 256));
             if (ff->t2f32) {
                 const float *t2 = (const float *)ff->t2;
@@ -967,8 +997,10 @@ static void rec_part(mdff_ctx *c, int polar, int lo, int C, int NCH, const doubl
             }
         } else {
             hipLaunchKernelGGL(k_sk_back_fin, dim3(cdiv(nsel, 256)), dim3(256), 0, c->st, fa);
+            if (fa.pq_part) npq = cdiv(nsel, 256);
         }
     }
+    return npq;
 }
 
 static void short_range(mdff_ctx *c)
@@ -1099,26 +1131,26 @@ static int build_t2(mdff_ctx *c, bool fp64 = false)
 
 // induced field of dipoles mu (no charges, no damping, pim.c:258-260) at
 // the polarisable ions: ef = base + self + dir + rec
-// with pq_part, the back-projection also writes the partial sums of
-// mu . (pol^-T mu - ef) per 256 polarisable ions and true is returned
-static bool induced_field(mdff_ctx *c, int lo, const double *mu, const double *base, double *ef, const int *act,
-                          double *pq_part = nullptr)
+// with pq_part, the finish also writes the partial sums of
+// mu . (pol^-T mu - ef); returns how many (0 when none were written)
+static int induced_field(mdff_ctx *c, int lo, const double *mu, const double *base, double *ef, const int *act,
+                         double *pq_part = nullptr)
 {
     int n = c->n;
     if (c->t2_ok) {
         // the total field at the polarisable ions comes out of the back-
-        // projection's finish, which also forms the real-space field of mu
-        // from the pair tensors (16 lanes per polarisable ion)
+        // projection's finish, with the real-space field of mu from the
+        // pair tensors (FP32 tensors: extra rows of the back-projection's
+        // launch; else formed in the finish)
         FieldFuse ff{ef, mu, base, pq_part, lo ? (const void *)c->t2f : (const void *)c->t2, lo ? 1 : 0};
-        rec_part(c, 1, lo, 3, 3, c->qz, mu, c->qz, mu, 1, 0, 0, nullptr, act, &ff);
-        return pq_part != nullptr;
+        return rec_part(c, 1, lo, 3, 3, c->qz, mu, c->qz, mu, 1, 0, 0, nullptr, act, &ff);
     }
     DirArgs a = dir_args(c, c->qz, mu, 0, 1, act);
     launch_dir(c, a, lo, 1, 0, 0, 0);
     rec_part(c, 1, lo, 3, 3, c->qz, mu, c->qz, mu, 1, 0, 0, nullptr, act);
     hipLaunchKernelGGL(k_field_sum, dim3(cdiv(3 * n, 256)), dim3(256), 0, c->st, n, c->selfa2, mu, c->ef_dir,
                        c->ef_rec, base, ef, act);
-    return false;
+    return 0;
 }
 
 // rmsd of mu_ind against ef (and u_pol when upol is given) and the loop
@@ -1327,7 +1359,8 @@ static int scf_loop_anchored(mdff_ctx *c, int istep)
     // maxima of p that scale the forward sum's weights
     double *wp = (c->mf_inner == KM_F16X3 && !(c->cg_precond && c->t2_ok && c->t2f != nullptr)) ? c->gwp : nullptr;
     double *rz[2] = {c->cg_part, c->cg_part + nb}, *rm = c->cg_part + 3 * nb;
-    // p.q partials: per 16 ions from the fused finish, else per CG block
+    // p.q partials: as many as the finish reports (per block of its launch),
+    // else k_cg_pq's per CG block
     double *pq = c->pq_buf;
     // preconditioner: z = pol^T r, or with c->cg_precond (and the FP32 pair
     // tensors) z = w + pol^T (T_dir w + 2 selfa2 w), w = pol^T r
@@ -1355,10 +1388,10 @@ static int scf_loop_anchored(mdff_ctx *c, int istep)
         int keep = c->mf_lo;
         c->mf_lo = c->mf_inner;
         c->wpart_ready = wp != nullptr;
-        const bool fused = induced_field(c, 1, c->cg_p, nullptr, c->dc_g, act, pq);
+        const int npq = induced_field(c, 1, c->cg_p, nullptr, c->dc_g, act, pq);
         c->mf_lo = keep;
-        const int nbq = fused ? cdiv(np * fin_t2_lanes(c), 256) : nb;
-        if (!fused)
+        const int nbq = npq > 0 ? npq : nb;
+        if (npq == 0)
             hipLaunchKernelGGL(k_cg_pq, dim3(nb), dim3(CG_BLOCK), 0, c->st, n, np, c->pidx, c->type, c->tab_d,
                                c->cg_p, c->dc_g, pq, act);
         if (pc) {
@@ -1744,7 +1777,7 @@ void mdff_destroy(mdff_ctx *c)
     double **dp[] = {&c->r, &c->v, &c->f, &c->fold, &c->rs, &c->s, &c->s0, &c->invmass, &c->massd, &c->q, &c->qz,
                      &c->mu_static, &c->mu_ind, &c->mu_tot, &c->hist, &c->ef_stat, &c->ef, &c->ef_dir, &c->ef_rec,
                      &c->efg, &c->efg_dir, &c->efg_rec, &c->f_sr, &c->f_dir, &c->f_rec, &c->acc_sr, &c->uw_dir,
-                     &c->selfbuf, &c->ke, &c->upol_i, &c->rmsd_i, &c->tmp_q, &c->tmp_mu, &c->tmp_qraw, &c->tmp_mur, &c->dc_R, &c->dc_d, &c->dc_g, &c->cg_z, &c->cg_p, &c->cg_part, &c->chk_part, &c->pq_buf, &c->ko_f, &c->ko_dmu, &c->t2ko, &c->cg_w, &c->scal,
+                     &c->selfbuf, &c->ke, &c->upol_i, &c->rmsd_i, &c->tmp_q, &c->tmp_mu, &c->tmp_qraw, &c->tmp_mur, &c->dc_R, &c->dc_d, &c->dc_g, &c->cg_z, &c->cg_p, &c->cg_part, &c->chk_part, &c->pq_buf, &c->t2dir, &c->ko_f, &c->ko_dmu, &c->t2ko, &c->cg_w, &c->scal,
                      &c->kvec, &c->Ak, &c->kcoe, &c->W, &c->ek, &c->kpart, &c->bpart, &c->ksum, &c->gsc, &c->gwp};
     for (auto pp : dp) dfree(*pp);
     dfree(c->type);
@@ -1907,7 +1940,7 @@ int mdff_create(mdff_ctx **out, const mdff_params *p, int nion, const int *typia
     ALLOC(c->dc_R, 3 * n); ALLOC(c->dc_d, 3 * n); ALLOC(c->dc_g, 3 * n);
     ALLOC(c->cg_z, 3 * n); ALLOC(c->cg_p, 3 * n); ALLOC(c->cg_part, 4 * (size_t)cdiv(n, CG_BLOCK));
     ALLOC(c->chk_part, 2 * (size_t)cdiv(n, 256));
-    ALLOC(c->pq_buf, (size_t)cdiv(n, 4));
+    ALLOC(c->pq_buf, (size_t)cdiv(n, 4)); ALLOC(c->t2dir, 3 * n);
     ALLOC(c->ko_f, 3 * n); ALLOC(c->ko_dmu, 3 * n); ALLOC(c->cg_w, 3 * n);
     ALLOC(c->scal, S_COUNT); ALLOC(c->ctl, C_COUNT); ALLOC(c->tab_d, 1);
     if (hipHostMalloc((void **)&c->ctl_h, sizeof(int) * C_COUNT) != hipSuccess) { mdff_destroy(c); return MDFF_ERR_NOMEM; }

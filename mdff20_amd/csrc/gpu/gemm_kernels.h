@@ -926,7 +926,41 @@ struct GBackArgs {
     int nsel, CH, NC, CT, ldk, kchunk;
     double *part;              // [nsplit][3 or 9][nsel]
     const int *active;
+    // grid rows t2_y0.. (none when t2_y0 = gridDim.y): the real-space field
+    // of the dipoles t2mu at the polarisable ions from the FP32 pair tensors
+    // (rows of k_t2_build), 32 ions per block, into t2out [3][n]
+    int t2_y0, t2_npol, t2_capP, t2_n;
+    const float *t2;
+    const int *t2_cnt, *t2_idx, *t2_pidx;
+    const double *t2mu;
+    double *t2out;
 };
+
+// the pair-tensor blocks of k_gemm_back: 16 lanes per polarisable ion, as
+// k_sk_back_fin_t2; every polarisable ion's field is written (zero for an
+// ion without dipole)
+__device__ __forceinline__ void back_t2_rows(const GBackArgs &a)
+{
+    constexpr int L = 16;
+    const int lane = threadIdx.x % L;
+    const int t = (((int)blockIdx.y - a.t2_y0) * (int)gridDim.x + (int)blockIdx.x) * (512 / L) + (int)threadIdx.x / L;
+    const int n = a.t2_n;
+    const double *mu = a.t2mu;
+    float ex = 0, ey = 0, ez = 0;
+    int i = -1;
+    if (t < a.t2_npol) {
+        i = a.t2_pidx[t];
+        if ((mu[i] != 0.0) || (mu[n + i] != 0.0) || (mu[2 * n + i] != 0.0))
+            t2_row_sum<float, L>((size_t)t * a.t2_capP, a.t2_cnt[t], lane, (size_t)a.t2_npol * a.t2_capP, a.t2_idx,
+                                 a.t2, mu, n, ex, ey, ez);
+    }
+    const double dx = group_sum<L>((double)ex), dy = group_sum<L>((double)ey), dz = group_sum<L>((double)ez);
+    if (i >= 0 && lane == 0) {
+        a.t2out[i] = dx;
+        a.t2out[n + i] = dy;
+        a.t2out[2 * n + i] = dz;
+    }
+}
 
 // column tiles a back-projection kernel is instantiated for (every launch
 // computes this many; columns past NC are zero or ignored)
@@ -964,11 +998,17 @@ inline size_t g_back_lds_bytes(int CTINST, int KX1)
 }
 
 // back-projection: block = 8 waves x 16 ions, all CT column tiles, one K
-// range (grid.y); epilogue as k_sk_back_mfma (x ex[nx], moments)
+// range (grid.y); epilogue as k_sk_back_mfma (x ex[nx], moments).  Grid
+// rows from t2_y0 on run back_t2_rows instead (vector-memory work beside
+// the matrix-core blocks of the same launch).
 template <int MODE, int CTMAX, typename TT>
 __global__ void __launch_bounds__(512, ((MODE == KM_F64 || CTMAX >= 13) ? 2 : 4)) k_gemm_back(GBackArgs a)
 {
     if (a.active && !*a.active) return;
+    if ((int)blockIdx.y >= a.t2_y0) {
+        back_t2_rows(a);
+        return;
+    }
     using T = typename GT<MODE>::T;
     constexpr int LD = GK + GLDS_PAD;
     extern __shared__ __align__(16) unsigned char smem_raw[];

@@ -65,14 +65,18 @@ def test_lane_widths_agree(lanes):
         assert t0["u_tot"] == pytest.approx(t1["u_tot"], rel=1e-13)
 
 
+@pytest.mark.parametrize("katoms", [0, -1])
 @pytest.mark.parametrize("lanes", [8, 16, 32, 64])
-def test_lane_widths_agree_mixed(lanes):
+def test_lane_widths_agree_mixed(lanes, katoms):
     """the mixed form's pair kernels and its CG steps' pair-tensor field at
-    every lane width: only the FP32 summation order differs"""
+    every lane width: only the FP32 summation order differs.  katoms 0: the
+    tensor field in the back-projection's extra rows; -1: in the finish
+    kernel after the vector-ALU back-projection (lanes per ion apply)"""
     p, cfg, _ = load_case("pim192_static")
     with m.Engine(p, cfg.typia, cfg.A, cfg.r, precision=m.MIXED) as e0, \
             m.Engine(p, cfg.typia, cfg.A, cfg.r, precision=m.MIXED) as e1:
-        e1.set_tuning(lanes=lanes)
+        e0.set_tuning(katoms=katoms)
+        e1.set_tuning(lanes=lanes, katoms=katoms)
         t0, t1 = e0.engforce(0), e1.engforce(0)
         assert rel(e0.forces(), e1.forces()) < 1e-6
         assert rel(e0.dipoles(), e1.dipoles()) < 1e-6
