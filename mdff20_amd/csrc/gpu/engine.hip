@@ -116,7 +116,11 @@ struct mdff_ctx {
                            {{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}}};
     double *kpart = nullptr, *bpart = nullptr;
     double *ksum = nullptr;   // [nkf][8] chunk-summed forward partials of the matrix-core path
-    int fwd_part = 0;         // last forward launch: 0 partials in kpart (k_sk_reduce sums them), 1 summed into ksum
+    int fwd_part = 0;         // last forward launch: 0 partials in kpart (k_sk_reduce sums them), 1 summed into ksum,
+                              // 2 FP32 partials of a matrix-core sum left in kpart for k_psum_reduce_cm
+    bool psum_defer = false;  // set by rec_part: the split-FP16 forward leaves its partials (fwd_part 2)
+    int fwd_nch = 0;          // fwd_part 2: K ranges of the partials
+    const double *fwd_inv = nullptr;   // fwd_part 2: channel scales to undo
     int nchunk = 0, nsplit = 0, chunk = 0, rows_per_split = 0;
     int nchunk_p = 0, chunk_p = 0;
     // tuning
@@ -738,7 +742,7 @@ static int launch_fwd_gemm(mdff_ctx *c, int polar, int lo, int C, const double *
         hipLaunchKernelGGL(k_aw_build16<KM_BF16X3>, dim3(cdiv(nselp, 256 * 4), c->Kx + 1, C), dim3(256), 0, c->st, nsel,
                            nselp, C, (const float *)t[0], idx, q_w, mu_w, c->n, c->aw[mode][0], c->aw[mode][1], wsc,
                            nwb, c->gsc + 8, active);
-    GFwdArgs a;
+    GFwdArgs a{};
     a.aw = c->aw[mode][0]; a.aw_lo = c->aw[mode][1];
     a.yz = c->yz[mode][polar][0]; a.yz_lo = c->yz[mode][polar][1];
     a.gty = (const float *)t[1]; a.gtz = (const float *)t[2];
@@ -750,10 +754,14 @@ static int launch_fwd_gemm(mdff_ctx *c, int polar, int lo, int C, const double *
     // K split to reach `blocks` blocks (K ranges of an even number of steps:
     // the kernel alternates two register sets; ldk = 2 nselp is a multiple
     // of 2 GK), then the chunk sums; with go = 0 only the split count
-    auto run = [&](int blocks, int go, const int *act, bool probe) {
+    // defer: leave the FP32 partials to k_psum_reduce_cm (the column maxima
+    // it fills are zeroed by the sum's first block)
+    const bool defer = c->psum_defer && mode == KM_F16X3;
+    auto run = [&](int blocks, int go, const int *act, bool probe, bool psum = true) {
         int nch = std::max(1, std::min(64, (blocks + gx - 1) / gx));
         GFwdArgs ga2 = a;
         ga2.active = act;
+        if (!psum) { ga2.cmz = c->gcm; ga2.ncmz = 6 * (c->Kx + 1); }
         ga2.kchunk = ((ga2.ldk + nch - 1) / nch + 2 * GK - 1) / (2 * GK) * (2 * GK);
         nch = (ga2.ldk + ga2.kchunk - 1) / ga2.kchunk;
         if (!go) return nch;
@@ -764,6 +772,7 @@ static int launch_fwd_gemm(mdff_ctx *c, int polar, int lo, int C, const double *
         else if (mode == KM_F16X3) { if (gen) gfwd_cw<KM_F16X3, true>(c, grid, CW, MT, ga2); else gfwd_cw<KM_F16X3>(c, grid, CW, MT, ga2); }
         else { if (gen) gfwd_cw<KM_BF16X3, true>(c, grid, CW, MT, ga2); else gfwd_cw<KM_BF16X3>(c, grid, CW, MT, ga2); }
         if (probe) probe_mark(c, 1);
+        if (!psum) return nch;
         if (mode == KM_F64)
             hipLaunchKernelGGL(k_part_sum<double>, dim3(cdiv(nval, 256)), dim3(256), 0, c->st,
                                (const double *)c->kpart, nch, M, c->P, c->ksum, C, winv, act);
@@ -776,8 +785,10 @@ static int launch_fwd_gemm(mdff_ctx *c, int polar, int lo, int C, const double *
                                     : gtune_blocks(c, mode, 0, MT, polar, [&](int b, int go) {
                                           return run(b, go, nullptr, false);
                                       });
-    run(fb, 1, active, true);
-    c->fwd_part = 1;
+    const int nch = run(fb, 1, active, true, !defer);
+    c->fwd_part = defer ? 2 : 1;
+    c->fwd_nch = nch;
+    c->fwd_inv = winv;
     *nchunk_out = 1;
     return 1;
 }
@@ -905,7 +916,13 @@ static int rec_part(mdff_ctx *c, int polar, int lo, int C, int NCH, const double
     const int bmode = lo ? c->mf_lo : KM_F64;
     const int CHb = NCH == 9 ? 6 : 3;
     c->fwd_part = 0;
+    // split FP16 with a back-projection: the reduction sums the forward's
+    // partials itself (k_psum_reduce_cm)
+    const bool cm_path = (do_ef || do_f) && c->use_mfma && c->gemm && bmode == KM_F16X3 &&
+                         (c->Kx + 1) * CHb * 2 <= MB_CT * 16 && wm_alloc(c, bmode) == MDFF_OK;
+    c->psum_defer = cm_path && (C == 1 || C == 3 || C == 4);
     const bool fwd_mf = launch_fwd_mfma(c, polar, lo, C, q_w, mu_w, active, &nchunk);
+    c->psum_defer = false;
     c->wpart_ready = 0;
     if (!fwd_mf) {
         if (C == 1) { if (mixed) launch_fwd_t<float, 1>(c, a, nchunk); else launch_fwd_t<double, 1>(c, a, nchunk); }
@@ -915,20 +932,35 @@ static int rec_part(mdff_ctx *c, int polar, int lo, int C, int NCH, const double
     KRedArgs r;
     r.Kx = c->Kx; r.Ky = c->Ky; r.Kz = c->Kz; r.P = c->P; r.nchunk = nchunk; r.C = C;
     const bool gemm_fwd = c->fwd_part != 0;
+    const bool raw = c->fwd_part == 2;
     r.part = gemm_fwd ? c->ksum : c->kpart; r.kvec = c->kvec;
     c->fwd_part = 0; r.Ak = c->Ak; r.kcoe = c->kcoe; r.W = c->W; r.ek = c->ek;
     r.do_stress = do_stress; r.active = active;
     // the back-projection's WM columns come out of the reduction: for split
-    // FP16 (column-scaled) the reduction gives the column maxima (zeroed by
-    // the partial sums) and k_wm_scaled the columns; the other modes form
-    // them in the reduction (k_sk_reduce_wm)
+    // FP16 (column-scaled) the reduction gives the column maxima and
+    // k_wm_scaled the columns (the reduction sums the forward's partials
+    // itself when the forward left them, raw); the other modes form them in
+    // the reduction (k_sk_reduce_wm)
     bool wm_built = false;
-    if ((do_ef || do_f) && gemm_fwd && c->use_mfma && c->gemm && bmode == KM_F16X3 &&
-        (c->Kx + 1) * CHb * 2 <= MB_CT * 16 && wm_alloc(c, bmode) == MDFF_OK) {
+    if (cm_path && gemm_fwd) {
         void *w0 = c->wm[bmode][0], *w1 = c->wm[bmode][1];
         const dim3 gr(cdiv(c->nkf, 256));
-        if (CHb == 6) hipLaunchKernelGGL(k_sk_reduce_cm<6>, gr, dim3(256), 0, c->st, r, c->gcm);
-        else hipLaunchKernelGGL(k_sk_reduce_cm<3>, gr, dim3(256), 0, c->st, r, c->gcm);
+        if (raw) {
+            const dim3 gp(cdiv(c->nkf, PSUM_PB));
+            const int M = (c->Kx + 1) * 2 * C;
+            const float *kp = (const float *)c->kpart;
+            const int nch = c->fwd_nch;
+            const double *inv = c->fwd_inv;
+#define PSUM_CM(CC, HH) hipLaunchKernelGGL((k_psum_reduce_cm<CC, HH>), gp, dim3(256), 0, c->st, r, kp, nch, M, inv, c->gcm)
+            if (C == 1) { if (CHb == 6) PSUM_CM(1, 6); else PSUM_CM(1, 3); }
+            else if (C == 3) { if (CHb == 6) PSUM_CM(3, 6); else PSUM_CM(3, 3); }
+            else { if (CHb == 6) PSUM_CM(4, 6); else PSUM_CM(4, 3); }
+#undef PSUM_CM
+        } else if (CHb == 6) {
+            hipLaunchKernelGGL(k_sk_reduce_cm<6>, gr, dim3(256), 0, c->st, r, c->gcm);
+        } else {
+            hipLaunchKernelGGL(k_sk_reduce_cm<3>, gr, dim3(256), 0, c->st, r, c->gcm);
+        }
         hipLaunchKernelGGL(k_wm_scaled, dim3(cdiv(c->Pk, 256), (c->Kx + 1) * CHb), dim3(256), 0, c->st, c->P, c->Pk,
                            c->Ky, c->Kz, CHb, c->W, c->gcm, w0, w1, c->gsc + 16, active);
         wm_built = true;

@@ -464,6 +464,118 @@ __global__ void __launch_bounds__(256) k_sk_reduce_cm(KRedArgs a, unsigned long 
     }
 }
 
+// k_part_sum and k_sk_reduce_cm in one pass over the forward sum's FP32
+// partial sums part[nchunk][M][P] (rows (nx, c, re/im), M = (Kx+1) 2C):
+// PSUM_PB k-vectors per block, PSUM_G thread groups each summing every
+// PSUM_G-th K range of the 2C rows of a k-vector (loads coalesced over the
+// points), the groups added in a fixed order, channel c times inv[c]; then
+// the reduction of sk_reduce_S and the column maxima of k_sk_reduce_cm (cm
+// zeroed by the forward sum's block (0, 0)).  grid ceil(nkf / PSUM_PB).
+#define PSUM_PB 64
+#define PSUM_G 4
+#define PSUM_U 2   // K ranges a thread loads per pass
+template <int C, int CH>
+__global__ void __launch_bounds__(256) k_psum_reduce_cm(KRedArgs a, const float *__restrict__ part, int nchunk,
+                                                        int M, const double *__restrict__ inv,
+                                                        unsigned long long *cm)
+{
+    static_assert(PSUM_PB * PSUM_G == 256 && PSUM_PB <= 64, "block shape");
+    if (a.active && !*a.active) return;
+    constexpr int R = 2 * C;   // rows per k-vector
+    __shared__ double sh[PSUM_G - 1][R][PSUM_PB];
+    const int P = a.P;
+    const int nkf = (a.Kx + 1) * P;
+    const int k0 = blockIdx.x * PSUM_PB;
+    const int l = threadIdx.x % PSUM_PB, g = threadIdx.x / PSUM_PB;
+    const int k = k0 + l;
+    const size_t nval = (size_t)M * P;
+    double s[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) s[r] = 0.0;
+    const bool live = k < nkf && a.Ak[k] != 0.0;
+    if (live) {
+        const int nx = k / P, p = k - nx * P;
+        const float *pb = part + (size_t)nx * R * P + p;
+        // K ranges g, g + G, ..., PSUM_U at a time (all row loads issued
+        // first), then one at a time
+        int ch = g;
+        for (; ch + (PSUM_U - 1) * PSUM_G < nchunk; ch += PSUM_U * PSUM_G) {
+            float x[PSUM_U][R];
+#pragma unroll
+            for (int u = 0; u < PSUM_U; u++)
+#pragma unroll
+                for (int r = 0; r < R; r++) x[u][r] = pb[(size_t)(ch + u * PSUM_G) * nval + (size_t)r * P];
+#pragma unroll
+            for (int r = 0; r < R; r++) {
+                double t = 0.0;
+#pragma unroll
+                for (int u = 0; u < PSUM_U; u++) t += (double)x[u][r];
+                s[r] += t;
+            }
+        }
+        for (; ch < nchunk; ch += PSUM_G) {
+            float x0[R];
+#pragma unroll
+            for (int r = 0; r < R; r++) x0[r] = pb[(size_t)ch * nval + (size_t)r * P];
+#pragma unroll
+            for (int r = 0; r < R; r++) s[r] += (double)x0[r];
+        }
+    }
+    if (g > 0) {
+#pragma unroll
+        for (int r = 0; r < R; r++) sh[g - 1][r][l] = s[r];
+    }
+    __syncthreads();
+    // wave 0 finishes: its lanes of group 0 own the block's k-vectors, the
+    // others take part in the wave maxima with nothing
+    if (threadIdx.x >= 64) return;
+    const bool own = g == 0 && k < nkf;
+    double v[CH];
+#pragma unroll
+    for (int m = 0; m < CH; m++) v[m] = 0.0;
+    int nx = -1;
+    if (own) {
+        double S[4][2] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
+        if (live) {
+#pragma unroll
+            for (int c = 0; c < C; c++) {
+                const double sc = inv ? inv[c] : 1.0;
+#pragma unroll
+                for (int ri = 0; ri < 2; ri++) {
+                    const int r = 2 * c + ri;
+                    double t = s[r];
+#pragma unroll
+                    for (int q = 0; q < PSUM_G - 1; q++) t += sh[q][r][l];
+                    S[c][ri] = t * sc;
+                }
+            }
+        }
+        double Wr, Wi;
+        sk_reduce_S(a, (size_t)k, S, Wr, Wi);
+        nx = k / P;
+        const int NZ = 2 * a.Kz + 1;
+        const int p = k - nx * P, iy = p / NZ, iz = p % NZ;
+#pragma unroll
+        for (int m = 0; m < CH; m++) {
+            const double mom = mb_moment(m, iy - a.Ky, iz - a.Kz);
+            const double x = fmax(fabs(Wr * mom), fabs(Wi * mom));
+            v[m] = isfinite(x) ? x : 0.0;
+        }
+    }
+    // block maxima per nx plane of the block's k-vectors (one wave)
+    const int kl = k0 + PSUM_PB - 1 < nkf ? k0 + PSUM_PB - 1 : nkf - 1;
+    const int nlo = k0 / P, nhi = kl / P;
+    for (int nxv = nlo; nxv <= nhi; nxv++) {
+#pragma unroll
+        for (int m = 0; m < CH; m++) {
+            double x = nx == nxv ? v[m] : 0.0;
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) x = fmax(x, __shfl_xor(x, o));
+            if (threadIdx.x == 0 && x > 0.0) atomicMax(cm + nxv * CH + m, (unsigned long long)__double_as_longlong(x));
+        }
+    }
+}
+
 // split-FP16 WM columns (nx, m, re/im) over the Pk points (zero past P),
 // each column scaled by 2^-e into [-1, 1] from its maximum cm[nx CH + m],
 // 2^e into cinv.  grid (Pk/256, (Kx+1) CH).
@@ -635,6 +747,8 @@ struct GFwdArgs {
     int MT, M, P, C, ldk, kchunk;
     double *part;             // [nchunk][M][P] (FP64, or FP32 in the other modes)
     const int *active;
+    unsigned long long *cmz;  // when set, block (0, 0) zeroes these ncmz column maxima
+    int ncmz;                 // for the k_psum_reduce_cm that follows
 };
 
 // row tiles a forward kernel is instantiated for: every launch computes
@@ -688,6 +802,8 @@ __global__ void __launch_bounds__(256) k_gemm_fwd(GFwdArgs a)
 {
     static_assert(!GEN || MODE != KM_F64, "the FP64 mode loads stored operands");
     if (a.active && !*a.active) return;
+    if (a.cmz && blockIdx.x == 0 && blockIdx.y == 0)
+        for (int i = threadIdx.x; i < a.ncmz; i += 256) a.cmz[i] = 0ull;
     using T = typename GT<MODE>::T;
     constexpr int LD = GK + GLDS_PAD;
     constexpr bool SPLIT = g_is_split<MODE>();
