@@ -176,6 +176,8 @@ __global__ void __launch_bounds__(256) k_short(ShortArgs a)
         if (T.lbh && T.A[ti][tj] != 0.0) {
             real rd2 = (real)d2;
             real ir2 = (real)1.0 / rd2, d = sqrt(rd2);
+            // FP32: 1/d as a product (FP64 keeps the reference's divisions)
+            const real id = sizeof(real) == 4 ? d * ir2 : (real)0;
             real B = (real)T.B[ti][tj];
             real erh = (real)T.A[ti][tj] * exp(-B * d);
             real ir6 = ir2 * ir2 * ir2;
@@ -188,8 +190,17 @@ __global__ void __launch_bounds__(256) k_short(ShortArgs a)
                 tt_damp<real>(T.E_TT, (real)T.BD[ti][tj], (real)1.0, d, f8, fd8, 8);
             }
             real ir6d = ir6 * f6, ir8d = ir8 * f8;
-            real ir7 = (real)6.0 * ir6d / d, ir9 = (real)8.0 * ir8d / d;
-            double w = (double)((B * erh - ir7 - ir9 + ir6 * fd6 + ir8 * fd8) / d);
+            real ir7, ir9, wr;
+            if (sizeof(real) == 4) {
+                ir7 = (real)6.0 * ir6d * id;
+                ir9 = (real)8.0 * ir8d * id;
+                wr = (B * erh - ir7 - ir9 + ir6 * fd6 + ir8 * fd8) * id;
+            } else {
+                ir7 = (real)6.0 * ir6d / d;
+                ir9 = (real)8.0 * ir8d / d;
+                wr = (B * erh - ir7 - ir9 + ir6 * fd6 + ir8 * fd8) / d;
+            }
+            double w = (double)wr;
             ub += 0.5 * (double)(erh - ir6d - ir8d);
             double px = w * x, py = w * y, pz = w * z;
             fx += px; fy += py; fz += pz;
@@ -299,7 +310,15 @@ __global__ void __launch_bounds__(256) k_ewald_dir(DirArgs a)
         const real mjx = (real)mjxd, mjy = (real)mjyd, mjz = (real)mjzd;
         const real x = (real)dx, y = (real)dy, z = (real)dz, d2 = (real)dd2;
         const real d = sqrt(d2);
-        const real dm1 = (real)1.0 / d, dm3 = dm1 / d2, dm5 = dm3 / d2, dm7 = dm5 / d2;
+        // FP32: one reciprocal and products for the inverse powers and the
+        // constant divisions (FP64 keeps the reference's divisions)
+        constexpr bool R32 = sizeof(real) == 4;
+        const real rd2 = R32 ? (real)1.0 / d2 : (real)0;
+        const real dm1 = R32 ? d * rd2 : (real)1.0 / d;
+        const real dm3 = R32 ? dm1 * rd2 : dm1 / d2;
+        const real dm5 = R32 ? dm3 * rd2 : dm3 / d2;
+        const real dm7 = R32 ? dm5 * rd2 : dm5 / d2;
+        const real third = (real)(1.0 / 3.0), fifteenth = (real)(1.0 / 15.0);
         const bool ldamp = a.lcouldamp && T.ldamp[ti][tj];
         real f1 = 1, fd1 = 0, f2 = 1, fd2 = 0;
         if (ldamp) {
@@ -311,10 +330,12 @@ __global__ void __launch_bounds__(256) k_ewald_dir(DirArgs a)
         const real F0 = fast_erfc(al * d);
         const real F1 = F0 + (real)2.0 * al * d * expon;
         const real d3 = d2 * d;
-        const real F2 = F1 + (real)4.0 * al3 * d3 * expon / (real)3.0;
-        const real F3 = F2 + (real)8.0 * al5 * d3 * d2 * expon / (real)15.0;
+        const real F2 = F1 + (R32 ? (real)4.0 * al3 * d3 * expon * third : (real)4.0 * al3 * d3 * expon / (real)3.0);
+        const real F3 =
+            F2 + (R32 ? (real)8.0 * al5 * d3 * d2 * expon * fifteenth : (real)8.0 * al5 * d3 * d2 * expon / (real)15.0);
         const real F1d1 = (real)1.0 - f1, F1d2 = (real)1.0 - f2;
-        const real F2d1 = F1d1 + (d / (real)3.0) * fd1, F2d2 = F1d2 + (d / (real)3.0) * fd2;
+        const real d3rd = R32 ? d * third : d / (real)3.0;
+        const real F2d1 = F1d1 + d3rd * fd1, F2d2 = F1d2 + d3rd * fd2;
         // T1 = -r F1/d^3 ; T2 = a2 r r - b2 I ; T3 = -c3 rrr + a2 (delta r)
         const real t1 = dm3 * F1, t1d1 = dm3 * F1d1, t1d2 = dm3 * F1d2;
         const real a2 = (real)3.0 * F2 * dm5, b2 = F1 * dm3;
