@@ -1044,11 +1044,16 @@ __global__ void __launch_bounds__(256) k_t2_build(T2Args a)
                     keep = true;
                     const real rx = (real)x, ry = (real)y, rz = (real)z, r2 = (real)d2;
                     const real ral = (real)al, ral2 = (real)al2, ral3 = (real)al3;
-                    const real d = sqrt(r2), dm1 = (real)1 / d, dm3 = dm1 / r2, dm5 = dm3 / r2;
+                    // FP32: one reciprocal and products (FP64: the reference's divisions)
+                    constexpr bool R32 = sizeof(real) == 4;
+                    const real d = sqrt(r2), ir2 = R32 ? (real)1 / r2 : (real)0;
+                    const real dm1 = R32 ? d * ir2 : (real)1 / d;
+                    const real dm3 = R32 ? dm1 * ir2 : dm1 / r2, dm5 = R32 ? dm3 * ir2 : dm3 / r2;
                     const real expon = exp(-ral2 * r2) * (real)rpi;
                     const real F0 = fast_erfc(ral * d);
                     const real F1 = F0 + (real)2 * ral * d * expon;
-                    const real F2 = F1 + (real)4 * ral3 * r2 * d * expon / (real)3;
+                    const real F2 = F1 + (R32 ? (real)4 * ral3 * r2 * d * expon * (real)(1.0 / 3.0)
+                                              : (real)4 * ral3 * r2 * d * expon / (real)3);
                     const real a2 = (real)3 * F2 * dm5, b2 = F1 * dm3;
                     T[0] = a2 * rx * rx - b2; T[1] = a2 * rx * ry; T[2] = a2 * rx * rz;
                     T[3] = a2 * ry * ry - b2; T[4] = a2 * ry * rz; T[5] = a2 * rz * rz - b2;
