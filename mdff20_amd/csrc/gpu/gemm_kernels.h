@@ -307,6 +307,27 @@ __global__ void __launch_bounds__(256) k_aw_build16(int nsel, int nselp, int C, 
     if (active && !*active) return;
     constexpr int V = GPack<MODE>::V, E = V / 2;
     __shared__ double wsc[4];
+    const int j0 = (blockIdx.x * 256 + threadIdx.x) * E;
+    const int nx = blockIdx.y, c = blockIdx.z;
+    // the table values and weights are loaded before the channel scales are
+    // reduced (the two load chains overlap)
+    double xr[E], xi[E], wt[E];
+#pragma unroll
+    for (int e = 0; e < E; e++) {
+        const int j = j0 + e;
+        xr[e] = 0.0;
+        xi[e] = 0.0;
+        wt[e] = 0.0;
+        if (j < nsel) {
+            const size_t ox = 2 * ((size_t)nx * nsel + j);
+            xr[e] = tx[ox];
+            xi[e] = tx[ox + 1];
+            const int i = idx ? idx[j] : j;
+            if (C == 1) wt[e] = q[i];
+            else if (C == 3) wt[e] = mu[(size_t)c * n + i];
+            else wt[e] = c == 0 ? q[i] : mu[(size_t)(c - 1) * n + i];
+        }
+    }
     if (wpart) {
         const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
         if (wv < C) {
@@ -322,38 +343,13 @@ __global__ void __launch_bounds__(256) k_aw_build16(int nsel, int nselp, int C, 
         }
         __syncthreads();
     }
-    const int j0 = (blockIdx.x * 256 + threadIdx.x) * E;
-    const int nx = blockIdx.y;
     if (j0 >= nselp) return;
-    double xr[E], xi[E];
-    int ion[E];
-#pragma unroll
-    for (int e = 0; e < E; e++) {
-        const int j = j0 + e;
-        xr[e] = 0.0;
-        xi[e] = 0.0;
-        ion[e] = -1;
-        if (j < nsel) {
-            const size_t ox = 2 * ((size_t)nx * nsel + j);
-            xr[e] = tx[ox];
-            xi[e] = tx[ox + 1];
-            ion[e] = idx ? idx[j] : j;
-        }
-    }
     const size_t ld = 2 * (size_t)nselp;
     {
-        const int c = blockIdx.z;
         float r0v[V], r1v[V];
 #pragma unroll
         for (int e = 0; e < E; e++) {
-            double w = 0.0;
-            if (ion[e] >= 0) {
-                const int i = ion[e];
-                if (C == 1) w = q[i];
-                else if (C == 3) w = mu[(size_t)c * n + i];
-                else w = c == 0 ? q[i] : mu[(size_t)(c - 1) * n + i];
-                if (wpart) w *= wsc[c];
-            }
+            const double w = wpart ? wt[e] * wsc[c] : wt[e];
             r0v[2 * e] = (float)(w * xr[e]);
             r0v[2 * e + 1] = (float)(-w * xi[e]);
             r1v[2 * e] = (float)(w * xi[e]);

@@ -794,6 +794,7 @@ __device__ __forceinline__ double back_fin_one(const BackFinArgs &a, int t, cons
 {
     int i = a.idx ? a.idx[t] : t;
     double H[3] = {0, 0, 0}, M[6] = {0, 0, 0, 0, 0, 0};
+#pragma unroll 4
     for (int s = 0; s < a.nsplit; s++) {
         const double *p = a.part + (size_t)s * a.NCH * a.nsel;
         for (int c = 0; c < 3; c++) H[c] += p[(size_t)c * a.nsel + t];

@@ -685,19 +685,30 @@ __global__ void __launch_bounds__(CG_BLOCK) k_cga_update(int n, int npol, int nb
 {
     if (!*active) return;
     __shared__ double sh[CG_BLOCK];
+    const int k = blockIdx.x * CG_BLOCK + threadIdx.x;
+    // the ion's operands are loaded before the partial sums (the two load
+    // chains overlap)
+    int i = 0, t = 0;
+    double pv[3] = {0, 0, 0}, mv[3] = {0, 0, 0}, ev[3] = {0, 0, 0}, tv[3] = {0, 0, 0};
+    if (k < npol) {
+        i = pidx[k];
+        t = type[i];
+        for (int a = 0; a < 3; a++) {
+            const size_t o = (size_t)a * n + i;
+            pv[a] = p[o]; mv[a] = mu[o]; ev[a] = ef[o]; tv[a] = Tp[o];
+        }
+    }
     const double rz = cg_sum_partials(rz_old, nb, sh);
     const double pq = cg_sum_partials(pq_part, nbq, sh);
     const double g = (pq != 0.0) ? rz / pq : 0.0;
-    const int k = blockIdx.x * CG_BLOCK + threadIdx.x;
     double acc = 0.0, rms = 0.0;
     if (k < npol) {
-        const int i = pidx[k], t = type[i];
         const double *P = T->pol[t], *I = T->invpol[t];
         double m[3], e[3], rr[3];
         for (int a = 0; a < 3; a++) {
             const size_t o = (size_t)a * n + i;
-            m[a] = mu[o] + g * p[o];
-            e[a] = ef[o] + g * Tp[o];
+            m[a] = mv[a] + g * pv[a];
+            e[a] = ev[a] + g * tv[a];
             mu[o] = m[a];
             ef[o] = e[a];
         }
@@ -736,18 +747,26 @@ __global__ void __launch_bounds__(CG_BLOCK) k_cga_next(int n, int npol, int nb, 
 {
     if (!ctl[C_SCF_ACTIVE]) return;
     __shared__ double sh[CG_BLOCK];
+    const int k = blockIdx.x * CG_BLOCK + threadIdx.x;
+    // the ion's operands before the partial sums (overlapping load chains)
+    int i = 0;
+    double zv[3] = {0, 0, 0}, po[3] = {0, 0, 0};
+    if (k < npol) {
+        i = pidx[k];
+        for (int a = 0; a < 3; a++) {
+            const size_t o = (size_t)a * n + i;
+            zv[a] = z[o]; po[a] = p[o];
+        }
+    }
     const double ro = cg_sum_partials(rz_old, nb, sh);
     const double rn = cg_sum_partials(rz_new, nb, sh);
     const double rm = cg_sum_partials(rm_part, nb, sh);
     const double b = (ro != 0.0) ? rn / ro : 0.0;
-    const int k = blockIdx.x * CG_BLOCK + threadIdx.x;
     double pv[3] = {0.0, 0.0, 0.0};
     if (k < npol) {
-        const int i = pidx[k];
         for (int a = 0; a < 3; a++) {
-            const size_t o = (size_t)a * n + i;
-            pv[a] = z[o] + b * p[o];
-            p[o] = pv[a];
+            pv[a] = zv[a] + b * po[a];
+            p[(size_t)a * n + i] = pv[a];
         }
     }
     if (wpart) cg_block_max3(pv, sh, wpart, nb);
