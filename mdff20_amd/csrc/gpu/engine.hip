@@ -119,6 +119,7 @@ struct mdff_ctx {
     int fwd_part = 0;         // last forward launch: 0 partials in kpart (k_sk_reduce sums them), 1 summed into ksum,
                               // 2 FP32 partials of a matrix-core sum left in kpart for k_psum_reduce_cm
     bool psum_defer = false;  // set by rec_part: the split-FP16 forward leaves its partials (fwd_part 2)
+    int psum_ch = 3;          // with psum_defer: moments (3 or 6) of the k_psum_reduce_cm that follows
     int fwd_nch = 0;          // fwd_part 2: K ranges of the partials
     const double *fwd_inv = nullptr;   // fwd_part 2: channel scales to undo
     int nchunk = 0, nsplit = 0, chunk = 0, rows_per_split = 0;
@@ -694,6 +695,19 @@ static int gtune_blocks(mdff_ctx *c, int mode, int dir, int tiles, int polar, F 
     return best;
 }
 
+// k_psum_reduce_cm over the FP32 partials of a deferred forward sum
+static void launch_psum_cm(mdff_ctx *c, const KRedArgs &r, int C, int CH, int nch, const double *inv)
+{
+    const dim3 gp(cdiv(c->nkf, PSUM_PB));
+    const int M = (c->Kx + 1) * 2 * C;
+    const float *kp = (const float *)c->kpart;
+#define PSUM_CM(CC, HH) hipLaunchKernelGGL((k_psum_reduce_cm<CC, HH>), gp, dim3(256), 0, c->st, r, kp, nch, M, inv, c->gcm)
+    if (C == 1) { if (CH == 6) PSUM_CM(1, 6); else PSUM_CM(1, 3); }
+    else if (C == 3) { if (CH == 6) PSUM_CM(3, 6); else PSUM_CM(3, 3); }
+    else { if (CH == 6) PSUM_CM(4, 6); else PSUM_CM(4, 3); }
+#undef PSUM_CM
+}
+
 // forward sum over stored operands; 0 when outside the kernel's shapes
 static int launch_fwd_gemm(mdff_ctx *c, int polar, int lo, int C, const double *q_w, const double *mu_w,
                            const int *active, int *nchunk_out)
@@ -757,6 +771,13 @@ static int launch_fwd_gemm(mdff_ctx *c, int polar, int lo, int C, const double *
     // defer: leave the FP32 partials to k_psum_reduce_cm (the column maxima
     // it fills are zeroed by the sum's first block)
     const bool defer = c->psum_defer && mode == KM_F16X3;
+    // a deferred forward's block count is timed with the reduction that sums
+    // its partials in the call (k_psum_reduce_cm; its outputs are written
+    // again by the call's own reduction)
+    KRedArgs tr{};
+    tr.Kx = c->Kx; tr.Ky = c->Ky; tr.Kz = c->Kz; tr.P = c->P; tr.nchunk = 1; tr.C = C;
+    tr.part = c->ksum; tr.kvec = c->kvec; tr.Ak = c->Ak; tr.kcoe = c->kcoe; tr.W = c->W; tr.ek = c->ek;
+    tr.do_stress = 0; tr.active = nullptr;
     auto run = [&](int blocks, int go, const int *act, bool probe, bool psum = true) {
         int nch = std::max(1, std::min(64, (blocks + gx - 1) / gx));
         GFwdArgs ga2 = a;
@@ -773,6 +794,10 @@ static int launch_fwd_gemm(mdff_ctx *c, int polar, int lo, int C, const double *
         else { if (gen) gfwd_cw<KM_BF16X3, true>(c, grid, CW, MT, ga2); else gfwd_cw<KM_BF16X3>(c, grid, CW, MT, ga2); }
         if (probe) probe_mark(c, 1);
         if (!psum) return nch;
+        if (defer) {
+            launch_psum_cm(c, tr, C, c->psum_ch, nch, winv);
+            return nch;
+        }
         if (mode == KM_F64)
             hipLaunchKernelGGL(k_part_sum<double>, dim3(cdiv(nval, 256)), dim3(256), 0, c->st,
                                (const double *)c->kpart, nch, M, c->P, c->ksum, C, winv, act);
@@ -921,6 +946,7 @@ static int rec_part(mdff_ctx *c, int polar, int lo, int C, int NCH, const double
     const bool cm_path = (do_ef || do_f) && c->use_mfma && c->gemm && bmode == KM_F16X3 &&
                          (c->Kx + 1) * CHb * 2 <= MB_CT * 16 && wm_alloc(c, bmode) == MDFF_OK;
     c->psum_defer = cm_path && (C == 1 || C == 3 || C == 4);
+    c->psum_ch = CHb;
     const bool fwd_mf = launch_fwd_mfma(c, polar, lo, C, q_w, mu_w, active, &nchunk);
     c->psum_defer = false;
     c->wpart_ready = 0;
@@ -946,16 +972,7 @@ static int rec_part(mdff_ctx *c, int polar, int lo, int C, int NCH, const double
         void *w0 = c->wm[bmode][0], *w1 = c->wm[bmode][1];
         const dim3 gr(cdiv(c->nkf, 256));
         if (raw) {
-            const dim3 gp(cdiv(c->nkf, PSUM_PB));
-            const int M = (c->Kx + 1) * 2 * C;
-            const float *kp = (const float *)c->kpart;
-            const int nch = c->fwd_nch;
-            const double *inv = c->fwd_inv;
-#define PSUM_CM(CC, HH) hipLaunchKernelGGL((k_psum_reduce_cm<CC, HH>), gp, dim3(256), 0, c->st, r, kp, nch, M, inv, c->gcm)
-            if (C == 1) { if (CHb == 6) PSUM_CM(1, 6); else PSUM_CM(1, 3); }
-            else if (C == 3) { if (CHb == 6) PSUM_CM(3, 6); else PSUM_CM(3, 3); }
-            else { if (CHb == 6) PSUM_CM(4, 6); else PSUM_CM(4, 3); }
-#undef PSUM_CM
+            launch_psum_cm(c, r, C, CHb, c->fwd_nch, c->fwd_inv);
         } else if (CHb == 6) {
             hipLaunchKernelGGL(k_sk_reduce_cm<6>, gr, dim3(256), 0, c->st, r, c->gcm);
         } else {
