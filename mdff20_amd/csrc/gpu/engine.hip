@@ -120,6 +120,7 @@ struct mdff_ctx {
                               // 2 FP32 partials of a matrix-core sum left in kpart for k_psum_reduce_cm
     bool psum_defer = false;  // set by rec_part: the split-FP16 forward leaves its partials (fwd_part 2)
     int psum_ch = 3;          // with psum_defer: moments (3 or 6) of the k_psum_reduce_cm that follows
+    int no_fuse = 0;          // tuning: separate partial sums, reduction and pair-tensor finish
     int fwd_nch = 0;          // fwd_part 2: K ranges of the partials
     const double *fwd_inv = nullptr;   // fwd_part 2: channel scales to undo
     int nchunk = 0, nsplit = 0, chunk = 0, rows_per_split = 0;
@@ -945,7 +946,7 @@ static int rec_part(mdff_ctx *c, int polar, int lo, int C, int NCH, const double
     // partials itself (k_psum_reduce_cm)
     const bool cm_path = (do_ef || do_f) && c->use_mfma && c->gemm && bmode == KM_F16X3 &&
                          (c->Kx + 1) * CHb * 2 <= MB_CT * 16 && wm_alloc(c, bmode) == MDFF_OK;
-    c->psum_defer = cm_path && (C == 1 || C == 3 || C == 4);
+    c->psum_defer = cm_path && !c->no_fuse && (C == 1 || C == 3 || C == 4);
     c->psum_ch = CHb;
     const bool fwd_mf = launch_fwd_mfma(c, polar, lo, C, q_w, mu_w, active, &nchunk);
     c->psum_defer = false;
@@ -1003,7 +1004,8 @@ static int rec_part(mdff_ctx *c, int polar, int lo, int C, int NCH, const double
         int nsplit = c->nsplit;
         // the FP32 pair-tensor field of the finish in extra rows of the
         // back-projection's launch, when it takes them
-        T2Rows t2r{ff && ff->t2 && ff->t2f32 ? (const float *)ff->t2 : nullptr, ff ? ff->mu : nullptr, c->t2dir, 0};
+        T2Rows t2r{ff && ff->t2 && ff->t2f32 && !c->no_fuse ? (const float *)ff->t2 : nullptr, ff ? ff->mu : nullptr,
+                   c->t2dir, 0};
         if (!launch_back_mfma(c, polar, lo, NCH, active, &nsplit, wm_built, t2r.t2 ? &t2r : nullptr)) {
             if (NCH == 9) { if (mixed) launch_back_t<float, 9>(c, b, nsel); else launch_back_t<double, 9>(c, b, nsel); }
             else { if (mixed) launch_back_t<float, 3>(c, b, nsel); else launch_back_t<double, 3>(c, b, nsel); }
@@ -2205,12 +2207,16 @@ int mdff_set_tuning(mdff_ctx *c, int lanes, int katoms, int ksplit)
     if (!c) return MDFF_ERR_ARG;
     if (lanes == 8 || lanes == 16 || lanes == 32 || lanes == 64) c->lanes = lanes;
     else if (lanes != 0) return MDFF_ERR_ARG;
-    // katoms: reciprocal-space kernel family.  -1 vector ALU; 1 matrix-core
-    // sums over stored operands with split-BF16 SCF correction sweeps; 2 the
-    // same with FP32 sweeps; 3 matrix cores with operands formed in LDS per
-    // call (split-BF16 sweeps).
-    if (katoms == 1) { c->use_mfma = 1; c->gemm = 1; c->mf_inner = KM_F16X3; }
-    else if (katoms == 2) { c->use_mfma = 1; c->gemm = 1; c->mf_inner = KM_F32; }
+    // katoms: reciprocal-space kernel family.  -1 vector ALU; 1 the streaming
+    // matrix-core sums with split-FP16 SCF steps; 2 the same with FP32 steps;
+    // 3 matrix cores with operands formed in LDS per call (split-BF16 steps);
+    // 4 as 1 with the partial sums, reduction and pair-tensor finish in
+    // separate kernels.
+    if (katoms == 1 || katoms == 4) {
+        c->use_mfma = 1; c->gemm = 1; c->mf_inner = KM_F16X3;
+        c->no_fuse = katoms == 4;
+    }
+    else if (katoms == 2) { c->use_mfma = 1; c->gemm = 1; c->mf_inner = KM_F32; c->no_fuse = 0; }
     else if (katoms == 3) { c->use_mfma = 1; c->gemm = 0; c->mf_inner = KM_BF16X3; }
     else if (katoms == -1) c->use_mfma = 0;
     else if (katoms != 0) return MDFF_ERR_ARG;

@@ -83,6 +83,26 @@ def test_lane_widths_agree_mixed(lanes, katoms):
         assert t0["u_tot"] == pytest.approx(t1["u_tot"], rel=1e-8)
 
 
+@pytest.mark.parametrize("case", ["pim192_static", "bsio_200_nvt"])
+def test_fused_kspace_kernels_match_separate(case):
+    """the mixed form's fused kernels (the forward's partial sums inside the
+    S(k) reduction, the CG direction's pair-tensor field in the back-
+    projection's launch) against the same arithmetic in separate kernels
+    (katoms 4): only the order of the FP64 partial-sum additions differs
+    (measured 1e-17 relative)"""
+    p, cfg, _ = load_case(case)
+    with m.Engine(p, cfg.typia, cfg.A, cfg.r, precision=m.MIXED) as ef, \
+            m.Engine(p, cfg.typia, cfg.A, cfg.r, precision=m.MIXED) as es:
+        es.set_tuning(katoms=4)
+        tf, ts = ef.engforce(0), es.engforce(0)
+        print(f"[gpu] fused vs separate {case}: forces {rel(ef.forces(), es.forces()):.2e}, "
+              f"dipoles {rel(ef.dipoles(), es.dipoles()):.2e}", flush=True)
+        assert rel(ef.forces(), es.forces()) < 1e-12
+        assert rel(ef.dipoles(), es.dipoles()) < 1e-12
+        assert tf["u_tot"] == pytest.approx(ts["u_tot"], rel=1e-13)
+        assert ef.counters()["scf_inner"] == es.counters()["scf_inner"]
+
+
 @pytest.mark.parametrize("katoms", [1, 2, -1])
 @pytest.mark.parametrize("case", ["pim192_static", "sio2_102_static", "bsio_200_nvt"])
 def test_mixed_form_accuracy(case, katoms):
