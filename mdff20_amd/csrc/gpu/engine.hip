@@ -142,6 +142,8 @@ struct mdff_ctx {
     void *wm[4][2] = {};      // [mode][hi,lo]: columns (nx,m,re/im) of the back sum
     double *gsc = nullptr;    // split-FP16 scales: weight inverses [8,16), WM column inverses from 16
     double *gwp = nullptr;    // split-FP16 per-block weight channel maxima [4][n/256]
+    double *gwq = nullptr;    // the same for the charges alone (all ions), formed once [n/256]
+    int gwq_ok = 0;
     unsigned long long *gcm = nullptr;   // split-FP16 WM column maxima [(Kx+1) 6] (double bits)
     double *chk_part = nullptr;          // SCF rmsd / u_pol partial sums [2][n/256]
     double *pq_buf = nullptr;            // CG p.q partial sums, one per fin_t2 block [n/4]
@@ -152,6 +154,7 @@ struct mdff_ctx {
     int mf_inner = KM_F16X3;  // mode of the mixed-form low-precision SCF steps
     int mf_anchor = KM_F16X3; // anchored CG: anchor evaluation in FP64 (0) or this mode over the FP32 tables
     int wpart_ready = 0;      // gwp holds the channel maxima of the next split-FP16 forward sum's weights
+    int wpart_nb = 0;         // with wpart_ready: partials per channel in gwp (0: one per 256 selected ions)
     int mf_blocks = 512;      // target blocks of the MFMA kernels
     int nsplit_max = 1;       // (ny,nz) splits the bpart buffer holds
     // host pinned mirrors
@@ -732,14 +735,24 @@ static int launch_fwd_gemm(mdff_ctx *c, int polar, int lo, int C, const double *
     const int *idx = polar ? c->pidx : nullptr;
     dim3 ga(cdiv(nselp, 256), c->Kx + 1);
     const double *wsc = nullptr, *winv = nullptr;
-    const int nwb = cdiv(nsel, 256);
+    const int nwb0 = cdiv(nsel, 256);
+    int nwb = (c->wpart_ready && c->wpart_nb > 0) ? c->wpart_nb : nwb0;
     if (mode == KM_F16X3) {
         // weights scaled into [-1, 1] per channel (maxima already in gwp
         // when the kernel that wrote the weights produced them)
-        if (!c->wpart_ready)
+        wsc = c->gwp;
+        if (C == 1 && !polar && q_w == c->q) {
+            // the charges alone (static field): their maxima never change
+            if (!c->gwq_ok)
+                hipLaunchKernelGGL(k_w_max, dim3(nwb0), dim3(256), 0, c->st, nsel, C, idx, q_w, mu_w, c->n, c->gwq,
+                                   (const int *)nullptr);
+            c->gwq_ok = 1;
+            wsc = c->gwq;
+            nwb = nwb0;
+        } else if (!c->wpart_ready) {
             hipLaunchKernelGGL(k_w_max, dim3(nwb), dim3(256), 0, c->st, nsel, C, idx, q_w, mu_w, c->n, c->gwp,
                                active);
-        wsc = c->gwp;
+        }
         winv = c->gsc + 8;
     }
     if (mode == KM_F64)
@@ -951,6 +964,7 @@ static int rec_part(mdff_ctx *c, int polar, int lo, int C, int NCH, const double
     const bool fwd_mf = launch_fwd_mfma(c, polar, lo, C, q_w, mu_w, active, &nchunk);
     c->psum_defer = false;
     c->wpart_ready = 0;
+    c->wpart_nb = 0;
     if (!fwd_mf) {
         if (C == 1) { if (mixed) launch_fwd_t<float, 1>(c, a, nchunk); else launch_fwd_t<double, 1>(c, a, nchunk); }
         else if (C == 3) { if (mixed) launch_fwd_t<float, 3>(c, a, nchunk); else launch_fwd_t<double, 3>(c, a, nchunk); }
@@ -1392,13 +1406,18 @@ static int scf_loop_anchored(mdff_ctx *c, int istep)
     hipLaunchKernelGGL(k_scf_begin, dim3(1), dim3(1), 0, c->st, ctl, (maxit > 0 || minit > 0) ? 1 : 0);
     AspcArgs asp;
     if (aspc_args(c, istep, &asp)) return MDFF_ERR_CONFIG;
+    // a split-FP16 anchor scales its weights by the predictor's channel
+    // maxima, which the predictor writes (all ions; zero where not polarisable)
+    const bool wmu = c->mf_anchor == KM_F16X3;
     hipLaunchKernelGGL(k_scf_mu, dim3(cdiv(n, 256)), dim3(256), 0, c->st, n, 1, asp, c->type, c->tab_d, c->hist, c->ef,
-                       c->mu_ind, c->upol_i, act);
+                       c->mu_ind, c->upol_i, act, wmu ? c->gwp : nullptr);
     // the anchor: the field of the predicted dipoles in FP64, or with
     // mf_anchor over the FP32 tables in that matrix-core mode
     if (c->mf_anchor) {
         const int keep = c->mf_lo;
         c->mf_lo = c->mf_anchor;
+        c->wpart_ready = wmu;
+        c->wpart_nb = wmu ? cdiv(n, 256) : 0;
         induced_field(c, 1, c->mu_ind, c->ef_stat, c->ef, act);
         c->mf_lo = keep;
     } else {
@@ -1829,7 +1848,7 @@ void mdff_destroy(mdff_ctx *c)
                      &c->mu_static, &c->mu_ind, &c->mu_tot, &c->hist, &c->ef_stat, &c->ef, &c->ef_dir, &c->ef_rec,
                      &c->efg, &c->efg_dir, &c->efg_rec, &c->f_sr, &c->f_dir, &c->f_rec, &c->acc_sr, &c->uw_dir,
                      &c->selfbuf, &c->ke, &c->upol_i, &c->rmsd_i, &c->tmp_q, &c->tmp_mu, &c->tmp_qraw, &c->tmp_mur, &c->dc_R, &c->dc_d, &c->dc_g, &c->cg_z, &c->cg_p, &c->cg_part, &c->chk_part, &c->pq_buf, &c->t2dir, &c->ko_f, &c->ko_dmu, &c->t2ko, &c->cg_w, &c->scal,
-                     &c->kvec, &c->Ak, &c->kcoe, &c->W, &c->ek, &c->kpart, &c->bpart, &c->ksum, &c->gsc, &c->gwp};
+                     &c->kvec, &c->Ak, &c->kcoe, &c->W, &c->ek, &c->kpart, &c->bpart, &c->ksum, &c->gsc, &c->gwp, &c->gwq};
     for (auto pp : dp) dfree(*pp);
     dfree(c->type);
     dfree(c->ctl);
@@ -2074,6 +2093,7 @@ int mdff_create(mdff_ctx **out, const mdff_params *p, int nion, const int *typia
         ALLOC(c->ek, 7 * c->nkf);
         ALLOC(c->gsc, 16 + 6 * (size_t)(c->Kx + 1));
         ALLOC(c->gwp, 4 * (size_t)cdiv(c->n, 256));
+        ALLOC(c->gwq, (size_t)cdiv(c->n, 256));
         if (hipMalloc((void **)&c->gcm, sizeof(unsigned long long) * 6 * (size_t)(c->Kx + 1)) != hipSuccess) {
             mdff_destroy(c);
             return MDFF_ERR_NOMEM;

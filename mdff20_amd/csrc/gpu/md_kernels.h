@@ -21,39 +21,48 @@ struct AspcArgs {
 // (pim.c:382-400) and shifts the history of each ion; later iterations take
 // mu = pol^T E on polarisable ions (induced_moment pim.c:617-641); every
 // ion also gets its share of u_pol = 1/2 mu.pol^-1.mu (get_upol pim.c:145-161)
+__device__ __forceinline__ void cg_block_max3(const double v[3], double *sh, double *wpart, int nb);
+
+// with wpart (256-thread blocks), also the block maxima of |mu| per channel
+// (the weight scales of a split-FP16 forward sum over mu)
 __global__ void k_scf_mu(int n, int first, AspcArgs asp, const int *__restrict__ type, const TypeTab *__restrict__ T,
                          double *__restrict__ hist, const double *__restrict__ ef, double *__restrict__ mu,
-                         double *__restrict__ upol, const int *active)
+                         double *__restrict__ upol, const int *active, double *wpart = nullptr)
 {
     if (!*active) return;
     int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    int t = type[i];
     double m[3] = {0.0, 0.0, 0.0};
-    const size_t n3 = (size_t)3 * n;
-    if (first) {
-        for (int a = 0; a < 3; a++) {
-            double v = 0.0;
-            for (int k = 0; k <= asp.eo; k++) v += asp.B[k] * hist[k * n3 + (size_t)a * n + i];
-            m[a] = v;
+    if (i < n) {
+        int t = type[i];
+        const size_t n3 = (size_t)3 * n;
+        if (first) {
+            for (int a = 0; a < 3; a++) {
+                double v = 0.0;
+                for (int k = 0; k <= asp.eo; k++) v += asp.B[k] * hist[k * n3 + (size_t)a * n + i];
+                m[a] = v;
+            }
+            for (int k = asp.shift; k > 0; k--)
+                for (int a = 0; a < 3; a++) hist[k * n3 + (size_t)a * n + i] = hist[(k - 1) * n3 + (size_t)a * n + i];
+        } else if (T->polar[t]) {
+            const double *P = T->pol[t];
+            double e0 = ef[i], e1 = ef[n + i], e2 = ef[2 * n + i];
+            for (int a = 0; a < 3; a++) m[a] = P[0 * 3 + a] * e0 + P[1 * 3 + a] * e1 + P[2 * 3 + a] * e2;
         }
-        for (int k = asp.shift; k > 0; k--)
-            for (int a = 0; a < 3; a++) hist[k * n3 + (size_t)a * n + i] = hist[(k - 1) * n3 + (size_t)a * n + i];
-    } else if (T->polar[t]) {
-        const double *P = T->pol[t];
-        double e0 = ef[i], e1 = ef[n + i], e2 = ef[2 * n + i];
-        for (int a = 0; a < 3; a++) m[a] = P[0 * 3 + a] * e0 + P[1 * 3 + a] * e1 + P[2 * 3 + a] * e2;
+        mu[i] = m[0];
+        mu[n + i] = m[1];
+        mu[2 * n + i] = m[2];
+        double up = 0.0;
+        if (T->polar[t]) {
+            const double *I = T->invpol[t];
+            for (int a = 0; a < 3; a++)
+                for (int b = 0; b < 3; b++) up += m[a] * I[3 * a + b] * m[b];
+        }
+        upol[i] = 0.5 * up;
     }
-    mu[i] = m[0];
-    mu[n + i] = m[1];
-    mu[2 * n + i] = m[2];
-    double up = 0.0;
-    if (T->polar[t]) {
-        const double *I = T->invpol[t];
-        for (int a = 0; a < 3; a++)
-            for (int b = 0; b < 3; b++) up += m[a] * I[3 * a + b] * m[b];
+    if (wpart) {
+        __shared__ double sh[256];
+        cg_block_max3(m, sh, wpart, gridDim.x);
     }
-    upol[i] = 0.5 * up;
 }
 
 // ASPC corrector (pim.c:403-420): mu = W pol^T E + (1-W) mu on iteration 0
