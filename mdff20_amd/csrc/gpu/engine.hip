@@ -191,8 +191,9 @@ static int sync_ctl(mdff_ctx *c)
 
 static int sync_scal(mdff_ctx *c)
 {
-    HIPCHK(hipMemcpyAsync(c->scal_h, c->scal, sizeof(double) * S_COUNT, hipMemcpyDeviceToHost, c->st));
-    HIPCHK(hipMemcpyAsync(c->ctl_h, c->ctl, sizeof(int) * C_COUNT, hipMemcpyDeviceToHost, c->st));
+    // scal and ctl are one allocation on each side: one copy
+    HIPCHK(hipMemcpyAsync(c->scal_h, c->scal, sizeof(double) * S_COUNT + sizeof(int) * C_COUNT,
+                          hipMemcpyDeviceToHost, c->st));
     HIPCHK(hipStreamSynchronize(c->st));
     return MDFF_OK;
 }
@@ -1851,7 +1852,7 @@ void mdff_destroy(mdff_ctx *c)
                      &c->kvec, &c->Ak, &c->kcoe, &c->W, &c->ek, &c->kpart, &c->bpart, &c->ksum, &c->gsc, &c->gwp, &c->gwq};
     for (auto pp : dp) dfree(*pp);
     dfree(c->type);
-    dfree(c->ctl);
+    c->ctl = nullptr;   // inside the scal allocation
     dfree(c->tab_d);
     dfree(c->pidx);
     dfree(c->pinv);
@@ -1876,8 +1877,9 @@ void mdff_destroy(mdff_ctx *c)
             if (c->aw[m][h]) (void)hipFree(c->aw[m][h]);
             if (c->wm[m][h]) (void)hipFree(c->wm[m][h]);
         }
-    if (c->ctl_h) (void)hipHostFree(c->ctl_h);
-    if (c->scal_h) (void)hipHostFree(c->scal_h);
+    if (c->scal_h) (void)hipHostFree(c->scal_h);   // ctl_h lies inside it
+    c->scal_h = nullptr;
+    c->ctl_h = nullptr;
     if (c->have_events)
         for (int i = 0; i < 8; i++) (void)hipEventDestroy(c->ev[i]);
     if (c->st2) (void)hipStreamSynchronize(c->st2);
@@ -2012,9 +2014,16 @@ int mdff_create(mdff_ctx **out, const mdff_params *p, int nion, const int *typia
     ALLOC(c->chk_part, 2 * (size_t)cdiv(n, 256));
     ALLOC(c->pq_buf, (size_t)cdiv(n, 4)); ALLOC(c->t2dir, 3 * n);
     ALLOC(c->ko_f, 3 * n); ALLOC(c->ko_dmu, 3 * n); ALLOC(c->cg_w, 3 * n);
-    ALLOC(c->scal, S_COUNT); ALLOC(c->ctl, C_COUNT); ALLOC(c->tab_d, 1);
-    if (hipHostMalloc((void **)&c->ctl_h, sizeof(int) * C_COUNT) != hipSuccess) { mdff_destroy(c); return MDFF_ERR_NOMEM; }
-    if (hipHostMalloc((void **)&c->scal_h, sizeof(double) * S_COUNT) != hipSuccess) { mdff_destroy(c); return MDFF_ERR_NOMEM; }
+    // scal [S_COUNT doubles] then ctl [C_COUNT ints] in one allocation, so
+    // that one copy brings both to the host
+    ALLOC(c->scal, S_COUNT + ((size_t)C_COUNT * sizeof(int) + sizeof(double) - 1) / sizeof(double));
+    c->ctl = (int *)(c->scal + S_COUNT);
+    ALLOC(c->tab_d, 1);
+    if (hipHostMalloc((void **)&c->scal_h, sizeof(double) * S_COUNT + sizeof(int) * C_COUNT) != hipSuccess) {
+        mdff_destroy(c);
+        return MDFF_ERR_NOMEM;
+    }
+    c->ctl_h = (int *)(c->scal_h + S_COUNT);
     memset(c->ctl_h, 0, sizeof(int) * C_COUNT);
     memset(c->scal_h, 0, sizeof(double) * S_COUNT);
     // per-ion host arrays (init_config config.c:124-198)
