@@ -155,6 +155,7 @@ struct mdff_ctx {
     int mf_anchor = KM_F16X3; // anchored CG: anchor evaluation in FP64 (0) or this mode over the FP32 tables
     int wpart_ready = 0;      // gwp holds the channel maxima of the next split-FP16 forward sum's weights
     int wpart_nb = 0;         // with wpart_ready: partials per channel in gwp (0: one per 256 selected ions)
+    int tot_done = 0;         // k_mu_tot formed mu_tot, the self terms and (2: also) gwp for the force call
     int mf_blocks = 512;      // target blocks of the MFMA kernels
     int nsplit_max = 1;       // (ny,nz) splits the bpart buffer holds
     // host pinned mirrors
@@ -1602,9 +1603,16 @@ static void coulomb_force(mdff_ctx *c, int lo)
     }
     int C = ld ? 4 : 1;
     int NCH = ld ? 9 : 3;
+    const int done = c->tot_done;
+    c->tot_done = 0;
+    if (done == 2 && C == 4) {
+        c->wpart_ready = 1;
+        c->wpart_nb = cdiv(n, 256);
+    }
     rec_part(c, 0, lo, C, NCH, c->q, c->mu_tot, c->q, c->mu_tot, 0, 1, 1, nullptr, nullptr);
     colsum(c, c->ek, (int)c->nkf, (int)c->nkf, 7, c->scal + S_U_REC);
-    hipLaunchKernelGGL(k_self_terms, dim3(cdiv(n, 256)), dim3(256), 0, c->st, n, c->q, c->mu_tot, c->selfbuf);
+    if (!done)
+        hipLaunchKernelGGL(k_self_terms, dim3(cdiv(n, 256)), dim3(256), 0, c->st, n, c->q, c->mu_tot, c->selfbuf);
     colsum(c, c->selfbuf, n, n, 2, c->scal + S_QSQ);
 }
 
@@ -1640,8 +1648,12 @@ static int engforce_impl(mdff_ctx *c, int istep)
             rc = c->p.algo_pim == MDFF_PIM_SCFKO ? scf_loop_ko(c, istep) : (lo ? scf_mixed(c, istep) : scf_loop(c, istep));
             if (rc != MDFF_OK && rc != MDFF_ERR_SCF) return rc;
             c->ldip_sticky = 1;
-            hipLaunchKernelGGL(k_axpy3, dim3(cdiv(3 * n, 256)), dim3(256), 0, c->st, 3 * n, c->mu_static, c->mu_ind,
-                               c->mu_tot);
+            // mu_tot with the force call's self terms and, for a split-FP16
+            // force sum, its weight maxima
+            const bool wt = lo && c->mf_lo == KM_F16X3;
+            hipLaunchKernelGGL(k_mu_tot, dim3(cdiv(n, 256)), dim3(256), 0, c->st, n, c->q, c->mu_static, c->mu_ind,
+                               c->mu_tot, c->selfbuf, wt ? c->gwp : nullptr);
+            c->tot_done = wt ? 2 : 1;
         } else {
             if (c->have_events) HIPCHK(hipEventRecord(c->ev[2], c->st));
             HIPCHK(hipMemcpyAsync(c->mu_tot, c->mu_static, sizeof(double) * 3 * n, hipMemcpyDeviceToDevice, c->st));

@@ -788,6 +788,49 @@ __global__ void __launch_bounds__(CG_BLOCK) k_cga_next(int n, int npol, int nb, 
     }
 }
 
+// mu_tot = mu_static + mu_ind (engforce after get_dipoles), with the self
+// terms of the force call (q^2 and |mu_tot|^2 per ion, as k_self_terms) and,
+// when wpart is given, the per-block maxima of |q|, |mu_tot_a| (four weight
+// channels of a split-FP16 force sum) at wpart[c * gridDim.x + block].
+// 256-thread blocks.
+__global__ void __launch_bounds__(256) k_mu_tot(int n, const double *__restrict__ q, const double *__restrict__ ms,
+                                                const double *__restrict__ mi, double *__restrict__ mt,
+                                                double *__restrict__ self2n, double *__restrict__ wpart)
+{
+    __shared__ double sh[4][4];
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    double v[4] = {0.0, 0.0, 0.0, 0.0};
+    if (i < n) {
+        const double qi = q[i];
+        double m2 = 0.0;
+        for (int a = 0; a < 3; a++) {
+            const size_t o = (size_t)a * n + i;
+            const double m = ms[o] + mi[o];
+            mt[o] = m;
+            m2 += m * m;
+            v[1 + a] = m;
+        }
+        v[0] = qi;
+        self2n[i] = qi * qi;
+        self2n[n + i] = m2;
+    }
+    if (!wpart) return;
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+        double x = isfinite(v[c]) ? fabs(v[c]) : 0.0;
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) x = fmax(x, __shfl_xor(x, o));
+        v[c] = x;
+    }
+    if ((threadIdx.x & 63) == 0)
+        for (int c = 0; c < 4; c++) sh[threadIdx.x >> 6][c] = v[c];
+    __syncthreads();
+    if (threadIdx.x < 4) {
+        const double r = fmax(fmax(sh[0][threadIdx.x], sh[1][threadIdx.x]), fmax(sh[2][threadIdx.x], sh[3][threadIdx.x]));
+        wpart[(size_t)threadIdx.x * gridDim.x + blockIdx.x] = r;
+    }
+}
+
 // ---------------------------------------------------------------------------
 // scfkO (momentpolaSCFkO pim.c:457-603, induced_moment_inner_kO :647-748),
 // per polarisable row.  The inner loop relaxes dmu towards pol^T (f + E(dmu))
