@@ -273,14 +273,16 @@ static int build_list(mdff_ctx *c, int force)
     double csr = c->p.cutshortrange + c->p.skindiff, clr = c->p.cutlongrange + c->p.skindiff;
     int all = !c->p.lverletL;
     for (int attempt = 0; attempt < 3; attempt++) {
-        set_int(c, C_OVERFLOW, 0);
-        set_int(c, C_MAXCNT, 0);
         if (all) {
             // no Verlet list: every pair is a candidate, independent of positions
+            set_int(c, C_OVERFLOW, 0);
+            set_int(c, C_MAXCNT, 0);
             set_int(c, C_REBUILD, (force || attempt > 0 || !c->have_list) ? 1 : 0);
         } else {
             if (force || attempt > 0) set_int(c, C_FIRST, 1);
-            hipLaunchKernelGGL(k_vcheck, dim3(1), dim3(1024), 0, c->st, n, c->cd, c->s, c->s0, c->p.skindiff, c->ctl);
+            // the check also clears the overflow flag and count of the build
+            hipLaunchKernelGGL(k_vcheck, dim3(1), dim3(1024), 0, c->st, n, c->cd, c->s, c->s0, c->p.skindiff, c->ctl,
+                               1);
             if (attempt > 0) hipLaunchKernelGGL(k_add_int, dim3(1), dim3(1), 0, c->st, c->ctl + C_NUPD, -1);
         }
         hipLaunchKernelGGL(k_build_allpairs, dim3(cdiv(n, 4)), dim3(256), 0, c->st, n, c->cd, c->s, csr * csr,
@@ -1405,14 +1407,16 @@ static int scf_loop_anchored(mdff_ctx *c, int istep)
     int *ctl = c->ctl;
     const int *act = c->ctl + C_SCF_ACTIVE;
     int maxit = c->p.max_scf_pim_iter, minit = c->p.min_scf_pim_iter;
-    hipLaunchKernelGGL(k_scf_begin, dim3(1), dim3(1), 0, c->st, ctl, (maxit > 0 || minit > 0) ? 1 : 0);
     AspcArgs asp;
     if (aspc_args(c, istep, &asp)) return MDFF_ERR_CONFIG;
-    // a split-FP16 anchor scales its weights by the predictor's channel
-    // maxima, which the predictor writes (all ions; zero where not polarisable)
+    // the predictor opens the SCF (k_scf_begin) when the loop runs at all; a
+    // split-FP16 anchor scales its weights by the predictor's channel maxima,
+    // which the predictor writes (all ions; zero where not polarisable)
+    const bool open = maxit > 0 || minit > 0;
+    if (!open) hipLaunchKernelGGL(k_scf_begin, dim3(1), dim3(1), 0, c->st, ctl, 0);
     const bool wmu = c->mf_anchor == KM_F16X3;
     hipLaunchKernelGGL(k_scf_mu, dim3(cdiv(n, 256)), dim3(256), 0, c->st, n, 1, asp, c->type, c->tab_d, c->hist, c->ef,
-                       c->mu_ind, c->upol_i, act, wmu ? c->gwp : nullptr);
+                       c->mu_ind, c->upol_i, act, wmu ? c->gwp : nullptr, open ? ctl : nullptr);
     // the anchor: the field of the predicted dipoles in FP64, or with
     // mf_anchor over the FP32 tables in that matrix-core mode
     if (c->mf_anchor) {

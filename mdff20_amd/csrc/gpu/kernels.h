@@ -49,9 +49,15 @@ __global__ void k_frac(int n, CellD c, const double *__restrict__ rx, const doub
 // check_verletlist verlet.c:124-170: the two largest displacements since the
 // last build; one block.
 __global__ void __launch_bounds__(1024) k_vcheck(int n, CellD c, const double *__restrict__ s,
-                                                 const double *__restrict__ s0, double skin, int *ctl)
+                                                 const double *__restrict__ s0, double skin, int *ctl, int reset = 0)
 {
     __shared__ double m1s[1024], m2s[1024];
+    // reset: clear the overflow flag and the largest count of the build that
+    // follows
+    if (reset && threadIdx.x == 0) {
+        ctl[C_OVERFLOW] = 0;
+        ctl[C_MAXCNT] = 0;
+    }
     double m1 = 0.0, m2 = 0.0;
     for (int i = threadIdx.x; i < n; i += 1024) {
         double x, y, z;

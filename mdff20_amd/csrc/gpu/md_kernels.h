@@ -24,12 +24,22 @@ struct AspcArgs {
 __device__ __forceinline__ void cg_block_max3(const double v[3], double *sh, double *wpart, int nb);
 
 // with wpart (256-thread blocks), also the block maxima of |mu| per channel
-// (the weight scales of a split-FP16 forward sum over mu)
+// (the weight scales of a split-FP16 forward sum over mu).  With begin_ctl
+// the kernel opens the SCF itself (k_scf_begin with the loop active): it
+// runs ungated and its first thread sets the flag and the iteration count.
 __global__ void k_scf_mu(int n, int first, AspcArgs asp, const int *__restrict__ type, const TypeTab *__restrict__ T,
                          double *__restrict__ hist, const double *__restrict__ ef, double *__restrict__ mu,
-                         double *__restrict__ upol, const int *active, double *wpart = nullptr)
+                         double *__restrict__ upol, const int *active, double *wpart = nullptr,
+                         int *begin_ctl = nullptr)
 {
-    if (!*active) return;
+    if (begin_ctl) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            begin_ctl[C_SCF_ACTIVE] = 1;
+            begin_ctl[C_ISCF] = 0;
+        }
+    } else if (!*active) {
+        return;
+    }
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     double m[3] = {0.0, 0.0, 0.0};
     if (i < n) {
@@ -301,12 +311,15 @@ __global__ void k_nhc_chain(NhcArgs a, double *scal)
                     G[i] = (v[i - 1] * v[i - 1] / a.Q[i - 1] - a.kT);
                     v[i] += G[i] * dts4;
                 }
+            // the two scalings around each kick use one exponential (same
+            // argument, same value)
 #pragma unroll
             for (int ih = NH - 2; ih > -1; ih--)
                 if (ih <= nh - 2) {
-                    v[ih] *= exp(-v[ih + 1] * dts8 / a.Q[ih + 1]);
+                    const double e = exp(-v[ih + 1] * dts8 / a.Q[ih + 1]);
+                    v[ih] *= e;
                     v[ih] += G[ih] * dts4;
-                    v[ih] *= exp(-v[ih + 1] * dts8 / a.Q[ih + 1]);
+                    v[ih] *= e;
                 }
             s *= exp(-v[0] * dts2 / a.Q[0]);
 #pragma unroll
@@ -316,9 +329,10 @@ __global__ void k_nhc_chain(NhcArgs a, double *scal)
 #pragma unroll
             for (int ih = 0; ih < NH - 2; ih++)
                 if (ih < nh - 2) {
-                    v[ih] *= exp(-v[ih + 1] * dts8 / a.Q[ih + 1]);
+                    const double e = exp(-v[ih + 1] * dts8 / a.Q[ih + 1]);
+                    v[ih] *= e;
                     v[ih] += G[ih] * dts4;
-                    v[ih] *= exp(-v[ih + 1] * dts8 / a.Q[ih + 1]);
+                    v[ih] *= e;
                     G[ih + 1] = (v[ih] * v[ih] / a.Q[ih] - a.kT);
                 }
 #pragma unroll
