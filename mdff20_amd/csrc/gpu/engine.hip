@@ -156,6 +156,7 @@ struct mdff_ctx {
     int wpart_ready = 0;      // gwp holds the channel maxima of the next split-FP16 forward sum's weights
     int wpart_nb = 0;         // with wpart_ready: partials per channel in gwp (0: one per 256 selected ions)
     int tot_done = 0;         // k_mu_tot formed mu_tot, the self terms and (2: also) gwp for the force call
+    int kin_fresh = 0;        // ke and S_KIN hold the current velocities' kinetic energy (within one run call)
     int mf_blocks = 512;      // target blocks of the MFMA kernels
     int nsplit_max = 1;       // (ny,nz) splits the bpart buffer holds
     // host pinned mirrors
@@ -1747,7 +1748,7 @@ static int step_vv(mdff_ctx *c, int istep)
 }
 
 // nhcn integration.c:140-182 with chain_nhcn :189-294
-static void nhc_chain(mdff_ctx *c)
+static void nhc_chain(mdff_ctx *c, bool reduce = true)
 {
     NhcArgs a;
     a.nh = c->p.nhc_n;
@@ -1761,15 +1762,21 @@ static void nhc_chain(mdff_ctx *c)
     hipLaunchKernelGGL(k_nhc_chain, dim3(1), dim3(1), 0, c->st, a, c->scal);
     hipLaunchKernelGGL(k_scale_v, dim3(cdiv(c->n, 256)), dim3(256), 0, c->st, c->n, c->scal + S_NHC_S, c->v,
                        c->massd, c->ke);
-    kin_reduce(c);
+    if (reduce) kin_reduce(c);
 }
 
 static int step_nhc(mdff_ctx *c, int istep)
 {
     int n = c->n;
-    hipLaunchKernelGGL(k_ke, dim3(cdiv(n, 256)), dim3(256), 0, c->st, n, c->v, c->massd, c->ke);
-    kin_reduce(c);
-    nhc_chain(c);
+    // the kinetic energy of the velocities the last step left (formed at its
+    // end, unless anything changed the velocities since)
+    if (!c->kin_fresh) {
+        hipLaunchKernelGGL(k_ke, dim3(cdiv(n, 256)), dim3(256), 0, c->st, n, c->v, c->massd, c->ke);
+        kin_reduce(c);
+    }
+    c->kin_fresh = 0;
+    // first half: S_KIN after the scaling is not read before k_kick forms it again
+    nhc_chain(c, false);
     hipLaunchKernelGGL(k_kick_drift, dim3(cdiv(n, 256)), dim3(256), 0, c->st, n, c->dt, c->r, c->v, c->f,
                        c->invmass);
     int rc = engforce_impl(c, istep);
@@ -1784,6 +1791,7 @@ static int step_nhc(mdff_ctx *c, int istep)
     a.L = 3.0 * (double)n;
     for (int i = 0; i < 16; i++) a.Q[i] = c->nhcQ[i];
     hipLaunchKernelGGL(k_nhc_envt, dim3(1), dim3(1), 0, c->st, a, c->scal);
+    c->kin_fresh = 1;
     return rc;
 }
 
@@ -1826,7 +1834,10 @@ static int md_step_impl(mdff_ctx *c, int istep)
         int r2 = build_list(c, 0);
         if (r2) return r2;
     }
-    if (c->rescale_allowed && istep < c->p.nequil && c->p.nequilT > 0 && istep % c->p.nequilT == 0) rescale_dev(c);
+    if (c->rescale_allowed && istep < c->p.nequil && c->p.nequilT > 0 && istep % c->p.nequilT == 0) {
+        rescale_dev(c);
+        c->kin_fresh = 0;
+    }
     return rc;
 }
 
@@ -2544,6 +2555,7 @@ int mdff_md_init(mdff_ctx *c, mdff_thermo *th)
 int mdff_md_step(mdff_ctx *c, int istep, mdff_thermo *th)
 {
     if (!c) return MDFF_ERR_ARG;
+    c->kin_fresh = 0;   // velocities may have been set since the last call
     int rc = md_step_impl(c, istep);
     if (rc && rc != MDFF_ERR_SCF) return rc;
     rc = finish_call(c, rc, th);
@@ -2555,6 +2567,7 @@ int mdff_md_run(mdff_ctx *c, int first_step, int nsteps, mdff_thermo *th)
 {
     if (!c) return MDFF_ERR_ARG;
     int worst = MDFF_OK;
+    c->kin_fresh = 0;   // velocities may have been set since the last call
     for (int k = 0; k < nsteps; k++) {
         int rc = md_step_impl(c, first_step + k);
         if (rc == MDFF_ERR_SCF) worst = rc;
