@@ -1615,10 +1615,11 @@ static void coulomb_force(mdff_ctx *c, int lo)
         c->wpart_nb = cdiv(n, 256);
     }
     rec_part(c, 0, lo, C, NCH, c->q, c->mu_tot, c->q, c->mu_tot, 0, 1, 1, nullptr, nullptr);
-    colsum(c, c->ek, (int)c->nkf, (int)c->nkf, 7, c->scal + S_U_REC);
     if (!done)
         hipLaunchKernelGGL(k_self_terms, dim3(cdiv(n, 256)), dim3(256), 0, c->st, n, c->q, c->mu_tot, c->selfbuf);
-    colsum(c, c->selfbuf, n, n, 2, c->scal + S_QSQ);
+    // the reciprocal-space energy/stress terms and the self terms, one launch
+    hipLaunchKernelGGL(k_colsum2, dim3(7 + 2), dim3(1024), 0, c->st, c->ek, (int)c->nkf, (int)c->nkf, 7,
+                       c->scal + S_U_REC, c->selfbuf, n, n, c->scal + S_QSQ);
 }
 
 static int engforce_impl(mdff_ctx *c, int istep)

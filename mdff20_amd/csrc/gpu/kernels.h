@@ -5,12 +5,9 @@
 // ---------------------------------------------------------------- reductions
 // out[k] = sum_i in[k*ld + i], i < n; one 1024-thread block per column k,
 // fixed summation order (bitwise reproducible).
-__global__ void __launch_bounds__(1024) k_colsum(const double *__restrict__ in, int n, int ld,
-                                                 double *__restrict__ out, const int *active)
+__device__ __forceinline__ void colsum_block(const double *__restrict__ col, int n, double *__restrict__ out)
 {
-    if (active && !*active) return;
     __shared__ double sh[16];
-    const double *col = in + (size_t)blockIdx.x * ld;
     // two independent accumulators, then wave and block sums in a fixed order
     double s0 = 0.0, s1 = 0.0;
     int i = threadIdx.x;
@@ -28,8 +25,26 @@ __global__ void __launch_bounds__(1024) k_colsum(const double *__restrict__ in, 
         double t = threadIdx.x < 16 ? sh[threadIdx.x] : 0.0;
 #pragma unroll
         for (int o = 8; o > 0; o >>= 1) t += __shfl_xor(t, o);
-        if (threadIdx.x == 0) out[blockIdx.x] = t;
+        if (threadIdx.x == 0) *out = t;
     }
+}
+
+__global__ void __launch_bounds__(1024) k_colsum(const double *__restrict__ in, int n, int ld,
+                                                 double *__restrict__ out, const int *active)
+{
+    if (active && !*active) return;
+    colsum_block(in + (size_t)blockIdx.x * ld, n, out + blockIdx.x);
+}
+
+// two k_colsum jobs in one launch: blocks 0..nc1-1 the first, the rest the
+// second (same order of summation as k_colsum)
+__global__ void __launch_bounds__(1024) k_colsum2(const double *__restrict__ in1, int n1, int ld1, int nc1,
+                                                  double *__restrict__ out1, const double *__restrict__ in2, int n2,
+                                                  int ld2, double *__restrict__ out2)
+{
+    const int b = blockIdx.x;
+    if (b < nc1) colsum_block(in1 + (size_t)b * ld1, n1, out1 + b);
+    else colsum_block(in2 + (size_t)(b - nc1) * ld2, n2, out2 + (b - nc1));
 }
 
 // ------------------------------------------------------------ fractional coords
