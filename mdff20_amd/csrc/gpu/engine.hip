@@ -157,6 +157,8 @@ struct mdff_ctx {
     int wpart_nb = 0;         // with wpart_ready: partials per channel in gwp (0: one per 256 selected ions)
     int tot_done = 0;         // k_mu_tot formed mu_tot, the self terms and (2: also) gwp for the force call
     int kin_fresh = 0;        // ke and S_KIN hold the current velocities' kinetic energy (within one run call)
+    double *vc_part = nullptr;     // k_vcheck's per-block two largest displacements [2][n/256]
+    unsigned *vc_cnt = nullptr;    // k_vcheck's finished-block count
     int mf_blocks = 512;      // target blocks of the MFMA kernels
     int nsplit_max = 1;       // (ny,nz) splits the bpart buffer holds
     // host pinned mirrors
@@ -282,8 +284,8 @@ static int build_list(mdff_ctx *c, int force)
         } else {
             if (force || attempt > 0) set_int(c, C_FIRST, 1);
             // the check also clears the overflow flag and count of the build
-            hipLaunchKernelGGL(k_vcheck, dim3(1), dim3(1024), 0, c->st, n, c->cd, c->s, c->s0, c->p.skindiff, c->ctl,
-                               1);
+            hipLaunchKernelGGL(k_vcheck, dim3(cdiv(n, 256)), dim3(256), 0, c->st, n, c->cd, c->s, c->s0,
+                               c->p.skindiff, c->ctl, 1, c->vc_part, c->vc_cnt);
             if (attempt > 0) hipLaunchKernelGGL(k_add_int, dim3(1), dim3(1), 0, c->st, c->ctl + C_NUPD, -1);
         }
         hipLaunchKernelGGL(k_build_allpairs, dim3(cdiv(n, 4)), dim3(256), 0, c->st, n, c->cd, c->s, csr * csr,
@@ -1876,10 +1878,11 @@ void mdff_destroy(mdff_ctx *c)
     double **dp[] = {&c->r, &c->v, &c->f, &c->fold, &c->rs, &c->s, &c->s0, &c->invmass, &c->massd, &c->q, &c->qz,
                      &c->mu_static, &c->mu_ind, &c->mu_tot, &c->hist, &c->ef_stat, &c->ef, &c->ef_dir, &c->ef_rec,
                      &c->efg, &c->efg_dir, &c->efg_rec, &c->f_sr, &c->f_dir, &c->f_rec, &c->acc_sr, &c->uw_dir,
-                     &c->selfbuf, &c->ke, &c->upol_i, &c->rmsd_i, &c->tmp_q, &c->tmp_mu, &c->tmp_qraw, &c->tmp_mur, &c->dc_R, &c->dc_d, &c->dc_g, &c->cg_z, &c->cg_p, &c->cg_part, &c->chk_part, &c->pq_buf, &c->t2dir, &c->ko_f, &c->ko_dmu, &c->t2ko, &c->cg_w, &c->scal,
+                     &c->selfbuf, &c->ke, &c->upol_i, &c->rmsd_i, &c->tmp_q, &c->tmp_mu, &c->tmp_qraw, &c->tmp_mur, &c->dc_R, &c->dc_d, &c->dc_g, &c->cg_z, &c->cg_p, &c->cg_part, &c->chk_part, &c->pq_buf, &c->t2dir, &c->vc_part, &c->ko_f, &c->ko_dmu, &c->t2ko, &c->cg_w, &c->scal,
                      &c->kvec, &c->Ak, &c->kcoe, &c->W, &c->ek, &c->kpart, &c->bpart, &c->ksum, &c->gsc, &c->gwp, &c->gwq};
     for (auto pp : dp) dfree(*pp);
     dfree(c->type);
+    dfree(c->vc_cnt);
     c->ctl = nullptr;   // inside the scal allocation
     dfree(c->tab_d);
     dfree(c->pidx);
@@ -2041,6 +2044,7 @@ int mdff_create(mdff_ctx **out, const mdff_params *p, int nion, const int *typia
     ALLOC(c->cg_z, 3 * n); ALLOC(c->cg_p, 3 * n); ALLOC(c->cg_part, 4 * (size_t)cdiv(n, CG_BLOCK));
     ALLOC(c->chk_part, 2 * (size_t)cdiv(n, 256));
     ALLOC(c->pq_buf, (size_t)cdiv(n, 4)); ALLOC(c->t2dir, 3 * n);
+    ALLOC(c->vc_part, 2 * (size_t)cdiv(n, 256)); ALLOC(c->vc_cnt, 1);
     ALLOC(c->ko_f, 3 * n); ALLOC(c->ko_dmu, 3 * n); ALLOC(c->cg_w, 3 * n);
     // scal [S_COUNT doubles] then ctl [C_COUNT ints] in one allocation, so
     // that one copy brings both to the host

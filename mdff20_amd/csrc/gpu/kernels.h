@@ -62,42 +62,71 @@ __global__ void k_frac(int n, CellD c, const double *__restrict__ rx, const doub
 
 // ------------------------------------------------------------------ Verlet
 // check_verletlist verlet.c:124-170: the two largest displacements since the
-// last build; one block.
-__global__ void __launch_bounds__(1024) k_vcheck(int n, CellD c, const double *__restrict__ s,
-                                                 const double *__restrict__ s0, double skin, int *ctl, int reset = 0)
+// last build.  One ion per thread, 256-thread blocks: each block's two
+// largest go to part[2 b], part[2 b + 1]; the block that finishes last
+// (atomic count in *cnt, reset by it) merges them and decides the rebuild.
+// The pair is the two largest of the set whatever the merge order.
+__device__ __forceinline__ void top2_merge(double &a1, double &a2, double b1, double b2)
 {
-    __shared__ double m1s[1024], m2s[1024];
-    // reset: clear the overflow flag and the largest count of the build that
-    // follows
-    if (reset && threadIdx.x == 0) {
-        ctl[C_OVERFLOW] = 0;
-        ctl[C_MAXCNT] = 0;
-    }
-    double m1 = 0.0, m2 = 0.0;
-    for (int i = threadIdx.x; i < n; i += 1024) {
-        double x, y, z;
-        min_image(c, s[i] - s0[i], s[n + i] - s0[n + i], s[2 * n + i] - s0[2 * n + i], x, y, z);
-        double d = sqrt(x * x + y * y + z * z);
-        if (d > m1) { m2 = m1; m1 = d; }
-        else if (d > m2) m2 = d;
-    }
+    if (a1 >= b1) { a2 = fmax(a2, b1); }
+    else { a2 = fmax(b2, a1); a1 = b1; }
+}
+
+__device__ __forceinline__ void top2_block(double &m1, double &m2, double *m1s, double *m2s)
+{
     m1s[threadIdx.x] = m1;
     m2s[threadIdx.x] = m2;
     __syncthreads();
-    for (int o = 512; o > 0; o >>= 1) {
-        if (threadIdx.x < o) {
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) {
             double a1 = m1s[threadIdx.x], a2 = m2s[threadIdx.x];
-            double b1 = m1s[threadIdx.x + o], b2 = m2s[threadIdx.x + o];
-            double t1, t2;
-            if (a1 >= b1) { t1 = a1; t2 = fmax(a2, b1); }
-            else { t1 = b1; t2 = fmax(b2, a1); }
-            m1s[threadIdx.x] = t1;
-            m2s[threadIdx.x] = t2;
+            top2_merge(a1, a2, m1s[threadIdx.x + o], m2s[threadIdx.x + o]);
+            m1s[threadIdx.x] = a1;
+            m2s[threadIdx.x] = a2;
         }
         __syncthreads();
     }
+    m1 = m1s[0];
+    m2 = m2s[0];
+}
+
+__global__ void __launch_bounds__(256) k_vcheck(int n, CellD c, const double *__restrict__ s,
+                                                const double *__restrict__ s0, double skin, int *ctl, int reset,
+                                                double *part, unsigned *cnt)
+{
+    __shared__ double m1s[256], m2s[256];
+    __shared__ int last;
+    // reset: clear the overflow flag and the largest count of the build that
+    // follows
+    if (reset && blockIdx.x == 0 && threadIdx.x == 0) {
+        ctl[C_OVERFLOW] = 0;
+        ctl[C_MAXCNT] = 0;
+    }
+    const int nb = gridDim.x;
+    double m1 = 0.0, m2 = 0.0;
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < n) {
+        double x, y, z;
+        min_image(c, s[i] - s0[i], s[n + i] - s0[n + i], s[2 * n + i] - s0[2 * n + i], x, y, z);
+        m1 = sqrt(x * x + y * y + z * z);
+    }
+    top2_block(m1, m2, m1s, m2s);
     if (threadIdx.x == 0) {
-        int rebuild = ctl[C_FIRST] || (m1s[0] + m2s[0] > skin);
+        part[2 * blockIdx.x] = m1;
+        part[2 * blockIdx.x + 1] = m2;
+        __threadfence();
+        last = atomicAdd(cnt, 1u) == (unsigned)(nb - 1);
+    }
+    __syncthreads();
+    if (!last) return;
+    __threadfence();
+    double a1 = 0.0, a2 = 0.0;
+    for (int b = threadIdx.x; b < nb; b += 256) top2_merge(a1, a2, part[2 * b], part[2 * b + 1]);
+    __syncthreads();
+    top2_block(a1, a2, m1s, m2s);
+    if (threadIdx.x == 0) {
+        *cnt = 0u;
+        int rebuild = ctl[C_FIRST] || (a1 + a2 > skin);
         ctl[C_REBUILD] = rebuild;
         if (rebuild) ctl[C_NUPD] += 1;
         ctl[C_FIRST] = 0;
