@@ -159,6 +159,7 @@ struct mdff_ctx {
     int kin_fresh = 0;        // ke and S_KIN hold the current velocities' kinetic energy (within one run call)
     double *vc_part = nullptr;     // k_vcheck's per-block two largest displacements [2][n/256]
     unsigned *vc_cnt = nullptr;    // k_vcheck's finished-block count
+    unsigned *chk_cnt = nullptr;   // k_scf_check's finished-block count
     int mf_blocks = 512;      // target blocks of the MFMA kernels
     int nsplit_max = 1;       // (ny,nz) splits the bpart buffer holds
     // host pinned mirrors
@@ -1226,14 +1227,12 @@ static int induced_field(mdff_ctx *c, int lo, const double *mu, const double *ba
 }
 
 // rmsd of mu_ind against ef (and u_pol when upol is given) and the loop
-// condition of momentpolaSCF pim.c:236, in two passes
+// condition of momentpolaSCF pim.c:236, in one launch (k_scf_check)
 static void scf_check(mdff_ctx *c, const double *upol, int maxit, int minit)
 {
     const int nb = cdiv(c->n, 256);
-    hipLaunchKernelGGL(k_scf_check_part, dim3(nb), dim3(256), 0, c->st, c->n, c->type, c->tab_d, c->mu_ind, c->ef,
-                       upol, c->ctl, c->chk_part);
-    hipLaunchKernelGGL(k_scf_check_fin, dim3(1), dim3(256), 0, c->st, nb, c->chk_part, upol ? 1 : 0, c->ctl, c->scal,
-                       c->npol, maxit, minit, c->p.conv_tol_ind);
+    hipLaunchKernelGGL(k_scf_check, dim3(nb), dim3(256), 0, c->st, c->n, c->type, c->tab_d, c->mu_ind, c->ef, upol,
+                       c->ctl, c->chk_part, c->chk_cnt, c->scal, c->npol, maxit, minit, c->p.conv_tol_ind);
 }
 
 static void scf_finish(mdff_ctx *c)
@@ -1883,6 +1882,7 @@ void mdff_destroy(mdff_ctx *c)
     for (auto pp : dp) dfree(*pp);
     dfree(c->type);
     dfree(c->vc_cnt);
+    dfree(c->chk_cnt);
     c->ctl = nullptr;   // inside the scal allocation
     dfree(c->tab_d);
     dfree(c->pidx);
@@ -2044,7 +2044,7 @@ int mdff_create(mdff_ctx **out, const mdff_params *p, int nion, const int *typia
     ALLOC(c->cg_z, 3 * n); ALLOC(c->cg_p, 3 * n); ALLOC(c->cg_part, 4 * (size_t)cdiv(n, CG_BLOCK));
     ALLOC(c->chk_part, 2 * (size_t)cdiv(n, 256));
     ALLOC(c->pq_buf, (size_t)cdiv(n, 4)); ALLOC(c->t2dir, 3 * n);
-    ALLOC(c->vc_part, 2 * (size_t)cdiv(n, 256)); ALLOC(c->vc_cnt, 1);
+    ALLOC(c->vc_part, 2 * (size_t)cdiv(n, 256)); ALLOC(c->vc_cnt, 1); ALLOC(c->chk_cnt, 1);
     ALLOC(c->ko_f, 3 * n); ALLOC(c->ko_dmu, 3 * n); ALLOC(c->cg_w, 3 * n);
     // scal [S_COUNT doubles] then ctl [C_COUNT ints] in one allocation, so
     // that one copy brings both to the host

@@ -514,17 +514,22 @@ __device__ __forceinline__ double cg_sum_partials(const double *part, int nb, do
     return cg_block_sum(v, sh);
 }
 
-// k_scf_check in two passes: per-block partial sums of the rmsd terms (and
-// of u_pol_i when given) over 256 ions each, then the finishing block sums
-// the partials in block order and applies the loop condition
-__global__ void __launch_bounds__(256) k_scf_check_part(int n, const int *__restrict__ type,
-                                                        const TypeTab *__restrict__ T, const double *__restrict__ mu,
-                                                        const double *__restrict__ ef,
-                                                        const double *__restrict__ upol_i, const int *ctl,
-                                                        double *part)
+// the rmsd of get_rmsd_mu (pim.c:431-449) over mu and ef, u_pol (when upol_i
+// is given) and the loop condition of momentpolaSCF pim.c:236: per-block
+// partial sums over 256 ions each; the block that finishes last (atomic
+// count in *cnt, reset by it) sums the partials in block order and applies
+// the condition.  The flag read first is written only after every block
+// has counted, so all blocks take the same branch.
+__global__ void __launch_bounds__(256) k_scf_check(int n, const int *__restrict__ type, const TypeTab *__restrict__ T,
+                                                   const double *__restrict__ mu, const double *__restrict__ ef,
+                                                   const double *__restrict__ upol_i, int *ctl, double *part,
+                                                   unsigned *cnt, double *scal, int npol, int maxit, int minit,
+                                                   double tol)
 {
     if (!ctl[C_SCF_ACTIVE]) return;
     __shared__ double sh[CG_BLOCK];
+    __shared__ int last;
+    const int nb = gridDim.x;
     const int i = blockIdx.x * 256 + threadIdx.x;
     double r = 0.0, u = 0.0;
     if (i < n) {
@@ -544,28 +549,25 @@ __global__ void __launch_bounds__(256) k_scf_check_part(int n, const int *__rest
     const double su = upol_i ? cg_block_sum(u, sh) : 0.0;
     if (threadIdx.x == 0) {
         part[blockIdx.x] = sr;
-        part[gridDim.x + blockIdx.x] = su;
+        part[nb + blockIdx.x] = su;
+        __threadfence();
+        last = atomicAdd(cnt, 1u) == (unsigned)(nb - 1);
     }
-}
-
-__global__ void __launch_bounds__(256) k_scf_check_fin(int nb, const double *__restrict__ part, int with_upol,
-                                                       int *ctl, double *scal, int npol, int maxit, int minit,
-                                                       double tol)
-{
-    if (!ctl[C_SCF_ACTIVE]) return;
-    __shared__ double sh[CG_BLOCK];
-    const double r = cg_sum_partials(part, nb, sh);
-    const double u = with_upol ? cg_sum_partials(part + nb, nb, sh) : 0.0;
+    __syncthreads();
+    if (!last) return;
+    __threadfence();
+    const double rs = cg_sum_partials(part, nb, sh);
+    const double us = upol_i ? cg_sum_partials(part + nb, nb, sh) : 0.0;
     if (threadIdx.x == 0) {
-        scal[S_RMSD2] = r;
-        if (with_upol) scal[S_UPOL] = u;
+        *cnt = 0u;
+        scal[S_RMSD2] = rs;
+        if (upol_i) scal[S_UPOL] = us;
         const int iscf = ctl[C_ISCF] + 1;
-        const double rmsd = sqrt(r / (double)(npol > 0 ? npol : 1));
+        const double rmsd = sqrt(rs / (double)(npol > 0 ? npol : 1));
         ctl[C_ISCF] = iscf;
         ctl[C_SCF_ACTIVE] = ((iscf < maxit && rmsd > tol) || iscf < minit) ? 1 : 0;
     }
 }
-
 
 // r = E - pol^-T mu, z = p = pol^T r (= pol^T E - mu), d = 0; partials of
 // r.z; with wpart, the block's channel maxima of |p| (cg_block_max3)
