@@ -1235,11 +1235,12 @@ static void scf_check(mdff_ctx *c, const double *upol, int maxit, int minit)
                        c->ctl, c->chk_part, c->chk_cnt, c->scal, c->npol, maxit, minit, c->p.conv_tol_ind);
 }
 
-static void scf_finish(mdff_ctx *c)
+static void scf_finish(mdff_ctx *c, bool store_hist = true)
 {
     int n = c->n;
     // dipia_ind[ia][0] = mu_ind (pim.c:281-285)
-    hipLaunchKernelGGL(k_store_hist, dim3(cdiv(3 * n, 256)), dim3(256), 0, c->st, n, c->mu_ind, c->hist);
+    if (store_hist)
+        hipLaunchKernelGGL(k_store_hist, dim3(cdiv(3 * n, 256)), dim3(256), 0, c->st, n, c->mu_ind, c->hist);
     c->last_nscf = c->ctl_h[C_ISCF];
     c->last_rmsd = sqrt(c->scal_h[S_RMSD2] / (double)(c->npol > 0 ? c->npol : 1));
 }
@@ -1494,10 +1495,11 @@ static int scf_loop_anchored(mdff_ctx *c, int istep)
         if (!c->ctl_h[C_SCF_ACTIVE] || launched >= limit) break;
         step();
     }
-    // u_pol of the final dipoles (pim.c:253,299)
-    hipLaunchKernelGGL(k_upol, dim3(cdiv(n, 256)), dim3(256), 0, c->st, n, c->type, c->tab_d, c->mu_ind, c->upol_i);
-    colsum(c, c->upol_i, n, n, 1, c->scal + S_UPOL);
-    scf_finish(c);
+    // u_pol of the final dipoles (pim.c:253,299), its total and the history
+    // store in one launch
+    hipLaunchKernelGGL(k_upol_hist, dim3(cdiv(n, 256)), dim3(256), 0, c->st, n, c->type, c->tab_d, c->mu_ind,
+                       c->upol_i, c->hist, c->chk_part, c->chk_cnt, c->scal);
+    scf_finish(c, false);
     if (c->mf_anchor) c->n_scf_inner += 1;
     else c->n_scf_fp64 += 1;
     c->n_scf_inner += std::max(0, c->last_nscf - 1);

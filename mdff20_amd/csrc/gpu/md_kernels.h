@@ -569,6 +569,49 @@ __global__ void __launch_bounds__(256) k_scf_check(int n, const int *__restrict_
     }
 }
 
+// the end of an SCF: u_pol of the final dipoles per ion (k_upol, get_upol
+// pim.c:145-161) and its total in scal[S_UPOL] (block partials, summed in
+// block order by the block that finishes last: atomic count in *cnt, reset
+// by it), and dipia_ind[ia][0] = mu (pim.c:281-285, k_store_hist)
+__global__ void __launch_bounds__(256) k_upol_hist(int n, const int *__restrict__ type, const TypeTab *__restrict__ T,
+                                                   const double *__restrict__ mu, double *__restrict__ upol_i,
+                                                   double *__restrict__ hist, double *part, unsigned *cnt,
+                                                   double *scal)
+{
+    __shared__ double sh[CG_BLOCK];
+    __shared__ int last;
+    const int nb = gridDim.x;
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    double u = 0.0;
+    if (i < n) {
+        const int t = type[i];
+        double m[3] = {mu[i], mu[n + i], mu[2 * n + i]};
+        double up = 0.0;
+        if (T->polar[t]) {
+            const double *I = T->invpol[t];
+            for (int a = 0; a < 3; a++)
+                for (int b = 0; b < 3; b++) up += m[a] * I[3 * a + b] * m[b];
+        }
+        u = 0.5 * up;
+        upol_i[i] = u;
+        for (int a = 0; a < 3; a++) hist[(size_t)a * n + i] = m[a];
+    }
+    const double su = cg_block_sum(u, sh);
+    if (threadIdx.x == 0) {
+        part[blockIdx.x] = su;
+        __threadfence();
+        last = atomicAdd(cnt, 1u) == (unsigned)(nb - 1);
+    }
+    __syncthreads();
+    if (!last) return;
+    __threadfence();
+    const double tot = cg_sum_partials(part, nb, sh);
+    if (threadIdx.x == 0) {
+        *cnt = 0u;
+        scal[S_UPOL] = tot;
+    }
+}
+
 // r = E - pol^-T mu, z = p = pol^T r (= pol^T E - mu), d = 0; partials of
 // r.z; with wpart, the block's channel maxima of |p| (cg_block_max3)
 __global__ void __launch_bounds__(CG_BLOCK) k_cg_init(int n, int npol, const int *__restrict__ pidx,
