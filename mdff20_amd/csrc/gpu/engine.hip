@@ -1764,9 +1764,14 @@ static void nhc_chain(mdff_ctx *c, bool reduce = true)
     for (int i = 0; i < 16; i++) a.Q[i] = c->nhcQ[i];
     for (int i = 0; i < 7; i++) a.w[i] = c->yosh[i];
     hipLaunchKernelGGL(k_nhc_chain, dim3(1), dim3(1), 0, c->st, a, c->scal);
-    hipLaunchKernelGGL(k_scale_v, dim3(cdiv(c->n, 256)), dim3(256), 0, c->st, c->n, c->scal + S_NHC_S, c->v,
-                       c->massd, c->ke);
-    if (reduce) kin_reduce(c);
+    // with reduce, the scaling kernel also sums the kinetic energy into S_KIN
+    // (the SCF-check partials and count are free here)
+    if (reduce)
+        hipLaunchKernelGGL(k_scale_v, dim3(cdiv(c->n, 256)), dim3(256), 0, c->st, c->n, c->scal + S_NHC_S, c->v,
+                           c->massd, c->ke, c->chk_part, c->chk_cnt, c->scal + S_KIN);
+    else
+        hipLaunchKernelGGL(k_scale_v, dim3(cdiv(c->n, 256)), dim3(256), 0, c->st, c->n, c->scal + S_NHC_S, c->v,
+                           c->massd, c->ke, (double *)nullptr, (unsigned *)nullptr, (double *)nullptr);
 }
 
 static int step_nhc(mdff_ctx *c, int istep)
@@ -1785,9 +1790,9 @@ static int step_nhc(mdff_ctx *c, int istep)
                        c->invmass);
     int rc = engforce_impl(c, istep);
     if (rc && rc != MDFF_ERR_SCF) return rc;
+    // the kick also sums the kinetic energy into S_KIN
     hipLaunchKernelGGL(k_kick, dim3(cdiv(n, 256)), dim3(256), 0, c->st, n, c->dt, c->v, c->f, c->invmass, c->massd,
-                       c->ke);
-    kin_reduce(c);
+                       c->ke, c->chk_part, c->chk_cnt, c->scal + S_KIN);
     nhc_chain(c);
     NhcArgs a;
     a.nh = c->p.nhc_n;

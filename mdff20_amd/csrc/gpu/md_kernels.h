@@ -246,16 +246,25 @@ __global__ void k_kick_drift(int n, double dt, double *r, double *v, const doubl
 }
 
 // prop_pos_vel_verlet integration.c:83-88
-__global__ void k_kick(int n, double dt, double *v, const double *f, const double *invm, const double *m, double *ke)
+__device__ void sum_to_scalar(double v, double *part, unsigned *cnt, double *out);
+
+// with part (256-thread blocks), the kinetic energy's total also goes to
+// *kin (sum_to_scalar)
+__global__ void k_kick(int n, double dt, double *v, const double *f, const double *invm, const double *m, double *ke,
+                       double *part = nullptr, unsigned *cnt = nullptr, double *kin = nullptr)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    double dt2 = dt * 0.5;
-    for (int a = 0; a < 3; a++) {
-        size_t k = (size_t)a * n + i;
-        v[k] += f[k] * dt2 * invm[i];
+    double e = 0.0;
+    if (i < n) {
+        double dt2 = dt * 0.5;
+        for (int a = 0; a < 3; a++) {
+            size_t k = (size_t)a * n + i;
+            v[k] += f[k] * dt2 * invm[i];
+        }
+        e = ke_of(v[i], v[n + i], v[2 * n + i], m[i]);
+        ke[i] = e;
     }
-    ke[i] = ke_of(v[i], v[n + i], v[2 * n + i], m[i]);
+    if (part) sum_to_scalar(e, part, cnt, kin);
 }
 
 __global__ void k_ke(int n, const double *v, const double *m, double *ke)
@@ -265,15 +274,20 @@ __global__ void k_ke(int n, const double *v, const double *m, double *ke)
     ke[i] = ke_of(v[i], v[n + i], v[2 * n + i], m[i]);
 }
 
-__global__ void k_scale_v(int n, const double *s, double *v, const double *m, double *ke)
+__global__ void k_scale_v(int n, const double *s, double *v, const double *m, double *ke, double *part = nullptr,
+                          unsigned *cnt = nullptr, double *kin = nullptr)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    double f = *s;
-    v[i] *= f;
-    v[n + i] *= f;
-    v[2 * n + i] *= f;
-    ke[i] = ke_of(v[i], v[n + i], v[2 * n + i], m[i]);
+    double e = 0.0;
+    if (i < n) {
+        double f = *s;
+        v[i] *= f;
+        v[n + i] *= f;
+        v[2 * n + i] *= f;
+        e = ke_of(v[i], v[n + i], v[2 * n + i], m[i]);
+        ke[i] = e;
+    }
+    if (part) sum_to_scalar(e, part, cnt, kin);
 }
 
 struct NhcArgs {
@@ -512,6 +526,29 @@ __device__ __forceinline__ double cg_sum_partials(const double *part, int nb, do
     double v = 0.0;
     for (int b = threadIdx.x; b < nb; b += CG_BLOCK) v += part[b];
     return cg_block_sum(v, sh);
+}
+
+// the sum over the grid of one value per thread into *out: block sums in
+// part[block] (CG_BLOCK-thread blocks), summed in block order by the block
+// that finishes last (atomic count in *cnt, reset by it)
+__device__ void sum_to_scalar(double v, double *part, unsigned *cnt, double *out)
+{
+    __shared__ double sh[CG_BLOCK];
+    __shared__ int last;
+    const double b = cg_block_sum(v, sh);
+    if (threadIdx.x == 0) {
+        part[blockIdx.x] = b;
+        __threadfence();
+        last = atomicAdd(cnt, 1u) == (unsigned)(gridDim.x - 1);
+    }
+    __syncthreads();
+    if (!last) return;
+    __threadfence();
+    const double tot = cg_sum_partials(part, gridDim.x, sh);
+    if (threadIdx.x == 0) {
+        *cnt = 0u;
+        *out = tot;
+    }
 }
 
 // the rmsd of get_rmsd_mu (pim.c:431-449) over mu and ef, u_pol (when upol_i
