@@ -938,6 +938,7 @@ struct FieldFuse {
     double *pq_part;   // optional: the CG step's p.q partial sums (k_sk_back_fin)
     const void *t2;    // optional: pair tensors whose field of mu the finish forms (k_sk_back_fin_t2)
     int t2f32;         // t2 in FP32
+    int join;          // join the second stream (its real-space field in ef_dir) before the finish
 };
 
 static int rec_part(mdff_ctx *c, int polar, int lo, int C, int NCH, const double *q_w, const double *mu_w,
@@ -1046,6 +1047,7 @@ static int rec_part(mdff_ctx *c, int polar, int lo, int C, int NCH, const double
         fa.type = c->type;
         fa.T = c->tab_d;
         fa.pq_part = ff ? ff->pq_part : nullptr;
+        if (ff && ff->join) aux_join(c);
         if (t2r.done) {
             // the pair-tensor field came out of the back-projection's launch
             fa.ef_dir = c->t2dir;
@@ -1106,7 +1108,6 @@ static void short_range(mdff_ctx *c)
 // with damping, field only
 static void static_field(mdff_ctx *c, int lo)
 {
-    int n = c->n;
     // real space on the second stream, k space here
     aux_fork(c);
     {
@@ -1115,10 +1116,10 @@ static void static_field(mdff_ctx *c, int lo)
         launch_dir(c, a, lo, 1, 0, 0, 0);
     }
     int C = c->ldip_static ? 4 : 1;
-    rec_part(c, 0, lo, C, 3, c->q, c->mu_static, c->q, c->mu_static, 1, 0, 0, nullptr, nullptr);
-    aux_join(c);
-    hipLaunchKernelGGL(k_field_sum, dim3(cdiv(3 * n, 256)), dim3(256), 0, c->st, n, c->selfa2, c->mu_static,
-                       c->ef_dir, c->ef_rec, (const double *)nullptr, c->ef_stat, (const int *)nullptr);
+    // the back-projection's finish forms ef_stat = self + dir + rec (after
+    // joining the real-space field), in place of k_field_sum
+    FieldFuse ff{c->ef_stat, c->mu_static, nullptr, nullptr, nullptr, 0, 1};
+    rec_part(c, 0, lo, C, 3, c->q, c->mu_static, c->q, c->mu_static, 1, 0, 0, nullptr, nullptr, &ff);
 }
 
 // ASPC order and coefficients of extrapolate_dipole_aspc pim.c:324-379
