@@ -113,9 +113,9 @@ def system(nunits, seed):
     return L, pos, typ
 
 
-def equilibrate(L, pos, typ, steps):
+def equilibrate(L, pos, typ, steps, kmax=16):
     import mdff20_amd as m
-    p = params_of(control(False))
+    p = params_of(control(False, kmax=kmax))
     A = np.diag([L, L, L])
     e = m.Engine(p, typ, A, pos.T.copy(), precision=m.FP64)
     e.init_velocities()
@@ -126,6 +126,47 @@ def equilibrate(L, pos, typ, steps):
     e.close()
     log(f"rigid-ion relaxation: {steps} steps in {time.time() - t0:.1f} s, T = {th['temp_r']:.0f} K")
     return r
+
+
+def batch_systems(nconf, seed, nmin=96, nmax=1536, rho=(2.0, 2.6), eps=1e-6):
+    """uneven batch of SiO2-like cells: N log-uniform in [nmin, nmax] (a
+    multiple of 3), density uniform in rho (g/cm^3), seeded 1.6 A packings;
+    per-cell Ewald alpha / kmax / cutoffs for relative accuracy eps
+    (rc = min(9 A, L/2))"""
+    import mdff20_amd as m
+    rng = np.random.default_rng(seed)
+    confs, ews = [], []
+    for c in range(nconf):
+        n = int(np.exp(rng.uniform(np.log(nmin), np.log(nmax))))
+        nu = max(nmin // 3, min(nmax // 3, int(round(n / 3))))
+        dens = rng.uniform(*rho)
+        L = (nu * (M_SI + 2 * M_O) / (dens / RHO_UNIT)) ** (1.0 / 3.0)
+        pos = packing(3 * nu, L, int(rng.integers(1 << 30)))
+        typ = np.array([0] * (2 * nu) + [1] * nu, dtype=np.int32)
+        confs.append((typ, np.diag([L, L, L]), pos.T.copy()))
+        ews.append(m.batch.ewald_for_accuracy(L, eps=eps))
+    return confs, ews
+
+
+def batch_relax(confs, ews, iters=30, gamma=0.01, cap=0.05):
+    """steepest descent of the rigid-ion model on a whole batch (moves
+    capped at `cap` A per ion and iteration): separates the close pairs of
+    a raw packing before the polarisable model sees it"""
+    import mdff20_amd as m
+    p = params_of(control(False))
+    t0 = time.time()
+    with m.batch.Batch(p, confs, ews, precision=m.FP64) as b:
+        r = np.concatenate([c[2] for c in confs], axis=1)
+        for _ in range(iters):
+            b.set_positions(r)
+            b.engforce()
+            d = gamma * b.forces()
+            s = np.sqrt(np.sum(d * d, axis=0))
+            d *= np.minimum(1.0, cap / np.maximum(s, 1e-300))
+            r = r + d
+        parts = b.split(r)
+    log(f"batch relaxation: {len(confs)} cells, {iters} steepest-descent steps in {time.time() - t0:.1f} s")
+    return [(t, A, rr.copy()) for (t, A, _), rr in zip(confs, parts)]
 
 
 def timed_run(p, typ, A, r, precision, steps, warmup):
@@ -303,6 +344,11 @@ def main():
     ap.add_argument("--pmc-iters", type=int, default=5)
     ap.add_argument("--pmc-summarize", nargs=2, default=None, metavar=("FETCH_DIR", "WRITE_DIR"))
     ap.add_argument("--trace-summarize", default=None, metavar="DIR")
+    ap.add_argument("--no-kernels", action="store_true",
+                    help="skip the per-kernel timing of every arithmetic variant and the accuracy call "
+                         "(a kernel trace then holds the timed MD steps of the chosen forms only)")
+    ap.add_argument("--state", default=None, metavar="NPZ",
+                    help="relaxed positions: read from NPZ when it exists, else relaxed and written there")
     args = ap.parse_args()
     if args.trace_summarize:
         trace_summarize(args.trace_summarize)
@@ -323,7 +369,13 @@ def main():
     L, pos, typ = system(args.units, args.seed)
     n = len(typ)
     log(f"N = {n}, L = {L:.3f} A")
-    r = equilibrate(L, pos, typ, args.relax)
+    if args.state and os.path.exists(args.state):
+        r = np.load(args.state, allow_pickle=False)["r"]
+        log(f"relaxed positions read from {args.state}")
+    else:
+        r = equilibrate(L, pos, typ, args.relax)
+        if args.state:
+            np.savez(args.state, r=r)
     A = np.diag([L, L, L])
     p_text = control(True)
     p = params_of(p_text)
@@ -345,7 +397,7 @@ def main():
             f"SCF FP64 evaluations/step {stats['scf_fp64_per_step']:.1f}, inner {stats['scf_inner_per_step']:.1f}")
     # accuracy of the mixed form against the FP64 form on one configuration
     acc = None
-    if "fp64" in forms and "mixed" in forms:
+    if "fp64" in forms and "mixed" in forms and not args.no_kernels:
         ef, em = m.Engine(p, typ, A, r, precision=m.FP64), m.Engine(p, typ, A, r, precision=m.MIXED)
         tf, tm = ef.engforce(0), em.engforce(0)
         Ff, Fm = ef.forces(), em.forces()
@@ -369,7 +421,7 @@ def main():
     variants = [(0, "f64", PEAK["f64"])] + ([(1, "f32", PEAK["f32"]), (2, "bf16x3", PEAK["bf16"]),
                                             (3, "f16x3", PEAK["bf16"])] if head == "mixed" else [])
     kern = {}
-    for lo, vname, peak in variants:
+    for lo, vname, peak in ([] if args.no_kernels else variants):
         for which, name, ch in [(0, "sk_forward_scf", 3), (1, "sk_back_scf", 3), (2, "dir_induced", 0),
                                 (3, "sk_forward_force", 4), (4, "sk_back_force", 9)]:
             if which == 2 and lo >= 2:

@@ -61,7 +61,10 @@ int mdff_create(mdff_ctx **out, const mdff_params *p, int nion, const int *typia
                 int precision, int device);
 void mdff_destroy(mdff_ctx *ctx);
 
-/* state transfer (host <-> device, synchronous) */
+/* state transfer (host <-> device, synchronous).  mdff_set_positions keeps
+ * the Verlet list: after moving ions by more than half the skin, call
+ * mdff_check_verlet before the next force call, as run_md checks the list
+ * after every step (md.c:241). */
 int mdff_set_positions(mdff_ctx *ctx, const double *rx, const double *ry, const double *rz);
 int mdff_set_velocities(mdff_ctx *ctx, const double *vx, const double *vy, const double *vz);
 int mdff_get_positions(mdff_ctx *ctx, double *rx, double *ry, double *rz);
