@@ -60,6 +60,28 @@ void TT_damping_functions(double b, double c, double r, double *f, double *fd, i
 void check_verletlist(void);
 void gen_pbc_verletlist(void);
 
+/* prop_agate integration.h:3, integration.c:298-333: one step of the
+ * control file's integrator (nve-lf, nve-vv, nvt-nhcn) with its force call
+ * on the device.  The positions and velocities of the globals are uploaded
+ * first (the caller may have changed them, e.g. rescale_velocities), and
+ * rx/ry/rz, vx/vy/vz, fx/fy/fz, e_kin, e_nvt and the force call's globals
+ * come back.  The thermostat chain (vxi, xi) and the leap-frog r(t-dt)
+ * stay on the device. */
+void prop_agate(int step);
+
+/* run_md md.h:48, md.c:193-282: the whole MD loop on the device.  Between
+ * two steps with output the steps run as one device launch sequence (step,
+ * Verlet check, rescale, md.c:235-248); at an output step the globals are
+ * updated and the reference's own info_thermo / write_config write stdout,
+ * OSZIFF, CONTFF and TRAJFF as md.c:250-277 does.  The induced-dipole
+ * history stays on the device (dipia_ind[.][0] holds the last step's). */
+void run_md(void);
+
+/* invertmatrix3x3 inverse_matrix.h, inverse_matrix.c:6-23 (LAPACK there):
+ * the closed-form 3x3 inverse in place; init_config calls it for every
+ * polarisable ion */
+void invertmatrix3x3(double A[3][3]);
+
 /* the engine parameters the drop-ins build from the reference's parsed
  * globals (what the device context is created from) */
 int mdff_dropin_params(mdff_params *p);

@@ -13,8 +13,9 @@
  *   mdff_get_dipoles     <- void get_dipoles(mu, upol)         coulombic.h:17, coulombic.c:67-130
  *   mdff_pair_short      <- engforce_bhmftd_pbc / engforce_nmlj_pbc  bhmftd.h:22, nmlj.h:29
  *   mdff_check_verlet    <- void check_verletlist(void)        verlet.h:21, verlet.c:124-170
- *   mdff_md_step         <- void prop_agate(int) + the body of run_md's loop
- *                                                              integration.h:3, md.c:229-248
+ *   mdff_propagate       <- void prop_agate(int)               integration.h:3, integration.c:298-333
+ *   mdff_md_step         <- prop_agate + the rest of the body of run_md's loop
+ *                                                              md.c:229-248
  *   mdff_md_run          <- void run_md(void)                  md.h:48, md.c:193-282 (no file output)
  *   mdff_tt_damping      <- TT_damping_functions(b,c,r,f,fd,order)  tt_damp.h:7
  * The global-state same-name wrappers (engforce(), multipole_ES(), ...) that
@@ -113,6 +114,10 @@ int mdff_get_dipoles_scf(mdff_ctx *ctx, int istep, double *mu, double *upol);
 
 /* run_md start (md.c:196-211): kinetic energy + force call at t=0. */
 int mdff_md_init(mdff_ctx *ctx, mdff_thermo *th);
+/* prop_agate (integration.c:298-333): one step of the configured
+ * integrator (nve-lf, nve-vv or nvt-nhcn) with its force call, nothing else
+ * (no Verlet check, no rescale).  State stays on the device. */
+int mdff_propagate(mdff_ctx *ctx, int istep, mdff_thermo *th);
 /* One MD step (prop_agate + Verlet check + equilibration rescale,
  * md.c:235-248).  State stays on the device. */
 int mdff_md_step(mdff_ctx *ctx, int istep, mdff_thermo *th);

@@ -75,6 +75,7 @@ def lib():
         "mdff_get_dipoles_scf": (C.c_int, [vp, C.c_int, dp, dp]),
         "mdff_md_init": (C.c_int, [vp, P(Thermo)]),
         "mdff_md_step": (C.c_int, [vp, C.c_int, P(Thermo)]),
+        "mdff_propagate": (C.c_int, [vp, C.c_int, P(Thermo)]),
         "mdff_md_run": (C.c_int, [vp, C.c_int, C.c_int, P(Thermo)]),
         "mdff_set_tuning": (C.c_int, [vp, C.c_int, C.c_int, C.c_int]),
         "mdff_set_kspace_shape": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, C.c_int]),
@@ -368,6 +369,12 @@ class Engine:
     def md_step(self, istep):
         t = Thermo()
         _check(self._L.mdff_md_step(self._h, istep, C.byref(t)), "md_step")
+        return t.as_dict()
+
+    def propagate(self, istep):
+        """prop_agate: one integrator step with its force call (no list check, no rescale)"""
+        t = Thermo()
+        _check(self._L.mdff_propagate(self._h, istep, C.byref(t)), "propagate")
         return t.as_dict()
 
     def md_run(self, first_step, nsteps):

@@ -1831,14 +1831,19 @@ static void rescale_dev(mdff_ctx *c)
     hipLaunchKernelGGL(k_sub_mean, dim3(cdiv(3 * n, 256)), dim3(256), 0, c->st, n, c->scal + S_VMEAN, c->v);
 }
 
+// prop_agate integration.c:298-333: the integrator's step with its force call
+static int prop_step(mdff_ctx *c, int istep)
+{
+    switch (c->integ) {
+    case MDFF_NVE_LF: return step_lf(c, istep);
+    case MDFF_NVT_NHCN: return step_nhc(c, istep);
+    default: return step_vv(c, istep);
+    }
+}
+
 static int md_step_impl(mdff_ctx *c, int istep)
 {
-    int rc;
-    switch (c->integ) {
-    case MDFF_NVE_LF: rc = step_lf(c, istep); break;
-    case MDFF_NVT_NHCN: rc = step_nhc(c, istep); break;
-    default: rc = step_vv(c, istep); break;
-    }
+    int rc = prop_step(c, istep);
     if (rc && rc != MDFF_ERR_SCF) return rc;
     if (c->p.lverletL) {
         int r2 = build_list(c, 0);
@@ -2570,6 +2575,17 @@ int mdff_md_step(mdff_ctx *c, int istep, mdff_thermo *th)
     if (!c) return MDFF_ERR_ARG;
     c->kin_fresh = 0;   // velocities may have been set since the last call
     int rc = md_step_impl(c, istep);
+    if (rc && rc != MDFF_ERR_SCF) return rc;
+    rc = finish_call(c, rc, th);
+    if (th) th->istep = istep;
+    return rc;
+}
+
+int mdff_propagate(mdff_ctx *c, int istep, mdff_thermo *th)
+{
+    if (!c) return MDFF_ERR_ARG;
+    c->kin_fresh = 0;   // velocities may have been set since the last call
+    int rc = prop_step(c, istep);
     if (rc && rc != MDFF_ERR_SCF) return rc;
     rc = finish_call(c, rc, th);
     if (th) th->istep = istep;

@@ -25,10 +25,15 @@
 #include "ewald.h"
 #include "field.h"
 #include "global.h"
+#include "integration.h"
+#include "inverse_matrix.h"
+#include "io.h"
+#include "kinetic.h"
 #include "md.h"
 #include "nmlj.h"
 #include "pim.h"
 #include "thermo.h"
+#include "timing.h"
 #include "verlet.h"
 
 #include "mdff_dropin.h"
@@ -80,10 +85,25 @@ static void params_from_globals(mdff_params *p)
      * overwrites cutlongrange (field.c:329-330) */
     p->cutshortrange = srcutsq > 0.0 ? sqrt(srcutsq) : cutshortrange;
     p->cutlongrange = lrcutsq > 0.0 ? sqrt(lrcutsq) : cutlongrange;
-    /* the context only evaluates forces here; the integrator stays the
-     * reference's own */
-    p->integrator = MDFF_NVE_VV;
-    p->nequil = 0;
+    /* the integrator of md.c's globals (check_md has converted them to
+     * internal units, md.c:120-141; the context takes control-file units) for
+     * prop_agate / run_md; a static run only evaluates forces */
+    if (lstatic) {
+        p->integrator = MDFF_NVE_VV;
+        p->nequil = 0;
+    } else {
+        p->integrator = egrator == 0 ? MDFF_NVE_LF : (egrator == 2 ? MDFF_NVT_NHCN : MDFF_NVE_VV);
+        p->dt = dt / time_unit;
+        p->temp = temp / boltz_unit;
+        p->tauTberendsen = tauTberendsen / time_unit;
+        p->timesca_thermo = timesca_thermo / time_unit;
+        p->nhc_n = nhc_n;
+        p->nhc_mults = nhc_mults;
+        p->nhc_yosh_order = nhc_yosh_order;
+        p->npas = npas;
+        p->nequil = nequil;
+        p->nequilT = nequilT;
+    }
     p->lnmlj = lnmlj;
     p->lbhmft = lbhmft;
     p->lbhmftd = lbhmftd;
@@ -200,14 +220,10 @@ static void store_induced(mdff_ctx *c)
     free(mu);
 }
 
-static void engforce_impl(void)
+/* the globals engforce sets (field.c:352-407) from one force call */
+static void force_globals(mdff_ctx *c, const mdff_thermo *t, int count_scf)
 {
-    mdff_ctx *c = ctx_sync();
-    mdff_thermo th;
-    int rc = mdff_engforce(c, istep, &th);
-    if (rc && rc != MDFF_ERR_SCF) fail("mdff_engforce", rc);
-    rc = mdff_get_forces(c, fx, fy, fz);
-    if (rc) fail("mdff_get_forces", rc);
+    mdff_thermo th = *t;
     if (lnmlj) {
         u_lj = th.u_lj;
         pvir_lj = th.pvir_lj;
@@ -226,13 +242,136 @@ static void engforce_impl(void)
         if (lpol) {
             ldip = true;
             /* the dipole Ewald calls the reference's SCF would have made */
-            cc_multipole_ES_dir_pim += th.n_scf + 1;
-            cc_multipole_ES_rec_pim += th.n_scf + 1;
+            if (count_scf) {
+                cc_multipole_ES_dir_pim += th.n_scf + 1;
+                cc_multipole_ES_rec_pim += th.n_scf + 1;
+            }
             store_induced(c);
         }
         for (int it = 0; it < ntype; it++)
             for (int j = 0; j < 3; j++)
                 if (dipit[it][j] != 0.0) ldip = true;
+    }
+}
+
+static void engforce_impl(void)
+{
+    mdff_ctx *c = ctx_sync();
+    mdff_thermo th;
+    int rc = mdff_engforce(c, istep, &th);
+    if (rc && rc != MDFF_ERR_SCF) fail("mdff_engforce", rc);
+    rc = mdff_get_forces(c, fx, fy, fz);
+    if (rc) fail("mdff_get_forces", rc);
+    force_globals(c, &th, 1);
+}
+
+/* positions, velocities and forces of the device into the globals */
+static void state_down(mdff_ctx *c)
+{
+    int rc = mdff_get_positions(c, rx, ry, rz);
+    if (!rc) rc = mdff_get_velocities(c, vx, vy, vz);
+    if (!rc) rc = mdff_get_forces(c, fx, fy, fz);
+    if (rc) fail("state download", rc);
+}
+
+/* the globals a propagator sets beside the force call's: e_kin, and e_nvt
+ * of the chain (integration.c:22-294) */
+static void step_globals(mdff_ctx *c, const mdff_thermo *th, int count_scf)
+{
+    force_globals(c, th, count_scf);
+    e_kin = th->e_kin;
+    e_nvt = th->e_nvt;
+    updatevl = th->verlet_updates;
+}
+
+static void prop_agate_impl(int step)
+{
+    mdff_ctx *c = ctx_sync();
+    /* the reference's loop may have rescaled the velocities since */
+    int rc = mdff_set_velocities(c, vx, vy, vz);
+    if (rc) fail("mdff_set_velocities", rc);
+    mdff_thermo th;
+    rc = mdff_propagate(c, step, &th);
+    if (rc && rc != MDFF_ERR_SCF) fail("mdff_propagate", rc);
+    state_down(c);
+    step_globals(c, &th, 1);
+}
+
+/* whether run_md writes anything after step s (md.c:250-277) */
+static int output_step(int s)
+{
+    return iopnode(s, npas, nprint) || iopnode(s, npas, fprint) || iopnode(s, npas, cprint) ||
+           iopsnode(s, npas, tprint, tstart);
+}
+
+static void run_md_impl(void)
+{
+    mdff_ctx *c = ctx_sync();
+    int rc = mdff_set_velocities(c, vx, vy, vz);
+    if (rc) fail("mdff_set_velocities", rc);
+    /* e_kin and the force call at t=0 (md.c:196-209) */
+    mdff_thermo th;
+    rc = mdff_md_init(c, &th);
+    if (rc && rc != MDFF_ERR_SCF) fail("mdff_md_init", rc);
+    rc = mdff_get_forces(c, fx, fy, fz);
+    if (rc) fail("mdff_get_forces", rc);
+    step_globals(c, &th, 1);
+    long long k0[4];
+    if ((rc = mdff_get_counters(c, k0))) fail("mdff_get_counters", rc);
+    if (ionode) {
+        SEPARATOR;
+        printf("properties at t=0\n");
+        SEPARATOR;
+        putchar('\n');
+    }
+    info_thermo(0, NULL);
+    if (ionode && npas > 0) {
+        SEPARATOR;
+        printf("starting main MD loop\n");
+        SEPARATOR;
+        putchar('\n');
+    }
+    FILE *fpOSZIFF = fopen("OSZIFF", "w");
+    if (NULL == fpOSZIFF) {
+        pError("opening OSZIFF file");
+        exit(-1);
+    }
+    /* the loop of md.c:228-279 on the device: the steps between two outputs
+     * run as one mdff_md_run (prop_agate, Verlet check, rescale), and the
+     * state comes to the host only where something is written */
+    statime(0);
+    for (int first = 1; first <= npas;) {
+        int last = first;
+        while (last < npas && !output_step(last)) last++;
+        rc = mdff_md_run(c, first, last - first + 1, &th);
+        if (rc && rc != MDFF_ERR_SCF) fail("mdff_md_run", rc);
+        istep = last;
+        step_globals(c, &th, 0);
+        if (iopnode(istep, npas, nprint)) {
+            statime(1);
+            mestime(&mdstepCPUtime, 1, 0);
+            writime("MD", istep, 1, 0);
+            statime(0);
+            info_thermo(0, NULL);
+        }
+        if (iopnode(istep, npas, fprint)) info_thermo(1, fpOSZIFF);
+        const int wc = iopnode(istep, npas, cprint), wt = iopsnode(istep, npas, tprint, tstart);
+        if (wc || wt) state_down(c);
+        if (wc) write_config("CONTFF", 'w');
+        if (wt) write_config("TRAJFF", 'a');
+        first = last + 1;
+    }
+    state_down(c);
+    write_config("CONTFF", 'w');
+    fclose(fpOSZIFF);
+    /* the SCF's dipole Ewald calls of all steps: one static-field call per
+     * force call, one per iteration and per inner scfkO sweep */
+    long long k1[4];
+    if ((rc = mdff_get_counters(c, k1))) fail("mdff_get_counters", rc);
+    if (lpim) {
+        const long long calls = (k1[0] - k0[0]) + (k1[1] - k0[1]) + (k1[2] - k0[2]);
+        cc_multipole_ES_dir_pim += (int)calls;
+        cc_multipole_ES_rec_pim += (int)calls;
     }
 }
 
@@ -322,6 +461,30 @@ static void check_verletlist_impl(void)
 
 void gen_pbc_verletlist(void) { check_verletlist(); }
 
+/* invertmatrix3x3 inverse_matrix.c:6-23 (LAPACK dgetrf/dgetri there): the
+ * inverse by the adjugate, in place; a singular matrix gives non-finite
+ * entries, where LAPACK's result is not reproduced */
+void invertmatrix3x3(double A[3][3])
+{
+    double m[3][3];
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) m[i][j] = A[i][j];
+    const double c00 = m[1][1] * m[2][2] - m[1][2] * m[2][1];
+    const double c01 = m[1][2] * m[2][0] - m[1][0] * m[2][2];
+    const double c02 = m[1][0] * m[2][1] - m[1][1] * m[2][0];
+    const double det = m[0][0] * c00 + m[0][1] * c01 + m[0][2] * c02;
+    const double id = 1.0 / det;
+    A[0][0] = c00 * id;
+    A[1][0] = c01 * id;
+    A[2][0] = c02 * id;
+    A[0][1] = (m[0][2] * m[2][1] - m[0][1] * m[2][2]) * id;
+    A[1][1] = (m[0][0] * m[2][2] - m[0][2] * m[2][0]) * id;
+    A[2][1] = (m[0][1] * m[2][0] - m[0][0] * m[2][1]) * id;
+    A[0][2] = (m[0][1] * m[1][2] - m[0][2] * m[1][1]) * id;
+    A[1][2] = (m[0][2] * m[1][0] - m[0][0] * m[1][2]) * id;
+    A[2][2] = (m[0][0] * m[1][1] - m[0][1] * m[1][0]) * id;
+}
+
 /* ---- the same-name entry points: the bodies above on a private rand() state */
 void engforce(void)
 {
@@ -358,5 +521,19 @@ void check_verletlist(void)
 {
     char *rs = rng_enter();
     check_verletlist_impl();
+    rng_leave(rs);
+}
+
+void prop_agate(int step)
+{
+    char *rs = rng_enter();
+    prop_agate_impl(step);
+    rng_leave(rs);
+}
+
+void run_md(void)
+{
+    char *rs = rng_enter();
+    run_md_impl();
     rng_leave(rs);
 }

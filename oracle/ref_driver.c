@@ -4,17 +4,17 @@
  * holds the reference's own src/*.c compiled unchanged except for
  * inverse_matrix.c (needs lapacke.h, absent) and mdff20.c (its main).
  *
- * It repeats main()'s initialisation order (mdff20.c:59-92) and then either
- * one static force call (mdff20.c:94-107) or the run_md loop (md.c:193-248)
- * without its file output, writing every observable at %.17g.
- *
- * init_config (config.c:124-198) calls the LAPACK inverse for polarisable
- * types, so this driver fills the per-ion arrays itself; the inverse is
- * only used by the polarisation energy (pim.c:145-161) and is formed here
- * for diagonal polarisability tensors, the only kind the fixtures use.
+ * It repeats main()'s initialisation order (mdff20.c:59-92), the
+ * reference's own init_config included, and then either one static force
+ * call (mdff20.c:94-107), the run_md loop body (md.c:229-248) without its
+ * file output, or run_md itself (md.c:193-282, which writes OSZIFF, CONTFF
+ * and TRAJFF into the case directory), writing every observable at %.17g.
+ * A polarisable input needs the LAPACK inverse: the plain reference build
+ * refuses it (exit status 5); the drop-in build runs it.
  *
  * usage: mdff20_ref_driver <dir with control and POSFF> static
  *        mdff20_ref_driver <dir> md <nsteps> <state-dump-period>
+ *        mdff20_ref_driver <dir> run_md
  *        mdff20_ref_driver <dir> time <n_induced_calls>
  */
 #include <stdio.h>
@@ -43,38 +43,35 @@
 
 static FILE *out;
 
-static void drv_init_config(void)
+/* present when the driver is linked with the same-name drop-ins */
+extern int mdff_dropin_params(void *p) __attribute__((weak));
+
+/* The reference's own init_config (config.c:124-198).  For a polarisable
+ * type it calls invertmatrix3x3 (inverse_matrix.c:6-23), which needs LAPACK:
+ * absent here, so the reference build leaves that unit out and cannot run a
+ * polarisable input.  Linked with the drop-ins, the drop-in
+ * invertmatrix3x3 of libmdff_gpu.so takes its place.  The `time` mode only
+ * times routines that never read the inverse (multipole_ES calls), so it
+ * runs init_config with the polarisable flags cleared and restores them. */
+static int ref_init_config(const char *mode)
 {
-    int cc = 0;
-    typia = malloc(nion * sizeof(*typia));
-    for (int it = 0; it < ntype; it++) {
-        invenionit[it] = 1. / nionit[it];
-        int ccs = cc;
-        cc += nionit[it];
-        for (int ia = ccs; ia < cc; ia++) {
-            typia[ia] = it;
-            massia[ia] = massit[it];
-            invemassia[ia] = 1.0 / massit[it];
-            if (lcoulombic) {
-                qia[ia] = qit[it];
-                for (int i = 0; i < 3; i++) {
-                    dipia[ia][i] = dipit[it][i];
-                    for (int j = 0; j < 3; j++) quadia[ia][i][j] = quadit[it][i][j];
-                }
-            }
-            if (lcoulombic && lpolar[it]) {
-                for (int i = 0; i < 3; i++)
-                    for (int j = 0; j < 3; j++) {
-                        polia[ia][i][j] = polit[it][i][j];
-                        invepolia[ia][i][j] = 0.0;
-                    }
-                for (int i = 0; i < 3; i++)
-                    if (polit[it][i][i] != 0.0) invepolia[ia][i][i] = 1.0 / polit[it][i][i];
-            }
-        }
+    int anypol = 0;
+    for (int it = 0; it < ntype; it++)
+        if (lcoulombic && lpolar[it]) anypol = 1;
+    if (!anypol || mdff_dropin_params) {
+        init_config();
+        return 0;
     }
-    nionit[ntype] = nion;
-    invenionit[ntype] = 1. / nionit[ntype];
+    if (strcmp(mode, "time") != 0) {
+        fprintf(stderr, "polarisable types: init_config needs invertmatrix3x3 (LAPACK), "
+                        "absent from this build\n");
+        return 5;
+    }
+    bool keep[NTYPEMAX];
+    for (int it = 0; it < NTYPEMAX; it++) { keep[it] = lpolar[it]; lpolar[it] = false; }
+    init_config();
+    for (int it = 0; it < NTYPEMAX; it++) lpolar[it] = keep[it];
+    return 0;
 }
 
 static void dump_vec(const char *key, const double *v, int n)
@@ -113,9 +110,6 @@ static void dump_state(void)
 
 static int scf_calls(void) { return cc_multipole_ES_dir_pim; }
 
-/* present when the driver is linked with the same-name drop-ins */
-extern int mdff_dropin_params(void *p) __attribute__((weak));
-
 static double wall(void) { return omp_get_wtime(); }
 
 int main(int argc, char **argv)
@@ -141,7 +135,8 @@ int main(int argc, char **argv)
     read_config();
     init_md(controlfn);
     init_field(controlfn);
-    drv_init_config();
+    int irc = ref_init_config(mode);
+    if (irc) return irc;
     do_split(nion, numprocs, myrank, &atomDec, "atoms");
     do_split(kcoul.nk, numprocs, myrank, &kcoul.kptDec, "k-points");
     if (egrator == 2) {
@@ -194,6 +189,15 @@ int main(int argc, char **argv)
             dump_scalars(istep, ns);
             if ((period > 0 && istep % period == 0) || istep == nsteps) dump_state();
         }
+    } else if (!strcmp(mode, "run_md")) {
+        /* main's MD branch (mdff20.c:88-92): velocities, then run_md */
+        init_velocities();
+        dump_vec("v0x", vx, nion); dump_vec("v0y", vy, nion); dump_vec("v0z", vz, nion);
+        int c0 = scf_calls();
+        run_md();
+        /* n_scf: the dipole Ewald calls of the SCF over the whole run */
+        dump_scalars(npas, lpim ? scf_calls() - c0 : 0);
+        dump_state();
     } else if (!strcmp(mode, "time")) {
         /* one force call split into the pieces of a PIM step, each timed */
         int nind = argc > 3 ? atoi(argv[3]) : 4;
@@ -211,7 +215,7 @@ int main(int argc, char **argv)
                      false, false, true, false, true, true, true, true);
         double t2 = wall();
         for (int ia = 0; ia < nion; ia++)
-            for (int i = 0; i < 3; i++) mu[ia][i] = lpolar[typia[ia]] ? polia[ia][i][i] * ef[ia][i] : 0.0;
+            for (int i = 0; i < 3; i++) mu[ia][i] = lpolar[typia[ia]] ? polit[typia[ia]][i][i] * ef[ia][i] : 0.0;
         for (int k = 0; k < nind; k++)
             multipole_ES(zq, mu, quadia, &u, &pv, tau, ef, efg, false, true, false, false,
                          false, false, true, false, true, true, true, false);

@@ -1,11 +1,13 @@
 """The same-name drop-ins of include/mdff_dropin.h inside the reference's
 own code: oracle/_ref/mdff20_dropin_driver is the reference's driver and
 routines with engforce, multipole_ES, get_dipoles, momentpolaSCF, the pair
-potentials, TT_damping_functions and the Verlet check replaced by
-mdff20_amd/dropin/mdff_dropin.c over libmdff_gpu.so (FP64).  Its %.17g dump
-is compared with the golden vectors the unmodified reference wrote: 1e-10
-relative on energies, forces, dipoles and stress, equal SCF iteration and
-list-rebuild counts; trajectories within the horizon the vectors cover."""
+potentials, TT_damping_functions, the Verlet check, prop_agate, run_md and
+invertmatrix3x3 replaced by mdff20_amd/dropin/mdff_dropin.c over
+libmdff_gpu.so (FP64).  Its %.17g dump is compared with the golden vectors
+(1e-10 relative on energies, forces, dipoles and stress, equal SCF iteration
+and list-rebuild counts; trajectories within the horizon the vectors cover),
+and its run_md with the reference's own run_md (stdout thermo blocks, OSZIFF,
+TRAJFF, CONTFF) or, for a polarisable run, with the oracle."""
 import os
 import shutil
 
@@ -74,3 +76,120 @@ def test_dropin_driver_matches_golden(case, tmp_path):
         assert rel(pos, pr) < TOL
         if "mu" in ref and "mu" in r:
             assert rel(r["mu"], ref["mu"][j], 1e-12) < TOL
+
+
+# ---------------------------------------------------------------- run_md
+def _set_keys(path, **kw):
+    """rewrite control keywords (removing earlier lines of the same key)"""
+    lines = [ln for ln in open(path).read().splitlines() if ln.split()[:1] not in ([k] for k in kw)]
+    lines += [f"{k} {v}" for k, v in kw.items()]
+    open(path, "w").write("\n".join(lines) + "\n")
+
+
+def _numbers(path):
+    """the numeric tokens of a text file, in order"""
+    out = []
+    for tok in open(path).read().split():
+        try:
+            out.append(float(tok))
+        except ValueError:
+            pass
+    return np.array(out)
+
+
+def _thermo_blocks(path):
+    """the numbers of the 'Thermodynamic information' blocks of a stdout"""
+    out, keep = [], False
+    for ln in open(path):
+        if "Thermodynamic information" in ln:
+            keep = True
+            continue
+        if keep and "=" in ln:
+            try:
+                out.append(float(ln.split("=")[1].split()[0]))
+            except (ValueError, IndexError):
+                pass
+        elif keep and ln.strip() == "" and out:
+            keep = False
+    return np.array(out)
+
+
+def _case_dir(tmp_path, case, tag, **keys):
+    src = os.path.join(GOLDEN, case)
+    d = str(tmp_path / f"{case}_{tag}")
+    os.makedirs(d)
+    for f in ("control", "POSFF"):
+        shutil.copy(os.path.join(src, f), d)
+    _set_keys(os.path.join(d, "control"), **keys)
+    return d
+
+
+@pytest.mark.parametrize("case,keys", [
+    ("melt216_nve", dict(npas=200, nprint=50, fprint=10, cprint=100, tprint=25)),
+    ("lj_fcc_nve", dict(npas=40, nprint=10, fprint=5, cprint=20, tprint=10)),
+])
+def test_dropin_run_md_matches_reference_run_md(case, keys, tmp_path):
+    """run_md replaced by the device loop: the reference's own run_md (CPU,
+    unmodified build) and the drop-in one write the same stdout thermo
+    blocks, OSZIFF, TRAJFF and CONTFF, and end in the same state (1e-10)"""
+    _need_driver()
+    if not refrun.available():
+        pytest.skip("oracle/_ref/mdff20_ref_driver not built")
+    print(f"[gpu] drop-in run_md {case}", flush=True)
+    dr = _case_dir(tmp_path, case, "ref", **keys)
+    dd = _case_dir(tmp_path, case, "dropin", **keys)
+    ref = refrun.run(dr, "run_md", threads=8, timeout=600)
+    out = refrun.run(dd, "run_md", exe=refrun.DRIVER_DROPIN, timeout=300)
+    assert np.array_equal(out["v0x"], ref["v0x"])
+    a, b = out["steps"][-1], ref["steps"][-1]
+    assert a["verlet_updates"] == b["verlet_updates"]
+    for j, key in enumerate(SCALAR_KEYS):
+        assert abs(a["scalars"][j] - b["scalars"][j]) <= TOL * max(abs(b["scalars"][j]), 1e-9), key
+    for key in ("rx", "ry", "rz", "vx", "vy", "vz"):
+        assert rel(a[key], b[key]) < TOL, key
+    assert rel(np.stack([a["fx"], a["fy"], a["fz"]]), np.stack([b["fx"], b["fy"], b["fz"]]), 1e-4) < TOL
+    # the files run_md writes, number by number (printed precision)
+    for f in ("OSZIFF", "TRAJFF", "CONTFF"):
+        x, y = _numbers(os.path.join(dd, f)), _numbers(os.path.join(dr, f))
+        assert x.shape == y.shape and x.size > 0, f
+        assert np.all(np.abs(x - y) <= 1e-6 * np.maximum(np.abs(y), 1e-3)), f
+    x, y = _thermo_blocks(os.path.join(dd, "ref_stdout.txt")), _thermo_blocks(os.path.join(dr, "ref_stdout.txt"))
+    assert x.shape == y.shape and x.size >= 5 * 13
+    assert np.all(np.abs(x - y) <= 1e-6 * np.maximum(np.abs(y), 1e-3))
+
+
+def test_dropin_run_md_nvt_pim_matches_oracle(tmp_path):
+    """a polarisable NVT run through the drop-in run_md (the reference's own
+    driver, init_config with the drop-in invertmatrix3x3) against the oracle
+    over the same 10 steps: final state at 1e-10, the same number of SCF
+    dipole Ewald calls and list rebuilds over the run"""
+    _need_driver()
+    from oracle.oracle import Oracle
+    import mdff20_amd as m
+    case = "pim192_nvt"
+    print(f"[gpu] drop-in run_md {case}", flush=True)
+    d = _case_dir(tmp_path, case, "dropin", npas=10, nprint=5, fprint=1, cprint=10, tprint=5)
+    out = refrun.run(d, "run_md", exe=refrun.DRIVER_DROPIN, timeout=300)
+    cfg = m.read_posff(os.path.join(d, "POSFF"), 2)
+    p = m.read_control(os.path.join(d, "control"), cfg.ntype)
+    o = Oracle(p, cfg.typia, cfg.A, cfg.r, threads=8)
+    o.init_velocities()
+    assert np.array_equal(out["v0x"], o.velocities()[0])
+    t = o.md_init()
+    calls = t["n_scf"] + 1
+    for s in range(1, 11):
+        t = o.md_step(s)
+        calls += t["n_scf"] + 1
+    a = out["steps"][-1]
+    assert a["n_scf"] == calls, (a["n_scf"], calls)
+    assert a["verlet_updates"] == t["verlet_updates"]
+    mine = a["scalars"]
+    ref = np.array([t[k] for k in SCALAR_KEYS])
+    assert np.all(np.abs(mine - ref) <= TOL * np.maximum(np.abs(ref), 1e-9)), (mine, ref)
+    r, v, f = o.state()
+    assert rel(np.stack([a["rx"], a["ry"], a["rz"]]), r) < TOL
+    assert rel(np.stack([a["vx"], a["vy"], a["vz"]]), v) < TOL
+    assert rel(np.stack([a["fx"], a["fy"], a["fz"]]), f, 1e-4) < TOL
+    assert rel(a["mu"], o.dipoles().ravel(), 1e-12) < TOL
+    assert len(_numbers(os.path.join(d, "OSZIFF"))) > 0
+    o.close()
