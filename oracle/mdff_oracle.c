@@ -899,20 +899,36 @@ static int aspc_order(int istep, int order)
  * set for this order; those slots multiply history entries that are still
  * zero during the ramp, so with finite slot contents every ion gets the
  * coefficients of the order computed at entry, which is what is used here. */
+static void aspc_predict_ion(const double *B, int eo, int shift, double (*hist)[3], double *mu)
+{
+    for (int i = 0; i < 3; i++) {
+        mu[i] = 0.0;
+        for (int k = 0; k <= eo; k++) mu[i] += B[k] * hist[k][i];
+    }
+    for (int k = shift; k > 0; k--)
+        for (int i = 0; i < 3; i++) hist[k][i] = hist[k - 1][i];
+}
+
 static void aspc_predict(orc_sys *s, double (*mu)[3], int istep)
 {
     double B[MDFF_MAX_EXTRAPOLATE_ORDER + 1], W;
     int eo = aspc_order(istep, s->p.extrapolate_order);
     aspc_coeffs(eo, B, &W);
     int shift = (eo != s->p.extrapolate_order) ? eo + 1 : eo;
-    for (int ia = 0; ia < s->nion; ia++) {
-        for (int i = 0; i < 3; i++) {
-            mu[ia][i] = 0.0;
-            for (int k = 0; k <= eo; k++) mu[ia][i] += B[k] * s->hist[ia][k][i];
-        }
-        for (int k = shift; k > 0; k--)
-            for (int i = 0; i < 3; i++) s->hist[ia][k][i] = s->hist[ia][k - 1][i];
-    }
+    for (int ia = 0; ia < s->nion; ia++) aspc_predict_ion(B, eo, shift, s->hist[ia], mu[ia]);
+}
+
+/* test hook: the predictor of one ion at step istep for extrapolate_order
+ * `order`: hist[6][3] in (shifted on return), mu[3] out, and the corrector
+ * weight W; returns the order used, -1 for an order without coefficients */
+int orc_aspc_predict1(int istep, int order, double *hist, double *mu, double *W)
+{
+    double B[MDFF_MAX_EXTRAPOLATE_ORDER + 1];
+    if (order < 0 || order > MDFF_MAX_EXTRAPOLATE_ORDER) return -1;
+    int eo = aspc_order(istep, order);
+    if (aspc_coeffs(eo, B, W)) return -1;
+    aspc_predict_ion(B, eo, (eo != order) ? eo + 1 : eo, (double(*)[3])hist, mu);
+    return eo;
 }
 
 /* corrector pim.c:403-420 */

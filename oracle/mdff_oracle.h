@@ -57,6 +57,11 @@ void orc_multipole_es(orc_sys *s, const double *q, const double *mu, int flags,
                       double *u, double *pvir, double tau[9], double *ef, double *efg,
                       double *fx, double *fy, double *fz);
 
+/* test hook: the ASPC predictor of one ion (pim.c:314-400) at step istep
+ * for extrapolate_order `order`; hist[6][3] in and shifted on return, mu[3]
+ * out, *W the corrector weight; returns the order used or -1 */
+int orc_aspc_predict1(int istep, int order, double *hist, double *mu, double *W);
+
 /* run_md (md.c:193-282) without file output: md_init does the t=0 force
  * call; md_step advances one step (prop_agate + list check + rescale) */
 int orc_md_init(orc_sys *s);

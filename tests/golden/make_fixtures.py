@@ -343,6 +343,47 @@ def main():
                                        conv_tol_ind=1e-8, min_scf_pim_iter=3, max_scf_pim_iter=400,
                                        algo_extrapolate_dipole="aspc", extrapolate_order=2, nprint=1000)
             run_case("pim192_nvt", ctl, mode="md", args=(10, 5), posff_writer=wr)
+    # ---- variants of the relaxed 216-ion melt the reference runs unmodified
+    # (no polarisable ions): static dipoles, damped BHMFTD dispersion, the
+    # Nose-Hoover chain and leap-frog
+    m216 = os.path.join(HERE, "melt216_nve")
+    if any(want(c) for c in ("melt216_dip_nve", "melt216_bhmftd_nve", "melt216_nvt", "melt216_lf")):
+        base = open(os.path.join(m216, "control")).read()
+        tmp = tempfile.mkdtemp()
+        shutil.copy(os.path.join(m216, "POSFF"), os.path.join(tmp, "POSFF"))
+        pos216 = os.path.join(tmp, "POSFF")
+
+        def vary(text, **kw):
+            keep = [ln for ln in text.splitlines() if ln.split()[:1] not in ([k] for k in kw)]
+            return "\n".join(keep) + "\n" + control_text(**kw)
+        if want("melt216_dip_nve"):
+            # a static dipole on every anion: charge-dipole and dipole-dipole
+            # Ewald terms, forces and stress (ewald.c:737-830, 999-1006)
+            run_case("melt216_dip_nve", vary(base, npas=50) + "dipit 0.05 -0.03 0.08\ndipit 0.0 0.0 0.0\n",
+                     pos216, mode="md", args=(50, 25))
+        if want("melt216_bhmftd_nve"):
+            # BHMFTD with Tang-Toennies damped C6/C8 terms (bhmftd.c:117-271)
+            run_case("melt216_bhmftd_nve",
+                     vary(base.replace("lbhmft true", "lbhmftd true"), Dbhmftd="40.0 55.0 70.0",
+                          BDbhmftd="3.1 2.7 3.4", npas=50), pos216, mode="md", args=(50, 25))
+        if want("melt216_nvt"):
+            # Nose-Hoover chain of length 3, Yoshida order 3, 2 multiple steps
+            # (integration.c:140-294, with the second half-sweep of :269-277)
+            run_case("melt216_nvt", vary(base, integrator="nvt-nhcn", nhc_n=3, nhc_yosh_order=3, nhc_mults=2,
+                                         timesca_thermo=0.02, temp=2500.0, npas=100), pos216,
+                     mode="md", args=(100, 50))
+        if want("melt216_lf"):
+            # leap-frog (integration.c:93-137) from r(t-dt) = 0 as the
+            # reference allocates it (config.c:61-70), no equilibration
+            run_case("melt216_lf", vary(base, integrator="nve-lf", nequil=0, npas=4), pos216,
+                     mode="md", args=(4, 2))
+    if want("twodipoles"):
+        # src/control_twodipoles.s with two unit dipoles along x
+        run_case("twodipoles", open(os.path.join(src, "control_twodipoles.s")).read(),
+                 posff_writer=lambda p: open(p, "w").write(
+                     "2\nTWODIP\n10.0 0.0 0.0\n0.0 10.0 0.0\n0.0 0.0 10.0\n2\nA B\n1 1\nCartesian\n"
+                     "A 0.0 0.0 0.0 0.0 0.0 0.0 0.0 0.0 0.0\n"
+                     "B 1.3 0.7 -0.4 0.0 0.0 0.0 0.0 0.0 0.0\n"))
     # ---- CCD LJ clusters (examples/ex2): reference energies and structures
     if want("ccd"):
         cdir = os.path.join(REF, "examples/ex2/config")

@@ -1,5 +1,6 @@
-"""HIP engine (FP64 form) against the reference's golden vectors and the
-oracle, through the C ABI.  Tolerance: 1e-10 relative (2-norm) on energies,
+"""HIP engine (FP64 form) against the golden vectors, through the C ABI.
+Cases without polarisable ions were written by the reference's own routines
+(tests/golden, source "reference"); the polarisable ones by the oracle.  Tolerance: 1e-10 relative (2-norm) on energies,
 forces, dipoles and stress; trajectories within the horizon the golden
 vectors cover."""
 import numpy as np
@@ -12,10 +13,10 @@ pytestmark = pytest.mark.gpu
 
 TOL = 1e-10
 
-STATIC = ["twocharges", "onedipole", "twocharges_twopol", "sio2_102_static", "lj_fcc_static",
+STATIC = ["twocharges", "onedipole", "twodipoles", "twocharges_twopol", "sio2_102_static", "lj_fcc_static",
           "pim192_static", "pim192_fixed", "sio2_102_ko_static"]
-MD = ["melt216_nve", "melt216_raw", "lj_fcc_nve", "sio2_102_nvt", "bsio_200_nvt", "pim192_nvt",
-      "sio2_102_ko_nvt", "bsio_200_ko_nvt"]
+MD = ["melt216_nve", "melt216_raw", "melt216_dip_nve", "melt216_bhmftd_nve", "melt216_nvt", "melt216_lf",
+      "lj_fcc_nve", "sio2_102_nvt", "bsio_200_nvt", "pim192_nvt", "sio2_102_ko_nvt", "bsio_200_ko_nvt"]
 
 
 def _engine(p, cfg, precision=m.FP64):

@@ -1,5 +1,9 @@
 """The oracle (CPU restatement, oracle/mdff_oracle.c) against the golden
-vectors written by the reference build (tests/golden/make_fixtures.py)."""
+vectors of tests/golden/make_fixtures.py.  Cases whose ref.npz records source
+"reference" were written by the reference's own routines (no polarisable
+ions: twocharges, onedipole, twodipoles, the melt216 family, lj_fcc) and pin
+the oracle; the polarisable cases were written by the oracle itself and are
+regression vectors (the polarisable path is pinned by test_oracle_pins.py)."""
 import numpy as np
 import pytest
 
@@ -8,16 +12,27 @@ from conftest import SCALAR_KEYS, load_case, ref_state, rel, scalars_of
 Oracle = pytest.importorskip("oracle.oracle").Oracle
 
 TOL = 1e-10
-STATIC = ["twocharges", "onedipole", "twocharges_twopol", "sio2_102_static", "lj_fcc_static",
+STATIC = ["twocharges", "onedipole", "twodipoles", "twocharges_twopol", "sio2_102_static", "lj_fcc_static",
           "pim192_static", "pim192_fixed", "sio2_102_ko_static"]
-MD = ["melt216_nve", "melt216_raw", "lj_fcc_nve", "sio2_102_nvt", "bsio_200_nvt", "pim192_nvt",
-      "sio2_102_ko_nvt", "bsio_200_ko_nvt"]
+MD = ["melt216_nve", "melt216_raw", "melt216_dip_nve", "melt216_bhmftd_nve", "melt216_nvt", "melt216_lf",
+      "lj_fcc_nve", "sio2_102_nvt", "bsio_200_nvt", "pim192_nvt", "sio2_102_ko_nvt", "bsio_200_ko_nvt"]
 
 
 def n_evals(case, t, o):
     """the golden n_scf counts the induced-dipole real-space Ewald calls of
     an SCF: one per iteration, plus every inner sweep in scfkO"""
     return t["n_scf"] + (o.ko_inner() if "_ko_" in case else 0)
+
+
+REFERENCE_CASES = {"twocharges", "onedipole", "twodipoles", "lj_fcc_static", "melt216_nve", "melt216_raw",
+                   "melt216_dip_nve", "melt216_bhmftd_nve", "melt216_nvt", "melt216_lf", "lj_fcc_nve"}
+
+
+@pytest.mark.parametrize("case", STATIC + MD)
+def test_fixture_source(case):
+    """which fixtures the reference wrote"""
+    _, _, ref = load_case(case)
+    assert str(ref["source"]) == ("reference" if case in REFERENCE_CASES else "oracle")
 
 
 @pytest.mark.parametrize("case", STATIC)

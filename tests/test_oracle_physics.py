@@ -265,21 +265,3 @@ def test_nhc_conserved_quantity():
         o.close()
     assert spread_h[1] < 0.1 * spread_e[1]
     assert spread_h[0] / spread_h[1] > 2.0
-
-
-@pytest.mark.parametrize("order", [1, 3, 5, 7])
-def test_yoshida_weights_sum_to_one(order):
-    """the Yoshida-Suzuki weights of chain_nhcn integrate one full step"""
-    import math
-    if order == 1:
-        w = [1.0]
-    elif order == 3:
-        w0 = 1.0 / (2.0 - 2.0 ** (1.0 / 3.0))
-        w = [w0, 1 - 2 * w0, w0]
-    elif order == 5:
-        w0 = 1.0 / (4.0 - 4.0 ** (1.0 / 3.0))
-        w = [w0] * 4 + [1 - 4 * w0]
-    else:
-        w = [.78451361047756, .235573213359357, -1.17767998417887]
-        w = w + [1 - 2 * sum(w)] + w[::-1]
-    assert math.fsum(w) == pytest.approx(1.0, abs=1e-14)
