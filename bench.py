@@ -169,12 +169,14 @@ def batch_relax(confs, ews, iters=30, gamma=0.01, cap=0.05):
     return [(t, A, rr.copy()) for (t, A, _), rr in zip(confs, parts)]
 
 
-def timed_run(p, typ, A, r, precision, steps, warmup):
+def timed_run(p, typ, A, r, precision, steps, warmup, fp64_cg=False):
     import mdff20_amd as m
     e = m.Engine(p, typ, A, r, precision=precision)
     if precision == m.MIXED:
         e.set_kspace_modes(VARIANT[MIXED_EVAL][1], VARIANT[MIXED_INNER][1])
         e.set_scf_anchor(VARIANT[MIXED_ANCHOR][1])
+    elif fp64_cg:
+        e.set_scf_solver(2, 0)
     e.init_velocities()
     e.md_init()
     if warmup:
@@ -187,6 +189,18 @@ def timed_run(p, typ, A, r, precision, steps, warmup):
     stats = dict(scf_fp64_per_step=(c1["scf_fp64"] - c0["scf_fp64"]) / steps,
                  scf_inner_per_step=(c1["scf_inner"] - c0["scf_inner"]) / steps,
                  rmsd=th["rmsd"], temp=th["temp_r"], h_tot=th["h_tot"])
+    if precision == m.MIXED:
+        # after the timed region: the FP64 residual of the SCF's final
+        # dipoles (the SCF stops on its carried low-precision residual)
+        e.set_scf_check(1)
+        chk = []
+        t2 = time.perf_counter()
+        for k in range(5):
+            chk.append(e.md_step(warmup + steps + 1 + k)["rmsd_fp64"])
+        stats["scf_rmsd_fp64"] = dict(steps=5, mean=float(np.mean(chk)), max=float(np.max(chk)),
+                                      tol=p.conv_tol_ind,
+                                      ms_per_step_with_check=1e3 * (time.perf_counter() - t2) / 5)
+        e.set_scf_check(0)
     return e, (t1 - t0), stats
 
 
@@ -339,7 +353,8 @@ def main():
     ap.add_argument("--seed", type=int, default=2026)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "16")))
-    ap.add_argument("--forms", default="fp64,mixed")
+    ap.add_argument("--forms", default="fp64,fp64_cg,mixed",
+                    help="fp64 (the reference's Jacobi SCF), fp64_cg (FP64 with the anchored CG SCF), mixed")
     ap.add_argument("--pmc-kernel", default=None, help="e.g. sk_forward_scf/bf16x3 (counter runs)")
     ap.add_argument("--pmc-iters", type=int, default=5)
     ap.add_argument("--pmc-summarize", nargs=2, default=None, metavar=("FETCH_DIR", "WRITE_DIR"))
@@ -386,8 +401,8 @@ def main():
     # the others are closed after their timed run (each form is timed alone)
     head = "mixed" if "mixed" in forms else forms[0]
     for f in forms:
-        prec = m.FP64 if f == "fp64" else m.MIXED
-        e, dt, stats = timed_run(p, typ, A, r, prec, args.steps, args.warmup)
+        prec = m.MIXED if f == "mixed" else m.FP64
+        e, dt, stats = timed_run(p, typ, A, r, prec, args.steps, args.warmup, fp64_cg=(f == "fp64_cg"))
         res[f] = dict(seconds=dt, steps_per_s=args.steps / dt, **stats)
         if f == head:
             eng[f] = e
@@ -492,6 +507,13 @@ def main():
         "forms": res,
         "mixed_over_fp64": (res["mixed"]["steps_per_s"] / res["fp64"]["steps_per_s"])
         if ("mixed" in res and "fp64" in res) else None,
+        # the speed-up split into the solver (anchored CG against the
+        # reference's Jacobi schedule, both FP64) and the arithmetic (mixed
+        # against FP64, both anchored CG)
+        "fp64cg_over_fp64": (res["fp64_cg"]["steps_per_s"] / res["fp64"]["steps_per_s"])
+        if ("fp64_cg" in res and "fp64" in res) else None,
+        "mixed_over_fp64cg": (res["mixed"]["steps_per_s"] / res["fp64_cg"]["steps_per_s"])
+        if ("mixed" in res and "fp64_cg" in res) else None,
         "accuracy_mixed_vs_fp64": acc,
         "kernels": kern,
         "roofline": roof,

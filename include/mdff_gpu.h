@@ -169,6 +169,14 @@ int mdff_set_scf_anchor(mdff_ctx *ctx, int anchor_mode);
  * counterpart in the reference (tuning). */
 int mdff_set_kspace_shape(mdff_ctx *ctx, int cw64, int cw32, int fwd_blocks, int back_blocks);
 
+/* Mixed form: every `every`-th force call (0 never, the default), after the
+ * SCF, the field of the final dipoles is formed again entirely in FP64
+ * (tables, pair tensors, matrix-core sums) and the rmsd of get_rmsd_mu
+ * (pim.c:431-449) evaluated on it: mdff_thermo.rmsd_fp64.  The SCF itself
+ * stops on the residual of its carried low-precision field (rmsd).  No
+ * counterpart in the reference (measurement support). */
+int mdff_set_scf_check(mdff_ctx *ctx, int every);
+
 /* Device timing of the last engforce by phase, in ms (HIP events on the
  * context stream): [0] short range, [1] static field, [2] SCF loop,
  * [3] final Ewald force, [4] neighbour list, [5] total. */
@@ -220,8 +228,11 @@ int mdff_set_scf_inner(mdff_ctx *ctx, int inner);
  *      12288-ion melt it needs as many steps as solver 2 (the slow modes are
  *      long-wavelength, set by the k-space part) and costs a field pass more
  * Solvers 1 and 2 need every polarisability symmetric positive definite;
- * otherwise solver 0 runs.  No counterpart in the reference: the FP64 form
- * keeps the reference's Jacobi schedule. */
+ * otherwise solver 0 runs.  In the FP64 form, solver 0 (the default) is the
+ * reference's Jacobi schedule and solver 2 the anchored CG with FP64 anchor
+ * and steps (same fixed point, fewer iterations, counts no longer the
+ * reference's); other values are rejected.  No counterpart in the
+ * reference. */
 int mdff_set_scf_solver(mdff_ctx *ctx, int solver, int inner);
 
 #ifdef __cplusplus

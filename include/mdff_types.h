@@ -121,7 +121,12 @@ typedef struct mdff_thermo {
     double temp_r, u_tot, e_tot, h_tot, pressure;
     /* engine diagnostics */
     int n_scf;             /* SCF iterations in the last force call */
-    double rmsd;           /* final SCF residual (pim.c:431-449 definition) */
+    double rmsd;           /* final SCF residual (pim.c:431-449 definition); in the
+                              mixed form the residual of the carried low-precision
+                              field the SCF stopped on */
+    double rmsd_fp64;      /* mixed form with mdff_set_scf_check: the same residual
+                              with the field of the final dipoles formed in FP64;
+                              -1 when not evaluated in this call */
     int verlet_updates;    /* list rebuilds so far (verlet.c:152-154)  */
 } mdff_thermo;
 

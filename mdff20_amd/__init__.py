@@ -86,6 +86,7 @@ def lib():
         "mdff_get_counters": (C.c_int, [vp, P(C.c_longlong)]),
         "mdff_set_scf_inner": (C.c_int, [vp, C.c_int]),
         "mdff_set_scf_solver": (C.c_int, [vp, C.c_int, C.c_int]),
+        "mdff_set_scf_check": (C.c_int, [vp, C.c_int]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -300,6 +301,10 @@ class Engine:
         real-space dipole tensors in the preconditioner."""
         _check(self._L.mdff_set_scf_solver(self._h, solver, inner), "set_scf_solver", False)
 
+    def set_scf_check(self, every):
+        """mixed form: the FP64 residual of the SCF's dipoles every `every`-th force call (thermo rmsd_fp64)"""
+        _check(self._L.mdff_set_scf_check(self._h, every), "set_scf_check", False)
+
     def set_tuning(self, lanes=0, katoms=0, ksplit=0):
         _check(self._L.mdff_set_tuning(self._h, lanes, katoms, ksplit), "set_tuning", False)
 
@@ -343,12 +348,15 @@ class Engine:
         _check(self._L.mdff_get_dipoles_scf(self._h, istep, _dp(mu), C.byref(up)), "get_dipoles")
         return mu, up.value
 
-    def multipole_es(self, q, mu, flags, forces=None):
+    def multipole_es(self, q, mu, flags, forces=None, theta=None):
         """multipole_ES on host arrays; returns dict(u, pvir, tau, ef, efg,
-        f) with f the coulombic-unit forces added to `forces` (or zeros)."""
+        f) with f the coulombic-unit forces added to `forces` (or zeros).
+        theta (quadrupoles, [n][3][3]) takes no part in any term, as in the
+        reference."""
         q = np.ascontiguousarray(q, dtype=np.float64)
         mu = np.ascontiguousarray(np.asarray(mu, dtype=np.float64).reshape(self.n, 3))
-        theta = np.zeros((self.n, 9))
+        theta = np.zeros((self.n, 9)) if theta is None else \
+            np.ascontiguousarray(np.asarray(theta, dtype=np.float64).reshape(self.n, 9))
         u, pv = C.c_double(), C.c_double()
         tau = np.zeros(9)
         ef = np.zeros((self.n, 3))

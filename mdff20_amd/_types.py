@@ -58,7 +58,7 @@ class Thermo(C.Structure):
         ("pvir_lj", _d), ("pvir_bhmftd", _d), ("pvir_coul", _d),
         ("tau_lj", (_d * 3) * 3), ("tau_bhmftd", (_d * 3) * 3), ("tau_coul", (_d * 3) * 3),
         ("temp_r", _d), ("u_tot", _d), ("e_tot", _d), ("h_tot", _d), ("pressure", _d),
-        ("n_scf", _i), ("rmsd", _d), ("verlet_updates", _i),
+        ("n_scf", _i), ("rmsd", _d), ("rmsd_fp64", _d), ("verlet_updates", _i),
     ]
 
     def as_dict(self):

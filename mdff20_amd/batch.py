@@ -278,7 +278,9 @@ class ParamSpec:
             if k in "ABCD":
                 g.append(grad["d" + k][e[1]][e[2]])
             elif k == "pol":
-                g.append(np.trace(grad["dpol"][e[1]]) if True else 0.0)
+                # params() scales the whole tensor polit_base by x / polit_base[0][0]
+                base = np.array([[self.base.polit[e[1]][a][b] for b in range(3)] for a in range(3)])
+                g.append(float(np.sum(grad["dpol"][e[1]] * base)) / base[0, 0])
             elif k == "damp_b":
                 g.append(grad["ddamp_b"][e[1]][e[2]])
             else:

@@ -440,6 +440,7 @@ static void thermo_of(mdff_batch *b, int c, mdff_thermo *th)
     const BConfD &K = b->conf_h[c];
     mdff_thermo t;
     memset(&t, 0, sizeof(t));
+    t.rmsd_fp64 = -1.0;
     double iv = K.iv;
     auto sym = [&](const double *w, double scale, double out[3][3]) {
         out[0][0] = w[0] * scale; out[0][1] = w[1] * scale; out[0][2] = w[2] * scale;

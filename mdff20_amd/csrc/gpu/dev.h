@@ -57,7 +57,7 @@ enum {
     S_U_REC = 21, S_W_REC = 22,    // 6 slots
     S_QSQ = 28, S_MUSQ = 29,
     S_KIN = 30, S_UPOL = 31, S_RMSD2 = 32,
-    S_ENVT = 33, S_NHC_S = 34,
+    S_ENVT = 33, S_NHC_S = 34, S_RMSD_CHK = 35,
     S_VXI = 40,                    // 16 slots
     S_XI = 56,                     // 16 slots
     S_LAMBDA = 72, S_VMEAN = 73,   // 3 slots

@@ -177,9 +177,12 @@ struct mdff_ctx {
     int last_nscf = 0;
     int hint_fp64 = 0, hint_mixed = 0, hint_cg = 0, hint_ko = 0; // iterations of the last SCF of each form
     double last_rmsd = 0;
+    int scf_check = 0;        // mixed form: FP64 residual of the SCF every scf_check-th force call (0 never)
+    int chk_done = 0;         // the last force call evaluated it (S_RMSD_CHK holds the sum)
     int scf_chunk = 4;
     int inner_iters = 8;      // mixed form: low-precision correction iterations per FP64 refinement
     int scf_solver = 2;       // mixed form: 0/1 Jacobi/CG defect correction, 2/3 anchored CG
+    int fp64_cg = 0;          // FP64 form: anchored CG with FP64 steps instead of the reference's Jacobi schedule
     int cg_ok = 0;            // every polarisability symmetric positive definite (CG applies)
     long long n_force = 0, n_scf_fp64 = 0, n_scf_inner = 0;
 };
@@ -1389,18 +1392,21 @@ static int scf_loop_mixed(mdff_ctx *c, int istep)
     return MDFF_OK;
 }
 
-// Mixed form, anchored conjugate gradients (the default): one FP64
-// evaluation of the field of the ASPC predictor (pim.c:382-400), then
-// conjugate-gradient steps on (pol^-T - T) mu = E_stat preconditioned by
-// pol^T, each applying the low-precision T to the search direction only and
-// carrying the field along (E <- E + gamma T p).  The rmsd of get_rmsd_mu
-// (pim.c:431-449) is formed from the updated dipoles and field after every
-// step and the loop condition of pim.c:236 applied to it on the device, one
-// SCF iteration per step.  The low precision enters only through T d, where
-// d is the change from the predictor, so the field error it leaves is of
-// the order of that precision times |T d|, far below the tolerance for a
-// predictor that is close.  The host launches as many steps as the last
-// SCF took before it polls.
+// Mixed form, anchored conjugate gradients (the default): one evaluation of
+// the field of the ASPC predictor (pim.c:382-400), the anchor, in FP64 or
+// (mf_anchor, the default) in a matrix-core mode over the FP32 tables and
+// pair tensors, then conjugate-gradient steps on (pol^-T - T) mu = E_stat
+// preconditioned by pol^T, each applying the low-precision T to the search
+// direction only and carrying the field along (E <- E + gamma T p).  The
+// rmsd of get_rmsd_mu (pim.c:431-449) is formed from the updated dipoles and
+// the carried field after every step and the loop condition of pim.c:236
+// applied to it on the device, one SCF iteration per step.  That rmsd is the
+// residual of the low-precision system: the static field (FP32 real space,
+// split-FP16 k space), the anchor and the steps each carry their own
+// rounding, of order 1e-7 of the field, so the true FP64 residual of the
+// final dipoles is larger than the tolerance asks; mdff_set_scf_check has it
+// evaluated in FP64 (thermo rmsd_fp64).  The host launches as many steps as
+// the last SCF took before it polls.
 // k_sk_back_fin forms the CG step's p.q partial sums in its blocks of 256
 // polarisable ions, the blocks of the CG kernels
 static_assert(CG_BLOCK == 256, "CG partial-sum blocks");
@@ -1418,14 +1424,17 @@ static int scf_loop_anchored(mdff_ctx *c, int istep)
     // which the predictor writes (all ions; zero where not polarisable)
     const bool open = maxit > 0 || minit > 0;
     if (!open) hipLaunchKernelGGL(k_scf_begin, dim3(1), dim3(1), 0, c->st, ctl, 0);
-    const bool wmu = c->mf_anchor == KM_F16X3;
+    // the FP64 form runs the same solver with FP64 anchor and steps
+    const int lo = lo_of(c);
+    const int anchor = lo ? c->mf_anchor : 0;
+    const bool wmu = anchor == KM_F16X3;
     hipLaunchKernelGGL(k_scf_mu, dim3(cdiv(n, 256)), dim3(256), 0, c->st, n, 1, asp, c->type, c->tab_d, c->hist, c->ef,
                        c->mu_ind, c->upol_i, act, wmu ? c->gwp : nullptr, open ? ctl : nullptr);
     // the anchor: the field of the predicted dipoles in FP64, or with
     // mf_anchor over the FP32 tables in that matrix-core mode
-    if (c->mf_anchor) {
+    if (anchor) {
         const int keep = c->mf_lo;
-        c->mf_lo = c->mf_anchor;
+        c->mf_lo = anchor;
         c->wpart_ready = wmu;
         c->wpart_nb = wmu ? cdiv(n, 256) : 0;
         induced_field(c, 1, c->mu_ind, c->ef_stat, c->ef, act);
@@ -1437,7 +1446,8 @@ static int scf_loop_anchored(mdff_ctx *c, int istep)
     const int nb = cdiv(np, CG_BLOCK);
     // split-FP16 steps: the kernels that write p also give the channel
     // maxima of p that scale the forward sum's weights
-    double *wp = (c->mf_inner == KM_F16X3 && !(c->cg_precond && c->t2_ok && c->t2f != nullptr)) ? c->gwp : nullptr;
+    double *wp = (lo && c->mf_inner == KM_F16X3 && !(c->cg_precond && c->t2_ok && c->t2f != nullptr)) ? c->gwp
+                                                                                                          : nullptr;
     double *rz[2] = {c->cg_part, c->cg_part + nb}, *rm = c->cg_part + 3 * nb;
     // p.q partials: as many as the finish reports (per block of its launch),
     // else k_cg_pq's per CG block
@@ -1466,9 +1476,9 @@ static int scf_loop_anchored(mdff_ctx *c, int istep)
     const int limit = std::max(maxit, minit) + 1;
     auto step = [&]() {
         int keep = c->mf_lo;
-        c->mf_lo = c->mf_inner;
+        if (lo) c->mf_lo = c->mf_inner;
         c->wpart_ready = wp != nullptr;
-        const int npq = induced_field(c, 1, c->cg_p, nullptr, c->dc_g, act, pq);
+        const int npq = induced_field(c, lo, c->cg_p, nullptr, c->dc_g, act, pq);
         c->mf_lo = keep;
         const int nbq = npq > 0 ? npq : nb;
         if (npq == 0)
@@ -1501,9 +1511,10 @@ static int scf_loop_anchored(mdff_ctx *c, int istep)
     hipLaunchKernelGGL(k_upol_hist, dim3(cdiv(n, 256)), dim3(256), 0, c->st, n, c->type, c->tab_d, c->mu_ind,
                        c->upol_i, c->hist, c->chk_part, c->chk_cnt, c->scal);
     scf_finish(c, false);
-    if (c->mf_anchor) c->n_scf_inner += 1;
+    if (anchor) c->n_scf_inner += 1;
     else c->n_scf_fp64 += 1;
-    c->n_scf_inner += std::max(0, c->last_nscf - 1);
+    if (lo) c->n_scf_inner += std::max(0, c->last_nscf - 1);
+    else c->n_scf_fp64 += std::max(0, c->last_nscf - 1);
     c->hint_cg = std::max(1, c->last_nscf - 1);
     if (c->last_rmsd > c->p.conv_tol_ind) return MDFF_ERR_SCF;
     return MDFF_OK;
@@ -1626,6 +1637,26 @@ static void coulomb_force(mdff_ctx *c, int lo)
                        c->scal + S_U_REC, c->selfbuf, n, n, c->scal + S_QSQ);
 }
 
+// FP64 residual of the mixed SCF's final dipoles: the static field and the
+// induced field formed entirely in FP64 (FP64 tables, pair tensors and
+// matrix-core sums), then sum over the polarisable ions of
+// (mu/pol - E)^2 as get_rmsd_mu (pim.c:431-449) into S_RMSD_CHK.  Runs
+// after the SCF, before the force call; the mixed ef_stat is no longer
+// needed then.
+static int scf_fp64_residual(mdff_ctx *c)
+{
+    int n = c->n;
+    launch_eikr(c, 0, 0, nullptr, true);
+    int rc = build_t2(c, true);
+    if (rc) return rc;
+    static_field(c, 0);
+    induced_field(c, 0, c->mu_ind, c->ef_stat, c->dc_g, nullptr);
+    hipLaunchKernelGGL(k_resid2, dim3(cdiv(n, 256)), dim3(256), 0, c->st, n, c->type, c->tab_d, c->mu_ind, c->dc_g,
+                       c->ke);
+    colsum(c, c->ke, n, n, 1, c->scal + S_RMSD_CHK);
+    return MDFF_OK;
+}
+
 static int engforce_impl(mdff_ctx *c, int istep)
 {
     int n = c->n;
@@ -1655,8 +1686,17 @@ static int engforce_impl(mdff_ctx *c, int istep)
             if ((rc = build_t2(c)) != MDFF_OK) return rc;
             static_field(c, lo);
             if (c->have_events) HIPCHK(hipEventRecord(c->ev[2], c->st));
-            rc = c->p.algo_pim == MDFF_PIM_SCFKO ? scf_loop_ko(c, istep) : (lo ? scf_mixed(c, istep) : scf_loop(c, istep));
+            rc = c->p.algo_pim == MDFF_PIM_SCFKO ? scf_loop_ko(c, istep)
+                 : lo                                 ? scf_mixed(c, istep)
+                 : (c->fp64_cg && c->cg_ok)           ? scf_loop_anchored(c, istep)
+                                                      : scf_loop(c, istep);
             if (rc != MDFF_OK && rc != MDFF_ERR_SCF) return rc;
+            c->chk_done = 0;
+            if (lo && c->scf_check > 0 && c->n_force % c->scf_check == 0) {
+                int r2 = scf_fp64_residual(c);
+                if (r2) return r2;
+                c->chk_done = 1;
+            }
             c->ldip_sticky = 1;
             // mu_tot with the force call's self terms and, for a split-FP16
             // force sum, its weight maxima
@@ -1728,6 +1768,7 @@ static void thermo_from_scal(mdff_ctx *c, mdff_thermo *th)
     t.pressure = t.pvir_lj + t.pvir_bhmftd + t.pvir_coul + t.temp_r * c->u.boltz_unit * iv;
     t.n_scf = c->last_nscf;
     t.rmsd = c->last_rmsd;
+    t.rmsd_fp64 = c->chk_done ? sqrt(S[S_RMSD_CHK] / (double)(c->npol > 0 ? c->npol : 1)) : -1.0;
     t.verlet_updates = c->ctl_h[C_NUPD];
     *th = t;
 }
@@ -2728,10 +2769,24 @@ int mdff_set_scf_inner(mdff_ctx *c, int inner)
 int mdff_set_scf_solver(mdff_ctx *c, int solver, int inner)
 {
     if (!c || solver < 0 || solver > 3 || inner < 0 || inner > 64) return MDFF_ERR_ARG;
+    if (!lo_of(c)) {
+        // FP64 form: 0 the reference's Jacobi schedule, 2 anchored CG in FP64
+        if (solver != 0 && solver != 2) return MDFF_ERR_ARG;
+        c->fp64_cg = solver == 2;
+        c->hint_cg = 0;
+        return MDFF_OK;
+    }
     c->scf_solver = solver;
     c->cg_precond = solver == 3;
     c->inner_iters = inner;
     c->hint_mixed = c->hint_cg = 0;
+    return MDFF_OK;
+}
+
+int mdff_set_scf_check(mdff_ctx *c, int every)
+{
+    if (!c || every < 0) return MDFF_ERR_ARG;
+    c->scf_check = every;
     return MDFF_OK;
 }
 

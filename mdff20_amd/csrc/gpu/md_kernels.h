@@ -551,6 +551,28 @@ __device__ void sum_to_scalar(double v, double *part, unsigned *cnt, double *out
     }
 }
 
+// per ion, (mu/pol - E)^2 summed over the diagonal entries with pol != 0
+// (the terms of get_rmsd_mu pim.c:431-449), zero for other ions
+__global__ void __launch_bounds__(256) k_resid2(int n, const int *__restrict__ type, const TypeTab *__restrict__ T,
+                                                const double *__restrict__ mu, const double *__restrict__ ef,
+                                                double *__restrict__ out)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const int t = type[i];
+    double r = 0.0;
+    if (T->polar[t]) {
+        for (int c = 0; c < 3; c++) {
+            const double paa = T->pol[t][4 * c];
+            if (paa != 0.0) {
+                const double e = mu[(size_t)c * n + i] / paa - ef[(size_t)c * n + i];
+                r += e * e;
+            }
+        }
+    }
+    out[i] = r;
+}
+
 // the rmsd of get_rmsd_mu (pim.c:431-449) over mu and ef, u_pol (when upol_i
 // is given) and the loop condition of momentpolaSCF pim.c:236: per-block
 // partial sums over 256 ions each; the block that finishes last (atomic

@@ -1560,6 +1560,7 @@ void orc_get_thermo(const orc_sys *s, mdff_thermo *th)
     orc_sys *m = (orc_sys *)s;
     finish_thermo(m);
     *th = m->th;
+    th->rmsd_fp64 = -1.0;   /* the oracle is FP64 throughout */
 }
 
 int orc_ko_inner(const orc_sys *s) { return s->ko_inner; }

@@ -288,3 +288,54 @@ def test_mfma_kspace_matches_vector_kernels(case):
         assert rel(np.array(t1["tau_coul"]), np.array(t0["tau_coul"])) < 1e-12
         if t0["u_pol"]:
             assert rel(e1.dipoles(), e0.dipoles()) < 1e-12
+
+
+def test_quadrupoles_take_no_part():
+    """static quadrupoles (quadit, theta) enter no term of the reference
+    (ewald.c:646-688 builds T5 and nothing reads it; ewald.c:379-491 never
+    uses theta): results with and without them are identical"""
+    p, cfg, _ = load_case("pim192_static")
+    q = m.Params()
+    import ctypes as C
+    C.pointer(q)[0] = p
+    for a in range(3):
+        for b in range(3):
+            q.quadit[0][a][b] = 0.3 if a == b else 0.1
+    q.quadit[0][2][2] = -0.6
+    with m.Engine(p, cfg.typia, cfg.A, cfg.r) as e0, m.Engine(q, cfg.typia, cfg.A, cfg.r) as e1:
+        t0, t1 = e0.engforce(0), e1.engforce(0)
+        assert np.array_equal(e0.forces(), e1.forces())
+        assert t0["u_tot"] == t1["u_tot"]
+        qch = np.array([p.qit[t] for t in cfg.typia])
+        mu = e0.dipoles()
+        theta = np.tile(np.array([[0.3, 0.1, 0.1], [0.1, 0.3, 0.1], [0.1, 0.1, -0.6]]), (cfg.nion, 1, 1))
+        flags = ES["LQ"] | ES["LD"] | ES["EF"] | ES["EFG"] | ES["FORCES"] | ES["STRESS"] | ES["DIR"] | ES["REC"]
+        a = e0.multipole_es(qch, mu, flags)
+        b = e0.multipole_es(qch, mu, flags | ES["LT"], theta=theta)
+        for k in ("ef", "efg", "f", "tau"):
+            assert np.array_equal(a[k], b[k]), k
+        assert a["u"] == b["u"]
+
+
+def test_fp64_anchored_cg_reaches_the_oracle_fixed_point():
+    """the FP64 form with the anchored CG solver (mdff_set_scf_solver 2):
+    the same fixed point as the reference's Jacobi SCF in the oracle, in
+    fewer iterations"""
+    import ctypes as C
+    p, cfg, _ = load_case("pim192_static")
+    q = m.Params()
+    C.pointer(q)[0] = p
+    q.conv_tol_ind = 1e-11
+    q.max_scf_pim_iter = 1000
+    o = Oracle(q, cfg.typia, cfg.A, cfg.r)
+    to = o.engforce(0)
+    with m.Engine(q, cfg.typia, cfg.A, cfg.r) as e:
+        e.set_scf_solver(2, 0)
+        t = e.engforce(0)
+        print(f"[gpu] FP64 anchored CG: {t['n_scf']} iterations, Jacobi {to['n_scf']}; "
+              f"dipoles {rel(e.dipoles(), o.dipoles()):.2e}, forces {rel(e.forces(), o.forces()):.2e}", flush=True)
+        assert t["n_scf"] < to["n_scf"]
+        assert t["rmsd"] <= 1e-11
+        assert rel(e.dipoles(), o.dipoles()) < 1e-9
+        assert rel(e.forces(), o.forces()) < 1e-9
+        assert t["u_tot"] == pytest.approx(to["u_tot"], rel=1e-12)

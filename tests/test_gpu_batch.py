@@ -228,3 +228,34 @@ def test_adam_through_autograd_function():
         th, hist = fit(b, spec, fref, steps=50, lr=0.01)
         print(f"  loss {hist[0]:.6e} -> {hist[-1]:.6e}", flush=True)
         assert len(hist) == 50 and hist[-1] < 0.5 * hist[0]
+
+
+def test_adjoint_pol_gradient_anisotropic():
+    """a diagonal, anisotropic polarisability: the 'pol' entry scales the
+    whole tensor, and its gradient from the adjoint (sum of dpol times the
+    tensor over its [0][0] entry) equals the finite difference"""
+    p, confs = _configs()
+    confs = confs[:2]
+    p = _tight(p)
+    for a, v in enumerate((1.2, 1.0, 0.8)):
+        p.polit[0][a][a] = v
+    print("[gpu] anisotropic polarisability gradient", flush=True)
+    with m.batch.Batch(p, [c[0] for c in confs], [c[1] for c in confs]) as b:
+        spec = m.batch.ParamSpec(p, [("pol", 0)], log=False)
+        b.set_params(spec.params(spec.values() * 1.05))
+        b.engforce()
+        fref = b.forces()
+        b.set_params(p)
+        g = b.adjoint(fref)
+        grad = spec.gather(g)[0]
+        x0 = spec.values()[0]
+        h = 1e-6 * x0
+        L = []
+        for sgn in (1, -1):
+            b.set_params(spec.params(np.array([x0 + sgn * h])))
+            b.engforce()
+            d = b.forces() - fref
+            L.append(float(np.sum(d * d)))
+        fd = (L[0] - L[1]) / (2 * h)
+        print(f"  adjoint {grad:.10e} fd {fd:.10e}", flush=True)
+        assert abs(grad - fd) <= 1e-6 * abs(fd)

@@ -233,6 +233,8 @@ def test_cfg3_mixed_1000_nvt_steps_vs_fp64(cfg3):
     p, typ, A, r, _ = cfg3
     worst = dict(force_rms=0.0, dipoles=0.0, energy=0.0, stress=0.0)
     with m.Engine(p, typ, A, r, precision=m.MIXED) as em, m.Engine(p, typ, A, r, precision=m.FP64) as ef:
+        # the FP64 residual of the mixed SCF's dipoles at every sampled step
+        em.set_scf_check(100)
         em.init_velocities()
         t = em.md_init()
         h0 = t["h_tot"]
@@ -247,8 +249,10 @@ def test_cfg3_mixed_1000_nvt_steps_vs_fp64(cfg3):
             err = _err_vs(em, ef, t, tf)
             for k in worst:
                 worst[k] = max(worst[k], err[k])
-            say(f"cfg3 mixed step {s0}: {err}, H - H0 = {t['h_tot'] - h0:.4e} eV, T {t['temp_r']:.0f} K")
+            say(f"cfg3 mixed step {s0}: {err}, SCF rmsd carried {t['rmsd']:.2e} FP64 {t['rmsd_fp64']:.2e}, "
+                f"H - H0 = {t['h_tot'] - h0:.4e} eV, T {t['temp_r']:.0f} K")
             assert err["force_rms"] <= 1e-5, (s0, err)
+            assert 0.0 <= t["rmsd_fp64"] < 1e-5, (s0, t["rmsd_fp64"])
         say(f"cfg3 mixed 1000 steps in {time.time() - t0:.1f} s (with 11 FP64 evaluations); worst {worst}")
         for k, tol in MIXED_TOL.items():
             assert worst[k] <= tol, (k, worst[k])

@@ -265,3 +265,21 @@ def test_nhc_conserved_quantity():
         o.close()
     assert spread_h[1] < 0.1 * spread_e[1]
     assert spread_h[0] / spread_h[1] > 2.0
+
+
+def test_quadrupoles_take_no_part():
+    """quadit sets lqua in the reference (field.c:201) but no term reads the
+    quadrupoles (ewald.c:646-688): the oracle's results are unchanged"""
+    import ctypes as C
+    p, cfg, _ = load_case("pim192_static")
+    q = m.Params()
+    C.pointer(q)[0] = p
+    q.quadit[0][0][0] = q.quadit[0][1][1] = 0.3
+    q.quadit[0][2][2] = -0.6
+    # one thread: the sums are then in a fixed order
+    o0 = Oracle(p, cfg.typia, cfg.A, cfg.r, threads=1)
+    t0 = o0.engforce(0)
+    f0 = o0.forces()
+    o1 = Oracle(q, cfg.typia, cfg.A, cfg.r, threads=1)
+    t1 = o1.engforce(0)
+    assert t0["u_tot"] == t1["u_tot"] and np.array_equal(f0, o1.forces())
