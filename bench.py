@@ -169,9 +169,11 @@ def batch_relax(confs, ews, iters=30, gamma=0.01, cap=0.05):
     return [(t, A, rr.copy()) for (t, A, _), rr in zip(confs, parts)]
 
 
-def timed_run(p, typ, A, r, precision, steps, warmup, fp64_cg=False):
+def timed_run(p, typ, A, r, precision, steps, warmup, fp64_cg=False, list_method=0):
     import mdff20_amd as m
     e = m.Engine(p, typ, A, r, precision=precision)
+    if list_method:
+        e.set_list_method(list_method)
     if precision == m.MIXED:
         e.set_kspace_modes(VARIANT[MIXED_EVAL][1], VARIANT[MIXED_INNER][1])
         e.set_scf_anchor(VARIANT[MIXED_ANCHOR][1])
@@ -359,6 +361,7 @@ def main():
     ap.add_argument("--pmc-iters", type=int, default=5)
     ap.add_argument("--pmc-summarize", nargs=2, default=None, metavar=("FETCH_DIR", "WRITE_DIR"))
     ap.add_argument("--trace-summarize", default=None, metavar="DIR")
+    ap.add_argument("--list-method", type=int, default=0, help="0 cell-grid neighbour build (default), 1 all pairs")
     ap.add_argument("--no-kernels", action="store_true",
                     help="skip the per-kernel timing of every arithmetic variant and the accuracy call "
                          "(a kernel trace then holds the timed MD steps of the chosen forms only)")
@@ -402,7 +405,8 @@ def main():
     head = "mixed" if "mixed" in forms else forms[0]
     for f in forms:
         prec = m.MIXED if f == "mixed" else m.FP64
-        e, dt, stats = timed_run(p, typ, A, r, prec, args.steps, args.warmup, fp64_cg=(f == "fp64_cg"))
+        e, dt, stats = timed_run(p, typ, A, r, prec, args.steps, args.warmup, fp64_cg=(f == "fp64_cg"),
+                                 list_method=args.list_method)
         res[f] = dict(seconds=dt, steps_per_s=args.steps / dt, **stats)
         if f == head:
             eng[f] = e
@@ -464,6 +468,17 @@ def main():
             kern[f"{name}/{vname}"] = dict(ms=ms, flops=flops, per_step=per, peak=peak,
                                            tflops=(flops / (ms * 1e-3) / 1e12) if flops else None)
     log(f"kernels ({head}): {json.dumps(kern)}")
+    # neighbour-list rebuild at this N: the cell-grid build against the
+    # all-pairs scan (the list is rebuilt at the current positions)
+    rebuild = None
+    if not args.no_kernels:
+        info = e.list_info()
+        t_cells = e.time_kernel(7, iters=5)
+        e.set_list_method(1)
+        t_all = e.time_kernel(7, iters=5)
+        e.set_list_method(0)
+        rebuild = dict(cells_ms=t_cells, all_pairs_ms=t_all, grid=info["nc"], reach=info["reach"])
+        log(f"neighbour rebuild: {rebuild}")
     roof = None
     mm = [(v["ms"] * v["per_step"], k) for k, v in kern.items() if v["flops"]]
     if mm:
@@ -516,6 +531,7 @@ def main():
         if ("mixed" in res and "fp64_cg" in res) else None,
         "accuracy_mixed_vs_fp64": acc,
         "kernels": kern,
+        "neighbour_rebuild": rebuild,
         "roofline": roof,
         "cpu_baseline": cpu,
     }

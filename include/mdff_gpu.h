@@ -169,6 +169,19 @@ int mdff_set_scf_anchor(mdff_ctx *ctx, int anchor_mode);
  * counterpart in the reference (tuning). */
 int mdff_set_kspace_shape(mdff_ctx *ctx, int cw64, int cw32, int fwd_blocks, int back_blocks);
 
+/* Neighbour-list build: method 0 (default) bins the ions into a cell grid
+ * and scans the cells around each ion when the box holds at least 3 cells
+ * of the list cutoff per axis and 4 times as many cells as the stencil
+ * around an ion, else scans all pairs; method 1 always scans all pairs (the
+ * reference's gen_pbc_verletlist verlet.c:54-122 does); method 2 uses the
+ * grid wherever one fits.  Rows come out in ascending neighbour index.
+ * Both give the same pair set; a change applies from the next rebuild (the
+ * list in place stays valid).  mdff_list_info:
+ * out[0] grid in use, out[1..3] cells per axis, out[4..6] stencil
+ * half-widths, out[7] row capacity.  No counterpart in the reference. */
+int mdff_set_list_method(mdff_ctx *ctx, int method);
+int mdff_list_info(mdff_ctx *ctx, int out[8]);
+
 /* Mixed form: every `every`-th force call (0 never, the default), after the
  * SCF, the field of the final dipoles is formed again entirely in FP64
  * (tables, pair tensors, matrix-core sums) and the rmsd of get_rmsd_mu
@@ -196,6 +209,8 @@ int mdff_get_timing(mdff_ctx *ctx, double t[6]);
  *   4 back-projection, force form (field + field gradient, all ions)
  *   5 short-range pair kernel
  *   6 real-space Ewald force kernel
+ *   7 a forced rebuild of the neighbour list at the current positions by
+ *     the method in use (cell grid or all pairs, lo ignored)
  * No counterpart in the reference (measurement support). */
 int mdff_time_kernel(mdff_ctx *ctx, int which, int lo, int iters, double *ms);
 

@@ -87,6 +87,8 @@ def lib():
         "mdff_set_scf_inner": (C.c_int, [vp, C.c_int]),
         "mdff_set_scf_solver": (C.c_int, [vp, C.c_int, C.c_int]),
         "mdff_set_scf_check": (C.c_int, [vp, C.c_int]),
+        "mdff_set_list_method": (C.c_int, [vp, C.c_int]),
+        "mdff_list_info": (C.c_int, [vp, ip]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -304,6 +306,15 @@ class Engine:
     def set_scf_check(self, every):
         """mixed form: the FP64 residual of the SCF's dipoles every `every`-th force call (thermo rmsd_fp64)"""
         _check(self._L.mdff_set_scf_check(self._h, every), "set_scf_check", False)
+
+    def set_list_method(self, method):
+        """0: cell-grid neighbour build where it pays (default), 1: all pairs, 2: cell grid wherever it fits"""
+        _check(self._L.mdff_set_list_method(self._h, method), "set_list_method", False)
+
+    def list_info(self):
+        o = (C.c_int * 8)()
+        _check(self._L.mdff_list_info(self._h, o), "list_info", False)
+        return dict(cells=bool(o[0]), nc=list(o[1:4]), reach=list(o[4:7]), cap=o[7])
 
     def set_tuning(self, lanes=0, katoms=0, ksplit=0):
         _check(self._L.mdff_set_tuning(self._h, lanes, katoms, ksplit), "set_tuning", False)

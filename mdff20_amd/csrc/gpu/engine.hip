@@ -157,6 +157,10 @@ struct mdff_ctx {
     int wpart_nb = 0;         // with wpart_ready: partials per channel in gwp (0: one per 256 selected ions)
     int tot_done = 0;         // k_mu_tot formed mu_tot, the self terms and (2: also) gwp for the force call
     int kin_fresh = 0;        // ke and S_KIN hold the current velocities' kinetic energy (within one run call)
+    CellGrid cg{};                 // cell grid of the neighbour build
+    int cells_set = 0;             // 1 grid chosen (cells_on tells whether it is used)
+    int cells_on = 0;              // the list is built from the cell grid (else all pairs)
+    int list_method = 0;           // 0 cells where the box allows, 1 all pairs always
     double *vc_part = nullptr;     // k_vcheck's per-block two largest displacements [2][n/256]
     unsigned *vc_cnt = nullptr;    // k_vcheck's finished-block count
     unsigned *chk_cnt = nullptr;   // k_scf_check's finished-block count
@@ -273,9 +277,95 @@ static void build_tables(mdff_ctx *c)
 }
 
 // ------------------------------------------------------------ neighbour list
+// The cell grid of the neighbour build, chosen once per context: per axis
+// nc cells of at least rmax / 2 across (rmax: the larger list cutoff, skin
+// included), so pairs within rmax are at most reach = 2 cells apart; with
+// fewer than 2 reach + 1 cells on an axis, cells of at least rmax (reach 1);
+// with fewer than 3 of those the box is too small for a grid and every pair
+// is scanned.  Widths are the perpendicular ones, 1/|B_d|, so triclinic
+// cells are covered.
+static int setup_cells(mdff_ctx *c)
+{
+    c->cells_set = 1;
+    c->cells_on = 0;
+    if (c->list_method == 1 || !c->p.lverletL) return MDFF_OK;
+    const double rmax = std::max(c->p.cutshortrange, c->p.cutlongrange) + c->p.skindiff;
+    double w[3];
+    for (int d = 0; d < 3; d++) {
+        const double *b = c->cell.B[d];
+        w[d] = 1.0 / sqrt(b[0] * b[0] + b[1] * b[1] + b[2] * b[2]);
+    }
+    int nc[3], reach[3];
+    bool ok = false;
+    for (int sub = 2; sub >= 1 && !ok; sub--) {
+        ok = true;
+        for (int d = 0; d < 3; d++) {
+            nc[d] = (int)floor(sub * w[d] / rmax);
+            reach[d] = (int)ceil(nc[d] * rmax / w[d] - 1e-9);
+            if (nc[d] < 1 || reach[d] < 1 || nc[d] < 2 * reach[d] + 1) ok = false;
+        }
+    }
+    if (!ok) return MDFF_OK;
+    const long long ncell = (long long)nc[0] * nc[1] * nc[2];
+    const int nst = (2 * reach[0] + 1) * (2 * reach[1] + 1) * (2 * reach[2] + 1);
+    if (ncell > (1 << 22) || nst > 125) return MDFF_OK;
+    // a stencil that covers most of the grid scans about every pair anyway:
+    // the all-pairs build is cheaper there unless the grid is asked for
+    if (c->list_method == 0 && ncell < 4LL * nst) return MDFF_OK;
+    CellGrid &g = c->cg;
+    for (int d = 0; d < 3; d++) {
+        g.nc[d] = nc[d];
+        g.reach[d] = reach[d];
+    }
+    g.ncell = (int)ncell;
+    g.nst = nst;
+    int rc;
+    if ((rc = dalloc(&g.ccount, (size_t)ncell)) || (rc = dalloc(&g.cstart, (size_t)ncell + 1)) ||
+        (rc = dalloc(&g.cidx, (size_t)c->n)) || (rc = dalloc(&g.csort, (size_t)c->n)) ||
+        (rc = dalloc(&g.cell, (size_t)c->n)) ||
+        (rc = dalloc(&g.slot, (size_t)c->n)))
+        return rc;
+    c->cells_on = 1;
+    return MDFF_OK;
+}
+
+static void free_cells(mdff_ctx *c)
+{
+    dfree(c->cg.ccount);
+    dfree(c->cg.cstart);
+    dfree(c->cg.cidx);
+    dfree(c->cg.csort);
+    dfree(c->cg.cell);
+    dfree(c->cg.slot);
+    c->cells_set = c->cells_on = 0;
+}
+
+// the list build after k_vcheck (gated on its rebuild flag): cell grid, or
+// every pair; g is the grid (its arrays may be a caller's scratch)
+static void launch_list_build(mdff_ctx *c, const CellGrid &g, bool cells, int all, double csr, double clr)
+{
+    int n = c->n;
+    if (cells && !all) {
+        hipLaunchKernelGGL(k_cell_bin, dim3(cdiv(n, 256)), dim3(256), 0, c->st, n, c->s, g, c->ctl);
+        hipLaunchKernelGGL(k_cell_scan, dim3(1), dim3(1024), 0, c->st, g, c->ctl);
+        hipLaunchKernelGGL(k_cell_fill, dim3(cdiv(n, 256)), dim3(256), 0, c->st, n, g, c->ctl);
+        hipLaunchKernelGGL(k_cell_rank, dim3(cdiv(n, 256)), dim3(256), 0, c->st, n, g, c->ctl);
+        hipLaunchKernelGGL(k_build_cells, dim3(g.ncell), dim3(256), 0, c->st, n, c->cd, c->s, g, csr * csr,
+                           clr * clr, c->cap, c->nbr, c->cnt, c->s0, c->ctl);
+        hipLaunchKernelGGL(k_row_sort, dim3(cdiv(n, 4)), dim3(256), 0, c->st, n, c->cap, c->nbr, c->cnt, c->ctl);
+    } else {
+        hipLaunchKernelGGL(k_build_allpairs, dim3(cdiv(n, 4)), dim3(256), 0, c->st, n, c->cd, c->s, csr * csr,
+                           clr * clr, all, c->cap, c->nbr, c->cnt, c->s0, c->ctl);
+    }
+}
+
 static int build_list(mdff_ctx *c, int force)
 {
     int n = c->n;
+    if (!c->cells_set) {
+        int rc = setup_cells(c);
+        if (rc) return rc;
+    }
     hipLaunchKernelGGL(k_frac, dim3(cdiv(n, 256)), dim3(256), 0, c->st, n, c->cd, c->r, c->r + n, c->r + 2 * n, c->s);
     double csr = c->p.cutshortrange + c->p.skindiff, clr = c->p.cutlongrange + c->p.skindiff;
     int all = !c->p.lverletL;
@@ -292,9 +382,16 @@ static int build_list(mdff_ctx *c, int force)
                                c->p.skindiff, c->ctl, 1, c->vc_part, c->vc_cnt);
             if (attempt > 0) hipLaunchKernelGGL(k_add_int, dim3(1), dim3(1), 0, c->st, c->ctl + C_NUPD, -1);
         }
-        hipLaunchKernelGGL(k_build_allpairs, dim3(cdiv(n, 4)), dim3(256), 0, c->st, n, c->cd, c->s, csr * csr,
-                           clr * clr, all, c->cap, c->nbr, c->cnt, c->s0, c->ctl);
+        // the build only when the check asks for it (the check's flag is
+        // read back first; most steps end here)
         int rc = sync_ctl(c);
+        if (rc) return rc;
+        if (!c->ctl_h[C_REBUILD]) {
+            c->have_list = 1;
+            return MDFF_OK;
+        }
+        launch_list_build(c, c->cg, c->cells_on, all, csr, clr);
+        rc = sync_ctl(c);
         if (rc) return rc;
         if (c->ctl_h[C_OVERFLOW] == 0) {
             c->have_list = 1;
@@ -1934,6 +2031,7 @@ void mdff_destroy(mdff_ctx *c)
                      &c->selfbuf, &c->ke, &c->upol_i, &c->rmsd_i, &c->tmp_q, &c->tmp_mu, &c->tmp_qraw, &c->tmp_mur, &c->dc_R, &c->dc_d, &c->dc_g, &c->cg_z, &c->cg_p, &c->cg_part, &c->chk_part, &c->pq_buf, &c->t2dir, &c->vc_part, &c->ko_f, &c->ko_dmu, &c->t2ko, &c->cg_w, &c->scal,
                      &c->kvec, &c->Ak, &c->kcoe, &c->W, &c->ek, &c->kpart, &c->bpart, &c->ksum, &c->gsc, &c->gwp, &c->gwq};
     for (auto pp : dp) dfree(*pp);
+    free_cells(c);
     dfree(c->type);
     dfree(c->vc_cnt);
     dfree(c->chk_cnt);
@@ -2650,6 +2748,31 @@ int mdff_md_run(mdff_ctx *c, int first_step, int nsteps, mdff_thermo *th)
 
 int mdff_time_kernel(mdff_ctx *c, int which, int lo, int iters, double *ms)
 {
+    if (which == 7) {
+        // forced rebuilds of the neighbour list at the current positions by
+        // the method in use (the next check measures displacements from them)
+        if (!c || !ms || iters < 1) return MDFF_ERR_ARG;
+        int n = c->n;
+        if (!c->cells_set && setup_cells(c)) return MDFF_ERR_NOMEM;
+        hipLaunchKernelGGL(k_frac, dim3(cdiv(n, 256)), dim3(256), 0, c->st, n, c->cd, c->r, c->r + n, c->r + 2 * n,
+                           c->s);
+        const double csr = c->p.cutshortrange + c->p.skindiff, clr = c->p.cutlongrange + c->p.skindiff;
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) return MDFF_ERR_DEVICE;
+        set_int(c, C_REBUILD, 1);
+        launch_list_build(c, c->cg, c->cells_on, !c->p.lverletL, csr, clr);
+        HIPCHK(hipEventRecord(e0, c->st));
+        for (int k = 0; k < iters; k++) launch_list_build(c, c->cg, c->cells_on, !c->p.lverletL, csr, clr);
+        HIPCHK(hipEventRecord(e1, c->st));
+        float t = 0.0f;
+        int rc = (hipEventSynchronize(e1) == hipSuccess && hipEventElapsedTime(&t, e0, e1) == hipSuccess)
+                     ? MDFF_OK : MDFF_ERR_DEVICE;
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        set_int(c, C_REBUILD, 0);
+        *ms = t / iters;
+        return rc;
+    }
     if (!c || !ms || iters < 1 || !c->p.lcoulombic || lo < 0 || lo > 3) return MDFF_ERR_ARG;
     if (lo && !lo_of(c)) return MDFF_ERR_ARG;
     // lo = 2: FP32 tables with the split-BF16 matrix-core mode
@@ -2780,6 +2903,27 @@ int mdff_set_scf_solver(mdff_ctx *c, int solver, int inner)
     c->cg_precond = solver == 3;
     c->inner_iters = inner;
     c->hint_mixed = c->hint_cg = 0;
+    return MDFF_OK;
+}
+
+int mdff_set_list_method(mdff_ctx *c, int method)
+{
+    if (!c || method < 0 || method > 2) return MDFF_ERR_ARG;
+    if (method == c->list_method) return MDFF_OK;
+    c->list_method = method;
+    free_cells(c);
+    return MDFF_OK;
+}
+
+int mdff_list_info(mdff_ctx *c, int out[8])
+{
+    if (!c || !out) return MDFF_ERR_ARG;
+    out[0] = c->cells_on;
+    for (int d = 0; d < 3; d++) {
+        out[1 + d] = c->cells_on ? c->cg.nc[d] : 0;
+        out[4 + d] = c->cells_on ? c->cg.reach[d] : 0;
+    }
+    out[7] = c->cap;
     return MDFF_OK;
 }
 

@@ -179,6 +179,241 @@ __global__ void __launch_bounds__(256) k_build_allpairs(int n, CellD c, const do
     }
 }
 
+// --------------------------------------------------------- cell binning
+// The list of gen_pbc_verletlist (verlet.c:54-122, an all-pairs scan there)
+// from a cell grid: ions binned by fractional position into
+// nc[0] x nc[1] x nc[2] cells, each cell's ions put in ascending index
+// order, then one wavefront per ion scans the (2 reach + 1)^3 cells around
+// its own.  The row of an ion lists its neighbours cell by cell in a fixed
+// stencil order, ascending index within a cell, so the list (and every sum
+// over it) is the same from run to run.
+struct CellGrid {
+    int nc[3], reach[3];  // cells per axis, stencil half-width per axis
+    int ncell, nst;       // cells, stencil cells (<= 125)
+    int *ccount;          // [ncell] ions per cell during binning (zero between builds)
+    int *cstart;          // [ncell + 1] first ion of each cell in cidx
+    int *cidx;            // [n] ions by cell, in arrival order
+    int *csort;           // [n] ions by cell, ascending index within a cell
+    int *cell;            // [n] cell of each ion
+    int *slot;            // [n] arrival slot of each ion in its cell
+};
+
+__device__ __forceinline__ int cell_coord(double sv, int nc)
+{
+    const double u = sv - floor(sv);   // [0, 1): positions are never wrapped
+    const int k = (int)(u * (double)nc);
+    return k < 0 ? 0 : (k >= nc ? nc - 1 : k);
+}
+
+__global__ void __launch_bounds__(256) k_cell_bin(int n, const double *__restrict__ s, CellGrid g, const int *ctl)
+{
+    if (!ctl[C_REBUILD]) return;
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const int cx = cell_coord(s[i], g.nc[0]), cy = cell_coord(s[n + i], g.nc[1]), cz = cell_coord(s[2 * n + i], g.nc[2]);
+    const int c = (cz * g.nc[1] + cy) * g.nc[0] + cx;
+    g.cell[i] = c;
+    g.slot[i] = atomicAdd(&g.ccount[c], 1);
+}
+
+// exclusive prefix of the cell counts (one block), which are cleared for
+// the next binning
+__global__ void __launch_bounds__(1024) k_cell_scan(CellGrid g, const int *ctl)
+{
+    if (!ctl[C_REBUILD]) return;
+    __shared__ int part[1024];
+    const int t = threadIdx.x;
+    const int per = (g.ncell + 1023) / 1024;
+    const int c0 = min(t * per, g.ncell), c1 = min(c0 + per, g.ncell);
+    int sum = 0;
+    for (int c = c0; c < c1; c++) sum += g.ccount[c];
+    part[t] = sum;
+    __syncthreads();
+    for (int o = 1; o < 1024; o <<= 1) {
+        const int v = t >= o ? part[t - o] : 0;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    int run = part[t] - sum;
+    for (int c = c0; c < c1; c++) {
+        const int k = g.ccount[c];
+        g.cstart[c] = run;
+        g.ccount[c] = 0;
+        run += k;
+    }
+    if (t == 1023) g.cstart[g.ncell] = part[1023];
+}
+
+__global__ void __launch_bounds__(256) k_cell_fill(int n, CellGrid g, const int *ctl)
+{
+    if (!ctl[C_REBUILD]) return;
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    g.cidx[g.cstart[g.cell[i]] + g.slot[i]] = i;
+}
+
+// each cell's ions in ascending index order (the arrival order of the
+// binning is not reproducible): ion i goes to the place of its rank among
+// the ions of its cell, counted over the unordered list
+__global__ void __launch_bounds__(256) k_cell_rank(int n, CellGrid g, const int *ctl)
+{
+    if (!ctl[C_REBUILD]) return;
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const int c = g.cell[i], b = g.cstart[c], e = g.cstart[c + 1];
+    int rank = 0;
+    for (int k = b; k < e; k++) rank += g.cidx[k] < i;
+    g.csort[b + rank] = i;
+}
+
+// the rows from the cell grid: the entries of k_build_allpairs (j, NBR_SR
+// within rc_sr + skin, NBR_LR within rc_lr + skin).  One block per home
+// cell: the ions of the stencil around it are staged in LDS in stencil
+// order (chunks of CB_CHUNK), and each wave scans them for the home ions it
+// owns (home ion k goes to wave k % 4), appending in order with ballots.
+#define CB_CHUNK 1024
+#define CB_HOME 256
+__global__ void __launch_bounds__(256) k_build_cells(int n, CellD c, const double *__restrict__ s, CellGrid g,
+                                                    double cut_sr2, double cut_lr2, int cap,
+                                                    unsigned *__restrict__ nbr, int *__restrict__ cnt,
+                                                    double *__restrict__ s0, int *ctl)
+{
+    if (!ctl[C_REBUILD]) return;
+    __shared__ int pre[129], cst[128];
+    __shared__ double qx[CB_CHUNK], qy[CB_CHUNK], qz[CB_CHUNK];
+    __shared__ int qj[CB_CHUNK];
+    __shared__ int hcount[CB_HOME];
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const int home = blockIdx.x;
+    const int cx = home % g.nc[0], cy = (home / g.nc[0]) % g.nc[1], cz = home / (g.nc[0] * g.nc[1]);
+    const int wx = 2 * g.reach[0] + 1, wy = 2 * g.reach[1] + 1;
+    // stencil prefix (wave 0: cells lane and lane + 64)
+    if (w == 0) {
+        int cntk[2], stk[2];
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            const int k = lane + 64 * h;
+            cntk[h] = 0;
+            stk[h] = 0;
+            if (k < g.nst) {
+                const int dx = k % wx - g.reach[0], dy = (k / wx) % wy - g.reach[1], dz = k / (wx * wy) - g.reach[2];
+                const int x = (cx + dx + g.nc[0]) % g.nc[0], y = (cy + dy + g.nc[1]) % g.nc[1];
+                const int z = (cz + dz + g.nc[2]) % g.nc[2];
+                const int cc = (z * g.nc[1] + y) * g.nc[0] + x;
+                stk[h] = g.cstart[cc];
+                cntk[h] = g.cstart[cc + 1] - stk[h];
+            }
+        }
+        int incl0 = cntk[0], incl1 = cntk[1];
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const int v0 = __shfl_up(incl0, o), v1 = __shfl_up(incl1, o);
+            if (lane >= o) {
+                incl0 += v0;
+                incl1 += v1;
+            }
+        }
+        const int tot0 = __shfl(incl0, 63);
+        pre[lane] = incl0 - cntk[0];
+        pre[lane + 64] = tot0 + incl1 - cntk[1];
+        if (lane == 63) pre[128] = tot0 + incl1;
+        cst[lane] = stk[0];
+        cst[lane + 64] = stk[1];
+    }
+    __syncthreads();
+    const int total = pre[128];
+    const int hb = g.cstart[home], nh = g.cstart[home + 1] - hb;
+    for (int h0 = 0; h0 < nh; h0 += CB_HOME) {
+        const int hn = min(CB_HOME, nh - h0);
+        for (int k = t; k < hn; k += 256) hcount[k] = 0;
+        for (int k0 = 0; k0 < total; k0 += CB_CHUNK) {
+            const int kn = min(CB_CHUNK, total - k0);
+            __syncthreads();   // the previous chunk is scanned, hcount is zeroed
+            for (int q = t; q < kn; q += 256) {
+                const int k = k0 + q;
+                int lo = 0, hi = g.nst - 1;
+                while (lo < hi) {
+                    const int mid = (lo + hi + 1) >> 1;
+                    if (pre[mid] <= k) lo = mid;
+                    else hi = mid - 1;
+                }
+                const int j = g.csort[cst[lo] + (k - pre[lo])];
+                qj[q] = j;
+                qx[q] = s[j];
+                qy[q] = s[n + j];
+                qz[q] = s[2 * n + j];
+            }
+            __syncthreads();
+            for (int hk = w; hk < hn; hk += 4) {
+                const int i = g.csort[hb + h0 + hk];
+                const double six = s[i], siy = s[n + i], siz = s[2 * n + i];
+                int count = hcount[hk];
+                for (int q0 = 0; q0 < kn; q0 += 64) {
+                    const int q = q0 + lane;
+                    unsigned e = 0;
+                    if (q < kn) {
+                        const int j = qj[q];
+                        if (j != i) {
+                            double x, y, z;
+                            min_image(c, six - qx[q], siy - qy[q], siz - qz[q], x, y, z);
+                            const double d2 = x * x + y * y + z * z;
+                            if (d2 <= cut_sr2) e |= NBR_SR;
+                            if (d2 <= cut_lr2) e |= NBR_LR;
+                            if (e) e |= (unsigned)j;
+                        }
+                    }
+                    const unsigned long long m = __ballot(e != 0);
+                    const int pos = count + __popcll(m & ((1ull << lane) - 1ull));
+                    if (e != 0 && pos < cap) nbr[(size_t)i * cap + pos] = e;
+                    count += __popcll(m);
+                }
+                if (lane == 0) hcount[hk] = count;
+            }
+        }
+        __syncthreads();
+        for (int hk = t; hk < hn; hk += 256) {
+            const int i = g.csort[hb + h0 + hk];
+            const int count = hcount[hk];
+            cnt[i] = count < cap ? count : cap;
+            if (count > cap) atomicMax(&ctl[C_OVERFLOW], count);
+            atomicMax(&ctl[C_MAXCNT], count);
+            s0[i] = s[i];
+            s0[n + i] = s[n + i];
+            s0[2 * n + i] = s[2 * n + i];
+        }
+        __syncthreads();
+    }
+}
+
+// each row in ascending neighbour index, as the all-pairs scan leaves it
+// (the pair kernels gather from monotone addresses, and both builds give
+// the same rows): one wavefront per ion places every entry at its rank,
+// counted over the row staged in LDS; rows longer than RS_MAX keep the
+// cell order
+#define RS_MAX 1024
+__global__ void __launch_bounds__(256) k_row_sort(int n, int cap, unsigned *__restrict__ nbr,
+                                                 const int *__restrict__ cnt, const int *ctl)
+{
+    if (!ctl[C_REBUILD]) return;
+    __shared__ unsigned row[4][RS_MAX];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int i = blockIdx.x * 4 + w;
+    if (i >= n) return;
+    const int m = cnt[i];
+    if (m > RS_MAX) return;
+    unsigned *r = nbr + (size_t)i * cap;
+    for (int p = lane; p < m; p += 64) row[w][p] = r[p];
+    __builtin_amdgcn_s_waitcnt(0);
+    __builtin_amdgcn_wave_barrier();
+    for (int p = lane; p < m; p += 64) {
+        const unsigned e = row[w][p], key = e & 0x3fffffffu;
+        int rank = 0;
+        for (int q = 0; q < m; q++) rank += (row[w][q] & 0x3fffffffu) < key;
+        r[rank] = e;
+    }
+}
+
 // ------------------------------------------------------------- short range
 struct ShortArgs {
     int n, cap;

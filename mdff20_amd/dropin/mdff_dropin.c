@@ -297,6 +297,34 @@ static void prop_agate_impl(int step)
     step_globals(c, &th, 1);
 }
 
+/* CONTFF / TRAJFF records of the globals in write_config's format
+ * (config.c:291-345) by the library's writer, which opens the file with a
+ * proper mode string (write_config passes the address of one char as the
+ * mode, whose undefined tail makes some of its opens fail) */
+static void config_out(const char *file, const char *mode)
+{
+    mdff_config cfg;
+    memset(&cfg, 0, sizeof(cfg));
+    cfg.nion = nion;
+    cfg.ntype = ntype;
+    snprintf(cfg.name, sizeof(cfg.name), "%s", configname);
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) cfg.A[i][j] = simuCell.A[i][j];
+    for (int t = 0; t < ntype && t < MDFF_NTYPEMAX; t++) {
+        snprintf(cfg.atypit[t], sizeof(cfg.atypit[t]), "%s", atypit[t]);
+        cfg.nionit[t] = nionit[t];
+    }
+    cfg.typia = typia;
+    cfg.rx = rx; cfg.ry = ry; cfg.rz = rz;
+    cfg.vx = vx; cfg.vy = vy; cfg.vz = vz;
+    cfg.fx = fx; cfg.fy = fy; cfg.fz = fz;
+    int rc = mdff_write_config(file, mode, &cfg);
+    if (rc) {
+        pError("opening file :");
+        printf(" %s\n", file);
+    }
+}
+
 /* whether run_md writes anything after step s (md.c:250-277) */
 static int output_step(int s)
 {
@@ -357,12 +385,12 @@ static void run_md_impl(void)
         if (iopnode(istep, npas, fprint)) info_thermo(1, fpOSZIFF);
         const int wc = iopnode(istep, npas, cprint), wt = iopsnode(istep, npas, tprint, tstart);
         if (wc || wt) state_down(c);
-        if (wc) write_config("CONTFF", 'w');
-        if (wt) write_config("TRAJFF", 'a');
+        if (wc) config_out("CONTFF", "w");
+        if (wt) config_out("TRAJFF", "a");
         first = last + 1;
     }
     state_down(c);
-    write_config("CONTFF", 'w');
+    config_out("CONTFF", "w");
     fclose(fpOSZIFF);
     /* the SCF's dipole Ewald calls of all steps: one static-field call per
      * force call, one per iteration and per inner scfkO sweep */

@@ -339,3 +339,35 @@ def test_fp64_anchored_cg_reaches_the_oracle_fixed_point():
         assert rel(e.dipoles(), o.dipoles()) < 1e-9
         assert rel(e.forces(), o.forces()) < 1e-9
         assert t["u_tot"] == pytest.approx(to["u_tot"], rel=1e-12)
+
+
+def test_cell_grid_list_equals_all_pairs():
+    """the cell-grid neighbour build against the all-pairs scan on the
+    1536-ion PIM melt (5 x 5 x 5 cells, asked for): both leave each row in ascending
+    neighbour index, so 20 FP64 NVT steps give bitwise the same energies,
+    forces, rebuilds and SCF counts"""
+    import bench
+    L, pos, typ = bench.system(512, 1536)
+    r = bench.equilibrate(L, pos, typ, 100, kmax=10)
+    p = bench.params_of(bench.control(True, alpha=0.3, kmax=10, rc=9.0))
+    A = np.diag([L, L, L])
+    with m.Engine(p, typ, A, r) as ec, m.Engine(p, typ, A, r) as ea:
+        assert not ec.list_info()["cells"]   # 125 cells, all in the stencil: all pairs by default
+        ec.set_list_method(2)
+        ea.set_list_method(1)
+        ec.check_verlet()
+        ic, ia = ec.list_info(), ea.list_info()
+        print(f"[gpu] cell grid {ic}, all pairs {ia}", flush=True)
+        assert ic["cells"] and ic["nc"] == [5, 5, 5] and ic["reach"] == [2, 2, 2] and not ia["cells"]
+        for e in (ec, ea):
+            e.init_velocities()
+        tc, ta = ec.md_init(), ea.md_init()
+        for s in range(1, 21):
+            tc, ta = ec.md_step(s), ea.md_step(s)
+            assert tc["verlet_updates"] == ta["verlet_updates"] and tc["n_scf"] == ta["n_scf"], s
+            assert tc["u_tot"] == ta["u_tot"], s
+        assert tc["verlet_updates"] >= 2
+        assert np.array_equal(ec.forces(), ea.forces())
+        assert np.array_equal(ec.positions(), ea.positions())
+        tcell, tall = ec.time_kernel(7, iters=5), ea.time_kernel(7, iters=5)
+        print(f"[gpu] list rebuild at N=1536: cells {tcell * 1e3:.1f} us, all pairs {tall * 1e3:.1f} us", flush=True)

@@ -148,8 +148,15 @@ def test_dropin_run_md_matches_reference_run_md(case, keys, tmp_path):
     for key in ("rx", "ry", "rz", "vx", "vy", "vz"):
         assert rel(a[key], b[key]) < TOL, key
     assert rel(np.stack([a["fx"], a["fy"], a["fz"]]), np.stack([b["fx"], b["fy"], b["fz"]]), 1e-4) < TOL
-    # the files run_md writes, number by number (printed precision)
-    for f in ("OSZIFF", "TRAJFF", "CONTFF"):
+    # the files run_md writes, number by number (printed precision).  The
+    # reference's write_config opens CONTFF / TRAJFF with the address of one
+    # char as the mode (config.c:309), so some of its opens fail at random
+    # ("opening file" on stdout); those files are compared only when every
+    # open succeeded
+    files = ["OSZIFF"]
+    if "opening file" not in open(os.path.join(dr, "ref_stdout.txt")).read():
+        files += ["TRAJFF", "CONTFF"]
+    for f in files:
         x, y = _numbers(os.path.join(dd, f)), _numbers(os.path.join(dr, f))
         assert x.shape == y.shape and x.size > 0, f
         assert np.all(np.abs(x - y) <= 1e-6 * np.maximum(np.abs(y), 1e-3)), f
