@@ -87,6 +87,17 @@ int mdff_init_velocities(mdff_ctx *ctx);
  * The first call always builds. */
 int mdff_check_verlet(mdff_ctx *ctx);
 
+/* mdff_check_verlet without any allocation: no counterpart in the
+ * reference; for callers that cannot allocate device memory (tight serving
+ * loops, code that forbids hipMalloc).  It runs the same check and rebuild,
+ * with its scratch (the check's block partials and count, the cell grid's
+ * arrays) in one fixed array per GPU held until the process ends (about
+ * 3 MiB), and never grows the row capacity: a list that needs more room
+ * returns MDFF_ERR_LIST_OVERFLOW (call mdff_check_verlet once to grow it).
+ * Limits: at most 131072 ions and 131072 grid cells, else MDFF_ERR_SCRATCH.
+ * Calls on one GPU share that array: calls that overlap on a GPU race. */
+int mdff_check_verlet_noalloc(mdff_ctx *ctx);
+
 /* One force evaluation at the current positions (engforce field.c:352-407):
  * short-range pair forces, PIM dipoles (SCF), multipole Ewald forces,
  * energies and stress.  istep is the step number seen by the ASPC ramp.

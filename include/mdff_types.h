@@ -46,7 +46,9 @@ enum {
                                   (reported, not fatal: the reference accepts
                                   it silently, pim.c:236)                  */
     MDFF_ERR_UNSUPPORTED = 8,  /* feature not built in this engine         */
-    MDFF_ERR_NOMEM = 9
+    MDFF_ERR_NOMEM = 9,
+    MDFF_ERR_SCRATCH = 10      /* larger than the fixed scratch of a
+                                  no-allocation entry point                */
 };
 
 typedef struct mdff_params {

@@ -69,6 +69,7 @@ def lib():
         "mdff_get_ewald": (C.c_int, [vp, dp, ip, ip]),
         "mdff_init_velocities": (C.c_int, [vp]),
         "mdff_check_verlet": (C.c_int, [vp]),
+        "mdff_check_verlet_noalloc": (C.c_int, [vp]),
         "mdff_engforce": (C.c_int, [vp, C.c_int, P(Thermo)]),
         "mdff_pair_short": (C.c_int, [vp, dp, dp, dp]),
         "mdff_multipole_es": (C.c_int, [vp, dp, dp, dp, C.c_int, dp, dp, dp, dp, dp, dp, dp, dp]),
@@ -339,6 +340,10 @@ class Engine:
 
     def check_verlet(self):
         _check(self._L.mdff_check_verlet(self._h), "check_verlet", False)
+
+    def check_verlet_noalloc(self):
+        """the Verlet check and rebuild with no device allocation (fixed scratch per GPU)"""
+        _check(self._L.mdff_check_verlet_noalloc(self._h), "check_verlet_noalloc", False)
 
     def engforce(self, istep=0):
         t = Thermo()

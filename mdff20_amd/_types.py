@@ -18,7 +18,8 @@ FP64, MIXED = 0, 1
 KS_BF16X3, KS_F32, KS_F16X3 = 1, 2, 3
 STATUS = {0: "ok", 1: "bad argument", 2: "invalid configuration", 3: "I/O error",
           4: "HIP runtime error", 5: "neighbour list overflow", 6: "non-finite result",
-          7: "SCF unconverged", 8: "unsupported", 9: "out of memory"}
+          7: "SCF unconverged", 8: "unsupported", 9: "out of memory",
+          10: "larger than the fixed scratch of a no-allocation entry point"}
 ERR_SCF = 7
 ES = dict(LQ=1 << 0, LD=1 << 1, LT=1 << 2, LDAMP=1 << 3, FORCES=1 << 4, STRESS=1 << 5,
           EF=1 << 6, EFG=1 << 7, DIR=1 << 8, REC=1 << 9, INPIM=1 << 10, UPDATE_SF=1 << 11)

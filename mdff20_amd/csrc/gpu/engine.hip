@@ -10,6 +10,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <mutex>
 #include <vector>
 
 #include "kernels.h"
@@ -284,11 +285,9 @@ static void build_tables(mdff_ctx *c)
 // with fewer than 3 of those the box is too small for a grid and every pair
 // is scanned.  Widths are the perpendicular ones, 1/|B_d|, so triclinic
 // cells are covered.
-static int setup_cells(mdff_ctx *c)
+static bool grid_dims(const mdff_ctx *c, CellGrid &g)
 {
-    c->cells_set = 1;
-    c->cells_on = 0;
-    if (c->list_method == 1 || !c->p.lverletL) return MDFF_OK;
+    if (c->list_method == 1 || !c->p.lverletL) return false;
     const double rmax = std::max(c->p.cutshortrange, c->p.cutlongrange) + c->p.skindiff;
     double w[3];
     for (int d = 0; d < 3; d++) {
@@ -305,25 +304,32 @@ static int setup_cells(mdff_ctx *c)
             if (nc[d] < 1 || reach[d] < 1 || nc[d] < 2 * reach[d] + 1) ok = false;
         }
     }
-    if (!ok) return MDFF_OK;
+    if (!ok) return false;
     const long long ncell = (long long)nc[0] * nc[1] * nc[2];
     const int nst = (2 * reach[0] + 1) * (2 * reach[1] + 1) * (2 * reach[2] + 1);
-    if (ncell > (1 << 22) || nst > 125) return MDFF_OK;
+    if (ncell > (1 << 22) || nst > 125) return false;
     // a stencil that covers most of the grid scans about every pair anyway:
     // the all-pairs build is cheaper there unless the grid is asked for
-    if (c->list_method == 0 && ncell < 4LL * nst) return MDFF_OK;
-    CellGrid &g = c->cg;
+    if (c->list_method == 0 && ncell < 4LL * nst) return false;
     for (int d = 0; d < 3; d++) {
         g.nc[d] = nc[d];
         g.reach[d] = reach[d];
     }
     g.ncell = (int)ncell;
     g.nst = nst;
+    return true;
+}
+
+static int setup_cells(mdff_ctx *c)
+{
+    c->cells_set = 1;
+    c->cells_on = 0;
+    CellGrid &g = c->cg;
+    if (!grid_dims(c, g)) return MDFF_OK;
     int rc;
-    if ((rc = dalloc(&g.ccount, (size_t)ncell)) || (rc = dalloc(&g.cstart, (size_t)ncell + 1)) ||
+    if ((rc = dalloc(&g.ccount, (size_t)g.ncell)) || (rc = dalloc(&g.cstart, (size_t)g.ncell + 1)) ||
         (rc = dalloc(&g.cidx, (size_t)c->n)) || (rc = dalloc(&g.csort, (size_t)c->n)) ||
-        (rc = dalloc(&g.cell, (size_t)c->n)) ||
-        (rc = dalloc(&g.slot, (size_t)c->n)))
+        (rc = dalloc(&g.cell, (size_t)c->n)) || (rc = dalloc(&g.slot, (size_t)c->n)))
         return rc;
     c->cells_on = 1;
     return MDFF_OK;
@@ -2519,6 +2525,92 @@ int mdff_check_verlet(mdff_ctx *c)
 {
     if (!c) return MDFF_ERR_ARG;
     return build_list(c, 0);
+}
+
+// ---------------------------------------------- the check without allocation
+// mdff_check_verlet_noalloc: the check and rebuild of mdff_check_verlet with
+// every buffer beyond the context's own (the check's block partials and
+// finished-block count, the cell grid's five arrays) in `scratch`, one
+// array per GPU that lives until the process ends, and no growth of the
+// row capacity.  Its address on each device comes from hipGetSymbolAddress
+// on the first call there (the array's name is not a device address on the
+// host).  Calls that overlap on one GPU share the array and race.
+#define NA_MAX_ION (1 << 17)
+#define NA_MAX_CELL (1 << 17)
+#define NA_OFF_PART 0                                               // double [2][NA_MAX_ION / 256]
+#define NA_OFF_CNT (NA_OFF_PART + 16 * (NA_MAX_ION / 256))          // unsigned, 256-byte slot
+#define NA_OFF_CCOUNT (NA_OFF_CNT + 256)                            // int [NA_MAX_CELL]
+#define NA_OFF_CSTART (NA_OFF_CCOUNT + 4 * NA_MAX_CELL)             // int [NA_MAX_CELL + 1], padded
+#define NA_OFF_IONS (NA_OFF_CSTART + 4 * NA_MAX_CELL + 256)         // int [4][NA_MAX_ION]
+#define NA_BYTES (NA_OFF_IONS + 16 * NA_MAX_ION)
+static __device__ __align__(256) unsigned char scratch[NA_BYTES];
+
+#define NA_MAX_DEV 64
+static void *g_scratch_ptr[NA_MAX_DEV];
+static std::mutex g_scratch_mu;
+
+static int scratch_address(int device, unsigned char **out)
+{
+    if (device < 0 || device >= NA_MAX_DEV) return MDFF_ERR_ARG;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        int cur = 0;
+        if (hipGetDevice(&cur) != hipSuccess || hipSetDevice(device) != hipSuccess) return MDFF_ERR_DEVICE;
+        void *p = nullptr;
+        const hipError_t err = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        (void)hipSetDevice(cur);
+        if (err != hipSuccess || !p) return MDFF_ERR_DEVICE;
+        g_scratch_ptr[device] = p;
+    }
+    *out = (unsigned char *)g_scratch_ptr[device];
+    return MDFF_OK;
+}
+
+int mdff_check_verlet_noalloc(mdff_ctx *c)
+{
+    if (!c) return MDFF_ERR_ARG;
+    const int n = c->n;
+    CellGrid g{};
+    const bool cells = grid_dims(c, g);
+    if (n > NA_MAX_ION || (cells && g.ncell > NA_MAX_CELL)) return MDFF_ERR_SCRATCH;
+    unsigned char *base = nullptr;
+    int rc = scratch_address(c->device, &base);
+    if (rc) return rc;
+    double *part = (double *)(base + NA_OFF_PART);
+    unsigned *bcnt = (unsigned *)(base + NA_OFF_CNT);
+    g.ccount = (int *)(base + NA_OFF_CCOUNT);
+    g.cstart = (int *)(base + NA_OFF_CSTART);
+    g.cidx = (int *)(base + NA_OFF_IONS);
+    g.csort = g.cidx + NA_MAX_ION;
+    g.cell = g.csort + NA_MAX_ION;
+    g.slot = g.cell + NA_MAX_ION;
+    // what must start at zero: the finished-block count and the cell counts
+    // (the kernels leave both at zero, a call cut short may not)
+    hipLaunchKernelGGL(k_zero, dim3(cdiv((256 + 4 * (size_t)NA_MAX_CELL) / 8, 256)), dim3(256), 0, c->st,
+                       (double *)(base + NA_OFF_CNT), (256 + 4 * (size_t)NA_MAX_CELL) / 8);
+    hipLaunchKernelGGL(k_frac, dim3(cdiv(n, 256)), dim3(256), 0, c->st, n, c->cd, c->r, c->r + n, c->r + 2 * n, c->s);
+    const double csr = c->p.cutshortrange + c->p.skindiff, clr = c->p.cutlongrange + c->p.skindiff;
+    const int all = !c->p.lverletL;
+    if (all) {
+        set_int(c, C_OVERFLOW, 0);
+        set_int(c, C_MAXCNT, 0);
+        set_int(c, C_REBUILD, c->have_list ? 0 : 1);
+    } else {
+        if (!c->have_list) set_int(c, C_FIRST, 1);
+        hipLaunchKernelGGL(k_vcheck, dim3(cdiv(n, 256)), dim3(256), 0, c->st, n, c->cd, c->s, c->s0, c->p.skindiff,
+                           c->ctl, 1, part, bcnt);
+    }
+    if ((rc = sync_ctl(c))) return rc;
+    if (!c->ctl_h[C_REBUILD]) return MDFF_OK;
+    launch_list_build(c, g, cells, all, csr, clr);
+    if ((rc = sync_ctl(c))) return rc;
+    if (c->ctl_h[C_OVERFLOW]) {
+        // the rows would need more room than the context holds
+        c->have_list = 0;
+        return MDFF_ERR_LIST_OVERFLOW;
+    }
+    c->have_list = 1;
+    return MDFF_OK;
 }
 
 static int finish_call(mdff_ctx *c, int rc, mdff_thermo *th)
