@@ -169,7 +169,7 @@ def batch_relax(confs, ews, iters=30, gamma=0.01, cap=0.05):
     return [(t, A, rr.copy()) for (t, A, _), rr in zip(confs, parts)]
 
 
-def timed_run(p, typ, A, r, precision, steps, warmup, fp64_cg=False, list_method=0):
+def timed_run(p, typ, A, r, precision, steps, warmup, fp64_cg=False, list_method=0, check=True):
     import mdff20_amd as m
     e = m.Engine(p, typ, A, r, precision=precision)
     if list_method:
@@ -191,7 +191,7 @@ def timed_run(p, typ, A, r, precision, steps, warmup, fp64_cg=False, list_method
     stats = dict(scf_fp64_per_step=(c1["scf_fp64"] - c0["scf_fp64"]) / steps,
                  scf_inner_per_step=(c1["scf_inner"] - c0["scf_inner"]) / steps,
                  rmsd=th["rmsd"], temp=th["temp_r"], h_tot=th["h_tot"])
-    if precision == m.MIXED:
+    if precision == m.MIXED and check:
         # after the timed region: the FP64 residual of the SCF's final
         # dipoles (the SCF stops on its carried low-precision residual)
         e.set_scf_check(1)
@@ -248,8 +248,16 @@ MIXED_EVAL, MIXED_INNER, MIXED_ANCHOR = "f16x3", "f16x3", "f16x3"
 CALIB_BYTES = 512 << 20
 
 
+# the real-space pair kernels behind mdff_time_kernel's which 5 and 6
+PAIR = {"short": (5, "k_short"), "dir_force": (6, "k_ewald_dir")}
+
+
 def kernel_prefix(kname):
     name, var = kname.split("/")
+    if name in PAIR:
+        which, pre = PAIR[name]
+        lo = VARIANT[var][0]
+        return which, lo, f"{pre}<{'float' if lo else 'double'},"
     which, pre = TIMED[name]
     lo, mode = VARIANT[var]
     return which, lo, f"{pre}<{mode},"
@@ -268,7 +276,10 @@ def pmc_run(args, kname):
     import mdff20_amd as m
     which, lo, _ = kernel_prefix(kname)
     L, pos, typ = system(args.units, args.seed)
-    r = equilibrate(L, pos, typ, args.relax)
+    if args.state and os.path.exists(args.state):
+        r = np.load(args.state, allow_pickle=False)["r"]
+    else:
+        r = equilibrate(L, pos, typ, args.relax)
     A = np.diag([L, L, L])
     p = params_of(control(True))
     e = m.Engine(p, typ, A, r, precision=m.MIXED)
@@ -331,6 +342,25 @@ def pmc_summarize(args, kname):
     print(json.dumps(out, indent=1))
 
 
+def pmc_table(args, kname, dirs):
+    """per-dispatch averages of every counter found in the rocprofv3 --pmc
+    output directories `dirs` for the timed launches of one kernel"""
+    _, _, prefix = kernel_prefix(kname)
+    out = {}
+    for d in dirs:
+        rows = read_counters(d)
+        by = {}
+        for x in rows:
+            if prefix not in x["Kernel_Name"]:
+                continue
+            key = (x["Counter_Name"], int(x["Dispatch_Id"]))
+            by[key] = by.get(key, 0.0) + float(x["Counter_Value"])
+        for cname in sorted({k[0] for k in by}):
+            vals = [v for (c, _), v in sorted(by.items(), key=lambda t: t[0][1]) if c == cname][-args.pmc_iters:]
+            out[cname] = sum(vals) / len(vals)
+    return out
+
+
 def trace_summarize(d, top=40):
     """markdown table of a rocprofv3 --kernel-trace --stats run"""
     import csv
@@ -360,11 +390,13 @@ def main():
     ap.add_argument("--pmc-kernel", default=None, help="e.g. sk_forward_scf/bf16x3 (counter runs)")
     ap.add_argument("--pmc-iters", type=int, default=5)
     ap.add_argument("--pmc-summarize", nargs=2, default=None, metavar=("FETCH_DIR", "WRITE_DIR"))
+    ap.add_argument("--pmc-table", nargs="+", default=None, metavar="DIR",
+                    help="per-dispatch averages of all counters of --pmc-kernel in these rocprofv3 output dirs")
     ap.add_argument("--trace-summarize", default=None, metavar="DIR")
     ap.add_argument("--list-method", type=int, default=0, help="0 cell-grid neighbour build (default), 1 all pairs")
     ap.add_argument("--no-kernels", action="store_true",
-                    help="skip the per-kernel timing of every arithmetic variant and the accuracy call "
-                         "(a kernel trace then holds the timed MD steps of the chosen forms only)")
+                    help="skip the per-kernel timing of every arithmetic variant, the accuracy call and the "
+                         "FP64 residual check (a kernel trace then holds the timed MD steps of the chosen forms only)")
     ap.add_argument("--state", default=None, metavar="NPZ",
                     help="relaxed positions: read from NPZ when it exists, else relaxed and written there")
     args = ap.parse_args()
@@ -373,6 +405,10 @@ def main():
         return
     if args.pmc_summarize:
         pmc_summarize(args, args.pmc_kernel)
+        return
+    if args.pmc_table:
+        print(json.dumps({"kernel": args.pmc_kernel, "prefix": kernel_prefix(args.pmc_kernel)[2],
+                          "per_dispatch": pmc_table(args, args.pmc_kernel, args.pmc_table)}, indent=1))
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (every box of this build has one GPU); --gpus {args.gpus} "
@@ -406,7 +442,7 @@ def main():
     for f in forms:
         prec = m.MIXED if f == "mixed" else m.FP64
         e, dt, stats = timed_run(p, typ, A, r, prec, args.steps, args.warmup, fp64_cg=(f == "fp64_cg"),
-                                 list_method=args.list_method)
+                                 list_method=args.list_method, check=not args.no_kernels)
         res[f] = dict(seconds=dt, steps_per_s=args.steps / dt, **stats)
         if f == head:
             eng[f] = e

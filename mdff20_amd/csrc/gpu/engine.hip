@@ -1897,7 +1897,7 @@ static int step_vv(mdff_ctx *c, int istep)
 }
 
 // nhcn integration.c:140-182 with chain_nhcn :189-294
-static void nhc_chain(mdff_ctx *c, bool reduce = true)
+static void nhc_chain(mdff_ctx *c, bool reduce = true, bool envt = false)
 {
     NhcArgs a;
     a.nh = c->p.nhc_n;
@@ -1906,9 +1906,17 @@ static void nhc_chain(mdff_ctx *c, bool reduce = true)
     a.dt = c->dt;
     a.kT = c->kT;
     a.L = 3.0 * (double)c->n;
-    for (int i = 0; i < 16; i++) a.Q[i] = c->nhcQ[i];
+    for (int i = 0; i < 16; i++) {
+        a.Q[i] = c->nhcQ[i];
+        a.iQ[i] = c->nhcQ[i] != 0.0 ? 1.0 / c->nhcQ[i] : 0.0;
+    }
     for (int i = 0; i < 7; i++) a.w[i] = c->yosh[i];
-    hipLaunchKernelGGL(k_nhc_chain, dim3(1), dim3(1), 0, c->st, a, c->scal);
+    if (a.nh <= 4)
+        hipLaunchKernelGGL(k_nhc_chain<4>, dim3(1), dim3(1), 0, c->st, a, c->scal, envt ? 1 : 0);
+    else if (a.nh <= 8)
+        hipLaunchKernelGGL(k_nhc_chain<8>, dim3(1), dim3(1), 0, c->st, a, c->scal, envt ? 1 : 0);
+    else
+        hipLaunchKernelGGL(k_nhc_chain<16>, dim3(1), dim3(1), 0, c->st, a, c->scal, envt ? 1 : 0);
     // with reduce, the scaling kernel also sums the kinetic energy into S_KIN
     // (the SCF-check partials and count are free here)
     if (reduce)
@@ -1938,13 +1946,8 @@ static int step_nhc(mdff_ctx *c, int istep)
     // the kick also sums the kinetic energy into S_KIN
     hipLaunchKernelGGL(k_kick, dim3(cdiv(n, 256)), dim3(256), 0, c->st, n, c->dt, c->v, c->f, c->invmass, c->massd,
                        c->ke, c->chk_part, c->chk_cnt, c->scal + S_KIN);
-    nhc_chain(c);
-    NhcArgs a;
-    a.nh = c->p.nhc_n;
-    a.kT = c->kT;
-    a.L = 3.0 * (double)n;
-    for (int i = 0; i < 16; i++) a.Q[i] = c->nhcQ[i];
-    hipLaunchKernelGGL(k_nhc_envt, dim3(1), dim3(1), 0, c->st, a, c->scal);
+    // second half, with e_nvt of the advanced chain
+    nhc_chain(c, true, true);
     c->kin_fresh = 1;
     return rc;
 }

@@ -293,17 +293,19 @@ __global__ void k_scale_v(int n, const double *s, double *v, const double *m, do
 struct NhcArgs {
     int nh, mults, yo;
     double dt, kT, L;
-    double Q[16];
+    double Q[16], iQ[16];   // thermostat masses and their inverses
     double w[7];
 };
 
 // chain_nhcn integration.c:189-294 on the thermostat scalars (one thread;
 // the forward sweep leaves vxi[n-2] unchanged, integration.c:269-277).  The
 // chain is held in registers: loops run over the maximum length 16 with
-// constant indices, guarded by the chain length.
-__global__ void k_nhc_chain(NhcArgs a, double *scal)
+// constant indices, guarded by the chain length; NH is the register
+// capacity (4, 8 or 16) chosen from the chain length at launch so that short
+// chains keep every term in registers.
+template <int NH>
+__global__ void __launch_bounds__(64) k_nhc_chain(NhcArgs a, double *scal, int envt)
 {
-    constexpr int NH = 16;
     const int nh = a.nh;
     double v[NH], x[NH], G[NH];
 #pragma unroll
@@ -317,12 +319,12 @@ __global__ void k_nhc_chain(NhcArgs a, double *scal)
     G[0] = 2.0 * kin - a.L * a.kT;
     for (int k = 0; k < a.mults; k++) {
         for (int j = 0; j < a.yo; j++) {
-            const double dts = (j < 7 ? a.w[j] : 0.0) * a.dt / ((double)a.mults);
+            const double dts = (j < 7 ? a.w[j] : 0.0) * a.dt * (1.0 / (double)a.mults);
             const double dts2 = dts * 0.5, dts4 = dts2 * 0.5, dts8 = dts4 * 0.5;
 #pragma unroll
             for (int i = 1; i < NH; i++)
                 if (i == nh - 1) {
-                    G[i] = (v[i - 1] * v[i - 1] / a.Q[i - 1] - a.kT);
+                    G[i] = (v[i - 1] * v[i - 1] * a.iQ[i - 1] - a.kT);
                     v[i] += G[i] * dts4;
                 }
             // the two scalings around each kick use one exponential (same
@@ -330,24 +332,24 @@ __global__ void k_nhc_chain(NhcArgs a, double *scal)
 #pragma unroll
             for (int ih = NH - 2; ih > -1; ih--)
                 if (ih <= nh - 2) {
-                    const double e = exp(-v[ih + 1] * dts8 / a.Q[ih + 1]);
+                    const double e = exp(-v[ih + 1] * dts8 * a.iQ[ih + 1]);
                     v[ih] *= e;
                     v[ih] += G[ih] * dts4;
                     v[ih] *= e;
                 }
-            s *= exp(-v[0] * dts2 / a.Q[0]);
+            s *= exp(-v[0] * dts2 * a.iQ[0]);
 #pragma unroll
             for (int i = 0; i < NH; i++)
-                if (i < nh) x[i] += v[i] * dts2 / a.Q[i];
+                if (i < nh) x[i] += v[i] * dts2 * a.iQ[i];
             G[0] = 2.0 * kin * s * s - a.L * a.kT;
 #pragma unroll
             for (int ih = 0; ih < NH - 2; ih++)
                 if (ih < nh - 2) {
-                    const double e = exp(-v[ih + 1] * dts8 / a.Q[ih + 1]);
+                    const double e = exp(-v[ih + 1] * dts8 * a.iQ[ih + 1]);
                     v[ih] *= e;
                     v[ih] += G[ih] * dts4;
                     v[ih] *= e;
-                    G[ih + 1] = (v[ih] * v[ih] / a.Q[ih] - a.kT);
+                    G[ih + 1] = (v[ih] * v[ih] * a.iQ[ih] - a.kT);
                 }
 #pragma unroll
             for (int i = 1; i < NH; i++)
@@ -361,19 +363,19 @@ __global__ void k_nhc_chain(NhcArgs a, double *scal)
             scal[S_XI + i] = x[i];
         }
     scal[S_NHC_S] = s;
-}
-
-// conserved-quantity term of nhcn integration.c:161-176
-__global__ void k_nhc_envt(NhcArgs a, double *scal)
-{
-    const double *vxi = scal + S_VXI, *xi = scal + S_XI;
-    double e = a.L * a.kT * xi[0] + vxi[0] * vxi[0] * 0.5 / a.Q[0];
-    double e3 = 0.0, e4 = 0.0;
-    for (int i = 1; i < a.nh; i++) {
-        e3 += vxi[i] * vxi[i] * 0.5 / a.Q[i];
-        e4 += a.kT * xi[i];
+    if (envt) {
+        // the conserved-quantity term of nhcn integration.c:161-176 from the
+        // chain just advanced
+        double e = a.L * a.kT * x[0] + v[0] * v[0] * 0.5 / a.Q[0];
+        double e3 = 0.0, e4 = 0.0;
+#pragma unroll
+        for (int i = 1; i < NH; i++)
+            if (i < nh) {
+                e3 += v[i] * v[i] * 0.5 / a.Q[i];
+                e4 += a.kT * x[i];
+            }
+        scal[S_ENVT] = e + e3 + e4;
     }
-    scal[S_ENVT] = e + e3 + e4;
 }
 
 // prop_leap_frog integration.c:101-128
