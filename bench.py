@@ -169,7 +169,7 @@ def batch_relax(confs, ews, iters=30, gamma=0.01, cap=0.05):
     return [(t, A, rr.copy()) for (t, A, _), rr in zip(confs, parts)]
 
 
-def timed_run(p, typ, A, r, precision, steps, warmup, fp64_cg=False, list_method=0, check=True):
+def timed_run(p, typ, A, r, precision, steps, warmup, fp64_cg=False, list_method=0, check=True, refine=False):
     import mdff20_amd as m
     e = m.Engine(p, typ, A, r, precision=precision)
     if list_method:
@@ -177,6 +177,8 @@ def timed_run(p, typ, A, r, precision, steps, warmup, fp64_cg=False, list_method
     if precision == m.MIXED:
         e.set_kspace_modes(VARIANT[MIXED_EVAL][1], VARIANT[MIXED_INNER][1])
         e.set_scf_anchor(VARIANT[MIXED_ANCHOR][1])
+        if refine:
+            e.set_scf_refine(True)
     elif fp64_cg:
         e.set_scf_solver(2, 0)
     e.init_velocities()
@@ -385,8 +387,9 @@ def main():
     ap.add_argument("--seed", type=int, default=2026)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "16")))
-    ap.add_argument("--forms", default="fp64,fp64_cg,mixed",
-                    help="fp64 (the reference's Jacobi SCF), fp64_cg (FP64 with the anchored CG SCF), mixed")
+    ap.add_argument("--forms", default="fp64,fp64_cg,mixed,mixed_refine",
+                    help="fp64 (the reference's Jacobi SCF), fp64_cg (FP64 with the anchored CG SCF), mixed, "
+                         "mixed_refine (mixed with the FP64 refinement pass of the SCF)")
     ap.add_argument("--pmc-kernel", default=None, help="e.g. sk_forward_scf/bf16x3 (counter runs)")
     ap.add_argument("--pmc-iters", type=int, default=5)
     ap.add_argument("--pmc-summarize", nargs=2, default=None, metavar=("FETCH_DIR", "WRITE_DIR"))
@@ -440,9 +443,10 @@ def main():
     # the others are closed after their timed run (each form is timed alone)
     head = "mixed" if "mixed" in forms else forms[0]
     for f in forms:
-        prec = m.MIXED if f == "mixed" else m.FP64
+        prec = m.MIXED if f in ("mixed", "mixed_refine") else m.FP64
         e, dt, stats = timed_run(p, typ, A, r, prec, args.steps, args.warmup, fp64_cg=(f == "fp64_cg"),
-                                 list_method=args.list_method, check=not args.no_kernels)
+                                 list_method=args.list_method, check=not args.no_kernels,
+                                 refine=(f == "mixed_refine"))
         res[f] = dict(seconds=dt, steps_per_s=args.steps / dt, **stats)
         if f == head:
             eng[f] = e
@@ -452,20 +456,31 @@ def main():
             f"SCF FP64 evaluations/step {stats['scf_fp64_per_step']:.1f}, inner {stats['scf_inner_per_step']:.1f}")
     # accuracy of the mixed form against the FP64 form on one configuration
     acc = None
+    acc_refine = None
     if "fp64" in forms and "mixed" in forms and not args.no_kernels:
-        ef, em = m.Engine(p, typ, A, r, precision=m.FP64), m.Engine(p, typ, A, r, precision=m.MIXED)
-        tf, tm = ef.engforce(0), em.engforce(0)
-        Ff, Fm = ef.forces(), em.forces()
-        muf, mum = ef.dipoles(), em.dipoles()
-        acc = dict(
-            force_rms_rel=float(np.sqrt(np.mean((Fm - Ff) ** 2)) / np.sqrt(np.mean(Ff ** 2))),
-            dipole_rms_rel=float(np.linalg.norm(mum - muf) / np.linalg.norm(muf)),
-            energy_rel=float(abs(tm["u_tot"] - tf["u_tot"]) / abs(tf["u_tot"])),
-            stress_rel=float(np.linalg.norm(np.array(tm["tau_coul"]) - np.array(tf["tau_coul"]))
-                             / np.linalg.norm(np.array(tf["tau_coul"]))))
+        ef = m.Engine(p, typ, A, r, precision=m.FP64)
+        tf = ef.engforce(0)
+        Ff, muf = ef.forces(), ef.dipoles()
+
+        def accuracy(refine):
+            em = m.Engine(p, typ, A, r, precision=m.MIXED)
+            if refine:
+                em.set_scf_refine(True)
+            tm = em.engforce(0)
+            Fm, mum = em.forces(), em.dipoles()
+            em.close()
+            return dict(
+                force_rms_rel=float(np.sqrt(np.mean((Fm - Ff) ** 2)) / np.sqrt(np.mean(Ff ** 2))),
+                dipole_rms_rel=float(np.linalg.norm(mum - muf) / np.linalg.norm(muf)),
+                energy_rel=float(abs(tm["u_tot"] - tf["u_tot"]) / abs(tf["u_tot"])),
+                stress_rel=float(np.linalg.norm(np.array(tm["tau_coul"]) - np.array(tf["tau_coul"]))
+                                 / np.linalg.norm(np.array(tf["tau_coul"]))))
+        acc = accuracy(False)
         log(f"mixed vs fp64: {acc}")
+        if "mixed_refine" in forms:
+            acc_refine = accuracy(True)
+            log(f"mixed with FP64 refinement vs fp64: {acc_refine}")
         ef.close()
-        em.close()
     # kernels of the reciprocal-space path, timed live with HIP events on the
     # context stream, in every arithmetic variant the run uses; the roofline
     # entry is the variant with the largest share of the step time
@@ -565,7 +580,10 @@ def main():
         if ("fp64_cg" in res and "fp64" in res) else None,
         "mixed_over_fp64cg": (res["mixed"]["steps_per_s"] / res["fp64_cg"]["steps_per_s"])
         if ("mixed" in res and "fp64_cg" in res) else None,
+        "mixed_refine_over_fp64": (res["mixed_refine"]["steps_per_s"] / res["fp64"]["steps_per_s"])
+        if ("mixed_refine" in res and "fp64" in res) else None,
         "accuracy_mixed_vs_fp64": acc,
+        "accuracy_mixed_refine_vs_fp64": acc_refine,
         "kernels": kern,
         "neighbour_rebuild": rebuild,
         "roofline": roof,

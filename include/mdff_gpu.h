@@ -201,6 +201,16 @@ int mdff_list_info(mdff_ctx *ctx, int out[8]);
  * counterpart in the reference (measurement support). */
 int mdff_set_scf_check(mdff_ctx *ctx, int every);
 
+/* Mixed form with the anchored CG solver: on = 1 adds an FP64 refinement
+ * pass after the SCF converges on its carried residual.  The static field
+ * and the field of the converged dipoles are formed in FP64, the loop
+ * condition of momentpolaSCF (pim.c:236) is applied to that FP64 residual,
+ * and while it is above the tolerance low-precision conjugate-gradient steps
+ * carry the correction from it.  The thermo's n_scf counts both passes.
+ * Off (0) by default.  No counterpart in the reference (the reference's SCF
+ * is FP64 throughout). */
+int mdff_set_scf_refine(mdff_ctx *ctx, int on);
+
 /* Device timing of the last engforce by phase, in ms (HIP events on the
  * context stream): [0] short range, [1] static field, [2] SCF loop,
  * [3] final Ewald force, [4] neighbour list, [5] total. */

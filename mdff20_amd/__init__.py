@@ -88,6 +88,7 @@ def lib():
         "mdff_set_scf_inner": (C.c_int, [vp, C.c_int]),
         "mdff_set_scf_solver": (C.c_int, [vp, C.c_int, C.c_int]),
         "mdff_set_scf_check": (C.c_int, [vp, C.c_int]),
+        "mdff_set_scf_refine": (C.c_int, [vp, C.c_int]),
         "mdff_set_list_method": (C.c_int, [vp, C.c_int]),
         "mdff_list_info": (C.c_int, [vp, ip]),
     }
@@ -307,6 +308,11 @@ class Engine:
     def set_scf_check(self, every):
         """mixed form: the FP64 residual of the SCF's dipoles every `every`-th force call (thermo rmsd_fp64)"""
         _check(self._L.mdff_set_scf_check(self._h, every), "set_scf_check", False)
+
+    def set_scf_refine(self, on=True):
+        """mixed form, anchored CG: an FP64 refinement pass after the SCF (FP64 static field and anchor,
+        low-precision correction steps until the FP64-anchored residual meets the tolerance)"""
+        _check(self._L.mdff_set_scf_refine(self._h, 1 if on else 0), "set_scf_refine", False)
 
     def set_list_method(self, method):
         """0: cell-grid neighbour build where it pays (default), 1: all pairs, 2: cell grid wherever it fits"""

@@ -181,8 +181,10 @@ struct mdff_ctx {
     double t_ms[6] = {0, 0, 0, 0, 0, 0};
     int last_nscf = 0;
     int hint_fp64 = 0, hint_mixed = 0, hint_cg = 0, hint_ko = 0; // iterations of the last SCF of each form
+    int hint_ref = 0;         // correction steps of the last FP64 refinement
     double last_rmsd = 0;
     int scf_check = 0;        // mixed form: FP64 residual of the SCF every scf_check-th force call (0 never)
+    int scf_refine = 0;       // mixed form, anchored CG: FP64 refinement of the converged dipoles
     int chk_done = 0;         // the last force call evaluated it (S_RMSD_CHK holds the sum)
     int scf_chunk = 4;
     int inner_iters = 8;      // mixed form: low-precision correction iterations per FP64 refinement
@@ -1514,29 +1516,38 @@ static int scf_loop_mixed(mdff_ctx *c, int istep)
 // polarisable ions, the blocks of the CG kernels
 static_assert(CG_BLOCK == 256, "CG partial-sum blocks");
 
-static int scf_loop_anchored(mdff_ctx *c, int istep)
+static int scf_loop_anchored(mdff_ctx *c, int istep, bool refine = false)
 {
     int n = c->n, np = c->npol;
     int *ctl = c->ctl;
     const int *act = c->ctl + C_SCF_ACTIVE;
     int maxit = c->p.max_scf_pim_iter, minit = c->p.min_scf_pim_iter;
-    AspcArgs asp;
-    if (aspc_args(c, istep, &asp)) return MDFF_ERR_CONFIG;
-    // the predictor opens the SCF (k_scf_begin) when the loop runs at all; a
-    // split-FP16 anchor scales its weights by the predictor's channel maxima,
-    // which the predictor writes (all ions; zero where not polarisable)
-    const bool open = maxit > 0 || minit > 0;
-    if (!open) hipLaunchKernelGGL(k_scf_begin, dim3(1), dim3(1), 0, c->st, ctl, 0);
     // the FP64 form runs the same solver with FP64 anchor and steps
     const int lo = lo_of(c);
-    const int anchor = lo ? c->mf_anchor : 0;
-    const bool wmu = anchor == KM_F16X3;
-    hipLaunchKernelGGL(k_scf_mu, dim3(cdiv(n, 256)), dim3(256), 0, c->st, n, 1, asp, c->type, c->tab_d, c->hist, c->ef,
-                       c->mu_ind, c->upol_i, act, wmu ? c->gwp : nullptr, open ? ctl : nullptr);
+    const int anchor = (lo && !refine) ? c->mf_anchor : 0;
+    if (refine) {
+        // FP64 refinement: the converged dipoles are the start, the anchor
+        // is their FP64 field, and the loop may stop at once
+        minit = 0;
+        hipLaunchKernelGGL(k_scf_begin, dim3(1), dim3(1), 0, c->st, ctl, 1);
+    } else {
+        AspcArgs asp;
+        if (aspc_args(c, istep, &asp)) return MDFF_ERR_CONFIG;
+        // the predictor opens the SCF (k_scf_begin) when the loop runs at
+        // all; a split-FP16 anchor scales its weights by the predictor's
+        // channel maxima, which the predictor writes (all ions; zero where
+        // not polarisable)
+        const bool open = maxit > 0 || minit > 0;
+        if (!open) hipLaunchKernelGGL(k_scf_begin, dim3(1), dim3(1), 0, c->st, ctl, 0);
+        const bool wmu = anchor == KM_F16X3;
+        hipLaunchKernelGGL(k_scf_mu, dim3(cdiv(n, 256)), dim3(256), 0, c->st, n, 1, asp, c->type, c->tab_d, c->hist,
+                           c->ef, c->mu_ind, c->upol_i, act, wmu ? c->gwp : nullptr, open ? ctl : nullptr);
+    }
     // the anchor: the field of the predicted dipoles in FP64, or with
     // mf_anchor over the FP32 tables in that matrix-core mode
     if (anchor) {
         const int keep = c->mf_lo;
+        const bool wmu = anchor == KM_F16X3;
         c->mf_lo = anchor;
         c->wpart_ready = wmu;
         c->wpart_nb = wmu ? cdiv(n, 256) : 0;
@@ -1601,7 +1612,8 @@ static int scf_loop_anchored(mdff_ctx *c, int istep)
         cur ^= 1;
         launched++;
     };
-    int first = std::min(limit, 1 + std::max(1, c->hint_cg > 0 ? c->hint_cg : c->inner_iters));
+    const int hint = refine ? c->hint_ref : c->hint_cg;
+    int first = std::min(limit, 1 + std::max(1, hint > 0 ? hint : c->inner_iters));
     while (launched < first) step();
     for (;;) {
         int rc = sync_scal(c);
@@ -1618,14 +1630,39 @@ static int scf_loop_anchored(mdff_ctx *c, int istep)
     else c->n_scf_fp64 += 1;
     if (lo) c->n_scf_inner += std::max(0, c->last_nscf - 1);
     else c->n_scf_fp64 += std::max(0, c->last_nscf - 1);
-    c->hint_cg = std::max(1, c->last_nscf - 1);
+    if (refine) c->hint_ref = std::max(1, c->last_nscf - 1);
+    else c->hint_cg = std::max(1, c->last_nscf - 1);
     if (c->last_rmsd > c->p.conv_tol_ind) return MDFF_ERR_SCF;
     return MDFF_OK;
 }
 
+// FP64 refinement of the mixed anchored CG's dipoles: the static field in
+// FP64 (FP64 tables of all ions; the FP64 pair tensors were built with the
+// FP32 ones), the FP64 field of the converged dipoles as a new anchor, and
+// the loop condition of pim.c:236 on that FP64 residual; while it is above
+// the tolerance, conjugate-gradient steps in the low precision carry the
+// correction (each applies T to the search direction only, whose size is of
+// the order of the residual, so their rounding stays far below it).  The
+// reported iterations are those of both passes; the residual is the FP64-
+// anchored one.
+static int scf_refine_fp64(mdff_ctx *c, int istep)
+{
+    const int first = c->last_nscf;
+    launch_eikr(c, 0, 0, nullptr, true);
+    static_field(c, 0);
+    int rc = scf_loop_anchored(c, istep, true);
+    c->last_nscf += first;
+    return rc;
+}
+
 static int scf_mixed(mdff_ctx *c, int istep)
 {
-    if (c->scf_solver >= 2 && c->cg_ok && c->npol > 0) return scf_loop_anchored(c, istep);
+    if (c->scf_solver >= 2 && c->cg_ok && c->npol > 0) {
+        int rc = scf_loop_anchored(c, istep);
+        const bool run = c->p.max_scf_pim_iter > 0 || c->p.min_scf_pim_iter > 0;
+        if (c->scf_refine && run && (rc == MDFF_OK || rc == MDFF_ERR_SCF)) rc = scf_refine_fp64(c, istep);
+        return rc;
+    }
     return scf_loop_mixed(c, istep);
 }
 
@@ -1786,7 +1823,7 @@ static int engforce_impl(mdff_ctx *c, int istep)
         if (pol && lo && scf_uses_fp64(c)) launch_eikr(c, 1, 0, nullptr);
         // get_dipoles coulombic.c:67-130
         if (c->lpol) {
-            if ((rc = build_t2(c)) != MDFF_OK) return rc;
+            if ((rc = build_t2(c, lo && c->scf_refine)) != MDFF_OK) return rc;
             static_field(c, lo);
             if (c->have_events) HIPCHK(hipEventRecord(c->ev[2], c->st));
             rc = c->p.algo_pim == MDFF_PIM_SCFKO ? scf_loop_ko(c, istep)
@@ -3026,6 +3063,14 @@ int mdff_set_scf_check(mdff_ctx *c, int every)
 {
     if (!c || every < 0) return MDFF_ERR_ARG;
     c->scf_check = every;
+    return MDFF_OK;
+}
+
+int mdff_set_scf_refine(mdff_ctx *c, int on)
+{
+    if (!c || on < 0 || on > 1) return MDFF_ERR_ARG;
+    c->scf_refine = on;
+    c->hint_ref = 0;
     return MDFF_OK;
 }
 

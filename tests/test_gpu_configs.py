@@ -226,6 +226,37 @@ def test_cfg3_mixed_force_call_vs_oracle(cfg3):
     assert rel(res[0][1], res[1][1]) < 1e-6
 
 
+def test_cfg3_mixed_fp64_refinement(cfg3):
+    """the mixed form with the FP64 refinement pass: the FP64 residual of
+    the final dipoles (evaluated again, entirely in FP64) meets the SCF
+    tolerance at the force call and over 20 NVT steps, and the dipoles come
+    closer to the oracle's than without the pass"""
+    p, typ, A, r, ref = cfg3
+    tol = p.conv_tol_ind
+    with m.Engine(p, typ, A, r, precision=m.MIXED) as e0:
+        e0.engforce(0)
+        base = rel(e0.dipoles(), ref["mu"])
+    with m.Engine(p, typ, A, r, precision=m.MIXED) as e:
+        e.set_scf_refine(True)
+        e.set_scf_check(1)
+        t = e.engforce(0)
+        err = _err_vs(e, _Ref(ref), t, ref["t"])
+        say(f"cfg3 mixed + FP64 refinement vs oracle: {err}, dipoles without the pass {base:.2e}, "
+            f"n_scf {t['n_scf']}, rmsd {t['rmsd']:.2e}, FP64 rmsd {t['rmsd_fp64']:.2e}")
+        for k, tl in MIXED_TOL.items():
+            assert err[k] <= tl, (k, err[k])
+        assert err["dipoles"] < base
+        assert 0.0 <= t["rmsd_fp64"] <= 1.05 * tol
+        e.init_velocities()
+        e.md_init()
+        worst = 0.0
+        for s in range(1, 21):
+            t = e.md_step(s)
+            worst = max(worst, t["rmsd_fp64"])
+            assert 0.0 <= t["rmsd_fp64"] <= 1.05 * tol, (s, t["rmsd_fp64"], t["rmsd"])
+        say(f"cfg3 mixed + FP64 refinement, 20 NVT steps: worst FP64 rmsd {worst:.2e}")
+
+
 def test_cfg3_mixed_1000_nvt_steps_vs_fp64(cfg3):
     """1000 mixed NVT steps; every 100 steps the FP64 form re-evaluates the
     mixed run's positions: force RMS <= 1e-5 relative at every sample, and
