@@ -77,6 +77,15 @@ void prop_agate(int step);
  * history stays on the device (dipia_ind[.][0] holds the last step's). */
 void run_md(void);
 
+/* get_gr radial_distribution.h:22, radial_distribution.c:180-231: the
+ * histogram of the current frame (rx/ry/rz, typia, simuCell) on the device
+ * (mdff_rdf.h), added into gr as the reference adds it (ordered pairs per
+ * type pair, and every pair once more into gr[.][0][0]); rx/ry/rz are left
+ * as its kardir/dirkar round trip leaves them.  nbins must hold every
+ * distance below cutgr at resg (the reference writes past gr otherwise);
+ * else an error message and exit(-1). */
+void get_gr(void);
+
 /* invertmatrix3x3 inverse_matrix.h, inverse_matrix.c:6-23 (LAPACK there):
  * the closed-form 3x3 inverse in place; init_config calls it for every
  * polarisable ion */

@@ -19,7 +19,7 @@ _lib = None
 
 __all__ = ["lib", "MdffError", "read_control", "read_posff", "Engine", "Params", "Thermo",
            "FP64", "MIXED", "KS_BF16X3", "KS_F32", "KS_F16X3", "ES", "NVE_LF", "NVE_VV", "NVT_NHCN", "PIM_SCF", "PIM_SCFKO", "PIM_PCG",
-           "device_count", "tt_damping", "init_velocities_host"]
+           "device_count", "tt_damping", "init_velocities_host", "Rdf", "rdf_defaults"]
 
 
 class MdffError(RuntimeError):
@@ -89,6 +89,19 @@ def lib():
         "mdff_set_scf_solver": (C.c_int, [vp, C.c_int, C.c_int]),
         "mdff_set_scf_check": (C.c_int, [vp, C.c_int]),
         "mdff_set_scf_refine": (C.c_int, [vp, C.c_int]),
+        "mdff_set_trajectory": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, C.c_int]),
+        "mdff_traj_info": (C.c_int, [vp, ip]),
+        "mdff_traj_frame": (C.c_int, [vp, C.c_int, ip, P(dp), P(dp), P(dp)]),
+        "mdff_traj_clear": (C.c_int, [vp]),
+        "mdff_rdf_create": (vp, [C.c_int, C.c_int, ip, C.c_int, C.c_double, C.c_double, ip]),
+        "mdff_rdf_destroy": (None, [vp]),
+        "mdff_rdf_defaults": (C.c_int, [dp, dp, ip]),
+        "mdff_rdf_reset": (C.c_int, [vp]),
+        "mdff_rdf_add_frame": (C.c_int, [vp, dp, dp, dp, dp, C.c_int]),
+        "mdff_rdf_sample": (C.c_int, [vp, vp]),
+        "mdff_rdf_sample_trajectory": (C.c_int, [vp, vp]),
+        "mdff_rdf_counts": (C.c_int, [vp, P(C.c_longlong), P(C.c_longlong)]),
+        "mdff_rdf_gr": (C.c_int, [vp, P(C.c_longlong), C.c_longlong, C.c_double, dp, dp]),
         "mdff_set_list_method": (C.c_int, [vp, C.c_int]),
         "mdff_list_info": (C.c_int, [vp, ip]),
     }
@@ -314,6 +327,28 @@ class Engine:
         low-precision correction steps until the FP64-anchored residual meets the tolerance)"""
         _check(self._L.mdff_set_scf_refine(self._h, 1 if on else 0), "set_scf_refine", False)
 
+    def set_trajectory(self, tprint, tstart=0, npas=0, capacity=16):
+        """device trajectory: r, v, f copied in device memory after every TRAJFF step of run_md
+        (istep % tprint == 0 or istep == npas, istep >= tstart), up to `capacity` frames"""
+        _check(self._L.mdff_set_trajectory(self._h, tprint, tstart, npas, capacity), "set_trajectory", False)
+
+    def traj_info(self):
+        o = (C.c_int * 3)()
+        _check(self._L.mdff_traj_info(self._h, o), "traj_info", False)
+        return dict(frames=o[0], capacity=o[1], dropped=o[2])
+
+    def traj_frame(self, k):
+        """(istep, r, v, f) of frame k, each (3, n)"""
+        n = self.n
+        out = np.zeros((3, 3, n))
+        ptrs = [(C.POINTER(C.c_double) * 3)(*[_dp(out[q, a]) for a in range(3)]) for q in range(3)]
+        st = C.c_int()
+        _check(self._L.mdff_traj_frame(self._h, k, C.byref(st), *ptrs), "traj_frame", False)
+        return st.value, out[0], out[1], out[2]
+
+    def traj_clear(self):
+        _check(self._L.mdff_traj_clear(self._h), "traj_clear", False)
+
     def set_list_method(self, method):
         """0: cell-grid neighbour build where it pays (default), 1: all pairs, 2: cell grid wherever it fits"""
         _check(self._L.mdff_set_list_method(self._h, method), "set_list_method", False)
@@ -414,3 +449,82 @@ class Engine:
 
 
 from . import batch  # noqa: E402  (batched engine and adjoint, include/mdff_batch.h)
+
+
+def rdf_defaults(A):
+    """cutgr and nbins of default_rdf (radial_distribution.c:37-42)"""
+    A = np.ascontiguousarray(np.asarray(A, dtype=np.float64).reshape(9))
+    cut, nb = C.c_double(), C.c_int()
+    _check(lib().mdff_rdf_defaults(_dp(A), C.byref(cut), C.byref(nb)), "rdf_defaults", False)
+    return cut.value, nb.value
+
+
+class Rdf:
+    """radial distribution histogram on the device (mdff_rdf.h; get_gr
+    radial_distribution.c:180-231)"""
+
+    def __init__(self, typia, ntype, nbins, resg, cutgr):
+        L = lib()
+        self._L = L
+        typia = np.ascontiguousarray(typia, dtype=np.int32)
+        self.n, self.ntype, self.nbins, self.resg, self.cutgr = typia.size, ntype, nbins, resg, cutgr
+        st = C.c_int()
+        h = L.mdff_rdf_create(self.n, ntype, typia.ctypes.data_as(C.POINTER(C.c_int)), nbins, resg, cutgr,
+                              C.byref(st))
+        if not h:
+            raise MdffError(st.value, "rdf_create")
+        self._h = C.c_void_p(h)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._L.mdff_rdf_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def add_frame(self, A, r, write_back=False):
+        """one host frame; returns the positions get_gr leaves (kardir + dirkar) when write_back"""
+        A = np.ascontiguousarray(np.asarray(A, dtype=np.float64).reshape(9))
+        r = np.asarray(r, dtype=np.float64)
+        if r.shape != (3, self.n):
+            raise ValueError(f"positions must be (3, {self.n}), got {r.shape}")
+        rx, ry, rz = (np.ascontiguousarray(r[k]).copy() for k in range(3))
+        _check(self._L.mdff_rdf_add_frame(self._h, _dp(A), _dp(rx), _dp(ry), _dp(rz), 1 if write_back else 0),
+               "rdf_add_frame", False)
+        return np.stack([rx, ry, rz]) if write_back else None
+
+    def sample(self, engine):
+        _check(self._L.mdff_rdf_sample(self._h, engine._h), "rdf_sample", False)
+
+    def sample_trajectory(self, engine):
+        _check(self._L.mdff_rdf_sample_trajectory(self._h, engine._h), "rdf_sample_trajectory", False)
+
+    def reset(self):
+        _check(self._L.mdff_rdf_reset(self._h), "rdf_reset", False)
+
+    def counts(self):
+        """ordered pair counts [nbins][ntype][ntype] and (frames, pairs out of range)"""
+        c = np.zeros((self.nbins, self.ntype, self.ntype), dtype=np.int64)
+        info = (C.c_longlong * 2)()
+        _check(self._L.mdff_rdf_counts(self._h, c.ctypes.data_as(C.POINTER(C.c_longlong)), info), "rdf_counts",
+               False)
+        return c, (info[0], info[1])
+
+    def gr(self, counts, nconf, omega):
+        """write_rdf's normalisation: r[nbins], g[nbins][ntype][ntype]"""
+        counts = np.ascontiguousarray(counts, dtype=np.int64)
+        r = np.zeros(self.nbins)
+        g = np.zeros((self.nbins, self.ntype, self.ntype))
+        _check(self._L.mdff_rdf_gr(self._h, counts.ctypes.data_as(C.POINTER(C.c_longlong)), nconf, omega, _dp(r),
+                                   _dp(g)), "rdf_gr", False)
+        return r, g

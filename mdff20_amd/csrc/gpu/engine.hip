@@ -20,6 +20,8 @@
 #include "typetab.h"
 #include "mdff_gpu.h"
 #include "mdff_host.h"
+#include "mdff_rdf.h"
+#include "engine_view.h"
 
 #define HIPCHK(x)                                                                                 \
     do {                                                                                          \
@@ -185,6 +187,11 @@ struct mdff_ctx {
     double last_rmsd = 0;
     int scf_check = 0;        // mixed form: FP64 residual of the SCF every scf_check-th force call (0 never)
     int scf_refine = 0;       // mixed form, anchored CG: FP64 refinement of the converged dipoles
+    // device trajectory: frames of r, v, f (9 n doubles each, SoA) at the
+    // TRAJFF cadence of run_md (md.c:275-277, iopsnode io.c:67-69)
+    int traj_every = 0, traj_start = 0, traj_npas = 0, traj_cap = 0, traj_n = 0, traj_drop = 0;
+    double *traj = nullptr;
+    std::vector<int> traj_step;
     int chk_done = 0;         // the last force call evaluated it (S_RMSD_CHK holds the sum)
     int scf_chunk = 4;
     int inner_iters = 8;      // mixed form: low-precision correction iterations per FP64 refinement
@@ -2025,6 +2032,22 @@ static int prop_step(mdff_ctx *c, int istep)
     }
 }
 
+// a frame of the device trajectory: copies on the stream, no host wait
+static int traj_append(mdff_ctx *c, int istep)
+{
+    if (c->traj_n >= c->traj_cap) {
+        c->traj_drop++;
+        return MDFF_OK;
+    }
+    const size_t n3 = (size_t)3 * c->n;
+    double *fr = c->traj + (size_t)c->traj_n * 3 * n3;
+    HIPCHK(hipMemcpyAsync(fr, c->r, sizeof(double) * n3, hipMemcpyDeviceToDevice, c->st));
+    HIPCHK(hipMemcpyAsync(fr + n3, c->v, sizeof(double) * n3, hipMemcpyDeviceToDevice, c->st));
+    HIPCHK(hipMemcpyAsync(fr + 2 * n3, c->f, sizeof(double) * n3, hipMemcpyDeviceToDevice, c->st));
+    c->traj_step[c->traj_n++] = istep;
+    return MDFF_OK;
+}
+
 static int md_step_impl(mdff_ctx *c, int istep)
 {
     int rc = prop_step(c, istep);
@@ -2036,6 +2059,11 @@ static int md_step_impl(mdff_ctx *c, int istep)
     if (c->rescale_allowed && istep < c->p.nequil && c->p.nequilT > 0 && istep % c->p.nequilT == 0) {
         rescale_dev(c);
         c->kin_fresh = 0;
+    }
+    // the state run_md writes to TRAJFF after this step (md.c:275-277)
+    if (c->traj_every > 0 && (istep % c->traj_every == 0 || istep == c->traj_npas) && istep >= c->traj_start) {
+        int r3 = traj_append(c, istep);
+        if (r3) return r3;
     }
     return rc;
 }
@@ -2078,6 +2106,7 @@ void mdff_destroy(mdff_ctx *c)
                      &c->kvec, &c->Ak, &c->kcoe, &c->W, &c->ek, &c->kpart, &c->bpart, &c->ksum, &c->gsc, &c->gwp, &c->gwq};
     for (auto pp : dp) dfree(*pp);
     free_cells(c);
+    dfree(c->traj);
     dfree(c->type);
     dfree(c->vc_cnt);
     dfree(c->chk_cnt);
@@ -2531,6 +2560,82 @@ static int get_aos3(mdff_ctx *c, const double *src, double *out)
 }
 
 int mdff_get_dipoles(mdff_ctx *c, double *mu) { return (c && mu) ? get_aos3(c, c->mu_tot, mu) : MDFF_ERR_ARG; }
+
+int mdff_set_trajectory(mdff_ctx *c, int tprint, int tstart, int npas, int capacity)
+{
+    if (!c || tprint < 0 || capacity < 0 || (tprint > 0 && capacity < 1)) return MDFF_ERR_ARG;
+    HIPCHK(hipStreamSynchronize(c->st));
+    dfree(c->traj);
+    c->traj_every = 0;
+    c->traj_cap = c->traj_n = c->traj_drop = 0;
+    c->traj_step.clear();
+    if (tprint == 0) return MDFF_OK;
+    if (hipMalloc((void **)&c->traj, sizeof(double) * 9 * (size_t)c->n * (size_t)capacity) != hipSuccess) {
+        c->traj = nullptr;
+        return MDFF_ERR_NOMEM;
+    }
+    c->traj_step.assign(capacity, 0);
+    c->traj_every = tprint;
+    c->traj_start = tstart;
+    c->traj_npas = npas;
+    c->traj_cap = capacity;
+    return MDFF_OK;
+}
+
+int mdff_traj_info(mdff_ctx *c, int out[3])
+{
+    if (!c || !out) return MDFF_ERR_ARG;
+    out[0] = c->traj_n;
+    out[1] = c->traj_cap;
+    out[2] = c->traj_drop;
+    return MDFF_OK;
+}
+
+int mdff_traj_frame(mdff_ctx *c, int k, int *istep, double *r[3], double *v[3], double *f[3])
+{
+    if (!c || k < 0 || k >= c->traj_n) return MDFF_ERR_ARG;
+    const size_t n = (size_t)c->n;
+    const double *fr = c->traj + (size_t)k * 9 * n;
+    double **dst[3] = {r, v, f};
+    for (int q = 0; q < 3; q++)
+        if (dst[q])
+            for (int a = 0; a < 3; a++)
+                if (dst[q][a])
+                    HIPCHK(hipMemcpyAsync(dst[q][a], fr + (3 * q + a) * n, sizeof(double) * n, hipMemcpyDeviceToHost,
+                                          c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    if (istep) *istep = c->traj_step[k];
+    return MDFF_OK;
+}
+
+int mdff_traj_clear(mdff_ctx *c)
+{
+    if (!c) return MDFF_ERR_ARG;
+    HIPCHK(hipStreamSynchronize(c->st));
+    c->traj_n = c->traj_drop = 0;
+    return MDFF_OK;
+}
+
+// the state the RDF unit reads: positions, cell, the trajectory's frames
+int mdff_engine_view(mdff_ctx *c, EngineView *v)
+{
+    if (!c || !v) return MDFF_ERR_ARG;
+    v->n = c->n;
+    v->device = c->device;
+    v->st = c->st;
+    v->r = c->r;
+    v->type = c->type;
+    v->ntype = c->ntype;
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) {
+            v->A[i][j] = c->cd.A[i][j];
+            v->B[i][j] = c->cd.B[i][j];
+        }
+    v->frames = c->traj;
+    v->nframes = c->traj_n;
+    v->frame_doubles = 9 * (size_t)c->n;
+    return MDFF_OK;
+}
 int mdff_get_field(mdff_ctx *c, double *ef) { return (c && ef) ? get_aos3(c, c->ef, ef) : MDFF_ERR_ARG; }
 
 int mdff_get_thermo(mdff_ctx *c, mdff_thermo *th)

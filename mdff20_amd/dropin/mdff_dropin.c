@@ -32,6 +32,7 @@
 #include "md.h"
 #include "nmlj.h"
 #include "pim.h"
+#include "radial_distribution.h"
 #include "thermo.h"
 #include "timing.h"
 #include "verlet.h"
@@ -39,6 +40,7 @@
 #include "mdff_dropin.h"
 #include "mdff_gpu.h"
 #include "mdff_host.h"
+#include "mdff_rdf.h"
 
 static mdff_ctx *g_ctx = NULL;
 static int g_nion = 0;
@@ -159,15 +161,23 @@ int mdff_dropin_params(mdff_params *p)
     return MDFF_OK;
 }
 
+/* the histogram handle of get_gr and the sizes it was made for */
+static mdff_rdf *g_rdf = NULL;
+static int g_rdf_n = 0, g_rdf_nt = 0, g_rdf_nb = 0;
+static double g_rdf_resg = 0.0, g_rdf_cut = 0.0;
+static int *g_rdf_type = NULL;
+
 void mdff_dropin_release(void)
 {
-    if (g_ctx) {
-        char *rs = rng_enter();
-        mdff_destroy(g_ctx);
-        rng_leave(rs);
-    }
+    char *rs = rng_enter();
+    if (g_ctx) mdff_destroy(g_ctx);
+    if (g_rdf) mdff_rdf_destroy(g_rdf);
+    rng_leave(rs);
     g_ctx = NULL;
     g_nion = 0;
+    g_rdf = NULL;
+    free(g_rdf_type);
+    g_rdf_type = NULL;
 }
 
 /* the device context for the current system; positions uploaded */
@@ -325,11 +335,28 @@ static void config_out(const char *file, const char *mode)
     }
 }
 
-/* whether run_md writes anything after step s (md.c:250-277) */
+/* whether run_md writes stdout, OSZIFF or CONTFF after step s
+ * (md.c:250-271) */
 static int output_step(int s)
 {
-    return iopnode(s, npas, nprint) || iopnode(s, npas, fprint) || iopnode(s, npas, cprint) ||
-           iopsnode(s, npas, tprint, tstart);
+    return iopnode(s, npas, nprint) || iopnode(s, npas, fprint) || iopnode(s, npas, cprint);
+}
+
+/* the TRAJFF records held in device memory, written in step order through
+ * the globals (write_config's format), then forgotten; the globals are
+ * left holding the last frame */
+static void traj_flush(mdff_ctx *c)
+{
+    int info[3];
+    int rc = mdff_traj_info(c, info);
+    if (rc) fail("mdff_traj_info", rc);
+    if (info[2] > 0) fail("device trajectory (frames dropped)", MDFF_ERR_ARG);
+    double *r[3] = {rx, ry, rz}, *v[3] = {vx, vy, vz}, *f[3] = {fx, fy, fz};
+    for (int k = 0; k < info[0]; k++) {
+        if ((rc = mdff_traj_frame(c, k, NULL, r, v, f))) fail("mdff_traj_frame", rc);
+        config_out("TRAJFF", "a");
+    }
+    if ((rc = mdff_traj_clear(c))) fail("mdff_traj_clear", rc);
 }
 
 static void run_md_impl(void)
@@ -364,15 +391,31 @@ static void run_md_impl(void)
         pError("opening OSZIFF file");
         exit(-1);
     }
+    /* TRAJFF records (md.c:275-277) stay in device memory (at most 1 GiB of
+     * frames) until the buffer is full or the run ends */
+    int tcap = 0, tpend = 0;
+    if (tprint > 0) {
+        int nt = 0;
+        for (int s = 1; s <= npas; s++) nt += iopsnode(s, npas, tprint, tstart) ? 1 : 0;
+        const long long fit = (1LL << 30) / (72LL * nion);
+        tcap = nt < fit ? nt : (int)(fit > 0 ? fit : 1);
+        if (tcap > 0 && (rc = mdff_set_trajectory(c, tprint, tstart, npas, tcap))) fail("mdff_set_trajectory", rc);
+    }
     /* the loop of md.c:228-279 on the device: the steps between two outputs
-     * run as one mdff_md_run (prop_agate, Verlet check, rescale), and the
-     * state comes to the host only where something is written */
+     * run as one mdff_md_run (prop_agate, Verlet check, rescale, the TRAJFF
+     * frame copy), and the state comes to the host only where something is
+     * written */
     statime(0);
     for (int first = 1; first <= npas;) {
-        int last = first;
-        while (last < npas && !output_step(last)) last++;
+        int last = first, pend = tpend;
+        for (;;) {
+            pend += (tcap > 0 && iopsnode(last, npas, tprint, tstart)) ? 1 : 0;
+            if (last >= npas || output_step(last) || (tcap > 0 && pend == tcap)) break;
+            last++;
+        }
         rc = mdff_md_run(c, first, last - first + 1, &th);
         if (rc && rc != MDFF_ERR_SCF) fail("mdff_md_run", rc);
+        tpend = pend;
         istep = last;
         step_globals(c, &th, 0);
         if (iopnode(istep, npas, nprint)) {
@@ -383,11 +426,19 @@ static void run_md_impl(void)
             info_thermo(0, NULL);
         }
         if (iopnode(istep, npas, fprint)) info_thermo(1, fpOSZIFF);
-        const int wc = iopnode(istep, npas, cprint), wt = iopsnode(istep, npas, tprint, tstart);
-        if (wc || wt) state_down(c);
-        if (wc) config_out("CONTFF", "w");
-        if (wt) config_out("TRAJFF", "a");
+        if (iopnode(istep, npas, cprint)) {
+            state_down(c);
+            config_out("CONTFF", "w");
+        }
+        if (tcap > 0 && tpend == tcap) {
+            traj_flush(c);
+            tpend = 0;
+        }
         first = last + 1;
+    }
+    if (tcap > 0) {
+        traj_flush(c);
+        if ((rc = mdff_set_trajectory(c, 0, 0, 0, 0))) fail("mdff_set_trajectory", rc);
     }
     state_down(c);
     config_out("CONTFF", "w");
@@ -542,6 +593,66 @@ void momentpolaSCF(double (*mu)[3], double *upol)
 {
     char *rs = rng_enter();
     momentpolaSCF_impl(mu, upol);
+    rng_leave(rs);
+}
+
+/* get_gr radial_distribution.c:180-231: one frame of the globals on the
+ * device; gr gains the ordered pair counts per (bin, type, type) and
+ * gr[.][0][0] also every pair (:219-220); rx/ry/rz come back as kardir and
+ * dirkar leave them (:194, :230) */
+static void get_gr_impl(void)
+{
+    int same = g_rdf && g_rdf_n == nion && g_rdf_nt == ntype && g_rdf_nb == nbins && g_rdf_resg == resg &&
+               g_rdf_cut == cutgr && memcmp(g_rdf_type, typia, sizeof(int) * nion) == 0;
+    if (!same) {
+        if (g_rdf) mdff_rdf_destroy(g_rdf);
+        g_rdf = NULL;
+        free(g_rdf_type);
+        g_rdf_type = malloc(sizeof(int) * nion);
+        if (!g_rdf_type) fail("get_gr (host memory)", MDFF_ERR_NOMEM);
+        memcpy(g_rdf_type, typia, sizeof(int) * nion);
+        int st = 0;
+        g_rdf = mdff_rdf_create(nion, ntype, typia, nbins, resg, cutgr, &st);
+        if (!g_rdf) {
+            /* nbins of default_rdf (resg 0.1) does not hold the distances
+             * below cutgr at a smaller resg; the reference writes past gr */
+            fprintf(stderr, "mdff drop-in: get_gr: %d bins of %g A do not reach cutgr %g A\n", nbins, resg, cutgr);
+            fail("mdff_rdf_create", st);
+        }
+        g_rdf_n = nion;
+        g_rdf_nt = ntype;
+        g_rdf_nb = nbins;
+        g_rdf_resg = resg;
+        g_rdf_cut = cutgr;
+        if (!g_atexit) {
+            atexit(mdff_dropin_release);
+            g_atexit = 1;
+        }
+    }
+    int rc = mdff_rdf_reset(g_rdf);
+    if (rc) fail("mdff_rdf_reset", rc);
+    if ((rc = mdff_rdf_add_frame(g_rdf, simuCell.A, rx, ry, rz, 1))) fail("mdff_rdf_add_frame", rc);
+    const size_t nc = (size_t)nbins * ntype * ntype;
+    long long *cnt = malloc(sizeof(long long) * nc);
+    if (!cnt) fail("get_gr (host memory)", MDFF_ERR_NOMEM);
+    if ((rc = mdff_rdf_counts(g_rdf, cnt, NULL))) fail("mdff_rdf_counts", rc);
+    for (int b = 0; b < nbins; b++) {
+        long long all = 0;
+        for (int i = 0; i < ntype; i++)
+            for (int j = 0; j < ntype; j++) {
+                const long long v = cnt[((size_t)b * ntype + i) * ntype + j];
+                gr[b][i][j] += (int)v;
+                all += v;
+            }
+        gr[b][0][0] += (int)all;
+    }
+    free(cnt);
+}
+
+void get_gr(void)
+{
+    char *rs = rng_enter();
+    get_gr_impl();
     rng_leave(rs);
 }
 

@@ -1571,3 +1571,80 @@ void orc_get_ewald(const orc_sys *s, double *alpha, int *kES, int *nk)
     if (kES) memcpy(kES, s->kES, sizeof(int) * 3);
     if (nk) *nk = s->nk;
 }
+
+/* ------------------------------------------------------- radial distribution */
+/* default_rdf radial_distribution.c:37-42 (resg 0.1; the widths of lattice
+ * cell.c:43-47) */
+void orc_rdf_defaults(const double *A9, double *cutgr, int *nbins)
+{
+    cell_t c;
+    cell_init(&c, A9, 1.0);
+    double w = c.w[0];
+    if (c.w[1] < w) w = c.w[1];
+    if (c.w[2] < w) w = c.w[2];
+    *cutgr = 0.5 * w;
+    *nbins = (int)(*cutgr / 0.1) + 1;
+}
+
+/* get_gr radial_distribution.c:180-231: kardir of every ion
+ * (cell.c:125-143), then for every ordered pair the fractional difference,
+ * its minimum image in cartesian (pbc.c:7-14), and within cutgr the bin
+ * (int)(r / resg); dirkar (cell.c:146-161) at the end */
+long long orc_rdf_frame(int nion, int ntype, const int *typia, const double *A9, double *rx, double *ry,
+                        double *rz, double cutgr, double resg, int nbins, long long *counts)
+{
+    cell_t c;
+    cell_init(&c, A9, 1.0);
+    const double cut2 = cutgr * cutgr;
+    for (int i = 0; i < nion; i++) {
+        double s[3];
+        frac(&c, rx[i], ry[i], rz[i], s);
+        rx[i] = s[0];
+        ry[i] = s[1];
+        rz[i] = s[2];
+    }
+    long long out = 0;
+    for (int ia = 0; ia < nion; ia++) {
+        const int ita = typia[ia];
+        for (int ja = 0; ja < nion; ja++) {
+            if (ja == ia) continue;
+            double d[3];
+            min_image(&c, rx[ia] - rx[ja], ry[ia] - ry[ja], rz[ia] - rz[ja], d);
+            const double rijsq = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+            if (rijsq < cut2) {
+                const int igr = (int)(sqrt(rijsq) / resg);
+                if (igr < 0 || igr >= nbins) {
+                    out++;
+                    continue;
+                }
+                counts[((size_t)igr * ntype + ita) * ntype + typia[ja]]++;
+            }
+        }
+    }
+    for (int i = 0; i < nion; i++) {
+        const double x = rx[i], y = ry[i], z = rz[i];
+        rx[i] = x * c.A[0][0] + y * c.A[1][0] + z * c.A[2][0];
+        ry[i] = x * c.A[0][1] + y * c.A[1][1] + z * c.A[2][1];
+        rz[i] = x * c.A[0][2] + y * c.A[1][2] + z * c.A[2][2];
+    }
+    return out;
+}
+
+/* write_rdf radial_distribution.c:120-141: the shell volume
+ * 4 pi (resg r^2 + resg^3 / 12) / Omega at the bin centre and the count
+ * divided by nconf vol n_it1 n_it2, in the reference's order of operations */
+void orc_rdf_normalise(int nbins, int ntype, const long long *gr, int nconf, double resg, double omega,
+                       const int *nionit, double *r, double *g)
+{
+    for (int igr = 0; igr < nbins; igr++) {
+        const double rr = ((double)igr + 0.5) * resg;
+        double vol = 4 * PI * (resg * rr * rr + (pow(resg, 3)) / 12);
+        vol = vol / omega;
+        r[igr] = rr;
+        for (int a = 0; a < ntype; a++)
+            for (int b = 0; b < ntype; b++) {
+                const size_t k = ((size_t)igr * ntype + a) * ntype + b;
+                g[k] = ((double)gr[k]) / (double)(nconf * vol * nionit[a] * nionit[b]);
+            }
+    }
+}

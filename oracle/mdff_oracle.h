@@ -67,6 +67,21 @@ int orc_aspc_predict1(int istep, int order, double *hist, double *mu, double *W)
 int orc_md_init(orc_sys *s);
 int orc_md_step(orc_sys *s, int istep);
 
+/* radial distribution (radial_distribution.c).  orc_rdf_defaults: cutgr and
+ * nbins of default_rdf (:37-42) for cell A9 (rows a, b, c).  orc_rdf_frame:
+ * get_gr (:180-231) over one frame, ordered pairs ia != ja, counts ADDED to
+ * counts[nbins][ntype][ntype] (without the reference's extra all-pair
+ * increment of gr[.][0][0], :219); the positions are left as get_gr leaves
+ * them (kardir then dirkar in place); returns the pairs whose bin fell
+ * outside [0, nbins) (not counted).  orc_rdf_normalise: the normalisation of
+ * write_rdf (:120-141) of counts gr[nbins][ntype][ntype] over nconf frames:
+ * r[igr] and g[igr][it1][it2]. */
+void orc_rdf_defaults(const double *A9, double *cutgr, int *nbins);
+long long orc_rdf_frame(int nion, int ntype, const int *typia, const double *A9, double *rx, double *ry,
+                        double *rz, double cutgr, double resg, int nbins, long long *counts);
+void orc_rdf_normalise(int nbins, int ntype, const long long *gr, int nconf, double resg, double omega,
+                       const int *nionit, double *r, double *g);
+
 #ifdef __cplusplus
 }
 #endif

@@ -47,6 +47,11 @@ def load():
         L.orc_md_init.restype = C.c_int
         L.orc_md_step.argtypes = [vp, C.c_int]
         L.orc_md_step.restype = C.c_int
+        llp = C.POINTER(C.c_longlong)
+        L.orc_rdf_defaults.argtypes = [dp, dp, ip]
+        L.orc_rdf_frame.argtypes = [C.c_int, C.c_int, ip, dp, dp, dp, dp, C.c_double, C.c_double, C.c_int, llp]
+        L.orc_rdf_frame.restype = C.c_longlong
+        L.orc_rdf_normalise.argtypes = [C.c_int, C.c_int, llp, C.c_int, C.c_double, C.c_double, ip, dp, dp]
         _lib = L
     return _lib
 
@@ -158,3 +163,43 @@ class Oracle:
         d = self.thermo()
         d["status"] = rc
         return d
+
+
+def rdf_defaults(A):
+    """cutgr, nbins of default_rdf (radial_distribution.c:37-42)"""
+    L = load()
+    A = np.ascontiguousarray(np.asarray(A, dtype=np.float64).reshape(9))
+    cut, nb = C.c_double(), C.c_int()
+    L.orc_rdf_defaults(_dp(A), C.byref(cut), C.byref(nb))
+    return cut.value, nb.value
+
+
+def rdf_frame(typia, ntype, A, r, cutgr, resg, nbins, counts=None):
+    """get_gr over one frame: counts[nbins][ntype][ntype] (added to), the
+    positions get_gr leaves, and the pairs out of the bin range"""
+    L = load()
+    typia = np.ascontiguousarray(typia, dtype=np.int32)
+    n = typia.size
+    if counts is None:
+        counts = np.zeros((nbins, ntype, ntype), dtype=np.int64)
+    A = np.ascontiguousarray(np.asarray(A, dtype=np.float64).reshape(9))
+    r = np.asarray(r, dtype=np.float64)
+    if r.shape != (3, n) or counts.shape != (nbins, ntype, ntype) or counts.dtype != np.int64:
+        raise ValueError("rdf_frame: positions (3, n) and int64 counts (nbins, ntype, ntype)")
+    rx, ry, rz = (np.ascontiguousarray(r[k]).copy() for k in range(3))
+    out = L.orc_rdf_frame(n, ntype, typia.ctypes.data_as(C.POINTER(C.c_int)), _dp(A), _dp(rx), _dp(ry), _dp(rz),
+                          cutgr, resg, nbins, counts.ctypes.data_as(C.POINTER(C.c_longlong)))
+    return counts, np.stack([rx, ry, rz]), out
+
+
+def rdf_normalise(counts, nconf, resg, omega, nionit):
+    """write_rdf's normalisation: r[nbins], g[nbins][ntype][ntype]"""
+    L = load()
+    counts = np.ascontiguousarray(counts, dtype=np.int64)
+    nb, nt = counts.shape[0], counts.shape[1]
+    ni = np.ascontiguousarray(nionit, dtype=np.int32)
+    r = np.zeros(nb)
+    g = np.zeros((nb, nt, nt))
+    L.orc_rdf_normalise(nb, nt, counts.ctypes.data_as(C.POINTER(C.c_longlong)), nconf, resg, omega,
+                        ni.ctypes.data_as(C.POINTER(C.c_int)), _dp(r), _dp(g))
+    return r, g

@@ -16,6 +16,13 @@
  *        mdff20_ref_driver <dir> md <nsteps> <state-dump-period>
  *        mdff20_ref_driver <dir> run_md
  *        mdff20_ref_driver <dir> time <n_induced_calls>
+ *        mdff20_ref_driver <dir> rdf [write]
+ *
+ * The rdf mode follows main's RDF branch (mdff20.c:108-110): the steps of
+ * init_rdf (radial_distribution.c:162-178) and rdf_readtraj (:75-99) over
+ * the case's TRAJFF, writing the integer counts of gr and the positions get_gr
+ * leaves; write_rdf (:101-150) runs only with `write` (its first bin indexes
+ * grr with the still uninitialised mp, :128).
  */
 #include <stdio.h>
 #include <stdlib.h>
@@ -40,6 +47,7 @@
 #include "verlet.h"
 #include "bhmftd.h"
 #include "nmlj.h"
+#include "radial_distribution.h"
 
 static FILE *out;
 
@@ -229,6 +237,34 @@ int main(int argc, char **argv)
         fprintf(out, "t_induced %.9g\n", nind > 0 ? (t3 - t2) / nind : 0.0);
         fprintf(out, "t_force %.9g\n", t4 - t3);
         fprintf(out, "n_induced %d\n", nind);
+    } else if (!strcmp(mode, "rdf")) {
+        default_rdf();
+        read_rdf(controlfn);
+        alloc_rdf();
+        FILE *fp = fopen("TRAJFF", "r");
+        if (!fp) return 6;
+        for (int ic = 0; ic < nconf; ic++) {
+            read_next_traj(fp);
+            get_gr();
+        }
+        fclose(fp);
+        fprintf(out, "rdf_nbins %d\n", nbins);
+        fprintf(out, "rdf_ntype %d\n", ntype);
+        fprintf(out, "rdf_nconf %d\n", nconf);
+        fprintf(out, "resg %.17g\n", resg);
+        fprintf(out, "cutgr %.17g\n", cutgr);
+        fprintf(out, "omega %.17g\n", simuCell.Omega);
+        double *g = malloc(sizeof(double) * (size_t)nbins * ntype * ntype);
+        for (int b = 0; b < nbins; b++)
+            for (int i = 0; i < ntype; i++)
+                for (int j = 0; j < ntype; j++) g[((size_t)b * ntype + i) * ntype + j] = (double)gr[b][i][j];
+        dump_vec("gr", g, nbins * ntype * ntype);
+        free(g);
+        double ni[NTYPEMAX];
+        for (int i = 0; i < ntype; i++) ni[i] = (double)nionit[i];
+        dump_vec("nionit", ni, ntype);
+        dump_vec("rx", rx, nion); dump_vec("ry", ry, nion); dump_vec("rz", rz, nion);
+        if (argc > 3 && !strcmp(argv[3], "write")) write_rdf();
     } else {
         fprintf(stderr, "unknown mode %s\n", mode);
         return 2;

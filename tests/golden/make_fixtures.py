@@ -219,6 +219,54 @@ def run_case(name, control, posff_src=None, posff_text=None, mode="static", args
     return out
 
 
+def parse_trajff(path, n):
+    """TRAJFF records (write_config config.c:291-345) -> [(A (3, 3), r (3, n))],
+    the values as read_next_traj (config.c:448-502) reads them"""
+    lines = open(path).read().split("\n")
+    out, i = [], 0
+    while i < len(lines) and lines[i].strip():
+        A = np.array([[float(x) for x in lines[i + 2 + k].split()] for k in range(3)])
+        rows = lines[i + 9:i + 9 + n]
+        r = np.array([[float(x) for x in ln.split()[1:4]] for ln in rows]).T.copy()
+        out.append((A, r))
+        i += 9 + n
+    return out
+
+
+def rdf_case(name, control, posff_text, frames, resg, seed_note):
+    """the reference's RDF mode (mdff20.c:108-110) over TRAJFF frames written
+    here: gr counts as get_gr leaves them (with its extra all-pair count in
+    gr[.][0][0]) and the positions after the last frame"""
+    import mdff20_amd as m
+    d = os.path.join(HERE, name)
+    os.makedirs(d, exist_ok=True)
+    ctl = control + f"lrdf true\nresg {resg}\nnconf {len(frames)}\n"
+    open(os.path.join(d, "control"), "w").write(ctl)
+    open(os.path.join(d, "POSFF"), "w").write(posff_text)
+    P = m.read_posff(os.path.join(d, "POSFF"))
+    with tempfile.TemporaryDirectory() as tmp:
+        for f in ("control", "POSFF"):
+            shutil.copy(os.path.join(d, f), os.path.join(tmp, f))
+        for A, r in frames:
+            P.A = np.asarray(A, dtype=float)
+            P.r = r
+            P.v = np.zeros_like(r)
+            P.f = np.zeros_like(r)
+            m.write_config(os.path.join(tmp, "TRAJFF"), P, mode="a")
+        read = parse_trajff(os.path.join(tmp, "TRAJFF"), P.nion)
+        res = refrun.run(tmp, "rdf", threads=1)
+    nb, nt = int(res["rdf_nbins"]), int(res["rdf_ntype"])
+    out = dict(source=np.array("reference"), typia=np.asarray(P.typia, dtype=np.int32),
+               atypit=np.array(P.atypit), nionit=np.array(P.nionit, dtype=np.int32),
+               A=np.stack([a for a, _ in read]), frames=np.stack([r for _, r in read]),
+               resg=np.array(float(res["resg"])), cutgr=np.array(float(res["cutgr"])), nbins=np.array(nb),
+               omega=np.array(float(res["omega"])), nconf=np.array(len(frames)),
+               gr=res["gr"].reshape(nb, nt, nt).astype(np.int64),
+               r_after=np.stack([res["rx"], res["ry"], res["rz"]]), note=np.array(seed_note))
+    np.savez_compressed(os.path.join(d, "ref.npz"), **out)
+    print(f"{name} (reference rdf): {len(frames)} frames, {nb} bins, {int(out['gr'][:, 0, 0].sum())} counts in gr[.][0][0]")
+
+
 def melt(n_units, seed, pos_seed):
     L = sio2_box(n_units)
     rng = np.random.default_rng(pos_seed)
@@ -384,6 +432,38 @@ def main():
                      "2\nTWODIP\n10.0 0.0 0.0\n0.0 10.0 0.0\n0.0 0.0 10.0\n2\nA B\n1 1\nCartesian\n"
                      "A 0.0 0.0 0.0 0.0 0.0 0.0 0.0 0.0 0.0\n"
                      "B 1.3 0.7 -0.4 0.0 0.0 0.0 0.0 0.0 0.0\n"))
+    # ---- radial distribution (radial_distribution.c): TRAJFF frames made here
+    if want("rdf_melt216"):
+        # the melt216 configuration and three frames displaced by up to 0.3 A
+        g = os.path.join(HERE, "melt216_nve")
+        ctl = open(os.path.join(g, "control")).read()
+        txt = open(os.path.join(g, "POSFF")).read()
+        import mdff20_amd as m
+        P = m.read_posff(os.path.join(g, "POSFF"))
+        rng = np.random.default_rng(20216)
+        r, frames = P.r.copy(), []
+        for _ in range(3):
+            r = r + rng.uniform(-0.3, 0.3, r.shape)
+            frames.append((P.A.copy(), r.copy()))
+        rdf_case("rdf_melt216", ctl, txt, frames, 0.1, "melt216 POSFF, uniform +-0.3 A steps, seed 20216")
+    if want("rdf_tric"):
+        # a triclinic cell, 150 O + 90 Si uniform in fractional coordinates,
+        # two frames, bins of 0.15 A
+        g = os.path.join(HERE, "melt216_nve")
+        ctl = open(os.path.join(g, "control")).read()
+        rng = np.random.default_rng(7150)
+        A = np.array([[13.0, 0.0, 0.0], [2.6, 12.4, 0.0], [-1.9, 3.1, 11.7]])
+        lines = ["240", "tric240"] + [" ".join(f"{x:.12f}" for x in row) for row in A]
+        lines += ["2", "O Si", "150 90", "Direct"]
+        s0 = rng.uniform(-0.5, 0.5, (240, 3))
+        lines += [f"{'O' if i < 150 else 'Si'} {s0[i, 0]:.12f} {s0[i, 1]:.12f} {s0[i, 2]:.12f} "
+                  "0.0 0.0 0.0 0.0 0.0 0.0" for i in range(240)]
+        txt = "\n".join(lines) + "\n"
+        frames = []
+        for _ in range(2):
+            s1 = rng.uniform(-0.5, 0.5, (240, 3))
+            frames.append((A, (s1 @ A).T.copy()))
+        rdf_case("rdf_tric", ctl, txt, frames, 0.15, "triclinic 240 ions, uniform fractional, seed 7150")
     # ---- CCD LJ clusters (examples/ex2): reference energies and structures
     if want("ccd"):
         cdir = os.path.join(REF, "examples/ex2/config")

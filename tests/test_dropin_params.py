@@ -22,7 +22,8 @@ IGNORED = {"nmassit", "nqit", "ndipit", "nquadit", "npolit", "integrator", "dt",
 LJ = {"sigmalj", "epslj", "plj", "qlj"}
 PIM = {"algo_pim", "algo_extrapolate_dipole", "extrapolate_order", "min_scf_pim_iter", "max_scf_pim_iter",
        "conv_tol_ind", "omegakO"}
-CASES = sorted(c for c in os.listdir(GOLDEN) if os.path.exists(os.path.join(GOLDEN, c, "control")))
+CASES = sorted(c for c in os.listdir(GOLDEN)
+               if os.path.exists(os.path.join(GOLDEN, c, "control")) and not c.startswith("rdf_"))
 
 
 @pytest.mark.parametrize("case", CASES)

@@ -200,3 +200,38 @@ def test_dropin_run_md_nvt_pim_matches_oracle(tmp_path):
     assert rel(a["mu"], o.dipoles().ravel(), 1e-12) < TOL
     assert len(_numbers(os.path.join(d, "OSZIFF"))) > 0
     o.close()
+
+
+@pytest.mark.parametrize("case", ["rdf_melt216", "rdf_tric"])
+def test_dropin_get_gr_matches_reference(case, tmp_path):
+    """the reference's RDF mode (init_rdf's steps over TRAJFF) with get_gr
+    replaced by the device histogram: gr and the positions it leaves equal
+    the unmodified reference's, integer for integer and bit for bit"""
+    _need_driver()
+    import mdff20_amd as m
+    print(f"[gpu] drop-in get_gr {case}", flush=True)
+    src = os.path.join(GOLDEN, case)
+    z = dict(np.load(os.path.join(src, "ref.npz"), allow_pickle=False))
+    outs = {}
+    for kind, exe in (("ref", refrun.DRIVER), ("dropin", refrun.DRIVER_DROPIN)):
+        if kind == "ref" and not refrun.available():
+            continue
+        d = str(tmp_path / kind)
+        os.makedirs(d)
+        for f in ("control", "POSFF"):
+            shutil.copy(os.path.join(src, f), d)
+        P = m.read_posff(os.path.join(d, "POSFF"))
+        for A, r in zip(z["A"], z["frames"]):
+            P.A, P.r = A, r
+            P.v = np.zeros_like(r)
+            P.f = np.zeros_like(r)
+            m.write_config(os.path.join(d, "TRAJFF"), P, mode="a")
+        outs[kind] = refrun.run(d, "rdf", exe=exe, timeout=300)
+    a = outs["dropin"]
+    nb = int(a["rdf_nbins"])
+    assert np.array_equal(a["gr"].reshape(z["gr"].shape).astype(np.int64), z["gr"]) and nb == int(z["nbins"])
+    assert np.array_equal(np.stack([a["rx"], a["ry"], a["rz"]]), z["r_after"])
+    if "ref" in outs:
+        b = outs["ref"]
+        for key in ("gr", "rx", "ry", "rz"):
+            assert np.array_equal(a[key], b[key]), key

@@ -9,7 +9,7 @@ import pytest
 import mdff20_amd as m
 from conftest import GOLDEN, ROOT, load_case
 
-HEADERS = ["mdff_gpu.h", "mdff_host.h", "mdff_batch.h"]
+HEADERS = ["mdff_gpu.h", "mdff_host.h", "mdff_batch.h", "mdff_rdf.h"]
 
 
 def declared_functions():
