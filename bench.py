@@ -530,6 +530,25 @@ def main():
         e.set_list_method(0)
         rebuild = dict(cells_ms=t_cells, all_pairs_ms=t_all, grid=info["nc"], reach=info["reach"])
         log(f"neighbour rebuild: {rebuild}")
+    # the radial distribution of the run's configuration (get_gr's histogram
+    # of every pair within half the box), timed live on the engine's stream;
+    # its CPU baseline is the oracle's get_gr restatement on one host core
+    rdf = None
+    if not args.no_kernels:
+        cut, nb = m.rdf_defaults(A)
+        with m.Rdf(typ, 2, nb, 0.1, cut) as h:
+            ms = h.time(e, iters=10)
+        pairs = n * (n - 1) / 2
+        rdf = dict(ms_per_frame=ms, frames_per_s=1e3 / ms, pair_tests_per_s=pairs / (ms * 1e-3), nbins=nb,
+                   cutgr=cut, pair_tests_per_frame=pairs)
+        if not args.no_cpu:
+            from oracle import oracle as O
+            r_now = e.positions()
+            t0 = time.perf_counter()
+            O.rdf_frame(typ, 2, A, r_now, cut, 0.1, nb)
+            rdf["cpu_baseline"] = dict(value=1.0 / (time.perf_counter() - t0), unit="frames/s", cores=1, kind="port",
+                                       sample=f"oracle get_gr restatement, one {n}-ion frame, ordered pairs")
+        log(f"rdf: {rdf}")
     roof = None
     mm = [(v["ms"] * v["per_step"], k) for k, v in kern.items() if v["flops"]]
     if mm:
@@ -586,6 +605,7 @@ def main():
         "accuracy_mixed_refine_vs_fp64": acc_refine,
         "kernels": kern,
         "neighbour_rebuild": rebuild,
+        "rdf": rdf,
         "roofline": roof,
         "cpu_baseline": cpu,
     }

@@ -54,6 +54,12 @@ int mdff_rdf_sample(mdff_rdf *h, mdff_ctx *ctx);
  * one launch. */
 int mdff_rdf_sample_trajectory(mdff_rdf *h, mdff_ctx *ctx);
 
+/* Measurement: the device time of one frame of the engine's positions
+ * (kardir and pair kernels), averaged over iters frames timed with HIP
+ * events on the engine's stream; the counts are reset afterwards.  No
+ * counterpart in the reference. */
+int mdff_rdf_time(mdff_rdf *h, mdff_ctx *ctx, int iters, double *ms);
+
 /* Ordered pair counts counts[nbins][ntype][ntype] of all frames so far, as
  * gr[igr][it1][it2] of get_gr (radial_distribution.c:220), without its
  * extra all-pair increment of gr[igr][0][0] (:219, which the drop-in get_gr

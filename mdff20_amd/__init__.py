@@ -99,6 +99,7 @@ def lib():
         "mdff_rdf_reset": (C.c_int, [vp]),
         "mdff_rdf_add_frame": (C.c_int, [vp, dp, dp, dp, dp, C.c_int]),
         "mdff_rdf_sample": (C.c_int, [vp, vp]),
+        "mdff_rdf_time": (C.c_int, [vp, vp, C.c_int, dp]),
         "mdff_rdf_sample_trajectory": (C.c_int, [vp, vp]),
         "mdff_rdf_counts": (C.c_int, [vp, P(C.c_longlong), P(C.c_longlong)]),
         "mdff_rdf_gr": (C.c_int, [vp, P(C.c_longlong), C.c_longlong, C.c_double, dp, dp]),
@@ -511,6 +512,12 @@ class Rdf:
 
     def reset(self):
         _check(self._L.mdff_rdf_reset(self._h), "rdf_reset", False)
+
+    def time(self, engine, iters=10):
+        """ms of device time per frame of the engine's positions (HIP events); resets the counts"""
+        ms = C.c_double()
+        _check(self._L.mdff_rdf_time(self._h, engine._h, iters, C.byref(ms)), "rdf_time", False)
+        return ms.value
 
     def counts(self):
         """ordered pair counts [nbins][ntype][ntype] and (frames, pairs out of range)"""

@@ -389,6 +389,43 @@ static int engine_frames(mdff_rdf *h, mdff_ctx *ctx, bool traj)
 
 int mdff_rdf_sample(mdff_rdf *h, mdff_ctx *ctx) { return engine_frames(h, ctx, false); }
 
+// iters histograms of the engine's positions between two HIP events on the
+// engine's stream (kardir and pair kernels); the counts are reset after
+int mdff_rdf_time(mdff_rdf *h, mdff_ctx *ctx, int iters, double *ms)
+{
+    if (!h || !ctx || iters < 1 || !ms) return MDFF_ERR_ARG;
+    EngineView v;
+    int rc = mdff_engine_view(ctx, &v);
+    if (rc) return rc;
+    if (v.n != h->n || v.ntype != h->ntype || v.device != h->device) return MDFF_ERR_ARG;
+    const size_t n = (size_t)h->n;
+    RdfCell c;
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) {
+            c.A[i][j] = v.A[i][j];
+            c.B[i][j] = v.B[i][j];
+        }
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) return MDFF_ERR_DEVICE;
+    hipLaunchKernelGGL(k_rdf_frac, dim3(cdiv(n, 256), 1), dim3(256), 0, v.st, h->n, c, v.r, 3 * n, h->s, nullptr);
+    rc = rdf_pairs(h, c, h->s, h->type, 1, v.st);
+    if (!rc && hipEventRecord(e0, v.st) != hipSuccess) rc = MDFF_ERR_DEVICE;
+    for (int k = 0; k < iters && !rc; k++) {
+        hipLaunchKernelGGL(k_rdf_frac, dim3(cdiv(n, 256), 1), dim3(256), 0, v.st, h->n, c, v.r, 3 * n, h->s,
+                           nullptr);
+        rc = rdf_pairs(h, c, h->s, h->type, 1, v.st);
+    }
+    float t = 0.0f;
+    if (!rc && (hipEventRecord(e1, v.st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                hipEventElapsedTime(&t, e0, e1) != hipSuccess))
+        rc = MDFF_ERR_DEVICE;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc) return rc;
+    *ms = (double)t / iters;
+    return mdff_rdf_reset(h);
+}
+
 int mdff_rdf_sample_trajectory(mdff_rdf *h, mdff_ctx *ctx) { return engine_frames(h, ctx, true); }
 
 int mdff_rdf_counts(mdff_rdf *h, long long *counts, long long info[2])
