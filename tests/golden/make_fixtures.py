@@ -395,7 +395,8 @@ def main():
     # (no polarisable ions): static dipoles, damped BHMFTD dispersion, the
     # Nose-Hoover chain and leap-frog
     m216 = os.path.join(HERE, "melt216_nve")
-    if any(want(c) for c in ("melt216_dip_nve", "melt216_bhmftd_nve", "melt216_nvt", "melt216_lf")):
+    if any(want(c) for c in ("melt216_dip_nve", "melt216_bhmftd_nve", "melt216_nvt", "melt216_lf",
+                             "melt216_quad_static")):
         base = open(os.path.join(m216, "control")).read()
         tmp = tempfile.mkdtemp()
         shutil.copy(os.path.join(m216, "POSFF"), os.path.join(tmp, "POSFF"))
@@ -409,6 +410,13 @@ def main():
             # Ewald terms, forces and stress (ewald.c:737-830, 999-1006)
             run_case("melt216_dip_nve", vary(base, npas=50) + "dipit 0.05 -0.03 0.08\ndipit 0.0 0.0 0.0\n",
                      pos216, mode="md", args=(50, 25))
+        if want("melt216_quad_static"):
+            # static quadrupoles on both types: the reference sets lqua
+            # (field.c:201) and builds its rank-5 tensor (ewald.c:646-688),
+            # but no energy, field or force term reads the quadrupoles
+            run_case("melt216_quad_static", vary(base, npas=0) + "lstatic true\n"
+                     "quadit 0.3 0.0 0.1 0.0 0.2 0.0 0.1 0.0 -0.5\nquadit -0.2 0.05 0.0 0.05 0.4 0.0 0.0 0.0 -0.2\n",
+                     pos216)
         if want("melt216_bhmftd_nve"):
             # BHMFTD with Tang-Toennies damped C6/C8 terms (bhmftd.c:117-271)
             run_case("melt216_bhmftd_nve",

@@ -13,7 +13,7 @@ Oracle = pytest.importorskip("oracle.oracle").Oracle
 
 TOL = 1e-10
 STATIC = ["twocharges", "onedipole", "twodipoles", "twocharges_twopol", "sio2_102_static", "lj_fcc_static",
-          "pim192_static", "pim192_fixed", "sio2_102_ko_static"]
+          "pim192_static", "pim192_fixed", "sio2_102_ko_static", "melt216_quad_static"]
 MD = ["melt216_nve", "melt216_raw", "melt216_dip_nve", "melt216_bhmftd_nve", "melt216_nvt", "melt216_lf",
       "lj_fcc_nve", "sio2_102_nvt", "bsio_200_nvt", "pim192_nvt", "sio2_102_ko_nvt", "bsio_200_ko_nvt"]
 
@@ -25,7 +25,8 @@ def n_evals(case, t, o):
 
 
 REFERENCE_CASES = {"twocharges", "onedipole", "twodipoles", "lj_fcc_static", "melt216_nve", "melt216_raw",
-                   "melt216_dip_nve", "melt216_bhmftd_nve", "melt216_nvt", "melt216_lf", "lj_fcc_nve"}
+                   "melt216_dip_nve", "melt216_bhmftd_nve", "melt216_nvt", "melt216_lf", "lj_fcc_nve",
+                   "melt216_quad_static"}
 
 
 @pytest.mark.parametrize("case", STATIC + MD)

@@ -30,7 +30,7 @@ import pytest
 from scipy.special import gammainc, gammaincc
 
 import mdff20_amd as m
-from conftest import load_case, rel
+from conftest import SCALAR_KEYS, load_case, ref_state, rel, scalars_of
 
 orc = pytest.importorskip("oracle.oracle")
 Oracle = orc.Oracle
@@ -388,3 +388,29 @@ def test_charge_stress_is_minus_strain_derivative():
         dU = (energy(E) - energy(-E)) / (2 * h)
         want = -dU if a == b else -dU / 2
         assert tau[a, b] == pytest.approx(want, rel=1e-6, abs=1e-6 * np.abs(tau).max()), (a, b, tau[a, b], want)
+
+
+def test_reference_ignores_static_quadrupoles():
+    """the reference's own static run with quadit on both types (lqua set,
+    field.c:201; the rank-5 tensor built, ewald.c:646-688) equals the oracle
+    with the quadrupoles cleared: no energy, force, stress or Ewald setting of
+    the reference reads them"""
+    import ctypes as C
+    p, cfg, ref = load_case("melt216_quad_static")
+    assert str(ref["source"]) == "reference"
+    assert p.nquadit == 2 and p.quadit[0][2][2] == -0.5
+    q = m.Params()
+    C.pointer(q)[0] = p
+    for t in range(2):
+        for i in range(3):
+            for j in range(3):
+                q.quadit[t][i][j] = 0.0
+    o = Oracle(q, cfg.typia, cfg.A, cfg.r, threads=1)
+    t = o.engforce(0)
+    sc = ref["scalars"][0]
+    mine = scalars_of(t)
+    for k in range(len(SCALAR_KEYS)):
+        assert abs(mine[k] - sc[k]) <= 1e-12 * max(abs(sc[k]), 1e-12), (SCALAR_KEYS[k], mine[k], sc[k])
+    _, _, fr = ref_state(ref, 0)
+    assert rel(o.forces(), fr, 1e-4) < 1e-12
+    assert rel(np.array(t["tau_coul"]), ref["tau_coul"][0], 1e-12) < 1e-12
