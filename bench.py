@@ -225,11 +225,13 @@ def cpu_baseline(p_text, L, pos, typ, n_scf, threads):
                 fh.write(f"{'O' if typ[i] == 0 else 'Si'} {pos[0, i]!r} {pos[1, i]!r} {pos[2, i]!r}\n")
         n_ind = 2
         t0 = time.time()
-        res = refrun.run(d, "time", n_ind, threads=threads, timeout=900)
+        res = refrun.run(d, "time", n_ind, threads=threads, o3=refrun.available(o3=True), timeout=900)
+        o3 = refrun.available(o3=True)
         wall = time.time() - t0
     t_step = res["t_short"] + res["t_static"] + n_scf * res["t_induced"] + res["t_force"]
     return dict(value=1.0 / t_step, unit="steps/s", cores=int(res["threads"]), kind="reference",
-                sample=(f"reference build (shipped -O0 flags, OpenMP) at N={len(typ)}: short range, static field, "
+                sample=(f"reference build ({'-O3 -Ofast' if o3 else 'shipped -O0'} flags, OpenMP) at N={len(typ)}: "
+                        f"short range, static field, "
                         f"{n_ind} induced-dipole Ewald calls and the force call timed once ({wall:.1f} s wall); "
                         f"step = t_short + t_static + {n_scf:.1f} x t_induced + t_force "
                         f"(SCF iterations per step of the FP64 GPU run)"),

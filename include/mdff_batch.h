@@ -66,8 +66,10 @@ typedef struct {
  * cutoff settings; with ew == NULL every configuration takes alphaES, kES,
  * cutlongrange and cutshortrange of p (lautoES applied per cell).
  * precision: MDFF_FP64 or MDFF_MIXED (mdff_gpu.h).  Cutoffs must not exceed
- * half the smallest cell width (minimum image).  Static dipoles and
- * quadrupoles are not supported (MDFF_ERR_UNSUPPORTED). */
+ * half the smallest cell width (minimum image).  Static dipoles are not
+ * supported (MDFF_ERR_UNSUPPORTED); static quadrupoles are accepted and, as
+ * in the reference (whose Ewald sums read no quadrupole, ewald.c:646-688),
+ * change no result. */
 int mdff_batch_create(mdff_batch **out, const mdff_params *p, int nconf, const int *nion, const int *typia,
                       const double *A9, const double *rx, const double *ry, const double *rz,
                       const mdff_batch_ewald *ew, int precision, int device);
