@@ -84,6 +84,10 @@ struct mdff_batch {
     double *W = nullptr, *ek = nullptr;
     double *tab64 = nullptr;
     float *tab32 = nullptr;
+    // the same tables column by column ([TS][n]): the back-projection's
+    // lanes (consecutive ions) read one column coalesced
+    double *tabT64 = nullptr;
+    float *tabT32 = nullptr;
     int TS = 0, by = 0, bz = 0;
     int *queue = nullptr;  // [2]: rho, [2]: back
     WItem *items_rho = nullptr, *items_back = nullptr;
@@ -185,7 +189,7 @@ static void launch_back(mdff_batch *b, int lo, int nch, const double *q, const d
 {
     BackBArgs a;
     a.conf = b->conf; a.kv = b->kv;
-    a.tab = lo ? (const void *)b->tab32 : (const void *)b->tab64;
+    a.tab = lo ? (const void *)b->tabT32 : (const void *)b->tabT64;
     a.TS = b->TS; a.by = b->by; a.bz = b->bz; a.n = b->n;
     a.W = b->W; a.items = b->items_back; a.nitems = b->nitems_back; a.queue = b->queue + 2;
     a.q = q; a.mu = mu; a.ef = ef; a.efg = efg; a.f = f; a.act = act;
@@ -381,9 +385,9 @@ static int engforce_b(mdff_batch *b)
     BCHK(hipMemcpyAsync(sr.data(), b->seg, sizeof(double) * nc * 14, hipMemcpyDeviceToHost, b->st));
     if (b->p.lcoulombic) {
         if (lo) hipLaunchKernelGGL(kb_eikr<float>, dim3(bdiv(n, 128)), dim3(128), 0, b->st, n, b->conf, b->cfg, b->r,
-                                   b->TS, b->by, b->bz, b->tab32);
+                                   b->TS, b->by, b->bz, b->tab32, b->tabT32);
         hipLaunchKernelGGL(kb_eikr<double>, dim3(bdiv(n, 128)), dim3(128), 0, b->st, n, b->conf, b->cfg, b->r, b->TS,
-                           b->by, b->bz, b->tab64);
+                           b->by, b->bz, b->tab64, b->tabT64);
         if (b->lpol) {
             // static field of the charges (momentpolaSCF pim.c:216-218), damped
             DirArgs a = bdir_args(b, b->q, b->mu0, b->ldmp, 0, nullptr);
@@ -491,7 +495,7 @@ void mdff_batch_destroy(mdff_batch *b)
     bfree(b->efg_rec); bfree(b->efg_dummy); bfree(b->f); bfree(b->f_sr); bfree(b->f_dir); bfree(b->f_rec);
     bfree(b->acc_sr); bfree(b->uw_dir); bfree(b->selfb); bfree(b->upol_i); bfree(b->rmsd_i);
     bfree(b->dc_R); bfree(b->dc_d); bfree(b->dc_g); bfree(b->nbr); bfree(b->cnt); bfree(b->ctl);
-    bfree(b->kv); bfree(b->W); bfree(b->ek); bfree(b->tab64); bfree(b->tab32); bfree(b->queue);
+    bfree(b->kv); bfree(b->W); bfree(b->ek); bfree(b->tab64); bfree(b->tab32); bfree(b->tabT64); bfree(b->tabT32); bfree(b->queue);
     bfree(b->items_rho); bfree(b->items_back); bfree(b->act); bfree(b->iscf); bfree(b->any);
     bfree(b->rmsd2); bfree(b->rmsd); bfree(b->seg); bfree(b->g); bfree(b->sdir); bfree(b->srhs); bfree(b->lam);
     bfree(b->eT); bfree(b->edel); bfree(b->cols); bfree(b->tsum);
@@ -654,7 +658,11 @@ int mdff_batch_create(mdff_batch **out, const mdff_params *p, int nconf, const i
     BALLOC(b->kv, std::max<size_t>(1, b->nktot)); BALLOC(b->W, 2 * std::max<size_t>(1, b->nktot));
     BALLOC(b->ek, 7 * std::max<size_t>(1, b->nktot));
     BALLOC(b->tab64, 2 * (size_t)n * b->TS);
-    if (precision == MDFF_MIXED) BALLOC(b->tab32, 2 * (size_t)n * b->TS);
+    BALLOC(b->tabT64, 2 * (size_t)n * b->TS);
+    if (precision == MDFF_MIXED) {
+        BALLOC(b->tab32, 2 * (size_t)n * b->TS);
+        BALLOC(b->tabT32, 2 * (size_t)n * b->TS);
+    }
     BALLOC(b->queue, 4); BALLOC(b->act, nconf); BALLOC(b->iscf, nconf); BALLOC(b->any, 1);
     BALLOC(b->rmsd2, nconf); BALLOC(b->rmsd, nconf); BALLOC(b->seg, (size_t)nconf * 40);
     BALLOC(b->g, n3); BALLOC(b->sdir, n3); BALLOC(b->srhs, n3); BALLOC(b->lam, n3); BALLOC(b->eT, n3);
@@ -936,9 +944,9 @@ int mdff_batch_autotune(mdff_batch *b, int out[4])
     if ((rc = build_list(b))) return rc;
     if (b->p.lcoulombic) {
         hipLaunchKernelGGL(kb_eikr<double>, dim3(bdiv(n, 128)), dim3(128), 0, b->st, n, b->conf, b->cfg, b->r, b->TS,
-                           b->by, b->bz, b->tab64);
+                           b->by, b->bz, b->tab64, b->tabT64);
         if (lo) hipLaunchKernelGGL(kb_eikr<float>, dim3(bdiv(n, 128)), dim3(128), 0, b->st, n, b->conf, b->cfg, b->r,
-                                   b->TS, b->by, b->bz, b->tab32);
+                                   b->TS, b->by, b->bz, b->tab32, b->tabT32);
     }
     double best;
     // real-space lanes: the Ewald force kernel

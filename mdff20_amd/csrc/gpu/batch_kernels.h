@@ -137,9 +137,12 @@ __global__ void __launch_bounds__(256) kb_list(int n, const BConfD *__restrict__
 // e^{i n theta_b} tables per ion, AoS with a fixed stride of TS complex
 // numbers: x at [0, Kx], y centred at by (n = -Ky..Ky), z centred at bz;
 // theta_b = 2 pi t_b, t_b = sum_a B[a][b]... as kardir, wrapped to [-1/2, 1/2].
+// tabT (when given) receives the same values column by column, complex
+// entry m of ion i at [m][i] (coalesced over consecutive ions).
 template <typename real>
 __global__ void kb_eikr(int n, const BConfD *__restrict__ conf, const int *__restrict__ cfg,
-                        const double *__restrict__ r, int TS, int by, int bz, real *__restrict__ tab)
+                        const double *__restrict__ r, int TS, int by, int bz, real *__restrict__ tab,
+                        real *__restrict__ tabT = nullptr)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -171,6 +174,14 @@ __global__ void kb_eikr(int n, const BConfD *__restrict__ conf, const int *__res
         sincos(th * (double)m, &sn, &cs);
         T[2 * (bz + m)] = (real)cs;
         T[2 * (bz + m) + 1] = (real)sn;
+    }
+    if (tabT) {
+        // the columns this configuration uses (x 0..Kx, y and z -K..K)
+        typedef real r2 __attribute__((ext_vector_type(2)));
+        auto col = [&](int m) { *(r2 *)(tabT + 2 * ((size_t)m * n + i)) = *(const r2 *)(T + 2 * m); };
+        for (int m = 0; m <= K.Kx; m++) col(m);
+        for (int m = -K.Ky; m <= K.Ky; m++) col(by + m);
+        for (int m = -K.Kz; m <= K.Kz; m++) col(bz + m);
     }
 }
 
@@ -274,7 +285,7 @@ __global__ void __launch_bounds__(256) kb_rho(RhoArgs a)
 struct BackBArgs {
     const BConfD *conf;
     const KVec *kv;
-    const void *tab;
+    const void *tab;       // tables column by column, [TS][n] complex (kb_eikr's tabT)
     int TS, by, bz;
     int n;
     const double *W;       // [nk][2]
@@ -313,12 +324,13 @@ __global__ void __launch_bounds__(256) kb_back(BackBArgs a)
 #pragma unroll
         for (int c = 0; c < NCH; c++) acc[c] = 0.0;
         if (valid) {
-            const real *T = tab + 2 * (size_t)i * a.TS;
+            typedef real r2 __attribute__((ext_vector_type(2)));
+            const r2 *T = (const r2 *)tab + i;
             for (int k = sl; k < K.nk; k += S) {
                 const KVec kv = a.kv[K.koff + k];
                 const double Wr = a.W[2 * ((size_t)K.koff + k)], Wi = a.W[2 * ((size_t)K.koff + k) + 1];
-                const int ox = 2 * kv.nx, oy = 2 * (a.by + kv.ny), oz = 2 * (a.bz + kv.nz);
-                double xr = T[ox], xi = T[ox + 1], yr = T[oy], yi = T[oy + 1], zr = T[oz], zi = T[oz + 1];
+                const r2 X = T[(size_t)kv.nx * n], Y = T[(size_t)(a.by + kv.ny) * n], Z = T[(size_t)(a.bz + kv.nz) * n];
+                double xr = X[0], xi = X[1], yr = Y[0], yi = Y[1], zr = Z[0], zi = Z[1];
                 double pr = xr * yr - xi * yi, pi = xr * yi + xi * yr;
                 double er = pr * zr - pi * zi, ei = pr * zi + pi * zr;
                 double zre = Wr * er - Wi * ei, zim = Wr * ei + Wi * er;
