@@ -169,6 +169,63 @@ def batch_relax(confs, ews, iters=30, gamma=0.01, cap=0.05):
     return [(t, A, rr.copy()) for (t, A, _), rr in zip(confs, parts)]
 
 
+def batch_bench(seed=4096):
+    """the batched engine on its two workloads: 512 uneven cells (N log-uniform
+    in [96, 1536], per-cell Ewald for 1e-6) force calls in the FP64 and mixed
+    forms, and the force-matching adjoint on 64 cells of 648 ions from NVT
+    dynamics (Adam steps through the autograd wrapper)"""
+    import mdff20_amd as m
+    out = {}
+    confs, ews = batch_systems(512, seed)
+    confs = batch_relax(confs, ews)
+    p = params_of(control(True))
+    nion = sum(len(c[0]) for c in confs)
+    for prec, name in ((m.FP64, "fp64"), (m.MIXED, "mixed")):
+        with m.batch.Batch(p, confs, ews, precision=prec) as b:
+            b.engforce()
+            t0 = time.perf_counter()
+            for _ in range(3):
+                b.engforce()
+            dt = (time.perf_counter() - t0) / 3
+        out[f"cfg4_{name}"] = dict(ms_per_force_call=1e3 * dt, configs_per_s=512 / dt, ions_per_s=nion / dt)
+    out["cfg4_cells"] = dict(n=512, ions=nion)
+    # cfg5: 4 seeded starts, 16 cells each from NVT dynamics (kmax 8, 21.4 A cell)
+    q = params_of(control(True, alpha=0.3, kmax=8, rc=9.0))
+    q.conv_tol_ind = 1e-10
+    cells = []
+    for s5 in range(4):
+        L, pos, typ = system(216, 648 + s5)
+        r = equilibrate(L, pos, typ, 200, kmax=8)
+        A = np.diag([L, L, L])
+        with m.Engine(q, typ, A, r, precision=m.FP64) as e:
+            e.init_velocities()
+            e.md_init()
+            for k in range(16):
+                e.md_run(10 * k + 1, 10)
+                cells.append((typ, A, e.positions()))
+    ew = dict(alpha=q.alphaES, kES=tuple(q.kES), rcut_long=q.cutlongrange, rcut_short=q.cutshortrange)
+    try:
+        from mdff20_amd.adjoint import fit
+    except ImportError:
+        return out
+    with m.batch.Batch(q, cells, [ew] * len(cells)) as b:
+        spec = m.batch.ParamSpec.default(q, q.ntype)
+        rng = np.random.default_rng(11)
+        v = spec.values()
+        b.set_params(spec.params(v + np.log(rng.uniform(0.9, 1.1, v.size))))
+        b.engforce()
+        fref = b.forces()
+        b.set_params(q)
+        fit(b, spec, fref, steps=2, lr=0.01)
+        t0 = time.perf_counter()
+        _, hist = fit(b, spec, fref, steps=20, lr=0.01)
+        dt = time.perf_counter() - t0
+    out["cfg5_adjoint"] = dict(cells=64, ions_per_cell=648, adam_steps=20, steps_per_s=20 / dt,
+                               loss_first=float(hist[0]), loss_last=float(hist[-1]))
+    log(f"batch: {out}")
+    return out
+
+
 def timed_run(p, typ, A, r, precision, steps, warmup, fp64_cg=False, list_method=0, check=True, refine=False):
     import mdff20_amd as m
     e = m.Engine(p, typ, A, r, precision=precision)
@@ -402,6 +459,7 @@ def main():
     ap.add_argument("--no-kernels", action="store_true",
                     help="skip the per-kernel timing of every arithmetic variant, the accuracy call and the "
                          "FP64 residual check (a kernel trace then holds the timed MD steps of the chosen forms only)")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched-engine workloads (512 cells, adjoint)")
     ap.add_argument("--state", default=None, metavar="NPZ",
                     help="relaxed positions: read from NPZ when it exists, else relaxed and written there")
     args = ap.parse_args()
@@ -551,6 +609,7 @@ def main():
             rdf["cpu_baseline"] = dict(value=1.0 / (time.perf_counter() - t0), unit="frames/s", cores=1, kind="port",
                                        sample=f"oracle get_gr restatement, one {n}-ion frame, ordered pairs")
         log(f"rdf: {rdf}")
+    batch = None if (args.no_kernels or args.no_batch) else batch_bench()
     roof = None
     mm = [(v["ms"] * v["per_step"], k) for k, v in kern.items() if v["flops"]]
     if mm:
@@ -608,6 +667,7 @@ def main():
         "kernels": kern,
         "neighbour_rebuild": rebuild,
         "rdf": rdf,
+        "batch": batch,
         "roofline": roof,
         "cpu_baseline": cpu,
     }
