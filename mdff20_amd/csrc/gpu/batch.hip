@@ -74,7 +74,6 @@ struct mdff_batch {
     double *efg_dummy = nullptr;
     double *f = nullptr, *f_sr = nullptr, *f_dir = nullptr, *f_rec = nullptr;
     double *acc_sr = nullptr, *uw_dir = nullptr, *selfb = nullptr, *upol_i = nullptr, *rmsd_i = nullptr;
-    double *dc_R = nullptr, *dc_d = nullptr, *dc_g = nullptr;
     unsigned *nbr = nullptr;
     int *cnt = nullptr, *ctl = nullptr;
     int cap = 0;
@@ -100,7 +99,6 @@ struct mdff_batch {
     double *cols = nullptr, *tsum = nullptr;
     // tuning
     int lanes = 16, kpb = 64, ipb = 64, bpc = 4, tuned = 0;
-    int inner_iters = 6;
     double last_ms = 0.0;
     long long sweeps = 0;
     std::vector<int> nscf_h;
@@ -300,56 +298,28 @@ static int scf(mdff_batch *b)
     hipLaunchKernelGGL(kb_set_conf_int, dim3(bdiv(nc, 256)), dim3(256), 0, b->st, nc, b->act,
                        (maxit > 0 || minit > 0) ? 1 : 0);
     hipLaunchKernelGGL(kb_set_conf_int, dim3(bdiv(nc, 256)), dim3(256), 0, b->st, nc, b->iscf, 0);
-    int lo = lo_b(b);
+    // the mixed form runs the same Jacobi sequence on its low-precision field
+    // (FP32 real space and k-space tables, FP64 sums): its fixed point sits
+    // within the field's rounding (about 1e-6 relative) of the FP64 one
+    const int lo = lo_b(b);
     int limit = std::max(maxit, minit) + 1;
     int launched = 0;
-    if (!lo) {
-        while (launched < limit) {
-            int chunk = launched == 0 ? std::max(1, std::min(minit, 4)) : 4;
-            for (int it = 0; it < chunk && launched < limit; it++, launched++) {
-                hipLaunchKernelGGL(kb_scf_mu, dim3(bdiv(n, 256)), dim3(256), 0, b->st, n, b->cfg, b->act,
-                                   launched == 0 ? 1 : 0, b->type, b->tab_d, b->hist, b->ef, b->mu, b->upol_i);
-                induced_field(b, 0, b->mu, b->ef_stat, b->ef, b->act);
-                hipLaunchKernelGGL(kb_scf_rmsd, dim3(bdiv(n, 256)), dim3(256), 0, b->st, n, b->cfg, b->act, b->type,
-                                   b->tab_d, b->mu, b->ef, b->rmsd_i);
-                segsum(b, b->rmsd_i, n, 1, 0, b->rmsd2);
-                hipLaunchKernelGGL(kb_scf_control, dim3(1), dim3(1024), 0, b->st, nc, b->conf, b->rmsd2, b->act,
-                                   b->iscf, b->rmsd, maxit, minit, b->p.conv_tol_ind, b->any);
-                b->sweeps++;
-            }
-            int rc = read_any(b);
-            if (rc) return rc;
-            if (!*b->any_h) break;
+    while (launched < limit) {
+        int chunk = launched == 0 ? std::max(1, std::min(minit, 4)) : 4;
+        for (int it = 0; it < chunk && launched < limit; it++, launched++) {
+            hipLaunchKernelGGL(kb_scf_mu, dim3(bdiv(n, 256)), dim3(256), 0, b->st, n, b->cfg, b->act,
+                               launched == 0 ? 1 : 0, b->type, b->tab_d, b->hist, b->ef, b->mu, b->upol_i);
+            induced_field(b, lo, b->mu, b->ef_stat, b->ef, b->act);
+            hipLaunchKernelGGL(kb_scf_rmsd, dim3(bdiv(n, 256)), dim3(256), 0, b->st, n, b->cfg, b->act, b->type,
+                               b->tab_d, b->mu, b->ef, b->rmsd_i);
+            segsum(b, b->rmsd_i, n, 1, 0, b->rmsd2);
+            hipLaunchKernelGGL(kb_scf_control, dim3(1), dim3(1024), 0, b->st, nc, b->conf, b->rmsd2, b->act,
+                               b->iscf, b->rmsd, maxit, minit, b->p.conv_tol_ind, b->any);
+            b->sweeps++;
         }
-    } else {
-        // defect correction: FP64 residual, FP32 correction sweeps
-        hipLaunchKernelGGL(kb_scf_mu, dim3(bdiv(n, 256)), dim3(256), 0, b->st, n, b->cfg, b->act, 1, b->type,
-                           b->tab_d, b->hist, b->ef, b->mu, b->upol_i);
-        while (launched < limit) {
-            int chunk = 2;
-            for (int it = 0; it < chunk && launched < limit; it++, launched++) {
-                induced_field(b, 0, b->mu, b->ef_stat, b->ef, b->act);
-                hipLaunchKernelGGL(kb_scf_rmsd, dim3(bdiv(n, 256)), dim3(256), 0, b->st, n, b->cfg, b->act, b->type,
-                                   b->tab_d, b->mu, b->ef, b->rmsd_i);
-                segsum(b, b->rmsd_i, n, 1, 0, b->rmsd2);
-                hipLaunchKernelGGL(kb_scf_control, dim3(1), dim3(1024), 0, b->st, nc, b->conf, b->rmsd2, b->act,
-                                   b->iscf, b->rmsd, maxit, minit, b->p.conv_tol_ind, b->any);
-                hipLaunchKernelGGL(kb_dc_residual, dim3(bdiv(n, 256)), dim3(256), 0, b->st, n, b->cfg, b->act, b->type,
-                                   b->tab_d, b->ef, b->mu, b->dc_R, b->dc_d);
-                for (int k = 0; k < b->inner_iters; k++) {
-                    induced_field(b, 1, b->dc_d, nullptr, b->dc_g, b->act);
-                    hipLaunchKernelGGL(kb_dc_update, dim3(bdiv(n, 256)), dim3(256), 0, b->st, n, b->cfg, b->act,
-                                       b->type, b->tab_d, b->dc_R, b->dc_g, b->dc_d);
-                }
-                hipLaunchKernelGGL(kb_axpy_gated, dim3(bdiv(3 * n, 256)), dim3(256), 0, b->st, n, b->cfg, b->act,
-                                   b->dc_d, b->mu);
-                b->sweeps += 1 + b->inner_iters;
-            }
-            int rc = read_any(b);
-            if (rc) return rc;
-            if (!*b->any_h) break;
-        }
-        hipLaunchKernelGGL(k_upol, dim3(bdiv(n, 256)), dim3(256), 0, b->st, n, b->type, b->tab_d, b->mu, b->upol_i);
+        int rc = read_any(b);
+        if (rc) return rc;
+        if (!*b->any_h) break;
     }
     // dipia_ind[0] = mu (pim.c:281-285): the next call starts here
     BCHK(hipMemcpyAsync(b->hist, b->mu, sizeof(double) * 3 * n, hipMemcpyDeviceToDevice, b->st));
@@ -494,7 +464,7 @@ void mdff_batch_destroy(mdff_batch *b)
     bfree(b->mu); bfree(b->mu0); bfree(b->hist); bfree(b->ef_stat); bfree(b->ef); bfree(b->ef_dir); bfree(b->ef_rec);
     bfree(b->efg_rec); bfree(b->efg_dummy); bfree(b->f); bfree(b->f_sr); bfree(b->f_dir); bfree(b->f_rec);
     bfree(b->acc_sr); bfree(b->uw_dir); bfree(b->selfb); bfree(b->upol_i); bfree(b->rmsd_i);
-    bfree(b->dc_R); bfree(b->dc_d); bfree(b->dc_g); bfree(b->nbr); bfree(b->cnt); bfree(b->ctl);
+    bfree(b->nbr); bfree(b->cnt); bfree(b->ctl);
     bfree(b->kv); bfree(b->W); bfree(b->ek); bfree(b->tab64); bfree(b->tab32); bfree(b->tabT64); bfree(b->tabT32); bfree(b->queue);
     bfree(b->items_rho); bfree(b->items_back); bfree(b->act); bfree(b->iscf); bfree(b->any);
     bfree(b->rmsd2); bfree(b->rmsd); bfree(b->seg); bfree(b->g); bfree(b->sdir); bfree(b->srhs); bfree(b->lam);
@@ -653,7 +623,6 @@ int mdff_batch_create(mdff_batch **out, const mdff_params *p, int nconf, const i
     BALLOC(b->f, n3); BALLOC(b->f_sr, n3); BALLOC(b->f_dir, n3); BALLOC(b->f_rec, n3);
     BALLOC(b->acc_sr, 14 * (size_t)n); BALLOC(b->uw_dir, 7 * (size_t)n); BALLOC(b->selfb, 2 * (size_t)n);
     BALLOC(b->upol_i, n); BALLOC(b->rmsd_i, n);
-    BALLOC(b->dc_R, n3); BALLOC(b->dc_d, n3); BALLOC(b->dc_g, n3);
     BALLOC(b->nbr, (size_t)n * b->cap); BALLOC(b->cnt, n); BALLOC(b->ctl, C_COUNT);
     BALLOC(b->kv, std::max<size_t>(1, b->nktot)); BALLOC(b->W, 2 * std::max<size_t>(1, b->nktot));
     BALLOC(b->ek, 7 * std::max<size_t>(1, b->nktot));

@@ -467,51 +467,6 @@ __global__ void kb_field_sum(int n, const int *__restrict__ cfg, const int *__re
     ef[i] = v;
 }
 
-// mixed-form SCF correction (defect correction, the FP64 residual drives it)
-__global__ void kb_dc_residual(int n, const int *__restrict__ cfg, const int *__restrict__ act,
-                               const int *__restrict__ type, const TypeTab *__restrict__ T,
-                               const double *__restrict__ ef, const double *__restrict__ mu, double *__restrict__ R,
-                               double *__restrict__ d)
-{
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n || !act[cfg[i]]) return;
-    int t = type[i];
-    for (int c = 0; c < 3; c++) {
-        double v = 0.0;
-        if (T->polar[t]) {
-            const double *P = T->pol[t];
-            v = P[0 * 3 + c] * ef[i] + P[1 * 3 + c] * ef[n + i] + P[2 * 3 + c] * ef[2 * n + i] - mu[(size_t)c * n + i];
-        }
-        R[(size_t)c * n + i] = v;
-        d[(size_t)c * n + i] = v;
-    }
-}
-
-__global__ void kb_dc_update(int n, const int *__restrict__ cfg, const int *__restrict__ act,
-                             const int *__restrict__ type, const TypeTab *__restrict__ T,
-                             const double *__restrict__ R, const double *__restrict__ g, double *__restrict__ d)
-{
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n || !act[cfg[i]]) return;
-    int t = type[i];
-    for (int c = 0; c < 3; c++) {
-        double v = 0.0;
-        if (T->polar[t]) {
-            const double *P = T->pol[t];
-            v = R[(size_t)c * n + i] + P[0 * 3 + c] * g[i] + P[1 * 3 + c] * g[n + i] + P[2 * 3 + c] * g[2 * n + i];
-        }
-        d[(size_t)c * n + i] = v;
-    }
-}
-
-__global__ void kb_axpy_gated(int n, const int *__restrict__ cfg, const int *__restrict__ act, const double *x,
-                              double *y)
-{
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= 3 * n || !act[cfg[i % n]]) return;
-    y[i] += x[i];
-}
-
 __global__ void kb_set_conf_int(int nconf, int *p, int v)
 {
     int c = blockIdx.x * blockDim.x + threadIdx.x;
