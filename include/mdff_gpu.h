@@ -142,12 +142,11 @@ void mdff_tt_damping(double b, double c, double r, double *f, double *fd, int or
 
 /* Tuning: lanes per ion in the neighbour kernels and the CG steps' pair-tensor
  * field (8, 16, 32 or 64, default 16; 0 keeps),
- * reciprocal-space kernel family katoms (-1 vector ALU; 1 the streaming
- * matrix-core sums, split-FP16 SCF steps in the mixed form, the default; 2
- * the same with FP32 SCF steps; 3 the older matrix-core kernels that form
- * whole operand tiles in LDS on every call; 4 as 1 with the forward's partial
- * sums, the S(k) reduction and the pair-tensor field of the CG steps in
- * separate kernels instead of fused; 0 keeps) and ksplit, the target
+ * reciprocal-space kernel family katoms (-1 vector ALU; 1 the matrix-core
+ * sums, split-FP16 SCF steps in the mixed form, the default; 2 the same with
+ * FP32 SCF steps; 4 as 1 with the forward's partial sums, the S(k) reduction
+ * and the pair-tensor field of the CG steps in separate kernels instead of
+ * fused; 0 keeps) and ksplit, the target
  * block count of the matrix-core kernels (0 keeps).  No counterpart in the
  * reference. */
 int mdff_set_tuning(mdff_ctx *ctx, int lanes, int katoms, int ksplit);

@@ -131,7 +131,6 @@ struct mdff_ctx {
     // tuning
     int lanes = 16;           // lanes per ion of the pair kernels and the pair-tensor field
     int use_mfma = 1;         // S(k) forward and back-projection on the matrix cores
-    int gemm = 1;             // matrix-core sums over stored operands (gemm_kernels.h)
     int g_cw[2] = {1, 2};     // column tiles per wave of the forward GEMM (FP64, FP32/BF16)
     int g_fblocks = 0;        // target blocks of the forward GEMM (0: one round of resident blocks)
     int g_bblocks = 0;        // target blocks of the back-projection GEMM (0: one round)
@@ -538,21 +537,6 @@ static void launch_back_t(mdff_ctx *c, const BackArgs &a, int nsel)
     hipLaunchKernelGGL((k_sk_back<real, NCH>), dim3(cdiv(nsel, 64), c->nsplit), dim3(64), sh, c->st, a);
 }
 
-// forward S(k) on the matrix cores (FP64, or split BF16 for the mixed form);
-// returns 0 when the shape is outside what the kernel is built for
-template <int MODE>
-static int fwd_mfma_t(mdff_ctx *c, SkMArgs &m, int nsel)
-{
-    constexpr int BC = 64 * MfTraits<MODE>::CW, SA = MfTraits<MODE>::SA;
-    int gx = (c->P + BC - 1) / BC;
-    int nchunk = std::max(1, std::min(32, (c->mf_blocks + gx - 1) / gx));
-    int chunk = ((nsel + nchunk - 1) / nchunk + SA - 1) / SA * SA;
-    nchunk = (nsel + chunk - 1) / chunk;
-    m.chunk = chunk;
-    hipLaunchKernelGGL(k_sk_fwd_mfma<MODE>, dim3(gx, nchunk), dim3(256), mf_fwd_lds_bytes<MODE>(), c->st, m);
-    return nchunk;
-}
-
 static int launch_fwd_gemm(mdff_ctx *c, int polar, int lo, int C, const double *q_w, const double *mu_w,
                            const int *active, int *nchunk_out);
 // a request to the back-projection for the pair-tensor field of mu at the
@@ -568,39 +552,15 @@ static int launch_back_gemm(mdff_ctx *c, int polar, int lo, int NCH, const int *
                             bool wm_built, T2Rows *t2r);
 static int wm_alloc(mdff_ctx *c, int mode);
 
+// forward S(k) on the matrix cores; 0 when the shape is outside the GEMM
+// kernels' tiles (or an operand buffer could not be allocated): the caller
+// then takes the vector-ALU sum
 static int launch_fwd_mfma(mdff_ctx *c, int polar, int lo, int C, const double *q_w, const double *mu_w,
                            const int *active, int *nchunk_out)
 {
     int M = (c->Kx + 1) * C * 2;
     if (!c->use_mfma || M > MF_MT * 16) return 0;
-    if (c->gemm && launch_fwd_gemm(c, polar, lo, C, q_w, mu_w, active, nchunk_out)) return 1;
-    int nsel = polar ? c->npol : c->n;
-    SkMArgs m;
-    m.nsel = nsel; m.Kx = c->Kx; m.Ky = c->Ky; m.Kz = c->Kz; m.P = c->P; m.C = C; m.M = M;
-    m.tx = c->tab3[lo][polar][0]; m.ty = c->tab3[lo][polar][1]; m.tz = c->tab3[lo][polar][2];
-    m.idx = polar ? c->pidx : nullptr;
-    m.q = q_w; m.mu = mu_w; m.n = c->n;
-    m.part = c->kpart;
-    m.active = active;
-    int mode = lo ? c->mf_lo : KM_F64;
-    // the operand-generating kernels have no split-FP16 form: FP32 instead
-    if (mode == KM_F64) *nchunk_out = fwd_mfma_t<KM_F64>(c, m, nsel);
-    else if (mode == KM_F32 || mode == KM_F16X3) *nchunk_out = fwd_mfma_t<KM_F32>(c, m, nsel);
-    else *nchunk_out = fwd_mfma_t<KM_BF16X3>(c, m, nsel);
-    return 1;
-}
-
-template <int MODE>
-static int back_mfma_t(mdff_ctx *c, BkMArgs &b, int nsel)
-{
-    constexpr int SP = MbTraits<MODE>::SP;
-    int gx = (nsel + 63) / 64;
-    int ns = std::max(1, std::min(c->nsplit_max, (c->mf_blocks + gx - 1) / gx));
-    int psplit = ((c->P + ns - 1) / ns + SP - 1) / SP * SP;
-    ns = (c->P + psplit - 1) / psplit;
-    b.psplit = psplit;
-    hipLaunchKernelGGL(k_sk_back_mfma<MODE>, dim3(gx, ns), dim3(256), mf_back_lds_bytes<MODE>(), c->st, b);
-    return ns;
+    return launch_fwd_gemm(c, polar, lo, C, q_w, mu_w, active, nchunk_out);
 }
 
 // back-projection on the matrix cores; returns 0 when outside the kernel's shapes
@@ -610,19 +570,7 @@ static int launch_back_mfma(mdff_ctx *c, int polar, int lo, int NCH, const int *
     int CH = NCH == 9 ? 6 : 3;
     int NC = (c->Kx + 1) * CH * 2;
     if (!c->use_mfma || NC > MB_CT * 16) return 0;
-    if (c->gemm && launch_back_gemm(c, polar, lo, NCH, active, nsplit_out, wm_built, t2r)) return 1;
-    int nsel = polar ? c->npol : c->n;
-    BkMArgs b;
-    b.nsel = nsel; b.Kx = c->Kx; b.Ky = c->Ky; b.Kz = c->Kz; b.P = c->P; b.CH = CH; b.NC = NC;
-    b.tx = c->tab3[lo][polar][0]; b.ty = c->tab3[lo][polar][1]; b.tz = c->tab3[lo][polar][2];
-    b.W = c->W;
-    b.part = c->bpart;
-    b.active = active;
-    int mode = lo ? c->mf_lo : KM_F64;
-    if (mode == KM_F64) *nsplit_out = back_mfma_t<KM_F64>(c, b, nsel);
-    else if (mode == KM_F32 || mode == KM_F16X3) *nsplit_out = back_mfma_t<KM_F32>(c, b, nsel);
-    else *nsplit_out = back_mfma_t<KM_BF16X3>(c, b, nsel);
-    return 1;
+    return launch_back_gemm(c, polar, lo, NCH, active, nsplit_out, wm_built, t2r);
 }
 
 // ------------------------------------------------ stored-operand matrix-core sums
@@ -1081,7 +1029,7 @@ static int rec_part(mdff_ctx *c, int polar, int lo, int C, int NCH, const double
     c->fwd_part = 0;
     // split FP16 with a back-projection: the reduction sums the forward's
     // partials itself (k_psum_reduce_cm)
-    const bool cm_path = (do_ef || do_f) && c->use_mfma && c->gemm && bmode == KM_F16X3 &&
+    const bool cm_path = (do_ef || do_f) && c->use_mfma && bmode == KM_F16X3 &&
                          (c->Kx + 1) * CHb * 2 <= MB_CT * 16 && wm_alloc(c, bmode) == MDFF_OK;
     c->psum_defer = cm_path && !c->no_fuse && (C == 1 || C == 3 || C == 4);
     c->psum_ch = CHb;
@@ -1120,7 +1068,7 @@ static int rec_part(mdff_ctx *c, int polar, int lo, int C, int NCH, const double
         hipLaunchKernelGGL(k_wm_scaled, dim3(cdiv(c->Pk, 256), (c->Kx + 1) * CHb), dim3(256), 0, c->st, c->P, c->Pk,
                            c->Ky, c->Kz, CHb, c->W, c->gcm, w0, w1, c->gsc + 16, active);
         wm_built = true;
-    } else if ((do_ef || do_f) && gemm_fwd && c->use_mfma && c->gemm && bmode != KM_F16X3 &&
+    } else if ((do_ef || do_f) && gemm_fwd && c->use_mfma && bmode != KM_F16X3 &&
         (c->Kx + 1) * CHb * 2 <= MB_CT * 16 && wm_alloc(c, bmode) == MDFF_OK) {
         void *w0 = c->wm[bmode][0], *w1 = c->wm[bmode][1];
         const dim3 gr(cdiv(c->nkf, 256));
@@ -2429,19 +2377,7 @@ int mdff_create(mdff_ctx **out, const mdff_params *p, int nion, const int *typia
 #undef ALLOC
     for (int i = 0; i < 8; i++)
         if (hipEventCreate(&c->ev[i]) != hipSuccess) { mdff_destroy(c); return MDFF_ERR_DEVICE; }
-    if (hipFuncSetAttribute((const void *)k_sk_fwd_mfma<KM_F64>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                            (int)mf_fwd_lds_bytes<KM_F64>()) != hipSuccess ||
-        hipFuncSetAttribute((const void *)k_sk_fwd_mfma<KM_BF16X3>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                            (int)mf_fwd_lds_bytes<KM_BF16X3>()) != hipSuccess ||
-        hipFuncSetAttribute((const void *)k_sk_back_mfma<KM_F64>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                            (int)mf_back_lds_bytes<KM_F64>()) != hipSuccess ||
-        hipFuncSetAttribute((const void *)k_sk_back_mfma<KM_BF16X3>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                            (int)mf_back_lds_bytes<KM_BF16X3>()) != hipSuccess ||
-        hipFuncSetAttribute((const void *)k_sk_fwd_mfma<KM_F32>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                            (int)mf_fwd_lds_bytes<KM_F32>()) != hipSuccess ||
-        hipFuncSetAttribute((const void *)k_sk_back_mfma<KM_F32>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                            (int)mf_back_lds_bytes<KM_F32>()) != hipSuccess ||
-        !g_back_attrs(c)) {
+    if (!g_back_attrs(c)) {
         mdff_destroy(c);
         return MDFF_ERR_DEVICE;
     }
@@ -2495,17 +2431,15 @@ int mdff_set_tuning(mdff_ctx *c, int lanes, int katoms, int ksplit)
     if (!c) return MDFF_ERR_ARG;
     if (lanes == 8 || lanes == 16 || lanes == 32 || lanes == 64) c->lanes = lanes;
     else if (lanes != 0) return MDFF_ERR_ARG;
-    // katoms: reciprocal-space kernel family.  -1 vector ALU; 1 the streaming
+    // katoms: reciprocal-space kernel family.  -1 vector ALU; 1 the
     // matrix-core sums with split-FP16 SCF steps; 2 the same with FP32 steps;
-    // 3 matrix cores with operands formed in LDS per call (split-BF16 steps);
     // 4 as 1 with the partial sums, reduction and pair-tensor finish in
     // separate kernels.
     if (katoms == 1 || katoms == 4) {
-        c->use_mfma = 1; c->gemm = 1; c->mf_inner = KM_F16X3;
+        c->use_mfma = 1; c->mf_inner = KM_F16X3;
         c->no_fuse = katoms == 4;
     }
-    else if (katoms == 2) { c->use_mfma = 1; c->gemm = 1; c->mf_inner = KM_F32; c->no_fuse = 0; }
-    else if (katoms == 3) { c->use_mfma = 1; c->gemm = 0; c->mf_inner = KM_BF16X3; }
+    else if (katoms == 2) { c->use_mfma = 1; c->mf_inner = KM_F32; c->no_fuse = 0; }
     else if (katoms == -1) c->use_mfma = 0;
     else if (katoms != 0) return MDFF_ERR_ARG;
     if (ksplit > 0) c->mf_blocks = ksplit;

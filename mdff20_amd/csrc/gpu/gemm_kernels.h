@@ -1110,7 +1110,7 @@ inline size_t g_back_lds_bytes(int CTINST, int KX1)
 }
 
 // back-projection: block = 8 waves x 16 ions, all CT column tiles, one K
-// range (grid.y); epilogue as k_sk_back_mfma (x ex[nx], moments).  Grid
+// range (grid.y); epilogue: x ex[nx] and the moments per ion.  Grid
 // rows from t2_y0 on run back_t2_rows instead (vector-memory work beside
 // the matrix-core blocks of the same launch).
 template <int MODE, int CTMAX, typename TT>
