@@ -91,6 +91,18 @@ struct mdff_batch {
     int *queue = nullptr;  // [2]: rho, [2]: back
     WItem *items_rho = nullptr, *items_back = nullptr;
     int nitems_rho = 0, nitems_back = 0;
+    // matrix-core k-space (kb_mf_*): every configuration's full k box, the
+    // box point -> k-vector map, the k-vector -> configuration map, the box
+    // sums per k-vector and W on the boxes; work items of the two sums
+    int mf = 0, cwf = 2;   // cwf: column tiles of a forward item
+    size_t nbox = 0;
+    int maxbox = 0;
+    std::vector<int> boxk_h;
+    int *boxk = nullptr, *kcfg = nullptr;
+    double *Sk = nullptr;
+    void *Wbox = nullptr;
+    WItem *items_mf = nullptr, *items_mb = nullptr;
+    int nitems_mf = 0, nitems_mb = 0;
     // per-configuration control and sums
     int *act = nullptr, *iscf = nullptr, *any = nullptr, *any_h = nullptr;
     double *rmsd2 = nullptr, *rmsd = nullptr, *seg = nullptr, *seg_h = nullptr;
@@ -127,6 +139,35 @@ static int build_items(mdff_batch *b)
     if (!ib.empty()) BCHK(hipMemcpy(b->items_back, ib.data(), sizeof(WItem) * ib.size(), hipMemcpyHostToDevice));
     b->nitems_rho = (int)ir.size();
     b->nitems_back = (int)ib.size();
+    if (b->mf) {
+        // largest work first: forward items ~ ions x row tiles, back items ~
+        // box points x column tiles
+        std::vector<std::pair<long long, WItem>> f, k;
+        for (int c = 0; c < b->nconf; c++) {
+            const BConfD &K = b->conf_h[c];
+            const long long P = (long long)(2 * K.Ky + 1) * (2 * K.Kz + 1);
+            const int cg = 16 * b->cwf;
+            for (int t = 0; t * cg < P; t++) f.push_back({-(long long)K.n * (K.Kx + 1), {c, t}});
+            for (int t = 0; t * 16 < K.n; t++) k.push_back({-P * (K.Kx + 1), {c, t}});
+        }
+        auto byw = [](const std::pair<long long, WItem> &x, const std::pair<long long, WItem> &y) {
+            return x.first != y.first ? x.first < y.first
+                                      : (x.second.conf != y.second.conf ? x.second.conf < y.second.conf
+                                                                        : x.second.tile < y.second.tile);
+        };
+        std::sort(f.begin(), f.end(), byw);
+        std::sort(k.begin(), k.end(), byw);
+        std::vector<WItem> fi, ki;
+        for (auto &x : f) fi.push_back(x.second);
+        for (auto &x : k) ki.push_back(x.second);
+        bfree(b->items_mf);
+        bfree(b->items_mb);
+        if ((rc = balloc(&b->items_mf, fi.size())) || (rc = balloc(&b->items_mb, ki.size()))) return rc;
+        if (!fi.empty()) BCHK(hipMemcpy(b->items_mf, fi.data(), sizeof(WItem) * fi.size(), hipMemcpyHostToDevice));
+        if (!ki.empty()) BCHK(hipMemcpy(b->items_mb, ki.data(), sizeof(WItem) * ki.size(), hipMemcpyHostToDevice));
+        b->nitems_mf = (int)fi.size();
+        b->nitems_mb = (int)ki.size();
+    }
     return MDFF_OK;
 }
 
@@ -149,8 +190,31 @@ static void rho_t(mdff_batch *b, RhoArgs &a)
 
 // rho(k), W and (when ek) the energy/stress terms of every active configuration
 static void launch_rho(mdff_batch *b, int lo, int mode, const double *q, const double *mu, const double *g,
-                       int do_ek, int do_stress, const int *act)
+                       int do_ek, int do_stress, const int *act, bool exact = false)
 {
+    if (b->mf && mode != RM_ADJ) {
+        // matrix-core sums over the k boxes, then rho, W, energy per k-vector.
+        // The mixed form's SCF fields (exact) keep FP64 arithmetic on its FP32
+        // tables: the field stays linear in the dipoles to FP64 rounding, as
+        // the Jacobi stop test at conv_tol_ind needs; its static field and
+        // force call use FP32 matrix instructions.
+        MFwdArgs f;
+        f.conf = b->conf;
+        f.tabT = lo ? (const void *)b->tabT32 : (const void *)b->tabT64;
+        f.by = b->by; f.bz = b->bz; f.n = b->n;
+        f.C = mode == RM_Q ? 1 : (mode == RM_MU ? 3 : 4);
+        f.q = q; f.mu = mu; f.boxk = b->boxk; f.items = b->items_mf; f.nitems = b->nitems_mf; f.Sk = b->Sk;
+        f.act = act;
+        const dim3 gf(bdiv(b->nitems_mf, 4));
+#define MFWD(TT, RT, MT) hipLaunchKernelGGL((kb_mf_fwd<TT, RT, MT, 2>), gf, dim3(256), 0, b->st, f)
+        if (!lo) { if (f.C == 1) MFWD(double, double, 2); else if (f.C == 3) MFWD(double, double, 6); else MFWD(double, double, 8); }
+        else if (exact) { if (f.C == 1) MFWD(float, double, 2); else if (f.C == 3) MFWD(float, double, 6); else MFWD(float, double, 8); }
+        else { if (f.C == 1) MFWD(float, float, 2); else if (f.C == 3) MFWD(float, float, 6); else MFWD(float, float, 8); }
+#undef MFWD
+        hipLaunchKernelGGL(kb_mf_rho, dim3(bdiv(b->nktot, 256)), dim3(256), 0, b->st, (int)b->nktot, b->kcfg, b->kv,
+                           b->Sk, f.C, b->W, do_ek ? b->ek : nullptr, do_stress, act);
+        return;
+    }
     RhoArgs a;
     a.conf = b->conf; a.kv = b->kv;
     a.tab = lo ? (const void *)b->tab32 : (const void *)b->tab64;
@@ -183,8 +247,29 @@ static void back_t(mdff_batch *b, BackBArgs &a)
 
 // field (and gradient, force) of W at every ion of the active configurations
 static void launch_back(mdff_batch *b, int lo, int nch, const double *q, const double *mu, double *ef, double *efg,
-                        double *f, const int *act)
+                        double *f, const int *act, bool exact = false)
 {
+    if (b->mf) {
+        // W on the boxes in the precision the sum reads (FP64 unless the FP32
+        // matrix instructions run), then the back-projection
+        const bool w32 = lo && !exact;
+        const dim3 gw(std::max(1, std::min(16, (int)bdiv(b->maxbox, 256))), b->nconf);
+        if (w32) hipLaunchKernelGGL(kb_mf_wbox<float>, gw, dim3(256), 0, b->st, b->conf, b->boxk, b->W, (float *)b->Wbox, act);
+        else hipLaunchKernelGGL(kb_mf_wbox<double>, gw, dim3(256), 0, b->st, b->conf, b->boxk, b->W, (double *)b->Wbox, act);
+        MBackArgs m;
+        m.conf = b->conf;
+        m.tabT = lo ? (const void *)b->tabT32 : (const void *)b->tabT64;
+        m.by = b->by; m.bz = b->bz; m.n = b->n; m.CH = nch == 9 ? 6 : 3;
+        m.Wbox = b->Wbox; m.items = b->items_mb; m.nitems = b->nitems_mb;
+        m.q = q; m.mu = mu; m.ef = ef; m.efg = efg; m.f = f; m.act = act;
+        const dim3 gb(bdiv(b->nitems_mb, 4));
+#define MBACK(TT, WT, RT, CT) hipLaunchKernelGGL((kb_mf_back<TT, WT, RT, CT>), gb, dim3(256), 0, b->st, m)
+        if (!lo) { if (m.CH == 3) MBACK(double, double, double, 6); else MBACK(double, double, double, 12); }
+        else if (exact) { if (m.CH == 3) MBACK(float, double, double, 6); else MBACK(float, double, double, 12); }
+        else { if (m.CH == 3) MBACK(float, float, float, 6); else MBACK(float, float, float, 12); }
+#undef MBACK
+        return;
+    }
     BackBArgs a;
     a.conf = b->conf; a.kv = b->kv;
     a.tab = lo ? (const void *)b->tabT32 : (const void *)b->tabT64;
@@ -276,8 +361,8 @@ static void induced_field(mdff_batch *b, int lo, const double *mu, const double 
     int n = b->n;
     DirArgs a = bdir_args(b, b->qz, mu, 0, 1, act);
     launch_bdir(b, a, lo, 0);
-    launch_rho(b, lo, RM_MU, b->qz, mu, nullptr, 0, 0, act);
-    launch_back(b, lo, 3, nullptr, nullptr, b->ef_rec, nullptr, nullptr, act);
+    launch_rho(b, lo, RM_MU, b->qz, mu, nullptr, 0, 0, act, true);
+    launch_back(b, lo, 3, nullptr, nullptr, b->ef_rec, nullptr, nullptr, act, true);
     hipLaunchKernelGGL(kb_field_sum, dim3(bdiv(3 * n, 256)), dim3(256), 0, b->st, n, b->cfg, act, b->conf, mu,
                        b->ef_dir, b->ef_rec, base, ef);
 }
@@ -469,6 +554,9 @@ void mdff_batch_destroy(mdff_batch *b)
     bfree(b->items_rho); bfree(b->items_back); bfree(b->act); bfree(b->iscf); bfree(b->any);
     bfree(b->rmsd2); bfree(b->rmsd); bfree(b->seg); bfree(b->g); bfree(b->sdir); bfree(b->srhs); bfree(b->lam);
     bfree(b->eT); bfree(b->edel); bfree(b->cols); bfree(b->tsum);
+    bfree(b->boxk); bfree(b->kcfg); bfree(b->Sk); bfree(b->items_mf); bfree(b->items_mb);
+    if (b->Wbox) (void)hipFree(b->Wbox);
+    b->Wbox = nullptr;
     if (b->any_h) (void)hipHostFree(b->any_h);
     if (b->seg_h) free(b->seg_h);
     for (int i = 0; i < 4; i++)
@@ -571,11 +659,17 @@ int mdff_batch_create(mdff_batch **out, const mdff_params *p, int nconf, const i
         K.npol = np;
         K.Kx = kes[0]; K.Ky = kes[1]; K.Kz = kes[2];
         K.koff = (int)kvh.size();
+        K.boxoff = (int)b->nbox;
         if (p->lcoulombic) {
             double a2 = alpha * alpha;
+            const size_t box = (size_t)(K.Kx + 1) * (2 * K.Ky + 1) * (2 * K.Kz + 1);
+            b->boxk_h.resize(b->nbox + box, -1);
+            b->nbox += box;
+            b->maxbox = std::max(b->maxbox, (int)box);
+            size_t e = K.boxoff;
             for (int nx = 0; nx <= K.Kx; nx++)
                 for (int ny = -K.Ky; ny <= K.Ky; ny++)
-                    for (int nz = -K.Kz; nz <= K.Kz; nz++) {
+                    for (int nz = -K.Kz; nz <= K.Kz; nz++, e++) {
                         bool in_half = (nx > 0) || (nx == 0 && ny > 0) || (nx == 0 && ny == 0 && nz > 0);
                         if (!in_half) continue;
                         const double(*B)[3] = cell.B;
@@ -588,6 +682,7 @@ int mdff_batch_create(mdff_batch **out, const mdff_params *p, int nconf, const i
                         v.Ak = exp(-kk * 0.25 / a2) / kk;
                         v.kcoe = 2.0 * (1.0 / kk + 1.0 / a2 / 4.0);
                         v.nx = nx; v.ny = ny; v.nz = nz; v.pad = 0;
+                        b->boxk_h[e] = (int)kvh.size() - K.koff;
                         kvh.push_back(v);
                     }
         }
@@ -626,6 +721,20 @@ int mdff_batch_create(mdff_batch **out, const mdff_params *p, int nconf, const i
     BALLOC(b->nbr, (size_t)n * b->cap); BALLOC(b->cnt, n); BALLOC(b->ctl, C_COUNT);
     BALLOC(b->kv, std::max<size_t>(1, b->nktot)); BALLOC(b->W, 2 * std::max<size_t>(1, b->nktot));
     BALLOC(b->ek, 7 * std::max<size_t>(1, b->nktot));
+    // matrix-core k-space when every box fits the kernels' tiles:
+    // (Kx + 1) 4 2 <= 16 BMF_MT rows and (Kx + 1) 6 2 <= 16 BMF_CT columns
+    b->mf = p->lcoulombic && b->nktot > 0 && b->nbox > 0;
+    for (int c = 0; c < nconf && b->mf; c++)
+        if (b->conf_h[c].Kx + 1 > 16) b->mf = 0;
+    b->cwf = 2;
+    if (b->mf) {
+        BALLOC(b->boxk, b->nbox); BALLOC(b->kcfg, b->nktot); BALLOC(b->Sk, 8 * b->nktot);
+        if (hipMalloc(&b->Wbox, b->nbox * 2 * sizeof(double)) != hipSuccess) {
+            b->Wbox = nullptr;
+            mdff_batch_destroy(b);
+            return MDFF_ERR_NOMEM;
+        }
+    }
     BALLOC(b->tab64, 2 * (size_t)n * b->TS);
     BALLOC(b->tabT64, 2 * (size_t)n * b->TS);
     if (precision == MDFF_MIXED) {
@@ -658,6 +767,16 @@ int mdff_batch_create(mdff_batch **out, const mdff_params *p, int nconf, const i
         (b->nktot && hipMemcpy(b->kv, kvh.data(), sizeof(KVec) * b->nktot, hipMemcpyHostToDevice) != hipSuccess)) {
         mdff_batch_destroy(b);
         return MDFF_ERR_DEVICE;
+    }
+    if (b->mf) {
+        std::vector<int> kc(b->nktot);
+        for (int c = 0; c < nconf; c++)
+            for (int k = 0; k < b->conf_h[c].nk; k++) kc[(size_t)b->conf_h[c].koff + k] = c;
+        if (hipMemcpy(b->boxk, b->boxk_h.data(), sizeof(int) * b->nbox, hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(b->kcfg, kc.data(), sizeof(int) * b->nktot, hipMemcpyHostToDevice) != hipSuccess) {
+            mdff_batch_destroy(b);
+            return MDFF_ERR_DEVICE;
+        }
     }
     if ((rc = build_items(b)) != MDFF_OK) { mdff_batch_destroy(b); return rc; }
     // every zero fill of the set-up (null stream) is done before the batch stream runs

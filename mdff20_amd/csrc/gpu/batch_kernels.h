@@ -378,6 +378,431 @@ __global__ void __launch_bounds__(256) kb_back(BackBArgs a)
     }
 }
 
+// ------------------------------------------- matrix-core k-space sums
+// The same sums as kb_rho / kb_back, recast per configuration as dense
+// GEMMs over its full k box (nx = 0..Kx, ny = -Ky..Ky, nz = -Kz..Kz; box
+// point p = (ny + Ky) NZ + (nz + Kz), NZ = 2 Kz + 1) on the separable
+// tables e^{ik.r} = ex[nx] ey[ny] ez[nz] (ewald.c:876-1051 semantics):
+//   forward  S_c(nx, p) = sum_j w_cj ex_j[nx] (ey ez)_j(p): rows (nx, c,
+//            re/im), columns p, K = (ion, re/im);
+//   back     T_m(j, nx) = sum_p (ey ez)_j(p) W(nx, p) mom_m(p): rows ions,
+//            columns (nx, m, re/im), K = (point, re/im), moments
+//            mom = {1, ny, nz, ny^2, ny nz, nz^2}; then ex_j[nx] and the nx
+//            factors per ion.
+// Complex products are embedded as real GEMMs: for a = ar + i ai times
+// b = br + i bi over K = (re, im), the row (re) takes (ar, -ai) and the row
+// (im) takes (ai, ar).  One wave computes one item: every row tile of a
+// configuration's column group (forward) or one 16-ion row tile against
+// every column tile (back), over the whole K range, so no partial sums.
+// v_mfma_f64_16x16x4f64 (FP64 form) and v_mfma_f32_16x16x4f32 (mixed form):
+// A[l % 16][k], B[k][l % 16] with k = l / 16.
+
+template <typename real>
+struct BMF;
+template <>
+struct BMF<double> {
+    typedef double v4 __attribute__((ext_vector_type(4)));
+    static __device__ __forceinline__ v4 mma(double a, double b, v4 c)
+    {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+    // row of accumulator element e held by lane l (whose column is l % 16)
+    static __device__ __forceinline__ int row(int l, int e) { return (l >> 4) + 4 * e; }
+};
+template <>
+struct BMF<float> {
+    typedef float v4 __attribute__((ext_vector_type(4)));
+    static __device__ __forceinline__ v4 mma(float a, float b, v4 c)
+    {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int row(int l, int e) { return 4 * (l >> 4) + e; }
+};
+
+#define BMF_MT 8    // row tiles of the forward sum: (Kx + 1) C 2 <= 16 BMF_MT
+#define BMF_CT 12   // column tiles of the back-projection: (Kx + 1) 6 2 <= 16 BMF_CT
+
+struct MFwdArgs {
+    const BConfD *conf;
+    const void *tabT;          // [TS][n] complex (TT), column m of ion i at m n + i
+    int by, bz, n, C;          // C: 1 (q), 3 (mu), 4 (q, mu)
+    const double *q, *mu;
+    const int *boxk;           // [box point (nx, p)] -> k index within the configuration, or -1
+    const WItem *items;        // (configuration, column group of CW tiles)
+    int nitems;
+    double *Sk;                // [nktot][C][re, im]
+    const int *act;
+};
+
+// TT: table type; real: arithmetic of the matrix instructions; MT row tiles
+// (at least the launch's largest), CW column tiles per item.  The operands
+// of step s + 1 are loaded while step s computes.
+template <typename TT, typename real, int MT, int CW>
+__global__ void __launch_bounds__(256) kb_mf_fwd(MFwdArgs a)
+{
+    using M = BMF<real>;
+    using v4 = typename M::v4;
+    typedef TT t2 __attribute__((ext_vector_type(2)));
+    const int lane = threadIdx.x & 63;
+    const int it = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (it >= a.nitems) return;
+    const WItem w = a.items[it];
+    if (!conf_active(a.act, w.conf)) return;
+    const BConfD &K = a.conf[w.conf];
+    const int g = lane >> 4, cl = lane & 15;
+    const int C = a.C, NZ = 2 * K.Kz + 1, P = (2 * K.Ky + 1) * NZ;
+    const int R = (K.Kx + 1) * C * 2, MTc = (R + 15) >> 4;
+    const int n = a.n;
+    const t2 *T = (const t2 *)a.tabT;
+    // the lane's A rows (nx, c, re/im) and B columns (ny, nz); rows and
+    // columns past the box repeat the last one (their results are dropped)
+    int xo[MT], wc[MT];
+    bool rim[MT];
+#pragma unroll
+    for (int t = 0; t < MT; t++) {
+        const int r = min(t * 16 + cl, R - 1);
+        xo[t] = (r / (2 * C)) * n;
+        const int c = (r >> 1) % C;
+        wc[t] = C == 4 ? c : (C == 3 ? c + 1 : 0);   // weight: 0 q, 1..3 mu
+        rim[t] = r & 1;
+    }
+    int yo[CW], zo[CW];
+#pragma unroll
+    for (int c = 0; c < CW; c++) {
+        const int p = min((w.tile * CW + c) * 16 + cl, P - 1);
+        const int iy = p / NZ;
+        yo[c] = (a.by - K.Ky + iy) * n;
+        zo[c] = (a.bz - K.Kz + (p - iy * NZ)) * n;
+    }
+    v4 acc[MT][CW];
+#pragma unroll
+    for (int t = 0; t < MT; t++)
+#pragma unroll
+        for (int c = 0; c < CW; c++) acc[t][c] = v4{0, 0, 0, 0};
+    const bool kim = g & 1;   // the lane's K component: im
+    const int ns = (K.n + 1) >> 1;
+    struct Set {
+        t2 Y[CW], Z[CW], X[MT];
+        double w[4];
+    };
+    // loads of step s (ions past the configuration repeat its last ion)
+    auto load = [&](Set &S, int s) {
+        const int jl = min(2 * s + (g >> 1), K.n - 1);
+        const int j = K.off + jl;
+#pragma unroll
+        for (int c = 0; c < CW; c++) {
+            S.Y[c] = T[yo[c] + j];
+            S.Z[c] = T[zo[c] + j];
+        }
+#pragma unroll
+        for (int t = 0; t < MT; t++) S.X[t] = T[xo[t] + j];
+        S.w[0] = C != 3 ? a.q[j] : 0.0;
+        S.w[1] = C >= 3 ? a.mu[j] : 0.0;
+        S.w[2] = C >= 3 ? a.mu[n + j] : 0.0;
+        S.w[3] = C >= 3 ? a.mu[2 * (size_t)n + j] : 0.0;
+    };
+    auto compute = [&](const Set &S, int s) {
+        const bool jok = 2 * s + (g >> 1) < K.n;
+        real bv[CW];
+#pragma unroll
+        for (int c = 0; c < CW; c++) {
+            const real yr = (real)S.Y[c].x, yi = (real)S.Y[c].y, zr = (real)S.Z[c].x, zi = (real)S.Z[c].y;
+            const real re = yr * zr - yi * zi, im = yr * zi + yi * zr;
+            bv[c] = jok ? (kim ? im : re) : (real)0;
+        }
+#pragma unroll
+        for (int t = 0; t < MT; t++) {
+            if (t < MTc) {
+                const int c = wc[t];
+                const real wv = (real)(c == 0 ? S.w[0] : (c == 1 ? S.w[1] : (c == 2 ? S.w[2] : S.w[3])));
+                const real ar = wv * (real)S.X[t].x, ai = wv * (real)S.X[t].y;
+                const real av = rim[t] ? (kim ? ar : ai) : (kim ? -ai : ar);
+#pragma unroll
+                for (int c2 = 0; c2 < CW; c2++) acc[t][c2] = M::mma(av, bv[c2], acc[t][c2]);
+            }
+        }
+    };
+    Set S0, S1;
+    load(S0, 0);
+    int s = 0;
+    for (; s + 1 < ns; s += 2) {
+        load(S1, s + 1);
+        compute(S0, s);
+        load(S0, min(s + 2, ns - 1));
+        compute(S1, s + 1);
+    }
+    if (s < ns) compute(S0, s);
+    // S values of the box points that are half-space k-vectors
+    const int *bk = a.boxk + K.boxoff;
+#pragma unroll
+    for (int t = 0; t < MT; t++) {
+        if (t >= MTc) break;
+#pragma unroll
+        for (int c = 0; c < CW; c++) {
+            const int p = (w.tile * CW + c) * 16 + cl;
+            if (p >= P) continue;
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                const int r = t * 16 + M::row(lane, e);
+                if (r >= R) continue;
+                const int k = bk[(r / (2 * C)) * P + p];
+                if (k >= 0) a.Sk[((size_t)(K.koff + k) * C + ((r >> 1) % C)) * 2 + (r & 1)] = (double)acc[t][c][e];
+            }
+        }
+    }
+}
+
+// rho(k) of every half-space k-vector from the box sums, then W = Ak
+// conj(rho), Ak |rho|^2 and the stress terms as kb_rho (C = 1: RM_Q, 3:
+// RM_MU, 4: RM_QMU)
+__global__ void __launch_bounds__(256) kb_mf_rho(int nktot, const int *__restrict__ kcfg,
+                                                 const KVec *__restrict__ kv, const double *__restrict__ Sk, int C,
+                                                 double *__restrict__ W, double *__restrict__ ek, int do_stress,
+                                                 const int *__restrict__ act)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nktot) return;
+    if (!conf_active(act, kcfg[k])) return;
+    const KVec v = kv[k];
+    const double *s = Sk + (size_t)k * C * 2;
+    double RR, RI;
+    if (C == 1) {
+        RR = s[0];
+        RI = s[1];
+    } else if (C == 3) {
+        RR = -(v.kx * s[1] + v.ky * s[3] + v.kz * s[5]);
+        RI = v.kx * s[0] + v.ky * s[2] + v.kz * s[4];
+    } else {
+        RR = s[0] - (v.kx * s[3] + v.ky * s[5] + v.kz * s[7]);
+        RI = s[1] + (v.kx * s[2] + v.ky * s[4] + v.kz * s[6]);
+    }
+    W[2 * (size_t)k] = v.Ak * RR;
+    W[2 * (size_t)k + 1] = -v.Ak * RI;
+    if (ek) {
+        const size_t nk = (size_t)nktot;
+        const double str = (RR * RR + RI * RI) * v.Ak;
+        ek[k] = str;
+        const double kc = v.kcoe;
+        double v1 = 0, v2 = 0, v3 = 0, v4 = 0, v5 = 0, v6 = 0;
+        if (do_stress) {
+            v1 = (1.0 - kc * v.kx * v.kx) * str; v2 = -kc * v.kx * v.ky * str;
+            v3 = -kc * v.kx * v.kz * str; v4 = (1.0 - kc * v.ky * v.ky) * str;
+            v5 = -kc * v.ky * v.kz * str; v6 = (1.0 - kc * v.kz * v.kz) * str;
+        }
+        ek[nk + k] = v1; ek[2 * nk + k] = v2; ek[3 * nk + k] = v3;
+        ek[4 * nk + k] = v4; ek[5 * nk + k] = v5; ek[6 * nk + k] = v6;
+    }
+}
+
+// W on every configuration's box (zero off the half-space list), in the
+// precision of the back-projection; grid (chunks, configurations)
+template <typename real>
+__global__ void __launch_bounds__(256) kb_mf_wbox(const BConfD *__restrict__ conf, const int *__restrict__ boxk,
+                                                  const double *__restrict__ W, real *__restrict__ Wbox,
+                                                  const int *__restrict__ act)
+{
+    const int c = blockIdx.y;
+    if (!conf_active(act, c)) return;
+    const BConfD &K = conf[c];
+    const int box = (K.Kx + 1) * (2 * K.Ky + 1) * (2 * K.Kz + 1);
+    typedef real r2 __attribute__((ext_vector_type(2)));
+    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < box; e += gridDim.x * blockDim.x) {
+        const int k = boxk[K.boxoff + e];
+        r2 v = {0, 0};
+        if (k >= 0) {
+            v.x = (real)W[2 * ((size_t)K.koff + k)];
+            v.y = (real)W[2 * ((size_t)K.koff + k) + 1];
+        }
+        ((r2 *)Wbox)[K.boxoff + e] = v;
+    }
+}
+
+struct MBackArgs {
+    const BConfD *conf;
+    const void *tabT;          // [TS][n] complex (TT)
+    int by, bz, n, CH;         // CH: 3 moments (field) or 6 (field and gradient)
+    const void *Wbox;          // [box point] complex (WT)
+    const WItem *items;        // (configuration, 16-ion row tile)
+    int nitems;
+    const double *q, *mu;
+    double *ef, *efg, *f;      // as kb_back
+    const int *act;
+};
+
+// TT: table type; WT: type of W on the box; real: arithmetic of the matrix
+// instructions; CT column tiles (at least the launch's largest).  The
+// operands of step s + 1 are loaded while step s computes.
+template <typename TT, typename WT, typename real, int CT>
+__global__ void __launch_bounds__(256) kb_mf_back(MBackArgs a)
+{
+    using M = BMF<real>;
+    using v4 = typename M::v4;
+    typedef TT t2 __attribute__((ext_vector_type(2)));
+    typedef WT w2 __attribute__((ext_vector_type(2)));
+    const int lane = threadIdx.x & 63;
+    const int it = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (it >= a.nitems) return;
+    const WItem w = a.items[it];
+    if (!conf_active(a.act, w.conf)) return;
+    const BConfD &K = a.conf[w.conf];
+    const int g = lane >> 4, cl = lane & 15;
+    const int CH = a.CH, NZ = 2 * K.Kz + 1, P = (2 * K.Ky + 1) * NZ;
+    const int NC = (K.Kx + 1) * CH * 2, CTc = (NC + 15) >> 4;
+    const int n = a.n;
+    const t2 *T = (const t2 *)a.tabT;
+    const w2 *Wb = (const w2 *)a.Wbox + K.boxoff;
+    // A row: the lane's ion; B columns (nx, m, re/im) of each tile
+    const int il = w.tile * 16 + cl;
+    const int i = K.off + (il < K.n ? il : K.n - 1);
+    int wo[CT], mm[CT];
+    bool cim[CT];
+#pragma unroll
+    for (int t = 0; t < CT; t++) {
+        const int col = min(t * 16 + cl, NC - 1);
+        wo[t] = (col / (2 * CH)) * P;
+        mm[t] = (col >> 1) % CH;
+        cim[t] = col & 1;
+    }
+    v4 acc[CT];
+#pragma unroll
+    for (int t = 0; t < CT; t++) acc[t] = v4{0, 0, 0, 0};
+    const bool kim = g & 1;
+    const int ns = (P + 1) >> 1;
+    struct Set {
+        t2 Y, Z;
+        w2 Wv[CT];
+        int ny, nz;
+        bool pok;
+    };
+    // the lane's point p = 2 s + g / 2 as (iy, iz), advanced by 2 per load
+    int lp = g >> 1, liy = 0, liz = lp;
+    auto load = [&](Set &S) {
+        S.pok = lp < P;
+        const int iy = S.pok ? liy : 2 * K.Ky, iz = S.pok ? liz : 2 * K.Kz;
+        const int pc = S.pok ? lp : P - 1;
+        S.ny = iy - K.Ky;
+        S.nz = iz - K.Kz;
+        S.Y = T[(a.by + S.ny) * n + i];
+        S.Z = T[(a.bz + S.nz) * n + i];
+#pragma unroll
+        for (int t = 0; t < CT; t++) S.Wv[t] = Wb[wo[t] + pc];
+        lp += 2;
+        liz += 2;
+        while (liz >= NZ) { liz -= NZ; liy++; }
+    };
+    auto compute = [&](const Set &S) {
+        const real yr = (real)S.Y.x, yi = (real)S.Y.y, zr = (real)S.Z.x, zi = (real)S.Z.y;
+        const real av = kim ? yr * zi + yi * zr : yr * zr - yi * zi;
+        const real fy = (real)S.ny, fz = (real)S.nz;
+#pragma unroll
+        for (int t = 0; t < CT; t++) {
+            if (t < CTc) {
+                const int m = mm[t];
+                const real mom = m == 0 ? (real)1 : (m == 1 ? fy : (m == 2 ? fz : (m == 3 ? fy * fy : (m == 4 ? fy * fz : fz * fz))));
+                const real wr = mom * (real)S.Wv[t].x, wi = mom * (real)S.Wv[t].y;
+                real bv = cim[t] ? (kim ? wr : wi) : (kim ? -wi : wr);
+                bv = S.pok ? bv : (real)0;
+                acc[t] = M::mma(av, bv, acc[t]);
+            }
+        }
+    };
+    Set S0, S1;
+    load(S0);
+    int s = 0;
+    for (; s + 1 < ns; s += 2) {
+        load(S1);
+        compute(S0);
+        load(S0);
+        compute(S1);
+    }
+    if (s < ns) compute(S0);
+    // epilogue: lanes of even columns hold T_re, their neighbours T_im;
+    // z = ex[nx] T per (ion, nx, m), summed with the nx factors into
+    // M_b = sum_k n_b Im(W e^{ik.r}) and N_bd = sum_k n_b n_d Re(W e^{ik.r})
+    double o[4][9];
+#pragma unroll
+    for (int e = 0; e < 4; e++)
+#pragma unroll
+        for (int c = 0; c < 9; c++) o[e][c] = 0.0;
+#pragma unroll
+    for (int t = 0; t < CT; t++) {
+        if (t >= CTc) break;
+        const int col = t * 16 + cl;
+        const bool use = col < NC && !(cl & 1);
+        const int nx = col / (2 * CH), m = (col >> 1) % CH;
+        const double fx = (double)nx;
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+            const double tr = (double)acc[t][e];
+            const double ti = __shfl_xor(tr, 1, 64);
+            const int rl = w.tile * 16 + M::row(lane, e);
+            const int ion = K.off + (rl < K.n ? rl : K.n - 1);
+            if (use) {
+                const t2 X = T[nx * n + ion];
+                const double xr = (double)X.x, xi = (double)X.y;
+                const double zr = xr * tr - xi * ti, zi = xr * ti + xi * tr;
+                o[e][0] += m == 0 ? fx * zi : 0.0;
+                o[e][1] += m == 1 ? zi : 0.0;
+                o[e][2] += m == 2 ? zi : 0.0;
+                o[e][3] += m == 0 ? fx * fx * zr : 0.0;
+                o[e][4] += m == 1 ? fx * zr : 0.0;
+                o[e][5] += m == 2 ? fx * zr : 0.0;
+                o[e][6] += m == 3 ? zr : 0.0;
+                o[e][7] += m == 4 ? zr : 0.0;
+                o[e][8] += m == 5 ? zr : 0.0;
+            }
+        }
+    }
+    const int nout = CH == 6 ? 9 : 3;
+#pragma unroll
+    for (int e = 0; e < 4; e++)
+#pragma unroll
+        for (int c = 0; c < 9; c++) {
+            if (c >= nout) break;
+            double v = o[e][c];
+            v += __shfl_xor(v, 1, 64);
+            v += __shfl_xor(v, 2, 64);
+            v += __shfl_xor(v, 4, 64);
+            v += __shfl_xor(v, 8, 64);
+            o[e][c] = v;
+        }
+    // lane cl = e < 4 of each lane group writes accumulator row e
+#pragma unroll
+    for (int e = 0; e < 4; e++) {
+        if (cl != e) continue;
+        const int rl = w.tile * 16 + M::row(lane, e);
+        if (rl >= K.n) continue;
+        const int ion = K.off + rl;
+        const double(*B)[3] = K.c.B;
+        const double s1 = MDFF_TPI * K.tfpiV, s2 = MDFF_TPI * MDFF_TPI * K.tfpiV;
+        double v[9];
+        for (int x = 0; x < 3; x++) v[x] = s1 * (B[x][0] * o[e][0] + B[x][1] * o[e][1] + B[x][2] * o[e][2]);
+        if (CH == 6) {
+            const double Nm[3][3] = {{o[e][3], o[e][4], o[e][5]}, {o[e][4], o[e][6], o[e][7]}, {o[e][5], o[e][7], o[e][8]}};
+            double BN[3][3];
+            for (int x = 0; x < 3; x++)
+                for (int d = 0; d < 3; d++) BN[x][d] = B[x][0] * Nm[0][d] + B[x][1] * Nm[1][d] + B[x][2] * Nm[2][d];
+            const int px[6] = {0, 0, 0, 1, 1, 2}, py[6] = {0, 1, 2, 1, 2, 2};
+            for (int c = 0; c < 6; c++)
+                v[3 + c] = s2 * (BN[px[c]][0] * B[py[c]][0] + BN[px[c]][1] * B[py[c]][1] + BN[px[c]][2] * B[py[c]][2]);
+        }
+        if (a.ef) { a.ef[ion] = v[0]; a.ef[n + ion] = v[1]; a.ef[2 * (size_t)n + ion] = v[2]; }
+        if (CH == 6 && a.efg)
+            for (int c = 0; c < 6; c++) a.efg[(size_t)c * n + ion] = v[3 + c];
+        if (a.f) {
+            const double qi = a.q[ion];
+            double fx = qi * v[0], fy = qi * v[1], fz = qi * v[2];
+            if (CH == 6) {
+                const double m0 = a.mu[ion], m1 = a.mu[n + ion], m2 = a.mu[2 * (size_t)n + ion];
+                fx += v[3] * m0 + v[4] * m1 + v[5] * m2;
+                fy += v[4] * m0 + v[6] * m1 + v[7] * m2;
+                fz += v[5] * m0 + v[7] * m1 + v[8] * m2;
+            }
+            a.f[ion] = fx; a.f[n + ion] = fy; a.f[2 * (size_t)n + ion] = fz;
+        }
+    }
+}
+
 // ------------------------------------------------------- SCF, per config
 // momentpolaSCF pim.c:236-278 per configuration: iteration 0 takes the
 // predictor (the last converged dipoles, extrapolate_dipole_aspc at order 0

@@ -27,6 +27,7 @@ struct BConfD {
     double iv;                            // 1 / (Omega press_unit)
     int off, n, npol;                     // ions [off, off + n), polarisable count
     int Kx, Ky, Kz, koff, nk;             // k box and half-space k-vectors [koff, koff + nk)
+    int boxoff;                           // offset of the full (nx >= 0) k box in the box-ordered arrays
 };
 
 // per-type and per-type-pair tables, device resident
