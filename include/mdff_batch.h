@@ -9,9 +9,11 @@
  * static configuration (field.c:352-407: engforce_bhmftd_pbc / nmlj,
  * get_dipoles -> momentpolaSCF from zero dipoles, multipole_ES), with its
  * own cell, Ewald alpha, k box and cutoffs.  Every phase of a force call is
- * one launch over the whole batch; the reciprocal-space phases run as
- * persistent grids that pull (configuration, tile) work items from a device
- * queue, with tile shapes chosen by mdff_batch_autotune.
+ * one launch over the whole batch (per run of configurations with the same
+ * k-box tile count); the reciprocal-space sums are matrix-core GEMMs over
+ * each configuration's k box, one (configuration, tile) work item per wave,
+ * largest first, with tile shapes chosen by mdff_batch_autotune (k boxes
+ * wider than 16 in x take the vector-ALU sums).
  *
  * Loss of mdff_batch_adjoint: L = sum over configurations and ions of
  * |F_i - F_ref,i|^2 (forces in eV/A).  Its gradient with respect to the
@@ -96,8 +98,10 @@ int mdff_batch_adjoint(mdff_batch *b, const double *fref_x, const double *fref_y
 
 /* Autotuner: times the candidate shapes of the batch kernels on this
  * batch and keeps the fastest.  out[4] = lanes per ion of the real-space
- * kernels, k-vectors per block of the structure-factor kernel, ions per
- * block of the back-projection, persistent blocks per CU. */
+ * kernels, box points (matrix-core sums: 32) or k-vectors (vector sums) per
+ * structure-factor item, ions per back-projection item (matrix-core: 16 or
+ * 32; vector: 64, 128 or 256), persistent blocks per CU of the vector sums.
+ * mdff_batch_set_tuning sets the same (0 keeps a value). */
 int mdff_batch_autotune(mdff_batch *b, int out[4]);
 int mdff_batch_set_tuning(mdff_batch *b, int lanes, int kpb, int ipb, int bpc);
 

@@ -94,15 +94,17 @@ struct mdff_batch {
     // matrix-core k-space (kb_mf_*): every configuration's full k box, the
     // box point -> k-vector map, the k-vector -> configuration map, the box
     // sums per k-vector and W on the boxes; work items of the two sums
-    int mf = 0, cwf = 2;   // cwf: column tiles of a forward item
-    size_t nbox = 0;
-    int maxbox = 0;
+    int mf = 0, mfr = 1;   // mfr: row tiles of a back-projection item (1 or 2)
+    size_t nbox = 0, naw = 0, nwm = 0;
     std::vector<int> boxk_h;
     int *boxk = nullptr, *kcfg = nullptr;
     double *Sk = nullptr;
-    void *Wbox = nullptr;
+    double *aw = nullptr, *wm = nullptr;   // operand blocks (FP64, or FP32 in their first half)
     WItem *items_mf = nullptr, *items_mb = nullptr;
     int nitems_mf = 0, nitems_mb = 0;
+    // items sorted by Kx (descending), then work: [begin, end) of each Kx
+    std::vector<std::pair<int, int>> rng_mf, rng_mb;
+    std::vector<int> kx_mf, kx_mb;
     // per-configuration control and sums
     int *act = nullptr, *iscf = nullptr, *any = nullptr, *any_h = nullptr;
     double *rmsd2 = nullptr, *rmsd = nullptr, *seg = nullptr, *seg_h = nullptr;
@@ -140,26 +142,38 @@ static int build_items(mdff_batch *b)
     b->nitems_rho = (int)ir.size();
     b->nitems_back = (int)ib.size();
     if (b->mf) {
-        // largest work first: forward items ~ ions x row tiles, back items ~
-        // box points x column tiles
-        std::vector<std::pair<long long, WItem>> f, k;
+        // per Kx (descending: the tile counts, and so the launch groups, fall
+        // with Kx), largest work first: forward items ~ ions, back items ~
+        // box points
+        std::vector<std::pair<std::pair<int, long long>, WItem>> f, k;
         for (int c = 0; c < b->nconf; c++) {
             const BConfD &K = b->conf_h[c];
             const long long P = (long long)(2 * K.Ky + 1) * (2 * K.Kz + 1);
-            const int cg = 16 * b->cwf;
-            for (int t = 0; t * cg < P; t++) f.push_back({-(long long)K.n * (K.Kx + 1), {c, t}});
-            for (int t = 0; t * 16 < K.n; t++) k.push_back({-P * (K.Kx + 1), {c, t}});
+            for (int t = 0; t * 32 < P; t++) f.push_back({{-K.Kx, -(long long)K.n}, {c, t}});
+            for (int t = 0; t * 16 * b->mfr < K.n; t++) k.push_back({{-K.Kx, -P}, {c, t}});
         }
-        auto byw = [](const std::pair<long long, WItem> &x, const std::pair<long long, WItem> &y) {
-            return x.first != y.first ? x.first < y.first
-                                      : (x.second.conf != y.second.conf ? x.second.conf < y.second.conf
-                                                                        : x.second.tile < y.second.tile);
+        auto byw = [](const std::pair<std::pair<int, long long>, WItem> &x,
+                      const std::pair<std::pair<int, long long>, WItem> &y) {
+            if (x.first != y.first) return x.first < y.first;
+            return x.second.conf != y.second.conf ? x.second.conf < y.second.conf : x.second.tile < y.second.tile;
         };
         std::sort(f.begin(), f.end(), byw);
         std::sort(k.begin(), k.end(), byw);
+        auto ranges = [](const std::vector<std::pair<std::pair<int, long long>, WItem>> &v,
+                         std::vector<std::pair<int, int>> &rng, std::vector<int> &kx, std::vector<WItem> &out) {
+            rng.clear(); kx.clear(); out.clear();
+            for (size_t i = 0; i < v.size(); i++) {
+                if (i == 0 || v[i].first.first != v[i - 1].first.first) {
+                    rng.push_back({(int)i, (int)i});
+                    kx.push_back(-v[i].first.first);
+                }
+                rng.back().second = (int)i + 1;
+                out.push_back(v[i].second);
+            }
+        };
         std::vector<WItem> fi, ki;
-        for (auto &x : f) fi.push_back(x.second);
-        for (auto &x : k) ki.push_back(x.second);
+        ranges(f, b->rng_mf, b->kx_mf, fi);
+        ranges(k, b->rng_mb, b->kx_mb, ki);
         bfree(b->items_mf);
         bfree(b->items_mb);
         if ((rc = balloc(&b->items_mf, fi.size())) || (rc = balloc(&b->items_mb, ki.size()))) return rc;
@@ -198,21 +212,37 @@ static void launch_rho(mdff_batch *b, int lo, int mode, const double *q, const d
         // tables: the field stays linear in the dipoles to FP64 rounding, as
         // the Jacobi stop test at conv_tol_ind needs; its static field and
         // force call use FP32 matrix instructions.
+        const int C = mode == RM_Q ? 1 : (mode == RM_MU ? 3 : 4);
+        const int form = !lo ? 0 : (exact ? 1 : 2);   // (tables, A, arithmetic): ddd, fdd, fff
+        const void *tab = lo ? (const void *)b->tabT32 : (const void *)b->tabT64;
+        const dim3 ga(16, b->nconf);
+        if (form == 0) hipLaunchKernelGGL((kb_mf_aw<double, double>), ga, dim3(256), 0, b->st, b->conf, tab, b->n, C, q, mu, b->aw, act);
+        else if (form == 1) hipLaunchKernelGGL((kb_mf_aw<float, double>), ga, dim3(256), 0, b->st, b->conf, tab, b->n, C, q, mu, b->aw, act);
+        else hipLaunchKernelGGL((kb_mf_aw<float, float>), ga, dim3(256), 0, b->st, b->conf, tab, b->n, C, q, mu, (float *)b->aw, act);
         MFwdArgs f;
-        f.conf = b->conf;
-        f.tabT = lo ? (const void *)b->tabT32 : (const void *)b->tabT64;
-        f.by = b->by; f.bz = b->bz; f.n = b->n;
-        f.C = mode == RM_Q ? 1 : (mode == RM_MU ? 3 : 4);
-        f.q = q; f.mu = mu; f.boxk = b->boxk; f.items = b->items_mf; f.nitems = b->nitems_mf; f.Sk = b->Sk;
-        f.act = act;
-        const dim3 gf(bdiv(b->nitems_mf, 4));
-#define MFWD(TT, RT, MT) hipLaunchKernelGGL((kb_mf_fwd<TT, RT, MT, 2>), gf, dim3(256), 0, b->st, f)
-        if (!lo) { if (f.C == 1) MFWD(double, double, 2); else if (f.C == 3) MFWD(double, double, 6); else MFWD(double, double, 8); }
-        else if (exact) { if (f.C == 1) MFWD(float, double, 2); else if (f.C == 3) MFWD(float, double, 6); else MFWD(float, double, 8); }
-        else { if (f.C == 1) MFWD(float, float, 2); else if (f.C == 3) MFWD(float, float, 6); else MFWD(float, float, 8); }
+        f.conf = b->conf; f.tabT = tab; f.aw = b->aw;
+        f.by = b->by; f.bz = b->bz; f.n = b->n; f.C = C;
+        f.boxk = b->boxk; f.Sk = b->Sk; f.act = act;
+        // one launch per run of Kx values with the same row-tile count
+        for (size_t gi = 0; gi < b->rng_mf.size();) {
+            const int MT = bmf_mt(b->kx_mf[gi], C);
+            size_t gj = gi;
+            while (gj + 1 < b->rng_mf.size() && bmf_mt(b->kx_mf[gj + 1], C) == MT) gj++;
+            f.items = b->items_mf + b->rng_mf[gi].first;
+            f.nitems = b->rng_mf[gj].second - b->rng_mf[gi].first;
+            const dim3 gf(bdiv(f.nitems, 4));
+#define MFWD(TT, AT, RT, M) hipLaunchKernelGGL((kb_mf_fwd<TT, AT, RT, M, 2>), gf, dim3(256), 0, b->st, f)
+#define MFWD_T(TT, AT, RT) do { if (MT <= 2) MFWD(TT, AT, RT, 2); else if (MT <= 4) MFWD(TT, AT, RT, 4); \
+                               else if (MT <= 6) MFWD(TT, AT, RT, 6); else MFWD(TT, AT, RT, 8); } while (0)
+            if (form == 0) MFWD_T(double, double, double);
+            else if (form == 1) MFWD_T(float, double, double);
+            else MFWD_T(float, float, float);
+#undef MFWD_T
 #undef MFWD
+            gi = gj + 1;
+        }
         hipLaunchKernelGGL(kb_mf_rho, dim3(bdiv(b->nktot, 256)), dim3(256), 0, b->st, (int)b->nktot, b->kcfg, b->kv,
-                           b->Sk, f.C, b->W, do_ek ? b->ek : nullptr, do_stress, act);
+                           b->Sk, C, b->W, do_ek ? b->ek : nullptr, do_stress, act);
         return;
     }
     RhoArgs a;
@@ -250,24 +280,39 @@ static void launch_back(mdff_batch *b, int lo, int nch, const double *q, const d
                         double *f, const int *act, bool exact = false)
 {
     if (b->mf) {
-        // W on the boxes in the precision the sum reads (FP64 unless the FP32
-        // matrix instructions run), then the back-projection
-        const bool w32 = lo && !exact;
-        const dim3 gw(std::max(1, std::min(16, (int)bdiv(b->maxbox, 256))), b->nconf);
-        if (w32) hipLaunchKernelGGL(kb_mf_wbox<float>, gw, dim3(256), 0, b->st, b->conf, b->boxk, b->W, (float *)b->Wbox, act);
-        else hipLaunchKernelGGL(kb_mf_wbox<double>, gw, dim3(256), 0, b->st, b->conf, b->boxk, b->W, (double *)b->Wbox, act);
+        // moment-weighted W columns in the precision the sum reads (FP64
+        // unless the FP32 matrix instructions run), then the back-projection
+        const int CH = nch == 9 ? 6 : 3;
+        const int form = !lo ? 0 : (exact ? 1 : 2);
+        const dim3 gw(16, b->nconf);
+        if (form == 2) hipLaunchKernelGGL(kb_mf_wm<float>, gw, dim3(256), 0, b->st, b->conf, b->boxk, b->W, CH, (float *)b->wm, act);
+        else hipLaunchKernelGGL(kb_mf_wm<double>, gw, dim3(256), 0, b->st, b->conf, b->boxk, b->W, CH, b->wm, act);
         MBackArgs m;
         m.conf = b->conf;
         m.tabT = lo ? (const void *)b->tabT32 : (const void *)b->tabT64;
-        m.by = b->by; m.bz = b->bz; m.n = b->n; m.CH = nch == 9 ? 6 : 3;
-        m.Wbox = b->Wbox; m.items = b->items_mb; m.nitems = b->nitems_mb;
+        m.wm = b->wm;
+        m.by = b->by; m.bz = b->bz; m.n = b->n; m.CH = CH;
         m.q = q; m.mu = mu; m.ef = ef; m.efg = efg; m.f = f; m.act = act;
-        const dim3 gb(bdiv(b->nitems_mb, 4));
-#define MBACK(TT, WT, RT, CT) hipLaunchKernelGGL((kb_mf_back<TT, WT, RT, CT>), gb, dim3(256), 0, b->st, m)
-        if (!lo) { if (m.CH == 3) MBACK(double, double, double, 6); else MBACK(double, double, double, 12); }
-        else if (exact) { if (m.CH == 3) MBACK(float, double, double, 6); else MBACK(float, double, double, 12); }
-        else { if (m.CH == 3) MBACK(float, float, float, 6); else MBACK(float, float, float, 12); }
+        for (size_t gi = 0; gi < b->rng_mb.size();) {
+            const int CT = bmf_ct(b->kx_mb[gi], CH);
+            size_t gj = gi;
+            while (gj + 1 < b->rng_mb.size() && bmf_ct(b->kx_mb[gj + 1], CH) == CT) gj++;
+            m.items = b->items_mb + b->rng_mb[gi].first;
+            m.nitems = b->rng_mb[gj].second - b->rng_mb[gi].first;
+            const dim3 gb(bdiv(m.nitems, 4));
+#define MBACK(TT, WT, RT, C_, R_) hipLaunchKernelGGL((kb_mf_back<TT, WT, RT, C_, R_>), gb, dim3(256), 0, b->st, m)
+#define MBACK_R(TT, WT, RT, C_) do { if (b->mfr == 2) MBACK(TT, WT, RT, C_, 2); else MBACK(TT, WT, RT, C_, 1); } while (0)
+#define MBACK_T(TT, WT, RT) do { if (CT <= 2) MBACK_R(TT, WT, RT, 2); else if (CT <= 4) MBACK_R(TT, WT, RT, 4); \
+                                 else if (CT <= 6) MBACK_R(TT, WT, RT, 6); else if (CT <= 8) MBACK_R(TT, WT, RT, 8); \
+                                 else if (CT <= 10) MBACK_R(TT, WT, RT, 10); else MBACK_R(TT, WT, RT, 12); } while (0)
+            if (form == 0) MBACK_T(double, double, double);
+            else if (form == 1) MBACK_T(float, double, double);
+            else MBACK_T(float, float, float);
+#undef MBACK_T
+#undef MBACK_R
 #undef MBACK
+            gi = gj + 1;
+        }
         return;
     }
     BackBArgs a;
@@ -555,8 +600,7 @@ void mdff_batch_destroy(mdff_batch *b)
     bfree(b->rmsd2); bfree(b->rmsd); bfree(b->seg); bfree(b->g); bfree(b->sdir); bfree(b->srhs); bfree(b->lam);
     bfree(b->eT); bfree(b->edel); bfree(b->cols); bfree(b->tsum);
     bfree(b->boxk); bfree(b->kcfg); bfree(b->Sk); bfree(b->items_mf); bfree(b->items_mb);
-    if (b->Wbox) (void)hipFree(b->Wbox);
-    b->Wbox = nullptr;
+    bfree(b->aw); bfree(b->wm);
     if (b->any_h) (void)hipHostFree(b->any_h);
     if (b->seg_h) free(b->seg_h);
     for (int i = 0; i < 4; i++)
@@ -660,12 +704,18 @@ int mdff_batch_create(mdff_batch **out, const mdff_params *p, int nconf, const i
         K.Kx = kes[0]; K.Ky = kes[1]; K.Kz = kes[2];
         K.koff = (int)kvh.size();
         K.boxoff = (int)b->nbox;
+        K.awoff = (int)b->naw;
+        K.wmoff = (int)b->nwm;
         if (p->lcoulombic) {
             double a2 = alpha * alpha;
             const size_t box = (size_t)(K.Kx + 1) * (2 * K.Ky + 1) * (2 * K.Kz + 1);
             b->boxk_h.resize(b->nbox + box, -1);
             b->nbox += box;
-            b->maxbox = std::max(b->maxbox, (int)box);
+            // operand blocks of the matrix-core sums, sized for 4 channels
+            // and 6 moments (the largest tile counts)
+            const size_t Pbox = (size_t)(2 * K.Ky + 1) * (2 * K.Kz + 1);
+            b->naw += (size_t)2 * nion[c] * 16 * bmf_mt(K.Kx, 4);
+            b->nwm += (size_t)2 * ((Pbox + 1) & ~(size_t)1) * 16 * bmf_ct(K.Kx, 6);
             size_t e = K.boxoff;
             for (int nx = 0; nx <= K.Kx; nx++)
                 for (int ny = -K.Ky; ny <= K.Ky; ny++)
@@ -726,14 +776,10 @@ int mdff_batch_create(mdff_batch **out, const mdff_params *p, int nconf, const i
     b->mf = p->lcoulombic && b->nktot > 0 && b->nbox > 0;
     for (int c = 0; c < nconf && b->mf; c++)
         if (b->conf_h[c].Kx + 1 > 16) b->mf = 0;
-    b->cwf = 2;
+    if (b->naw > (size_t)INT32_MAX || b->nwm > (size_t)INT32_MAX) b->mf = 0;
     if (b->mf) {
         BALLOC(b->boxk, b->nbox); BALLOC(b->kcfg, b->nktot); BALLOC(b->Sk, 8 * b->nktot);
-        if (hipMalloc(&b->Wbox, b->nbox * 2 * sizeof(double)) != hipSuccess) {
-            b->Wbox = nullptr;
-            mdff_batch_destroy(b);
-            return MDFF_ERR_NOMEM;
-        }
+        BALLOC(b->aw, b->naw); BALLOC(b->wm, b->nwm);
     }
     BALLOC(b->tab64, 2 * (size_t)n * b->TS);
     BALLOC(b->tabT64, 2 * (size_t)n * b->TS);
@@ -998,7 +1044,13 @@ int mdff_batch_set_tuning(mdff_batch *b, int lanes, int kpb, int ipb, int bpc)
     if (!b) return MDFF_ERR_ARG;
     if (lanes) { if (lanes != 16 && lanes != 32 && lanes != 64) return MDFF_ERR_ARG; b->lanes = lanes; }
     if (kpb) { if (kpb != 32 && kpb != 64 && kpb != 128) return MDFF_ERR_ARG; b->kpb = kpb; }
-    if (ipb) { if (ipb != 64 && ipb != 128 && ipb != 256) return MDFF_ERR_ARG; b->ipb = ipb; }
+    // ions per back-projection item: 16 or 32 (matrix-core sums), 64, 128 or
+    // 256 (vector-ALU sums)
+    if (ipb) {
+        if (ipb == 16 || ipb == 32) b->mfr = ipb / 16;
+        else if (ipb == 64 || ipb == 128 || ipb == 256) b->ipb = ipb;
+        else return MDFF_ERR_ARG;
+    }
     if (bpc) { if (bpc < 1 || bpc > 32) return MDFF_ERR_ARG; b->bpc = bpc; }
     b->tuned = 1;
     int rc = build_items(b);
@@ -1050,7 +1102,24 @@ int mdff_batch_autotune(mdff_batch *b, int out[4])
         }
         b->lanes = pick;
     }
-    if (b->p.lcoulombic && b->nktot > 0) {
+    if (b->p.lcoulombic && b->nktot > 0 && b->mf) {
+        // matrix-core sums: 16 or 32 ions per back-projection item (the
+        // forward's items, 32 box points of one configuration, are fixed)
+        int pr = b->mfr;
+        best = 1e30;
+        for (int r : {1, 2}) {
+            b->mfr = r;
+            if ((rc = build_items(b))) return rc;
+            double ms;
+            if ((rc = time_it(b, [&]() {
+                     launch_back(b, lo, 9, b->q, b->mu, b->ef_rec, b->efg_rec, b->f_rec, nullptr);
+                 }, 5, &ms)))
+                return rc;
+            if (ms < best) { best = ms; pr = r; }
+        }
+        b->mfr = pr;
+        if ((rc = build_items(b))) return rc;
+    } else if (b->p.lcoulombic && b->nktot > 0) {
         int pk = b->kpb, pi = b->ipb, pb = b->bpc;
         best = 1e30;
         for (int bpc : {2, 4, 8}) {
@@ -1079,7 +1148,12 @@ int mdff_batch_autotune(mdff_batch *b, int out[4])
         if ((rc = build_items(b))) return rc;
     }
     b->tuned = 1;
-    if (out) { out[0] = b->lanes; out[1] = b->kpb; out[2] = b->ipb; out[3] = b->bpc; }
+    if (out) {
+        out[0] = b->lanes;
+        out[1] = b->mf ? 32 : b->kpb;
+        out[2] = b->mf ? 16 * b->mfr : b->ipb;
+        out[3] = b->bpc;
+    }
     return MDFF_OK;
 }
 

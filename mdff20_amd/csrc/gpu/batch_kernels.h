@@ -391,11 +391,17 @@ __global__ void __launch_bounds__(256) kb_back(BackBArgs a)
 //            factors per ion.
 // Complex products are embedded as real GEMMs: for a = ar + i ai times
 // b = br + i bi over K = (re, im), the row (re) takes (ar, -ai) and the row
-// (im) takes (ai, ar).  One wave computes one item: every row tile of a
-// configuration's column group (forward) or one 16-ion row tile against
-// every column tile (back), over the whole K range, so no partial sums.
-// v_mfma_f64_16x16x4f64 (FP64 form) and v_mfma_f32_16x16x4f32 (mixed form):
-// A[l % 16][k], B[k][l % 16] with k = l / 16.
+// (im) takes (ai, ar).  The operand that depends on the weights (forward:
+// w ex rows; back: moment-weighted W columns) is written once per call in
+// the embedded, K-major layout the lanes read (kb_mf_aw, kb_mf_wm); the
+// other ((ey ez)_j(p)) is formed in registers.  One wave computes one
+// item: a configuration's column group against all its row tiles (forward)
+// or R 16-ion row tiles against all its column tiles (back), over the
+// whole K range: no partial sums.  A launch covers the configurations of
+// one tile count (compile-time MT / CT).  v_mfma_f64_16x16x4f64 (FP64 form,
+// and the mixed form's SCF fields) and v_mfma_f32_16x16x4f32 (the mixed
+// form's static field and force call): A[l % 16][k], B[k][l % 16] with
+// k = l / 16.
 
 template <typename real>
 struct BMF;
@@ -422,22 +428,92 @@ struct BMF<float> {
 #define BMF_MT 8    // row tiles of the forward sum: (Kx + 1) C 2 <= 16 BMF_MT
 #define BMF_CT 12   // column tiles of the back-projection: (Kx + 1) 6 2 <= 16 BMF_CT
 
+// tile counts of a configuration, rounded up to even (the launch groups)
+__host__ __device__ __forceinline__ int bmf_mt(int Kx, int C) { return ((Kx + 1) * C * 2 + 31) / 32 * 2; }
+__host__ __device__ __forceinline__ int bmf_ct(int Kx, int CH) { return ((Kx + 1) * CH * 2 + 31) / 32 * 2; }
+
+// forward A operand: AW[awoff + kk Rp + row] (kk = 2 j_local + re/im, Rp =
+// 16 bmf_mt(Kx, C)), rows past (Kx + 1) C 2 zero; grid (chunks,
+// configurations)
+template <typename TT, typename AT>
+__global__ void __launch_bounds__(256) kb_mf_aw(const BConfD *__restrict__ conf, const void *__restrict__ tabT,
+                                                int n, int C, const double *__restrict__ q,
+                                                const double *__restrict__ mu, AT *__restrict__ aw,
+                                                const int *__restrict__ act)
+{
+    const int c = blockIdx.y;
+    if (!conf_active(act, c)) return;
+    const BConfD &K = conf[c];
+    typedef TT t2 __attribute__((ext_vector_type(2)));
+    const t2 *T = (const t2 *)tabT;
+    const int R = (K.Kx + 1) * C * 2, Rp = 16 * bmf_mt(K.Kx, C);
+    const int tot = 2 * K.n * Rp;
+    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < tot; e += gridDim.x * blockDim.x) {
+        const int kk = e / Rp, row = e - kk * Rp;
+        AT v = (AT)0;
+        if (row < R) {
+            const int j = K.off + (kk >> 1), nx = row / (2 * C), ch = (row >> 1) % C;
+            const int wc = C == 4 ? ch : (C == 3 ? ch + 1 : 0);   // 0 q, 1..3 mu
+            const double w = wc == 0 ? q[j] : mu[(size_t)(wc - 1) * n + j];
+            const t2 X = T[(size_t)nx * n + j];
+            const double ar = w * (double)X.x, ai = w * (double)X.y;
+            const bool kim = kk & 1, rim = row & 1;
+            v = (AT)(rim ? (kim ? ar : ai) : (kim ? -ai : ar));
+        }
+        aw[(size_t)K.awoff + e] = v;
+    }
+}
+
+// back B operand: WM[wmoff + kk NCp + col] (kk = 2 p + re/im over the box
+// points rounded up to even, NCp = 16 bmf_ct(Kx, CH)), the moment-weighted W
+// of the half-space k-vectors (zero elsewhere and past (Kx + 1) CH 2)
+template <typename WT>
+__global__ void __launch_bounds__(256) kb_mf_wm(const BConfD *__restrict__ conf, const int *__restrict__ boxk,
+                                                const double *__restrict__ W, int CH, WT *__restrict__ wm,
+                                                const int *__restrict__ act)
+{
+    const int c = blockIdx.y;
+    if (!conf_active(act, c)) return;
+    const BConfD &K = conf[c];
+    const int NZ = 2 * K.Kz + 1, P = (2 * K.Ky + 1) * NZ, Pp = (P + 1) & ~1;
+    const int NC = (K.Kx + 1) * CH * 2, NCp = 16 * bmf_ct(K.Kx, CH);
+    const int tot = 2 * Pp * NCp;
+    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < tot; e += gridDim.x * blockDim.x) {
+        const int kk = e / NCp, col = e - kk * NCp, pp = kk >> 1;
+        WT v = (WT)0;
+        if (col < NC && pp < P) {
+            const int nx = col / (2 * CH), m = (col >> 1) % CH;
+            const int k = boxk[K.boxoff + nx * P + pp];
+            if (k >= 0) {
+                const int iy = pp / NZ;
+                const double ny = (double)(iy - K.Ky), nz = (double)(pp - iy * NZ - K.Kz);
+                const double mom = m == 0 ? 1.0 : (m == 1 ? ny : (m == 2 ? nz : (m == 3 ? ny * ny : (m == 4 ? ny * nz : nz * nz))));
+                const double wr = mom * W[2 * ((size_t)K.koff + k)], wi = mom * W[2 * ((size_t)K.koff + k) + 1];
+                const bool kim = kk & 1, cim = col & 1;
+                v = (WT)(cim ? (kim ? wr : wi) : (kim ? -wi : wr));
+            }
+        }
+        wm[(size_t)K.wmoff + e] = v;
+    }
+}
+
 struct MFwdArgs {
     const BConfD *conf;
     const void *tabT;          // [TS][n] complex (TT), column m of ion i at m n + i
+    const void *aw;            // A operands (kb_mf_aw)
     int by, bz, n, C;          // C: 1 (q), 3 (mu), 4 (q, mu)
-    const double *q, *mu;
     const int *boxk;           // [box point (nx, p)] -> k index within the configuration, or -1
-    const WItem *items;        // (configuration, column group of CW tiles)
+    const WItem *items;        // (configuration, column group of CW tiles), this launch's range
     int nitems;
     double *Sk;                // [nktot][C][re, im]
     const int *act;
 };
 
-// TT: table type; real: arithmetic of the matrix instructions; MT row tiles
-// (at least the launch's largest), CW column tiles per item.  The operands
-// of step s + 1 are loaded while step s computes.
-template <typename TT, typename real, int MT, int CW>
+// TT: table type; AT: A operand type; real: arithmetic of the matrix
+// instructions; MT row tiles (every item of the launch has bmf_mt = MT);
+// CW column tiles per item.  Operands of step s + 1 load while step s
+// computes.
+template <typename TT, typename AT, typename real, int MT, int CW>
 __global__ void __launch_bounds__(256) kb_mf_fwd(MFwdArgs a)
 {
     using M = BMF<real>;
@@ -451,21 +527,13 @@ __global__ void __launch_bounds__(256) kb_mf_fwd(MFwdArgs a)
     const BConfD &K = a.conf[w.conf];
     const int g = lane >> 4, cl = lane & 15;
     const int C = a.C, NZ = 2 * K.Kz + 1, P = (2 * K.Ky + 1) * NZ;
-    const int R = (K.Kx + 1) * C * 2, MTc = (R + 15) >> 4;
+    const int R = (K.Kx + 1) * C * 2;
+    constexpr int Rp = 16 * MT;
     const int n = a.n;
     const t2 *T = (const t2 *)a.tabT;
-    // the lane's A rows (nx, c, re/im) and B columns (ny, nz); rows and
-    // columns past the box repeat the last one (their results are dropped)
-    int xo[MT], wc[MT];
-    bool rim[MT];
-#pragma unroll
-    for (int t = 0; t < MT; t++) {
-        const int r = min(t * 16 + cl, R - 1);
-        xo[t] = (r / (2 * C)) * n;
-        const int c = (r >> 1) % C;
-        wc[t] = C == 4 ? c : (C == 3 ? c + 1 : 0);   // weight: 0 q, 1..3 mu
-        rim[t] = r & 1;
-    }
+    // the lane's A column (row of the sum) in the AW block and its B columns
+    // (ny, nz); columns past the box repeat the last one (dropped)
+    const AT *A = (const AT *)a.aw + K.awoff + cl;
     int yo[CW], zo[CW];
 #pragma unroll
     for (int c = 0; c < CW; c++) {
@@ -482,10 +550,11 @@ __global__ void __launch_bounds__(256) kb_mf_fwd(MFwdArgs a)
     const bool kim = g & 1;   // the lane's K component: im
     const int ns = (K.n + 1) >> 1;
     struct Set {
-        t2 Y[CW], Z[CW], X[MT];
-        double w[4];
+        t2 Y[CW], Z[CW];
+        AT X[MT];
     };
-    // loads of step s (ions past the configuration repeat its last ion)
+    // step s: ions 2 s, 2 s + 1 (the last ion repeated past the configuration,
+    // against zero B)
     auto load = [&](Set &S, int s) {
         const int jl = min(2 * s + (g >> 1), K.n - 1);
         const int j = K.off + jl;
@@ -494,12 +563,9 @@ __global__ void __launch_bounds__(256) kb_mf_fwd(MFwdArgs a)
             S.Y[c] = T[yo[c] + j];
             S.Z[c] = T[zo[c] + j];
         }
+        const AT *Ar = A + (size_t)(2 * jl + (g & 1)) * Rp;   // K row 2 jl + re/im of the block
 #pragma unroll
-        for (int t = 0; t < MT; t++) S.X[t] = T[xo[t] + j];
-        S.w[0] = C != 3 ? a.q[j] : 0.0;
-        S.w[1] = C >= 3 ? a.mu[j] : 0.0;
-        S.w[2] = C >= 3 ? a.mu[n + j] : 0.0;
-        S.w[3] = C >= 3 ? a.mu[2 * (size_t)n + j] : 0.0;
+        for (int t = 0; t < MT; t++) S.X[t] = Ar[t * 16];
     };
     auto compute = [&](const Set &S, int s) {
         const bool jok = 2 * s + (g >> 1) < K.n;
@@ -512,14 +578,9 @@ __global__ void __launch_bounds__(256) kb_mf_fwd(MFwdArgs a)
         }
 #pragma unroll
         for (int t = 0; t < MT; t++) {
-            if (t < MTc) {
-                const int c = wc[t];
-                const real wv = (real)(c == 0 ? S.w[0] : (c == 1 ? S.w[1] : (c == 2 ? S.w[2] : S.w[3])));
-                const real ar = wv * (real)S.X[t].x, ai = wv * (real)S.X[t].y;
-                const real av = rim[t] ? (kim ? ar : ai) : (kim ? -ai : ar);
+            const real av = (real)S.X[t];
 #pragma unroll
-                for (int c2 = 0; c2 < CW; c2++) acc[t][c2] = M::mma(av, bv[c2], acc[t][c2]);
-            }
+            for (int c = 0; c < CW; c++) acc[t][c] = M::mma(av, bv[c], acc[t][c]);
         }
     };
     Set S0, S1;
@@ -536,7 +597,6 @@ __global__ void __launch_bounds__(256) kb_mf_fwd(MFwdArgs a)
     const int *bk = a.boxk + K.boxoff;
 #pragma unroll
     for (int t = 0; t < MT; t++) {
-        if (t >= MTc) break;
 #pragma unroll
         for (int c = 0; c < CW; c++) {
             const int p = (w.tile * CW + c) * 16 + cl;
@@ -594,51 +654,27 @@ __global__ void __launch_bounds__(256) kb_mf_rho(int nktot, const int *__restric
     }
 }
 
-// W on every configuration's box (zero off the half-space list), in the
-// precision of the back-projection; grid (chunks, configurations)
-template <typename real>
-__global__ void __launch_bounds__(256) kb_mf_wbox(const BConfD *__restrict__ conf, const int *__restrict__ boxk,
-                                                  const double *__restrict__ W, real *__restrict__ Wbox,
-                                                  const int *__restrict__ act)
-{
-    const int c = blockIdx.y;
-    if (!conf_active(act, c)) return;
-    const BConfD &K = conf[c];
-    const int box = (K.Kx + 1) * (2 * K.Ky + 1) * (2 * K.Kz + 1);
-    typedef real r2 __attribute__((ext_vector_type(2)));
-    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < box; e += gridDim.x * blockDim.x) {
-        const int k = boxk[K.boxoff + e];
-        r2 v = {0, 0};
-        if (k >= 0) {
-            v.x = (real)W[2 * ((size_t)K.koff + k)];
-            v.y = (real)W[2 * ((size_t)K.koff + k) + 1];
-        }
-        ((r2 *)Wbox)[K.boxoff + e] = v;
-    }
-}
-
 struct MBackArgs {
     const BConfD *conf;
     const void *tabT;          // [TS][n] complex (TT)
+    const void *wm;            // B operands (kb_mf_wm)
     int by, bz, n, CH;         // CH: 3 moments (field) or 6 (field and gradient)
-    const void *Wbox;          // [box point] complex (WT)
-    const WItem *items;        // (configuration, 16-ion row tile)
+    const WItem *items;        // (configuration, group of R 16-ion row tiles), this launch's range
     int nitems;
     const double *q, *mu;
     double *ef, *efg, *f;      // as kb_back
     const int *act;
 };
 
-// TT: table type; WT: type of W on the box; real: arithmetic of the matrix
-// instructions; CT column tiles (at least the launch's largest).  The
-// operands of step s + 1 are loaded while step s computes.
-template <typename TT, typename WT, typename real, int CT>
+// TT: table type; WT: B operand type; real: arithmetic of the matrix
+// instructions; CT column tiles (every item of the launch has bmf_ct = CT);
+// R row tiles per item.  Operands of step s + 1 load while step s computes.
+template <typename TT, typename WT, typename real, int CT, int R>
 __global__ void __launch_bounds__(256) kb_mf_back(MBackArgs a)
 {
     using M = BMF<real>;
     using v4 = typename M::v4;
     typedef TT t2 __attribute__((ext_vector_type(2)));
-    typedef WT w2 __attribute__((ext_vector_type(2)));
     const int lane = threadIdx.x & 63;
     const int it = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (it >= a.nitems) return;
@@ -647,63 +683,62 @@ __global__ void __launch_bounds__(256) kb_mf_back(MBackArgs a)
     const BConfD &K = a.conf[w.conf];
     const int g = lane >> 4, cl = lane & 15;
     const int CH = a.CH, NZ = 2 * K.Kz + 1, P = (2 * K.Ky + 1) * NZ;
-    const int NC = (K.Kx + 1) * CH * 2, CTc = (NC + 15) >> 4;
+    const int NC = (K.Kx + 1) * CH * 2;
+    constexpr int NCp = 16 * CT;
     const int n = a.n;
     const t2 *T = (const t2 *)a.tabT;
-    const w2 *Wb = (const w2 *)a.Wbox + K.boxoff;
-    // A row: the lane's ion; B columns (nx, m, re/im) of each tile
-    const int il = w.tile * 16 + cl;
-    const int i = K.off + (il < K.n ? il : K.n - 1);
-    int wo[CT], mm[CT];
-    bool cim[CT];
+    // the lane's B column in the WM block (K row g of each step)
+    const WT *B = (const WT *)a.wm + K.wmoff + g * NCp + cl;
+    // A rows: the lane's ion in each row tile
+    int ion[R];
 #pragma unroll
-    for (int t = 0; t < CT; t++) {
-        const int col = min(t * 16 + cl, NC - 1);
-        wo[t] = (col / (2 * CH)) * P;
-        mm[t] = (col >> 1) % CH;
-        cim[t] = col & 1;
+    for (int r = 0; r < R; r++) {
+        const int il = (w.tile * R + r) * 16 + cl;
+        ion[r] = K.off + (il < K.n ? il : K.n - 1);
     }
-    v4 acc[CT];
+    v4 acc[R][CT];
 #pragma unroll
-    for (int t = 0; t < CT; t++) acc[t] = v4{0, 0, 0, 0};
+    for (int r = 0; r < R; r++)
+#pragma unroll
+        for (int t = 0; t < CT; t++) acc[r][t] = v4{0, 0, 0, 0};
     const bool kim = g & 1;
     const int ns = (P + 1) >> 1;
     struct Set {
-        t2 Y, Z;
-        w2 Wv[CT];
-        int ny, nz;
-        bool pok;
+        t2 Y[R], Z[R];
+        WT Wv[CT];
     };
-    // the lane's point p = 2 s + g / 2 as (iy, iz), advanced by 2 per load
+    // the lane's point p = 2 s + g / 2 as (iy, iz), advanced by 2 per load;
+    // points past the box repeat the last one against zero W (kb_mf_wm)
     int lp = g >> 1, liy = 0, liz = lp;
+    const WT *Bs = B;
     auto load = [&](Set &S) {
-        S.pok = lp < P;
-        const int iy = S.pok ? liy : 2 * K.Ky, iz = S.pok ? liz : 2 * K.Kz;
-        const int pc = S.pok ? lp : P - 1;
-        S.ny = iy - K.Ky;
-        S.nz = iz - K.Kz;
-        S.Y = T[(a.by + S.ny) * n + i];
-        S.Z = T[(a.bz + S.nz) * n + i];
+        const bool pok = lp < P;
+        const int iy = pok ? liy : 2 * K.Ky, iz = pok ? liz : 2 * K.Kz;
+        const int yo = (a.by - K.Ky + iy) * n, zo = (a.bz - K.Kz + iz) * n;
 #pragma unroll
-        for (int t = 0; t < CT; t++) S.Wv[t] = Wb[wo[t] + pc];
+        for (int r = 0; r < R; r++) {
+            S.Y[r] = T[yo + ion[r]];
+            S.Z[r] = T[zo + ion[r]];
+        }
+#pragma unroll
+        for (int t = 0; t < CT; t++) S.Wv[t] = Bs[t * 16];
+        Bs += lp + 2 < 2 * ns ? 4 * NCp : 0;
         lp += 2;
         liz += 2;
         while (liz >= NZ) { liz -= NZ; liy++; }
     };
     auto compute = [&](const Set &S) {
-        const real yr = (real)S.Y.x, yi = (real)S.Y.y, zr = (real)S.Z.x, zi = (real)S.Z.y;
-        const real av = kim ? yr * zi + yi * zr : yr * zr - yi * zi;
-        const real fy = (real)S.ny, fz = (real)S.nz;
+        real av[R];
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            const real yr = (real)S.Y[r].x, yi = (real)S.Y[r].y, zr = (real)S.Z[r].x, zi = (real)S.Z[r].y;
+            av[r] = kim ? yr * zi + yi * zr : yr * zr - yi * zi;
+        }
 #pragma unroll
         for (int t = 0; t < CT; t++) {
-            if (t < CTc) {
-                const int m = mm[t];
-                const real mom = m == 0 ? (real)1 : (m == 1 ? fy : (m == 2 ? fz : (m == 3 ? fy * fy : (m == 4 ? fy * fz : fz * fz))));
-                const real wr = mom * (real)S.Wv[t].x, wi = mom * (real)S.Wv[t].y;
-                real bv = cim[t] ? (kim ? wr : wi) : (kim ? -wi : wr);
-                bv = S.pok ? bv : (real)0;
-                acc[t] = M::mma(av, bv, acc[t]);
-            }
+            const real bv = (real)S.Wv[t];
+#pragma unroll
+            for (int r = 0; r < R; r++) acc[r][t] = M::mma(av[r], bv, acc[r][t]);
         }
     };
     Set S0, S1;
@@ -719,86 +754,88 @@ __global__ void __launch_bounds__(256) kb_mf_back(MBackArgs a)
     // epilogue: lanes of even columns hold T_re, their neighbours T_im;
     // z = ex[nx] T per (ion, nx, m), summed with the nx factors into
     // M_b = sum_k n_b Im(W e^{ik.r}) and N_bd = sum_k n_b n_d Re(W e^{ik.r})
-    double o[4][9];
-#pragma unroll
-    for (int e = 0; e < 4; e++)
-#pragma unroll
-        for (int c = 0; c < 9; c++) o[e][c] = 0.0;
-#pragma unroll
-    for (int t = 0; t < CT; t++) {
-        if (t >= CTc) break;
-        const int col = t * 16 + cl;
-        const bool use = col < NC && !(cl & 1);
-        const int nx = col / (2 * CH), m = (col >> 1) % CH;
-        const double fx = (double)nx;
-#pragma unroll
-        for (int e = 0; e < 4; e++) {
-            const double tr = (double)acc[t][e];
-            const double ti = __shfl_xor(tr, 1, 64);
-            const int rl = w.tile * 16 + M::row(lane, e);
-            const int ion = K.off + (rl < K.n ? rl : K.n - 1);
-            if (use) {
-                const t2 X = T[nx * n + ion];
-                const double xr = (double)X.x, xi = (double)X.y;
-                const double zr = xr * tr - xi * ti, zi = xr * ti + xi * tr;
-                o[e][0] += m == 0 ? fx * zi : 0.0;
-                o[e][1] += m == 1 ? zi : 0.0;
-                o[e][2] += m == 2 ? zi : 0.0;
-                o[e][3] += m == 0 ? fx * fx * zr : 0.0;
-                o[e][4] += m == 1 ? fx * zr : 0.0;
-                o[e][5] += m == 2 ? fx * zr : 0.0;
-                o[e][6] += m == 3 ? zr : 0.0;
-                o[e][7] += m == 4 ? zr : 0.0;
-                o[e][8] += m == 5 ? zr : 0.0;
-            }
-        }
-    }
     const int nout = CH == 6 ? 9 : 3;
 #pragma unroll
-    for (int e = 0; e < 4; e++)
+    for (int r = 0; r < R; r++) {
+        double o[4][9];
 #pragma unroll
-        for (int c = 0; c < 9; c++) {
-            if (c >= nout) break;
-            double v = o[e][c];
-            v += __shfl_xor(v, 1, 64);
-            v += __shfl_xor(v, 2, 64);
-            v += __shfl_xor(v, 4, 64);
-            v += __shfl_xor(v, 8, 64);
-            o[e][c] = v;
-        }
-    // lane cl = e < 4 of each lane group writes accumulator row e
+        for (int e = 0; e < 4; e++)
 #pragma unroll
-    for (int e = 0; e < 4; e++) {
-        if (cl != e) continue;
-        const int rl = w.tile * 16 + M::row(lane, e);
-        if (rl >= K.n) continue;
-        const int ion = K.off + rl;
-        const double(*B)[3] = K.c.B;
-        const double s1 = MDFF_TPI * K.tfpiV, s2 = MDFF_TPI * MDFF_TPI * K.tfpiV;
-        double v[9];
-        for (int x = 0; x < 3; x++) v[x] = s1 * (B[x][0] * o[e][0] + B[x][1] * o[e][1] + B[x][2] * o[e][2]);
-        if (CH == 6) {
-            const double Nm[3][3] = {{o[e][3], o[e][4], o[e][5]}, {o[e][4], o[e][6], o[e][7]}, {o[e][5], o[e][7], o[e][8]}};
-            double BN[3][3];
-            for (int x = 0; x < 3; x++)
-                for (int d = 0; d < 3; d++) BN[x][d] = B[x][0] * Nm[0][d] + B[x][1] * Nm[1][d] + B[x][2] * Nm[2][d];
-            const int px[6] = {0, 0, 0, 1, 1, 2}, py[6] = {0, 1, 2, 1, 2, 2};
-            for (int c = 0; c < 6; c++)
-                v[3 + c] = s2 * (BN[px[c]][0] * B[py[c]][0] + BN[px[c]][1] * B[py[c]][1] + BN[px[c]][2] * B[py[c]][2]);
-        }
-        if (a.ef) { a.ef[ion] = v[0]; a.ef[n + ion] = v[1]; a.ef[2 * (size_t)n + ion] = v[2]; }
-        if (CH == 6 && a.efg)
-            for (int c = 0; c < 6; c++) a.efg[(size_t)c * n + ion] = v[3 + c];
-        if (a.f) {
-            const double qi = a.q[ion];
-            double fx = qi * v[0], fy = qi * v[1], fz = qi * v[2];
-            if (CH == 6) {
-                const double m0 = a.mu[ion], m1 = a.mu[n + ion], m2 = a.mu[2 * (size_t)n + ion];
-                fx += v[3] * m0 + v[4] * m1 + v[5] * m2;
-                fy += v[4] * m0 + v[6] * m1 + v[7] * m2;
-                fz += v[5] * m0 + v[7] * m1 + v[8] * m2;
+            for (int c = 0; c < 9; c++) o[e][c] = 0.0;
+#pragma unroll
+        for (int t = 0; t < CT; t++) {
+            const int col = t * 16 + cl;
+            const bool use = col < NC && !(cl & 1);
+            const int nx = col / (2 * CH), m = (col >> 1) % CH;
+            const double fx = (double)nx;
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                const double tr = (double)acc[r][t][e];
+                const double ti = __shfl_xor(tr, 1, 64);
+                const int rl = (w.tile * R + r) * 16 + M::row(lane, e);
+                const int io = K.off + (rl < K.n ? rl : K.n - 1);
+                if (use) {
+                    const t2 X = T[(nx < K.Kx ? nx : K.Kx) * n + io];
+                    const double xr = (double)X.x, xi = (double)X.y;
+                    const double zr = xr * tr - xi * ti, zi = xr * ti + xi * tr;
+                    o[e][0] += m == 0 ? fx * zi : 0.0;
+                    o[e][1] += m == 1 ? zi : 0.0;
+                    o[e][2] += m == 2 ? zi : 0.0;
+                    o[e][3] += m == 0 ? fx * fx * zr : 0.0;
+                    o[e][4] += m == 1 ? fx * zr : 0.0;
+                    o[e][5] += m == 2 ? fx * zr : 0.0;
+                    o[e][6] += m == 3 ? zr : 0.0;
+                    o[e][7] += m == 4 ? zr : 0.0;
+                    o[e][8] += m == 5 ? zr : 0.0;
+                }
             }
-            a.f[ion] = fx; a.f[n + ion] = fy; a.f[2 * (size_t)n + ion] = fz;
+        }
+#pragma unroll
+        for (int e = 0; e < 4; e++)
+#pragma unroll
+            for (int c = 0; c < 9; c++) {
+                if (c >= nout) break;
+                double v = o[e][c];
+                v += __shfl_xor(v, 1, 64);
+                v += __shfl_xor(v, 2, 64);
+                v += __shfl_xor(v, 4, 64);
+                v += __shfl_xor(v, 8, 64);
+                o[e][c] = v;
+            }
+        // lane cl = e < 4 of each lane group writes accumulator row e
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+            if (cl != e) continue;
+            const int rl = (w.tile * R + r) * 16 + M::row(lane, e);
+            if (rl >= K.n) continue;
+            const int io = K.off + rl;
+            const double(*Bm)[3] = K.c.B;
+            const double s1 = MDFF_TPI * K.tfpiV, s2 = MDFF_TPI * MDFF_TPI * K.tfpiV;
+            double v[9];
+            for (int x = 0; x < 3; x++) v[x] = s1 * (Bm[x][0] * o[e][0] + Bm[x][1] * o[e][1] + Bm[x][2] * o[e][2]);
+            if (CH == 6) {
+                const double Nm[3][3] = {{o[e][3], o[e][4], o[e][5]}, {o[e][4], o[e][6], o[e][7]}, {o[e][5], o[e][7], o[e][8]}};
+                double BN[3][3];
+                for (int x = 0; x < 3; x++)
+                    for (int d = 0; d < 3; d++) BN[x][d] = Bm[x][0] * Nm[0][d] + Bm[x][1] * Nm[1][d] + Bm[x][2] * Nm[2][d];
+                const int px[6] = {0, 0, 0, 1, 1, 2}, py[6] = {0, 1, 2, 1, 2, 2};
+                for (int c = 0; c < 6; c++)
+                    v[3 + c] = s2 * (BN[px[c]][0] * Bm[py[c]][0] + BN[px[c]][1] * Bm[py[c]][1] + BN[px[c]][2] * Bm[py[c]][2]);
+            }
+            if (a.ef) { a.ef[io] = v[0]; a.ef[n + io] = v[1]; a.ef[2 * (size_t)n + io] = v[2]; }
+            if (CH == 6 && a.efg)
+                for (int c = 0; c < 6; c++) a.efg[(size_t)c * n + io] = v[3 + c];
+            if (a.f) {
+                const double qi = a.q[io];
+                double fx = qi * v[0], fy = qi * v[1], fz = qi * v[2];
+                if (CH == 6) {
+                    const double m0 = a.mu[io], m1 = a.mu[n + io], m2 = a.mu[2 * (size_t)n + io];
+                    fx += v[3] * m0 + v[4] * m1 + v[5] * m2;
+                    fy += v[4] * m0 + v[6] * m1 + v[7] * m2;
+                    fz += v[5] * m0 + v[7] * m1 + v[8] * m2;
+                }
+                a.f[io] = fx; a.f[n + io] = fy; a.f[2 * (size_t)n + io] = fz;
+            }
         }
     }
 }

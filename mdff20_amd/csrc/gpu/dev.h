@@ -28,6 +28,7 @@ struct BConfD {
     int off, n, npol;                     // ions [off, off + n), polarisable count
     int Kx, Ky, Kz, koff, nk;             // k box and half-space k-vectors [koff, koff + nk)
     int boxoff;                           // offset of the full (nx >= 0) k box in the box-ordered arrays
+    int awoff, wmoff;                     // offsets of the matrix-core sums' operand blocks
 };
 
 // per-type and per-type-pair tables, device resident

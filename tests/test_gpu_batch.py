@@ -107,11 +107,11 @@ def test_batch_autotune_keeps_results():
         b.engforce()
         F0 = b.forces()
         tune = b.autotune()
-        assert tune["lanes"] in (16, 32, 64) and tune["kpb"] in (32, 64, 128) and tune["ipb"] in (64, 128, 256)
+        assert tune["lanes"] in (16, 32, 64) and tune["kpb"] in (32, 64, 128) and tune["ipb"] in (16, 32, 64, 128, 256)
         b.reset_dipoles()
         b.engforce()
         assert rel(b.forces(), F0) < 1e-12
-        for lanes, kpb, ipb, bpc in [(32, 32, 128, 2), (64, 128, 256, 8)]:
+        for lanes, kpb, ipb, bpc in [(32, 32, 128, 2), (64, 128, 256, 8), (16, 0, 32, 0), (0, 0, 16, 0)]:
             b.set_tuning(lanes, kpb, ipb, bpc)
             b.reset_dipoles()
             b.engforce()
