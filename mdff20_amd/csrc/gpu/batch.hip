@@ -110,6 +110,7 @@ struct mdff_batch {
     double *rmsd2 = nullptr, *rmsd = nullptr, *seg = nullptr, *seg_h = nullptr;
     // adjoint
     double *g = nullptr, *sdir = nullptr, *srhs = nullptr, *lam = nullptr, *eT = nullptr, *edel = nullptr;
+    double *mu_prev = nullptr, *dmu = nullptr;   // carried-field SCF of the mixed form
     double *cols = nullptr, *tsum = nullptr;
     // tuning
     int lanes = 16, kpb = 64, ipb = 64, bpc = 4, tuned = 0;
@@ -400,14 +401,16 @@ static int build_list(mdff_batch *b)
     return MDFF_ERR_LIST_OVERFLOW;
 }
 
-// field of the dipoles mu at every ion of the active configurations, plus base
-static void induced_field(mdff_batch *b, int lo, const double *mu, const double *base, double *ef, const int *act)
+// field of the dipoles mu at every ion of the active configurations, plus
+// base (exact: FP64 k-space arithmetic in the mixed form)
+static void induced_field(mdff_batch *b, int lo, const double *mu, const double *base, double *ef, const int *act,
+                          bool exact = true)
 {
     int n = b->n;
     DirArgs a = bdir_args(b, b->qz, mu, 0, 1, act);
     launch_bdir(b, a, lo, 0);
-    launch_rho(b, lo, RM_MU, b->qz, mu, nullptr, 0, 0, act, true);
-    launch_back(b, lo, 3, nullptr, nullptr, b->ef_rec, nullptr, nullptr, act, true);
+    launch_rho(b, lo, RM_MU, b->qz, mu, nullptr, 0, 0, act, exact);
+    launch_back(b, lo, 3, nullptr, nullptr, b->ef_rec, nullptr, nullptr, act, exact);
     hipLaunchKernelGGL(kb_field_sum, dim3(bdiv(3 * n, 256)), dim3(256), 0, b->st, n, b->cfg, act, b->conf, mu,
                        b->ef_dir, b->ef_rec, base, ef);
 }
@@ -428,10 +431,13 @@ static int scf(mdff_batch *b)
     hipLaunchKernelGGL(kb_set_conf_int, dim3(bdiv(nc, 256)), dim3(256), 0, b->st, nc, b->act,
                        (maxit > 0 || minit > 0) ? 1 : 0);
     hipLaunchKernelGGL(kb_set_conf_int, dim3(bdiv(nc, 256)), dim3(256), 0, b->st, nc, b->iscf, 0);
-    // the mixed form runs the same Jacobi sequence on its low-precision field
-    // (FP32 real space and k-space tables, FP64 sums): its fixed point sits
-    // within the field's rounding (about 1e-6 relative) of the FP64 one
+    // the mixed form runs the same Jacobi sequence on a carried field:
+    // E_k = E_{k-1} + T (mu_k - mu_{k-1}), the increment's field in FP32
+    // (FP32 real space, FP32 matrix-core k-space), whose rounding shrinks
+    // with the increment, so the stop test at conv_tol_ind stays reachable;
+    // sweep 0 forms E_0 = E_stat + T mu_0 whole
     const int lo = lo_b(b);
+    const bool carry = lo && b->mf;
     int limit = std::max(maxit, minit) + 1;
     int launched = 0;
     while (launched < limit) {
@@ -439,7 +445,16 @@ static int scf(mdff_batch *b)
         for (int it = 0; it < chunk && launched < limit; it++, launched++) {
             hipLaunchKernelGGL(kb_scf_mu, dim3(bdiv(n, 256)), dim3(256), 0, b->st, n, b->cfg, b->act,
                                launched == 0 ? 1 : 0, b->type, b->tab_d, b->hist, b->ef, b->mu, b->upol_i);
-            induced_field(b, lo, b->mu, b->ef_stat, b->ef, b->act);
+            if (!carry) {
+                induced_field(b, lo, b->mu, b->ef_stat, b->ef, b->act);
+            } else if (launched == 0) {
+                induced_field(b, lo, b->mu, b->ef_stat, b->ef, b->act, false);
+                BCHK(hipMemcpyAsync(b->mu_prev, b->mu, sizeof(double) * 3 * n, hipMemcpyDeviceToDevice, b->st));
+            } else {
+                hipLaunchKernelGGL(kb_scf_dmu, dim3(bdiv(3 * n, 256)), dim3(256), 0, b->st, n, b->cfg, b->act, b->mu,
+                                   b->mu_prev, b->dmu);
+                induced_field(b, lo, b->dmu, b->ef, b->ef, b->act, false);
+            }
             hipLaunchKernelGGL(kb_scf_rmsd, dim3(bdiv(n, 256)), dim3(256), 0, b->st, n, b->cfg, b->act, b->type,
                                b->tab_d, b->mu, b->ef, b->rmsd_i);
             segsum(b, b->rmsd_i, n, 1, 0, b->rmsd2);
@@ -600,7 +615,7 @@ void mdff_batch_destroy(mdff_batch *b)
     bfree(b->rmsd2); bfree(b->rmsd); bfree(b->seg); bfree(b->g); bfree(b->sdir); bfree(b->srhs); bfree(b->lam);
     bfree(b->eT); bfree(b->edel); bfree(b->cols); bfree(b->tsum);
     bfree(b->boxk); bfree(b->kcfg); bfree(b->Sk); bfree(b->items_mf); bfree(b->items_mb);
-    bfree(b->aw); bfree(b->wm);
+    bfree(b->aw); bfree(b->wm); bfree(b->mu_prev); bfree(b->dmu);
     if (b->any_h) (void)hipHostFree(b->any_h);
     if (b->seg_h) free(b->seg_h);
     for (int i = 0; i < 4; i++)
@@ -780,6 +795,7 @@ int mdff_batch_create(mdff_batch **out, const mdff_params *p, int nconf, const i
     if (b->mf) {
         BALLOC(b->boxk, b->nbox); BALLOC(b->kcfg, b->nktot); BALLOC(b->Sk, 8 * b->nktot);
         BALLOC(b->aw, b->naw); BALLOC(b->wm, b->nwm);
+        if (precision == MDFF_MIXED) { BALLOC(b->mu_prev, n3); BALLOC(b->dmu, n3); }
     }
     BALLOC(b->tab64, 2 * (size_t)n * b->TS);
     BALLOC(b->tabT64, 2 * (size_t)n * b->TS);

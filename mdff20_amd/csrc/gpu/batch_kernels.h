@@ -869,6 +869,18 @@ __global__ void kb_scf_mu(int n, const int *__restrict__ cfg, const int *__restr
     upol[i] = 0.5 * up;
 }
 
+// dmu = mu - mu_prev, mu_prev = mu on the active configurations (the
+// increment whose field a carried-field SCF sweep adds)
+__global__ void kb_scf_dmu(int n, const int *__restrict__ cfg, const int *__restrict__ act,
+                           const double *__restrict__ mu, double *__restrict__ mu_prev, double *__restrict__ dmu)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= 3 * n || !act[cfg[i % n]]) return;
+    const double v = mu[i];
+    dmu[i] = v - mu_prev[i];
+    mu_prev[i] = v;
+}
+
 // get_rmsd_mu pim.c:431-449 per ion
 __global__ void kb_scf_rmsd(int n, const int *__restrict__ cfg, const int *__restrict__ act,
                             const int *__restrict__ type, const TypeTab *__restrict__ T, const double *__restrict__ mu,
