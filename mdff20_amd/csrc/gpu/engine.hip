@@ -126,6 +126,7 @@ struct mdff_ctx {
     int no_fuse = 0;          // tuning: separate partial sums, reduction and pair-tensor finish
     int fwd_nch = 0;          // fwd_part 2: K ranges of the partials
     const double *fwd_inv = nullptr;   // fwd_part 2: channel scales to undo
+    bool fwd_comb = false;             // fwd_part 2: partials hold rho = i k.S_mu (one channel)
     int nchunk = 0, nsplit = 0, chunk = 0, rows_per_split = 0;
     int nchunk_p = 0, chunk_p = 0;
     // tuning
@@ -675,7 +676,7 @@ template <int MODE, int CW, bool GEN>
 static void gfwd_mt(mdff_ctx *c, dim3 grid, int MT, const GFwdArgs &a)
 {
     const int MTI = g_mt_inst(MT);
-    const size_t lds = g_fwd_lds_bytes<MODE>(MTI, GEN);
+    const size_t lds = std::max(g_fwd_lds_bytes<MODE>(MTI, GEN), a.comb ? g_fwd_comb_lds_bytes(MTI, CW) : (size_t)0);
     if (MTI == 3) hipLaunchKernelGGL((k_gemm_fwd<MODE, CW, 3, GEN>), grid, dim3(256), lds, c->st, a);
     else if (MTI == 5) hipLaunchKernelGGL((k_gemm_fwd<MODE, CW, 5, GEN>), grid, dim3(256), lds, c->st, a);
     else if (MTI == 7) hipLaunchKernelGGL((k_gemm_fwd<MODE, CW, 7, GEN>), grid, dim3(256), lds, c->st, a);
@@ -856,6 +857,11 @@ static int launch_fwd_gemm(mdff_ctx *c, int polar, int lo, int C, const double *
     // defer: leave the FP32 partials to k_psum_reduce_cm (the column maxima
     // it fills are zeroed by the sum's first block)
     const bool defer = c->psum_defer && mode == KM_F16X3;
+    // three dipole channels: the sum writes rho = i k.S_mu (one channel)
+    const bool comb = defer && C == 3 && !c->no_fuse;
+    a.comb = comb;
+    a.kvec = c->kvec;
+    a.cinv = winv;
     // a deferred forward's block count is timed with the reduction that sums
     // its partials in the call (k_psum_reduce_cm; its outputs are written
     // again by the call's own reduction)
@@ -880,7 +886,13 @@ static int launch_fwd_gemm(mdff_ctx *c, int polar, int lo, int C, const double *
         if (probe) probe_mark(c, 1);
         if (!psum) return nch;
         if (defer) {
-            launch_psum_cm(c, tr, C, c->psum_ch, nch, winv);
+            if (comb) {
+                KRedArgs t1 = tr;
+                t1.C = 1;
+                launch_psum_cm(c, t1, 1, c->psum_ch, nch, nullptr);
+            } else {
+                launch_psum_cm(c, tr, C, c->psum_ch, nch, winv);
+            }
             return nch;
         }
         if (mode == KM_F64)
@@ -899,6 +911,7 @@ static int launch_fwd_gemm(mdff_ctx *c, int polar, int lo, int C, const double *
     c->fwd_part = defer ? 2 : 1;
     c->fwd_nch = nch;
     c->fwd_inv = winv;
+    c->fwd_comb = comb;
     *nchunk_out = 1;
     return 1;
 }
@@ -1058,7 +1071,11 @@ static int rec_part(mdff_ctx *c, int polar, int lo, int C, int NCH, const double
     if (cm_path && gemm_fwd) {
         void *w0 = c->wm[bmode][0], *w1 = c->wm[bmode][1];
         const dim3 gr(cdiv(c->nkf, 256));
-        if (raw) {
+        if (raw && c->fwd_comb) {
+            KRedArgs r1 = r;
+            r1.C = 1;
+            launch_psum_cm(c, r1, 1, CHb, c->fwd_nch, nullptr);
+        } else if (raw) {
             launch_psum_cm(c, r, C, CHb, c->fwd_nch, c->fwd_inv);
         } else if (CHb == 6) {
             hipLaunchKernelGGL(k_sk_reduce_cm<6>, gr, dim3(256), 0, c->st, r, c->gcm);
@@ -2979,11 +2996,18 @@ int mdff_time_kernel(mdff_ctx *c, int which, int lo, int iters, double *ms)
     auto once = [&]() {
         int dummy = 0;
         switch (which) {
-        case 0: case 3:
-            if (launch_fwd_mfma(c, polar, lo, C, a.q, mu, nullptr, &dummy)) break;
+        case 0: case 3: {
+            // as a force call launches it: the split-FP16 sum leaves its
+            // partials to the reduction (and folds the SCF's dipole channels)
+            c->psum_defer = !c->no_fuse;
+            c->psum_ch = NCH == 9 ? 6 : 3;
+            const bool done = launch_fwd_mfma(c, polar, lo, C, a.q, mu, nullptr, &dummy);
+            c->psum_defer = false;
+            if (done) break;
             if (lo) { if (C == 3) launch_fwd_t<float, 3>(c, a, nchunk); else launch_fwd_t<float, 4>(c, a, nchunk); }
             else { if (C == 3) launch_fwd_t<double, 3>(c, a, nchunk); else launch_fwd_t<double, 4>(c, a, nchunk); }
             break;
+        }
         case 1: case 4:
             if (launch_back_mfma(c, polar, lo, NCH, nullptr, &dummy)) break;
             if (lo) { if (NCH == 3) launch_back_t<float, 3>(c, b, nsel); else launch_back_t<float, 9>(c, b, nsel); }

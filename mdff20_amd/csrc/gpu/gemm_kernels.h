@@ -745,7 +745,17 @@ struct GFwdArgs {
     const int *active;
     unsigned long long *cmz;  // when set, block (0, 0) zeroes these ncmz column maxima
     int ncmz;                 // for the k_psum_reduce_cm that follows
+    // comb (three dipole channels, rows (nx, c, re/im) with M = 6 (Kx+1)):
+    // the partial sums are written as rho = i k.S_mu per box point, rows
+    // (nx, re/im), with the channels' weight scales cinv[3] and the k vectors
+    // kvec[(nx, p)][3] folded in (the reduction that follows takes them as
+    // one charge-like channel)
+    int comb;
+    const double *kvec, *cinv;
 };
+
+// LDS of a forward launch with the combined epilogue: the 4 waves' tiles
+inline size_t g_fwd_comb_lds_bytes(int MT, int CW) { return (size_t)4 * MT * 16 * (CW * 16 + 1) * sizeof(float); }
 
 // row tiles a forward kernel is instantiated for: every launch computes
 // this many (zero rows past M) so that no matrix instruction is predicated
@@ -970,6 +980,39 @@ __global__ void __launch_bounds__(256) k_gemm_fwd(GFwdArgs a)
     for (int kb = k0; kb < k1; kb += 2 * GK) {
         step(S0, kb);
         step(S1, kb + GK);
+    }
+    if constexpr (MODE != KM_F64) {
+        if (a.comb) {
+            // rho = i k.S_mu of the wave's points: its tiles through LDS (every
+            // wave is past the K loop's last LDS read first), then one (nx,
+            // point) per lane and pass
+            constexpr int TLD = CW * 16 + 1;
+            __syncthreads();
+            float *tw = (float *)smem_raw + (size_t)wave * MTMAX * 16 * TLD;
+#pragma unroll
+            for (int t = 0; t < MTMAX; t++)
+#pragma unroll
+                for (int c = 0; c < CW; c++)
+#pragma unroll
+                    for (int i = 0; i < 4; i++) tw[(t * 16 + 4 * g + i) * TLD + c * 16 + cl] = acc[t][c][i];
+            __syncthreads();
+            const double s0 = a.cinv ? a.cinv[0] : 1.0, s1 = a.cinv ? a.cinv[1] : 1.0, s2 = a.cinv ? a.cinv[2] : 1.0;
+            const int KX1 = a.M / 6;
+            float *outc = (float *)a.part + (size_t)by * 2 * KX1 * a.P;
+            for (int q = lane; q < KX1 * CW * 16; q += 64) {
+                const int nx = q / (CW * 16), cc = q - nx * (CW * 16);
+                const int p = ct0 * 16 + cc;
+                if (p >= a.P) continue;
+                const float *r0 = tw + (size_t)(nx * 6) * TLD + cc;
+                const size_t kf = (size_t)nx * a.P + p;
+                const double kx = a.kvec[3 * kf] * s0, ky = a.kvec[3 * kf + 1] * s1, kz = a.kvec[3 * kf + 2] * s2;
+                const double sxr = r0[0], sxi = r0[TLD], syr = r0[2 * TLD], syi = r0[3 * TLD], szr = r0[4 * TLD],
+                             szi = r0[5 * TLD];
+                outc[(size_t)(2 * nx) * a.P + p] = (float)(-(kx * sxi + ky * syi + kz * szi));
+                outc[(size_t)(2 * nx + 1) * a.P + p] = (float)(kx * sxr + ky * syr + kz * szr);
+            }
+            return;
+        }
     }
     // partial sums of this K range, [row][point] (lanes write consecutive
     // points; FP32 accumulators are stored as FP32)
