@@ -169,7 +169,7 @@ def batch_relax(confs, ews, iters=30, gamma=0.01, cap=0.05):
     return [(t, A, rr.copy()) for (t, A, _), rr in zip(confs, parts)]
 
 
-def batch_bench(seed=4096):
+def batch_bench(seed=4096, which=("fp64", "mixed", "cfg5")):
     """the batched engine on its two workloads: 512 uneven cells (N log-uniform
     in [96, 1536], per-cell Ewald for 1e-6) force calls in the FP64 and mixed
     forms, and the force-matching adjoint on 64 cells of 648 ions from NVT
@@ -181,6 +181,8 @@ def batch_bench(seed=4096):
     p = params_of(control(True))
     nion = sum(len(c[0]) for c in confs)
     for prec, name in ((m.FP64, "fp64"), (m.MIXED, "mixed")):
+        if name not in which:
+            continue
         with m.batch.Batch(p, confs, ews, precision=prec) as b:
             b.engforce()
             # each call a fresh static evaluation, SCF from zero dipoles as
@@ -200,6 +202,8 @@ def batch_bench(seed=4096):
                                    scf_sweeps_mean=float(np.mean(nscf)), scf_sweeps_max=int(np.max(nscf)),
                                    warm_ms_per_force_call=1e3 * dw)
     out["cfg4_cells"] = dict(n=512, ions=nion)
+    if "cfg5" not in which:
+        return out
     # cfg5: 4 seeded starts, 16 cells each from NVT dynamics (kmax 8, 21.4 A cell)
     q = params_of(control(True, alpha=0.3, kmax=8, rc=9.0))
     q.conv_tol_ind = 1e-10
@@ -471,6 +475,9 @@ def main():
                     help="skip the per-kernel timing of every arithmetic variant, the accuracy call and the "
                          "FP64 residual check (a kernel trace then holds the timed MD steps of the chosen forms only)")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-engine workloads (512 cells, adjoint)")
+    ap.add_argument("--batch-only", default=None, metavar="LIST",
+                    help="run only these batched-engine workloads (comma-separated: fp64, mixed, cfg5) and print "
+                         "their figures (kernel traces of one form)")
     ap.add_argument("--state", default=None, metavar="NPZ",
                     help="relaxed positions: read from NPZ when it exists, else relaxed and written there")
     args = ap.parse_args()
@@ -493,6 +500,9 @@ def main():
         raise RuntimeError("no HIP device")
     if args.pmc_kernel:
         pmc_run(args, args.pmc_kernel)
+        return
+    if args.batch_only:
+        print(json.dumps(batch_bench(which=tuple(args.batch_only.split(",")))))
         return
     L, pos, typ = system(args.units, args.seed)
     n = len(typ)
