@@ -110,7 +110,11 @@ struct mdff_batch {
     double *rmsd2 = nullptr, *rmsd = nullptr, *seg = nullptr, *seg_h = nullptr;
     // adjoint
     double *g = nullptr, *sdir = nullptr, *srhs = nullptr, *lam = nullptr, *eT = nullptr, *edel = nullptr;
-    double *mu_prev = nullptr, *dmu = nullptr;   // carried-field SCF of the mixed form
+    // conjugate-gradient SCF of the mixed form: r, z, p, w = T p, q = A p
+    // [3][n]; per-ion terms [2][n]; per-configuration sums and scalars
+    int cg = 0;
+    double *cg_r = nullptr, *cg_z = nullptr, *cg_p = nullptr, *cg_w = nullptr, *cg_q = nullptr, *cg_part = nullptr;
+    double *cg_s = nullptr, *cg_pq = nullptr, *cg_rz = nullptr, *cg_beta = nullptr;
     double *cols = nullptr, *tsum = nullptr;
     // tuning
     int lanes = 16, kpb = 64, ipb = 64, bpc = 4, tuned = 0;
@@ -431,13 +435,51 @@ static int scf(mdff_batch *b)
     hipLaunchKernelGGL(kb_set_conf_int, dim3(bdiv(nc, 256)), dim3(256), 0, b->st, nc, b->act,
                        (maxit > 0 || minit > 0) ? 1 : 0);
     hipLaunchKernelGGL(kb_set_conf_int, dim3(bdiv(nc, 256)), dim3(256), 0, b->st, nc, b->iscf, 0);
-    // the mixed form runs the same Jacobi sequence on a carried field:
-    // E_k = E_{k-1} + T (mu_k - mu_{k-1}), the increment's field in FP32
-    // (FP32 real space, FP32 matrix-core k-space), whose rounding shrinks
-    // with the increment, so the stop test at conv_tol_ind stays reachable;
-    // sweep 0 forms E_0 = E_stat + T mu_0 whole
     const int lo = lo_b(b);
-    const bool carry = lo && b->mf;
+    if (b->cg) {
+        // the mixed form: conjugate gradients per configuration (the system
+        // and its carried residual: kb_cg_*), T applied in FP32 (FP32 real
+        // space and matrix-core k-space) to the search direction only;
+        // iteration 0 forms E_0 = E_stat + T mu_0 whole
+        const dim3 g1(bdiv(n, 256)), g3(bdiv(3 * n, 256));
+        hipLaunchKernelGGL(kb_scf_mu, g1, dim3(256), 0, b->st, n, b->cfg, b->act, 1, b->type, b->tab_d, b->hist, b->ef,
+                           b->mu, b->upol_i);
+        induced_field(b, lo, b->mu, b->ef_stat, b->ef, b->act, false);
+        hipLaunchKernelGGL(kb_cg_start, g1, dim3(256), 0, b->st, n, b->cfg, b->act, b->type, b->tab_d, b->ef, b->mu,
+                           b->cg_r, b->cg_z, b->cg_p, b->cg_part);
+        segsum(b, b->cg_part, n, 2, 0, b->cg_s);
+        hipLaunchKernelGGL(kb_cg_control, dim3(1), dim3(1024), 0, b->st, nc, b->conf, b->cg_s, 1, b->act, b->iscf,
+                           b->rmsd, b->cg_rz, b->cg_beta, maxit, minit, b->p.conv_tol_ind, b->any);
+        b->sweeps++;
+        int done = 1;
+        const int lim = std::max(maxit, minit) + 1;
+        while (done < lim) {
+            int rc = read_any(b);
+            if (rc) return rc;
+            if (!*b->any_h) break;
+            for (int it = 0; it < 4 && done < lim; it++, done++) {
+                induced_field(b, lo, b->cg_p, nullptr, b->cg_w, b->act, false);
+                hipLaunchKernelGGL(kb_cg_pq, g1, dim3(256), 0, b->st, n, b->cfg, b->act, b->type, b->tab_d, b->cg_p,
+                                   b->cg_w, b->cg_q, b->cg_part);
+                segsum(b, b->cg_part, n, 1, 0, b->cg_pq);
+                hipLaunchKernelGGL(kb_cg_update, g1, dim3(256), 0, b->st, n, b->cfg, b->act, b->type, b->tab_d,
+                                   b->cg_rz, b->cg_pq, b->cg_p, b->cg_q, b->cg_w, b->mu, b->cg_r, b->ef, b->cg_z,
+                                   b->cg_part);
+                segsum(b, b->cg_part, n, 2, 0, b->cg_s);
+                hipLaunchKernelGGL(kb_cg_control, dim3(1), dim3(1024), 0, b->st, nc, b->conf, b->cg_s, 0, b->act,
+                                   b->iscf, b->rmsd, b->cg_rz, b->cg_beta, maxit, minit, b->p.conv_tol_ind, b->any);
+                hipLaunchKernelGGL(kb_cg_dir, g3, dim3(256), 0, b->st, n, b->cfg, b->act, b->cg_beta, b->cg_z, b->cg_p);
+                b->sweeps++;
+            }
+        }
+        hipLaunchKernelGGL(kb_cg_upol, g1, dim3(256), 0, b->st, n, b->type, b->tab_d, b->mu, b->upol_i);
+        BCHK(hipMemcpyAsync(b->hist, b->mu, sizeof(double) * 3 * n, hipMemcpyDeviceToDevice, b->st));
+        return MDFF_OK;
+    }
+    // the FP64 form (and a mixed batch whose polarisabilities are not
+    // symmetric) follows the reference's Jacobi sequence; the mixed form's
+    // fields then keep FP64 k-space arithmetic on its FP32 tables (linear in
+    // mu to FP64 rounding, as the stop test at conv_tol_ind needs)
     int limit = std::max(maxit, minit) + 1;
     int launched = 0;
     while (launched < limit) {
@@ -445,16 +487,7 @@ static int scf(mdff_batch *b)
         for (int it = 0; it < chunk && launched < limit; it++, launched++) {
             hipLaunchKernelGGL(kb_scf_mu, dim3(bdiv(n, 256)), dim3(256), 0, b->st, n, b->cfg, b->act,
                                launched == 0 ? 1 : 0, b->type, b->tab_d, b->hist, b->ef, b->mu, b->upol_i);
-            if (!carry) {
-                induced_field(b, lo, b->mu, b->ef_stat, b->ef, b->act);
-            } else if (launched == 0) {
-                induced_field(b, lo, b->mu, b->ef_stat, b->ef, b->act, false);
-                BCHK(hipMemcpyAsync(b->mu_prev, b->mu, sizeof(double) * 3 * n, hipMemcpyDeviceToDevice, b->st));
-            } else {
-                hipLaunchKernelGGL(kb_scf_dmu, dim3(bdiv(3 * n, 256)), dim3(256), 0, b->st, n, b->cfg, b->act, b->mu,
-                                   b->mu_prev, b->dmu);
-                induced_field(b, lo, b->dmu, b->ef, b->ef, b->act, false);
-            }
+            induced_field(b, lo, b->mu, b->ef_stat, b->ef, b->act);
             hipLaunchKernelGGL(kb_scf_rmsd, dim3(bdiv(n, 256)), dim3(256), 0, b->st, n, b->cfg, b->act, b->type,
                                b->tab_d, b->mu, b->ef, b->rmsd_i);
             segsum(b, b->rmsd_i, n, 1, 0, b->rmsd2);
@@ -615,7 +648,9 @@ void mdff_batch_destroy(mdff_batch *b)
     bfree(b->rmsd2); bfree(b->rmsd); bfree(b->seg); bfree(b->g); bfree(b->sdir); bfree(b->srhs); bfree(b->lam);
     bfree(b->eT); bfree(b->edel); bfree(b->cols); bfree(b->tsum);
     bfree(b->boxk); bfree(b->kcfg); bfree(b->Sk); bfree(b->items_mf); bfree(b->items_mb);
-    bfree(b->aw); bfree(b->wm); bfree(b->mu_prev); bfree(b->dmu);
+    bfree(b->aw); bfree(b->wm);
+    bfree(b->cg_r); bfree(b->cg_z); bfree(b->cg_p); bfree(b->cg_w); bfree(b->cg_q); bfree(b->cg_part);
+    bfree(b->cg_s); bfree(b->cg_pq); bfree(b->cg_rz); bfree(b->cg_beta);
     if (b->any_h) (void)hipHostFree(b->any_h);
     if (b->seg_h) free(b->seg_h);
     for (int i = 0; i < 4; i++)
@@ -795,7 +830,21 @@ int mdff_batch_create(mdff_batch **out, const mdff_params *p, int nconf, const i
     if (b->mf) {
         BALLOC(b->boxk, b->nbox); BALLOC(b->kcfg, b->nktot); BALLOC(b->Sk, 8 * b->nktot);
         BALLOC(b->aw, b->naw); BALLOC(b->wm, b->nwm);
-        if (precision == MDFF_MIXED) { BALLOC(b->mu_prev, n3); BALLOC(b->dmu, n3); }
+    }
+    // the mixed form's CG SCF needs the matrix-core sums and symmetric
+    // polarisability tensors (the system must be symmetric)
+    b->cg = precision == MDFF_MIXED && b->mf && b->lpol;
+    for (int t = 0; t < b->ntype && b->cg; t++)
+        if (b->tab.polar[t])
+            for (int i = 0; i < 3; i++)
+                for (int j = 0; j < 3; j++)
+                    if (fabs(b->tab.pol[t][3 * i + j] - b->tab.pol[t][3 * j + i]) >
+                        1e-12 * (fabs(b->tab.pol[t][3 * i + j]) + fabs(b->tab.pol[t][3 * j + i])))
+                        b->cg = 0;
+    if (b->cg) {
+        BALLOC(b->cg_r, n3); BALLOC(b->cg_z, n3); BALLOC(b->cg_p, n3); BALLOC(b->cg_w, n3); BALLOC(b->cg_q, n3);
+        BALLOC(b->cg_part, 2 * (size_t)n); BALLOC(b->cg_s, 2 * (size_t)nconf); BALLOC(b->cg_pq, nconf);
+        BALLOC(b->cg_rz, nconf); BALLOC(b->cg_beta, nconf);
     }
     BALLOC(b->tab64, 2 * (size_t)n * b->TS);
     BALLOC(b->tabT64, 2 * (size_t)n * b->TS);

@@ -869,16 +869,181 @@ __global__ void kb_scf_mu(int n, const int *__restrict__ cfg, const int *__restr
     upol[i] = 0.5 * up;
 }
 
-// dmu = mu - mu_prev, mu_prev = mu on the active configurations (the
-// increment whose field a carried-field SCF sweep adds)
-__global__ void kb_scf_dmu(int n, const int *__restrict__ cfg, const int *__restrict__ act,
-                           const double *__restrict__ mu, double *__restrict__ mu_prev, double *__restrict__ dmu)
+// ------------------------------- conjugate-gradient SCF of the mixed batch
+// Per configuration, the fixed point mu = pol^T E(mu), E(mu) = E_stat + T mu,
+// is the symmetric system (pol^-T - T) mu = E_stat (pol symmetric, T the
+// dipole tensor with its self term): preconditioned by pol^T, with the field
+// E and the residual r = E - pol^-T mu carried along (each step applies T
+// to the search direction only).  Per-ion terms go to part[col][n] and are
+// summed per configuration by kb_segsum; the stop test is get_rmsd_mu's
+// (pim.c:431-449) on the carried field.
+
+// (pol^-T x)_c = sum_d invpol[d][c] x_d; (pol^T x)_c = sum_d pol[d][c] x_d
+__device__ __forceinline__ void cg_polT(const double *P, const double x[3], double y[3])
+{
+    for (int c = 0; c < 3; c++) y[c] = P[0 * 3 + c] * x[0] + P[1 * 3 + c] * x[1] + P[2 * 3 + c] * x[2];
+}
+
+__device__ __forceinline__ double cg_rmsd_ion(const double *P, const double m[3], const double e[3])
+{
+    double r = 0.0;
+    for (int c = 0; c < 3; c++) {
+        const double paa = P[4 * c];
+        if (paa != 0.0) {
+            const double d = m[c] / paa - e[c];
+            r += d * d;
+        }
+    }
+    return r;
+}
+
+// start: r = E - pol^-T mu, z = pol^T r, p = z; part[0] = r.z, part[1] the
+// rmsd term of (mu, E)
+__global__ void kb_cg_start(int n, const int *__restrict__ cfg, const int *__restrict__ act,
+                            const int *__restrict__ type, const TypeTab *__restrict__ T, const double *__restrict__ ef,
+                            const double *__restrict__ mu, double *__restrict__ r, double *__restrict__ z,
+                            double *__restrict__ p, double *__restrict__ part)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= 3 * n || !act[cfg[i % n]]) return;
-    const double v = mu[i];
-    dmu[i] = v - mu_prev[i];
-    mu_prev[i] = v;
+    if (i >= n) return;
+    double rz = 0.0, rm = 0.0;
+    if (act[cfg[i]]) {
+        const int t = type[i];
+        double rv[3] = {0, 0, 0}, zv[3] = {0, 0, 0};
+        if (T->polar[t]) {
+            const double m[3] = {mu[i], mu[n + i], mu[2 * n + i]}, e[3] = {ef[i], ef[n + i], ef[2 * n + i]};
+            double im[3];
+            cg_polT(T->invpol[t], m, im);
+            for (int c = 0; c < 3; c++) rv[c] = e[c] - im[c];
+            cg_polT(T->pol[t], rv, zv);
+            rz = rv[0] * zv[0] + rv[1] * zv[1] + rv[2] * zv[2];
+            rm = cg_rmsd_ion(T->pol[t], m, e);
+        }
+        for (int c = 0; c < 3; c++) {
+            r[(size_t)c * n + i] = rv[c];
+            z[(size_t)c * n + i] = zv[c];
+            p[(size_t)c * n + i] = zv[c];
+        }
+    }
+    part[i] = rz;
+    part[n + i] = rm;
+}
+
+// q = pol^-T p - w (w = T p); part[0] = p.q
+__global__ void kb_cg_pq(int n, const int *__restrict__ cfg, const int *__restrict__ act,
+                         const int *__restrict__ type, const TypeTab *__restrict__ T, const double *__restrict__ p,
+                         const double *__restrict__ w, double *__restrict__ q, double *__restrict__ part)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double pq = 0.0;
+    if (act[cfg[i]]) {
+        const int t = type[i];
+        const double pv[3] = {p[i], p[n + i], p[2 * n + i]};
+        double ip[3] = {0, 0, 0};
+        if (T->polar[t]) cg_polT(T->invpol[t], pv, ip);
+        for (int c = 0; c < 3; c++) {
+            const double qc = ip[c] - w[(size_t)c * n + i];
+            q[(size_t)c * n + i] = qc;
+            pq += pv[c] * qc;
+        }
+    }
+    part[i] = pq;
+}
+
+// alpha = rz / pq per configuration; mu += alpha p, r -= alpha q, E += alpha
+// w, z = pol^T r; part[0] = r.z, part[1] the rmsd term of (mu, E)
+__global__ void kb_cg_update(int n, const int *__restrict__ cfg, const int *__restrict__ act,
+                             const int *__restrict__ type, const TypeTab *__restrict__ T,
+                             const double *__restrict__ rz, const double *__restrict__ pqs,
+                             const double *__restrict__ p, const double *__restrict__ q,
+                             const double *__restrict__ w, double *__restrict__ mu, double *__restrict__ r,
+                             double *__restrict__ ef, double *__restrict__ z, double *__restrict__ part)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int c0 = cfg[i];
+    double rzn = 0.0, rm = 0.0;
+    if (act[c0]) {
+        const double pq = pqs[c0];
+        const double al = pq > 0.0 ? rz[c0] / pq : 0.0;
+        const int t = type[i];
+        double m[3], rv[3], e[3];
+        for (int c = 0; c < 3; c++) {
+            const size_t k = (size_t)c * n + i;
+            m[c] = mu[k] + al * p[k];
+            rv[c] = r[k] - al * q[k];
+            e[c] = ef[k] + al * w[k];
+            mu[k] = m[c];
+            r[k] = rv[c];
+            ef[k] = e[c];
+        }
+        double zv[3] = {0, 0, 0};
+        if (T->polar[t]) {
+            cg_polT(T->pol[t], rv, zv);
+            rzn = rv[0] * zv[0] + rv[1] * zv[1] + rv[2] * zv[2];
+            rm = cg_rmsd_ion(T->pol[t], m, e);
+        }
+        for (int c = 0; c < 3; c++) z[(size_t)c * n + i] = zv[c];
+    }
+    part[i] = rzn;
+    part[n + i] = rm;
+}
+
+// the loop condition of pim.c:236 per configuration on its rmsd (sums
+// s[c][0] = r.z, s[c][1] = rmsd^2 npol), and beta = r.z / r.z_old; one block
+__global__ void __launch_bounds__(1024) kb_cg_control(int nconf, const BConfD *__restrict__ conf,
+                                                      const double *__restrict__ s, int first, int *act, int *iscf,
+                                                      double *rmsd, double *rz, double *beta, int maxit, int minit,
+                                                      double tol, int *any)
+{
+    __shared__ int sany;
+    if (threadIdx.x == 0) sany = 0;
+    __syncthreads();
+    for (int c = threadIdx.x; c < nconf; c += blockDim.x) {
+        if (!act[c]) continue;
+        const int it = iscf[c] + 1;
+        const int np = conf[c].npol;
+        const double rm = sqrt(s[2 * c + 1] / (double)(np > 0 ? np : 1));
+        const double rzn = s[2 * c];
+        iscf[c] = it;
+        rmsd[c] = rm;
+        beta[c] = (!first && rz[c] > 0.0) ? rzn / rz[c] : 0.0;
+        rz[c] = rzn;
+        const int a = ((it < maxit && rm > tol) || it < minit) ? 1 : 0;
+        act[c] = a;
+        if (a) atomicOr(&sany, 1);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) *any = sany;
+}
+
+// p = z + beta p on the active configurations
+__global__ void kb_cg_dir(int n, const int *__restrict__ cfg, const int *__restrict__ act,
+                          const double *__restrict__ beta, const double *__restrict__ z, double *__restrict__ p)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= 3 * n) return;
+    const int c0 = cfg[i % n];
+    if (!act[c0]) return;
+    p[i] = z[i] + beta[c0] * p[i];
+}
+
+// u_pol = mu.pol^-1 mu / 2 of the final dipoles (pim.c:253,299)
+__global__ void kb_cg_upol(int n, const int *__restrict__ type, const TypeTab *__restrict__ T,
+                           const double *__restrict__ mu, double *__restrict__ upol)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int t = type[i];
+    double up = 0.0;
+    if (T->polar[t]) {
+        const double m[3] = {mu[i], mu[n + i], mu[2 * n + i]};
+        const double *I = T->invpol[t];
+        for (int c = 0; c < 3; c++)
+            for (int d = 0; d < 3; d++) up += m[c] * I[3 * c + d] * m[d];
+    }
+    upol[i] = 0.5 * up;
 }
 
 // get_rmsd_mu pim.c:431-449 per ion
