@@ -112,7 +112,8 @@ struct mdff_batch {
     double *g = nullptr, *sdir = nullptr, *srhs = nullptr, *lam = nullptr, *eT = nullptr, *edel = nullptr;
     // conjugate-gradient SCF of the mixed form: r, z, p, w = T p, q = A p
     // [3][n]; per-ion terms [2][n]; per-configuration sums and scalars
-    int cg = 0;
+    int cg = 0, cg_sym = 0;
+    double *cg_e = nullptr;   // the adjoint solve's carried s + T lam
     double *cg_r = nullptr, *cg_z = nullptr, *cg_p = nullptr, *cg_w = nullptr, *cg_q = nullptr, *cg_part = nullptr;
     double *cg_s = nullptr, *cg_pq = nullptr, *cg_rz = nullptr, *cg_beta = nullptr;
     double *cols = nullptr, *tsum = nullptr;
@@ -446,10 +447,11 @@ static int scf(mdff_batch *b)
                            b->mu, b->upol_i);
         induced_field(b, lo, b->mu, b->ef_stat, b->ef, b->act, false);
         hipLaunchKernelGGL(kb_cg_start, g1, dim3(256), 0, b->st, n, b->cfg, b->act, b->type, b->tab_d, b->ef, b->mu,
-                           b->cg_r, b->cg_z, b->cg_p, b->cg_part);
+                           b->cg_r, b->cg_z, b->cg_p, b->cg_part, 0);
         segsum(b, b->cg_part, n, 2, 0, b->cg_s);
         hipLaunchKernelGGL(kb_cg_control, dim3(1), dim3(1024), 0, b->st, nc, b->conf, b->cg_s, 1, b->act, b->iscf,
-                           b->rmsd, b->cg_rz, b->cg_beta, maxit, minit, b->p.conv_tol_ind, b->any);
+                           b->rmsd, b->cg_rz, b->cg_beta, maxit, minit, b->p.conv_tol_ind, b->any,
+                           (const double *)nullptr);
         b->sweeps++;
         int done = 1;
         const int lim = std::max(maxit, minit) + 1;
@@ -464,10 +466,11 @@ static int scf(mdff_batch *b)
                 segsum(b, b->cg_part, n, 1, 0, b->cg_pq);
                 hipLaunchKernelGGL(kb_cg_update, g1, dim3(256), 0, b->st, n, b->cfg, b->act, b->type, b->tab_d,
                                    b->cg_rz, b->cg_pq, b->cg_p, b->cg_q, b->cg_w, b->mu, b->cg_r, b->ef, b->cg_z,
-                                   b->cg_part);
+                                   b->cg_part, 0);
                 segsum(b, b->cg_part, n, 2, 0, b->cg_s);
                 hipLaunchKernelGGL(kb_cg_control, dim3(1), dim3(1024), 0, b->st, nc, b->conf, b->cg_s, 0, b->act,
-                                   b->iscf, b->rmsd, b->cg_rz, b->cg_beta, maxit, minit, b->p.conv_tol_ind, b->any);
+                                   b->iscf, b->rmsd, b->cg_rz, b->cg_beta, maxit, minit, b->p.conv_tol_ind, b->any,
+                                   (const double *)nullptr);
                 hipLaunchKernelGGL(kb_cg_dir, g3, dim3(256), 0, b->st, n, b->cfg, b->act, b->cg_beta, b->cg_z, b->cg_p);
                 b->sweeps++;
             }
@@ -650,7 +653,7 @@ void mdff_batch_destroy(mdff_batch *b)
     bfree(b->boxk); bfree(b->kcfg); bfree(b->Sk); bfree(b->items_mf); bfree(b->items_mb);
     bfree(b->aw); bfree(b->wm);
     bfree(b->cg_r); bfree(b->cg_z); bfree(b->cg_p); bfree(b->cg_w); bfree(b->cg_q); bfree(b->cg_part);
-    bfree(b->cg_s); bfree(b->cg_pq); bfree(b->cg_rz); bfree(b->cg_beta);
+    bfree(b->cg_s); bfree(b->cg_pq); bfree(b->cg_rz); bfree(b->cg_beta); bfree(b->cg_e);
     if (b->any_h) (void)hipHostFree(b->any_h);
     if (b->seg_h) free(b->seg_h);
     for (int i = 0; i < 4; i++)
@@ -831,18 +834,21 @@ int mdff_batch_create(mdff_batch **out, const mdff_params *p, int nconf, const i
         BALLOC(b->boxk, b->nbox); BALLOC(b->kcfg, b->nktot); BALLOC(b->Sk, 8 * b->nktot);
         BALLOC(b->aw, b->naw); BALLOC(b->wm, b->nwm);
     }
-    // the mixed form's CG SCF needs the matrix-core sums and symmetric
-    // polarisability tensors (the system must be symmetric)
-    b->cg = precision == MDFF_MIXED && b->mf && b->lpol;
-    for (int t = 0; t < b->ntype && b->cg; t++)
+    // conjugate gradients (the mixed form's SCF, the FP64 form's adjoint
+    // solve) need symmetric polarisability tensors (a symmetric system); the
+    // SCF's also the matrix-core sums
+    b->cg_sym = b->lpol;
+    for (int t = 0; t < b->ntype && b->cg_sym; t++)
         if (b->tab.polar[t])
             for (int i = 0; i < 3; i++)
                 for (int j = 0; j < 3; j++)
                     if (fabs(b->tab.pol[t][3 * i + j] - b->tab.pol[t][3 * j + i]) >
                         1e-12 * (fabs(b->tab.pol[t][3 * i + j]) + fabs(b->tab.pol[t][3 * j + i])))
-                        b->cg = 0;
-    if (b->cg) {
+                        b->cg_sym = 0;
+    b->cg = precision == MDFF_MIXED && b->mf && b->cg_sym;
+    if (b->cg_sym) {
         BALLOC(b->cg_r, n3); BALLOC(b->cg_z, n3); BALLOC(b->cg_p, n3); BALLOC(b->cg_w, n3); BALLOC(b->cg_q, n3);
+        BALLOC(b->cg_e, n3);
         BALLOC(b->cg_part, 2 * (size_t)n); BALLOC(b->cg_s, 2 * (size_t)nconf); BALLOC(b->cg_pq, nconf);
         BALLOC(b->cg_rz, nconf); BALLOC(b->cg_beta, nconf);
     }
@@ -1041,7 +1047,11 @@ int mdff_batch_adjoint(mdff_batch *b, const double *fref_x, const double *fref_y
                            b->efg_rec, b->g, b->edel, b->srhs, b->rmsd_i);
         double *s2 = T + 8, *r2 = T + 8 + nc;
         segsum(b, b->rmsd_i, n, 1, 0, s2);
-        // (pol^-1 - T) lam = s by Jacobi sweeps from lam = pol s
+        // (pol^-1 - T) lam = s from lam = pol s: conjugate gradients per
+        // configuration (symmetric pol, as the SCF of the mixed form, in FP64),
+        // then Jacobi sweeps until the recomputed residual meets the test
+        // (one sweep after a converged CG: its residual is carried, not
+        // recomputed)
         hipLaunchKernelGGL(kb_adj_start, dim3(bdiv(n, 256)), dim3(256), 0, b->st, n, b->type, b->tab_d, b->srhs,
                            b->lam);
         hipLaunchKernelGGL(kb_set_conf_int, dim3(bdiv(nc, 256)), dim3(256), 0, b->st, nc, b->act, 1);
@@ -1049,6 +1059,37 @@ int mdff_batch_adjoint(mdff_batch *b, const double *fref_x, const double *fref_y
         const int maxit = 1000;
         const double tol = 1e-13;
         int launched = 0;
+        if (b->cg_sym) {
+            const dim3 g1(bdiv(n, 256)), g3(bdiv(3 * n, 256));
+            induced_field(b, 0, b->lam, nullptr, b->eT, b->act);
+            hipLaunchKernelGGL(kb_add3, g3, dim3(256), 0, b->st, n3, b->srhs, b->eT, b->cg_e);
+            hipLaunchKernelGGL(kb_cg_start, g1, dim3(256), 0, b->st, n, b->cfg, b->act, b->type, b->tab_d, b->cg_e,
+                               b->lam, b->cg_r, b->cg_z, b->cg_p, b->cg_part, 1);
+            segsum(b, b->cg_part, n, 2, 0, b->cg_s);
+            hipLaunchKernelGGL(kb_cg_control, dim3(1), dim3(1024), 0, b->st, nc, b->conf, b->cg_s, 1, b->act,
+                               b->iscf, b->rmsd, b->cg_rz, b->cg_beta, maxit, 0, tol, b->any, (const double *)s2);
+            int done = 1;
+            while (done < maxit) {
+                if ((rc = read_any(b))) return rc;
+                if (!*b->any_h) break;
+                for (int k = 0; k < 4 && done < maxit; k++, done++) {
+                    induced_field(b, 0, b->cg_p, nullptr, b->cg_w, b->act);
+                    hipLaunchKernelGGL(kb_cg_pq, g1, dim3(256), 0, b->st, n, b->cfg, b->act, b->type, b->tab_d,
+                                       b->cg_p, b->cg_w, b->cg_q, b->cg_part);
+                    segsum(b, b->cg_part, n, 1, 0, b->cg_pq);
+                    hipLaunchKernelGGL(kb_cg_update, g1, dim3(256), 0, b->st, n, b->cfg, b->act, b->type, b->tab_d,
+                                       b->cg_rz, b->cg_pq, b->cg_p, b->cg_q, b->cg_w, b->lam, b->cg_r, b->cg_e,
+                                       b->cg_z, b->cg_part, 1);
+                    segsum(b, b->cg_part, n, 2, 0, b->cg_s);
+                    hipLaunchKernelGGL(kb_cg_control, dim3(1), dim3(1024), 0, b->st, nc, b->conf, b->cg_s, 0, b->act,
+                                       b->iscf, b->rmsd, b->cg_rz, b->cg_beta, maxit, 0, tol, b->any,
+                                       (const double *)s2);
+                    hipLaunchKernelGGL(kb_cg_dir, g3, dim3(256), 0, b->st, n, b->cfg, b->act, b->cg_beta, b->cg_z,
+                                       b->cg_p);
+                }
+            }
+            hipLaunchKernelGGL(kb_set_conf_int, dim3(bdiv(nc, 256)), dim3(256), 0, b->st, nc, b->act, 1);
+        }
         while (launched < maxit) {
             for (int k = 0; k < 4 && launched < maxit; k++, launched++) {
                 induced_field(b, 0, b->lam, nullptr, b->eT, b->act);

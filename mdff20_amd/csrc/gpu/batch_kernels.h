@@ -898,11 +898,12 @@ __device__ __forceinline__ double cg_rmsd_ion(const double *P, const double m[3]
 }
 
 // start: r = E - pol^-T mu, z = pol^T r, p = z; part[0] = r.z, part[1] the
-// rmsd term of (mu, E)
+// rmsd term of (mu, E) (adj = 0: the SCF) or |r|^2 (adj = 1: the adjoint
+// solve, whose "field" is its right-hand side plus T lam)
 __global__ void kb_cg_start(int n, const int *__restrict__ cfg, const int *__restrict__ act,
                             const int *__restrict__ type, const TypeTab *__restrict__ T, const double *__restrict__ ef,
                             const double *__restrict__ mu, double *__restrict__ r, double *__restrict__ z,
-                            double *__restrict__ p, double *__restrict__ part)
+                            double *__restrict__ p, double *__restrict__ part, int adj)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -917,7 +918,7 @@ __global__ void kb_cg_start(int n, const int *__restrict__ cfg, const int *__res
             for (int c = 0; c < 3; c++) rv[c] = e[c] - im[c];
             cg_polT(T->pol[t], rv, zv);
             rz = rv[0] * zv[0] + rv[1] * zv[1] + rv[2] * zv[2];
-            rm = cg_rmsd_ion(T->pol[t], m, e);
+            rm = adj ? rv[0] * rv[0] + rv[1] * rv[1] + rv[2] * rv[2] : cg_rmsd_ion(T->pol[t], m, e);
         }
         for (int c = 0; c < 3; c++) {
             r[(size_t)c * n + i] = rv[c];
@@ -952,13 +953,13 @@ __global__ void kb_cg_pq(int n, const int *__restrict__ cfg, const int *__restri
 }
 
 // alpha = rz / pq per configuration; mu += alpha p, r -= alpha q, E += alpha
-// w, z = pol^T r; part[0] = r.z, part[1] the rmsd term of (mu, E)
+// w, z = pol^T r; part[0] = r.z, part[1] as kb_cg_start's
 __global__ void kb_cg_update(int n, const int *__restrict__ cfg, const int *__restrict__ act,
                              const int *__restrict__ type, const TypeTab *__restrict__ T,
                              const double *__restrict__ rz, const double *__restrict__ pqs,
                              const double *__restrict__ p, const double *__restrict__ q,
                              const double *__restrict__ w, double *__restrict__ mu, double *__restrict__ r,
-                             double *__restrict__ ef, double *__restrict__ z, double *__restrict__ part)
+                             double *__restrict__ ef, double *__restrict__ z, double *__restrict__ part, int adj)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -982,7 +983,7 @@ __global__ void kb_cg_update(int n, const int *__restrict__ cfg, const int *__re
         if (T->polar[t]) {
             cg_polT(T->pol[t], rv, zv);
             rzn = rv[0] * zv[0] + rv[1] * zv[1] + rv[2] * zv[2];
-            rm = cg_rmsd_ion(T->pol[t], m, e);
+            rm = adj ? rv[0] * rv[0] + rv[1] * rv[1] + rv[2] * rv[2] : cg_rmsd_ion(T->pol[t], m, e);
         }
         for (int c = 0; c < 3; c++) z[(size_t)c * n + i] = zv[c];
     }
@@ -991,11 +992,13 @@ __global__ void kb_cg_update(int n, const int *__restrict__ cfg, const int *__re
 }
 
 // the loop condition of pim.c:236 per configuration on its rmsd (sums
-// s[c][0] = r.z, s[c][1] = rmsd^2 npol), and beta = r.z / r.z_old; one block
+// s[c][0] = r.z, s[c][1] = rmsd^2 npol), or with s2 (the adjoint solve)
+// |r| <= tol |rhs| (s[c][1] = |r|^2, s2[c] = |rhs|^2); and beta = r.z /
+// r.z_old; one block
 __global__ void __launch_bounds__(1024) kb_cg_control(int nconf, const BConfD *__restrict__ conf,
                                                       const double *__restrict__ s, int first, int *act, int *iscf,
                                                       double *rmsd, double *rz, double *beta, int maxit, int minit,
-                                                      double tol, int *any)
+                                                      double tol, int *any, const double *__restrict__ s2)
 {
     __shared__ int sany;
     if (threadIdx.x == 0) sany = 0;
@@ -1004,7 +1007,8 @@ __global__ void __launch_bounds__(1024) kb_cg_control(int nconf, const BConfD *_
         if (!act[c]) continue;
         const int it = iscf[c] + 1;
         const int np = conf[c].npol;
-        const double rm = sqrt(s[2 * c + 1] / (double)(np > 0 ? np : 1));
+        const double rm = s2 ? (s2[c] > 0.0 ? sqrt(s[2 * c + 1] / s2[c]) : 0.0)
+                             : sqrt(s[2 * c + 1] / (double)(np > 0 ? np : 1));
         const double rzn = s[2 * c];
         iscf[c] = it;
         rmsd[c] = rm;
@@ -1016,6 +1020,13 @@ __global__ void __launch_bounds__(1024) kb_cg_control(int nconf, const BConfD *_
     }
     __syncthreads();
     if (threadIdx.x == 0) *any = sany;
+}
+
+// e = a + b (3n), the adjoint solve's carried "field" s + T lam
+__global__ void kb_add3(size_t m, const double *__restrict__ a, const double *__restrict__ b, double *__restrict__ e)
+{
+    const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (i < m) e[i] = a[i] + b[i];
 }
 
 // p = z + beta p on the active configurations
