@@ -28,7 +28,7 @@ sys.path.insert(0, ROOT)
 
 M_O, M_SI = 15.99994, 28.084
 RHO_UNIT = 1.660538782
-PEAK = {"f64": 78.6e12, "f32": 157.3e12, "bf16": 2.5e15, "hbm": 8.0e12}
+PEAK = {"f64": 78.6e12, "f32": 157.3e12, "f16": 2.5e15, "hbm": 8.0e12}
 # BHMFT parameters of tests/golden (make_fixtures.pair_params, seed 20349):
 # upper triangles O-O, O-Si, Si-Si
 PAIR_A = (13843.775422359393, 9907.395718764405, 13891.629698472487)
@@ -313,10 +313,10 @@ def cpu_baseline(p_text, L, pos, typ, n_scf, threads):
 
 # the S(k) GEMM kernels behind mdff_time_kernel's (which, lo): kernel-name
 # prefix (the template's first argument is the arithmetic mode: 0 FP64,
-# 1 split-BF16, 2 FP32, 3 split-FP16)
+# 2 FP32, 3 split-FP16)
 TIMED = {"sk_forward_scf": (0, "k_gemm_fwd"), "sk_back_scf": (1, "k_gemm_back"),
          "sk_forward_force": (3, "k_gemm_fwd"), "sk_back_force": (4, "k_gemm_back")}
-VARIANT = {"f64": (0, 0), "f32": (1, 2), "bf16x3": (2, 1), "f16x3": (3, 3)}   # name -> (lo, template mode)
+VARIANT = {"f64": (0, 0), "f32": (1, 2), "f16x3": (3, 3)}   # name -> (lo, template mode)
 # arithmetic of the mixed form's matrix-core sums: static field and force
 # call (eval), SCF steps (inner) and the anchor evaluation of the SCF
 # (booked with the inner evaluations when it is not FP64)
@@ -464,7 +464,7 @@ def main():
     ap.add_argument("--forms", default="fp64,fp64_cg,mixed,mixed_refine",
                     help="fp64 (the reference's Jacobi SCF), fp64_cg (FP64 with the anchored CG SCF), mixed, "
                          "mixed_refine (mixed with the FP64 refinement pass of the SCF)")
-    ap.add_argument("--pmc-kernel", default=None, help="e.g. sk_forward_scf/bf16x3 (counter runs)")
+    ap.add_argument("--pmc-kernel", default=None, help="e.g. sk_forward_scf/f16x3 (counter runs)")
     ap.add_argument("--pmc-iters", type=int, default=5)
     ap.add_argument("--pmc-summarize", nargs=2, default=None, metavar=("FETCH_DIR", "WRITE_DIR"))
     ap.add_argument("--pmc-table", nargs="+", default=None, metavar="DIR",
@@ -569,8 +569,8 @@ def main():
     a_, k_, nk = e.ewald()
     npol = e.counters()["npol"]
     st = res[head]
-    variants = [(0, "f64", PEAK["f64"])] + ([(1, "f32", PEAK["f32"]), (2, "bf16x3", PEAK["bf16"]),
-                                            (3, "f16x3", PEAK["bf16"])] if head == "mixed" else [])
+    variants = [(0, "f64", PEAK["f64"])] + ([(1, "f32", PEAK["f32"]), (3, "f16x3", PEAK["f16"])]
+                                           if head == "mixed" else [])
     kern = {}
     for lo, vname, peak in ([] if args.no_kernels else variants):
         for which, name, ch in [(0, "sk_forward_scf", 3), (1, "sk_back_scf", 3), (2, "dir_induced", 0),

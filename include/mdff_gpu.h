@@ -151,14 +151,13 @@ void mdff_tt_damping(double b, double c, double r, double *f, double *fd, int or
  * reference. */
 int mdff_set_tuning(mdff_ctx *ctx, int lanes, int katoms, int ksplit);
 
-/* Arithmetic of the mixed form's matrix-core k-space sums (stored-operand
- * path): MDFF_KS_F32 (FP32 operands), MDFF_KS_BF16X3 (split BF16, three
- * products, about 16 bits), MDFF_KS_F16X3 (split FP16 with power-of-two
- * row scaling, three products, about 22 bits).  eval_mode applies to the
+/* Arithmetic of the mixed form's matrix-core k-space sums: MDFF_KS_F32
+ * (FP32 operands) or MDFF_KS_F16X3 (split FP16 with power-of-two row
+ * scaling, three products, about 22 bits).  eval_mode applies to the
  * static field and the force call, inner_mode to the low-precision SCF
- * steps; 0 keeps a setting.  Defaults: MDFF_KS_F16X3 for both.  No
- * counterpart in the reference. */
-enum { MDFF_KS_BF16X3 = 1, MDFF_KS_F32 = 2, MDFF_KS_F16X3 = 3 };
+ * steps; 0 keeps a setting, other values are rejected.  Defaults:
+ * MDFF_KS_F16X3 for both.  No counterpart in the reference. */
+enum { MDFF_KS_F32 = 2, MDFF_KS_F16X3 = 3 };
 int mdff_set_kspace_modes(mdff_ctx *ctx, int eval_mode, int inner_mode);
 
 /* Anchored-CG SCF (solver 2): the anchor, the one field evaluation at the
@@ -220,8 +219,8 @@ int mdff_get_timing(mdff_ctx *ctx, double t[6]);
  * matrix-core S(k) GEMM kernels (which 0, 1, 3, 4 on the matrix-core path)
  * are timed alone, each launch between its own pair of HIP events (operand
  * builds and the partial-sum pass excluded); the others between two events
- * around all launches.  lo = 0 the FP64 variant, 1 FP32, 2 split-BF16, 3
- * split-FP16 (1-3 in mixed contexts only).  which:
+ * around all launches.  lo = 0 the FP64 variant, 1 FP32, 3 split-FP16
+ * (1 and 3 in mixed contexts only).  which:
  *   0 S(k) forward, SCF form (dipoles of the polarisable ions, 3 channels)
  *   1 back-projection, SCF form (field at the polarisable ions)
  *   2 real-space induced dipole field
@@ -235,39 +234,28 @@ int mdff_get_timing(mdff_ctx *ctx, double t[6]);
 int mdff_time_kernel(mdff_ctx *ctx, int which, int lo, int iters, double *ms);
 
 /* Counters since creation: [0] force calls, [1] FP64 SCF field
- * evaluations, [2] FP32 correction sweeps (mixed form), [3] polarisable
+ * evaluations, [2] low-precision SCF field evaluations (mixed form: the
+ * CG steps, and the anchor when it is low precision), [3] polarisable
  * ions.  No counterpart in the reference (measurement support). */
 int mdff_get_counters(mdff_ctx *ctx, long long out[4]);
 
-/* Mixed form: low-precision iterations of the SCF correction per FP64
- * field evaluation (default 8). */
-int mdff_set_scf_inner(mdff_ctx *ctx, int inner);
-
-/* Mixed form: the SCF solver.
- *   0  defect correction: FP64 field and rmsd of the current dipoles, then
- *      `inner` low-precision Jacobi sweeps (the reference's update,
- *      pim.c:262-268) on the correction equation
- *      (pol^-T - T) d = E - pol^-T mu; repeated until the FP64 rmsd passes
- *   1  the same with `inner` steps of conjugate gradients preconditioned by
- *      the polarisability
- *   2  anchored conjugate gradients (default): one FP64 field evaluation of
- *      the ASPC predictor, then preconditioned CG steps that apply the
- *      low-precision T to the search direction only and carry the field
- *      along; the rmsd of get_rmsd_mu (pim.c:431-449) is formed from the
- *      updated dipoles and field after every step, one SCF iteration per
- *      step; `inner` is the number of steps launched before the first
- *      convergence poll of a context (later calls use the last count).
- *      Preconditioner pol^T
- *   3  the same with the preconditioner pol^T + pol^T T_dir pol^T (T_dir:
- *      the real-space dipole tensors and the self term, FP32); on the
- *      12288-ion melt it needs as many steps as solver 2 (the slow modes are
- *      long-wavelength, set by the k-space part) and costs a field pass more
- * Solvers 1 and 2 need every polarisability symmetric positive definite;
- * otherwise solver 0 runs.  In the FP64 form, solver 0 (the default) is the
- * reference's Jacobi schedule and solver 2 the anchored CG with FP64 anchor
- * and steps (same fixed point, fewer iterations, counts no longer the
- * reference's); other values are rejected.  No counterpart in the
- * reference. */
+/* The SCF solver of the PIM dipoles (algo_moment_from_pola scf).
+ *   0  the reference's Jacobi schedule (momentpolaSCF pim.c:177-306) with
+ *      FP64 fields: the FP64 form's default
+ *   2  anchored conjugate gradients, the mixed form's default: one field
+ *      evaluation of the ASPC predictor (the anchor; FP64, or a matrix-core
+ *      mode set by mdff_set_scf_anchor), then CG steps on
+ *      (pol^-T - T) mu = E_stat preconditioned by pol^T that apply T (low
+ *      precision in the mixed form, FP64 in the FP64 form) to the search
+ *      direction only and carry the field along; the rmsd of get_rmsd_mu
+ *      (pim.c:431-449) is formed from the updated dipoles and field after
+ *      every step, one SCF iteration per step (same fixed point as 0, fewer
+ *      iterations, counts no longer the reference's)
+ * Solver 2 needs every polarisability symmetric positive definite;
+ * otherwise solver 0 runs.  `inner` (0-64) is the number of CG steps
+ * launched before a context's first convergence poll (later SCFs launch as
+ * many as the last one took).  Other solver values are rejected.  No
+ * counterpart in the reference. */
 int mdff_set_scf_solver(mdff_ctx *ctx, int solver, int inner);
 
 #ifdef __cplusplus

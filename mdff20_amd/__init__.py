@@ -10,7 +10,7 @@ import os
 
 import numpy as np
 
-from ._types import (ES, ERR_SCF, FP64, KS_BF16X3, KS_F16X3, KS_F32, MIXED, NVE_LF, NVE_VV, NVT_NHCN, PIM_PCG, PIM_SCF,
+from ._types import (ES, ERR_SCF, FP64, KS_F16X3, KS_F32, MIXED, NVE_LF, NVE_VV, NVT_NHCN, PIM_PCG, PIM_SCF,
                      PIM_SCFKO, STATUS, Config, Params, Thermo)
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
@@ -18,7 +18,7 @@ LIB_PATH = os.path.join(_HERE, "libmdff_gpu.so")
 _lib = None
 
 __all__ = ["lib", "MdffError", "read_control", "read_posff", "Engine", "Params", "Thermo",
-           "FP64", "MIXED", "KS_BF16X3", "KS_F32", "KS_F16X3", "ES", "NVE_LF", "NVE_VV", "NVT_NHCN", "PIM_SCF", "PIM_SCFKO", "PIM_PCG",
+           "FP64", "MIXED", "KS_F32", "KS_F16X3", "ES", "NVE_LF", "NVE_VV", "NVT_NHCN", "PIM_SCF", "PIM_SCFKO", "PIM_PCG",
            "device_count", "tt_damping", "init_velocities_host", "Rdf", "rdf_defaults"]
 
 
@@ -85,7 +85,6 @@ def lib():
         "mdff_get_timing": (C.c_int, [vp, dp]),
         "mdff_time_kernel": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, dp]),
         "mdff_get_counters": (C.c_int, [vp, P(C.c_longlong)]),
-        "mdff_set_scf_inner": (C.c_int, [vp, C.c_int]),
         "mdff_set_scf_solver": (C.c_int, [vp, C.c_int, C.c_int]),
         "mdff_set_scf_check": (C.c_int, [vp, C.c_int]),
         "mdff_set_scf_refine": (C.c_int, [vp, C.c_int]),
@@ -310,13 +309,10 @@ class Engine:
         _check(self._L.mdff_get_counters(self._h, c), "get_counters", False)
         return dict(force_calls=c[0], scf_fp64=c[1], scf_inner=c[2], npol=c[3])
 
-    def set_scf_inner(self, k):
-        _check(self._L.mdff_set_scf_inner(self._h, k), "set_scf_inner", False)
-
     def set_scf_solver(self, solver, inner):
-        """Mixed form: 0 / 1 Jacobi / CG defect correction (inner steps per FP64 evaluation),
-        2 anchored CG (one FP64 evaluation per SCF; the default), 3 the same with the
-        real-space dipole tensors in the preconditioner."""
+        """0 the reference's Jacobi schedule with FP64 fields (the FP64 form's default), 2 anchored
+        CG (the mixed form's default; symmetric polarisabilities); inner = CG steps launched before
+        the first convergence poll"""
         _check(self._L.mdff_set_scf_solver(self._h, solver, inner), "set_scf_solver", False)
 
     def set_scf_check(self, every):
@@ -363,13 +359,13 @@ class Engine:
         _check(self._L.mdff_set_tuning(self._h, lanes, katoms, ksplit), "set_tuning", False)
 
     def set_kspace_modes(self, eval_mode=0, inner_mode=0):
-        """mixed form's matrix-core arithmetic: KS_F32, KS_BF16X3 or KS_F16X3 for the static field and
+        """mixed form's matrix-core arithmetic: KS_F32 or KS_F16X3 for the static field and
         force call (eval_mode) and the low-precision SCF steps (inner_mode); 0 keeps"""
         _check(self._L.mdff_set_kspace_modes(self._h, eval_mode, inner_mode), "set_kspace_modes", False)
 
     def set_scf_anchor(self, anchor_mode=0):
         """anchored CG (SCF solver 2): the anchor field evaluation in FP64 (0) or in a matrix-core mode
-        (KS_F32, KS_BF16X3, KS_F16X3) over the FP32 tables"""
+        (KS_F32, KS_F16X3) over the FP32 tables"""
         _check(self._L.mdff_set_scf_anchor(self._h, anchor_mode), "set_scf_anchor", False)
 
     def set_kspace_shape(self, cw64=0, cw32=0, fwd_blocks=0, back_blocks=0):

@@ -15,7 +15,7 @@ TRUNC_NO, TRUNC_LINEAR, TRUNC_QUADRATIC = 0, 1, 2
 PIM_SCF, PIM_SCFKO, PIM_PCG = 0, 1, 2
 FP64, MIXED = 0, 1
 # matrix-core arithmetic of the mixed form's k-space sums (mdff_set_kspace_modes)
-KS_BF16X3, KS_F32, KS_F16X3 = 1, 2, 3
+KS_F32, KS_F16X3 = 2, 3
 STATUS = {0: "ok", 1: "bad argument", 2: "invalid configuration", 3: "I/O error",
           4: "HIP runtime error", 5: "neighbour list overflow", 6: "non-finite result",
           7: "SCF unconverged", 8: "unsupported", 9: "out of memory",

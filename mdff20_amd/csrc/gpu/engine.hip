@@ -89,10 +89,8 @@ struct mdff_ctx {
     double *ke = nullptr;      // n
     double *upol_i = nullptr, *rmsd_i = nullptr;  // n
     double *tmp_q = nullptr, *tmp_mu = nullptr, *tmp_qraw = nullptr, *tmp_mur = nullptr;  // drop-in scratch
-    double *dc_R = nullptr, *dc_d = nullptr, *dc_g = nullptr;  // mixed SCF correction
-    double *cg_z = nullptr, *cg_p = nullptr, *cg_part = nullptr; // its conjugate-gradient form
-    double *cg_w = nullptr;   // pol^T r of the preconditioned anchored CG
-    int cg_precond = 0;       // anchored CG: 0 preconditioner pol^T, 1 pol^T + pol^T T_dir pol^T
+    double *dc_R = nullptr, *dc_d = nullptr, *dc_g = nullptr;  // anchored CG: residual, step, T p
+    double *cg_z = nullptr, *cg_p = nullptr, *cg_part = nullptr; // preconditioned residual, direction, sums
     double *ko_f = nullptr, *ko_dmu = nullptr; // scfkO: residual field and inner-loop dipoles
     double *t2ko = nullptr;   // scfkO: dipole tensors of the polarisable pairs at alpha 0.001
     double *scal = nullptr;
@@ -132,7 +130,7 @@ struct mdff_ctx {
     // tuning
     int lanes = 16;           // lanes per ion of the pair kernels and the pair-tensor field
     int use_mfma = 1;         // S(k) forward and back-projection on the matrix cores
-    int g_cw[2] = {1, 2};     // column tiles per wave of the forward GEMM (FP64, FP32/BF16)
+    int g_cw[2] = {1, 2};     // column tiles per wave of the forward GEMM (FP64, FP32/split-FP16)
     int g_fblocks = 0;        // target blocks of the forward GEMM (0: one round of resident blocks)
     int g_bblocks = 0;        // target blocks of the back-projection GEMM (0: one round)
     int ncu = 256;            // compute units of the device
@@ -182,7 +180,7 @@ struct mdff_ctx {
     int have_events = 0;
     double t_ms[6] = {0, 0, 0, 0, 0, 0};
     int last_nscf = 0;
-    int hint_fp64 = 0, hint_mixed = 0, hint_cg = 0, hint_ko = 0; // iterations of the last SCF of each form
+    int hint_fp64 = 0, hint_cg = 0, hint_ko = 0; // iterations of the last SCF of each solver
     int hint_ref = 0;         // correction steps of the last FP64 refinement
     double last_rmsd = 0;
     int scf_check = 0;        // mixed form: FP64 residual of the SCF every scf_check-th force call (0 never)
@@ -194,8 +192,8 @@ struct mdff_ctx {
     std::vector<int> traj_step;
     int chk_done = 0;         // the last force call evaluated it (S_RMSD_CHK holds the sum)
     int scf_chunk = 4;
-    int inner_iters = 8;      // mixed form: low-precision correction iterations per FP64 refinement
-    int scf_solver = 2;       // mixed form: 0/1 Jacobi/CG defect correction, 2/3 anchored CG
+    int cg_first = 8;         // anchored CG: steps launched before a context's first convergence poll
+    int scf_solver = 2;       // mixed form: 0 the reference's Jacobi schedule (FP64 fields), 2 anchored CG
     int fp64_cg = 0;          // FP64 form: anchored CG with FP64 steps instead of the reference's Jacobi schedule
     int cg_ok = 0;            // every polarisability symmetric positive definite (CG applies)
     long long n_force = 0, n_scf_fp64 = 0, n_scf_inner = 0;
@@ -271,7 +269,7 @@ struct OnAux {
 static void build_tables(mdff_ctx *c)
 {
     typetab_build(c->p, c->ntype, c->srcutsq, c->tab);
-    // the CG correction needs pol^-T - T symmetric positive definite: each
+    // the anchored CG needs pol^-T - T symmetric positive definite: each
     // polarisability symmetric with positive leading minors
     c->cg_ok = 1;
     for (int t = 0; t < c->ntype; t++) {
@@ -510,7 +508,7 @@ static void launch_eikr(mdff_ctx *c, int polar, int lo, const int *active, bool 
     with_polar = with_polar && !polar && c->npol > 0;
     // stored operands built from these tables are stale now
     for (int ps = polar; ps <= (with_polar ? 1 : polar); ps++) {
-        if (lo) { c->yz_ok[KM_F32][ps] = 0; c->yz_ok[KM_BF16X3][ps] = 0; c->yz_ok[KM_F16X3][ps] = 0; }
+        if (lo) { c->yz_ok[KM_F32][ps] = 0; c->yz_ok[KM_F16X3][ps] = 0; }
         else c->yz_ok[KM_F64][ps] = 0;
     }
     void **t = c->tab3[lo][polar];
@@ -576,7 +574,7 @@ static int launch_back_mfma(mdff_ctx *c, int polar, int lo, int NCH, const int *
 
 // ------------------------------------------------ stored-operand matrix-core sums
 static size_t g_elt(int mode) { return mode == KM_F64 ? 8 : (mode == KM_F32 ? 4 : 2); }
-static bool g_split_mode(int mode) { return mode == KM_BF16X3 || mode == KM_F16X3; }
+static bool g_split_mode(int mode) { return mode == KM_F16X3; }
 
 // allocation made while the context works: zeroed in stream order, since a
 // null-stream memset is not ordered with the non-blocking context stream
@@ -616,8 +614,7 @@ static int ensure_yz(mdff_ctx *c, int mode, int polar, int lo)
                        (const float *)t[1], (const float *)t[2], y0, y1, z0, z1)
     if (mode == KM_F64) YZL(KM_F64, double);
     else if (mode == KM_F32) YZ16(KM_F32);
-    else if (mode == KM_F16X3) YZ16(KM_F16X3);
-    else YZ16(KM_BF16X3);
+    else YZ16(KM_F16X3);
 #undef YZL
 #undef YZ16
     c->yz_ok[mode][polar] = 1;
@@ -647,8 +644,7 @@ static const void *gfwd_kernel(int mode, int CW, int MT)
 {
     if (mode == KM_F64) return gfwd_kernel_m<KM_F64>(CW, MT);
     if (mode == KM_F32) return gfwd_kernel_m<KM_F32>(CW, MT);
-    if (mode == KM_F16X3) return gfwd_kernel_m<KM_F16X3>(CW, MT);
-    return gfwd_kernel_m<KM_BF16X3>(CW, MT);
+    return gfwd_kernel_m<KM_F16X3>(CW, MT);
 }
 
 template <int MODE, typename TT>
@@ -666,8 +662,7 @@ static const void *gback_kernel(int mode, int CT)
 {
     if (mode == KM_F64) return gback_kernel_t<KM_F64, double>(CT);
     if (mode == KM_F32) return gback_kernel_t<KM_F32, float>(CT);
-    if (mode == KM_F16X3) return gback_kernel_t<KM_F16X3, float>(CT);
-    return gback_kernel_t<KM_BF16X3, float>(CT);
+    return gback_kernel_t<KM_F16X3, float>(CT);
 }
 
 // launch the instantiation for the shape's tile count (GEN: B formed from
@@ -723,7 +718,7 @@ static bool g_back_attrs(mdff_ctx *)
     // the stored-operand path takes, (Kx+1) 6 2 <= 16 MB_CT
     const int KX1 = 16 * MB_CT / 12;
     return g_back_attrs_t<KM_F64, double>(KX1) && g_back_attrs_t<KM_F32, float>(KX1) &&
-           g_back_attrs_t<KM_BF16X3, float>(KX1) && g_back_attrs_t<KM_F16X3, float>(KX1);
+           g_back_attrs_t<KM_F16X3, float>(KX1);
 }
 
 // Block target of one stored-operand sum (mode, direction, tile count,
@@ -834,12 +829,8 @@ static int launch_fwd_gemm(mdff_ctx *c, int polar, int lo, int C, const double *
         hipLaunchKernelGGL(k_aw_build16<KM_F32>, dim3(cdiv(nselp, 256 * 2), c->Kx + 1, C), dim3(256), 0, c->st, nsel,
                            nselp, C, (const float *)t[0], idx, q_w, mu_w, c->n, c->aw[mode][0], c->aw[mode][1], wsc,
                            nwb, c->gsc + 8, active);
-    else if (mode == KM_F16X3)
-        hipLaunchKernelGGL(k_aw_build16<KM_F16X3>, dim3(cdiv(nselp, 256 * 4), c->Kx + 1, C), dim3(256), 0, c->st, nsel,
-                           nselp, C, (const float *)t[0], idx, q_w, mu_w, c->n, c->aw[mode][0], c->aw[mode][1], wsc,
-                           nwb, c->gsc + 8, active);
     else
-        hipLaunchKernelGGL(k_aw_build16<KM_BF16X3>, dim3(cdiv(nselp, 256 * 4), c->Kx + 1, C), dim3(256), 0, c->st, nsel,
+        hipLaunchKernelGGL(k_aw_build16<KM_F16X3>, dim3(cdiv(nselp, 256 * 4), c->Kx + 1, C), dim3(256), 0, c->st, nsel,
                            nselp, C, (const float *)t[0], idx, q_w, mu_w, c->n, c->aw[mode][0], c->aw[mode][1], wsc,
                            nwb, c->gsc + 8, active);
     GFwdArgs a{};
@@ -881,8 +872,7 @@ static int launch_fwd_gemm(mdff_ctx *c, int polar, int lo, int C, const double *
         if (probe) probe_mark(c, 0);
         if (mode == KM_F64) gfwd_cw<KM_F64>(c, grid, CW, MT, ga2);
         else if (mode == KM_F32) { if (gen) gfwd_cw<KM_F32, true>(c, grid, CW, MT, ga2); else gfwd_cw<KM_F32>(c, grid, CW, MT, ga2); }
-        else if (mode == KM_F16X3) { if (gen) gfwd_cw<KM_F16X3, true>(c, grid, CW, MT, ga2); else gfwd_cw<KM_F16X3>(c, grid, CW, MT, ga2); }
-        else { if (gen) gfwd_cw<KM_BF16X3, true>(c, grid, CW, MT, ga2); else gfwd_cw<KM_BF16X3>(c, grid, CW, MT, ga2); }
+        else { if (gen) gfwd_cw<KM_F16X3, true>(c, grid, CW, MT, ga2); else gfwd_cw<KM_F16X3>(c, grid, CW, MT, ga2); }
         if (probe) probe_mark(c, 1);
         if (!psum) return nch;
         if (defer) {
@@ -950,11 +940,8 @@ static int launch_back_gemm(mdff_ctx *c, int polar, int lo, int NCH, const int *
     else if (mode == KM_F64)
         hipLaunchKernelGGL(k_wm_build<KM_F64>, gw, dim3(256), 0, c->st, c->P, c->Pk, c->Ky, c->Kz, CH, c->W,
                            c->wm[mode][0], c->wm[mode][1], active);
-    else if (mode == KM_F32)
-        hipLaunchKernelGGL(k_wm_build<KM_F32>, gw, dim3(256), 0, c->st, c->P, c->Pk, c->Ky, c->Kz, CH, c->W,
-                           c->wm[mode][0], c->wm[mode][1], active);
     else
-        hipLaunchKernelGGL(k_wm_build<KM_BF16X3>, gw, dim3(256), 0, c->st, c->P, c->Pk, c->Ky, c->Kz, CH, c->W,
+        hipLaunchKernelGGL(k_wm_build<KM_F32>, gw, dim3(256), 0, c->st, c->P, c->Pk, c->Ky, c->Kz, CH, c->W,
                            c->wm[mode][0], c->wm[mode][1], active);
     GBackArgs b{};
     b.yzt = c->yzt[mode][polar][0]; b.yzt_lo = c->yzt[mode][polar][1];
@@ -988,8 +975,7 @@ static int launch_back_gemm(mdff_ctx *c, int polar, int lo, int NCH, const int *
         if (probe) probe_mark(c, 0);
         if (mode == KM_F64) gback_ct<KM_F64, double>(c, grid, CT, gb2);
         else if (mode == KM_F32) gback_ct<KM_F32, float>(c, grid, CT, gb2);
-        else if (mode == KM_F16X3) gback_ct<KM_F16X3, float>(c, grid, CT, gb2);
-        else gback_ct<KM_BF16X3, float>(c, grid, CT, gb2);
+        else gback_ct<KM_F16X3, float>(c, grid, CT, gb2);
         if (probe) probe_mark(c, 1);
         return nsp;
     };
@@ -1090,8 +1076,7 @@ static int rec_part(mdff_ctx *c, int polar, int lo, int C, int NCH, const double
         void *w0 = c->wm[bmode][0], *w1 = c->wm[bmode][1];
         const dim3 gr(cdiv(c->nkf, 256));
         if (bmode == KM_F64) hipLaunchKernelGGL(k_sk_reduce_wm<KM_F64>, gr, dim3(256), 0, c->st, r, CHb, c->Pk, w0, w1);
-        else if (bmode == KM_F32) hipLaunchKernelGGL(k_sk_reduce_wm<KM_F32>, gr, dim3(256), 0, c->st, r, CHb, c->Pk, w0, w1);
-        else hipLaunchKernelGGL(k_sk_reduce_wm<KM_BF16X3>, gr, dim3(256), 0, c->st, r, CHb, c->Pk, w0, w1);
+        else hipLaunchKernelGGL(k_sk_reduce_wm<KM_F32>, gr, dim3(256), 0, c->st, r, CHb, c->Pk, w0, w1);
         wm_built = true;
     } else {
         hipLaunchKernelGGL(k_sk_reduce, dim3(cdiv(c->nkf, 256)), dim3(256), 0, c->st, r);
@@ -1235,12 +1220,15 @@ static int aspc_args(mdff_ctx *c, int istep, AspcArgs *asp)
 // dipole tensors of all polarisable pairs at the current positions (per
 // force call, after k_frac), and for scfkO the same at KO_ALPHA; rows hold
 // the list capacity, so they never overflow
-// whether the SCF evaluates the induced field in FP64: the FP64 form, scfkO
-// (FP64 in both forms), and the mixed form's solvers but anchored CG with a
-// low-precision anchor
+// whether the mixed form's SCF runs the anchored CG (solver 2 with
+// symmetric positive definite polarisabilities), and whether the SCF
+// evaluates the induced field in FP64: the FP64 form, scfkO (FP64 in both
+// forms), and the mixed form but anchored CG with a low-precision anchor
+static bool scf_anchored_ok(const mdff_ctx *c) { return c->scf_solver == 2 && c->cg_ok && c->npol > 0; }
+
 static bool scf_uses_fp64(const mdff_ctx *c)
 {
-    return !lo_of(c) || c->p.algo_pim == MDFF_PIM_SCFKO || !(c->scf_solver >= 2 && c->cg_ok && c->mf_anchor);
+    return !lo_of(c) || c->p.algo_pim == MDFF_PIM_SCFKO || !(scf_anchored_ok(c) && c->mf_anchor);
 }
 
 // T2 pair tensors of the polarisable ions; the FP64 copy only when the SCF
@@ -1367,108 +1355,6 @@ static int scf_loop(mdff_ctx *c, int istep)
     return MDFF_OK;
 }
 
-// Mixed form of the SCF: defect correction.  Each outer iteration evaluates
-// the field E = E_stat + T mu of the current dipoles in FP64 and the
-// residual rmsd exactly as get_rmsd_mu (pim.c:431-449); while it is above
-// conv_tol_ind, the correction equation (pol^-T - T) d = E - pol^-T mu is
-// solved approximately with the low-precision operator and d is added to
-// mu.  The fixed point is the FP64 one; the low precision only limits how
-// much each outer step gains.  The start is the ASPC predictor
-// (pim.c:382-400).  The correction is inner_iters steps of conjugate
-// gradients preconditioned by pol^T (scf_solver 1, when every
-// polarisability is symmetric positive definite), else inner_iters Jacobi
-// sweeps d = R + pol^T (T d) with R = pol^T E - mu.
-static void scf_correct_jacobi(mdff_ctx *c, const int *act)
-{
-    int n = c->n;
-    hipLaunchKernelGGL(k_dc_residual, dim3(cdiv(n, 256)), dim3(256), 0, c->st, n, c->type, c->tab_d, c->ef, c->mu_ind,
-                       c->dc_R, c->dc_d, act);
-    for (int k = 0; k < c->inner_iters; k++) {
-        induced_field(c, 1, c->dc_d, nullptr, c->dc_g, act);
-        hipLaunchKernelGGL(k_dc_update, dim3(cdiv(n, 256)), dim3(256), 0, c->st, n, c->type, c->tab_d, c->dc_R,
-                           c->dc_g, c->dc_d, act);
-    }
-    hipLaunchKernelGGL(k_axpy_gated, dim3(cdiv(3 * n, 256)), dim3(256), 0, c->st, 3 * n, c->dc_d, c->mu_ind, act);
-}
-
-static void scf_correct_cg(mdff_ctx *c, const int *act)
-{
-    int n = c->n, np = c->npol;
-    int nb = cdiv(np, CG_BLOCK);
-    double *rz[2] = {c->cg_part, c->cg_part + nb}, *pq = c->cg_part + 2 * nb;
-    hipLaunchKernelGGL(k_cg_init, dim3(nb), dim3(CG_BLOCK), 0, c->st, n, np, c->pidx, c->type, c->tab_d, c->ef,
-                       c->mu_ind, c->dc_R, c->cg_z, c->cg_p, c->dc_d, rz[0], act);
-    int cur = 0;
-    for (int k = 0; k < c->inner_iters; k++) {
-        induced_field(c, 1, c->cg_p, nullptr, c->dc_g, act);
-        hipLaunchKernelGGL(k_cg_pq, dim3(nb), dim3(CG_BLOCK), 0, c->st, n, np, c->pidx, c->type, c->tab_d, c->cg_p,
-                           c->dc_g, pq, act);
-        hipLaunchKernelGGL(k_cg_update, dim3(nb), dim3(CG_BLOCK), 0, c->st, n, np, nb, c->pidx, c->type, c->tab_d,
-                           c->cg_p, c->dc_g, c->dc_d, c->dc_R, c->cg_z, rz[cur], pq, rz[cur ^ 1], act);
-        if (k + 1 < c->inner_iters)
-            hipLaunchKernelGGL(k_cg_next, dim3(nb), dim3(CG_BLOCK), 0, c->st, n, np, nb, c->pidx, c->cg_z, c->cg_p,
-                               rz[cur], rz[cur ^ 1], act);
-        cur ^= 1;
-    }
-    hipLaunchKernelGGL(k_cg_apply, dim3(nb), dim3(CG_BLOCK), 0, c->st, n, np, c->pidx, c->dc_d, c->mu_ind, act);
-}
-
-static int scf_loop_mixed(mdff_ctx *c, int istep)
-{
-    int n = c->n;
-    const int *act = c->ctl + C_SCF_ACTIVE;
-    int maxit = c->p.max_scf_pim_iter, minit = c->p.min_scf_pim_iter;
-    hipLaunchKernelGGL(k_scf_begin, dim3(1), dim3(1), 0, c->st, c->ctl, (maxit > 0 || minit > 0) ? 1 : 0);
-    AspcArgs asp;
-    if (aspc_args(c, istep, &asp)) return MDFF_ERR_CONFIG;
-    hipLaunchKernelGGL(k_scf_mu, dim3(cdiv(n, 256)), dim3(256), 0, c->st, n, 1, asp, c->type, c->tab_d, c->hist, c->ef,
-                       c->mu_ind, c->upol_i, act);
-    bool cg = c->scf_solver >= 1 && c->cg_ok && c->npol > 0;
-    // FP64 evaluation of the field and residual check, and the correction
-    // between two of them; the host polls the flag after as many
-    // evaluations as the last SCF took (at least two: one correction
-    // usually converges), then after every further one, so a converged SCF
-    // leaves no skipped launches behind
-    int launched = 0;
-    int limit = std::max(maxit, minit) + 1;
-    auto eval = [&]() {
-        induced_field(c, 0, c->mu_ind, c->ef_stat, c->ef, act);
-        scf_check(c, nullptr, maxit, minit);
-        launched++;
-    };
-    auto correct = [&]() {
-        // the correction only steers the FP64-residual iteration, so it may
-        // run at split-BF16 accuracy
-        int keep = c->mf_lo;
-        c->mf_lo = c->mf_inner;
-        if (cg) scf_correct_cg(c, act);
-        else scf_correct_jacobi(c, act);
-        c->mf_lo = keep;
-    };
-    eval();
-    int first = std::min(limit, std::max(std::max(2, minit), c->hint_mixed));
-    while (launched < first) {
-        correct();
-        eval();
-    }
-    for (;;) {
-        int rc = sync_scal(c);
-        if (rc) return rc;
-        if (!c->ctl_h[C_SCF_ACTIVE] || launched >= limit) break;
-        correct();
-        eval();
-    }
-    // u_pol of the dipoles whose field was evaluated last (pim.c:253,299)
-    hipLaunchKernelGGL(k_upol, dim3(cdiv(n, 256)), dim3(256), 0, c->st, n, c->type, c->tab_d, c->mu_ind, c->upol_i);
-    colsum(c, c->upol_i, n, n, 1, c->scal + S_UPOL);
-    scf_finish(c);
-    c->n_scf_fp64 += c->last_nscf;
-    c->hint_mixed = c->last_nscf;
-    c->n_scf_inner += (long long)c->inner_iters * std::max(0, c->last_nscf - 1);
-    if (c->last_rmsd > c->p.conv_tol_ind) return MDFF_ERR_SCF;
-    return MDFF_OK;
-}
-
 // Mixed form, anchored conjugate gradients (the default): one evaluation of
 // the field of the ASPC predictor (pim.c:382-400), the anchor, in FP64 or
 // (mf_anchor, the default) in a matrix-core mode over the FP32 tables and
@@ -1532,32 +1418,14 @@ static int scf_loop_anchored(mdff_ctx *c, int istep, bool refine = false)
     const int nb = cdiv(np, CG_BLOCK);
     // split-FP16 steps: the kernels that write p also give the channel
     // maxima of p that scale the forward sum's weights
-    double *wp = (lo && c->mf_inner == KM_F16X3 && !(c->cg_precond && c->t2_ok && c->t2f != nullptr)) ? c->gwp
-                                                                                                          : nullptr;
+    double *wp = (lo && c->mf_inner == KM_F16X3) ? c->gwp : nullptr;
     double *rz[2] = {c->cg_part, c->cg_part + nb}, *rm = c->cg_part + 3 * nb;
     // p.q partials: as many as the finish reports (per block of its launch),
     // else k_cg_pq's per CG block
     double *pq = c->pq_buf;
-    // preconditioner: z = pol^T r, or with c->cg_precond (and the FP32 pair
-    // tensors) z = w + pol^T (T_dir w + 2 selfa2 w), w = pol^T r
-    const bool pc = c->cg_precond && c->t2_ok && c->t2f != nullptr;
-    const unsigned tb = cdiv((size_t)n * 16, 256);
-    const double sself = 2.0 * c->selfa2;
-    auto precond = [&](double *rzout, int set_p) {
-        hipLaunchKernelGGL((k_t2_field<float, 16>), dim3(tb), dim3(256), 0, c->st, n, c->capP, np, c->pinv, c->cntp,
-                           c->idxp, c->t2f, c->cg_w, c->dc_g, act);
-        hipLaunchKernelGGL(k_cg_precond, dim3(nb), dim3(CG_BLOCK), 0, c->st, n, np, c->pidx, c->type, c->tab_d,
-                           c->dc_R, c->cg_w, c->dc_g, sself, c->cg_z, c->cg_p, set_p, rzout, act);
-    };
-    if (pc) {
-        // r and w = pol^T r (k_cg_init's z), then z = P r and p = z
-        hipLaunchKernelGGL(k_cg_init, dim3(nb), dim3(CG_BLOCK), 0, c->st, n, np, c->pidx, c->type, c->tab_d, c->ef,
-                           c->mu_ind, c->dc_R, c->cg_w, c->cg_p, c->dc_d, rz[0], act);
-        precond(rz[0], 1);
-    } else {
-        hipLaunchKernelGGL(k_cg_init, dim3(nb), dim3(CG_BLOCK), 0, c->st, n, np, c->pidx, c->type, c->tab_d, c->ef,
-                           c->mu_ind, c->dc_R, c->cg_z, c->cg_p, c->dc_d, rz[0], act, wp);
-    }
+    // preconditioner z = pol^T r
+    hipLaunchKernelGGL(k_cg_init, dim3(nb), dim3(CG_BLOCK), 0, c->st, n, np, c->pidx, c->type, c->tab_d, c->ef,
+                       c->mu_ind, c->dc_R, c->cg_z, c->cg_p, c->dc_d, rz[0], act, wp);
     int cur = 0, launched = 1;
     const int limit = std::max(maxit, minit) + 1;
     auto step = [&]() {
@@ -1570,22 +1438,16 @@ static int scf_loop_anchored(mdff_ctx *c, int istep, bool refine = false)
         if (npq == 0)
             hipLaunchKernelGGL(k_cg_pq, dim3(nb), dim3(CG_BLOCK), 0, c->st, n, np, c->pidx, c->type, c->tab_d,
                                c->cg_p, c->dc_g, pq, act);
-        if (pc) {
-            hipLaunchKernelGGL(k_cga_update_pc, dim3(nb), dim3(CG_BLOCK), 0, c->st, n, np, nb, nbq, c->pidx, c->type,
-                               c->tab_d, c->cg_p, c->dc_g, c->mu_ind, c->ef, c->dc_R, c->cg_w, rz[cur], pq, rm, act);
-            precond(rz[cur ^ 1], 0);
-        } else {
-            hipLaunchKernelGGL(k_cga_update, dim3(nb), dim3(CG_BLOCK), 0, c->st, n, np, nb, nbq, c->pidx, c->type,
-                               c->tab_d, c->cg_p, c->dc_g, c->mu_ind, c->ef, c->dc_R, c->cg_z, rz[cur], pq,
-                               rz[cur ^ 1], rm, act);
-        }
+        hipLaunchKernelGGL(k_cga_update, dim3(nb), dim3(CG_BLOCK), 0, c->st, n, np, nb, nbq, c->pidx, c->type,
+                           c->tab_d, c->cg_p, c->dc_g, c->mu_ind, c->ef, c->dc_R, c->cg_z, rz[cur], pq, rz[cur ^ 1],
+                           rm, act);
         hipLaunchKernelGGL(k_cga_next, dim3(nb), dim3(CG_BLOCK), 0, c->st, n, np, nb, c->pidx, c->cg_z, c->cg_p,
                            rz[cur], rz[cur ^ 1], rm, c->ctl, c->scal, maxit, minit, c->p.conv_tol_ind, wp);
         cur ^= 1;
         launched++;
     };
     const int hint = refine ? c->hint_ref : c->hint_cg;
-    int first = std::min(limit, 1 + std::max(1, hint > 0 ? hint : c->inner_iters));
+    int first = std::min(limit, 1 + std::max(1, hint > 0 ? hint : c->cg_first));
     while (launched < first) step();
     for (;;) {
         int rc = sync_scal(c);
@@ -1627,15 +1489,15 @@ static int scf_refine_fp64(mdff_ctx *c, int istep)
     return rc;
 }
 
+// the mixed form's SCF: anchored CG (symmetric polarisabilities), else the
+// reference's Jacobi schedule with FP64 fields (scf_loop)
 static int scf_mixed(mdff_ctx *c, int istep)
 {
-    if (c->scf_solver >= 2 && c->cg_ok && c->npol > 0) {
-        int rc = scf_loop_anchored(c, istep);
-        const bool run = c->p.max_scf_pim_iter > 0 || c->p.min_scf_pim_iter > 0;
-        if (c->scf_refine && run && (rc == MDFF_OK || rc == MDFF_ERR_SCF)) rc = scf_refine_fp64(c, istep);
-        return rc;
-    }
-    return scf_loop_mixed(c, istep);
+    if (!scf_anchored_ok(c)) return scf_loop(c, istep);
+    int rc = scf_loop_anchored(c, istep);
+    const bool run = c->p.max_scf_pim_iter > 0 || c->p.min_scf_pim_iter > 0;
+    if (c->scf_refine && run && (rc == MDFF_OK || rc == MDFF_ERR_SCF)) rc = scf_refine_fp64(c, istep);
+    return rc;
 }
 
 // scfkO, momentpolaSCFkO pim.c:457-603 (FP64 in both forms): the SCF loop
@@ -2067,7 +1929,7 @@ void mdff_destroy(mdff_ctx *c)
     double **dp[] = {&c->r, &c->v, &c->f, &c->fold, &c->rs, &c->s, &c->s0, &c->invmass, &c->massd, &c->q, &c->qz,
                      &c->mu_static, &c->mu_ind, &c->mu_tot, &c->hist, &c->ef_stat, &c->ef, &c->ef_dir, &c->ef_rec,
                      &c->efg, &c->efg_dir, &c->efg_rec, &c->f_sr, &c->f_dir, &c->f_rec, &c->acc_sr, &c->uw_dir,
-                     &c->selfbuf, &c->ke, &c->upol_i, &c->rmsd_i, &c->tmp_q, &c->tmp_mu, &c->tmp_qraw, &c->tmp_mur, &c->dc_R, &c->dc_d, &c->dc_g, &c->cg_z, &c->cg_p, &c->cg_part, &c->chk_part, &c->pq_buf, &c->t2dir, &c->vc_part, &c->ko_f, &c->ko_dmu, &c->t2ko, &c->cg_w, &c->scal,
+                     &c->selfbuf, &c->ke, &c->upol_i, &c->rmsd_i, &c->tmp_q, &c->tmp_mu, &c->tmp_qraw, &c->tmp_mur, &c->dc_R, &c->dc_d, &c->dc_g, &c->cg_z, &c->cg_p, &c->cg_part, &c->chk_part, &c->pq_buf, &c->t2dir, &c->vc_part, &c->ko_f, &c->ko_dmu, &c->t2ko, &c->scal,
                      &c->kvec, &c->Ak, &c->kcoe, &c->W, &c->ek, &c->kpart, &c->bpart, &c->ksum, &c->gsc, &c->gwp, &c->gwq};
     for (auto pp : dp) dfree(*pp);
     free_cells(c);
@@ -2237,7 +2099,7 @@ int mdff_create(mdff_ctx **out, const mdff_params *p, int nion, const int *typia
     ALLOC(c->chk_part, 2 * (size_t)cdiv(n, 256));
     ALLOC(c->pq_buf, (size_t)cdiv(n, 4)); ALLOC(c->t2dir, 3 * n);
     ALLOC(c->vc_part, 2 * (size_t)cdiv(n, 256)); ALLOC(c->vc_cnt, 1); ALLOC(c->chk_cnt, 1);
-    ALLOC(c->ko_f, 3 * n); ALLOC(c->ko_dmu, 3 * n); ALLOC(c->cg_w, 3 * n);
+    ALLOC(c->ko_f, 3 * n); ALLOC(c->ko_dmu, 3 * n);
     // scal [S_COUNT doubles] then ctl [C_COUNT ints] in one allocation, so
     // that one copy brings both to the host
     ALLOC(c->scal, S_COUNT + ((size_t)C_COUNT * sizeof(int) + sizeof(double) - 1) / sizeof(double));
@@ -2427,7 +2289,7 @@ int mdff_set_kspace_shape(mdff_ctx *c, int cw64, int cw32, int fwd_blocks, int b
 int mdff_set_scf_anchor(mdff_ctx *c, int anchor_mode)
 {
     if (!c) return MDFF_ERR_ARG;
-    if (!(anchor_mode == 0 || anchor_mode == KM_BF16X3 || anchor_mode == KM_F32 || anchor_mode == KM_F16X3))
+    if (!(anchor_mode == 0 || anchor_mode == KM_F32 || anchor_mode == KM_F16X3))
         return MDFF_ERR_ARG;
     c->mf_anchor = anchor_mode;
     return MDFF_OK;
@@ -2436,7 +2298,7 @@ int mdff_set_scf_anchor(mdff_ctx *c, int anchor_mode)
 int mdff_set_kspace_modes(mdff_ctx *c, int eval_mode, int inner_mode)
 {
     if (!c) return MDFF_ERR_ARG;
-    auto ok = [](int m) { return m == 0 || m == KM_BF16X3 || m == KM_F32 || m == KM_F16X3; };
+    auto ok = [](int m) { return m == 0 || m == KM_F32 || m == KM_F16X3; };
     if (!ok(eval_mode) || !ok(inner_mode)) return MDFF_ERR_ARG;
     if (eval_mode) c->mf_lo = eval_mode;
     if (inner_mode) c->mf_inner = inner_mode;
@@ -2961,15 +2823,14 @@ int mdff_time_kernel(mdff_ctx *c, int which, int lo, int iters, double *ms)
         *ms = t / iters;
         return rc;
     }
-    if (!c || !ms || iters < 1 || !c->p.lcoulombic || lo < 0 || lo > 3) return MDFF_ERR_ARG;
+    if (!c || !ms || iters < 1 || !c->p.lcoulombic || !(lo == 0 || lo == 1 || lo == 3)) return MDFF_ERR_ARG;
     if (lo && !lo_of(c)) return MDFF_ERR_ARG;
-    // lo = 2: FP32 tables with the split-BF16 matrix-core mode
+    // lo = 3: FP32 tables with the split-FP16 matrix-core mode
     struct ModeGuard {
         mdff_ctx *c; int keep;
         ~ModeGuard() { c->mf_lo = keep; }
     } guard{c, c->mf_lo};
     if (lo == 1) c->mf_lo = KM_F32;
-    else if (lo == 2) { c->mf_lo = KM_BF16X3; lo = 1; }
     else if (lo == 3) { c->mf_lo = KM_F16X3; lo = 1; }
     int n = c->n;
     int polar = (which == 0 || which == 1 || which == 2);
@@ -3077,27 +2938,13 @@ int mdff_get_counters(mdff_ctx *c, long long out[4])
     return MDFF_OK;
 }
 
-int mdff_set_scf_inner(mdff_ctx *c, int inner)
-{
-    if (!c || inner < 0 || inner > 64) return MDFF_ERR_ARG;
-    c->inner_iters = inner;
-    return MDFF_OK;
-}
-
 int mdff_set_scf_solver(mdff_ctx *c, int solver, int inner)
 {
-    if (!c || solver < 0 || solver > 3 || inner < 0 || inner > 64) return MDFF_ERR_ARG;
-    if (!lo_of(c)) {
-        // FP64 form: 0 the reference's Jacobi schedule, 2 anchored CG in FP64
-        if (solver != 0 && solver != 2) return MDFF_ERR_ARG;
-        c->fp64_cg = solver == 2;
-        c->hint_cg = 0;
-        return MDFF_OK;
-    }
-    c->scf_solver = solver;
-    c->cg_precond = solver == 3;
-    c->inner_iters = inner;
-    c->hint_mixed = c->hint_cg = 0;
+    if (!c || (solver != 0 && solver != 2) || inner < 0 || inner > 64) return MDFF_ERR_ARG;
+    if (lo_of(c)) c->scf_solver = solver;
+    else c->fp64_cg = solver == 2;
+    c->cg_first = inner;
+    c->hint_cg = 0;
     return MDFF_OK;
 }
 

@@ -14,8 +14,8 @@
 // per lane; the small one is staged through LDS and shared by the block's
 // waves.  For the 16x16x4 FP64/FP32 instructions a lane group g = lane/16
 // takes K = 8g + s at step s of an 8-step block, for both operands (any
-// permutation of K leaves the sum unchanged); the 16x16x32 BF16 and FP16
-// instructions take K = 8g..8g+7 in one step.
+// permutation of K leaves the sum unchanged); the 16x16x32 FP16
+// instruction takes K = 8g..8g+7 in one step.
 //
 // Split-FP16 (KM_F16X3) keeps 11 + 11 bits of every operand value, but only
 // near 1: values below the normal range keep an absolute resolution of
@@ -32,28 +32,25 @@ struct GT<KM_F64> { using T = double; };
 template <>
 struct GT<KM_F32> { using T = float; };
 template <>
-struct GT<KM_BF16X3> { using T = __bf16; };
-template <>
 struct GT<KM_F16X3> { using T = _Float16; };
 
-// split modes: two arrays (hi, lo) of 16-bit values and three products
+// the split mode: two arrays (hi, lo) of 16-bit values and three products
 template <int MODE>
-constexpr bool g_is_split() { return MODE == KM_BF16X3 || MODE == KM_F16X3; }
+constexpr bool g_is_split() { return MODE == KM_F16X3; }
 
 template <int MODE>
-using gvec_t = typename std::conditional<MODE == KM_F16X3, mf_h8, mf_bf8>::type;
+using gvec_t = mf_h8;
 
 template <int MODE>
 __device__ __forceinline__ mf_f4 g_mma_split(gvec_t<MODE> a, gvec_t<MODE> b, mf_f4 c)
 {
-    if constexpr (MODE == KM_F16X3) return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
-    else return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+    return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
 }
 
 typedef double g_d8 __attribute__((ext_vector_type(8)));
 typedef float g_f8 __attribute__((ext_vector_type(8)));
 
-// store x into the operand arrays of a mode (hi and, for split BF16, lo)
+// store x into the operand arrays of a mode (hi and, for split FP16, lo)
 template <int MODE>
 __device__ __forceinline__ void g_store(void *hi, void *lo, size_t i, double x)
 {
@@ -61,16 +58,12 @@ __device__ __forceinline__ void g_store(void *hi, void *lo, size_t i, double x)
         ((double *)hi)[i] = x;
     } else if constexpr (MODE == KM_F32) {
         ((float *)hi)[i] = (float)x;
-    } else if constexpr (MODE == KM_F16X3) {
+    } else {
         // through FP32: a direct double-to-half conversion is a long
         // software sequence; the low part absorbs any double rounding of hi
         _Float16 h = (_Float16)(float)x;
         ((_Float16 *)hi)[i] = h;
         ((_Float16 *)lo)[i] = (_Float16)(float)(x - (double)h);
-    } else {
-        __bf16 h = (__bf16)(float)x;
-        ((__bf16 *)hi)[i] = h;
-        ((__bf16 *)lo)[i] = (__bf16)(float)(x - (double)(float)h);
     }
 }
 
@@ -85,17 +78,12 @@ __device__ __forceinline__ void g_store2(void *hi, void *lo, size_t i, double x0
     } else if constexpr (MODE == KM_F32) {
         typedef float f2 __attribute__((ext_vector_type(2)));
         *(f2 *)((float *)hi + i) = f2{(float)x0, (float)x1};
-    } else if constexpr (MODE == KM_F16X3) {
+    } else {
         // through FP32 (see g_store)
         typedef _Float16 h2 __attribute__((ext_vector_type(2)));
         const _Float16 a = (_Float16)(float)x0, b = (_Float16)(float)x1;
         *(h2 *)((_Float16 *)hi + i) = h2{a, b};
         *(h2 *)((_Float16 *)lo + i) = h2{(_Float16)(float)(x0 - (double)a), (_Float16)(float)(x1 - (double)b)};
-    } else {
-        typedef __bf16 b2 __attribute__((ext_vector_type(2)));
-        const __bf16 a = (__bf16)(float)x0, b = (__bf16)(float)x1;
-        *(b2 *)((__bf16 *)hi + i) = b2{a, b};
-        *(b2 *)((__bf16 *)lo + i) = b2{(__bf16)(float)(x0 - (double)(float)a), (__bf16)(float)(x1 - (double)(float)b)};
     }
 }
 
@@ -699,8 +687,7 @@ __device__ __forceinline__ GRun<MODE> g_load(const void *base, size_t off)
     GRun<MODE> r;
     if constexpr (MODE == KM_F64) r.v = *(const g_d8 *)((const double *)base + off);
     else if constexpr (MODE == KM_F32) r.v = *(const g_f8 *)((const float *)base + off);
-    else if constexpr (MODE == KM_F16X3) r.v = *(const mf_h8 *)((const _Float16 *)base + off);
-    else r.v = *(const mf_bf8 *)((const __bf16 *)base + off);
+    else r.v = *(const mf_h8 *)((const _Float16 *)base + off);
     return r;
 }
 

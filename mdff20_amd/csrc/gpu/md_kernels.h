@@ -413,52 +413,6 @@ __global__ void k_sub_mean(int n, const double *sums, double *v)
     v[i] -= sums[a] * on;
 }
 
-// mixed-form SCF correction: R = pol^T E - mu, d = R on polarisable ions
-__global__ void k_dc_residual(int n, const int *__restrict__ type, const TypeTab *__restrict__ T,
-                              const double *__restrict__ ef, const double *__restrict__ mu, double *__restrict__ R,
-                              double *__restrict__ d, const int *active)
-{
-    if (!*active) return;
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    int t = type[i];
-    for (int a = 0; a < 3; a++) {
-        double v = 0.0;
-        if (T->polar[t]) {
-            const double *P = T->pol[t];
-            v = P[0 * 3 + a] * ef[i] + P[1 * 3 + a] * ef[n + i] + P[2 * 3 + a] * ef[2 * n + i] - mu[(size_t)a * n + i];
-        }
-        R[(size_t)a * n + i] = v;
-        d[(size_t)a * n + i] = v;
-    }
-}
-
-// d = R + pol^T g (one Jacobi sweep of the correction equation)
-__global__ void k_dc_update(int n, const int *__restrict__ type, const TypeTab *__restrict__ T,
-                            const double *__restrict__ R, const double *__restrict__ g, double *__restrict__ d,
-                            const int *active)
-{
-    if (!*active) return;
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    int t = type[i];
-    for (int a = 0; a < 3; a++) {
-        double v = 0.0;
-        if (T->polar[t]) {
-            const double *P = T->pol[t];
-            v = R[(size_t)a * n + i] + P[0 * 3 + a] * g[i] + P[1 * 3 + a] * g[n + i] + P[2 * 3 + a] * g[2 * n + i];
-        }
-        d[(size_t)a * n + i] = v;
-    }
-}
-
-__global__ void k_axpy_gated(int m, const double *x, double *y, const int *active)
-{
-    if (!*active) return;
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < m) y[i] += x[i];
-}
-
 // get_upol pim.c:145-161 per ion
 __global__ void k_upol(int n, const int *__restrict__ type, const TypeTab *__restrict__ T,
                        const double *__restrict__ mu, double *__restrict__ out)
@@ -477,13 +431,13 @@ __global__ void k_upol(int n, const int *__restrict__ type, const TypeTab *__res
 }
 
 // ---------------------------------------------------------------------------
-// Mixed-form SCF correction by preconditioned conjugate gradients on
-//   (pol^-T - T) d = E - pol^-T mu      (mu = pol^T E at the fixed point;
+// Anchored conjugate gradients of the SCF on
+//   (pol^-T - T) mu = E_stat      (mu = pol^T E at the fixed point;
 // SPD for a symmetric, positive polarisability and a converging SCF)
-// with pol^T as preconditioner.  Vectors live on the
-// polarisable rows (ion pidx[k]); dot products are per-block partials that
-// every block of the next kernel sums in the same fixed order, so the
-// whole iteration stays on the device.
+// with pol^T as preconditioner.  Vectors live on the polarisable rows (ion
+// pidx[k]); dot products are per-block partials that every block of the
+// next kernel sums in the same fixed order, so the whole iteration stays on
+// the device.
 #define CG_BLOCK 256
 
 __device__ __forceinline__ double cg_block_sum(double v, double *sh)
@@ -726,66 +680,6 @@ __global__ void __launch_bounds__(CG_BLOCK) k_cg_pq(int n, int npol, const int *
     }
     double s = cg_block_sum(acc, sh);
     if (threadIdx.x == 0) pq_part[blockIdx.x] = s;
-}
-
-// gamma = r.z / p.q; d += gamma p; r -= gamma q; z = pol^T r; partials of
-// the new r.z
-__global__ void __launch_bounds__(CG_BLOCK) k_cg_update(int n, int npol, int nb, const int *__restrict__ pidx,
-                                                        const int *__restrict__ type, const TypeTab *__restrict__ T,
-                                                        const double *__restrict__ p, const double *__restrict__ Tp,
-                                                        double *d, double *r, double *z,
-                                                        const double *__restrict__ rz_old,
-                                                        const double *__restrict__ pq_part, double *rz_new,
-                                                        const int *active)
-{
-    if (!*active) return;
-    __shared__ double sh[CG_BLOCK];
-    const double rz = cg_sum_partials(rz_old, nb, sh);
-    const double pq = cg_sum_partials(pq_part, nb, sh);
-    const double g = (pq != 0.0) ? rz / pq : 0.0;
-    const int k = blockIdx.x * CG_BLOCK + threadIdx.x;
-    double acc = 0.0;
-    if (k < npol) {
-        const int i = pidx[k], t = type[i];
-        const double *P = T->pol[t], *I = T->invpol[t];
-        double pv[3] = {p[i], p[n + i], p[2 * n + i]};
-        double rr[3];
-        for (int a = 0; a < 3; a++) {
-            const size_t o = (size_t)a * n + i;
-            double q = I[a] * pv[0] + I[3 + a] * pv[1] + I[6 + a] * pv[2] - Tp[o];
-            d[o] += g * pv[a];
-            rr[a] = r[o] - g * q;
-            r[o] = rr[a];
-        }
-        for (int a = 0; a < 3; a++) {
-            double zz = P[0 * 3 + a] * rr[0] + P[1 * 3 + a] * rr[1] + P[2 * 3 + a] * rr[2];
-            z[(size_t)a * n + i] = zz;
-            acc += rr[a] * zz;
-        }
-    }
-    double s = cg_block_sum(acc, sh);
-    if (threadIdx.x == 0) rz_new[blockIdx.x] = s;
-}
-
-// beta = r.z (new) / r.z (old); p = z + beta p
-__global__ void __launch_bounds__(CG_BLOCK) k_cg_next(int n, int npol, int nb, const int *__restrict__ pidx,
-                                                      const double *__restrict__ z, double *p,
-                                                      const double *__restrict__ rz_old,
-                                                      const double *__restrict__ rz_new, const int *active)
-{
-    if (!*active) return;
-    __shared__ double sh[CG_BLOCK];
-    const double ro = cg_sum_partials(rz_old, nb, sh);
-    const double rn = cg_sum_partials(rz_new, nb, sh);
-    const double b = (ro != 0.0) ? rn / ro : 0.0;
-    const int k = blockIdx.x * CG_BLOCK + threadIdx.x;
-    if (k < npol) {
-        const int i = pidx[k];
-        for (int a = 0; a < 3; a++) {
-            const size_t o = (size_t)a * n + i;
-            p[o] = z[o] + b * p[o];
-        }
-    }
 }
 
 // mu += d on the polarisable rows
@@ -1062,84 +956,3 @@ __global__ void k_ko_resid(int n, int npol, const int *__restrict__ pidx, const 
     }
 }
 
-// Preconditioned form of the anchored CG: z = P r with
-//   P = pol^T + pol^T T_dir pol^T
-// (the first two terms of (pol^-T - T)^-1 = pol^T (I - T pol^T)^-1 with the
-// real-space dipole tensors and the self term for T), symmetric for a
-// symmetric polarisability.  The update writes w = pol^T r; the field of w
-// comes from k_t2_field; k_cg_precond forms z and the partial sums of r.z.
-__global__ void __launch_bounds__(CG_BLOCK) k_cga_update_pc(int n, int npol, int nb, int nbq,
-                                                            const int *__restrict__ pidx,
-                                                            const int *__restrict__ type,
-                                                            const TypeTab *__restrict__ T,
-                                                            const double *__restrict__ p,
-                                                            const double *__restrict__ Tp, double *mu, double *ef,
-                                                            double *r, double *w, const double *__restrict__ rz_old,
-                                                            const double *__restrict__ pq_part, double *rm_part,
-                                                            const int *active)
-{
-    if (!*active) return;
-    __shared__ double sh[CG_BLOCK];
-    const double rz = cg_sum_partials(rz_old, nb, sh);
-    const double pq = cg_sum_partials(pq_part, nbq, sh);
-    const double g = (pq != 0.0) ? rz / pq : 0.0;
-    const int k = blockIdx.x * CG_BLOCK + threadIdx.x;
-    double rms = 0.0;
-    if (k < npol) {
-        const int i = pidx[k], t = type[i];
-        const double *P = T->pol[t], *I = T->invpol[t];
-        double m[3], e[3], rr[3];
-        for (int a = 0; a < 3; a++) {
-            const size_t o = (size_t)a * n + i;
-            m[a] = mu[o] + g * p[o];
-            e[a] = ef[o] + g * Tp[o];
-            mu[o] = m[a];
-            ef[o] = e[a];
-        }
-        for (int a = 0; a < 3; a++) {
-            rr[a] = e[a] - (I[a] * m[0] + I[3 + a] * m[1] + I[6 + a] * m[2]);
-            r[(size_t)a * n + i] = rr[a];
-            const double paa = P[4 * a];
-            if (paa != 0.0) {
-                const double d = m[a] / paa - e[a];
-                rms += d * d;
-            }
-        }
-        for (int a = 0; a < 3; a++)
-            w[(size_t)a * n + i] = P[0 * 3 + a] * rr[0] + P[1 * 3 + a] * rr[1] + P[2 * 3 + a] * rr[2];
-    }
-    const double sr = cg_block_sum(rms, sh);
-    if (threadIdx.x == 0) rm_part[blockIdx.x] = sr;
-}
-
-// z = w + pol^T (T_dir w + sself w); partial sums of r.z; p = z on the
-// first call of a solve
-__global__ void __launch_bounds__(CG_BLOCK) k_cg_precond(int n, int npol, const int *__restrict__ pidx,
-                                                         const int *__restrict__ type, const TypeTab *__restrict__ T,
-                                                         const double *__restrict__ r, const double *__restrict__ w,
-                                                         const double *__restrict__ tw, double sself, double *z,
-                                                         double *p, int set_p, double *rz_part, const int *active)
-{
-    if (!*active) return;
-    __shared__ double sh[CG_BLOCK];
-    const int k = blockIdx.x * CG_BLOCK + threadIdx.x;
-    double acc = 0.0;
-    if (k < npol) {
-        const int i = pidx[k], t = type[i];
-        const double *P = T->pol[t];
-        double f[3];
-        for (int a = 0; a < 3; a++) {
-            const size_t o = (size_t)a * n + i;
-            f[a] = tw[o] + sself * w[o];
-        }
-        for (int a = 0; a < 3; a++) {
-            const size_t o = (size_t)a * n + i;
-            const double zz = w[o] + (P[0 * 3 + a] * f[0] + P[1 * 3 + a] * f[1] + P[2 * 3 + a] * f[2]);
-            z[o] = zz;
-            if (set_p) p[o] = zz;
-            acc += r[o] * zz;
-        }
-    }
-    const double s = cg_block_sum(acc, sh);
-    if (threadIdx.x == 0) rz_part[blockIdx.x] = s;
-}

@@ -7,16 +7,14 @@
 // embedded in a real GEMM with K = 2 x ions (re, im):
 //   row (nx,c,re): A' = ( w exr, -w exi ) ; row (nx,c,im): A' = ( w exi, w exr )
 //   B'(j,0) = Re(ey ez), B'(j,1) = Im(ey ez).
-// Four arithmetic modes:
+// Three arithmetic modes:
 //   KM_F64    v_mfma_f64_16x16x4f64, FP64 operands and accumulators
 //   KM_F32    v_mfma_f32_16x16x4f32, FP32 operands and accumulators
-//   KM_BF16X3 v_mfma_f32_16x16x32_bf16 on split operands x = hi + lo
-//             (hi = bf16(x), lo = bf16(x - hi)); A.B ~ Ah.Bh + Ah.Bl + Al.Bh,
-//             FP32 accumulation, about 16 significant bits per product.
-//   KM_F16X3  v_mfma_f32_16x16x32_f16 on split operands (hi = f16(x),
-//             lo = f16(x - hi)) scaled into [-1, 1], same three products:
-//             about 22 bits.
-// Result lane layouts (checked on gfx950): f32/bf16 16x16 D[4(l/16)+i][l%16],
+//   KM_F16X3  v_mfma_f32_16x16x32_f16 on split operands x = hi + lo
+//             (hi = f16(x), lo = f16(x - hi)) scaled into [-1, 1];
+//             A.B ~ Ah.Bh + Ah.Bl + Al.Bh with FP32 accumulation, about 22
+//             significant bits per product.
+// Result lane layouts (checked on gfx950): f32/f16 16x16 D[4(l/16)+i][l%16],
 // f64 16x16x4 D[(l/16)+4i][l%16]; operands A[l%16][k], B[k][l%16] with
 // k = l/16 (x4) or 8(l/16)..+7 (x32).
 #pragma once
@@ -24,11 +22,10 @@
 
 #include "dev.h"
 
-enum { KM_F64 = 0, KM_BF16X3 = 1, KM_F32 = 2, KM_F16X3 = 3 };
+enum { KM_F64 = 0, KM_F32 = 2, KM_F16X3 = 3 };
 
 typedef double mf_d4 __attribute__((ext_vector_type(4)));
 typedef float mf_f4 __attribute__((ext_vector_type(4)));
-typedef __bf16 mf_bf8 __attribute__((ext_vector_type(8)));
 typedef _Float16 mf_h8 __attribute__((ext_vector_type(8)));
 
 #define MF_MT 9    // max row tiles of the forward sum: (Kx+1) * C * 2 <= 144
