@@ -437,8 +437,10 @@ def pmc_table(args, kname, dirs):
     return out
 
 
-def trace_summarize(d, top=40):
-    """markdown table of a rocprofv3 --kernel-trace --stats run"""
+def trace_summarize(d, top=40, kname=None, last=5):
+    """markdown table of a rocprofv3 --kernel-trace --stats run; with kname
+    (a --pmc-kernel run) also the average duration of that kernel's last
+    `last` dispatches, the launches its time_kernel call timed"""
     import csv
     import glob
     f = glob.glob(os.path.join(d, "**", "*kernel_stats.csv"), recursive=True)[0]
@@ -449,6 +451,14 @@ def trace_summarize(d, top=40):
         print(f"| `{x['Name'][:100]}` | {x['Calls']} | {float(x['TotalDurationNs']) / 1e6:.2f} | "
               f"{float(x['Percentage']):.1f} | {float(x['AverageNs']) / 1e3:.2f} | {float(x['MinNs']) / 1e3:.2f} | "
               f"{float(x['MaxNs']) / 1e3:.2f} |")
+    if kname:
+        _, _, prefix = kernel_prefix(kname)
+        t = glob.glob(os.path.join(d, "**", "*kernel_trace.csv"), recursive=True)[0]
+        disp = sorted((int(x["Start_Timestamp"]), int(x["End_Timestamp"])) for x in csv.DictReader(open(t))
+                      if prefix.replace(" ", "") in x["Kernel_Name"].replace(" ", ""))[-last:]
+        us = [(b - a) / 1e3 for a, b in disp]
+        print(f"\n{kname} ({prefix}...): last {len(us)} dispatches, average {sum(us) / len(us):.2f} us "
+              f"(min {min(us):.2f}, max {max(us):.2f})")
 
 
 def main():
@@ -482,7 +492,7 @@ def main():
                     help="relaxed positions: read from NPZ when it exists, else relaxed and written there")
     args = ap.parse_args()
     if args.trace_summarize:
-        trace_summarize(args.trace_summarize)
+        trace_summarize(args.trace_summarize, kname=args.pmc_kernel, last=args.pmc_iters)
         return
     if args.pmc_summarize:
         pmc_summarize(args, args.pmc_kernel)

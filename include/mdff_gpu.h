@@ -217,9 +217,10 @@ int mdff_get_timing(mdff_ctx *ctx, double t[6]);
 /* Time one kernel of the engine at the current state: `iters` launches
  * back to back on the context stream; ms = average per launch.  The
  * matrix-core S(k) GEMM kernels (which 0, 1, 3, 4 on the matrix-core path)
- * are timed alone, each launch between its own pair of HIP events (operand
- * builds and the partial-sum pass excluded); the others between two events
- * around all launches.  lo = 0 the FP64 variant, 1 FP32, 3 split-FP16
+ * are timed alone: one pass of the sequence with the GEMM launched `iters`
+ * times back to back between a pair of HIP events (operand builds and the
+ * partial-sum pass outside them); the others between two events around
+ * `iters` passes.  lo = 0 the FP64 variant, 1 FP32, 3 split-FP16
  * (1 and 3 in mixed contexts only).  which:
  *   0 S(k) forward, SCF form (dipoles of the polarisable ions, 3 channels)
  *   1 back-projection, SCF form (field at the polarisable ions)

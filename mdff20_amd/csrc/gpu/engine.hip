@@ -175,8 +175,9 @@ struct mdff_ctx {
     double yosh[7];
     // timing
     hipEvent_t ev[8];
-    hipEvent_t *probe_ev = nullptr; // mdff_time_kernel: events around each k-space GEMM launch
+    hipEvent_t *probe_ev = nullptr; // mdff_time_kernel: events around a k-space GEMM launch
     int probe_k = 0, probe_max = 0;
+    int probe_rep = 1;        // the GEMM launched this many times back to back between the events
     int have_events = 0;
     double t_ms[6] = {0, 0, 0, 0, 0, 0};
     int last_nscf = 0;
@@ -219,6 +220,12 @@ static int sync_scal(mdff_ctx *c)
 }
 
 // mdff_time_kernel brackets each k-space GEMM launch with a pair of events
+// launches of a k-space GEMM between its probe events
+static int probe_reps(const mdff_ctx *c, bool probe)
+{
+    return (probe && c->probe_ev && c->probe_k < c->probe_max) ? c->probe_rep : 1;
+}
+
 static void probe_mark(mdff_ctx *c, int end)
 {
     if (!c->probe_ev || c->probe_k >= c->probe_max) return;
@@ -869,10 +876,13 @@ static int launch_fwd_gemm(mdff_ctx *c, int polar, int lo, int C, const double *
         nch = (ga2.ldk + ga2.kchunk - 1) / ga2.kchunk;
         if (!go) return nch;
         dim3 grid(gx, nch);
+        const int reps = probe_reps(c, probe);
         if (probe) probe_mark(c, 0);
-        if (mode == KM_F64) gfwd_cw<KM_F64>(c, grid, CW, MT, ga2);
-        else if (mode == KM_F32) { if (gen) gfwd_cw<KM_F32, true>(c, grid, CW, MT, ga2); else gfwd_cw<KM_F32>(c, grid, CW, MT, ga2); }
-        else { if (gen) gfwd_cw<KM_F16X3, true>(c, grid, CW, MT, ga2); else gfwd_cw<KM_F16X3>(c, grid, CW, MT, ga2); }
+        for (int r = 0; r < reps; r++) {
+            if (mode == KM_F64) gfwd_cw<KM_F64>(c, grid, CW, MT, ga2);
+            else if (mode == KM_F32) { if (gen) gfwd_cw<KM_F32, true>(c, grid, CW, MT, ga2); else gfwd_cw<KM_F32>(c, grid, CW, MT, ga2); }
+            else { if (gen) gfwd_cw<KM_F16X3, true>(c, grid, CW, MT, ga2); else gfwd_cw<KM_F16X3>(c, grid, CW, MT, ga2); }
+        }
         if (probe) probe_mark(c, 1);
         if (!psum) return nch;
         if (defer) {
@@ -972,10 +982,13 @@ static int launch_back_gemm(mdff_ctx *c, int polar, int lo, int NCH, const int *
         if (!go) return nsp;
         gb2.t2_y0 = nsp;
         dim3 grid(gx, nsp + nt2);
+        const int reps = probe_reps(c, probe);
         if (probe) probe_mark(c, 0);
-        if (mode == KM_F64) gback_ct<KM_F64, double>(c, grid, CT, gb2);
-        else if (mode == KM_F32) gback_ct<KM_F32, float>(c, grid, CT, gb2);
-        else gback_ct<KM_F16X3, float>(c, grid, CT, gb2);
+        for (int r = 0; r < reps; r++) {
+            if (mode == KM_F64) gback_ct<KM_F64, double>(c, grid, CT, gb2);
+            else if (mode == KM_F32) gback_ct<KM_F32, float>(c, grid, CT, gb2);
+            else gback_ct<KM_F16X3, float>(c, grid, CT, gb2);
+        }
         if (probe) probe_mark(c, 1);
         return nsp;
     };
@@ -2895,33 +2908,42 @@ int mdff_time_kernel(mdff_ctx *c, int which, int lo, int iters, double *ms)
     once();
     c->fwd_part = 0;
     // the k-space GEMM kernels are timed alone (operand builds and the
-    // partial-sum pass excluded), each launch between its own two events
+    // partial-sum pass excluded): `iters` launches of the GEMM back to back
+    // between one pair of events, inside one pass of the sequence
     std::vector<hipEvent_t> pe;
-    if (which == 0 || which == 1 || which == 3 || which == 4) {
-        pe.resize(2 * (size_t)iters);
-        for (auto &e : pe)
-            if (hipEventCreate(&e) != hipSuccess) e = nullptr;
-        bool ok = true;
-        for (auto e : pe) ok = ok && e != nullptr;
-        if (ok) { c->probe_ev = pe.data(); c->probe_k = 0; c->probe_max = iters; }
+    const bool gemm = which == 0 || which == 1 || which == 3 || which == 4;
+    if (gemm) {
+        pe.resize(2);
+        bool ok = hipEventCreate(&pe[0]) == hipSuccess;
+        ok = hipEventCreate(&pe[1]) == hipSuccess && ok;
+        if (ok) { c->probe_ev = pe.data(); c->probe_k = 0; c->probe_max = 1; c->probe_rep = iters; }
+        else pe.clear();
     }
-    HIPCHK(hipEventRecord(c->ev[6], c->st));
-    for (int k = 0; k < iters; k++) once();
-    HIPCHK(hipEventRecord(c->ev[7], c->st));
-    HIPCHK(hipEventSynchronize(c->ev[7]));
-    float t = 0.0f;
-    HIPCHK(hipEventElapsedTime(&t, c->ev[6], c->ev[7]));
-    *ms = (double)t / iters;
-    if (c->probe_ev && c->probe_k == iters) {
-        double sum = 0.0;
-        for (int k = 0; k < iters; k++) {
-            float tk = 0.0f;
-            if (hipEventElapsedTime(&tk, pe[2 * k], pe[2 * k + 1]) == hipSuccess) sum += tk;
+    bool timed = false;
+    if (c->probe_ev) {
+        once();
+        // a sum that took another path (vector ALU) set no probe events
+        const bool hit = c->probe_k == 1;
+        c->probe_ev = nullptr;
+        c->probe_rep = 1;
+        float tk = 0.0f;
+        if (hit && hipEventSynchronize(pe[1]) == hipSuccess && hipEventElapsedTime(&tk, pe[0], pe[1]) == hipSuccess) {
+            *ms = (double)tk / iters;
+            timed = true;
         }
-        *ms = sum / iters;
+    }
+    if (!timed) {
+        HIPCHK(hipEventRecord(c->ev[6], c->st));
+        for (int k = 0; k < iters; k++) once();
+        HIPCHK(hipEventRecord(c->ev[7], c->st));
+        HIPCHK(hipEventSynchronize(c->ev[7]));
+        float t = 0.0f;
+        HIPCHK(hipEventElapsedTime(&t, c->ev[6], c->ev[7]));
+        *ms = (double)t / iters;
     }
     c->probe_ev = nullptr;
     c->probe_k = c->probe_max = 0;
+    c->probe_rep = 1;
     for (auto e : pe)
         if (e) (void)hipEventDestroy(e);
     c->fwd_part = 0;
