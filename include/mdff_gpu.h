@@ -235,9 +235,10 @@ int mdff_get_timing(mdff_ctx *ctx, double t[6]);
 int mdff_time_kernel(mdff_ctx *ctx, int which, int lo, int iters, double *ms);
 
 /* Counters since creation: [0] force calls, [1] FP64 SCF field
- * evaluations, [2] low-precision SCF field evaluations (mixed form: the
- * CG steps, and the anchor when it is low precision), [3] polarisable
- * ions.  No counterpart in the reference (measurement support). */
+ * evaluations, [2] inner field evaluations: the mixed form's low-precision
+ * ones (the CG steps, and the anchor when it is low precision) and scfkO's
+ * inner real-space sweeps, [3] polarisable ions.  No counterpart in the
+ * reference (measurement support). */
 int mdff_get_counters(mdff_ctx *ctx, long long out[4]);
 
 /* The SCF solver of the PIM dipoles (algo_moment_from_pola scf).
