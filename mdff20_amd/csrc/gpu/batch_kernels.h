@@ -1391,7 +1391,7 @@ __global__ void kb_adj_rhs(int n, const int *__restrict__ type, const TypeTab *_
     s2[i] = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
 }
 
-// lam = pol^T s as the start of the adjoint iteration
+// lam = pol s as the start of the adjoint iteration
 __global__ void kb_adj_start(int n, const int *__restrict__ type, const TypeTab *__restrict__ T,
                              const double *__restrict__ s, double *__restrict__ lam)
 {
@@ -1402,14 +1402,14 @@ __global__ void kb_adj_start(int n, const int *__restrict__ type, const TypeTab 
         double v = 0.0;
         if (T->polar[t]) {
             const double *P = T->pol[t];
-            v = P[0 * 3 + c] * s[i] + P[1 * 3 + c] * s[n + i] + P[2 * 3 + c] * s[2 * n + i];
+            v = P[c * 3 + 0] * s[i] + P[c * 3 + 1] * s[n + i] + P[c * 3 + 2] * s[2 * n + i];
         }
         lam[(size_t)c * n + i] = v;
     }
 }
 
 // one Jacobi sweep of (pol^-1 - T) lam = s: residual r = pol^-1 lam - s - T lam
-// (|r|^2 per ion), then lam = pol^T (s + T lam); eT = T lam
+// (|r|^2 per ion), then lam = pol (s + T lam); eT = T lam
 __global__ void kb_adj_sweep(int n, const int *__restrict__ cfg, const int *__restrict__ act,
                              const int *__restrict__ type, const TypeTab *__restrict__ T, const double *__restrict__ s,
                              const double *__restrict__ eT, double *__restrict__ lam, double *__restrict__ r2)
@@ -1426,7 +1426,7 @@ __global__ void kb_adj_sweep(int n, const int *__restrict__ cfg, const int *__re
             double v = I[3 * c] * l[0] + I[3 * c + 1] * l[1] + I[3 * c + 2] * l[2] - b[c];
             rr += v * v;
         }
-        for (int c = 0; c < 3; c++) lam[(size_t)c * n + i] = P[0 * 3 + c] * b[0] + P[1 * 3 + c] * b[1] + P[2 * 3 + c] * b[2];
+        for (int c = 0; c < 3; c++) lam[(size_t)c * n + i] = P[c * 3 + 0] * b[0] + P[c * 3 + 1] * b[1] + P[c * 3 + 2] * b[2];
     }
     r2[i] = rr;
 }
@@ -1453,23 +1453,25 @@ __global__ void __launch_bounds__(1024) kb_adj_control(int nconf, const double *
     if (threadIdx.x == 0) *any = sany;
 }
 
-// polarisability gradient per ion: coul (pol^-1 lam)_a (pol^-1 mu)_b, out[9][n]
+// polarisability gradient per ion, out[9][n]: with R = pol^-T mu - E the
+// SCF residual (mu = pol^T E, pim.c:637), d(pol^-T)/d pol_ab mu =
+// -pol^-T e_b (pol^-T mu)_a, so dL/d pol_ab = coul (pol^-T mu)_a (pol^-1 lam)_b
 __global__ void kb_adj_pol(int n, const int *__restrict__ type, const TypeTab *__restrict__ T, double coul,
                            const double *__restrict__ lam, const double *__restrict__ mu, double *__restrict__ out)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     int t = type[i];
-    double u[3] = {0, 0, 0}, v[3] = {0, 0, 0};
+    double u[3] = {0, 0, 0}, w[3] = {0, 0, 0};
     if (T->polar[t]) {
         const double *I = T->invpol[t];
         double l[3] = {lam[i], lam[n + i], lam[2 * n + i]};
         double m[3] = {mu[i], mu[n + i], mu[2 * n + i]};
         for (int c = 0; c < 3; c++) {
-            u[c] = I[3 * c] * l[0] + I[3 * c + 1] * l[1] + I[3 * c + 2] * l[2];
-            v[c] = I[3 * c] * m[0] + I[3 * c + 1] * m[1] + I[3 * c + 2] * m[2];
+            u[c] = I[3 * c] * l[0] + I[3 * c + 1] * l[1] + I[3 * c + 2] * l[2];   // (pol^-1 lam)_c
+            w[c] = I[c] * m[0] + I[3 + c] * m[1] + I[6 + c] * m[2];               // (pol^-T mu)_c
         }
     }
-    for (int c = 0; c < 3; c++)
-        for (int d = 0; d < 3; d++) out[(size_t)(3 * c + d) * n + i] = coul * u[c] * v[d];
+    for (int a = 0; a < 3; a++)
+        for (int b = 0; b < 3; b++) out[(size_t)(3 * a + b) * n + i] = coul * w[a] * u[b];
 }
