@@ -373,3 +373,41 @@ def test_cell_grid_list_equals_all_pairs():
         assert np.array_equal(ec.positions(), ea.positions())
         tcell, tall = ec.time_kernel(7, iters=5), ea.time_kernel(7, iters=5)
         print(f"[gpu] list rebuild at N=1536: cells {tcell * 1e3:.1f} us, all pairs {tall * 1e3:.1f} us", flush=True)
+
+
+def test_nonsymmetric_polarisability():
+    """a polarisability tensor that is not symmetric, read transposed by the
+    SCF (mu = pol^T E, pim.c:637).  The rmsd of get_rmsd_mu (pim.c:431-449)
+    divides by the diagonal entries only, so it does not vanish at the fixed
+    point: the loop runs to max_scf_pim_iter and reports it unconverged, as
+    the reference does.  The FP64 form equals the oracle (1e-10, the same
+    count); the mixed form cannot run the anchored CG (its system would not
+    be symmetric) and takes the reference's Jacobi schedule on FP64 fields,
+    within 1e-5 of the FP64 form"""
+    import ctypes as C
+    p, cfg, _ = load_case("pim192_static")
+    q = m.Params()
+    C.pointer(q)[0] = p
+    q.conv_tol_ind = 1e-11
+    q.max_scf_pim_iter = 80
+    pt = [t for t in range(q.ntype) if q.polit[t][0][0] != 0.0][0]
+    q.polit[pt][0][1] += 0.1 * q.polit[pt][0][0]
+    q.polit[pt][2][0] -= 0.05 * q.polit[pt][0][0]
+    o = Oracle(q, cfg.typia, cfg.A, cfg.r)
+    to = o.engforce(0)
+    with m.Engine(q, cfg.typia, cfg.A, cfg.r) as ef, \
+            m.Engine(q, cfg.typia, cfg.A, cfg.r, precision=m.MIXED) as em:
+        tf, tm = ef.engforce(0), em.engforce(0)
+        print(f"[gpu] non-symmetric pol: FP64 {tf['n_scf']} / oracle {to['n_scf']} iterations, "
+              f"mixed {tm['n_scf']}; FP64 dipoles {rel(ef.dipoles(), o.dipoles()):.2e}, "
+              f"mixed forces {rel(em.forces(), ef.forces()):.2e}", flush=True)
+        assert tf["status"] == m.ERR_SCF and tm["status"] == m.ERR_SCF and to["status"] == m.ERR_SCF
+        assert tf["n_scf"] == to["n_scf"] == tm["n_scf"]
+        assert rel(ef.dipoles(), o.dipoles()) < 1e-10
+        assert rel(ef.forces(), o.forces()) < 1e-10
+        assert tf["u_tot"] == pytest.approx(to["u_tot"], rel=1e-10)
+        assert rel(em.dipoles(), ef.dipoles()) < 1e-5
+        assert rel(em.forces(), ef.forces()) < 1e-5
+        assert tm["u_tot"] == pytest.approx(tf["u_tot"], rel=1e-6)
+        cm = em.counters()
+        assert cm["scf_fp64"] > 0 and cm["scf_inner"] == 0

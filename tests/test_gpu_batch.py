@@ -265,8 +265,10 @@ def test_nonsymmetric_polarisability_batch():
     """a polarisability tensor that is not symmetric (the SCF reads it
     transposed, pim.c:637): the conjugate-gradient solves need a symmetric
     system, so the mixed SCF and the adjoint solve run their Jacobi
-    sequences; the mixed forces stay within 1e-5 of the FP64 batch and the
-    adjoint's 'pol' gradient equals the finite difference"""
+    sequences (the SCF to max_scf_pim_iter: the rmsd of pim.c:431-449
+    divides by the diagonal entries only and does not vanish); the mixed
+    forces stay within 1e-5 of the FP64 batch and the adjoint's 'pol'
+    gradient equals the finite difference"""
     p, confs = _configs()
     confs = confs[:2]
     p = _tight(p)
