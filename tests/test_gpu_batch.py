@@ -307,3 +307,49 @@ def test_nonsymmetric_polarisability_batch():
         fd = (L[0] - L[1]) / (2 * h)
         print(f"  adjoint {grad:.10e} fd {fd:.10e}, {g['adjoint_iterations']} sweeps", flush=True)
         assert abs(grad - fd) <= 1e-6 * abs(fd)
+
+
+def test_batch_vector_alu_kspace():
+    """a configuration whose k box is wider than the matrix-core tiles take
+    (kES 16 in x: 17 nx planes) sends the whole batch's k-space sums to the
+    vector-ALU kernels: FP64 against the oracle per configuration (1e-10,
+    the same SCF counts), the mixed batch within 1e-5 of it, and the
+    adjoint's 'pol' gradient against the finite difference"""
+    from oracle.oracle import Oracle
+    p, confs = _configs()
+    confs = [confs[0], (confs[2][0], dict(confs[2][1], kES=(16, 6, 6)))]
+    print("[gpu] batch vector-ALU k space", flush=True)
+    with m.batch.Batch(p, [c[0] for c in confs], [c[1] for c in confs], precision=m.FP64) as b, \
+            m.batch.Batch(p, [c[0] for c in confs], [c[1] for c in confs], precision=m.MIXED) as bm:
+        th, tm = b.engforce(), bm.engforce()
+        F, Fm = b.split(b.forces()), bm.split(bm.forces())
+        for c, ((t, A, r), e) in enumerate(confs):
+            o = Oracle(_oracle_params(p, e), t, A, r)
+            to = o.engforce(0)
+            for k in KEYS:
+                assert th[c][k] == pytest.approx(to[k], rel=1e-10, abs=1e-12), (c, k, th[c][k], to[k])
+            assert th[c]["n_scf"] == to["n_scf"], c
+            assert rel(F[c], o.forces()) < 1e-10, c
+            o.close()
+            frms = np.sqrt(np.mean((Fm[c] - F[c]) ** 2)) / np.sqrt(np.mean(F[c] ** 2))
+            assert frms < 1e-5, (c, frms)
+    q = _tight(p)
+    with m.batch.Batch(q, [c[0] for c in confs], [c[1] for c in confs]) as b:
+        spec = m.batch.ParamSpec(q, [("pol", 0)], log=False)
+        b.set_params(spec.params(spec.values() * 1.05))
+        b.engforce()
+        fref = b.forces()
+        b.set_params(q)
+        g = b.adjoint(fref)
+        grad = spec.gather(g)[0]
+        x0 = spec.values()[0]
+        h = 1e-6 * x0
+        L = []
+        for sgn in (1, -1):
+            b.set_params(spec.params(np.array([x0 + sgn * h])))
+            b.engforce()
+            d = b.forces() - fref
+            L.append(float(np.sum(d * d)))
+        fd = (L[0] - L[1]) / (2 * h)
+        print(f"  adjoint {grad:.10e} fd {fd:.10e}", flush=True)
+        assert abs(grad - fd) <= 1e-6 * abs(fd)
