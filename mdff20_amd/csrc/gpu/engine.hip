@@ -569,13 +569,18 @@ static int launch_fwd_mfma(mdff_ctx *c, int polar, int lo, int C, const double *
     return launch_fwd_gemm(c, polar, lo, C, q_w, mu_w, active, nchunk_out);
 }
 
+// whether the back-projection GEMM takes a k box with CH moments: its
+// column tiles (MB_CT) and the nx planes of its staged ex table (GB_KX1_MAX)
+static bool g_back_fits(const mdff_ctx *c, int CH)
+{
+    return (c->Kx + 1) * CH * 2 <= MB_CT * 16 && c->Kx + 1 <= GB_KX1_MAX;
+}
+
 // back-projection on the matrix cores; returns 0 when outside the kernel's shapes
 static int launch_back_mfma(mdff_ctx *c, int polar, int lo, int NCH, const int *active, int *nsplit_out,
                             bool wm_built = false, T2Rows *t2r = nullptr)
 {
-    int CH = NCH == 9 ? 6 : 3;
-    int NC = (c->Kx + 1) * CH * 2;
-    if (!c->use_mfma || NC > MB_CT * 16) return 0;
+    if (!c->use_mfma || !g_back_fits(c, NCH == 9 ? 6 : 3)) return 0;
     return launch_back_gemm(c, polar, lo, NCH, active, nsplit_out, wm_built, t2r);
 }
 
@@ -934,7 +939,7 @@ static int launch_back_gemm(mdff_ctx *c, int polar, int lo, int NCH, const int *
     const int CH = NCH == 9 ? 6 : 3;
     const int NC = (c->Kx + 1) * CH * 2;
     const int CT = (NC + 15) / 16;
-    if (CT > MB_CT) return 0;
+    if (CT > MB_CT || !g_back_fits(c, CH)) return 0;
     const int mode = lo ? c->mf_lo : KM_F64;
     // stored YZt in the FP64 mode only (the others form it from the tables)
     if (mode == KM_F64 && ensure_yz(c, mode, polar, lo)) return 0;
@@ -1042,7 +1047,7 @@ static int rec_part(mdff_ctx *c, int polar, int lo, int C, int NCH, const double
     // split FP16 with a back-projection: the reduction sums the forward's
     // partials itself (k_psum_reduce_cm)
     const bool cm_path = (do_ef || do_f) && c->use_mfma && bmode == KM_F16X3 &&
-                         (c->Kx + 1) * CHb * 2 <= MB_CT * 16 && wm_alloc(c, bmode) == MDFF_OK;
+                         g_back_fits(c, CHb) && wm_alloc(c, bmode) == MDFF_OK;
     c->psum_defer = cm_path && !c->no_fuse && (C == 1 || C == 3 || C == 4);
     c->psum_ch = CHb;
     const bool fwd_mf = launch_fwd_mfma(c, polar, lo, C, q_w, mu_w, active, &nchunk);
@@ -1085,7 +1090,7 @@ static int rec_part(mdff_ctx *c, int polar, int lo, int C, int NCH, const double
                            c->Ky, c->Kz, CHb, c->W, c->gcm, w0, w1, c->gsc + 16, active);
         wm_built = true;
     } else if ((do_ef || do_f) && gemm_fwd && c->use_mfma && bmode != KM_F16X3 &&
-        (c->Kx + 1) * CHb * 2 <= MB_CT * 16 && wm_alloc(c, bmode) == MDFF_OK) {
+        g_back_fits(c, CHb) && wm_alloc(c, bmode) == MDFF_OK) {
         void *w0 = c->wm[bmode][0], *w1 = c->wm[bmode][1];
         const dim3 gr(cdiv(c->nkf, 256));
         if (bmode == KM_F64) hipLaunchKernelGGL(k_sk_reduce_wm<KM_F64>, gr, dim3(256), 0, c->st, r, CHb, c->Pk, w0, w1);

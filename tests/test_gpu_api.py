@@ -411,3 +411,35 @@ def test_nonsymmetric_polarisability():
         assert tm["u_tot"] == pytest.approx(tf["u_tot"], rel=1e-6)
         cm = em.counters()
         assert cm["scf_fp64"] > 0 and cm["scf_inner"] == 0
+
+
+def test_kbox_beyond_the_matrix_core_tiles():
+    """kES = (18, 8, 8): the force call's forward sum (4 channels, 2 x 4 x 19
+    = 152 rows) is wider than the matrix-core tiles take (144) and every
+    back-projection has more nx planes (19) than its staged ex table holds
+    (17), so those run on the vector ALU while the SCF's forward sums (3
+    channels, 114 rows) stay on the matrix cores: FP64 equals the oracle
+    (1e-10, the same SCF count) and the mixed form stays within 1e-5 of
+    FP64"""
+    import ctypes as C
+    p, cfg, _ = load_case("pim192_static")
+    q = m.Params()
+    C.pointer(q)[0] = p
+    q.lautoES = 0
+    q.kES[0], q.kES[1], q.kES[2] = 18, 8, 8
+    o = Oracle(q, cfg.typia, cfg.A, cfg.r)
+    to = o.engforce(0)
+    with m.Engine(q, cfg.typia, cfg.A, cfg.r) as ef, \
+            m.Engine(q, cfg.typia, cfg.A, cfg.r, precision=m.MIXED) as em:
+        tf, tm = ef.engforce(0), em.engforce(0)
+        print(f"[gpu] k box 18x8x8: FP64 vs oracle forces {rel(ef.forces(), o.forces()):.2e}, "
+              f"mixed vs FP64 {rel(em.forces(), ef.forces()):.2e}", flush=True)
+        assert tf["status"] == 0 and tm["status"] == 0
+        assert tf["n_scf"] == to["n_scf"]
+        assert rel(ef.forces(), o.forces()) < 1e-10
+        assert rel(ef.dipoles(), o.dipoles()) < 1e-10
+        assert tf["u_tot"] == pytest.approx(to["u_tot"], rel=1e-10)
+        assert rel(np.array(tf["tau_coul"]), np.array(to["tau_coul"])) < 1e-10
+        assert rel(em.forces(), ef.forces()) < 1e-5
+        assert rel(em.dipoles(), ef.dipoles()) < 1e-5
+        assert tm["u_tot"] == pytest.approx(tf["u_tot"], rel=1e-6)
