@@ -443,3 +443,38 @@ def test_kbox_beyond_the_matrix_core_tiles():
         assert rel(em.forces(), ef.forces()) < 1e-5
         assert rel(em.dipoles(), ef.dipoles()) < 1e-5
         assert tm["u_tot"] == pytest.approx(tf["u_tot"], rel=1e-6)
+
+
+def _sheared(cfg, shear=((1, 0, 0.15), (2, 0, 0.10), (2, 1, 0.05))):
+    """the fixture's ions at the same fractional coordinates in a triclinic
+    cell: lattice vector i gains shear x L along axis j"""
+    A = np.array(cfg.A, dtype=np.float64)
+    L = A[0][0]
+    s = np.linalg.solve(A.T, np.asarray(cfg.r))
+    A2 = A.copy()
+    for i, j, f in shear:
+        A2[i][j] += f * L
+    return A2, A2.T @ s
+
+
+def test_triclinic_cell():
+    """a triclinic cell (reciprocal basis with off-diagonal terms in every
+    k-vector, kardir/dirkar cell.c:125-161): FP64 equals the oracle (1e-10,
+    the same SCF count) and the mixed form stays within 1e-5 of FP64"""
+    p, cfg, _ = load_case("pim192_static")
+    A2, r2 = _sheared(cfg)
+    o = Oracle(p, cfg.typia, A2, r2)
+    to = o.engforce(0)
+    with m.Engine(p, cfg.typia, A2, r2) as ef, m.Engine(p, cfg.typia, A2, r2, precision=m.MIXED) as em:
+        tf, tm = ef.engforce(0), em.engforce(0)
+        print(f"[gpu] triclinic: FP64 vs oracle forces {rel(ef.forces(), o.forces()):.2e}, "
+              f"mixed vs FP64 {rel(em.forces(), ef.forces()):.2e}", flush=True)
+        assert tf["status"] == 0 and tm["status"] == 0 and to["status"] == 0
+        assert tf["n_scf"] == to["n_scf"]
+        assert rel(ef.forces(), o.forces()) < 1e-10
+        assert rel(ef.dipoles(), o.dipoles()) < 1e-10
+        assert tf["u_tot"] == pytest.approx(to["u_tot"], rel=1e-10)
+        assert rel(np.array(tf["tau_coul"]), np.array(to["tau_coul"])) < 1e-10
+        assert rel(em.forces(), ef.forces()) < 1e-5
+        assert rel(em.dipoles(), ef.dipoles()) < 1e-5
+        assert tm["u_tot"] == pytest.approx(tf["u_tot"], rel=1e-6)

@@ -353,3 +353,36 @@ def test_batch_vector_alu_kspace():
         fd = (L[0] - L[1]) / (2 * h)
         print(f"  adjoint {grad:.10e} fd {fd:.10e}", flush=True)
         assert abs(grad - fd) <= 1e-6 * abs(fd)
+
+
+def test_batch_triclinic_configuration():
+    """a triclinic configuration beside a cubic one (each with its own
+    reciprocal basis in the matrix-core back-projection's epilogue): FP64
+    against the oracle per configuration (1e-10, the same SCF counts) and
+    the mixed batch within 1e-5"""
+    from oracle.oracle import Oracle
+    p, confs = _configs()
+    (t, A, r), e = confs[0]
+    A = np.asarray(A, dtype=np.float64)
+    s = np.linalg.solve(A.T, np.asarray(r))
+    A2 = A.copy()
+    A2[1][0] += 0.15 * A[0][0]
+    A2[2][0] += 0.10 * A[0][0]
+    A2[2][1] += 0.05 * A[0][0]
+    confs = [confs[1], ((t, A2, A2.T @ s), e)]
+    print("[gpu] batch triclinic", flush=True)
+    with m.batch.Batch(p, [c[0] for c in confs], [c[1] for c in confs], precision=m.FP64) as b, \
+            m.batch.Batch(p, [c[0] for c in confs], [c[1] for c in confs], precision=m.MIXED) as bm:
+        th, tm = b.engforce(), bm.engforce()
+        F, Fm = b.split(b.forces()), bm.split(bm.forces())
+        for c, ((tt, AA, rr), ee) in enumerate(confs):
+            o = Oracle(_oracle_params(p, ee), tt, AA, rr)
+            to = o.engforce(0)
+            for k in KEYS:
+                assert th[c][k] == pytest.approx(to[k], rel=1e-10, abs=1e-12), (c, k, th[c][k], to[k])
+            assert th[c]["n_scf"] == to["n_scf"], c
+            assert rel(F[c], o.forces()) < 1e-10, c
+            assert rel(np.array(th[c]["tau_coul"]), np.array(to["tau_coul"])) < 1e-10, c
+            o.close()
+            frms = np.sqrt(np.mean((Fm[c] - F[c]) ** 2)) / np.sqrt(np.mean(F[c] ** 2))
+            assert frms < 1e-5, (c, frms)
