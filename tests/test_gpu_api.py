@@ -5,7 +5,7 @@ import numpy as np
 import pytest
 
 import mdff20_amd as m
-from conftest import load_case, rel
+from conftest import load_case, rel, scalars_of
 
 pytestmark = pytest.mark.gpu
 Oracle = pytest.importorskip("oracle.oracle").Oracle
@@ -478,3 +478,46 @@ def test_triclinic_cell():
         assert rel(em.forces(), ef.forces()) < 1e-5
         assert rel(em.dipoles(), ef.dipoles()) < 1e-5
         assert tm["u_tot"] == pytest.approx(tf["u_tot"], rel=1e-6)
+
+
+def test_neighbour_list_grows_on_overflow():
+    """ions packed into a fifth of a long box: the list capacity estimated
+    from the mean density (1.3 x the mean count + 64, rounded up to 64) is
+    too small, so the first build overflows and the rows grow (build_list);
+    three NVT steps then equal the oracle's (1e-10, the same SCF counts and
+    rebuilds)"""
+    p, cfg, _ = load_case("bsio_200_nvt")
+    A = np.asarray(cfg.A, dtype=np.float64)
+    r = np.asarray(cfg.r)
+    typ = np.asarray(cfg.typia)
+    rs, ts = [], []
+    for ix in range(2):
+        for iy in range(2):
+            for iz in range(2):
+                rs.append(r + (ix * A[0] + iy * A[1] + iz * A[2])[:, None])
+                ts.append(typ)
+    rr, tt = np.concatenate(rs, axis=1), np.concatenate(ts)
+    order = np.argsort(tt, kind="stable")
+    rr, tt = rr[:, order], tt[order].astype(np.int32)
+    A2 = 2.0 * A
+    A2[0] *= 5.0
+    o = Oracle(p, tt, A2, rr)
+    rc = max(p.cutshortrange, p.cutlongrange) + p.skindiff
+    est = 4.0 / 3.0 * np.pi * rc ** 3 * len(tt) / abs(np.linalg.det(A2))
+    cap0 = (int(est * 1.3) + 64 + 63) // 64 * 64
+    with m.Engine(p, tt, A2, rr) as e:
+        e.init_velocities()
+        o.init_velocities()
+        assert rel(e.velocities(), o.velocities()) == 0.0
+        for s in range(4):
+            t = e.md_init() if s == 0 else e.md_step(s)
+            to = o.md_init() if s == 0 else o.md_step(s)
+            if s == 0:
+                cap1 = e.list_info()["cap"]
+                print(f"[gpu] list capacity {cap0} -> {cap1}", flush=True)
+                assert cap1 > cap0
+            assert t["n_scf"] == to["n_scf"], s
+            assert t["verlet_updates"] == to["verlet_updates"], s
+            assert rel(scalars_of(t), scalars_of(to), 1e-12) < 1e-10, s
+            assert rel(e.forces(), o.forces()) < 1e-10, s
+    o.close()
