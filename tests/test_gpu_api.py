@@ -521,3 +521,28 @@ def test_neighbour_list_grows_on_overflow():
             assert rel(scalars_of(t), scalars_of(to), 1e-12) < 1e-10, s
             assert rel(e.forces(), o.forces()) < 1e-10, s
     o.close()
+
+
+@pytest.mark.parametrize("case", ["lj_fcc_static", "sio2_102_ko_static", "melt216_bhmftd_nve", "melt216_dip_nve",
+                                  "melt216_quad_static"])
+def test_mixed_form_other_models(case):
+    """the mixed form on the models the accuracy tests above leave out:
+    NMLJ pairs (lj_fcc: FP64 in both forms, so equal), scfkO (FP64 SCF,
+    low-precision static field),
+    undamped BHMFTD, static dipoles without polarisation, static
+    quadrupoles: forces within 1e-5 (RMS, relative) of the FP64 form,
+    energy within 1e-6"""
+    print(f"[gpu] mixed other models {case}", flush=True)
+    p, cfg, _ = load_case(case)
+    with m.Engine(p, cfg.typia, cfg.A, cfg.r, precision=m.FP64) as ef, \
+            m.Engine(p, cfg.typia, cfg.A, cfg.r, precision=m.MIXED) as em:
+        tf, tm = ef.engforce(0), em.engforce(0)
+        assert tf["status"] == 0 and tm["status"] == 0
+        Ff, Fm = ef.forces(), em.forces()
+        frms = np.sqrt(np.mean((Fm - Ff) ** 2)) / np.sqrt(np.mean(Ff ** 2))
+        print(f"  forces {frms:.2e}, energy {abs(tm['u_tot'] - tf['u_tot']) / abs(tf['u_tot']):.2e}, "
+              f"n_scf {tf['n_scf']} / {tm['n_scf']}", flush=True)
+        assert frms < 1e-5, frms
+        assert tm["u_tot"] == pytest.approx(tf["u_tot"], rel=1e-6)
+        if tf["u_pol"]:
+            assert rel(em.dipoles(), ef.dipoles()) < 1e-5
