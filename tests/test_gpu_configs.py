@@ -485,3 +485,32 @@ def test_cfg5_50_adam_steps(cfg5):
         d0 = np.linalg.norm(spec.values() - spec.values(target))
         d1 = np.linalg.norm(th - spec.values(target))
         assert d1 < d0
+
+
+@pytest.mark.gpu
+def test_four_times_the_melt():
+    """49152 ions (four times the 12288-ion melt at the same density and k
+    box: 32768 polarisable ions in the SCF's sums): the mixed form against
+    the FP64 form at this size, force RMS <= 1e-5 and dipoles <= 1e-5, and
+    the FP64 form's anchored CG reaching the Jacobi schedule's dipoles
+    within the SCF tolerance's reach (1e-5)"""
+    L, pos, typ = bench.system(16384, 7)
+    r = bench.equilibrate(L, pos, typ, 60)
+    p = bench.params_of(bench.control(True))
+    A = np.diag([L, L, L])
+    print(f"[gpu] 49152 ions, L = {L:.2f}", flush=True)
+    with m.Engine(p, typ, A, r, precision=m.FP64) as ef, m.Engine(p, typ, A, r, precision=m.MIXED) as em:
+        tf, tm = ef.engforce(0), em.engforce(0)
+        assert tf["status"] == 0 and tm["status"] == 0
+        Ff, Fm = ef.forces(), em.forces()
+        frms = np.sqrt(np.mean((Fm - Ff) ** 2)) / np.sqrt(np.mean(Ff ** 2))
+        print(f"  mixed vs FP64: forces {frms:.2e}, dipoles {rel(em.dipoles(), ef.dipoles()):.2e}, "
+              f"SCF {tf['n_scf']} / {tm['n_scf']}", flush=True)
+        assert frms < 1e-5
+        assert rel(em.dipoles(), ef.dipoles()) < 1e-5
+        assert tm["u_tot"] == pytest.approx(tf["u_tot"], rel=1e-6)
+        mu_j = ef.dipoles()
+        ef.set_scf_solver(2, 0)
+        tc = ef.engforce(0)
+        assert tc["status"] == 0
+        assert rel(ef.dipoles(), mu_j) < 1e-5
