@@ -546,3 +546,34 @@ def test_mixed_form_other_models(case):
         assert tm["u_tot"] == pytest.approx(tf["u_tot"], rel=1e-6)
         if tf["u_pol"]:
             assert rel(em.dipoles(), ef.dipoles()) < 1e-5
+
+
+@pytest.mark.parametrize("chain", [2, 5, 10])
+def test_nhc_chain_lengths(chain):
+    """Nose-Hoover chains of other lengths than the fixtures' 3 (the chain
+    kernel is instantiated for up to 4, 8 and 16 thermostats): 20 NVT steps
+    equal the oracle's chain_nhcn (integration.c:189-294), 1e-10 on every
+    thermo scalar, positions and velocities; chains shorter than 2 (the
+    reference's sweep reads thermostat nhc_n - 2, integration.c:251) or
+    longer than 16 are rejected at creation"""
+    import ctypes as C
+    p, cfg, _ = load_case("melt216_nvt")
+    q = m.Params()
+    C.pointer(q)[0] = p
+    for bad in (1, 17):
+        q.nhc_n = bad
+        with pytest.raises(m.MdffError):
+            m.Engine(q, cfg.typia, cfg.A, cfg.r)
+    q.nhc_n = chain
+    o = Oracle(q, cfg.typia, cfg.A, cfg.r)
+    with m.Engine(q, cfg.typia, cfg.A, cfg.r) as e:
+        e.init_velocities()
+        o.init_velocities()
+        for s in range(21):
+            t = e.md_init() if s == 0 else e.md_step(s)
+            to = o.md_init() if s == 0 else o.md_step(s)
+            assert rel(scalars_of(t), scalars_of(to), 1e-12) < 1e-10, (chain, s)
+        print(f"[gpu] NHC-{chain}: e_nvt {t['e_nvt']:.10e} / {to['e_nvt']:.10e}", flush=True)
+        assert rel(e.positions(), o.positions()) < 1e-10
+        assert rel(e.velocities(), o.velocities()) < 1e-10
+    o.close()
