@@ -577,3 +577,29 @@ def test_nhc_chain_lengths(chain):
         assert rel(e.positions(), o.positions()) < 1e-10
         assert rel(e.velocities(), o.velocities()) < 1e-10
     o.close()
+
+
+@pytest.mark.parametrize("order,mults", [(1, 1), (3, 2), (5, 1), (7, 3), (9, 1)])
+def test_nhc_yoshida_orders(order, mults):
+    """the Yoshida-Suzuki weights of chain_nhcn (integration.c:205-236:
+    orders 1, 3, 5, 7, and 9, whose weights the reference leaves at zero)
+    with nhc_mults sub-steps: 20 NVT steps equal the oracle, 1e-10 on every
+    thermo scalar, positions and velocities"""
+    import ctypes as C
+    p, cfg, _ = load_case("melt216_nvt")
+    q = m.Params()
+    C.pointer(q)[0] = p
+    q.nhc_yosh_order = order
+    q.nhc_mults = mults
+    o = Oracle(q, cfg.typia, cfg.A, cfg.r)
+    with m.Engine(q, cfg.typia, cfg.A, cfg.r) as e:
+        e.init_velocities()
+        o.init_velocities()
+        for s in range(21):
+            t = e.md_init() if s == 0 else e.md_step(s)
+            to = o.md_init() if s == 0 else o.md_step(s)
+            assert rel(scalars_of(t), scalars_of(to), 1e-12) < 1e-10, (order, mults, s)
+        print(f"[gpu] Yoshida {order} x {mults}: e_nvt {t['e_nvt']:.10e} / {to['e_nvt']:.10e}", flush=True)
+        assert rel(e.positions(), o.positions()) < 1e-10
+        assert rel(e.velocities(), o.velocities()) < 1e-10
+    o.close()
