@@ -603,3 +603,31 @@ def test_nhc_yoshida_orders(order, mults):
         assert rel(e.positions(), o.positions()) < 1e-10
         assert rel(e.velocities(), o.velocities()) < 1e-10
     o.close()
+
+
+@pytest.mark.parametrize("order", [0, 1, 3, 4, 5])
+def test_aspc_orders_md(order):
+    """the ASPC predictor-corrector (pim.c:314-424) at the orders the
+    fixtures leave out (they use 2), with its ramp over the first steps and
+    the history shift: 12 PIM NVT steps equal the oracle, the same SCF
+    count every step, 1e-10 on every thermo scalar and the dipoles"""
+    import ctypes as C
+    p, cfg, _ = load_case("pim192_nvt")
+    q = m.Params()
+    C.pointer(q)[0] = p
+    q.extrapolate_order = order
+    o = Oracle(q, cfg.typia, cfg.A, cfg.r)
+    with m.Engine(q, cfg.typia, cfg.A, cfg.r) as e:
+        e.init_velocities()
+        o.init_velocities()
+        ns = []
+        for s in range(13):
+            t = e.md_init() if s == 0 else e.md_step(s)
+            to = o.md_init() if s == 0 else o.md_step(s)
+            ns.append(t["n_scf"])
+            assert t["n_scf"] == to["n_scf"], (order, s, t["n_scf"], to["n_scf"])
+            assert rel(scalars_of(t), scalars_of(to), 1e-12) < 1e-10, (order, s)
+        print(f"[gpu] ASPC order {order}: SCF iterations {ns}", flush=True)
+        assert rel(e.dipoles(), o.dipoles()) < 1e-10
+        assert rel(e.positions(), o.positions()) < 1e-10
+    o.close()
