@@ -631,3 +631,27 @@ def test_aspc_orders_md(order):
         assert rel(e.dipoles(), o.dipoles()) < 1e-10
         assert rel(e.positions(), o.positions()) < 1e-10
     o.close()
+
+
+@pytest.mark.parametrize("precision", [m.FP64, m.MIXED])
+def test_ion_order_invariance(precision):
+    """the ions in a random order (types interleaved instead of in blocks):
+    energies, the SCF count and the per-ion forces and dipoles (permuted
+    back) equal those of the fixture's order, to 1e-10 in FP64 and 1e-5 in
+    the mixed form (whose sums change their order)"""
+    p, cfg, _ = load_case("bsio_200_nvt")
+    typ = np.asarray(cfg.typia, dtype=np.int32)
+    r = np.asarray(cfg.r)
+    perm = np.random.default_rng(11).permutation(len(typ))
+    tol = 1e-10 if precision == m.FP64 else 1e-5
+    with m.Engine(p, typ, cfg.A, r, precision=precision) as e0, \
+            m.Engine(p, typ[perm].copy(), cfg.A, r[:, perm].copy(), precision=precision) as e1:
+        t0, t1 = e0.engforce(0), e1.engforce(0)
+        print(f"[gpu] ion order ({'FP64' if precision == m.FP64 else 'mixed'}): forces "
+              f"{rel(e1.forces(), e0.forces()[:, perm]):.2e}, SCF {t0['n_scf']} / {t1['n_scf']}", flush=True)
+        assert t0["status"] == 0 and t1["status"] == 0
+        if precision == m.FP64:
+            assert t1["n_scf"] == t0["n_scf"]
+        assert t1["u_tot"] == pytest.approx(t0["u_tot"], rel=tol)
+        assert rel(e1.forces(), e0.forces()[:, perm]) < tol
+        assert rel(e1.dipoles(), e0.dipoles()[perm]) < tol
