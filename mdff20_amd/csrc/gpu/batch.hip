@@ -126,6 +126,7 @@ struct mdff_batch {
 };
 
 static int lo_b(const mdff_batch *b) { return b->precision == MDFF_MIXED; }
+static void cg_flags(mdff_batch *b);
 
 // ------------------------------------------------------------ work items
 static int build_items(mdff_batch *b)
@@ -834,19 +835,10 @@ int mdff_batch_create(mdff_batch **out, const mdff_params *p, int nconf, const i
         BALLOC(b->boxk, b->nbox); BALLOC(b->kcfg, b->nktot); BALLOC(b->Sk, 8 * b->nktot);
         BALLOC(b->aw, b->naw); BALLOC(b->wm, b->nwm);
     }
-    // conjugate gradients (the mixed form's SCF, the FP64 form's adjoint
-    // solve) need symmetric polarisability tensors (a symmetric system); the
-    // SCF's also the matrix-core sums
-    b->cg_sym = b->lpol;
-    for (int t = 0; t < b->ntype && b->cg_sym; t++)
-        if (b->tab.polar[t])
-            for (int i = 0; i < 3; i++)
-                for (int j = 0; j < 3; j++)
-                    if (fabs(b->tab.pol[t][3 * i + j] - b->tab.pol[t][3 * j + i]) >
-                        1e-12 * (fabs(b->tab.pol[t][3 * i + j]) + fabs(b->tab.pol[t][3 * j + i])))
-                        b->cg_sym = 0;
-    b->cg = precision == MDFF_MIXED && b->mf && b->cg_sym;
-    if (b->cg_sym) {
+    // conjugate-gradient buffers (used while the polarisabilities allow it,
+    // cg_flags)
+    cg_flags(b);
+    if (b->lpol) {
         BALLOC(b->cg_r, n3); BALLOC(b->cg_z, n3); BALLOC(b->cg_p, n3); BALLOC(b->cg_w, n3); BALLOC(b->cg_q, n3);
         BALLOC(b->cg_e, n3);
         BALLOC(b->cg_part, 2 * (size_t)n); BALLOC(b->cg_s, 2 * (size_t)nconf); BALLOC(b->cg_pq, nconf);
@@ -912,6 +904,28 @@ int mdff_batch_set_positions(mdff_batch *b, const double *rx, const double *ry, 
     return MDFF_OK;
 }
 
+// Conjugate gradients (the mixed form's SCF, the FP64 form's adjoint solve)
+// need every polarisability tensor symmetric positive definite (then the
+// system pol^-T - T of a converging SCF is too); the SCF's also the
+// matrix-core sums.  Otherwise both run their Jacobi sequences.
+static void cg_flags(mdff_batch *b)
+{
+    bool spd = b->lpol;
+    for (int t = 0; t < b->ntype && spd; t++) {
+        if (!b->tab.polar[t]) continue;
+        const double *m = b->tab.pol[t];
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++)
+                if (fabs(m[3 * i + j] - m[3 * j + i]) > 1e-12 * (fabs(m[3 * i + j]) + fabs(m[3 * j + i]))) spd = false;
+        const double m2 = m[0] * m[4] - m[1] * m[3];
+        const double m3 = m[0] * (m[4] * m[8] - m[5] * m[7]) - m[1] * (m[3] * m[8] - m[5] * m[6]) +
+                          m[2] * (m[3] * m[7] - m[4] * m[6]);
+        if (!(m[0] > 0.0 && m2 > 0.0 && m3 > 0.0)) spd = false;
+    }
+    b->cg_sym = spd ? 1 : 0;
+    b->cg = (b->precision == MDFF_MIXED && b->mf && spd) ? 1 : 0;
+}
+
 int mdff_batch_set_params(mdff_batch *b, const mdff_params *p)
 {
     if (!b || !p) return MDFF_ERR_ARG;
@@ -930,6 +944,7 @@ int mdff_batch_set_params(mdff_batch *b, const mdff_params *p)
     b->p.cutshortrange = keep.cutshortrange;
     typetab_build(b->p, b->ntype, b->p.cutshortrange * b->p.cutshortrange, b->tab);
     BCHK(hipMemcpy(b->tab_d, &b->tab, sizeof(TypeTab), hipMemcpyHostToDevice));
+    cg_flags(b);
     return MDFF_OK;
 }
 

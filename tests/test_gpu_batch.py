@@ -386,3 +386,35 @@ def test_batch_triclinic_configuration():
             o.close()
             frms = np.sqrt(np.mean((Fm[c] - F[c]) ** 2)) / np.sqrt(np.mean(F[c] ** 2))
             assert frms < 1e-5, (c, frms)
+
+
+def test_set_params_switches_the_scf_solver():
+    """set_params from symmetric to non-symmetric polarisabilities and back:
+    the solver follows the tensors (conjugate gradients only while every
+    polarisability is symmetric positive definite), so a batch whose
+    parameters were changed gives the forces of a batch created with them
+    (mixed form, both from zero dipoles: 1e-12 relative)"""
+    p, confs = _configs()
+    confs = confs[:2]
+    q = m.Params()
+    C.pointer(q)[0] = p
+    q.conv_tol_ind = 1e-10
+    q.max_scf_pim_iter = 400
+    ns = m.Params()
+    C.pointer(ns)[0] = q
+    ns.polit[0][0][1] = 0.1 * ns.polit[0][0][0]
+    print("[gpu] set_params symmetric <-> non-symmetric", flush=True)
+    args = ([c[0] for c in confs], [c[1] for c in confs])
+    with m.batch.Batch(q, *args, precision=m.MIXED) as b, m.batch.Batch(ns, *args, precision=m.MIXED) as bn:
+        b.engforce()
+        b.set_params(ns)
+        b.reset_dipoles()
+        b.engforce()
+        bn.engforce()
+        assert rel(b.forces(), bn.forces()) < 1e-12
+        b.set_params(q)
+        b.reset_dipoles()
+        b.engforce()
+        with m.batch.Batch(q, *args, precision=m.MIXED) as bq:
+            bq.engforce()
+            assert rel(b.forces(), bq.forces()) < 1e-12
