@@ -78,8 +78,12 @@ int mdff_batch_create(mdff_batch **out, const mdff_params *p, int nconf, const i
 void mdff_batch_destroy(mdff_batch *b);
 
 int mdff_batch_set_positions(mdff_batch *b, const double *rx, const double *ry, const double *rz);
-/* new force-field parameters (same types, charges and polarisable set);
- * the SCF of the next call starts from the last converged dipoles */
+/* new force-field parameters (same types, charges and polarisable set:
+ * MDFF_ERR_CONFIG when they differ, or when no ion would stay polarisable
+ * or one would become so); damping, pair potentials and the SCF solver
+ * (conjugate gradients only for symmetric positive definite
+ * polarisabilities) follow the new values.  The SCF of the next call
+ * starts from the last converged dipoles. */
 int mdff_batch_set_params(mdff_batch *b, const mdff_params *p);
 /* zero the dipoles the next SCF starts from (the state of a fresh batch) */
 int mdff_batch_reset_dipoles(mdff_batch *b);

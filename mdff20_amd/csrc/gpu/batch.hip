@@ -935,8 +935,24 @@ int mdff_batch_set_params(mdff_batch *b, const mdff_params *p)
         for (int k = 0; k < 3; k++)
             if (p->dipit[t][k] != 0.0) return MDFF_ERR_UNSUPPORTED;
     }
+    // the terms that exist follow the new parameters (damping, pair
+    // potentials); whether any ion is polarisable may not change (it sizes
+    // the SCF's buffers)
+    int pol = 0, dmp = 0;
+    for (int t = 0; t < b->ntype; t++) {
+        for (int j = 0; j < 3; j++)
+            for (int k = 0; k < 3; k++)
+                if (p->polit[t][j][k] != 0.0) pol = 1;
+        for (int i = 0; i < b->ntype; i++)
+            for (int k = 0; k < b->ntype; k++)
+                if (p->lpoldamping[t][i][k]) dmp = 1;
+    }
+    if (!p->lcoulombic) pol = 0;
+    if (pol != b->lpol) return MDFF_ERR_CONFIG;
     mdff_params keep = b->p;
     b->p = *p;
+    b->ldmp = dmp;
+    b->lnonbonded = p->lnmlj || p->lbhmft || p->lbhmftd;
     // Ewald settings and cutoffs stay those of creation
     b->p.alphaES = keep.alphaES;
     memcpy(b->p.kES, keep.kES, sizeof(keep.kES));

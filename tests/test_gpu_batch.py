@@ -418,3 +418,32 @@ def test_set_params_switches_the_scf_solver():
         with m.batch.Batch(q, *args, precision=m.MIXED) as bq:
             bq.engforce()
             assert rel(b.forces(), bq.forces()) < 1e-12
+
+
+def test_set_params_switches_damping():
+    """set_params that turns the Tang-Toennies damping of the charge-dipole
+    terms off and on again: the batch gives the forces of a batch created
+    with those parameters (FP64, both from zero dipoles: 1e-12)"""
+    p, confs = _configs()
+    confs = confs[:2]
+    nd = m.Params()
+    C.pointer(nd)[0] = p
+    for t in range(nd.ntype):
+        for i in range(nd.ntype):
+            for k in range(nd.ntype):
+                nd.lpoldamping[t][i][k] = 0
+    print("[gpu] set_params damping off / on", flush=True)
+    args = ([c[0] for c in confs], [c[1] for c in confs])
+    with m.batch.Batch(p, *args) as b, m.batch.Batch(nd, *args) as bn, m.batch.Batch(p, *args) as bp:
+        b.engforce()
+        b.set_params(nd)
+        b.reset_dipoles()
+        b.engforce()
+        bn.engforce()
+        bp.engforce()
+        assert rel(b.forces(), bn.forces()) < 1e-12
+        assert rel(bn.forces(), bp.forces()) > 1e-8   # the damping matters here
+        b.set_params(p)
+        b.reset_dipoles()
+        b.engforce()
+        assert rel(b.forces(), bp.forces()) < 1e-12
