@@ -355,14 +355,17 @@ def test_batch_vector_alu_kspace():
         assert abs(grad - fd) <= 1e-6 * abs(fd)
 
 
-def test_batch_triclinic_configuration():
+@pytest.mark.parametrize("kx", [6, 16])
+def test_batch_triclinic_configuration(kx):
     """a triclinic configuration beside a cubic one (each with its own
-    reciprocal basis in the matrix-core back-projection's epilogue): FP64
-    against the oracle per configuration (1e-10, the same SCF counts) and
-    the mixed batch within 1e-5"""
+    reciprocal basis in the back-projection's epilogue; kES 16 in x sends
+    the batch to the vector-ALU k-space sums): FP64 against the oracle per
+    configuration (1e-10, the same SCF counts) and the mixed batch within
+    1e-5"""
     from oracle.oracle import Oracle
     p, confs = _configs()
     (t, A, r), e = confs[0]
+    e = dict(e, kES=(kx, 6, 6))
     A = np.asarray(A, dtype=np.float64)
     s = np.linalg.solve(A.T, np.asarray(r))
     A2 = A.copy()
