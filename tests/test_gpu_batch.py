@@ -450,3 +450,43 @@ def test_set_params_switches_damping():
         b.reset_dipoles()
         b.engforce()
         assert rel(b.forces(), bp.forces()) < 1e-12
+
+
+def test_adjoint_triclinic():
+    """the adjoint of a batch holding a triclinic configuration (its real-
+    space, reciprocal-space and damping terms through the configuration's
+    own cell): the gradients of A, pol and the damping b against central
+    finite differences, as test_adjoint_matches_finite_differences"""
+    p, confs = _configs()
+    (t, A, r), e = confs[0]
+    A = np.asarray(A, dtype=np.float64)
+    s = np.linalg.solve(A.T, np.asarray(r))
+    A2 = A.copy()
+    A2[1][0] += 0.15 * A[0][0]
+    A2[2][1] += 0.05 * A[0][0]
+    confs = [confs[1], ((t, A2, A2.T @ s), e)]
+    q = _tight(p)
+    print("[gpu] adjoint, triclinic configuration", flush=True)
+    entries = [("A", 0, 1), ("pol", 0), ("damp_b", 0, 1)]
+    with m.batch.Batch(q, [c[0] for c in confs], [c[1] for c in confs]) as b:
+        b.set_params(_perturbed(m.batch.ParamSpec(q, entries), 5))
+        b.engforce()
+        fref = b.forces()
+        b.set_params(q)
+        g = b.adjoint(fref)
+        spec = m.batch.ParamSpec(q, entries, log=False)
+        th0 = spec.values()
+        grad = spec.gather(g)
+        for k, en in enumerate(entries):
+            h = 1e-6 * abs(th0[k])
+            L = []
+            for sgn in (1, -1):
+                th = th0.copy()
+                th[k] += sgn * h
+                b.set_params(spec.params(th))
+                b.engforce()
+                d = b.forces() - fref
+                L.append(float(np.sum(d * d)))
+            fd = (L[0] - L[1]) / (2 * h)
+            print(f"  {en}: adjoint {grad[k]:.10e} fd {fd:.10e}", flush=True)
+            assert abs(grad[k] - fd) <= 1e-6 * max(abs(fd), 1e-8 * abs(g["loss"]) / max(abs(th0[k]), 1e-30)), en
