@@ -146,8 +146,8 @@ class Batch:
                False)
 
     def set_params(self, params):
-        self.params = params
         _check(self._L.mdff_batch_set_params(self._h, C.byref(params)), "batch_set_params", False)
+        self.params = params
 
     def reset_dipoles(self):
         _check(self._L.mdff_batch_reset_dipoles(self._h), "batch_reset_dipoles", False)

@@ -934,6 +934,8 @@ int mdff_batch_set_params(mdff_batch *b, const mdff_params *p)
         if (p->qit[t] != b->p.qit[t] || (p->lpolar[t] != 0) != (b->p.lpolar[t] != 0)) return MDFF_ERR_CONFIG;
         for (int k = 0; k < 3; k++)
             if (p->dipit[t][k] != 0.0) return MDFF_ERR_UNSUPPORTED;
+        // the SCF of mdff_batch_create's check: momentpolaSCF only
+        if (p->lcoulombic && p->lpolar[t] && p->algo_pim != MDFF_PIM_SCF) return MDFF_ERR_UNSUPPORTED;
     }
     // the terms that exist follow the new parameters (damping, pair
     // potentials); whether any ion is polarisable may not change (it sizes

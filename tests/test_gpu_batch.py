@@ -450,6 +450,12 @@ def test_set_params_switches_damping():
         b.reset_dipoles()
         b.engforce()
         assert rel(b.forces(), bp.forces()) < 1e-12
+        # what the batch does not run is refused, as at creation: scfkO
+        ko = m.Params()
+        C.pointer(ko)[0] = p
+        ko.algo_pim = m.PIM_SCFKO
+        with pytest.raises(m.MdffError):
+            b.set_params(ko)
 
 
 def test_adjoint_triclinic():
