@@ -1316,28 +1316,54 @@ __global__ void __launch_bounds__(512, ((MODE == KM_F64 || CTMAX >= 13) ? 2 : 4)
         __syncthreads();
         const int jl = round * RW * 16 + er;         // ion within the block
         double o[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-        for (int nx = es; nx < KX1; nx += TPI) {
-            const double xr = (double)exs[nx * EXLD + 2 * jl], xi = (double)exs[nx * EXLD + 2 * jl + 1];
-            const double fnx = (double)nx;
-            const et *rowp = tile + er * TLD + nx * a.CH * 2;
-            const double *cs = cscs + nx * a.CH;
-            auto z = [&](int m, double &zr, double &zi) {
-                const double tre = (double)rowp[2 * m] * cs[m], tim = (double)rowp[2 * m + 1] * cs[m];
-                zr = xr * tre - xi * tim;
-                zi = xr * tim + xi * tre;
-            };
-            double zr, zi;
-            z(0, zr, zi); o[0] += fnx * zi; o[3] += fnx * fnx * zr;
-            z(1, zr, zi); o[1] += zi; o[4] += fnx * zr;
-            z(2, zr, zi); o[2] += zi; o[5] += fnx * zr;
-            if (a.CH == 6) {
-                z(3, zr, zi); o[6] += zr;
-                z(4, zr, zi); o[7] += zr;
-                z(5, zr, zi); o[8] += zr;
+        // every nx plane of this thread (at most GB_KX1_MAX / TPI, rounded
+        // up), unrolled so that the LDS reads of all planes are issued first
+        constexpr int NXI = (GB_KX1_MAX + TPI - 1) / TPI;
+        if (a.CH == 3) {
+            // field only: Im z of the three moments
+#pragma unroll
+            for (int it = 0; it < NXI; it++) {
+                const int nx = es + it * TPI;
+                if (nx < KX1) {
+                    const double xr = (double)exs[nx * EXLD + 2 * jl], xi = (double)exs[nx * EXLD + 2 * jl + 1];
+                    const et *rowp = tile + er * TLD + nx * 6;
+                    const double *cs = cscs + nx * 3;
+                    double zi[3];
+#pragma unroll
+                    for (int m = 0; m < 3; m++)
+                        zi[m] = (xr * (double)rowp[2 * m + 1] + xi * (double)rowp[2 * m]) * cs[m];
+                    o[0] += (double)nx * zi[0];
+                    o[1] += zi[1];
+                    o[2] += zi[2];
+                }
+            }
+        } else {
+#pragma unroll
+            for (int it = 0; it < NXI; it++) {
+                const int nx = es + it * TPI;
+                if (nx < KX1) {
+                    const double xr = (double)exs[nx * EXLD + 2 * jl], xi = (double)exs[nx * EXLD + 2 * jl + 1];
+                    const double fnx = (double)nx;
+                    const et *rowp = tile + er * TLD + nx * 12;
+                    const double *cs = cscs + nx * 6;
+                    auto z = [&](int m, double &zr, double &zi) {
+                        const double tre = (double)rowp[2 * m] * cs[m], tim = (double)rowp[2 * m + 1] * cs[m];
+                        zr = xr * tre - xi * tim;
+                        zi = xr * tim + xi * tre;
+                    };
+                    double zr, zi;
+                    z(0, zr, zi); o[0] += fnx * zi; o[3] += fnx * fnx * zr;
+                    z(1, zr, zi); o[1] += zi; o[4] += fnx * zr;
+                    z(2, zr, zi); o[2] += zi; o[5] += fnx * zr;
+                    z(3, zr, zi); o[6] += zr;
+                    z(4, zr, zi); o[7] += zr;
+                    z(5, zr, zi); o[8] += zr;
+                }
             }
         }
 #pragma unroll
         for (int c = 0; c < 9; c++) {
+            if (c >= nch) continue;
             double v = o[c];
 #pragma unroll
             for (int w = 1; w < TPI; w <<= 1) v += __shfl_xor(v, w, 64);
