@@ -502,8 +502,8 @@ def main():
                           "per_dispatch": pmc_table(args, args.pmc_kernel, args.pmc_table)}, indent=1))
         return
     if args.gpus != 1:
-        print(f"bench.py measures one GPU only (every box of this build has one GPU); --gpus {args.gpus} "
-              f"is not supported: the multi-GPU sharding of this path is described in DESIGN.md", file=sys.stderr)
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported (an engine runs one system "
+              f"or one batch on one GPU)", file=sys.stderr)
         sys.exit(2)
     import mdff20_amd as m
     if m.device_count() < 1:

@@ -7,7 +7,7 @@
  * thermostat) on one GPU and one HIP stream.  Every entry point returns an
  * mdff status code (mdff_types.h) and never exits the process.
  *
- * Reference interfaces replaced (SURVEY of FilipeVasconcelos/mdff20, src/):
+ * Reference interfaces replaced (FilipeVasconcelos/mdff20, src/):
  *   mdff_engforce        <- void engforce(void)                field.h:36, field.c:352-407
  *   mdff_multipole_es    <- void multipole_ES(...)             ewald.h:21-24, ewald.c:75-348
  *   mdff_get_dipoles     <- void get_dipoles(mu, upol)         coulombic.h:17, coulombic.c:67-130
