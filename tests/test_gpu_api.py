@@ -689,3 +689,27 @@ def test_cell_grid_list_triclinic():
             assert tc["u_tot"] == ta["u_tot"], st
         assert np.array_equal(ec.forces(), ea.forces())
         assert np.array_equal(ec.positions(), ea.positions())
+
+
+@pytest.mark.parametrize("cw", [1, 4])
+@pytest.mark.parametrize("case", ["pim192_static", "sio2_102_static"])
+def test_kspace_shape_column_tiles(case, cw):
+    """the forward sum's other column-tile widths (1 or 4 tiles of 16 box
+    points per wavefront instead of 2; the table slice a block stages grows
+    with them) and fixed block counts: the mixed form within 1e-6 of its
+    default shape and 1e-5 of the FP64 form"""
+    print(f"[gpu] k-space shape {case} cw={cw}", flush=True)
+    p, cfg, _ = load_case(case)
+    p.conv_tol_ind = 1e-9   # the fixtures' looser tolerances separate the forms' SCF end points more
+    p.max_scf_pim_iter = 400
+    p.min_scf_pim_iter = 1
+    with m.Engine(p, cfg.typia, cfg.A, cfg.r, precision=m.FP64) as ef, \
+            m.Engine(p, cfg.typia, cfg.A, cfg.r, precision=m.MIXED) as e2, \
+            m.Engine(p, cfg.typia, cfg.A, cfg.r, precision=m.MIXED) as ew:
+        ew.set_kspace_shape(cw64=cw, cw32=cw, fwd_blocks=300, back_blocks=200)
+        tf, t2, tw = ef.engforce(0), e2.engforce(0), ew.engforce(0)
+        assert tf["status"] == 0 and t2["status"] == 0 and tw["status"] == 0
+        assert rel(ew.forces(), e2.forces()) < 1e-6
+        assert rel(ew.dipoles(), e2.dipoles()) < 1e-6
+        assert rel(ew.forces(), ef.forces()) < 1e-5
+        assert tw["u_tot"] == pytest.approx(tf["u_tot"], rel=1e-6)
